@@ -1,0 +1,255 @@
+/*
+ * execution.c -- the algorithm drivers, kept as host C code.  One host thread
+ * runs the iteration loop and calls the per-phase C ABI once per phase.
+ *
+ *   PageRank   execution_pr.c:92-168
+ *   CC         execution_cc.c:94-268
+ *   BFS        execution_bfs.c:120-276
+ *
+ * The reference's thread barriers become stream order.  The only host/device
+ * synchronisation per CC/BFS iteration is phase_op_combine_global_var_from_buf,
+ * which returns converge_vote for the termination check and the engine choice,
+ * as in the reference.  PageRank never synchronises inside its loop: S stays
+ * in the device reduce buffer between the edge and vertex phases.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "grazelle_host.h"
+#include "grazelle_phases.h"
+#include "grazelle_selector.h"
+#include "gz_host_internal.h"
+
+static uint64_t* gz_reduce_buffer = NULL;
+static gz_iteration_record_t* gz_records = NULL;
+static uint64_t gz_records_count = 0, gz_records_cap = 0;
+
+void gz_run_options_default(gz_run_options_t* o)
+{
+    memset(o, 0, sizeof(*o));
+    o->algorithm = GZ_ALG_PR;
+    o->engine = GZ_ENGINE_HYBRID;
+    o->iterations = 1;
+    o->stat_mode = GZ_STAT_COUNT_AND_OUTDEG;
+    o->pull_variant = GZ_PULL_SCHED_AWARE;
+    o->damping = 0.875;
+    o->root = 0;
+}
+
+static double gz_now_ms(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec * 1e3 + (double)ts.tv_nsec * 1e-6;
+}
+
+static void gz_record(const gz_iteration_record_t* r)
+{
+    if (gz_records_count == gz_records_cap) {
+        gz_records_cap = gz_records_cap ? 2 * gz_records_cap : 256;
+        gz_records = (gz_iteration_record_t*)realloc(gz_records, gz_records_cap * sizeof(*gz_records));
+    }
+    gz_records[gz_records_count++] = *r;
+}
+
+uint64_t gz_execution_records(gz_iteration_record_t* out, uint64_t capacity)
+{
+    const uint64_t n = gz_records_count < capacity ? gz_records_count : capacity;
+    if (out && n) memcpy(out, gz_records, n * sizeof(*out));
+    return gz_records_count;
+}
+
+void gz_execution_prepare(const gz_run_options_t* o)
+{
+    const uint64_t V = gz_graph.num_vertices;
+    const uint64_t fwords = (V + 63) / 64;
+    gz_phase_config.damping = o->damping;
+    gz_phase_config.stat_mode = o->stat_mode;
+    gz_phase_config.pull_variant = o->pull_variant;
+    gz_phase_config.collect_stats = o->collect_stats;
+    if (gz_reduce_buffer == NULL) gz_reduce_buffer = gz_reduce_buffer_create();
+    gz_device_zero(gz_reduce_buffer, 8 * sizeof(uint64_t));
+    if (gz_graph.pull_vectors != NULL) gz_merge_buffer_allocate(gz_pull_units_for(gz_graph.pull_vector_count));
+
+    switch (o->algorithm) {
+    case GZ_ALG_PR:
+        gz_dev_init_pr((double*)gz_graph.vertex_props, (double*)gz_graph.vertex_accumulators,
+                       gz_graph.vertex_outdegrees, V);
+        gz_dev_fill_u64(gz_graph.frontier_has_info, ~0ull, fwords);
+        gz_dev_fill_u64(gz_graph.frontier_wants_info, ~0ull, fwords);
+        break;
+    case GZ_ALG_CC:
+        gz_dev_iota_u64((uint64_t*)gz_graph.vertex_props, V);
+        gz_dev_fill_u64((uint64_t*)gz_graph.vertex_accumulators, 0, V + 8);
+        gz_dev_fill_u64(gz_graph.frontier_has_info, ~0ull, fwords);
+        gz_dev_fill_u64(gz_graph.frontier_wants_info, ~0ull, fwords);
+        break;
+    case GZ_ALG_BFS: {
+        if (o->root >= V) {
+            fprintf(stderr, "grazelle: BFS root %llu is not a vertex of a %llu-vertex graph\n",
+                    (unsigned long long)o->root, (unsigned long long)V);
+            exit(66);
+        }
+        const uint64_t r = o->root;
+        gz_dev_fill_u64((uint64_t*)gz_graph.vertex_props, GZ_BFS_UNVISITED, V);
+        gz_dev_store_u64((uint64_t*)gz_graph.vertex_props, r, GZ_BFS_PARENT_NONE);
+        gz_dev_fill_u64((uint64_t*)gz_graph.vertex_accumulators, 0, V + 8);
+        gz_dev_fill_u64(gz_graph.frontier_has_info, 0, fwords);
+        gz_dev_store_u64(gz_graph.frontier_has_info, r >> 6, 1ull << (r & 63));
+        gz_dev_fill_u64(gz_graph.frontier_wants_info, ~0ull, fwords);
+        gz_dev_store_u64(gz_graph.frontier_wants_info, r >> 6, ~(1ull << (r & 63)));
+        break;
+    }
+    default:
+        fprintf(stderr, "grazelle: unknown algorithm %u\n", o->algorithm);
+        exit(67);
+    }
+    gz_records_count = 0;
+}
+
+static void gz_swap_frontiers(void)
+{
+    /* execution_cc.c:245-250 / execution_bfs.c:253-258 */
+    void* t = gz_graph.frontier_has_info;
+    gz_graph.frontier_has_info = (uint64_t*)gz_graph.vertex_accumulators;
+    gz_graph.vertex_accumulators = t;
+}
+
+static void gz_run_pr(const gz_run_options_t* o, gz_run_result_t* r)
+{
+    const uint64_t V = gz_graph.num_vertices;
+    int use_push = (o->engine == GZ_ENGINE_PUSH);
+    if (use_push && gz_graph.push_vectors == NULL) use_push = 0;
+    if (!use_push && gz_graph.pull_vectors == NULL) use_push = 1;
+    gz_timer_t* t = o->profile ? gz_timer_create() : NULL;
+    uint64_t ctr;
+    for (ctr = 0; ctr < o->iterations; ++ctr) {
+        gz_iteration_record_t rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.iteration = ctr + 1;
+        if (!o->vertex_only) {
+            if (t) gz_timer_start(t);
+            phase_op_reset_global_accum();
+            if (!use_push) {
+                r->pull_iterations += 1;
+                rec.engine = 1;
+                perform_edge_pull_phase_pr(gz_graph.pull_vectors, gz_graph.pull_vector_count);
+                if (o->pull_variant == GZ_PULL_SCHED_AWARE)
+                    edge_pull_op_merge_with_merge_buffer(gz_merge_buffer, gz_pull_units_for(gz_graph.pull_vector_count),
+                                                         (double*)gz_graph.vertex_accumulators);
+            } else {
+                r->push_iterations += 1;
+                rec.engine = 2;
+                perform_edge_push_phase_pr(gz_graph.push_vectors, gz_graph.push_vector_count);
+            }
+            phase_op_write_global_accum_to_buf(gz_reduce_buffer);
+            if (t) {
+                gz_timer_stop(t);
+                rec.edge_ms = gz_timer_elapsed_ms(t);
+            }
+            if (o->collect_stats) gz_phase_stats_read(&rec.vectors, &rec.edges);
+        }
+        if (!o->edge_only) perform_vertex_phase_pr(0, V, gz_reduce_buffer);
+        if (o->profile || o->collect_stats) gz_record(&rec);
+    }
+    r->iterations = ctr;
+    if (t) gz_timer_destroy(t);
+}
+
+/* Shared loop of execution_cc.c:94-268 and execution_bfs.c:120-276. */
+static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
+{
+    const int bfs = (o->algorithm == GZ_ALG_BFS);
+    const uint64_t V = gz_graph.num_vertices, E = gz_graph.num_edges;
+    uint64_t vote = 0;
+    if (bfs) {
+        const double* od = gz_graph_outdegrees_host();
+        if (o->stat_mode != GZ_STAT_COUNT_ONLY) vote += (uint64_t)od[o->root];
+        if (o->stat_mode != GZ_STAT_OUTDEG_ONLY) vote += 1;
+    } else {
+        if (o->stat_mode != GZ_STAT_COUNT_ONLY) vote += E;
+        if (o->stat_mode != GZ_STAT_OUTDEG_ONLY) vote += V;
+    }
+    const uint64_t threshold = (o->stat_mode == GZ_STAT_COUNT_ONLY) ? V / 2 : E / 5;
+    gz_timer_t* t = (o->profile || o->engine == GZ_ENGINE_LEARNED) ? gz_timer_create() : NULL;
+    gz_selector_state_t sel;
+    gz_selector_begin(&sel, V, E);
+    uint64_t ctr = 0;
+    uint32_t prev_engine = 0;
+    double prev_ms = 0.0;
+    while (1) {
+        ctr += 1;
+        if (o->max_iterations && ctr > o->max_iterations) {
+            ctr -= 1;
+            break;
+        }
+        int use_pull;
+        if (o->engine == GZ_ENGINE_PULL)
+            use_pull = 1;
+        else if (o->engine == GZ_ENGINE_PUSH)
+            use_pull = 0;
+        else if (o->engine == GZ_ENGINE_LEARNED && gz_selector_loaded())
+            use_pull = gz_selector_choose_pull(&sel, ctr, vote, prev_engine, prev_ms);
+        else
+            use_pull = vote > threshold;
+        if (use_pull && gz_graph.pull_vectors == NULL) use_pull = 0;
+        if (!use_pull && gz_graph.push_vectors == NULL) use_pull = 1;
+
+        gz_iteration_record_t rec;
+        memset(&rec, 0, sizeof(rec));
+        rec.iteration = ctr;
+        rec.engine = use_pull ? 1 : 2;
+        rec.frontier_stat = vote;
+        if (t) gz_timer_start(t);
+        gz_phase_config.bfs_level = (uint32_t)ctr;
+        phase_op_reset_global_accum();
+        if (use_pull) {
+            r->pull_iterations += 1;
+            if (bfs)
+                perform_edge_pull_phase_bfs(gz_graph.pull_vectors, gz_graph.pull_vector_count);
+            else
+                perform_edge_pull_phase_cc(gz_graph.pull_vectors, gz_graph.pull_vector_count);
+        } else {
+            r->push_iterations += 1;
+            if (bfs)
+                perform_edge_push_phase_bfs(gz_graph.push_vectors, gz_graph.push_vector_count);
+            else
+                perform_edge_push_phase_cc(gz_graph.push_vectors, gz_graph.push_vector_count);
+        }
+        phase_op_write_global_accum_to_buf(gz_reduce_buffer);
+        if (t) gz_timer_stop(t);
+        const uint64_t next_vote = phase_op_combine_global_var_from_buf(gz_reduce_buffer);
+        if (t) rec.edge_ms = gz_timer_elapsed_ms(t);
+        prev_engine = rec.engine;
+        prev_ms = rec.edge_ms;
+        if (o->collect_stats) gz_phase_stats_read(&rec.vectors, &rec.edges);
+        if (o->profile || o->collect_stats) gz_record(&rec);
+        gz_selector_observe(&sel, vote, next_vote);
+        vote = next_vote;
+        if (0 == vote) break;
+        gz_swap_frontiers();
+        if (bfs)
+            perform_vertex_phase_bfs(0, V, NULL);
+        else
+            perform_vertex_phase_cc(0, V, NULL);
+    }
+    r->iterations = ctr;
+    if (t) gz_timer_destroy(t);
+}
+
+void gz_execution_run(const gz_run_options_t* o, gz_run_result_t* r)
+{
+    memset(r, 0, sizeof(*r));
+    gz_records_count = 0;
+    gz_synchronize();
+    const double t0 = gz_now_ms();
+    if (o->algorithm == GZ_ALG_PR)
+        gz_run_pr(o, r);
+    else
+        gz_run_hybrid(o, r);
+    gz_synchronize();
+    r->elapsed_ms = gz_now_ms() - t0;
+}

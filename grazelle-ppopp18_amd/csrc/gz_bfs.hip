@@ -1,0 +1,309 @@
+/*
+ * gz_bfs.hip -- Breadth-First Search phases on gfx950 (level-synchronous,
+ * hybrid top-down / bottom-up).
+ *
+ *   perform_edge_pull_phase_bfs  phase_edge_pull_bfs.asm:217-311 (step :53-211)
+ *   perform_edge_push_phase_bfs  phase_edge_push_bfs.asm:208-302 (step :53-202)
+ *   perform_vertex_phase_bfs     phase_vertex_bfs.asm:64-170
+ *
+ * State per vertex is one uint64 word, (depth << 48) | parent, with
+ * 0xffffffffffffffff for "not visited" and parent 0xffffffffffff for "none"
+ * (the root).  Frontiers are the reference's bitmaps: HasInfo (visited in the
+ * previous level), WantsInfo (not yet visited), HasInfo* (visited in this
+ * level).
+ *
+ * Push (top-down): for each active source and each valid lane whose
+ * destination is in WantsInfo, atomic-min the destination word with
+ * (level << 48 | source).  Exactly one lane sees the "not visited" old value;
+ * it clears WantsInfo, sets HasInfo* and counts the frontier statistic once
+ * (the reference's weak check and clear are not atomic, phase_edge_push_bfs.asm
+ * :108-156).  The surviving parent is the smallest visiting source, which is
+ * the parent the reference picks when one thread walks sources in order.
+ *
+ * Pull (bottom-up): the unvisited destinations are compacted, their vectors
+ * walked; a vector qualifies when its destination wants information and some
+ * valid source is in HasInfo.  The parent is the largest qualifying source of
+ * the destination's first qualifying vector (phase_edge_pull_bfs.asm:175-196
+ * takes the max within the vector; a single thread stops at the first such
+ * vector because it clears WantsInfo).  Pass A records each destination's
+ * first qualifying vector with an atomic min over vector indices, pass B
+ * resolves the parent and updates the frontiers.  Both engines therefore give
+ * the same words on every run.
+ */
+#include "gz_slots.h"
+
+__device__ __forceinline__ bool gz_vec_has_frontier_source(const gz_lanes& v, const uint64_t* __restrict__ has_info,
+                                                           uint32_t& edges)
+{
+    bool q = false;
+    const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (gz_lane_valid(l[k])) {
+            edges += 1;
+            q = q || gz_bit_test(has_info, gz_lane_id(l[k]));
+        }
+    }
+    return q;
+}
+
+template <bool kActive>
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_bfs(const gz_vec_t* __restrict__ vecs, gz_slot_source src,
+                                                               uint64_t chunk_len,
+                                                               const uint64_t* __restrict__ has_info,
+                                                               uint64_t* __restrict__ wants_info,
+                                                               uint64_t* __restrict__ words,
+                                                               uint64_t* __restrict__ next_frontier,
+                                                               const double* __restrict__ outdeg, uint32_t stat_mode,
+                                                               uint64_t level, uint64_t* __restrict__ partials,
+                                                               uint64_t* __restrict__ stats, uint32_t collect)
+{
+    const int lane = gz_lane();
+    const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
+    const uint64_t W = (uint64_t)gridDim.x * GZ_WAVES_PER_BLOCK;
+    const uint64_t T = gz_slot_total<kActive>(src);
+    uint64_t vote = 0;
+    uint32_t vectors = 0, edges = 0;
+    for (uint64_t chunk = gw; chunk * chunk_len < T; chunk += W) {
+        const uint64_t j0 = chunk * chunk_len;
+        const uint64_t jend = (j0 + chunk_len < T) ? j0 + chunk_len : T;
+        uint64_t a = gz_slot_begin<kActive>(src, j0);
+        for (uint64_t base = j0; base < jend; base += GZ_WAVE) {
+            const bool valid = base + (uint64_t)lane < jend;
+            uint64_t entry;
+            const uint64_t vi = gz_slot_step<kActive>(src, a, base, lane, valid, &entry);
+            if (!valid) continue;
+            const gz_lanes v = gz_load_vector(vecs, vi);
+            const uint64_t s = gz_shared_vertex(v);
+            if (!kActive && !gz_bit_test(has_info, s)) continue;
+            vectors += 1;
+            const unsigned long long key = ((unsigned long long)level << 48) | (unsigned long long)s;
+            const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (!gz_lane_valid(l[k])) continue;
+                edges += 1;
+                const uint64_t d = gz_lane_id(l[k]);
+                if (!gz_bit_test(wants_info, d)) continue;
+                const unsigned long long old = atomicMin((unsigned long long*)&words[d], key);
+                if (old == GZ_BFS_UNVISITED) {
+                    gz_bit_clear(wants_info, d);
+                    gz_bit_set(next_frontier, d);
+                    vote += gz_frontier_stat(outdeg, d, stat_mode);
+                }
+            }
+        }
+    }
+    const uint64_t total = gz_wave_sum_u64(vote);
+    if (lane == 0) partials[gw] = total;
+    if (collect) {
+        const uint64_t tv = gz_wave_sum_u64(vectors), te = gz_wave_sum_u64(edges);
+        if (lane == 0) {
+            stats[2 * gw] = tv;
+            stats[2 * gw + 1] = te;
+        }
+    }
+}
+
+/* Pass A of the bottom-up step. */
+template <bool kActive>
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_find(const gz_vec_t* __restrict__ vecs,
+                                                                    gz_slot_source src, uint64_t chunk_len,
+                                                                    const uint64_t* __restrict__ has_info,
+                                                                    const uint64_t* __restrict__ wants_info,
+                                                                    uint64_t* __restrict__ first_vector,
+                                                                    uint64_t* __restrict__ stats, uint32_t collect)
+{
+    const int lane = gz_lane();
+    const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
+    const uint64_t W = (uint64_t)gridDim.x * GZ_WAVES_PER_BLOCK;
+    const uint64_t T = gz_slot_total<kActive>(src);
+    uint32_t vectors = 0, edges = 0;
+    for (uint64_t chunk = gw; chunk * chunk_len < T; chunk += W) {
+        const uint64_t j0 = chunk * chunk_len;
+        const uint64_t jend = (j0 + chunk_len < T) ? j0 + chunk_len : T;
+        uint64_t a = gz_slot_begin<kActive>(src, j0);
+        uint64_t carry_d = ~0ull;
+        bool carry_found = false;
+        for (uint64_t base = j0; base < jend; base += GZ_WAVE) {
+            const bool valid = base + (uint64_t)lane < jend;
+            uint64_t entry;
+            const uint64_t vi = gz_slot_step<kActive>(src, a, base, lane, valid, &entry);
+            uint64_t d = ~0ull;
+            bool cand = false;
+            if (valid) {
+                const gz_lanes v = gz_load_vector(vecs, vi);
+                d = gz_shared_vertex(v);
+                if (kActive || gz_bit_test(wants_info, d)) {
+                    vectors += 1;
+                    cand = gz_vec_has_frontier_source(v, has_info, edges);
+                }
+            }
+            const uint64_t d_prev = __shfl_up((unsigned long long)d, 1, GZ_WAVE);
+            const uint64_t heads = __ballot(lane == 0 || d != d_prev);
+            const uint64_t cands = __ballot(cand);
+            const int seg = gz_segment_start(heads, lane);
+            const uint64_t before = ((1ull << lane) - 1ull) & ~((1ull << seg) - 1ull);
+            const bool carried = (d == carry_d) && carry_found;
+            if (cand && !(cands & before) && !carried)
+                atomicMin((unsigned long long*)&first_vector[d], (unsigned long long)vi);
+            const int last = (int)((jend - 1 - base) < (uint64_t)(GZ_WAVE - 1) ? (jend - 1 - base) : (uint64_t)(GZ_WAVE - 1));
+            const uint64_t last_d = __shfl((unsigned long long)d, last, GZ_WAVE);
+            const int last_seg = __shfl(seg, last, GZ_WAVE);
+            const uint64_t last_mask = ((last == 63) ? ~0ull : ((2ull << last) - 1ull)) & ~((1ull << last_seg) - 1ull);
+            carry_found = (cands & last_mask) != 0 || (last_d == carry_d && carry_found);
+            carry_d = last_d;
+        }
+    }
+    if (collect) {
+        const uint64_t tv = gz_wave_sum_u64(vectors), te = gz_wave_sum_u64(edges);
+        if (lane == 0) {
+            stats[2 * gw] = tv;
+            stats[2 * gw + 1] = te;
+        }
+    }
+}
+
+/* Pass B: visit every destination that found a qualifying vector. */
+template <bool kActive>
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_visit(const gz_vec_t* __restrict__ vecs,
+                                                                     const uint64_t* __restrict__ list,
+                                                                     const uint64_t* __restrict__ totals,
+                                                                     uint64_t num_vertices,
+                                                                     const uint64_t* __restrict__ has_info,
+                                                                     uint64_t* __restrict__ wants_info,
+                                                                     uint64_t* __restrict__ first_vector,
+                                                                     uint64_t* __restrict__ words,
+                                                                     uint64_t* __restrict__ next_frontier,
+                                                                     const double* __restrict__ outdeg,
+                                                                     uint32_t stat_mode, uint64_t level,
+                                                                     uint64_t* __restrict__ partials)
+{
+    const int lane = gz_lane();
+    const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
+    const uint64_t n = kActive ? totals[0] : num_vertices;
+    uint64_t vote = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t d = kActive ? list[i] : i;
+        const uint64_t w = first_vector[d];
+        if (w == ~0ull) continue;
+        first_vector[d] = ~0ull;
+        const gz_lanes v = gz_load_vector(vecs, w);
+        uint64_t parent = 0;
+        const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (gz_lane_valid(l[k])) {
+                const uint64_t s = gz_lane_id(l[k]);
+                if (gz_bit_test(has_info, s) && s > parent) parent = s;
+            }
+        }
+        words[d] = ((uint64_t)level << 48) | parent;
+        gz_bit_clear(wants_info, d);
+        gz_bit_set(next_frontier, d);
+        vote += gz_frontier_stat(outdeg, d, stat_mode);
+    }
+    const uint64_t total = gz_wave_sum_u64(vote);
+    if (lane == 0) partials[gw] = total;
+}
+
+static gz_slot_source gz_bfs_source(const uint64_t* offsets, uint64_t count)
+{
+    gz_slot_source src;
+    src.offsets = offsets;
+    src.list = gz_scratch.active_vertices;
+    src.prefix = gz_scratch.active_prefix;
+    src.totals = gz_scratch.active_totals;
+    src.sweep_count = count;
+    return src;
+}
+
+extern "C" void perform_edge_push_phase_bfs(const gz_vec_t* edge_list, uint64_t edge_list_count)
+{
+    const uint64_t chunk_len = 256;
+    gz_scratch_reserve(0, gz_graph.num_vertices);
+    const bool active = gz_phase_config.push_frontier_skip && gz_graph.push_vertex_offsets != nullptr;
+    gz_slot_source src = gz_bfs_source(gz_graph.push_vertex_offsets, edge_list_count);
+    uint64_t blocks;
+    if (active) {
+        gz_frontier_compact(gz_graph.frontier_has_info, gz_graph.push_vertex_offsets, gz_graph.num_vertices);
+        blocks = GZ_PERSISTENT_BLOCKS;
+    } else {
+        blocks = ((edge_list_count + chunk_len - 1) / chunk_len + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+        if (blocks == 0) blocks = 1;
+    }
+    const uint64_t waves = blocks * GZ_WAVES_PER_BLOCK;
+    gz_scratch_reserve(waves, gz_graph.num_vertices);
+    gz_scratch.partials_is_double = 0;
+    gz_scratch.partials_count = waves;
+    gz_scratch.stats_count = gz_phase_config.collect_stats ? waves : 0;
+    const uint64_t level = gz_phase_config.bfs_level;
+    if (active)
+        hipLaunchKernelGGL(gz_k_edge_push_bfs<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
+                           src, chunk_len, gz_graph.frontier_has_info, gz_graph.frontier_wants_info,
+                           (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
+                           gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, level,
+                           (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats);
+    else
+        hipLaunchKernelGGL(gz_k_edge_push_bfs<false>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
+                           src, chunk_len, gz_graph.frontier_has_info, gz_graph.frontier_wants_info,
+                           (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
+                           gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, level,
+                           (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats);
+    GZ_LAUNCH_CHECK();
+}
+
+extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t edge_list_count)
+{
+    const uint64_t chunk_len = 256;
+    gz_scratch_reserve(0, gz_graph.num_vertices);
+    const bool active = gz_phase_config.push_frontier_skip && gz_graph.pull_vertex_offsets != nullptr;
+    gz_slot_source src = gz_bfs_source(gz_graph.pull_vertex_offsets, edge_list_count);
+    uint64_t blocks;
+    if (active) {
+        gz_frontier_compact(gz_graph.frontier_wants_info, gz_graph.pull_vertex_offsets, gz_graph.num_vertices);
+        blocks = GZ_PERSISTENT_BLOCKS;
+    } else {
+        blocks = ((edge_list_count + chunk_len - 1) / chunk_len + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+        if (blocks == 0) blocks = 1;
+    }
+    uint64_t visit_blocks = GZ_PERSISTENT_BLOCKS;
+    const uint64_t waves = ((blocks > visit_blocks) ? blocks : visit_blocks) * GZ_WAVES_PER_BLOCK;
+    gz_scratch_reserve(waves, gz_graph.num_vertices);
+    gz_scratch.partials_is_double = 0;
+    gz_scratch.partials_count = visit_blocks * GZ_WAVES_PER_BLOCK;
+    gz_scratch.stats_count = gz_phase_config.collect_stats ? blocks * GZ_WAVES_PER_BLOCK : 0;
+    const uint64_t level = gz_phase_config.bfs_level;
+    if (active)
+        hipLaunchKernelGGL(gz_k_edge_pull_bfs_find<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
+                           edge_list, src, chunk_len, gz_graph.frontier_has_info, gz_graph.frontier_wants_info,
+                           gz_scratch.bfs_first_vector, gz_scratch.stats, gz_phase_config.collect_stats);
+    else
+        hipLaunchKernelGGL(gz_k_edge_pull_bfs_find<false>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
+                           edge_list, src, chunk_len, gz_graph.frontier_has_info, gz_graph.frontier_wants_info,
+                           gz_scratch.bfs_first_vector, gz_scratch.stats, gz_phase_config.collect_stats);
+    GZ_LAUNCH_CHECK();
+    if (active)
+        hipLaunchKernelGGL(gz_k_edge_pull_bfs_visit<true>, dim3((unsigned)visit_blocks), dim3(GZ_BLOCK), 0, gz_stream,
+                           edge_list, gz_scratch.active_vertices, gz_scratch.active_totals, gz_graph.num_vertices,
+                           gz_graph.frontier_has_info, gz_graph.frontier_wants_info, gz_scratch.bfs_first_vector,
+                           (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
+                           gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, level,
+                           (uint64_t*)gz_scratch.partials);
+    else
+        hipLaunchKernelGGL(gz_k_edge_pull_bfs_visit<false>, dim3((unsigned)visit_blocks), dim3(GZ_BLOCK), 0, gz_stream,
+                           edge_list, gz_scratch.active_vertices, gz_scratch.active_totals, gz_graph.num_vertices,
+                           gz_graph.frontier_has_info, gz_graph.frontier_wants_info, gz_scratch.bfs_first_vector,
+                           (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
+                           gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, level,
+                           (uint64_t*)gz_scratch.partials);
+    GZ_LAUNCH_CHECK();
+}
+
+extern "C" void gz_clear_bitmap_range(uint64_t* bitmap, uint64_t vertex_start, uint64_t vertex_count);
+
+extern "C" void perform_vertex_phase_bfs(uint64_t vertex_start, uint64_t vertex_count, const uint64_t* reduce_buffer)
+{
+    (void)reduce_buffer;
+    gz_clear_bitmap_range((uint64_t*)gz_graph.vertex_accumulators, vertex_start, vertex_count);
+}

@@ -1,0 +1,230 @@
+/*
+ * gz_cc.hip -- Connected Components phases on gfx950 (min-label propagation
+ * along directed edges, uint64 labels).
+ *
+ *   perform_edge_pull_phase_cc  phase_edge_pull_cc.asm:398-529 (step :50-325, final write :357-392)
+ *   perform_edge_push_phase_cc  phase_edge_push_cc.asm:579-673 (step :51-573)
+ *   perform_vertex_phase_cc     phase_vertex_cc.asm:64-170
+ *
+ * Pull: the same wavefront chunking and segmented scan as the PageRank pull,
+ * with min in place of sum.  Every run of a destination (including the runs
+ * cut by chunk boundaries) is applied with a 64-bit atomic min, the GPU form
+ * of the asm's lock cmpxchg at each destination change; a label that
+ * actually drops sets the destination's HasInfo* bit and adds
+ * phase_helper_add_frontier_stat to the wavefront's vote.  Gathers read the
+ * labels in place, so updates made earlier in the sweep propagate within it,
+ * as in the reference.
+ *
+ * Push: only sources in HasInfo send (the strong frontier check).  The
+ * frontier is compacted first and the kernel walks the active sources'
+ * vectors with all lanes busy; each valid lane applies an atomic min of the
+ * source's label (read once per vector, as the asm reads props[src] once).
+ */
+#include "gz_slots.h"
+
+__device__ __forceinline__ void gz_cc_apply(uint64_t* __restrict__ labels, uint64_t* __restrict__ next_frontier,
+                                            const double* __restrict__ outdeg, uint32_t stat_mode, uint64_t d,
+                                            uint64_t m, uint64_t& vote)
+{
+    if (m < labels[d]) {
+        const unsigned long long old = atomicMin((unsigned long long*)&labels[d], (unsigned long long)m);
+        if (old > m) {
+            gz_bit_set(next_frontier, d);
+            vote += gz_frontier_stat(outdeg, d, stat_mode);
+        }
+    }
+}
+
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_cc(const gz_vec_t* __restrict__ vecs, uint64_t nvec,
+                                                              uint64_t chunk_len, uint64_t nchunks,
+                                                              uint64_t* __restrict__ labels,
+                                                              uint64_t* __restrict__ next_frontier,
+                                                              const double* __restrict__ outdeg, uint32_t stat_mode,
+                                                              uint64_t* __restrict__ partials,
+                                                              uint64_t* __restrict__ stats, uint32_t collect)
+{
+    const int lane = gz_lane();
+    const uint64_t chunk = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
+    if (chunk >= nchunks) return;
+    const uint64_t b = chunk * chunk_len;
+    const uint64_t e = (b + chunk_len < nvec) ? b + chunk_len : nvec;
+
+    uint64_t carry_d = ~0ull, carry_m = ~0ull;
+    uint64_t vote = 0;
+    uint32_t edges = 0;
+    for (uint64_t base = b; base < e; base += GZ_WAVE) {
+        const uint64_t i = base + (uint64_t)lane;
+        const bool valid = i < e;
+        uint64_t d = ~0ull, m = ~0ull;
+        if (valid) {
+            const gz_lanes v = gz_load_vector(vecs, i);
+            d = gz_shared_vertex(v);
+            const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (gz_lane_valid(l[k])) {
+                    const uint64_t x = labels[gz_lane_id(l[k])];
+                    m = (x < m) ? x : m;
+                    edges += 1;
+                }
+            }
+        }
+        const uint64_t d_prev = __shfl_up((unsigned long long)d, 1, GZ_WAVE);
+        const uint64_t heads = __ballot(lane == 0 || d != d_prev);
+        const int seg = gz_segment_start(heads, lane);
+        uint64_t s = gz_segscan_min(m, lane, seg);
+        if (d == carry_d && carry_m < s) s = carry_m;
+        const int last = (int)((e - 1 - base) < (uint64_t)(GZ_WAVE - 1) ? (e - 1 - base) : (uint64_t)(GZ_WAVE - 1));
+        const bool seg_end = (lane == GZ_WAVE - 1) || ((heads >> (lane + 1)) & 1ull);
+        if (valid && seg_end && lane != last) gz_cc_apply(labels, next_frontier, outdeg, stat_mode, d, s, vote);
+        carry_d = __shfl((unsigned long long)d, last, GZ_WAVE);
+        carry_m = __shfl((unsigned long long)s, last, GZ_WAVE);
+    }
+    if (lane == 0) gz_cc_apply(labels, next_frontier, outdeg, stat_mode, carry_d, carry_m, vote);
+    const uint64_t total = gz_wave_sum_u64(vote);
+    if (lane == 0) partials[chunk] = total;
+    if (collect) {
+        const uint64_t tot_edges = gz_wave_sum_u64(edges);
+        if (lane == 0) {
+            stats[2 * chunk] = e - b;
+            stats[2 * chunk + 1] = tot_edges;
+        }
+    }
+}
+
+extern "C" void perform_edge_pull_phase_cc(const gz_vec_t* edge_list, uint64_t edge_list_count)
+{
+    const uint64_t nchunks = gz_chunks_for(edge_list_count);
+    gz_scratch_reserve(nchunks, gz_graph.num_vertices);
+    gz_scratch.partials_is_double = 0;
+    gz_scratch.partials_count = nchunks;
+    gz_scratch.stats_count = gz_phase_config.collect_stats ? nchunks : 0;
+    if (nchunks == 0) return;
+    const uint64_t chunk_len = (uint64_t)((gz_phase_config.chunk_vectors + GZ_WAVE - 1) / GZ_WAVE * GZ_WAVE);
+    const uint64_t blocks = (nchunks + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+    hipLaunchKernelGGL(gz_k_edge_pull_cc, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
+                       edge_list_count, chunk_len, nchunks, (uint64_t*)gz_graph.vertex_props,
+                       (uint64_t*)gz_graph.vertex_accumulators, gz_graph.vertex_outdegrees, gz_phase_config.stat_mode,
+                       (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats);
+    GZ_LAUNCH_CHECK();
+}
+
+template <bool kActive>
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_cc(const gz_vec_t* __restrict__ vecs, gz_slot_source src,
+                                                              uint64_t chunk_len, const uint64_t* __restrict__ has_info,
+                                                              uint64_t* __restrict__ labels,
+                                                              uint64_t* __restrict__ next_frontier,
+                                                              const double* __restrict__ outdeg, uint32_t stat_mode,
+                                                              uint64_t* __restrict__ partials,
+                                                              uint64_t* __restrict__ stats, uint32_t collect)
+{
+    const int lane = gz_lane();
+    const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
+    const uint64_t W = (uint64_t)gridDim.x * GZ_WAVES_PER_BLOCK;
+    const uint64_t T = gz_slot_total<kActive>(src);
+    uint64_t vote = 0;
+    uint32_t vectors = 0, edges = 0;
+    for (uint64_t chunk = gw; chunk * chunk_len < T; chunk += W) {
+        const uint64_t j0 = chunk * chunk_len;
+        const uint64_t jend = (j0 + chunk_len < T) ? j0 + chunk_len : T;
+        uint64_t a = gz_slot_begin<kActive>(src, j0);
+        for (uint64_t base = j0; base < jend; base += GZ_WAVE) {
+            const bool valid = base + (uint64_t)lane < jend;
+            uint64_t entry;
+            const uint64_t vi = gz_slot_step<kActive>(src, a, base, lane, valid, &entry);
+            if (!valid) continue;
+            const gz_lanes v = gz_load_vector(vecs, vi);
+            const uint64_t s = gz_shared_vertex(v);
+            if (!kActive && !gz_bit_test(has_info, s)) continue;
+            vectors += 1;
+            const uint64_t msg = labels[s];
+            const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                if (gz_lane_valid(l[k])) {
+                    edges += 1;
+                    gz_cc_apply(labels, next_frontier, outdeg, stat_mode, gz_lane_id(l[k]), msg, vote);
+                }
+            }
+        }
+    }
+    const uint64_t total = gz_wave_sum_u64(vote);
+    if (lane == 0) partials[gw] = total;
+    if (collect) {
+        const uint64_t tv = gz_wave_sum_u64(vectors), te = gz_wave_sum_u64(edges);
+        if (lane == 0) {
+            stats[2 * gw] = tv;
+            stats[2 * gw + 1] = te;
+        }
+    }
+}
+
+/* Launch a frontier-driven push over the push list for sources in has_info. */
+static void gz_cc_launch_push(const gz_vec_t* edge_list, uint64_t edge_list_count)
+{
+    const uint64_t chunk_len = 256;
+    gz_slot_source src;
+    src.offsets = gz_graph.push_vertex_offsets;
+    src.list = gz_scratch.active_vertices;
+    src.prefix = gz_scratch.active_prefix;
+    src.totals = gz_scratch.active_totals;
+    src.sweep_count = edge_list_count;
+    const bool active = gz_phase_config.push_frontier_skip && gz_graph.push_vertex_offsets != nullptr;
+    uint64_t blocks;
+    if (active) {
+        gz_frontier_compact(gz_graph.frontier_has_info, gz_graph.push_vertex_offsets, gz_graph.num_vertices);
+        blocks = GZ_PERSISTENT_BLOCKS;
+    } else {
+        blocks = ((edge_list_count + chunk_len - 1) / chunk_len + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+        if (blocks == 0) blocks = 1;
+    }
+    const uint64_t waves = blocks * GZ_WAVES_PER_BLOCK;
+    gz_scratch_reserve(waves, gz_graph.num_vertices);
+    gz_scratch.partials_is_double = 0;
+    gz_scratch.partials_count = waves;
+    gz_scratch.stats_count = gz_phase_config.collect_stats ? waves : 0;
+    if (active)
+        hipLaunchKernelGGL(gz_k_edge_push_cc<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list, src,
+                           chunk_len, gz_graph.frontier_has_info, (uint64_t*)gz_graph.vertex_props,
+                           (uint64_t*)gz_graph.vertex_accumulators, gz_graph.vertex_outdegrees,
+                           gz_phase_config.stat_mode, (uint64_t*)gz_scratch.partials, gz_scratch.stats,
+                           gz_phase_config.collect_stats);
+    else
+        hipLaunchKernelGGL(gz_k_edge_push_cc<false>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
+                           src, chunk_len, gz_graph.frontier_has_info, (uint64_t*)gz_graph.vertex_props,
+                           (uint64_t*)gz_graph.vertex_accumulators, gz_graph.vertex_outdegrees,
+                           gz_phase_config.stat_mode, (uint64_t*)gz_scratch.partials, gz_scratch.stats,
+                           gz_phase_config.collect_stats);
+    GZ_LAUNCH_CHECK();
+}
+
+extern "C" void perform_edge_push_phase_cc(const gz_vec_t* edge_list, uint64_t edge_list_count)
+{
+    gz_scratch_reserve(0, gz_graph.num_vertices);
+    gz_cc_launch_push(edge_list, edge_list_count);
+}
+
+/* Clear the HasInfo* words covering [vertex_start, vertex_start+count)
+ * (phase_vertex_cc.asm:64-170 zeroes 512 vertices per step). */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_clear_words(uint64_t* __restrict__ words, uint64_t first, uint64_t end)
+{
+    for (uint64_t w = first + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < end; w += (uint64_t)gridDim.x * blockDim.x)
+        words[w] = 0;
+}
+
+extern "C" void gz_clear_bitmap_range(uint64_t* bitmap, uint64_t vertex_start, uint64_t vertex_count)
+{
+    if (vertex_count == 0) return;
+    const uint64_t first = vertex_start >> 6;
+    const uint64_t end = (vertex_start + vertex_count + 63) >> 6;
+    uint64_t blocks = (end - first + GZ_BLOCK - 1) / GZ_BLOCK;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(gz_k_clear_words, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, first, end);
+    GZ_LAUNCH_CHECK();
+}
+
+extern "C" void perform_vertex_phase_cc(uint64_t vertex_start, uint64_t vertex_count, const uint64_t* reduce_buffer)
+{
+    (void)reduce_buffer;
+    gz_clear_bitmap_range((uint64_t*)gz_graph.vertex_accumulators, vertex_start, vertex_count);
+}

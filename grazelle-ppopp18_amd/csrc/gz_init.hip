@@ -1,0 +1,73 @@
+/*
+ * gz_init.hip -- device-side initialisation of vertex arrays and frontiers
+ * (graph_helper_initialize_vertex_info, graphdata.c:279-287, and
+ * graph_helper_create_and_initialize_frontiers, graphdata.c:290-319, with the
+ * per-algorithm callbacks execution_pr.c:64-88, execution_cc.c:234-258,
+ * execution_bfs.c:70-116).  Filling on the device keeps repeated runs (many
+ * BFS roots) off PCIe.
+ */
+#include "gz_internal.h"
+#include "gz_host_internal.h"
+
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_init_pr(double* __restrict__ props, double* __restrict__ acc,
+                                                         const double* __restrict__ outdeg, uint64_t n)
+{
+    const double nv = (double)n;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n + 8; v += (uint64_t)gridDim.x * blockDim.x) {
+        if (v < n) {
+            const double od = outdeg[v];
+            props[v] = (1.0 / nv) / (0.0 == od ? nv : od);
+        } else {
+            props[v] = 0.0;
+        }
+        acc[v] = 0.0;
+    }
+}
+
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_fill(uint64_t* __restrict__ p, uint64_t value, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = value;
+}
+
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_iota(uint64_t* __restrict__ p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = i;
+}
+
+__global__ void gz_k_store(uint64_t* __restrict__ p, uint64_t index, uint64_t value) { p[index] = value; }
+
+static unsigned gz_blocks_for(uint64_t n)
+{
+    uint64_t b = (n + GZ_BLOCK - 1) / GZ_BLOCK;
+    if (b < 1) b = 1;
+    if (b > 8192) b = 8192;
+    return (unsigned)b;
+}
+
+extern "C" void gz_dev_init_pr(double* props, double* acc, const double* outdeg, uint64_t n)
+{
+    hipLaunchKernelGGL(gz_k_init_pr, dim3(gz_blocks_for(n + 8)), dim3(GZ_BLOCK), 0, gz_stream, props, acc, outdeg, n);
+    GZ_LAUNCH_CHECK();
+}
+
+extern "C" void gz_dev_fill_u64(uint64_t* p, uint64_t value, uint64_t n)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(gz_k_fill, dim3(gz_blocks_for(n)), dim3(GZ_BLOCK), 0, gz_stream, p, value, n);
+    GZ_LAUNCH_CHECK();
+}
+
+extern "C" void gz_dev_iota_u64(uint64_t* p, uint64_t n)
+{
+    if (n == 0) return;
+    hipLaunchKernelGGL(gz_k_iota, dim3(gz_blocks_for(n)), dim3(GZ_BLOCK), 0, gz_stream, p, n);
+    GZ_LAUNCH_CHECK();
+}
+
+extern "C" void gz_dev_store_u64(uint64_t* p, uint64_t index, uint64_t value)
+{
+    hipLaunchKernelGGL(gz_k_store, dim3(1), dim3(1), 0, gz_stream, p, index, value);
+    GZ_LAUNCH_CHECK();
+}

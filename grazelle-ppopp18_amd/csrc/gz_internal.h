@@ -1,0 +1,172 @@
+/*
+ * gz_internal.h -- declarations shared by the HIP translation units of
+ * libgrazelle.so: error checking, launch geometry, device helpers for the
+ * Vector-Sparse format, wavefront primitives and the internal scratch.
+ */
+#ifndef GZ_INTERNAL_H
+#define GZ_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "grazelle_phases.h"
+
+#define GZ_WAVE 64
+#define GZ_BLOCK 256
+#define GZ_WAVES_PER_BLOCK (GZ_BLOCK / GZ_WAVE)
+
+/* Persistent grid used by the frontier-driven kernels: enough waves to fill
+ * 256 CUs several times over, each looping over its share of the work. */
+#define GZ_PERSISTENT_BLOCKS 2048
+
+#define GZ_CHECK(call)                                                                   \
+    do {                                                                                 \
+        hipError_t gz_err_ = (call);                                                     \
+        if (gz_err_ != hipSuccess) {                                                     \
+            fprintf(stderr, "grazelle: HIP error '%s' from %s at %s:%d\n",               \
+                    hipGetErrorString(gz_err_), #call, __FILE__, __LINE__);              \
+            exit(70);                                                                    \
+        }                                                                                \
+    } while (0)
+
+#define GZ_LAUNCH_CHECK() GZ_CHECK(hipGetLastError())
+
+/* ---- internal scratch, owned by gz_runtime.hip ---------------------------- */
+
+typedef struct gz_scratch_t {
+    void* partials;              /* per-wave partial sums (double or uint64) */
+    uint64_t partials_capacity;  /* entries */
+    uint64_t partials_count;     /* entries written by the last edge phase */
+    uint32_t partials_is_double; /* 1 for PR (S), 0 for CC/BFS (vote) */
+    uint64_t* stats;             /* per-wave [vectors, edges] pairs when collecting */
+    uint64_t stats_count;
+    uint64_t* reduce_tmp;        /* second-level reduction slots + done counter */
+    /* active-vertex compaction (frontier skip) */
+    uint64_t* active_vertices;   /* [V] */
+    uint64_t* active_prefix;     /* [V+1] exclusive prefix of vector counts */
+    uint64_t* active_block_sums; /* per-block [vertices, vectors] */
+    uint64_t* active_totals;     /* [0] = active vertices, [1] = active vectors */
+    uint64_t active_capacity;
+    uint64_t* bfs_first_vector;  /* [V] first qualifying pull vector of each dst, ~0 when none */
+    uint64_t scratch_vertices;   /* V the vertex-sized buffers were sized for */
+} gz_scratch_t;
+
+extern gz_scratch_t gz_scratch;
+extern hipStream_t gz_stream;
+
+/* Make sure the scratch can hold `waves` partials and vertex-sized buffers for V. */
+void gz_scratch_reserve(uint64_t waves, uint64_t num_vertices);
+/* Number of wave chunks covering n vectors at the configured chunk length. */
+uint64_t gz_chunks_for(uint64_t n);
+/* Reduce gz_scratch.partials[0..count) into dst[0] (bit pattern). */
+void gz_reduce_partials(uint64_t* dst);
+/* Build the compacted active-vertex list from `bitmap` against `offsets`. */
+void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices);
+
+/* ---- device helpers ------------------------------------------------------- */
+
+typedef unsigned long long gz_u64x2 __attribute__((ext_vector_type(2)));
+
+struct gz_lanes {
+    uint64_t l0, l1, l2, l3;
+};
+
+__device__ __forceinline__ gz_lanes gz_load_vector(const gz_vec_t* __restrict__ list, uint64_t i)
+{
+    const gz_u64x2* p = reinterpret_cast<const gz_u64x2*>(list + i);
+    const gz_u64x2 a = __builtin_nontemporal_load(p);
+    const gz_u64x2 b = __builtin_nontemporal_load(p + 1);
+    gz_lanes v;
+    v.l0 = a.x;
+    v.l1 = a.y;
+    v.l2 = b.x;
+    v.l3 = b.y;
+    return v;
+}
+
+__device__ __forceinline__ uint64_t gz_shared_vertex(const gz_lanes& v)
+{
+    return ((v.l0 >> 48) & 0x7fffull) | (((v.l1 >> 48) & 0x7fffull) << 15) |
+           (((v.l2 >> 48) & 0x7fffull) << 30) | (((v.l3 >> 48) & 0x7ull) << 45);
+}
+
+__device__ __forceinline__ bool gz_lane_valid(uint64_t l) { return (l >> 63) != 0; }
+__device__ __forceinline__ uint64_t gz_lane_id(uint64_t l) { return l & GZ_LANE_NEIGHBOUR_MASK; }
+
+__device__ __forceinline__ int gz_lane() { return (int)(threadIdx.x & (GZ_WAVE - 1)); }
+
+/* Index of the segment head at or below `lane`, given the ballot of heads. */
+__device__ __forceinline__ int gz_segment_start(uint64_t heads, int lane)
+{
+    const uint64_t upto = heads & ((2ull << lane) - 1ull);
+    return 63 - __clzll((long long)upto);
+}
+
+/* Inclusive segmented scans across the wavefront; segments are runs of equal
+ * keys, described by their start lane. */
+__device__ __forceinline__ double gz_segscan_sum(double x, int lane, int seg_start)
+{
+#pragma unroll
+    for (int o = 1; o < GZ_WAVE; o <<= 1) {
+        const double t = __shfl_up(x, o, GZ_WAVE);
+        if (lane - o >= seg_start) x += t;
+    }
+    return x;
+}
+
+__device__ __forceinline__ uint64_t gz_segscan_min(uint64_t x, int lane, int seg_start)
+{
+#pragma unroll
+    for (int o = 1; o < GZ_WAVE; o <<= 1) {
+        const uint64_t t = __shfl_up(x, o, GZ_WAVE);
+        if (lane - o >= seg_start && t < x) x = t;
+    }
+    return x;
+}
+
+__device__ __forceinline__ double gz_wave_sum(double x)
+{
+#pragma unroll
+    for (int o = GZ_WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, GZ_WAVE);
+    return x;
+}
+
+__device__ __forceinline__ uint64_t gz_wave_sum_u64(uint64_t x)
+{
+#pragma unroll
+    for (int o = GZ_WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, GZ_WAVE);
+    return x;
+}
+
+/* Bitmaps are arrays of uint64 words, bit v%64 of word v/64 (LSB first).  The
+ * device updates them through 32-bit atomics on the same bytes. */
+__device__ __forceinline__ bool gz_bit_test(const uint64_t* __restrict__ bm, uint64_t v)
+{
+    return (bm[v >> 6] >> (v & 63)) & 1ull;
+}
+
+__device__ __forceinline__ void gz_bit_set(uint64_t* bm, uint64_t v)
+{
+    unsigned int* w = reinterpret_cast<unsigned int*>(bm) + (v >> 5);
+    atomicOr(w, 1u << (v & 31));
+}
+
+__device__ __forceinline__ void gz_bit_clear(uint64_t* bm, uint64_t v)
+{
+    unsigned int* w = reinterpret_cast<unsigned int*>(bm) + (v >> 5);
+    atomicAnd(w, ~(1u << (v & 31)));
+}
+
+/* phase_helper_add_frontier_stat (phasehelpers.inc:388-399): +1 unless the
+ * count is disabled, +(uint64)outdeg(v) unless outdegrees are disabled. */
+__device__ __forceinline__ uint64_t gz_frontier_stat(const double* __restrict__ outdeg, uint64_t v, uint32_t mode)
+{
+    uint64_t s = 0;
+    if (mode != GZ_STAT_OUTDEG_ONLY) s += 1;
+    if (mode != GZ_STAT_COUNT_ONLY) s += (uint64_t)llrint(outdeg[v]);
+    return s;
+}
+
+#endif /* GZ_INTERNAL_H */

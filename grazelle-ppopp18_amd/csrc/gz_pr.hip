@@ -1,0 +1,237 @@
+/*
+ * gz_pr.hip -- PageRank phases on gfx950.
+ *
+ *   perform_edge_pull_phase_pr  phase_edge_pull_pr.asm:189-307 (step :45-171, merge entry :174-183)
+ *   perform_edge_push_phase_pr  phase_edge_push_pr.asm:360-425 (traditional atomic variant)
+ *   perform_vertex_phase_pr     phase_vertex_pr.asm:190-285   (init :27-72, step :75-134)
+ *
+ * Pull: a scheduling unit is one wavefront's chunk of `chunk_vectors`
+ * consecutive vectors (the reference's unit of ~Nv/(32*threads) vectors,
+ * scheduler_pull.inc:94-138).  The wavefront streams its chunk 64 vectors
+ * per step, one vector (two 128-bit loads) per lane, gathers up to four FP64
+ * source properties, and combines lanes that share a destination with a
+ * segmented inclusive scan.  Destinations whose run ends inside the chunk are
+ * stored directly (plain store, as the asm does when the destination
+ * changes); the chunk's last run goes to its merge-buffer record, merged by
+ * edge_pull_op_merge_with_merge_buffer.  The global PageRank sum S is the sum
+ * of all gathered values; each wavefront writes its share to the partials.
+ * No floating-point atomics, so results are reproducible run to run.
+ */
+#include "gz_internal.h"
+
+/* Set by the edge phases: the scheduler-aware pull overwrites every
+ * accumulator it owns, the traditional pull and the push accumulate with
+ * atomics, so the following vertex phase must zero them
+ * (phase_vertex_pr.asm:101-113). */
+static uint32_t gz_pr_acc_needs_zero = 0;
+
+/* Sum of the gathered properties of one vector, in the lane order of the
+ * asm's vhaddpd / vextractf128 / vaddpd sequence: (g2+g3)+(g0+g1). */
+__device__ __forceinline__ double gz_pr_vector_sum(const gz_lanes& v, const double* __restrict__ props, uint32_t& edges)
+{
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+    if (gz_lane_valid(v.l0)) { g0 = props[gz_lane_id(v.l0)]; edges += 1; }
+    if (gz_lane_valid(v.l1)) { g1 = props[gz_lane_id(v.l1)]; edges += 1; }
+    if (gz_lane_valid(v.l2)) { g2 = props[gz_lane_id(v.l2)]; edges += 1; }
+    if (gz_lane_valid(v.l3)) { g3 = props[gz_lane_id(v.l3)]; edges += 1; }
+    return (g2 + g3) + (g0 + g1);
+}
+
+template <bool kSchedAware>
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr(const gz_vec_t* __restrict__ vecs, uint64_t nvec,
+                                                              uint64_t chunk_len, uint64_t nchunks,
+                                                              const double* __restrict__ props,
+                                                              double* __restrict__ acc,
+                                                              mergeaccum_t* __restrict__ merge,
+                                                              double* __restrict__ partials,
+                                                              uint64_t* __restrict__ stats, uint32_t collect)
+{
+    const int lane = gz_lane();
+    const uint64_t chunk = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
+    if (chunk >= nchunks) return;
+    const uint64_t b = chunk * chunk_len;
+    const uint64_t e = (b + chunk_len < nvec) ? b + chunk_len : nvec;
+
+    const gz_lanes first = gz_load_vector(vecs, b);
+    const uint64_t first_d = gz_shared_vertex(first);
+    uint64_t carry_d = first_d;
+    double carry_v = 0.0;
+    double s_lane = 0.0;
+    uint32_t edges = 0;
+
+    for (uint64_t base = b; base < e; base += GZ_WAVE) {
+        const uint64_t i = base + (uint64_t)lane;
+        const bool valid = i < e;
+        uint64_t d = ~0ull;
+        double g = 0.0;
+        if (valid) {
+            const gz_lanes v = gz_load_vector(vecs, i);
+            d = gz_shared_vertex(v);
+            g = gz_pr_vector_sum(v, props, edges);
+        }
+        s_lane += g;
+        const uint64_t d_prev = __shfl_up((unsigned long long)d, 1, GZ_WAVE);
+        const uint64_t heads = __ballot(lane == 0 || d != d_prev);
+        const int seg = gz_segment_start(heads, lane);
+        double s = gz_segscan_sum(g, lane, seg);
+        if (d == carry_d) s = s + carry_v;
+        const int last = (int)((e - 1 - base) < (uint64_t)(GZ_WAVE - 1) ? (e - 1 - base) : (uint64_t)(GZ_WAVE - 1));
+        const bool seg_end = (lane == GZ_WAVE - 1) || ((heads >> (lane + 1)) & 1ull);
+        if (valid && seg_end && lane != last) {
+            if (kSchedAware)
+                acc[d] = s;
+            else
+                unsafeAtomicAdd(&acc[d], s);
+        }
+        carry_d = __shfl((unsigned long long)d, last, GZ_WAVE);
+        carry_v = __shfl(s, last, GZ_WAVE);
+    }
+
+    const double S = gz_wave_sum(s_lane);
+    if (!kSchedAware && lane == 0) unsafeAtomicAdd(&acc[carry_d], carry_v);
+    if (lane == 0) {
+        if (kSchedAware) {
+            mergeaccum_t r;
+            r.initial_vertex_id = first_d;
+            r.final_vertex_id = carry_d;
+            r.final_partial_value = carry_v;
+            r.__padding__ = 0;
+            merge[chunk] = r;
+        }
+        partials[chunk] = S;
+    }
+    if (collect) {
+        const uint64_t tot_edges = gz_wave_sum_u64(edges);
+        if (lane == 0) {
+            stats[2 * chunk] = e - b;
+            stats[2 * chunk + 1] = tot_edges;
+        }
+    }
+}
+
+extern "C" void perform_edge_pull_phase_pr(const gz_vec_t* edge_list, uint64_t edge_list_count)
+{
+    const uint64_t nchunks = gz_chunks_for(edge_list_count);
+    gz_scratch_reserve(nchunks, gz_graph.num_vertices);
+    gz_scratch.partials_is_double = 1;
+    gz_scratch.partials_count = nchunks;
+    gz_scratch.stats_count = gz_phase_config.collect_stats ? nchunks : 0;
+    if (nchunks == 0) return;
+    const bool sa = gz_phase_config.pull_variant == GZ_PULL_SCHED_AWARE;
+    gz_pr_acc_needs_zero = sa ? 0u : 1u;
+    if (sa && gz_merge_buffer_count < nchunks) {
+        fprintf(stderr, "grazelle: merge buffer holds %llu units, the pull phase needs %llu\n",
+                (unsigned long long)gz_merge_buffer_count, (unsigned long long)nchunks);
+        exit(72);
+    }
+    const uint64_t chunk_len = (uint64_t)((gz_phase_config.chunk_vectors + GZ_WAVE - 1) / GZ_WAVE * GZ_WAVE);
+    const uint64_t blocks = (nchunks + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+    if (sa)
+        hipLaunchKernelGGL(gz_k_edge_pull_pr<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
+                           edge_list_count, chunk_len, nchunks, (const double*)gz_graph.vertex_props,
+                           (double*)gz_graph.vertex_accumulators, gz_merge_buffer, (double*)gz_scratch.partials,
+                           gz_scratch.stats, gz_phase_config.collect_stats);
+    else
+        hipLaunchKernelGGL(gz_k_edge_pull_pr<false>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
+                           edge_list_count, chunk_len, nchunks, (const double*)gz_graph.vertex_props,
+                           (double*)gz_graph.vertex_accumulators, gz_merge_buffer, (double*)gz_scratch.partials,
+                           gz_scratch.stats, gz_phase_config.collect_stats);
+    GZ_LAUNCH_CHECK();
+}
+
+/* Traditional push: every valid lane adds the source's outbound message to
+ * its destination's accumulator with a hardware FP64 atomic (the asm's
+ * lock cmpxchg loop), and S gains the message once per valid lane. */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_pr(const gz_vec_t* __restrict__ vecs, uint64_t nvec,
+                                                              uint64_t chunk_len, uint64_t nchunks,
+                                                              const double* __restrict__ props,
+                                                              double* __restrict__ acc, double* __restrict__ partials,
+                                                              uint64_t* __restrict__ stats, uint32_t collect)
+{
+    const int lane = gz_lane();
+    const uint64_t chunk = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
+    if (chunk >= nchunks) return;
+    const uint64_t b = chunk * chunk_len;
+    const uint64_t e = (b + chunk_len < nvec) ? b + chunk_len : nvec;
+    double s_lane = 0.0;
+    uint32_t edges = 0;
+    for (uint64_t i = b + (uint64_t)lane; i < e; i += GZ_WAVE) {
+        const gz_lanes v = gz_load_vector(vecs, i);
+        const double msg = props[gz_shared_vertex(v)];
+        const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (gz_lane_valid(l[k])) {
+                unsafeAtomicAdd(&acc[gz_lane_id(l[k])], msg);
+                s_lane += msg;
+                edges += 1;
+            }
+        }
+    }
+    const double S = gz_wave_sum(s_lane);
+    if (lane == 0) partials[chunk] = S;
+    if (collect) {
+        const uint64_t tot_edges = gz_wave_sum_u64(edges);
+        if (lane == 0) {
+            stats[2 * chunk] = e - b;
+            stats[2 * chunk + 1] = tot_edges;
+        }
+    }
+}
+
+extern "C" void perform_edge_push_phase_pr(const gz_vec_t* edge_list, uint64_t edge_list_count)
+{
+    gz_pr_acc_needs_zero = 1;
+    const uint64_t nchunks = gz_chunks_for(edge_list_count);
+    gz_scratch_reserve(nchunks, gz_graph.num_vertices);
+    gz_scratch.partials_is_double = 1;
+    gz_scratch.partials_count = nchunks;
+    gz_scratch.stats_count = gz_phase_config.collect_stats ? nchunks : 0;
+    if (nchunks == 0) return;
+    const uint64_t chunk_len = (uint64_t)((gz_phase_config.chunk_vectors + GZ_WAVE - 1) / GZ_WAVE * GZ_WAVE);
+    const uint64_t blocks = (nchunks + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+    hipLaunchKernelGGL(gz_k_edge_push_pr, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
+                       edge_list_count, chunk_len, nchunks, (const double*)gz_graph.vertex_props,
+                       (double*)gz_graph.vertex_accumulators, (double*)gz_scratch.partials, gz_scratch.stats,
+                       gz_phase_config.collect_stats);
+    GZ_LAUNCH_CHECK();
+}
+
+/* props[v] = ((acc[v] + (1-S)/V) * d + (1-d)/V) / outdeg'(v), outdeg' = V
+ * for sinks, as separate IEEE mul/add/div (this file is built with
+ * -ffp-contract=off, matching the asm's vaddpd/vmulpd/vaddpd/vdivpd). */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr(double* __restrict__ props, double* __restrict__ acc,
+                                                           const double* __restrict__ outdeg, uint64_t start,
+                                                           uint64_t end, double num_vertices,
+                                                           const uint64_t* __restrict__ reduce_buffer,
+                                                           double damping, uint32_t zero_acc)
+{
+    const uint64_t sbits = reduce_buffer[0];
+    const double S = __longlong_as_double((long long)sbits);
+    const double c = (1.0 - S) / num_vertices;
+    const double k = (1.0 - damping) / num_vertices;
+    for (uint64_t v = start + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < end;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        double od = outdeg[v];
+        if (od == 0.0) od = num_vertices;
+        const double a = acc[v];
+        if (zero_acc) acc[v] = 0.0;
+        double x = a + c;
+        x = x * damping;
+        x = x + k;
+        props[v] = x / od;
+    }
+}
+
+extern "C" void perform_vertex_phase_pr(uint64_t vertex_start, uint64_t vertex_count, const uint64_t* reduce_buffer)
+{
+    if (vertex_count == 0) return;
+    const uint64_t end = vertex_start + vertex_count;
+    uint64_t blocks = (vertex_count + GZ_BLOCK - 1) / GZ_BLOCK;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(gz_k_vertex_pr, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
+                       (double*)gz_graph.vertex_props, (double*)gz_graph.vertex_accumulators, gz_graph.vertex_outdegrees,
+                       vertex_start, end, (double)gz_graph.num_vertices, reduce_buffer, gz_phase_config.damping,
+                       gz_pr_acc_needs_zero);
+    GZ_LAUNCH_CHECK();
+}

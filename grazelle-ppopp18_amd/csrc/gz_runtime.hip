@@ -1,0 +1,414 @@
+/*
+ * gz_runtime.hip -- the thin C-ABI runtime that replaces threads.c and
+ * numanodes.c (threads.h:37-82, numanodes.h:27-61), plus the common phase
+ * operators of phases.h:49-70:
+ *   phase_op_reset_global_accum          (phases.asm:24-30)
+ *   phase_op_write_global_accum_to_buf   (phases.asm:34-44)
+ *   phase_op_combine_global_var_from_buf (phases.c:27-37)
+ *   edge_pull_op_merge_with_merge_buffer (phases.c:41-70)
+ * One host thread drives one device through one stream; work that the
+ * reference split over threads is split over wavefronts.
+ */
+#include "gz_internal.h"
+
+#include <string.h>
+
+gz_phase_graph_t gz_graph;
+gz_phase_config_t gz_phase_config = {
+    0.875,                    /* damping: const_damping_factor, constants.asm:75-79 */
+    GZ_STAT_COUNT_AND_OUTDEG, /* stat_mode */
+    GZ_PULL_SCHED_AWARE,      /* pull_variant */
+    512,                      /* chunk_vectors */
+    1,                        /* bfs_level */
+    0,                        /* collect_stats */
+    1,                        /* push_frontier_skip */
+};
+mergeaccum_t* gz_merge_buffer = nullptr;
+uint64_t gz_merge_buffer_count = 0;
+
+gz_scratch_t gz_scratch;
+hipStream_t gz_stream = nullptr;
+static int gz_device_id = -1;
+static char gz_device_name_buf[256];
+
+/* ---- runtime hooks -------------------------------------------------------- */
+
+extern "C" int gz_runtime_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+extern "C" int gz_runtime_init(int device)
+{
+    if (gz_stream != nullptr) return 0;
+    int n = gz_runtime_device_count();
+    if (n <= 0) {
+        fprintf(stderr, "grazelle: no HIP device is visible\n");
+        exit(70);
+    }
+    if (device < 0 || device >= n) device = 0;
+    GZ_CHECK(hipSetDevice(device));
+    GZ_CHECK(hipStreamCreateWithFlags(&gz_stream, hipStreamNonBlocking));
+    hipDeviceProp_t prop;
+    GZ_CHECK(hipGetDeviceProperties(&prop, device));
+    snprintf(gz_device_name_buf, sizeof(gz_device_name_buf), "%s (%s)", prop.name, prop.gcnArchName);
+    gz_device_id = device;
+    memset(&gz_scratch, 0, sizeof(gz_scratch));
+    return 0;
+}
+
+extern "C" const char* gz_runtime_device_name(void)
+{
+    return gz_device_id >= 0 ? gz_device_name_buf : "none";
+}
+
+static void gz_require_runtime(void)
+{
+    if (gz_stream == nullptr) gz_runtime_init(0);
+}
+
+extern "C" void* gz_runtime_stream(void)
+{
+    gz_require_runtime();
+    return (void*)gz_stream;
+}
+
+extern "C" void* gz_device_malloc(size_t bytes)
+{
+    gz_require_runtime();
+    void* p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t err = hipMalloc(&p, bytes);
+    if (err != hipSuccess) {
+        fprintf(stderr, "grazelle: device allocation of %zu bytes failed: %s\n", bytes, hipGetErrorString(err));
+        exit(71);
+    }
+    gz_device_zero(p, bytes);
+    return p;
+}
+
+extern "C" void gz_device_free(void* ptr)
+{
+    if (ptr != nullptr) GZ_CHECK(hipFree(ptr));
+}
+
+extern "C" void gz_copy_to_device(void* dst, const void* src, size_t bytes)
+{
+    gz_require_runtime();
+    if (bytes == 0) return;
+    GZ_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, gz_stream));
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+}
+
+extern "C" void gz_copy_to_host(void* dst, const void* src, size_t bytes)
+{
+    gz_require_runtime();
+    if (bytes == 0) return;
+    GZ_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, gz_stream));
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+}
+
+__global__ void gz_k_zero(uint64_t* __restrict__ p, uint64_t words)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+__global__ void gz_k_zero_bytes(unsigned char* __restrict__ p, uint64_t bytes)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < bytes; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0;
+}
+
+extern "C" void gz_device_zero(void* ptr, size_t bytes)
+{
+    gz_require_runtime();
+    if (bytes == 0 || ptr == nullptr) return;
+    const uint64_t words = bytes / 8;
+    if (((uintptr_t)ptr & 7) == 0 && words > 0) {
+        uint64_t blocks = (words + GZ_BLOCK - 1) / GZ_BLOCK;
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(gz_k_zero, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, (uint64_t*)ptr, words);
+        GZ_LAUNCH_CHECK();
+        const uint64_t tail = bytes - words * 8;
+        if (tail) {
+            hipLaunchKernelGGL(gz_k_zero_bytes, dim3(1), dim3(64), 0, gz_stream, (unsigned char*)ptr + words * 8, tail);
+            GZ_LAUNCH_CHECK();
+        }
+    } else {
+        uint64_t blocks = (bytes + GZ_BLOCK - 1) / GZ_BLOCK;
+        if (blocks > 8192) blocks = 8192;
+        hipLaunchKernelGGL(gz_k_zero_bytes, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, (unsigned char*)ptr, (uint64_t)bytes);
+        GZ_LAUNCH_CHECK();
+    }
+}
+
+extern "C" void gz_synchronize(void)
+{
+    gz_require_runtime();
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+}
+
+extern "C" uint64_t* gz_reduce_buffer_create(void)
+{
+    return (uint64_t*)gz_device_malloc(8 * sizeof(uint64_t));
+}
+
+extern "C" void gz_reduce_buffer_destroy(uint64_t* reduce_buffer)
+{
+    gz_device_free(reduce_buffer);
+}
+
+struct gz_timer_t {
+    hipEvent_t start, stop;
+};
+
+extern "C" gz_timer_t* gz_timer_create(void)
+{
+    gz_require_runtime();
+    gz_timer_t* t = (gz_timer_t*)malloc(sizeof(gz_timer_t));
+    GZ_CHECK(hipEventCreate(&t->start));
+    GZ_CHECK(hipEventCreate(&t->stop));
+    return t;
+}
+
+extern "C" void gz_timer_destroy(gz_timer_t* t)
+{
+    if (!t) return;
+    GZ_CHECK(hipEventDestroy(t->start));
+    GZ_CHECK(hipEventDestroy(t->stop));
+    free(t);
+}
+
+extern "C" void gz_timer_start(gz_timer_t* t) { GZ_CHECK(hipEventRecord(t->start, gz_stream)); }
+extern "C" void gz_timer_stop(gz_timer_t* t) { GZ_CHECK(hipEventRecord(t->stop, gz_stream)); }
+
+extern "C" float gz_timer_elapsed_ms(gz_timer_t* t)
+{
+    float ms = 0.f;
+    GZ_CHECK(hipEventSynchronize(t->stop));
+    GZ_CHECK(hipEventElapsedTime(&ms, t->start, t->stop));
+    return ms;
+}
+
+/* ---- scratch -------------------------------------------------------------- */
+
+uint64_t gz_chunks_for(uint64_t n)
+{
+    uint64_t ch = gz_phase_config.chunk_vectors;
+    if (ch < GZ_WAVE) ch = GZ_WAVE;
+    ch = (ch + GZ_WAVE - 1) / GZ_WAVE * GZ_WAVE;
+    return (n + ch - 1) / ch;
+}
+
+extern "C" uint64_t gz_pull_units_for(uint64_t vector_count) { return gz_chunks_for(vector_count); }
+
+void gz_scratch_reserve(uint64_t waves, uint64_t num_vertices)
+{
+    gz_require_runtime();
+    if (waves < (uint64_t)GZ_PERSISTENT_BLOCKS * GZ_WAVES_PER_BLOCK) waves = (uint64_t)GZ_PERSISTENT_BLOCKS * GZ_WAVES_PER_BLOCK;
+    if (waves > gz_scratch.partials_capacity) {
+        gz_device_free(gz_scratch.partials);
+        gz_device_free(gz_scratch.stats);
+        gz_scratch.partials = gz_device_malloc(waves * sizeof(uint64_t));
+        gz_scratch.stats = (uint64_t*)gz_device_malloc(waves * 2 * sizeof(uint64_t));
+        gz_scratch.partials_capacity = waves;
+    }
+    if (gz_scratch.reduce_tmp == nullptr) gz_scratch.reduce_tmp = (uint64_t*)gz_device_malloc(2048 * sizeof(uint64_t));
+    if (num_vertices > gz_scratch.scratch_vertices) {
+        gz_device_free(gz_scratch.active_vertices);
+        gz_device_free(gz_scratch.active_prefix);
+        gz_device_free(gz_scratch.bfs_first_vector);
+        gz_scratch.active_vertices = (uint64_t*)gz_device_malloc(num_vertices * sizeof(uint64_t));
+        gz_scratch.active_prefix = (uint64_t*)gz_device_malloc((num_vertices + 1) * sizeof(uint64_t));
+        gz_scratch.bfs_first_vector = (uint64_t*)gz_device_malloc(num_vertices * sizeof(uint64_t));
+        GZ_CHECK(hipMemsetAsync(gz_scratch.bfs_first_vector, 0xff, num_vertices * sizeof(uint64_t), gz_stream));
+        const uint64_t words = (num_vertices + 63) / 64;
+        const uint64_t blocks = (words + GZ_BLOCK - 1) / GZ_BLOCK + 1;
+        gz_device_free(gz_scratch.active_block_sums);
+        gz_scratch.active_block_sums = (uint64_t*)gz_device_malloc((2 * blocks + 8) * sizeof(uint64_t));
+        if (gz_scratch.active_totals == nullptr) gz_scratch.active_totals = (uint64_t*)gz_device_malloc(8 * sizeof(uint64_t));
+        gz_scratch.scratch_vertices = num_vertices;
+        gz_scratch.active_capacity = num_vertices;
+    }
+}
+
+extern "C" void gz_merge_buffer_allocate(uint64_t units)
+{
+    gz_require_runtime();
+    if (units == 0) units = 1;
+    if (units > gz_merge_buffer_count) {
+        gz_device_free(gz_merge_buffer);
+        gz_merge_buffer = (mergeaccum_t*)gz_device_malloc(units * sizeof(mergeaccum_t));
+    }
+    gz_merge_buffer_count = units;
+    /* graph_data_allocate_merge_buffers initialises every record to ~0 / 0.0 */
+    GZ_CHECK(hipMemsetAsync(gz_merge_buffer, 0xff, units * sizeof(mergeaccum_t), gz_stream));
+}
+
+/* ---- reductions ----------------------------------------------------------- */
+
+template <typename T>
+__device__ __forceinline__ T gz_block_sum(T x, T* lds)
+{
+    for (int o = GZ_WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, GZ_WAVE);
+    const int w = threadIdx.x / GZ_WAVE;
+    if ((threadIdx.x & (GZ_WAVE - 1)) == 0) lds[w] = x;
+    __syncthreads();
+    T r = 0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < (int)(blockDim.x / GZ_WAVE); ++i) r += lds[i];
+    }
+    __syncthreads();
+    return r;
+}
+
+/* Two-stage deterministic reduction: block b sums a fixed slice, the last
+ * block to finish sums the block results in index order. */
+template <typename T>
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_reduce(const T* __restrict__ in, uint64_t count, T* __restrict__ slots,
+                                                        unsigned int* __restrict__ done, uint64_t* __restrict__ out)
+{
+    __shared__ T lds[GZ_WAVES_PER_BLOCK];
+    __shared__ bool last;
+    const uint64_t per = (count + gridDim.x - 1) / gridDim.x;
+    const uint64_t b = (uint64_t)blockIdx.x * per;
+    const uint64_t e = (b + per < count) ? b + per : count;
+    T x = 0;
+    for (uint64_t i = b + threadIdx.x; i < e; i += blockDim.x) x += in[i];
+    T r = gz_block_sum<T>(x, lds);
+    if (threadIdx.x == 0) {
+        slots[blockIdx.x] = r;
+        __threadfence();
+        const unsigned int prev = atomicAdd(done, 1u);
+        last = (prev == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    T y = 0;
+    for (uint64_t i = threadIdx.x; i < gridDim.x; i += blockDim.x) y += ((volatile T*)slots)[i];
+    T total = gz_block_sum<T>(y, lds);
+    if (threadIdx.x == 0) {
+        T t = total;
+        out[0] = *reinterpret_cast<uint64_t*>(&t);
+        *done = 0u;
+    }
+}
+
+void gz_reduce_partials(uint64_t* dst)
+{
+    const uint64_t count = gz_scratch.partials_count;
+    uint64_t blocks = (count + 4095) / 4096;
+    if (blocks < 1) blocks = 1;
+    if (blocks > 1024) blocks = 1024;
+    unsigned int* done = reinterpret_cast<unsigned int*>(gz_scratch.reduce_tmp + 2040);
+    if (gz_scratch.partials_is_double) {
+        hipLaunchKernelGGL(gz_k_reduce<double>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
+                           (const double*)gz_scratch.partials, count, (double*)gz_scratch.reduce_tmp, done, dst);
+    } else {
+        hipLaunchKernelGGL(gz_k_reduce<unsigned long long>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
+                           (const unsigned long long*)gz_scratch.partials, count, (unsigned long long*)gz_scratch.reduce_tmp,
+                           done, dst);
+    }
+    GZ_LAUNCH_CHECK();
+}
+
+/* ---- common phase operators ----------------------------------------------- */
+
+extern "C" void phase_op_reset_global_accum(void)
+{
+    gz_require_runtime();
+    gz_scratch.partials_count = 0;
+    gz_scratch.stats_count = 0;
+}
+
+extern "C" void phase_op_write_global_accum_to_buf(uint64_t* reduce_buffer)
+{
+    gz_require_runtime();
+    gz_reduce_partials(reduce_buffer);
+}
+
+extern "C" uint64_t phase_op_combine_global_var_from_buf(uint64_t* reduce_buffer)
+{
+    uint64_t v = 0;
+    gz_copy_to_host(&v, reduce_buffer, sizeof(v));
+    return v;
+}
+
+/* Parallel form of the serial merge in phases.c:41-70.  Records sharing a
+ * final vertex form a run; the first record of each run (its head) folds the
+ * run's partials in record order and, when the record after the run starts
+ * on the same vertex (that unit stored its first vertex directly), folds the
+ * accumulator too.  A record whose initial id has bit 63 set is unused. */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_merge_sum(const mergeaccum_t* __restrict__ mb, uint64_t count,
+                                                           double* __restrict__ acc)
+{
+    const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= count) return;
+    const mergeaccum_t r = mb[u];
+    if (r.initial_vertex_id & GZ_LANE_VALID_BIT) return;
+    if (u > 0) {
+        const mergeaccum_t p = mb[u - 1];
+        if (!(p.initial_vertex_id & GZ_LANE_VALID_BIT) && p.final_vertex_id == r.final_vertex_id) return;
+    }
+    double proposed = r.final_partial_value;
+    uint64_t j = u + 1;
+    while (j < count && mb[j].final_vertex_id == r.final_vertex_id) {
+        proposed = proposed + mb[j].final_partial_value;
+        ++j;
+    }
+    if (j < count && mb[j].initial_vertex_id == r.final_vertex_id) proposed = proposed + acc[mb[j].initial_vertex_id];
+    acc[r.final_vertex_id] = proposed;
+}
+
+extern "C" void edge_pull_op_merge_with_merge_buffer(mergeaccum_t* merge_buffer, uint64_t count, double* vertex_accumulators)
+{
+    gz_require_runtime();
+    if (count == 0) return;
+    const uint64_t blocks = (count + GZ_BLOCK - 1) / GZ_BLOCK;
+    hipLaunchKernelGGL(gz_k_merge_sum, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, merge_buffer, count,
+                       vertex_accumulators);
+    GZ_LAUNCH_CHECK();
+}
+
+/* ---- statistics ----------------------------------------------------------- */
+
+extern "C" void gz_phase_stats_read(uint64_t* vectors, uint64_t* edges)
+{
+    gz_require_runtime();
+    *vectors = 0;
+    *edges = 0;
+    const uint64_t n = gz_scratch.stats_count;
+    if (n == 0) return;
+    uint64_t* h = (uint64_t*)malloc(n * 2 * sizeof(uint64_t));
+    gz_copy_to_host(h, gz_scratch.stats, n * 2 * sizeof(uint64_t));
+    for (uint64_t i = 0; i < n; ++i) {
+        *vectors += h[2 * i];
+        *edges += h[2 * i + 1];
+    }
+    free(h);
+}
+
+extern "C" void gz_runtime_shutdown(void)
+{
+    if (gz_stream == nullptr) return;
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    gz_device_free(gz_scratch.partials);
+    gz_device_free(gz_scratch.stats);
+    gz_device_free(gz_scratch.reduce_tmp);
+    gz_device_free(gz_scratch.active_vertices);
+    gz_device_free(gz_scratch.active_prefix);
+    gz_device_free(gz_scratch.active_block_sums);
+    gz_device_free(gz_scratch.active_totals);
+    gz_device_free(gz_scratch.bfs_first_vector);
+    gz_device_free(gz_merge_buffer);
+    gz_merge_buffer = nullptr;
+    gz_merge_buffer_count = 0;
+    memset(&gz_scratch, 0, sizeof(gz_scratch));
+    GZ_CHECK(hipStreamDestroy(gz_stream));
+    gz_stream = nullptr;
+    gz_device_id = -1;
+}

@@ -1,0 +1,266 @@
+/*
+ * selector_kernels.hip -- forward, backward and Adam kernels of the learned
+ * engine selector (8->32->32->2 FP32 MLP, ReLU hidden layers, linear output).
+ * No counterpart in the reference.
+ *
+ * Parameter layout (GZ_SEL_PARAMS floats):
+ *   W1[32][8]  b1[32]  W2[32][32]  b2[32]  W3[2][32]  b3[2]
+ * Training minimises the mean over samples and outputs of the squared error
+ * against measured log edge-phase times.  The gradient of one sample is
+ * computed by one thread; a block folds its threads' gradients in LDS and
+ * writes one partial per block, and the Adam kernel sums the block partials
+ * in index order.
+ */
+#include "gz_internal.h"
+#include "grazelle_selector.h"
+
+#define O_W1 0
+#define O_B1 (O_W1 + GZ_SEL_H * GZ_SEL_IN)
+#define O_W2 (O_B1 + GZ_SEL_H)
+#define O_B2 (O_W2 + GZ_SEL_H * GZ_SEL_H)
+#define O_W3 (O_B2 + GZ_SEL_H)
+#define O_B3 (O_W3 + GZ_SEL_OUT * GZ_SEL_H)
+
+__device__ __forceinline__ void gz_sel_forward(const float* __restrict__ p, const float* x, float* h1, float* h2,
+                                               float* y)
+{
+    for (int j = 0; j < GZ_SEL_H; ++j) {
+        float a = p[O_B1 + j];
+        for (int i = 0; i < GZ_SEL_IN; ++i) a += p[O_W1 + j * GZ_SEL_IN + i] * x[i];
+        h1[j] = a > 0.f ? a : 0.f;
+    }
+    for (int j = 0; j < GZ_SEL_H; ++j) {
+        float a = p[O_B2 + j];
+        for (int i = 0; i < GZ_SEL_H; ++i) a += p[O_W2 + j * GZ_SEL_H + i] * h1[i];
+        h2[j] = a > 0.f ? a : 0.f;
+    }
+    for (int k = 0; k < GZ_SEL_OUT; ++k) {
+        float a = p[O_B3 + k];
+        for (int i = 0; i < GZ_SEL_H; ++i) a += p[O_W3 + k * GZ_SEL_H + i] * h2[i];
+        y[k] = a;
+    }
+}
+
+__global__ void gz_k_sel_forward(const float* __restrict__ p, const float* __restrict__ X, float* __restrict__ Y,
+                                 uint64_t n)
+{
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    float x[GZ_SEL_IN], h1[GZ_SEL_H], h2[GZ_SEL_H], y[GZ_SEL_OUT];
+    for (int i = 0; i < GZ_SEL_IN; ++i) x[i] = X[s * GZ_SEL_IN + i];
+    gz_sel_forward(p, x, h1, h2, y);
+    for (int k = 0; k < GZ_SEL_OUT; ++k) Y[s * GZ_SEL_OUT + k] = y[k];
+}
+
+/* Gradient of L = 1/(n*2) * sum_s sum_k (y_k - t_k)^2 w.r.t. every parameter. */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_grad(const float* __restrict__ p, const float* __restrict__ X,
+                                                          const float* __restrict__ T, uint64_t n,
+                                                          float* __restrict__ block_grads, float* __restrict__ block_loss)
+{
+    __shared__ float g[GZ_SEL_PARAMS];
+    __shared__ float lsum[GZ_BLOCK];
+    for (int i = threadIdx.x; i < GZ_SEL_PARAMS; i += blockDim.x) g[i] = 0.f;
+    __syncthreads();
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float loss = 0.f;
+    if (s < n) {
+        float x[GZ_SEL_IN], h1[GZ_SEL_H], h2[GZ_SEL_H], y[GZ_SEL_OUT];
+        for (int i = 0; i < GZ_SEL_IN; ++i) x[i] = X[s * GZ_SEL_IN + i];
+        gz_sel_forward(p, x, h1, h2, y);
+        const float scale = 1.0f / (float)(n * GZ_SEL_OUT);
+        float dy[GZ_SEL_OUT];
+        for (int k = 0; k < GZ_SEL_OUT; ++k) {
+            const float r = y[k] - T[s * GZ_SEL_OUT + k];
+            loss += r * r;
+            dy[k] = 2.f * r * scale;
+        }
+        float dh2[GZ_SEL_H];
+        for (int i = 0; i < GZ_SEL_H; ++i) dh2[i] = 0.f;
+        for (int k = 0; k < GZ_SEL_OUT; ++k) {
+            atomicAdd(&g[O_B3 + k], dy[k]);
+            for (int i = 0; i < GZ_SEL_H; ++i) {
+                atomicAdd(&g[O_W3 + k * GZ_SEL_H + i], dy[k] * h2[i]);
+                dh2[i] += dy[k] * p[O_W3 + k * GZ_SEL_H + i];
+            }
+        }
+        float dh1[GZ_SEL_H];
+        for (int i = 0; i < GZ_SEL_H; ++i) dh1[i] = 0.f;
+        for (int j = 0; j < GZ_SEL_H; ++j) {
+            const float da = h2[j] > 0.f ? dh2[j] : 0.f;
+            if (da == 0.f) continue;
+            atomicAdd(&g[O_B2 + j], da);
+            for (int i = 0; i < GZ_SEL_H; ++i) {
+                atomicAdd(&g[O_W2 + j * GZ_SEL_H + i], da * h1[i]);
+                dh1[i] += da * p[O_W2 + j * GZ_SEL_H + i];
+            }
+        }
+        for (int j = 0; j < GZ_SEL_H; ++j) {
+            const float da = h1[j] > 0.f ? dh1[j] : 0.f;
+            if (da == 0.f) continue;
+            atomicAdd(&g[O_B1 + j], da);
+            for (int i = 0; i < GZ_SEL_IN; ++i) atomicAdd(&g[O_W1 + j * GZ_SEL_IN + i], da * x[i]);
+        }
+    }
+    lsum[threadIdx.x] = loss;
+    __syncthreads();
+    for (int i = threadIdx.x; i < GZ_SEL_PARAMS; i += blockDim.x) block_grads[(uint64_t)blockIdx.x * GZ_SEL_PARAMS + i] = g[i];
+    if (threadIdx.x == 0) {
+        float t = 0.f;
+        for (int i = 0; i < (int)blockDim.x; ++i) t += lsum[i];
+        block_loss[blockIdx.x] = t;
+    }
+}
+
+__global__ void gz_k_sel_sum_grads(const float* __restrict__ block_grads, uint32_t blocks, float* __restrict__ grad)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= GZ_SEL_PARAMS) return;
+    float s = 0.f;
+    for (uint32_t b = 0; b < blocks; ++b) s += block_grads[(uint64_t)b * GZ_SEL_PARAMS + i];
+    grad[i] = s;
+}
+
+__global__ void gz_k_sel_adam(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                              const float* __restrict__ grad, float lr, float beta1, float beta2, float eps,
+                              float bc1, float bc2)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= GZ_SEL_PARAMS) return;
+    const float g = grad[i];
+    const float mi = beta1 * m[i] + (1.f - beta1) * g;
+    const float vi = beta2 * v[i] + (1.f - beta2) * g * g;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
+}
+
+/* ---- host entry points ---------------------------------------------------- */
+
+static float* gz_sel_dev_params = nullptr;
+static float* gz_sel_dev_io = nullptr;
+
+extern "C" int gz_selector_device_upload(const float* params)
+{
+    if (gz_sel_dev_params == nullptr) gz_sel_dev_params = (float*)gz_device_malloc(GZ_SEL_PARAMS * sizeof(float));
+    gz_copy_to_device(gz_sel_dev_params, params, GZ_SEL_PARAMS * sizeof(float));
+    return 0;
+}
+
+extern "C" int gz_selector_device_forward(const float* x, float* y)
+{
+    if (gz_sel_dev_params == nullptr) return -1;
+    if (gz_sel_dev_io == nullptr) gz_sel_dev_io = (float*)gz_device_malloc(64 * sizeof(float));
+    gz_copy_to_device(gz_sel_dev_io, x, GZ_SEL_IN * sizeof(float));
+    hipLaunchKernelGGL(gz_k_sel_forward, dim3(1), dim3(64), 0, gz_stream, gz_sel_dev_params, gz_sel_dev_io,
+                       gz_sel_dev_io + 32, (uint64_t)1);
+    GZ_LAUNCH_CHECK();
+    gz_copy_to_host(y, gz_sel_dev_io + 32, GZ_SEL_OUT * sizeof(float));
+    return 0;
+}
+
+/* Batched forward on the device: X[n][8] -> Y[n][2] (host arrays). */
+extern "C" int gz_selector_forward_batch(const float* params, const float* X, float* Y, uint64_t n)
+{
+    float* dp = (float*)gz_device_malloc(GZ_SEL_PARAMS * sizeof(float));
+    float* dx = (float*)gz_device_malloc((n ? n : 1) * GZ_SEL_IN * sizeof(float));
+    float* dy = (float*)gz_device_malloc((n ? n : 1) * GZ_SEL_OUT * sizeof(float));
+    gz_copy_to_device(dp, params, GZ_SEL_PARAMS * sizeof(float));
+    gz_copy_to_device(dx, X, n * GZ_SEL_IN * sizeof(float));
+    if (n) {
+        hipLaunchKernelGGL(gz_k_sel_forward, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, gz_stream, dp, dx, dy, n);
+        GZ_LAUNCH_CHECK();
+    }
+    gz_copy_to_host(Y, dy, n * GZ_SEL_OUT * sizeof(float));
+    gz_device_free(dp);
+    gz_device_free(dx);
+    gz_device_free(dy);
+    return 0;
+}
+
+/* Full-batch gradient on the device (host arrays in and out); returns the
+ * mean squared error. */
+extern "C" float gz_selector_gradient(const float* params, const float* X, const float* T, uint64_t n, float* grad)
+{
+    const uint32_t blocks = (uint32_t)((n + GZ_BLOCK - 1) / GZ_BLOCK);
+    float* dp = (float*)gz_device_malloc(GZ_SEL_PARAMS * sizeof(float));
+    float* dx = (float*)gz_device_malloc((n ? n : 1) * GZ_SEL_IN * sizeof(float));
+    float* dt = (float*)gz_device_malloc((n ? n : 1) * GZ_SEL_OUT * sizeof(float));
+    float* bg = (float*)gz_device_malloc((blocks ? blocks : 1) * (uint64_t)GZ_SEL_PARAMS * sizeof(float));
+    float* bl = (float*)gz_device_malloc((blocks ? blocks : 1) * sizeof(float));
+    float* dg = (float*)gz_device_malloc(GZ_SEL_PARAMS * sizeof(float));
+    gz_copy_to_device(dp, params, GZ_SEL_PARAMS * sizeof(float));
+    gz_copy_to_device(dx, X, n * GZ_SEL_IN * sizeof(float));
+    gz_copy_to_device(dt, T, n * GZ_SEL_OUT * sizeof(float));
+    float loss = 0.f;
+    if (blocks) {
+        hipLaunchKernelGGL(gz_k_sel_grad, dim3(blocks), dim3(GZ_BLOCK), 0, gz_stream, dp, dx, dt, n, bg, bl);
+        GZ_LAUNCH_CHECK();
+        hipLaunchKernelGGL(gz_k_sel_sum_grads, dim3((GZ_SEL_PARAMS + 255) / 256), dim3(256), 0, gz_stream, bg, blocks, dg);
+        GZ_LAUNCH_CHECK();
+        float* hl = (float*)malloc(blocks * sizeof(float));
+        gz_copy_to_host(hl, bl, blocks * sizeof(float));
+        for (uint32_t b = 0; b < blocks; ++b) loss += hl[b];
+        free(hl);
+        loss /= (float)(n * GZ_SEL_OUT);
+    }
+    gz_copy_to_host(grad, dg, GZ_SEL_PARAMS * sizeof(float));
+    gz_device_free(dp);
+    gz_device_free(dx);
+    gz_device_free(dt);
+    gz_device_free(bg);
+    gz_device_free(bl);
+    gz_device_free(dg);
+    return loss;
+}
+
+/* Train with full-batch Adam for `steps` steps, entirely on the device.
+ * params is updated in place; loss_history (optional) receives the loss
+ * before each step. */
+extern "C" int gz_selector_train(float* params, const float* X, const float* T, uint64_t n, uint32_t steps, float lr,
+                                 float* loss_history)
+{
+    const uint32_t blocks = (uint32_t)((n + GZ_BLOCK - 1) / GZ_BLOCK);
+    if (blocks == 0) return -1;
+    float* dp = (float*)gz_device_malloc(GZ_SEL_PARAMS * sizeof(float));
+    float* dm = (float*)gz_device_malloc(GZ_SEL_PARAMS * sizeof(float));
+    float* dv = (float*)gz_device_malloc(GZ_SEL_PARAMS * sizeof(float));
+    float* dx = (float*)gz_device_malloc(n * GZ_SEL_IN * sizeof(float));
+    float* dt = (float*)gz_device_malloc(n * GZ_SEL_OUT * sizeof(float));
+    float* bg = (float*)gz_device_malloc(blocks * (uint64_t)GZ_SEL_PARAMS * sizeof(float));
+    float* bl = (float*)gz_device_malloc(blocks * sizeof(float));
+    float* dg = (float*)gz_device_malloc(GZ_SEL_PARAMS * sizeof(float));
+    gz_copy_to_device(dp, params, GZ_SEL_PARAMS * sizeof(float));
+    gz_copy_to_device(dx, X, n * GZ_SEL_IN * sizeof(float));
+    gz_copy_to_device(dt, T, n * GZ_SEL_OUT * sizeof(float));
+    float* hl = (float*)malloc(blocks * sizeof(float));
+    const float b1 = 0.9f, b2 = 0.999f;
+    float p1 = 1.f, p2 = 1.f;
+    for (uint32_t s = 0; s < steps; ++s) {
+        hipLaunchKernelGGL(gz_k_sel_grad, dim3(blocks), dim3(GZ_BLOCK), 0, gz_stream, dp, dx, dt, n, bg, bl);
+        GZ_LAUNCH_CHECK();
+        hipLaunchKernelGGL(gz_k_sel_sum_grads, dim3((GZ_SEL_PARAMS + 255) / 256), dim3(256), 0, gz_stream, bg, blocks, dg);
+        GZ_LAUNCH_CHECK();
+        p1 *= b1;
+        p2 *= b2;
+        hipLaunchKernelGGL(gz_k_sel_adam, dim3((GZ_SEL_PARAMS + 255) / 256), dim3(256), 0, gz_stream, dp, dm, dv, dg, lr, b1,
+                           b2, 1e-8f, 1.f - p1, 1.f - p2);
+        GZ_LAUNCH_CHECK();
+        if (loss_history) {
+            gz_copy_to_host(hl, bl, blocks * sizeof(float));
+            float l = 0.f;
+            for (uint32_t b = 0; b < blocks; ++b) l += hl[b];
+            loss_history[s] = l / (float)(n * GZ_SEL_OUT);
+        }
+    }
+    gz_copy_to_host(params, dp, GZ_SEL_PARAMS * sizeof(float));
+    free(hl);
+    gz_device_free(dp);
+    gz_device_free(dm);
+    gz_device_free(dv);
+    gz_device_free(dx);
+    gz_device_free(dt);
+    gz_device_free(bg);
+    gz_device_free(bl);
+    gz_device_free(dg);
+    return 0;
+}

@@ -1,0 +1,353 @@
+"""Python binding of libgrazelle.so (ctypes over the C ABI in include/).
+
+This is a thin host-side mirror for tests, benchmarks and scripting: graph
+generation and loading, the algorithm drivers and result download.  All
+computation happens in the library's HIP kernels; there is no Python or CPU
+fallback.  Importing this module raises if the library has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libgrazelle.so")
+
+ALG = {"pr": 0, "cc": 1, "bfs": 2}
+ENGINE = {"hybrid": 0, "pull": 1, "push": 2, "learned": 3}
+STAT = {"both": 0, "outdeg": 1, "count": 2}
+PULL_VARIANT = {"sched_aware": 0, "traditional": 1}
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"libgrazelle.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+
+lib = C.CDLL(LIB_PATH)
+
+u64 = C.c_uint64
+u32 = C.c_uint32
+vp = C.c_void_p
+
+
+class RunOptions(C.Structure):
+    _fields_ = [
+        ("algorithm", u32), ("engine", u32), ("iterations", u32), ("stat_mode", u32),
+        ("pull_variant", u32), ("edge_only", u32), ("vertex_only", u32), ("profile", u32),
+        ("collect_stats", u32), ("max_iterations", u32), ("damping", C.c_double), ("root", u64),
+    ]
+
+
+class RunResult(C.Structure):
+    _fields_ = [("iterations", u64), ("pull_iterations", u64), ("push_iterations", u64),
+                ("elapsed_ms", C.c_double)]
+
+
+class IterationRecord(C.Structure):
+    _fields_ = [("iteration", u64), ("engine", u32), ("pad", u32), ("frontier_stat", u64),
+                ("vectors", u64), ("edges", u64), ("edge_ms", C.c_double)]
+
+
+class RmatParams(C.Structure):
+    _fields_ = [("scale", u32), ("num_vertices", u64), ("num_edges", u64), ("a", C.c_double),
+                ("b", C.c_double), ("c", C.c_double), ("seed", u64), ("edge_offset", u64)]
+
+
+class PhaseConfig(C.Structure):
+    _fields_ = [("damping", C.c_double), ("stat_mode", u32), ("pull_variant", u32),
+                ("chunk_vectors", u32), ("bfs_level", u32), ("collect_stats", u32),
+                ("push_frontier_skip", u32)]
+
+
+class PhaseGraph(C.Structure):
+    _fields_ = [("num_vertices", u64), ("num_edges", u64), ("pull_vectors", vp), ("pull_vector_count", u64),
+                ("pull_vertex_offsets", vp), ("push_vectors", vp), ("push_vector_count", u64),
+                ("push_vertex_offsets", vp), ("vertex_outdegrees", vp), ("vertex_props", vp),
+                ("vertex_accumulators", vp), ("frontier_has_info", vp), ("frontier_wants_info", vp)]
+
+
+def _sig(name, res, *args):
+    f = getattr(lib, name)
+    f.restype = res
+    f.argtypes = list(args)
+    return f
+
+
+P = C.POINTER
+_sig("gz_runtime_device_count", C.c_int)
+_sig("gz_runtime_init", C.c_int, C.c_int)
+_sig("gz_runtime_shutdown", None)
+_sig("gz_runtime_device_name", C.c_char_p)
+_sig("gz_synchronize", None)
+_sig("gz_edge_list_write", C.c_int, C.c_char_p, u64, u64, vp)
+_sig("gz_edge_list_read", C.c_int, C.c_char_p, P(u64), P(u64), P(P(u64)))
+_sig("gz_edges_sort", u64, vp, u64, C.c_int, C.c_int)
+_sig("gz_pack_vectors", u64, vp, u64, C.c_int, vp)
+_sig("gz_count_vectors", u64, vp, u64, C.c_int)
+_sig("gz_build_vertex_index", None, vp, u64, u64, vp, P(u64), P(u64))
+_sig("gz_build_vertex_offsets", None, vp, u64, u64, vp)
+_sig("gz_count_outdegrees", None, vp, u64, u64, vp)
+_sig("gz_rmat_draw", None, P(RmatParams), vp)
+_sig("gz_splitmix64", u64, u64)
+_sig("gz_graph_load_pairs", C.c_int, u64, u64, vp, u64, vp)
+_sig("gz_graph_release", None)
+_sig("gz_graph_num_vertices", u64)
+_sig("gz_graph_num_edges", u64)
+_sig("gz_graph_pull_vector_count", u64)
+_sig("gz_graph_push_vector_count", u64)
+_sig("gz_graph_download_vectors", None, C.c_int, vp)
+_sig("gz_graph_download_offsets", None, C.c_int, vp)
+_sig("gz_run_options_default", None, P(RunOptions))
+_sig("gz_execution_prepare", None, P(RunOptions))
+_sig("gz_execution_run", None, P(RunOptions), P(RunResult))
+_sig("gz_execution_records", u64, vp, u64)
+_sig("gz_result_pr_ranks", None, vp)
+_sig("gz_result_pr_props", None, vp)
+_sig("gz_result_cc_labels", None, vp)
+_sig("gz_result_bfs", None, vp, vp)
+_sig("gz_pull_units_for", u64, u64)
+_sig("gz_main", C.c_int, C.c_int, P(C.c_char_p))
+_sig("gz_selector_set_weights", C.c_int, vp, vp, vp)
+_sig("gz_selector_clear", None)
+_sig("gz_selector_forward_batch", C.c_int, vp, vp, vp, u64)
+_sig("gz_selector_gradient", C.c_float, vp, vp, vp, u64, vp)
+_sig("gz_selector_train", C.c_int, vp, vp, vp, u64, u32, C.c_float, vp)
+
+phase_config = PhaseConfig.in_dll(lib, "gz_phase_config")
+phase_graph = PhaseGraph.in_dll(lib, "gz_graph")
+
+SEL_PARAMS = 32 * 8 + 32 + 32 * 32 + 32 + 2 * 32 + 2
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(vp)
+
+
+def device_count() -> int:
+    return int(lib.gz_runtime_device_count())
+
+
+# ---- graph construction (host C code) ---------------------------------------
+
+def rmat_edges(scale: int, num_edges: int, seed: int, a=0.57, b=0.19, c=0.19, num_vertices: int = 0,
+               edge_offset: int = 0) -> np.ndarray:
+    """Raw seeded R-MAT edges as an (E, 2) uint64 array of (src, dst)."""
+    p = RmatParams(scale, num_vertices, num_edges, a, b, c, seed, edge_offset)
+    pairs = np.empty((num_edges, 2), dtype=np.uint64)
+    lib.gz_rmat_draw(C.byref(p), _ptr(pairs))
+    return pairs
+
+
+def sort_edges(pairs: np.ndarray, by_dst: bool, dedupe: bool = True) -> np.ndarray:
+    """Sorted copy grouped by dst (pull) or src (push), optionally deduplicated
+    and without self-loops."""
+    out = np.ascontiguousarray(pairs, dtype=np.uint64).copy()
+    n = lib.gz_edges_sort(_ptr(out), out.shape[0], int(by_dst), int(dedupe))
+    return out[:n]
+
+
+@dataclass
+class EdgeLists:
+    num_vertices: int
+    pull: np.ndarray  # (E, 2) grouped by dst
+    push: np.ndarray  # (E, 2) grouped by src
+
+    @property
+    def num_edges(self) -> int:
+        return int(self.pull.shape[0])
+
+
+def make_graph(pairs: np.ndarray, num_vertices: int, symmetrize: bool = False) -> EdgeLists:
+    if symmetrize:
+        pairs = np.concatenate([pairs, pairs[:, ::-1]], axis=0)
+    pull = sort_edges(pairs, by_dst=True)
+    push = sort_edges(pull, by_dst=False, dedupe=False)
+    return EdgeLists(num_vertices, pull, push)
+
+
+def rmat_graph(scale: int, edge_factor: int, seed: int, symmetrize: bool = False) -> EdgeLists:
+    nv = 1 << scale
+    return make_graph(rmat_edges(scale, nv * edge_factor, seed), nv, symmetrize)
+
+
+def pack_vectors(pairs: np.ndarray, by_dst: bool) -> np.ndarray:
+    pairs = np.ascontiguousarray(pairs, dtype=np.uint64)
+    out = np.zeros((max(1, pairs.shape[0]), 4), dtype=np.uint64)
+    n = lib.gz_pack_vectors(_ptr(pairs), pairs.shape[0], int(by_dst), _ptr(out))
+    return out[:n]
+
+
+def vertex_index(vectors: np.ndarray, num_vertices: int):
+    idx = np.zeros(num_vertices + 64, dtype=np.uint64)
+    s, e = u64(0), u64(0)
+    v = np.ascontiguousarray(vectors, dtype=np.uint64)
+    lib.gz_build_vertex_index(_ptr(v), v.shape[0], num_vertices, _ptr(idx), C.byref(s), C.byref(e))
+    return idx, s.value, e.value
+
+
+def vertex_offsets(vectors: np.ndarray, num_vertices: int) -> np.ndarray:
+    off = np.zeros(num_vertices + 1, dtype=np.uint64)
+    v = np.ascontiguousarray(vectors, dtype=np.uint64)
+    lib.gz_build_vertex_offsets(_ptr(v), v.shape[0], num_vertices, _ptr(off))
+    return off
+
+
+def write_edge_list(path: str, num_vertices: int, pairs: np.ndarray) -> None:
+    pairs = np.ascontiguousarray(pairs, dtype=np.uint64)
+    rc = lib.gz_edge_list_write(path.encode(), num_vertices, pairs.shape[0], _ptr(pairs))
+    if rc != 0:
+        raise OSError(f"cannot write {path} ({rc})")
+
+
+def write_graph_files(base: str, g: EdgeLists) -> None:
+    write_edge_list(base + "-pull", g.num_vertices, g.pull)
+    write_edge_list(base + "-push", g.num_vertices, g.push)
+
+
+# ---- device side -------------------------------------------------------------
+
+def init(device: int = 0) -> str:
+    lib.gz_runtime_init(device)
+    return lib.gz_runtime_device_name().decode()
+
+
+def load(g: EdgeLists, pull: bool = True, push: bool = True) -> None:
+    init()
+    pl = np.ascontiguousarray(g.pull, dtype=np.uint64)
+    ps = np.ascontiguousarray(g.push, dtype=np.uint64)
+    lib.gz_graph_load_pairs(g.num_vertices, pl.shape[0], _ptr(pl) if pull else None, ps.shape[0],
+                            _ptr(ps) if push else None)
+
+
+def set_chunk_vectors(n: int) -> None:
+    phase_config.chunk_vectors = n
+
+
+def set_frontier_skip(on: bool) -> None:
+    phase_config.push_frontier_skip = int(on)
+
+
+def options(algorithm="pr", engine="hybrid", iterations=1, stat="both", pull_variant="sched_aware",
+            damping=0.875, root=0, profile=False, collect_stats=False, max_iterations=0,
+            edge_only=False, vertex_only=False) -> RunOptions:
+    o = RunOptions()
+    lib.gz_run_options_default(C.byref(o))
+    o.algorithm = ALG[algorithm]
+    o.engine = ENGINE[engine]
+    o.iterations = iterations
+    o.stat_mode = STAT[stat]
+    o.pull_variant = PULL_VARIANT[pull_variant]
+    o.damping = damping
+    o.root = root
+    o.profile = int(profile)
+    o.collect_stats = int(collect_stats)
+    o.max_iterations = max_iterations
+    o.edge_only = int(edge_only)
+    o.vertex_only = int(vertex_only)
+    return o
+
+
+def run(o: RunOptions) -> RunResult:
+    r = RunResult()
+    lib.gz_execution_prepare(C.byref(o))
+    lib.gz_execution_run(C.byref(o), C.byref(r))
+    return r
+
+
+def records() -> list[IterationRecord]:
+    n = lib.gz_execution_records(None, 0)
+    arr = (IterationRecord * max(1, n))()
+    lib.gz_execution_records(C.cast(arr, vp), n)
+    return list(arr[:n])
+
+
+def num_vertices() -> int:
+    return int(lib.gz_graph_num_vertices())
+
+
+def pr_ranks() -> np.ndarray:
+    out = np.empty(num_vertices(), dtype=np.float64)
+    lib.gz_result_pr_ranks(_ptr(out))
+    return out
+
+
+def pr_props() -> np.ndarray:
+    out = np.empty(num_vertices(), dtype=np.float64)
+    lib.gz_result_pr_props(_ptr(out))
+    return out
+
+
+def cc_labels() -> np.ndarray:
+    out = np.empty(num_vertices(), dtype=np.uint64)
+    lib.gz_result_cc_labels(_ptr(out))
+    return out
+
+
+def bfs_result():
+    n = num_vertices()
+    parents = np.empty(n, dtype=np.int64)
+    depths = np.empty(n, dtype=np.uint64)
+    lib.gz_result_bfs(_ptr(parents), _ptr(depths))
+    return parents, depths
+
+
+def device_vectors(pull: bool) -> np.ndarray:
+    n = lib.gz_graph_pull_vector_count() if pull else lib.gz_graph_push_vector_count()
+    out = np.zeros((max(1, n), 4), dtype=np.uint64)
+    lib.gz_graph_download_vectors(int(pull), _ptr(out))
+    return out[:n]
+
+
+def device_offsets(pull: bool) -> np.ndarray:
+    out = np.zeros(num_vertices() + 1, dtype=np.uint64)
+    lib.gz_graph_download_offsets(int(pull), _ptr(out))
+    return out
+
+
+def release() -> None:
+    lib.gz_graph_release()
+
+
+def main(argv: list[str]) -> int:
+    args = (C.c_char_p * len(argv))(*[a.encode() for a in argv])
+    return int(lib.gz_main(len(argv), args))
+
+
+# ---- learned selector -------------------------------------------------------
+
+def selector_forward(params: np.ndarray, X: np.ndarray) -> np.ndarray:
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.empty((X.shape[0], 2), dtype=np.float32)
+    lib.gz_selector_forward_batch(_ptr(params), _ptr(X), _ptr(Y), X.shape[0])
+    return Y
+
+
+def selector_gradient(params: np.ndarray, X: np.ndarray, T: np.ndarray):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    g = np.empty(SEL_PARAMS, dtype=np.float32)
+    loss = lib.gz_selector_gradient(_ptr(params), _ptr(X), _ptr(T), X.shape[0], _ptr(g))
+    return float(loss), g
+
+
+def selector_train(params: np.ndarray, X: np.ndarray, T: np.ndarray, steps: int, lr: float):
+    p = np.ascontiguousarray(params, dtype=np.float32).copy()
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    T = np.ascontiguousarray(T, dtype=np.float32)
+    hist = np.empty(steps, dtype=np.float32)
+    lib.gz_selector_train(_ptr(p), _ptr(X), _ptr(T), X.shape[0], steps, lr, _ptr(hist))
+    return p, hist
+
+
+def selector_set(params: np.ndarray, mean: np.ndarray, std: np.ndarray) -> None:
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    m = np.ascontiguousarray(mean, dtype=np.float32)
+    s = np.ascontiguousarray(std, dtype=np.float32)
+    lib.gz_selector_set_weights(_ptr(p), _ptr(m), _ptr(s))
+
+
+def selector_clear() -> None:
+    lib.gz_selector_clear()

@@ -1,0 +1,157 @@
+/*
+ * grazelle_host.h -- host-side C API of libgrazelle.so: edge-list files,
+ * Vector-Sparse ingress, the algorithm drivers and the command line.  These
+ * are the reference's host files (main.c, cmdline.c, graphdata.c,
+ * execution_{pr,cc,bfs}.c, scheduler.c) kept as host C code that reaches the
+ * GPU only through grazelle_phases.h.
+ */
+#ifndef GRAZELLE_HOST_H
+#define GRAZELLE_HOST_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "grazelle_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* -------- binary edge-list files (README.md:50-62, graphdata.c:533-591) --
+ * little-endian uint64: V, E, then E pairs (src, dst).  A "-pull" file is
+ * grouped by dst ascending, a "-push" file by src ascending.  `pairs` holds
+ * 2*E words laid out as in the file. */
+int gz_edge_list_write(const char* path, uint64_t num_vertices, uint64_t num_edges, const uint64_t* pairs);
+/* Returns 0 and a malloc'd pair array, or -1 (missing file), -2 (truncated
+ * header), -3 (truncated body). */
+int gz_edge_list_read(const char* path, uint64_t* num_vertices, uint64_t* num_edges, uint64_t** pairs);
+
+/* Sort pairs by (dst, src) when by_dst, else by (src, dst); drop self-loops
+ * and duplicate edges when dedupe.  Returns the new edge count. */
+uint64_t gz_edges_sort(uint64_t* pairs, uint64_t num_edges, int by_dst, int dedupe);
+
+/* -------- Vector-Sparse ingress (graphdata.c:143-173, 618-788) ----------- */
+
+/* Pack a grouped edge list into 4-edge vectors.  group_by_dst selects the
+ * pull layout (shared vertex = dst).  `out` needs room for num_edges vectors.
+ * Returns the vector count. */
+uint64_t gz_pack_vectors(const uint64_t* pairs, uint64_t num_edges, int group_by_dst, gz_vec_t* out);
+/* Vectors the packer would produce (sum over groups of ceil(deg/4)). */
+uint64_t gz_count_vectors(const uint64_t* pairs, uint64_t num_edges, int group_by_dst);
+/* Reference-format vertex index, index[V+64] (graphdata.c:219-267). */
+void gz_build_vertex_index(const gz_vec_t* vectors, uint64_t vector_count, uint64_t num_vertices,
+                           uint64_t* index, uint64_t* index_start, uint64_t* index_end);
+/* CSR-style offsets over vectors, offsets[V+1]: vectors of v are
+ * [offsets[v], offsets[v+1]). */
+void gz_build_vertex_offsets(const gz_vec_t* vectors, uint64_t vector_count, uint64_t num_vertices, uint64_t* offsets);
+/* FP64 outdegrees counted from the source column (graphdata.c:792-821). */
+void gz_count_outdegrees(const uint64_t* pairs, uint64_t num_edges, uint64_t num_vertices, double* outdeg);
+
+/* -------- seeded R-MAT generator ----------------------------------------- */
+
+typedef struct gz_rmat_params_t {
+    uint32_t scale;            /* vertex ids drawn from [0, 2^scale) */
+    uint64_t num_vertices;     /* 0: 2^scale; else ids >= num_vertices are folded by modulo */
+    uint64_t num_edges;        /* raw edges drawn before dedupe */
+    double a, b, c;            /* quadrant probabilities, d = 1 - a - b - c */
+    uint64_t seed;
+    uint64_t edge_offset;      /* first edge counter (lets callers draw disjoint streams) */
+} gz_rmat_params_t;
+
+/* Draw raw R-MAT edges into pairs[2*num_edges] (src,dst interleaved). */
+void gz_rmat_draw(const gz_rmat_params_t* p, uint64_t* pairs);
+/* splitmix64 step used by the generator (exposed for tests). */
+uint64_t gz_splitmix64(uint64_t x);
+
+/* -------- graph loading into device memory (graphdata.c:957-1049) -------- */
+
+/* Load from host pair arrays already grouped as the files would be.  Either
+ * list may be NULL (then that layout is absent).  Returns 0 or dies. */
+int gz_graph_load_pairs(uint64_t num_vertices, uint64_t pull_edges, const uint64_t* pull_pairs,
+                        uint64_t push_edges, const uint64_t* push_pairs);
+/* Reference entry point: reads "<base>-pull" and "<base>-push".  Exits 255
+ * with a message when a file cannot be read, like graphdata.c:963-967. */
+void graph_data_read_from_file(const char* filename_gather, const char* filename_scatter,
+                               uint32_t num_numa_nodes, const uint32_t* numa_nodes);
+void graph_data_allocate_merge_buffers(uint64_t num_threads, uint32_t num_numa_nodes, const uint32_t* numa_nodes);
+void gz_graph_release(void);
+
+/* Host mirrors filled at load time (for output, statistics and tests). */
+uint64_t gz_graph_num_vertices(void);
+uint64_t gz_graph_num_edges(void);
+uint64_t gz_graph_pull_vector_count(void);
+uint64_t gz_graph_push_vector_count(void);
+const double* gz_graph_outdegrees_host(void);
+/* Copy the device Vector-Sparse lists back (tests). */
+void gz_graph_download_vectors(int pull, gz_vec_t* out);
+void gz_graph_download_offsets(int pull, uint64_t* out);
+
+/* -------- algorithm drivers (execution_{pr,cc,bfs}.c) -------------------- */
+
+enum gz_algorithm { GZ_ALG_PR = 0, GZ_ALG_CC = 1, GZ_ALG_BFS = 2 };
+enum gz_engine_mode {
+    GZ_ENGINE_HYBRID = 0,       /* fixed threshold (execution_cc.c:124-142) */
+    GZ_ENGINE_PULL = 1,         /* EXPERIMENT_EDGE_FORCE_PULL */
+    GZ_ENGINE_PUSH = 2,         /* EXPERIMENT_EDGE_FORCE_PUSH */
+    GZ_ENGINE_LEARNED = 3       /* MLP selector (gz_selector_*) */
+};
+
+typedef struct gz_run_options_t {
+    uint32_t algorithm;         /* enum gz_algorithm */
+    uint32_t engine;            /* enum gz_engine_mode; PR honours PULL/PUSH only */
+    uint32_t iterations;        /* PR iteration count (-N) */
+    uint32_t stat_mode;         /* enum gz_stat_mode */
+    uint32_t pull_variant;      /* enum gz_pull_variant */
+    uint32_t edge_only;         /* EXPERIMENT_EDGE_ONLY (PR) */
+    uint32_t vertex_only;       /* EXPERIMENT_VERTEX_ONLY (PR) */
+    uint32_t profile;           /* record per-iteration engine / time / frontier (ITERATION_PROFILE) */
+    uint32_t collect_stats;     /* record vectors / edges per iteration (ITERATION_STATS) */
+    uint32_t max_iterations;    /* CC/BFS safety cap, 0 = none */
+    double damping;             /* PR damping factor */
+    uint64_t root;              /* BFS search root */
+} gz_run_options_t;
+
+void gz_run_options_default(gz_run_options_t* o);
+
+typedef struct gz_iteration_record_t {
+    uint64_t iteration;         /* 1-based */
+    uint32_t engine;            /* 1 = pull, 2 = push */
+    uint32_t pad;
+    uint64_t frontier_stat;     /* converge_vote that selected the engine */
+    uint64_t vectors;           /* vectors examined (collect_stats) */
+    uint64_t edges;             /* valid lanes examined (collect_stats) */
+    double edge_ms;             /* edge phase time (profile) */
+} gz_iteration_record_t;
+
+typedef struct gz_run_result_t {
+    uint64_t iterations;        /* total_iterations_executed */
+    uint64_t pull_iterations;   /* total_iterations_used_gather */
+    uint64_t push_iterations;   /* total_iterations_used_scatter */
+    double elapsed_ms;          /* wall time of the iteration loop, device-synchronised */
+} gz_run_result_t;
+
+/* Initialise props, accumulators and frontiers for the algorithm
+ * (graph_helper_initialize_vertex_info / _create_and_initialize_frontiers). */
+void gz_execution_prepare(const gz_run_options_t* o);
+/* Run the iteration loop of execution_impl_{pr,cc,bfs}. */
+void gz_execution_run(const gz_run_options_t* o, gz_run_result_t* r);
+/* Per-iteration records of the last run when profile/collect_stats was set. */
+uint64_t gz_execution_records(gz_iteration_record_t* out, uint64_t capacity);
+
+/* Results.  PR: rank(v) = props(v) * outdeg'(v); CC: label; BFS: parent (-1
+ * for root / unreached) and depth (UINT64_MAX unreached). */
+void gz_result_pr_ranks(double* out);
+void gz_result_pr_props(double* out);
+void gz_result_cc_labels(uint64_t* out);
+void gz_result_bfs(int64_t* parents, uint64_t* depths);
+/* The -o text output (graphdata.c:1119-1138). */
+int graph_data_write_ranks_to_file(const char* filename, uint32_t algorithm);
+
+/* Command line (main.c + cmdline.c); returns the process exit code. */
+int gz_main(int argc, char** argv);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GRAZELLE_HOST_H */

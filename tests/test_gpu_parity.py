@@ -1,0 +1,185 @@
+"""HIP path versus the CPU oracle, through the C ABI.
+
+Bars: PageRank (FP64) within 1e-12 maximum relative error per vertex (the
+summation order differs from the oracle's sequential order); CC labels
+bit-exact; BFS depths, parents and the per-iteration frontier statistic
+bit-exact against the oracle's single-thread order (the GPU resolves every
+race deterministically to the parent a single reference thread picks).
+"""
+import numpy as np
+import pytest
+
+import graphs
+
+pytestmark = pytest.mark.gpu
+
+PR_TOL = 1e-12
+
+
+def _orc_graph(orc, g):
+    return orc.Graph(g.num_vertices, g.pull, g.push)
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+@pytest.fixture(scope="module")
+def rmat14(gz):
+    return gz.rmat_graph(14, 16, seed=11)
+
+
+@pytest.mark.parametrize("chunk", [64, 512, 4096])
+def test_pr_pull_sched_aware(gz, orc, device, rmat14, chunk):
+    g = rmat14
+    gz.set_chunk_vectors(chunk)
+    gz.load(g)
+    gz.run(gz.options("pr", iterations=20))
+    ranks = gz.pr_ranks()
+    _, ref, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=20))
+    assert _rel(ranks, ref) <= PR_TOL
+    assert abs(ranks.sum() - 1.0) < 1e-9
+    gz.set_chunk_vectors(512)
+
+
+def test_pr_deterministic(gz, device, rmat14):
+    gz.load(rmat14)
+    gz.run(gz.options("pr", iterations=5))
+    a = gz.pr_props()
+    gz.run(gz.options("pr", iterations=5))
+    b = gz.pr_props()
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+@pytest.mark.parametrize("variant", ["traditional", "push"])
+def test_pr_variants(gz, orc, device, rmat14, variant):
+    g = rmat14
+    gz.load(g)
+    if variant == "push":
+        gz.run(gz.options("pr", iterations=10, engine="push"))
+        _, ref, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=10, engine="push"))
+    else:
+        gz.run(gz.options("pr", iterations=10, pull_variant="traditional"))
+        _, ref, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=10, traditional=True))
+    assert _rel(gz.pr_ranks(), ref) <= PR_TOL
+
+
+def test_pr_damping_085(gz, orc, device):
+    g = gz.rmat_graph(12, 16, seed=4)
+    gz.load(g)
+    gz.run(gz.options("pr", iterations=20, damping=0.85))
+    _, ref, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=20, damping=0.85))
+    assert _rel(gz.pr_ranks(), ref) <= PR_TOL
+
+
+def test_pr_hub_spans_units(gz, orc, device):
+    g = graphs.hub_graph(gz)
+    for chunk in (64, 128):
+        gz.set_chunk_vectors(chunk)
+        gz.load(g)
+        gz.run(gz.options("pr", iterations=20))
+        _, ref, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=20))
+        assert _rel(gz.pr_ranks(), ref) <= PR_TOL
+    gz.set_chunk_vectors(512)
+
+
+@pytest.mark.parametrize("name_idx", range(6))
+def test_small_cases_all_algorithms(gz, orc, device, name_idx):
+    name, g = graphs.small_cases(gz)[name_idx]
+    G = _orc_graph(orc, g)
+    gz.load(g)
+    gz.run(gz.options("pr", iterations=20))
+    _, ref, _ = orc.pagerank(G, orc.options(iterations=20))
+    assert _rel(gz.pr_ranks(), ref) <= PR_TOL, name
+    for engine in ("hybrid", "pull", "push"):
+        gz.run(gz.options("cc", engine=engine))
+        lab, _, _ = orc.cc(G, orc.options(engine=engine))
+        assert np.array_equal(gz.cc_labels(), lab), (name, engine)
+        gz.run(gz.options("bfs", engine=engine, root=0))
+        par, dep, log, _ = orc.bfs(G, orc.options(engine=engine, root=0))
+        p, d = gz.bfs_result()
+        assert np.array_equal(d, dep), (name, engine)
+        assert np.array_equal(p, par), (name, engine)
+
+
+@pytest.mark.parametrize("engine", ["hybrid", "pull", "push"])
+@pytest.mark.parametrize("skip", [True, False])
+def test_cc_labels(gz, orc, device, engine, skip):
+    g = graphs.planted_trees(gz)
+    gz.set_frontier_skip(skip)
+    gz.load(g)
+    gz.run(gz.options("cc", engine=engine))
+    lab, _, _ = orc.cc(_orc_graph(orc, g), orc.options(engine=engine))
+    gz.set_frontier_skip(True)
+    assert np.array_equal(gz.cc_labels(), lab)
+    # planted trees: every tree is its own component labelled by its smallest id
+    base = 1 << 10
+    labs = gz.cc_labels()
+    for t in range(8):
+        ids = np.arange(base + t * 100, base + (t + 1) * 100)
+        assert np.all(labs[ids] == base + t * 100)
+
+
+def test_cc_directed_rmat(gz, orc, device, rmat14):
+    gz.load(rmat14)
+    gz.run(gz.options("cc"))
+    lab, _, _ = orc.cc(_orc_graph(orc, rmat14), orc.options())
+    assert np.array_equal(gz.cc_labels(), lab)
+
+
+@pytest.mark.parametrize("engine", ["hybrid", "pull", "push"])
+@pytest.mark.parametrize("stat", ["both", "outdeg", "count"])
+def test_bfs_bit_exact(gz, orc, device, rmat14, engine, stat):
+    g = rmat14
+    G = _orc_graph(orc, g)
+    gz.load(g)
+    for root in (0, 5, 777):
+        gz.run(gz.options("bfs", engine=engine, stat=stat, root=root, profile=True))
+        par, dep, log, r = orc.bfs(G, orc.options(engine=engine, stat=stat, root=root))
+        p, d = gz.bfs_result()
+        assert np.array_equal(d, dep)
+        assert np.array_equal(p, par)
+        recs = gz.records()
+        assert [(x.engine, x.frontier_stat) for x in recs] == log
+
+
+@pytest.mark.parametrize("skip", [True, False])
+def test_bfs_frontier_skip_modes(gz, orc, device, skip):
+    g = graphs.hub_graph(gz)
+    gz.set_frontier_skip(skip)
+    gz.load(g)
+    gz.run(gz.options("bfs", root=2))
+    par, dep, _, _ = orc.bfs(_orc_graph(orc, g), orc.options(root=2))
+    gz.set_frontier_skip(True)
+    p, d = gz.bfs_result()
+    assert np.array_equal(d, dep) and np.array_equal(p, par)
+
+
+def test_device_layout_matches_host_packer(gz, orc, device, rmat14):
+    g = rmat14
+    gz.load(g)
+    for pull, pairs in ((True, g.pull), (False, g.push)):
+        dv = gz.device_vectors(pull)
+        assert np.array_equal(dv, orc.pack(pairs, by_dst=pull))
+        off = gz.device_offsets(pull)
+        assert off[-1] == dv.shape[0]
+        assert np.all(np.diff(off.astype(np.int64)) >= 0)
+
+
+def test_cli_output_matches_oracle(gz, orc, device, tmp_path, rmat14):
+    import subprocess, os
+    g = gz.rmat_graph(10, 8, seed=9)
+    base = str(tmp_path / "g")
+    gz.write_graph_files(base, g)
+    exe = os.path.join(os.path.dirname(gz.LIB_PATH), "grazelle")
+    out = str(tmp_path / "o.txt")
+    res = subprocess.run([exe, "-i", base, "-a", "cc", "-o", out], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    lab, _, _ = orc.cc(_orc_graph(orc, g), orc.options())
+    want = "".join(f"{i} {float(x):.0f}\n" for i, x in enumerate(lab))
+    assert open(out).read() == want
+    res = subprocess.run([exe, "-i", base, "-N", "20", "-o", out], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 0 and "PageRank Sum" in res.stdout
+    _, ranks, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=20))
+    got = [float(l.split()[1]) for l in open(out)]
+    assert np.allclose(got, ranks, rtol=1e-5, atol=0)
