@@ -70,6 +70,7 @@ void gz_execution_prepare(const gz_run_options_t* o)
     gz_phase_config.stat_mode = o->stat_mode;
     gz_phase_config.pull_variant = o->pull_variant;
     gz_phase_config.collect_stats = o->collect_stats;
+    gz_graph_reset_roles();
     if (gz_reduce_buffer == NULL) gz_reduce_buffer = gz_reduce_buffer_create();
     gz_device_zero(gz_reduce_buffer, 8 * sizeof(uint64_t));
     if (gz_graph.pull_vectors != NULL) gz_merge_buffer_allocate(gz_pull_units_for(gz_graph.pull_vector_count));
@@ -83,7 +84,7 @@ void gz_execution_prepare(const gz_run_options_t* o)
         break;
     case GZ_ALG_CC:
         gz_dev_iota_u64((uint64_t*)gz_graph.vertex_props, V);
-        gz_dev_fill_u64((uint64_t*)gz_graph.vertex_accumulators, 0, V + 8);
+        gz_dev_fill_u64((uint64_t*)gz_graph.vertex_accumulators, 0, fwords);
         gz_dev_fill_u64(gz_graph.frontier_has_info, ~0ull, fwords);
         gz_dev_fill_u64(gz_graph.frontier_wants_info, ~0ull, fwords);
         break;
@@ -96,7 +97,7 @@ void gz_execution_prepare(const gz_run_options_t* o)
         const uint64_t r = o->root;
         gz_dev_fill_u64((uint64_t*)gz_graph.vertex_props, GZ_BFS_UNVISITED, V);
         gz_dev_store_u64((uint64_t*)gz_graph.vertex_props, r, GZ_BFS_PARENT_NONE);
-        gz_dev_fill_u64((uint64_t*)gz_graph.vertex_accumulators, 0, V + 8);
+        gz_dev_fill_u64((uint64_t*)gz_graph.vertex_accumulators, 0, fwords);
         gz_dev_fill_u64(gz_graph.frontier_has_info, 0, fwords);
         gz_dev_store_u64(gz_graph.frontier_has_info, r >> 6, 1ull << (r & 63));
         gz_dev_fill_u64(gz_graph.frontier_wants_info, ~0ull, fwords);

@@ -254,6 +254,18 @@ typedef struct gz_host_graph_t {
 static gz_host_graph_t G;
 
 uint64_t gz_graph_num_vertices(void) { return G.num_vertices; }
+
+/* Give the frontier and accumulator arrays back their load-time roles.  The
+ * CC/BFS drivers swap gz_graph.frontier_has_info and
+ * gz_graph.vertex_accumulators every iteration; the accumulator array holds
+ * V+8 words and the bitmaps only ceil(V/64)+8, so every run must start from
+ * the original assignment. */
+void gz_graph_reset_roles(void)
+{
+    gz_graph.vertex_accumulators = G.d_acc;
+    gz_graph.frontier_has_info = G.d_has;
+    gz_graph.frontier_wants_info = G.d_wants;
+}
 uint64_t gz_graph_num_edges(void) { return G.num_edges; }
 uint64_t gz_graph_pull_vector_count(void) { return G.pull_vectors; }
 uint64_t gz_graph_push_vector_count(void) { return G.push_vectors; }

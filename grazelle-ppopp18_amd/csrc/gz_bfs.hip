@@ -15,10 +15,12 @@
  * Push (top-down): for each active source and each valid lane whose
  * destination is in WantsInfo, atomic-min the destination word with
  * (level << 48 | source).  Exactly one lane sees the "not visited" old value;
- * it clears WantsInfo, sets HasInfo* and counts the frontier statistic once
- * (the reference's weak check and clear are not atomic, phase_edge_push_bfs.asm
- * :108-156).  The surviving parent is the smallest visiting source, which is
- * the parent the reference picks when one thread walks sources in order.
+ * it sets HasInfo* and counts the frontier statistic once (the reference's
+ * weak check and clear are not atomic, phase_edge_push_bfs.asm:108-156).
+ * WantsInfo stays fixed during the kernel and loses the visited vertices in a
+ * second pass, so every visiting source competes and the surviving parent is
+ * the smallest one: the parent the reference picks when one thread walks the
+ * sources in order.
  *
  * Pull (bottom-up): the unvisited destinations are compacted, their vectors
  * walked; a vector qualifies when its destination wants information and some
@@ -87,7 +89,6 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_bfs(const gz_vec_t* _
                 if (!gz_bit_test(wants_info, d)) continue;
                 const unsigned long long old = atomicMin((unsigned long long*)&words[d], key);
                 if (old == GZ_BFS_UNVISITED) {
-                    gz_bit_clear(wants_info, d);
                     gz_bit_set(next_frontier, d);
                     vote += gz_frontier_stat(outdeg, d, stat_mode);
                 }
@@ -103,6 +104,18 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_bfs(const gz_vec_t* _
             stats[2 * gw + 1] = te;
         }
     }
+}
+
+/* WantsInfo &= ~HasInfo*: the push kernel reads WantsInfo as the snapshot of
+ * "not visited before this level" and leaves it unchanged, so that every
+ * visiting source takes part in the atomic min whatever the order of the
+ * lanes; the visited vertices leave WantsInfo here, after the kernel. */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_bfs_retire_wants(uint64_t* __restrict__ wants_info,
+                                                                 const uint64_t* __restrict__ next_frontier,
+                                                                 uint64_t words)
+{
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words; w += (uint64_t)gridDim.x * blockDim.x)
+        wants_info[w] &= ~next_frontier[w];
 }
 
 /* Pass A of the bottom-up step. */
@@ -250,6 +263,12 @@ extern "C" void perform_edge_push_phase_bfs(const gz_vec_t* edge_list, uint64_t 
                            (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
                            gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, level,
                            (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats);
+    GZ_LAUNCH_CHECK();
+    const uint64_t words = (gz_graph.num_vertices + 63) / 64;
+    uint64_t rblocks = (words + GZ_BLOCK - 1) / GZ_BLOCK;
+    if (rblocks > 4096) rblocks = 4096;
+    hipLaunchKernelGGL(gz_k_bfs_retire_wants, dim3((unsigned)rblocks), dim3(GZ_BLOCK), 0, gz_stream,
+                       gz_graph.frontier_wants_info, (const uint64_t*)gz_graph.vertex_accumulators, words);
     GZ_LAUNCH_CHECK();
 }
 

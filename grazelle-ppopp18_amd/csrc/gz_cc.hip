@@ -69,6 +69,10 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_cc(const gz_vec_t* __
                 }
             }
         }
+        /* a run carried from the previous step that ended at the boundary */
+        const uint64_t d0 = __shfl((unsigned long long)d, 0, GZ_WAVE);
+        if (lane == 0 && base != b && d0 != carry_d)
+            gz_cc_apply(labels, next_frontier, outdeg, stat_mode, carry_d, carry_m, vote);
         const uint64_t d_prev = __shfl_up((unsigned long long)d, 1, GZ_WAVE);
         const uint64_t heads = __ballot(lane == 0 || d != d_prev);
         const int seg = gz_segment_start(heads, lane);

@@ -70,6 +70,15 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr(const gz_vec_t* __
             g = gz_pr_vector_sum(v, props, edges);
         }
         s_lane += g;
+        /* the run carried from the previous step ended exactly at the step
+         * boundary: it is complete, store it */
+        const uint64_t d0 = __shfl((unsigned long long)d, 0, GZ_WAVE);
+        if (lane == 0 && base != b && d0 != carry_d) {
+            if (kSchedAware)
+                acc[carry_d] = carry_v;
+            else
+                unsafeAtomicAdd(&acc[carry_d], carry_v);
+        }
         const uint64_t d_prev = __shfl_up((unsigned long long)d, 1, GZ_WAVE);
         const uint64_t heads = __ballot(lane == 0 || d != d_prev);
         const int seg = gz_segment_start(heads, lane);

@@ -61,7 +61,12 @@ def test_pr_variants(gz, orc, device, rmat14, variant):
     else:
         gz.run(gz.options("pr", iterations=10, pull_variant="traditional"))
         _, ref, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=10, traditional=True))
-    assert _rel(gz.pr_ranks(), ref) <= PR_TOL
+    # These are the reference's atomic comparison variants: FP64 atomic adds
+    # land in scheduling order, so a destination with in-degree n carries up
+    # to ~n*eps of order-dependent rounding per iteration (n is several
+    # thousand for this graph's hubs).  The scheduler-aware pull, the main
+    # path, is held to 1e-12 above.
+    assert _rel(gz.pr_ranks(), ref) <= 1e-11
 
 
 def test_pr_damping_085(gz, orc, device):
