@@ -251,6 +251,7 @@ extern "C" void perform_edge_push_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     gz_scratch.partials_count = waves;
     gz_scratch.stats_count = gz_phase_config.collect_stats ? waves : 0;
     const uint64_t level = gz_phase_config.bfs_level;
+    gz_ktimer_begin(GZ_K_PUSH_BFS);
     if (active)
         hipLaunchKernelGGL(gz_k_edge_push_bfs<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
                            src, chunk_len, gz_graph.frontier_has_info, gz_graph.frontier_wants_info,
@@ -264,6 +265,7 @@ extern "C" void perform_edge_push_phase_bfs(const gz_vec_t* edge_list, uint64_t 
                            gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, level,
                            (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats);
     GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_PUSH_BFS);
     const uint64_t words = (gz_graph.num_vertices + 63) / 64;
     uint64_t rblocks = (words + GZ_BLOCK - 1) / GZ_BLOCK;
     if (rblocks > 4096) rblocks = 4096;
@@ -293,6 +295,7 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     gz_scratch.partials_count = visit_blocks * GZ_WAVES_PER_BLOCK;
     gz_scratch.stats_count = gz_phase_config.collect_stats ? blocks * GZ_WAVES_PER_BLOCK : 0;
     const uint64_t level = gz_phase_config.bfs_level;
+    gz_ktimer_begin(GZ_K_PULL_BFS_FIND);
     if (active)
         hipLaunchKernelGGL(gz_k_edge_pull_bfs_find<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
                            edge_list, src, chunk_len, gz_graph.frontier_has_info, gz_graph.frontier_wants_info,
@@ -302,6 +305,8 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
                            edge_list, src, chunk_len, gz_graph.frontier_has_info, gz_graph.frontier_wants_info,
                            gz_scratch.bfs_first_vector, gz_scratch.stats, gz_phase_config.collect_stats);
     GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_PULL_BFS_FIND);
+    gz_ktimer_begin(GZ_K_PULL_BFS_VISIT);
     if (active)
         hipLaunchKernelGGL(gz_k_edge_pull_bfs_visit<true>, dim3((unsigned)visit_blocks), dim3(GZ_BLOCK), 0, gz_stream,
                            edge_list, gz_scratch.active_vertices, gz_scratch.active_totals, gz_graph.num_vertices,
@@ -317,6 +322,7 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
                            gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, level,
                            (uint64_t*)gz_scratch.partials);
     GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_PULL_BFS_VISIT);
 }
 
 extern "C" void gz_clear_bitmap_range(uint64_t* bitmap, uint64_t vertex_start, uint64_t vertex_count);

@@ -106,11 +106,13 @@ extern "C" void perform_edge_pull_phase_cc(const gz_vec_t* edge_list, uint64_t e
     if (nchunks == 0) return;
     const uint64_t chunk_len = (uint64_t)((gz_phase_config.chunk_vectors + GZ_WAVE - 1) / GZ_WAVE * GZ_WAVE);
     const uint64_t blocks = (nchunks + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+    gz_ktimer_begin(GZ_K_PULL_CC);
     hipLaunchKernelGGL(gz_k_edge_pull_cc, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
                        edge_list_count, chunk_len, nchunks, (uint64_t*)gz_graph.vertex_props,
                        (uint64_t*)gz_graph.vertex_accumulators, gz_graph.vertex_outdegrees, gz_phase_config.stat_mode,
                        (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats);
     GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_PULL_CC);
 }
 
 template <bool kActive>
@@ -187,6 +189,7 @@ static void gz_cc_launch_push(const gz_vec_t* edge_list, uint64_t edge_list_coun
     gz_scratch.partials_is_double = 0;
     gz_scratch.partials_count = waves;
     gz_scratch.stats_count = gz_phase_config.collect_stats ? waves : 0;
+    gz_ktimer_begin(GZ_K_PUSH_CC);
     if (active)
         hipLaunchKernelGGL(gz_k_edge_push_cc<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list, src,
                            chunk_len, gz_graph.frontier_has_info, (uint64_t*)gz_graph.vertex_props,
@@ -200,6 +203,7 @@ static void gz_cc_launch_push(const gz_vec_t* edge_list, uint64_t edge_list_coun
                            gz_phase_config.stat_mode, (uint64_t*)gz_scratch.partials, gz_scratch.stats,
                            gz_phase_config.collect_stats);
     GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_PUSH_CC);
 }
 
 extern "C" void perform_edge_push_phase_cc(const gz_vec_t* edge_list, uint64_t edge_list_count)

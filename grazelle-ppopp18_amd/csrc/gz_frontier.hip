@@ -163,6 +163,7 @@ void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64
         gz_device_zero(gz_scratch.active_prefix, sizeof(uint64_t));
         return;
     }
+    gz_ktimer_begin(GZ_K_COMPACT);
     hipLaunchKernelGGL(gz_k_frontier_count, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, offsets,
                        num_vertices, words, gz_scratch.active_block_sums);
     GZ_LAUNCH_CHECK();
@@ -173,4 +174,5 @@ void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64
                        num_vertices, words, gz_scratch.active_block_sums, gz_scratch.active_vertices,
                        gz_scratch.active_prefix);
     GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_COMPACT);
 }

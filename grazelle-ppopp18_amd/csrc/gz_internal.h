@@ -56,6 +56,10 @@ typedef struct gz_scratch_t {
 extern gz_scratch_t gz_scratch;
 extern hipStream_t gz_stream;
 
+/* Bracket a kernel launch for gz_kernel_timing_*; no-ops when disabled. */
+void gz_ktimer_begin(int id);
+void gz_ktimer_end(int id);
+
 /* Make sure the scratch can hold `waves` partials and vertex-sized buffers for V. */
 void gz_scratch_reserve(uint64_t waves, uint64_t num_vertices);
 /* Number of wave chunks covering n vectors at the configured chunk length. */

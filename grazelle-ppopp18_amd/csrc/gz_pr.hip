@@ -135,6 +135,7 @@ extern "C" void perform_edge_pull_phase_pr(const gz_vec_t* edge_list, uint64_t e
     }
     const uint64_t chunk_len = (uint64_t)((gz_phase_config.chunk_vectors + GZ_WAVE - 1) / GZ_WAVE * GZ_WAVE);
     const uint64_t blocks = (nchunks + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+    gz_ktimer_begin(GZ_K_PULL_PR);
     if (sa)
         hipLaunchKernelGGL(gz_k_edge_pull_pr<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
                            edge_list_count, chunk_len, nchunks, (const double*)gz_graph.vertex_props,
@@ -146,6 +147,7 @@ extern "C" void perform_edge_pull_phase_pr(const gz_vec_t* edge_list, uint64_t e
                            (double*)gz_graph.vertex_accumulators, gz_merge_buffer, (double*)gz_scratch.partials,
                            gz_scratch.stats, gz_phase_config.collect_stats);
     GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_PULL_PR);
 }
 
 /* Traditional push: every valid lane adds the source's outbound message to
@@ -199,11 +201,13 @@ extern "C" void perform_edge_push_phase_pr(const gz_vec_t* edge_list, uint64_t e
     if (nchunks == 0) return;
     const uint64_t chunk_len = (uint64_t)((gz_phase_config.chunk_vectors + GZ_WAVE - 1) / GZ_WAVE * GZ_WAVE);
     const uint64_t blocks = (nchunks + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+    gz_ktimer_begin(GZ_K_PUSH_PR);
     hipLaunchKernelGGL(gz_k_edge_push_pr, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
                        edge_list_count, chunk_len, nchunks, (const double*)gz_graph.vertex_props,
                        (double*)gz_graph.vertex_accumulators, (double*)gz_scratch.partials, gz_scratch.stats,
                        gz_phase_config.collect_stats);
     GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_PUSH_PR);
 }
 
 /* props[v] = ((acc[v] + (1-S)/V) * d + (1-d)/V) / outdeg'(v), outdeg' = V
@@ -238,9 +242,11 @@ extern "C" void perform_vertex_phase_pr(uint64_t vertex_start, uint64_t vertex_c
     const uint64_t end = vertex_start + vertex_count;
     uint64_t blocks = (vertex_count + GZ_BLOCK - 1) / GZ_BLOCK;
     if (blocks > 4096) blocks = 4096;
+    gz_ktimer_begin(GZ_K_VERTEX_PR);
     hipLaunchKernelGGL(gz_k_vertex_pr, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
                        (double*)gz_graph.vertex_props, (double*)gz_graph.vertex_accumulators, gz_graph.vertex_outdegrees,
                        vertex_start, end, (double)gz_graph.num_vertices, reduce_buffer, gz_phase_config.damping,
                        gz_pr_acc_needs_zero);
     GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_VERTEX_PR);
 }

@@ -145,6 +145,29 @@ extern "C" void gz_device_zero(void* ptr, size_t bytes)
     }
 }
 
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_copy(gz_u64x2* __restrict__ dst, const gz_u64x2* __restrict__ src,
+                                                      uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" void gz_device_copy(void* dst, const void* src, size_t bytes)
+{
+    gz_require_runtime();
+    if (bytes == 0) return;
+    if ((((uintptr_t)dst | (uintptr_t)src | bytes) & 15) == 0) {
+        const uint64_t n = bytes / 16;
+        uint64_t blocks = (n + GZ_BLOCK - 1) / GZ_BLOCK;
+        if (blocks > 16384) blocks = 16384;
+        hipLaunchKernelGGL(gz_k_copy, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, (gz_u64x2*)dst,
+                           (const gz_u64x2*)src, n);
+        GZ_LAUNCH_CHECK();
+    } else {
+        GZ_CHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, gz_stream));
+    }
+}
+
 extern "C" void gz_synchronize(void)
 {
     gz_require_runtime();
@@ -191,6 +214,72 @@ extern "C" float gz_timer_elapsed_ms(gz_timer_t* t)
     GZ_CHECK(hipEventSynchronize(t->stop));
     GZ_CHECK(hipEventElapsedTime(&ms, t->start, t->stop));
     return ms;
+}
+
+/* ---- per-kernel timing ---------------------------------------------------- */
+
+#define GZ_KT_POOL 2048
+static int gz_kt_on = 0;
+static hipEvent_t* gz_kt_ev = nullptr; /* [GZ_KT_POOL][2] */
+static int gz_kt_ids[GZ_KT_POOL];
+static int gz_kt_used = 0;
+static int gz_kt_open = -1;
+static double gz_kt_ms[GZ_K_COUNT];
+static uint64_t gz_kt_launches[GZ_K_COUNT];
+
+static void gz_kt_flush(void)
+{
+    for (int i = 0; i < gz_kt_used; ++i) {
+        float ms = 0.f;
+        GZ_CHECK(hipEventSynchronize(gz_kt_ev[2 * i + 1]));
+        GZ_CHECK(hipEventElapsedTime(&ms, gz_kt_ev[2 * i], gz_kt_ev[2 * i + 1]));
+        gz_kt_ms[gz_kt_ids[i]] += ms;
+        gz_kt_launches[gz_kt_ids[i]] += 1;
+    }
+    gz_kt_used = 0;
+}
+
+extern "C" void gz_kernel_timing_enable(int on)
+{
+    gz_require_runtime();
+    if (on && gz_kt_ev == nullptr) {
+        gz_kt_ev = (hipEvent_t*)malloc(2 * GZ_KT_POOL * sizeof(hipEvent_t));
+        for (int i = 0; i < 2 * GZ_KT_POOL; ++i) GZ_CHECK(hipEventCreate(&gz_kt_ev[i]));
+    }
+    if (!on && gz_kt_ev != nullptr) gz_kt_flush();
+    gz_kt_on = on;
+}
+
+extern "C" void gz_kernel_timing_reset(void)
+{
+    if (gz_kt_ev != nullptr) gz_kt_flush();
+    for (int i = 0; i < GZ_K_COUNT; ++i) {
+        gz_kt_ms[i] = 0.0;
+        gz_kt_launches[i] = 0;
+    }
+}
+
+extern "C" void gz_kernel_timing_read(int id, double* total_ms, uint64_t* launches)
+{
+    if (gz_kt_ev != nullptr) gz_kt_flush();
+    *total_ms = (id >= 0 && id < GZ_K_COUNT) ? gz_kt_ms[id] : 0.0;
+    *launches = (id >= 0 && id < GZ_K_COUNT) ? gz_kt_launches[id] : 0;
+}
+
+void gz_ktimer_begin(int id)
+{
+    if (!gz_kt_on) return;
+    if (gz_kt_used == GZ_KT_POOL) gz_kt_flush();
+    gz_kt_open = id;
+    GZ_CHECK(hipEventRecord(gz_kt_ev[2 * gz_kt_used], gz_stream));
+}
+
+void gz_ktimer_end(int id)
+{
+    if (!gz_kt_on || gz_kt_open != id) return;
+    GZ_CHECK(hipEventRecord(gz_kt_ev[2 * gz_kt_used + 1], gz_stream));
+    gz_kt_ids[gz_kt_used++] = id;
+    gz_kt_open = -1;
 }
 
 /* ---- scratch -------------------------------------------------------------- */
@@ -305,6 +394,7 @@ void gz_reduce_partials(uint64_t* dst)
     if (blocks < 1) blocks = 1;
     if (blocks > 1024) blocks = 1024;
     unsigned int* done = reinterpret_cast<unsigned int*>(gz_scratch.reduce_tmp + 2040);
+    gz_ktimer_begin(GZ_K_REDUCE);
     if (gz_scratch.partials_is_double) {
         hipLaunchKernelGGL(gz_k_reduce<double>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
                            (const double*)gz_scratch.partials, count, (double*)gz_scratch.reduce_tmp, done, dst);
@@ -314,6 +404,7 @@ void gz_reduce_partials(uint64_t* dst)
                            done, dst);
     }
     GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_REDUCE);
 }
 
 /* ---- common phase operators ----------------------------------------------- */
@@ -369,9 +460,11 @@ extern "C" void edge_pull_op_merge_with_merge_buffer(mergeaccum_t* merge_buffer,
     gz_require_runtime();
     if (count == 0) return;
     const uint64_t blocks = (count + GZ_BLOCK - 1) / GZ_BLOCK;
+    gz_ktimer_begin(GZ_K_MERGE);
     hipLaunchKernelGGL(gz_k_merge_sum, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, merge_buffer, count,
                        vertex_accumulators);
     GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_MERGE);
 }
 
 /* ---- statistics ----------------------------------------------------------- */

@@ -108,6 +108,9 @@ _sig("gz_result_cc_labels", None, vp)
 _sig("gz_result_bfs", None, vp, vp)
 _sig("gz_pull_units_for", u64, u64)
 _sig("gz_main", C.c_int, C.c_int, P(C.c_char_p))
+_sig("gz_kernel_timing_enable", None, C.c_int)
+_sig("gz_kernel_timing_reset", None)
+_sig("gz_kernel_timing_read", None, C.c_int, P(C.c_double), P(u64))
 _sig("gz_selector_set_weights", C.c_int, vp, vp, vp)
 _sig("gz_selector_clear", None)
 _sig("gz_selector_forward_batch", C.c_int, vp, vp, vp, u64)
@@ -169,6 +172,46 @@ def make_graph(pairs: np.ndarray, num_vertices: int, symmetrize: bool = False) -
 def rmat_graph(scale: int, edge_factor: int, seed: int, symmetrize: bool = False) -> EdgeLists:
     nv = 1 << scale
     return make_graph(rmat_edges(scale, nv * edge_factor, seed), nv, symmetrize)
+
+
+def planted_trees(first_id: int, trees: int, tree_size: int, seed: int) -> np.ndarray:
+    """Edges (child, parent) of `trees` random recursive trees of `tree_size`
+    vertices with ids from first_id on: vertex k of a tree attaches to a
+    uniformly drawn earlier vertex of the same tree."""
+    rng = np.random.default_rng(seed)
+    k = np.arange(1, tree_size, dtype=np.uint64)
+    out = []
+    for t in range(trees):
+        base = np.uint64(first_id + t * tree_size)
+        parent = (rng.random(tree_size - 1) * k.astype(np.float64)).astype(np.uint64)
+        out.append(np.stack([base + k, base + parent], axis=1))
+    return np.concatenate(out) if out else np.zeros((0, 2), np.uint64)
+
+
+WORKLOADS = {
+    # name: (description, builder)
+    "pr-rmat16": "PageRank graph: R-MAT scale 16, edge factor 16, a/b/c = 0.57/0.19/0.19, seeded",
+    "cc-rmat22-sym-trees": "CC graph: symmetrised R-MAT scale 22, edge factor 16, plus 64 planted 1000-vertex trees",
+    "bfs-rmat24": "BFS graph: R-MAT scale 24, edge factor 16",
+    "pr-rmat20": "R-MAT scale 20, edge factor 16",
+    "cc-rmat16-sym-trees": "symmetrised R-MAT scale 16 plus 64 planted 100-vertex trees (reduced CC shape)",
+}
+
+
+def workload_graph(name: str, seed: int = 1) -> EdgeLists:
+    if name == "pr-rmat16":
+        return rmat_graph(16, 16, seed)
+    if name == "pr-rmat20":
+        return rmat_graph(20, 16, seed)
+    if name == "bfs-rmat24":
+        return rmat_graph(24, 16, seed)
+    if name in ("cc-rmat22-sym-trees", "cc-rmat16-sym-trees"):
+        scale, tsize = (22, 1000) if name.startswith("cc-rmat22") else (16, 100)
+        nv0 = 1 << scale
+        rm = rmat_edges(scale, nv0 * 16, seed)
+        tr = planted_trees(nv0, 64, tsize, seed + 1000)
+        return make_graph(np.concatenate([rm, tr]), nv0 + 64 * tsize, symmetrize=True)
+    raise KeyError(name)
 
 
 def pack_vectors(pairs: np.ndarray, by_dst: bool) -> np.ndarray:
@@ -307,6 +350,33 @@ def device_offsets(pull: bool) -> np.ndarray:
 
 def release() -> None:
     lib.gz_graph_release()
+
+
+KERNELS = ["pull_pr", "push_pr", "vertex_pr", "merge", "pull_cc", "push_cc", "pull_bfs_find",
+           "pull_bfs_visit", "push_bfs", "compact", "reduce"]
+
+
+def kernel_timing(on: bool) -> None:
+    lib.gz_kernel_timing_enable(int(on))
+    lib.gz_kernel_timing_reset()
+
+
+def kernel_times() -> dict:
+    """{kernel: (total_ms, launches)} recorded with HIP events since the last reset."""
+    out = {}
+    for i, k in enumerate(KERNELS):
+        ms, n = C.c_double(), u64()
+        lib.gz_kernel_timing_read(i, C.byref(ms), C.byref(n))
+        out[k] = (ms.value, n.value)
+    return out
+
+
+def pull_vector_count() -> int:
+    return int(lib.gz_graph_pull_vector_count())
+
+
+def push_vector_count() -> int:
+    return int(lib.gz_graph_push_vector_count())
 
 
 def main(argv: list[str]) -> int:

@@ -157,10 +157,22 @@ void gz_device_free(void* ptr);
 void gz_copy_to_device(void* dst, const void* src, size_t bytes);
 void gz_copy_to_host(void* dst, const void* src, size_t bytes);
 void gz_device_zero(void* ptr, size_t bytes);  /* fill kernel, safe under graph capture */
+void gz_device_copy(void* dst, const void* src, size_t bytes); /* device-to-device, stream ordered */
 void gz_synchronize(void);
 void* gz_runtime_stream(void);                 /* hipStream_t every phase is launched on */
 uint64_t* gz_reduce_buffer_create(void);       /* device reduce buffer, 8 entries */
 void gz_reduce_buffer_destroy(uint64_t* reduce_buffer);
+
+/* Per-kernel launch timing with HIP events on the phase stream (off by
+ * default).  Each phase brackets its main kernel; totals are kept per id. */
+enum gz_kernel_id {
+    GZ_K_PULL_PR = 0, GZ_K_PUSH_PR, GZ_K_VERTEX_PR, GZ_K_MERGE, GZ_K_PULL_CC, GZ_K_PUSH_CC,
+    GZ_K_PULL_BFS_FIND, GZ_K_PULL_BFS_VISIT, GZ_K_PUSH_BFS, GZ_K_COMPACT, GZ_K_REDUCE, GZ_K_COUNT
+};
+void gz_kernel_timing_enable(int on);
+void gz_kernel_timing_reset(void);
+/* Synchronises; total milliseconds and launches recorded for `id`. */
+void gz_kernel_timing_read(int id, double* total_ms, uint64_t* launches);
 
 /* Stream-ordered timer built on HIP events. */
 typedef struct gz_timer_t gz_timer_t;
