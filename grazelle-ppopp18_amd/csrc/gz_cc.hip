@@ -21,6 +21,7 @@
  * source's label (read once per vector, as the asm reads props[src] once).
  */
 #include "gz_slots.h"
+#include "gz_walk.h"
 
 __device__ __forceinline__ void gz_cc_apply(uint64_t* __restrict__ labels, uint64_t* __restrict__ next_frontier,
                                             const double* __restrict__ outdeg, uint32_t stat_mode, uint64_t d,
@@ -35,6 +36,35 @@ __device__ __forceinline__ void gz_cc_apply(uint64_t* __restrict__ labels, uint6
     }
 }
 
+struct gz_cc_pull_op {
+    uint64_t* __restrict__ labels;
+    uint64_t* __restrict__ next_frontier;
+    const double* __restrict__ outdeg;
+    uint32_t stat_mode;
+    uint64_t vote;
+    __device__ __forceinline__ uint64_t identity() const { return ~0ull; }
+    __device__ __forceinline__ uint64_t combine(uint64_t a, uint64_t b) const { return a < b ? a : b; }
+    __device__ __forceinline__ uint64_t gather(const gz_lanes& v, uint32_t& edges) const
+    {
+        uint64_t m = ~0ull;
+        const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (gz_lane_valid(l[k])) {
+                const uint64_t x = labels[gz_lane_id(l[k])];
+                m = (x < m) ? x : m;
+                edges += 1;
+            }
+        }
+        return m;
+    }
+    __device__ __forceinline__ void store(uint64_t d, uint64_t m)
+    {
+        gz_cc_apply(labels, next_frontier, outdeg, stat_mode, d, m, vote);
+    }
+};
+
+template <int VPL>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_cc(const gz_vec_t* __restrict__ vecs, uint64_t nvec,
                                                               uint64_t chunk_len, uint64_t nchunks,
                                                               uint64_t* __restrict__ labels,
@@ -48,44 +78,12 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_cc(const gz_vec_t* __
     if (chunk >= nchunks) return;
     const uint64_t b = chunk * chunk_len;
     const uint64_t e = (b + chunk_len < nvec) ? b + chunk_len : nvec;
-
-    uint64_t carry_d = ~0ull, carry_m = ~0ull;
-    uint64_t vote = 0;
+    gz_cc_pull_op op{labels, next_frontier, outdeg, stat_mode, 0};
+    uint64_t first_d, carry_d, carry_m;
     uint32_t edges = 0;
-    for (uint64_t base = b; base < e; base += GZ_WAVE) {
-        const uint64_t i = base + (uint64_t)lane;
-        const bool valid = i < e;
-        uint64_t d = ~0ull, m = ~0ull;
-        if (valid) {
-            const gz_lanes v = gz_load_vector(vecs, i);
-            d = gz_shared_vertex(v);
-            const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
-#pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                if (gz_lane_valid(l[k])) {
-                    const uint64_t x = labels[gz_lane_id(l[k])];
-                    m = (x < m) ? x : m;
-                    edges += 1;
-                }
-            }
-        }
-        /* a run carried from the previous step that ended at the boundary */
-        const uint64_t d0 = __shfl((unsigned long long)d, 0, GZ_WAVE);
-        if (lane == 0 && base != b && d0 != carry_d)
-            gz_cc_apply(labels, next_frontier, outdeg, stat_mode, carry_d, carry_m, vote);
-        const uint64_t d_prev = __shfl_up((unsigned long long)d, 1, GZ_WAVE);
-        const uint64_t heads = __ballot(lane == 0 || d != d_prev);
-        const int seg = gz_segment_start(heads, lane);
-        uint64_t s = gz_segscan_min(m, lane, seg);
-        if (d == carry_d && carry_m < s) s = carry_m;
-        const int last = (int)((e - 1 - base) < (uint64_t)(GZ_WAVE - 1) ? (e - 1 - base) : (uint64_t)(GZ_WAVE - 1));
-        const bool seg_end = (lane == GZ_WAVE - 1) || ((heads >> (lane + 1)) & 1ull);
-        if (valid && seg_end && lane != last) gz_cc_apply(labels, next_frontier, outdeg, stat_mode, d, s, vote);
-        carry_d = __shfl((unsigned long long)d, last, GZ_WAVE);
-        carry_m = __shfl((unsigned long long)s, last, GZ_WAVE);
-    }
-    if (lane == 0) gz_cc_apply(labels, next_frontier, outdeg, stat_mode, carry_d, carry_m, vote);
-    const uint64_t total = gz_wave_sum_u64(vote);
+    gz_walk_chunk<VPL, uint64_t>(vecs, b, e, op, first_d, carry_d, carry_m, edges);
+    if (lane == 0) op.store(carry_d, carry_m);
+    const uint64_t total = gz_wave_sum_u64(op.vote);
     if (lane == 0) partials[chunk] = total;
     if (collect) {
         const uint64_t tot_edges = gz_wave_sum_u64(edges);
@@ -104,13 +102,21 @@ extern "C" void perform_edge_pull_phase_cc(const gz_vec_t* edge_list, uint64_t e
     gz_scratch.partials_count = nchunks;
     gz_scratch.stats_count = gz_phase_config.collect_stats ? nchunks : 0;
     if (nchunks == 0) return;
-    const uint64_t chunk_len = (uint64_t)((gz_phase_config.chunk_vectors + GZ_WAVE - 1) / GZ_WAVE * GZ_WAVE);
+    const uint64_t chunk_len = gz_chunk_len();
     const uint64_t blocks = (nchunks + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+    const uint32_t vpl = gz_vectors_per_lane();
+#define GZ_PULL_CC_ARGS                                                                                       \
+    dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list, edge_list_count, chunk_len, nchunks,      \
+        (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators, gz_graph.vertex_outdegrees, \
+        gz_phase_config.stat_mode, (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats
     gz_ktimer_begin(GZ_K_PULL_CC);
-    hipLaunchKernelGGL(gz_k_edge_pull_cc, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
-                       edge_list_count, chunk_len, nchunks, (uint64_t*)gz_graph.vertex_props,
-                       (uint64_t*)gz_graph.vertex_accumulators, gz_graph.vertex_outdegrees, gz_phase_config.stat_mode,
-                       (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats);
+    if (vpl == 4)
+        hipLaunchKernelGGL(gz_k_edge_pull_cc<4>, GZ_PULL_CC_ARGS);
+    else if (vpl == 2)
+        hipLaunchKernelGGL(gz_k_edge_pull_cc<2>, GZ_PULL_CC_ARGS);
+    else
+        hipLaunchKernelGGL(gz_k_edge_pull_cc<1>, GZ_PULL_CC_ARGS);
+#undef GZ_PULL_CC_ARGS
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_PULL_CC);
 }

@@ -64,6 +64,9 @@ void gz_ktimer_end(int id);
 void gz_scratch_reserve(uint64_t waves, uint64_t num_vertices);
 /* Number of wave chunks covering n vectors at the configured chunk length. */
 uint64_t gz_chunks_for(uint64_t n);
+/* Chunk length in vectors (a multiple of 64 x vectors per lane). */
+uint64_t gz_chunk_len(void);
+uint32_t gz_vectors_per_lane(void);
 /* Reduce gz_scratch.partials[0..count) into dst[0] (bit pattern). */
 void gz_reduce_partials(uint64_t* dst);
 /* Build the compacted active-vertex list from `bitmap` against `offsets`. */

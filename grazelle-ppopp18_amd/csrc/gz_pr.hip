@@ -17,7 +17,7 @@
  * of all gathered values; each wavefront writes its share to the partials.
  * No floating-point atomics, so results are reproducible run to run.
  */
-#include "gz_internal.h"
+#include "gz_walk.h"
 
 /* Set by the edge phases: the scheduler-aware pull overwrites every
  * accumulator it owns, the traditional pull and the push accumulate with
@@ -37,7 +37,29 @@ __device__ __forceinline__ double gz_pr_vector_sum(const gz_lanes& v, const doub
     return (g2 + g3) + (g0 + g1);
 }
 
-template <bool kSchedAware>
+struct gz_pr_pull_op {
+    const double* __restrict__ props;
+    double* __restrict__ acc;
+    bool sched_aware;
+    double sum;
+    __device__ __forceinline__ double identity() const { return 0.0; }
+    __device__ __forceinline__ double combine(double a, double b) const { return a + b; }
+    __device__ __forceinline__ double gather(const gz_lanes& v, uint32_t& edges)
+    {
+        const double g = gz_pr_vector_sum(v, props, edges);
+        sum += g;
+        return g;
+    }
+    __device__ __forceinline__ void store(uint64_t d, double s)
+    {
+        if (sched_aware)
+            acc[d] = s;
+        else
+            unsafeAtomicAdd(&acc[d], s);
+    }
+};
+
+template <bool kSchedAware, int VPL>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr(const gz_vec_t* __restrict__ vecs, uint64_t nvec,
                                                               uint64_t chunk_len, uint64_t nchunks,
                                                               const double* __restrict__ props,
@@ -51,53 +73,12 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr(const gz_vec_t* __
     if (chunk >= nchunks) return;
     const uint64_t b = chunk * chunk_len;
     const uint64_t e = (b + chunk_len < nvec) ? b + chunk_len : nvec;
-
-    const gz_lanes first = gz_load_vector(vecs, b);
-    const uint64_t first_d = gz_shared_vertex(first);
-    uint64_t carry_d = first_d;
-    double carry_v = 0.0;
-    double s_lane = 0.0;
+    gz_pr_pull_op op{props, acc, kSchedAware, 0.0};
+    uint64_t first_d, carry_d;
+    double carry_v;
     uint32_t edges = 0;
-
-    for (uint64_t base = b; base < e; base += GZ_WAVE) {
-        const uint64_t i = base + (uint64_t)lane;
-        const bool valid = i < e;
-        uint64_t d = ~0ull;
-        double g = 0.0;
-        if (valid) {
-            const gz_lanes v = gz_load_vector(vecs, i);
-            d = gz_shared_vertex(v);
-            g = gz_pr_vector_sum(v, props, edges);
-        }
-        s_lane += g;
-        /* the run carried from the previous step ended exactly at the step
-         * boundary: it is complete, store it */
-        const uint64_t d0 = __shfl((unsigned long long)d, 0, GZ_WAVE);
-        if (lane == 0 && base != b && d0 != carry_d) {
-            if (kSchedAware)
-                acc[carry_d] = carry_v;
-            else
-                unsafeAtomicAdd(&acc[carry_d], carry_v);
-        }
-        const uint64_t d_prev = __shfl_up((unsigned long long)d, 1, GZ_WAVE);
-        const uint64_t heads = __ballot(lane == 0 || d != d_prev);
-        const int seg = gz_segment_start(heads, lane);
-        double s = gz_segscan_sum(g, lane, seg);
-        if (d == carry_d) s = s + carry_v;
-        const int last = (int)((e - 1 - base) < (uint64_t)(GZ_WAVE - 1) ? (e - 1 - base) : (uint64_t)(GZ_WAVE - 1));
-        const bool seg_end = (lane == GZ_WAVE - 1) || ((heads >> (lane + 1)) & 1ull);
-        if (valid && seg_end && lane != last) {
-            if (kSchedAware)
-                acc[d] = s;
-            else
-                unsafeAtomicAdd(&acc[d], s);
-        }
-        carry_d = __shfl((unsigned long long)d, last, GZ_WAVE);
-        carry_v = __shfl(s, last, GZ_WAVE);
-    }
-
-    const double S = gz_wave_sum(s_lane);
-    if (!kSchedAware && lane == 0) unsafeAtomicAdd(&acc[carry_d], carry_v);
+    gz_walk_chunk<VPL, double>(vecs, b, e, op, first_d, carry_d, carry_v, edges);
+    const double S = gz_wave_sum(op.sum);
     if (lane == 0) {
         if (kSchedAware) {
             mergeaccum_t r;
@@ -106,6 +87,8 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr(const gz_vec_t* __
             r.final_partial_value = carry_v;
             r.__padding__ = 0;
             merge[chunk] = r;
+        } else {
+            unsafeAtomicAdd(&acc[carry_d], carry_v);
         }
         partials[chunk] = S;
     }
@@ -116,6 +99,24 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr(const gz_vec_t* __
             stats[2 * chunk + 1] = tot_edges;
         }
     }
+}
+
+template <bool kSchedAware>
+static void gz_launch_pull_pr(uint64_t blocks, const gz_vec_t* edge_list, uint64_t count, uint64_t chunk_len,
+                              uint64_t nchunks)
+{
+    const uint32_t vpl = gz_vectors_per_lane();
+#define GZ_PULL_PR_ARGS                                                                                        \
+    dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list, count, chunk_len, nchunks,                 \
+        (const double*)gz_graph.vertex_props, (double*)gz_graph.vertex_accumulators, gz_merge_buffer,          \
+        (double*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats
+    if (vpl == 4)
+        hipLaunchKernelGGL((gz_k_edge_pull_pr<kSchedAware, 4>), GZ_PULL_PR_ARGS);
+    else if (vpl == 2)
+        hipLaunchKernelGGL((gz_k_edge_pull_pr<kSchedAware, 2>), GZ_PULL_PR_ARGS);
+    else
+        hipLaunchKernelGGL((gz_k_edge_pull_pr<kSchedAware, 1>), GZ_PULL_PR_ARGS);
+#undef GZ_PULL_PR_ARGS
 }
 
 extern "C" void perform_edge_pull_phase_pr(const gz_vec_t* edge_list, uint64_t edge_list_count)
@@ -133,19 +134,13 @@ extern "C" void perform_edge_pull_phase_pr(const gz_vec_t* edge_list, uint64_t e
                 (unsigned long long)gz_merge_buffer_count, (unsigned long long)nchunks);
         exit(72);
     }
-    const uint64_t chunk_len = (uint64_t)((gz_phase_config.chunk_vectors + GZ_WAVE - 1) / GZ_WAVE * GZ_WAVE);
+    const uint64_t chunk_len = gz_chunk_len();
     const uint64_t blocks = (nchunks + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
     gz_ktimer_begin(GZ_K_PULL_PR);
     if (sa)
-        hipLaunchKernelGGL(gz_k_edge_pull_pr<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
-                           edge_list_count, chunk_len, nchunks, (const double*)gz_graph.vertex_props,
-                           (double*)gz_graph.vertex_accumulators, gz_merge_buffer, (double*)gz_scratch.partials,
-                           gz_scratch.stats, gz_phase_config.collect_stats);
+        gz_launch_pull_pr<true>(blocks, edge_list, edge_list_count, chunk_len, nchunks);
     else
-        hipLaunchKernelGGL(gz_k_edge_pull_pr<false>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
-                           edge_list_count, chunk_len, nchunks, (const double*)gz_graph.vertex_props,
-                           (double*)gz_graph.vertex_accumulators, gz_merge_buffer, (double*)gz_scratch.partials,
-                           gz_scratch.stats, gz_phase_config.collect_stats);
+        gz_launch_pull_pr<false>(blocks, edge_list, edge_list_count, chunk_len, nchunks);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_PULL_PR);
 }
@@ -199,7 +194,7 @@ extern "C" void perform_edge_push_phase_pr(const gz_vec_t* edge_list, uint64_t e
     gz_scratch.partials_count = nchunks;
     gz_scratch.stats_count = gz_phase_config.collect_stats ? nchunks : 0;
     if (nchunks == 0) return;
-    const uint64_t chunk_len = (uint64_t)((gz_phase_config.chunk_vectors + GZ_WAVE - 1) / GZ_WAVE * GZ_WAVE);
+    const uint64_t chunk_len = gz_chunk_len();
     const uint64_t blocks = (nchunks + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
     gz_ktimer_begin(GZ_K_PUSH_PR);
     hipLaunchKernelGGL(gz_k_edge_push_pr, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,

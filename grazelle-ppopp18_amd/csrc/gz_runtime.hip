@@ -22,6 +22,7 @@ gz_phase_config_t gz_phase_config = {
     1,                        /* bfs_level */
     0,                        /* collect_stats */
     1,                        /* push_frontier_skip */
+    1,                        /* vectors_per_lane */
 };
 mergeaccum_t* gz_merge_buffer = nullptr;
 uint64_t gz_merge_buffer_count = 0;
@@ -284,11 +285,24 @@ void gz_ktimer_end(int id)
 
 /* ---- scratch -------------------------------------------------------------- */
 
+uint32_t gz_vectors_per_lane(void)
+{
+    const uint32_t v = gz_phase_config.vectors_per_lane;
+    return (v >= 4) ? 4u : (v >= 2 ? 2u : 1u);
+}
+
+uint64_t gz_chunk_len(void)
+{
+    /* a multiple of one wavefront step (64 lanes x vectors per lane) */
+    const uint64_t step = (uint64_t)GZ_WAVE * gz_vectors_per_lane();
+    uint64_t ch = gz_phase_config.chunk_vectors;
+    if (ch < step) ch = step;
+    return (ch + step - 1) / step * step;
+}
+
 uint64_t gz_chunks_for(uint64_t n)
 {
-    uint64_t ch = gz_phase_config.chunk_vectors;
-    if (ch < GZ_WAVE) ch = GZ_WAVE;
-    ch = (ch + GZ_WAVE - 1) / GZ_WAVE * GZ_WAVE;
+    const uint64_t ch = gz_chunk_len();
     return (n + ch - 1) / ch;
 }
 

@@ -1,0 +1,150 @@
+/*
+ * gz_walk.h -- the wavefront chunk walker shared by the pull sweeps
+ * (PageRank sum, CC min).
+ *
+ * A wavefront owns the vectors [b, e) of one scheduling unit.  Each step covers
+ * 64*VPL vectors: lane L holds VPL consecutive vectors (one 32*VPL-byte
+ * contiguous slice, so a step reads 2*VPL KiB with every byte used) and the
+ * next step's slice is loaded before the current one is reduced.
+ *
+ * Reduction of a step, keyed by destination ("runs" of equal destinations):
+ *   1. lane-local: the lane folds its VPL values; runs that start and end
+ *      inside the lane (interior runs) are complete and stored at once; the
+ *      lane keeps its head run (h_d, h_s) and tail run (t_d, t_s), which are
+ *      the same run when the lane holds a single destination;
+ *   2. cross-lane: an inclusive segmented scan of t_s keyed by t_d joins the
+ *      tails of lanes with the single-run lanes that follow them; a lane whose
+ *      head continues the previous lane's tail adds the previous lane's
+ *      scanned value to its head;
+ *   3. the run still open at the last valid lane is carried to the next step,
+ *      and a carried run that the next step does not continue is stored.
+ * The run open when the chunk ends is returned to the caller (merge record or
+ * direct atomic), together with the chunk's first destination.
+ *
+ * Op supplies: T identity(); T combine(T, T); T gather(const gz_lanes&,
+ * uint32_t& edges); void store(uint64_t d, T value) for a complete run.
+ */
+#ifndef GZ_WALK_H
+#define GZ_WALK_H
+
+#include "gz_internal.h"
+
+template <typename T>
+__device__ __forceinline__ T gz_shfl_up_t(T x, int o)
+{
+    return __shfl_up(x, o, GZ_WAVE);
+}
+
+template <>
+__device__ __forceinline__ uint64_t gz_shfl_up_t<uint64_t>(uint64_t x, int o)
+{
+    return (uint64_t)__shfl_up((unsigned long long)x, o, GZ_WAVE);
+}
+
+template <typename T>
+__device__ __forceinline__ T gz_shfl_t(T x, int src)
+{
+    return __shfl(x, src, GZ_WAVE);
+}
+
+template <>
+__device__ __forceinline__ uint64_t gz_shfl_t<uint64_t>(uint64_t x, int src)
+{
+    return (uint64_t)__shfl((unsigned long long)x, src, GZ_WAVE);
+}
+
+template <int VPL>
+struct gz_slice {
+    gz_lanes v[VPL];
+};
+
+template <int VPL>
+__device__ __forceinline__ void gz_load_slice(gz_slice<VPL>& s, const gz_vec_t* __restrict__ vecs, uint64_t first,
+                                              uint64_t e)
+{
+#pragma unroll
+    for (int k = 0; k < VPL; ++k) {
+        const uint64_t i = first + (uint64_t)k;
+        if (i < e) {
+            s.v[k] = gz_load_vector(vecs, i);
+        } else {
+            s.v[k].l0 = s.v[k].l1 = s.v[k].l2 = s.v[k].l3 = 0;
+        }
+    }
+}
+
+template <int VPL, typename T, typename Op>
+__device__ __forceinline__ void gz_walk_chunk(const gz_vec_t* __restrict__ vecs, uint64_t b, uint64_t e, Op& op,
+                                              uint64_t& first_d, uint64_t& carry_d, T& carry_v, uint32_t& edges)
+{
+    const int lane = gz_lane();
+    const uint64_t step = (uint64_t)GZ_WAVE * VPL;
+    first_d = gz_shared_vertex(gz_load_vector(vecs, b));
+    carry_d = first_d;
+    carry_v = op.identity();
+    gz_slice<VPL> cur, nxt;
+    gz_load_slice<VPL>(cur, vecs, b + (uint64_t)lane * VPL, e);
+    for (uint64_t base = b; base < e; base += step) {
+        const uint64_t my_first = base + (uint64_t)lane * VPL;
+        if (base + step < e) gz_load_slice<VPL>(nxt, vecs, my_first + step, e);
+        /* 1. lane-local runs */
+        int nvalid = 0;
+        uint64_t h_d = ~0ull, t_d = ~0ull;
+        T h_s = op.identity(), t_s = op.identity();
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) {
+            if (my_first + (uint64_t)k >= e) break;
+            const uint64_t d = gz_shared_vertex(cur.v[k]);
+            const T g = op.gather(cur.v[k], edges);
+            if (nvalid == 0) {
+                h_d = t_d = d;
+                h_s = t_s = g;
+            } else if (d == t_d) {
+                t_s = op.combine(t_s, g);
+                if (t_d == h_d) h_s = t_s;
+            } else {
+                if (t_d != h_d) op.store(t_d, t_s); /* interior run, complete */
+                t_d = d;
+                t_s = g;
+            }
+            ++nvalid;
+        }
+        const bool has = nvalid > 0;
+        const bool single = has && (h_d == t_d);
+        /* a carried run the step does not continue is complete */
+        const uint64_t h0 = gz_shfl_t<uint64_t>(h_d, 0);
+        if (lane == 0 && base != b && h0 != carry_d) op.store(carry_d, carry_v);
+        /* 2. cross-lane segmented scan of tails keyed by t_d */
+        const uint64_t prev_t = gz_shfl_up_t<uint64_t>(t_d, 1);
+        const uint64_t heads = __ballot(lane == 0 || t_d != prev_t);
+        const int seg = gz_segment_start(heads, lane);
+        T S = t_s;
+#pragma unroll
+        for (int o = 1; o < GZ_WAVE; o <<= 1) {
+            const T x = gz_shfl_up_t<T>(S, o);
+            if (lane - o >= seg) S = op.combine(S, x);
+        }
+        /* lanes whose scan segment starts at lane 0 and continues the carry */
+        if (seg == 0 && t_d == carry_d && has) S = op.combine(S, carry_v);
+        const T S_prev = gz_shfl_up_t<T>(S, 1);
+        /* the head of a multi-run lane completes inside the lane */
+        if (has && !single) {
+            T head = h_s;
+            if (lane == 0) {
+                if (h_d == carry_d) head = op.combine(head, carry_v);
+            } else if (prev_t == h_d) {
+                head = op.combine(head, S_prev);
+            }
+            op.store(h_d, head);
+        }
+        /* 3. tails: complete when the next lane starts another destination */
+        const uint64_t next_h = gz_shfl_t<uint64_t>(h_d, (lane + 1) & (GZ_WAVE - 1));
+        const int last = __popcll(__ballot(has)) - 1; /* valid lanes form a prefix */
+        if (has && lane != last && next_h != t_d) op.store(t_d, S);
+        carry_d = gz_shfl_t<uint64_t>(t_d, last);
+        carry_v = gz_shfl_t<T>(S, last);
+        if (base + step < e) cur = nxt;
+    }
+}
+
+#endif /* GZ_WALK_H */

@@ -57,7 +57,7 @@ class RmatParams(C.Structure):
 class PhaseConfig(C.Structure):
     _fields_ = [("damping", C.c_double), ("stat_mode", u32), ("pull_variant", u32),
                 ("chunk_vectors", u32), ("bfs_level", u32), ("collect_stats", u32),
-                ("push_frontier_skip", u32)]
+                ("push_frontier_skip", u32), ("vectors_per_lane", u32)]
 
 
 class PhaseGraph(C.Structure):
@@ -265,6 +265,10 @@ def load(g: EdgeLists, pull: bool = True, push: bool = True) -> None:
 
 def set_chunk_vectors(n: int) -> None:
     phase_config.chunk_vectors = n
+
+
+def set_vectors_per_lane(n: int) -> None:
+    phase_config.vectors_per_lane = n
 
 
 def set_frontier_skip(on: bool) -> None:

@@ -91,6 +91,7 @@ typedef struct gz_phase_config_t {
     uint32_t bfs_level;                    /* BFS depth written by the current edge phase (1-based) */
     uint32_t collect_stats;                /* nonzero: count vectors / valid lanes examined (ITERATION_STATS) */
     uint32_t push_frontier_skip;           /* nonzero: push phases visit only frontier sources (default 1) */
+    uint32_t vectors_per_lane;             /* pull sweeps: consecutive vectors per lane per step (1, 2 or 4) */
 } gz_phase_config_t;
 
 extern gz_phase_config_t gz_phase_config;

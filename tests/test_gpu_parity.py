@@ -29,17 +29,20 @@ def rmat14(gz):
     return gz.rmat_graph(14, 16, seed=11)
 
 
+@pytest.mark.parametrize("vpl", [1, 2, 4])
 @pytest.mark.parametrize("chunk", [64, 512, 4096])
-def test_pr_pull_sched_aware(gz, orc, device, rmat14, chunk):
+def test_pr_pull_sched_aware(gz, orc, device, rmat14, chunk, vpl):
     g = rmat14
     gz.set_chunk_vectors(chunk)
+    gz.set_vectors_per_lane(vpl)
     gz.load(g)
     gz.run(gz.options("pr", iterations=20))
     ranks = gz.pr_ranks()
+    gz.set_chunk_vectors(512)
+    gz.set_vectors_per_lane(1)
     _, ref, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=20))
     assert _rel(ranks, ref) <= PR_TOL
     assert abs(ranks.sum() - 1.0) < 1e-9
-    gz.set_chunk_vectors(512)
 
 
 def test_pr_deterministic(gz, device, rmat14):
@@ -77,15 +80,33 @@ def test_pr_damping_085(gz, orc, device):
     assert _rel(gz.pr_ranks(), ref) <= PR_TOL
 
 
-def test_pr_hub_spans_units(gz, orc, device):
+@pytest.mark.parametrize("vpl", [1, 2, 4])
+def test_pr_hub_spans_units(gz, orc, device, vpl):
     g = graphs.hub_graph(gz)
-    for chunk in (64, 128):
+    gz.set_vectors_per_lane(vpl)
+    _, ref, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=20))
+    for chunk in (64, 128, 256):
         gz.set_chunk_vectors(chunk)
         gz.load(g)
         gz.run(gz.options("pr", iterations=20))
-        _, ref, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=20))
-        assert _rel(gz.pr_ranks(), ref) <= PR_TOL
-    gz.set_chunk_vectors(512)
+        got = gz.pr_ranks()
+        gz.set_chunk_vectors(512)
+        assert _rel(got, ref) <= PR_TOL
+    gz.set_vectors_per_lane(1)
+
+
+@pytest.mark.parametrize("vpl", [1, 2, 4])
+def test_cc_pull_vectors_per_lane(gz, orc, device, rmat14, vpl):
+    gz.set_vectors_per_lane(vpl)
+    for chunk in (128, 512):
+        gz.set_chunk_vectors(chunk)
+        gz.load(rmat14)
+        gz.run(gz.options("cc", engine="pull"))
+        got = gz.cc_labels()
+        gz.set_chunk_vectors(512)
+        lab, _, _ = orc.cc(_orc_graph(orc, rmat14), orc.options(engine="pull"))
+        assert np.array_equal(got, lab)
+    gz.set_vectors_per_lane(1)
 
 
 @pytest.mark.parametrize("name_idx", range(6))

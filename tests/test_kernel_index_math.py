@@ -107,6 +107,132 @@ def test_pull_emulation_boundary_runs(gz, orc):
         assert acc[dst] == 200 + 201 + 202 + 203
 
 
+def emulate_walk(vectors, props, chunk, V, vpl):
+    """Replay of gz_walk_chunk (gz_walk.h) with the PageRank sum op: lanes hold
+    `vpl` consecutive vectors; lane-local runs, a segmented scan of lane tails,
+    head fix-ups, and the carry between steps.  Returns acc after the direct
+    stores plus the merge records."""
+    n = vectors.shape[0]
+    d_all = shared_of(vectors)
+    valid = (vectors >> np.uint64(63)).astype(bool)
+    ids = (vectors & np.uint64((1 << 48) - 1)).astype(np.int64)
+    gv = np.where(valid, props[np.where(valid, ids, 0)], 0.0)
+    g_all = (gv[:, 2] + gv[:, 3]) + (gv[:, 0] + gv[:, 1])
+    acc = np.zeros(V)
+    stores = np.zeros(V, dtype=np.int64)
+
+    def store(d, s):
+        assert 0 <= d < V
+        acc[d] = s
+        stores[d] += 1
+
+    step = WAVE * vpl
+    merge = []
+    for c in range((n + chunk - 1) // chunk):
+        b, e = c * chunk, min(c * chunk + chunk, n)
+        first_d = d_all[b]
+        carry_d, carry_v = first_d, 0.0
+        for base in range(b, e, step):
+            h_d = [-1] * WAVE; t_d = [-1] * WAVE; h_s = [0.0] * WAVE; t_s = [0.0] * WAVE
+            has = [False] * WAVE
+            for L in range(WAVE):
+                nvalid = 0
+                for k in range(vpl):
+                    i = base + L * vpl + k
+                    if i >= e:
+                        break
+                    d, g = d_all[i], g_all[i]
+                    if nvalid == 0:
+                        h_d[L] = t_d[L] = d; h_s[L] = t_s[L] = g
+                    elif d == t_d[L]:
+                        t_s[L] += g
+                        if t_d[L] == h_d[L]:
+                            h_s[L] = t_s[L]
+                    else:
+                        if t_d[L] != h_d[L]:
+                            store(t_d[L], t_s[L])
+                        t_d[L], t_s[L] = d, g
+                    nvalid += 1
+                has[L] = nvalid > 0
+            if base != b and h_d[0] != carry_d:
+                store(carry_d, carry_v)
+            S = [0.0] * WAVE
+            seg_start = 0
+            for L in range(WAVE):
+                if L == 0 or t_d[L] != t_d[L - 1]:
+                    seg_start = L
+                    S[L] = t_s[L]
+                else:
+                    S[L] = S[L - 1] + t_s[L]
+                if seg_start == 0 and t_d[L] == carry_d and has[L]:
+                    pass
+            # carry into the first segment
+            for L in range(WAVE):
+                first_seg = all(t_d[k] == t_d[0] for k in range(L + 1))
+                if first_seg and t_d[L] == carry_d and has[L]:
+                    S[L] += carry_v
+            last = sum(has) - 1
+            for L in range(WAVE):
+                if not has[L]:
+                    continue
+                single = h_d[L] == t_d[L]
+                if not single:
+                    head = h_s[L]
+                    if L == 0:
+                        if h_d[0] == carry_d:
+                            head += carry_v
+                    elif t_d[L - 1] == h_d[L]:
+                        head += S[L - 1]
+                    store(h_d[L], head)
+                next_h = h_d[(L + 1) % WAVE]
+                if L != last and next_h != t_d[L]:
+                    store(t_d[L], S[L])
+            carry_d, carry_v = t_d[last], S[last]
+        merge.append((first_d, carry_d, carry_v, 0))
+    return acc, stores, np.array(merge, dtype=[("initial", "<u8"), ("final", "<u8"), ("partial", "<f8"),
+                                                ("pad", "<u8")])
+
+
+@pytest.mark.parametrize("vpl", [1, 2, 4])
+@pytest.mark.parametrize("chunk", [128, 512, 1024])
+def test_walker_emulation_matches_dense(gz, orc, vpl, chunk):
+    g = graphs.hub_graph(gz, num_vertices=600, hub_in=2000, seed=vpl * 7 + chunk)
+    V = g.num_vertices
+    vecs = orc.pack(g.pull, True)
+    props = np.random.default_rng(3).random(V + 8)
+    acc, stores, merge = emulate_walk(vecs, props, chunk, V, vpl)
+    orc.merge(merge, acc)
+    dense = np.zeros(V)
+    np.add.at(dense, g.pull[:, 1].astype(np.int64), props[g.pull[:, 0].astype(np.int64)])
+    has_in = np.bincount(g.pull[:, 1].astype(np.int64), minlength=V) > 0
+    assert np.allclose(acc[has_in], dense[has_in], rtol=1e-13, atol=0)
+    assert np.all(stores[~has_in] == 0)
+    # every destination is stored at most once per chunk it touches
+    assert stores.max() <= 1 + (vecs.shape[0] + chunk - 1) // chunk
+
+
+@pytest.mark.parametrize("vpl", [2, 4])
+def test_walker_emulation_boundary_shapes(gz, orc, vpl):
+    """Runs of 1..9 vectors so that heads, tails, single-run lanes and step
+    boundaries all occur."""
+    V = 2000
+    rng = np.random.default_rng(vpl)
+    pairs, dst = [], 1
+    while dst < 1500:
+        deg = int(rng.integers(1, 37))
+        pairs += [(1500 + s, dst) for s in range(deg)]
+        dst += 1
+    el = graphs.edge_lists(gz, pairs, V)
+    vecs = orc.pack(el.pull, True)
+    props = np.arange(V + 8, dtype=np.float64)
+    for chunk in (64 * vpl, 256, 768):
+        acc, _, merge = emulate_walk(vecs, props, chunk, V, vpl)
+        orc.merge(merge, acc)
+        dense = np.zeros(V)
+        np.add.at(dense, el.pull[:, 1].astype(np.int64), props[el.pull[:, 0].astype(np.int64)])
+        assert np.array_equal(acc[1:1500], dense[1:1500])
+
+
 def compact(bitmap_bits, offsets):
     """Replay of gz_frontier_compact: active vertices owning vectors and the
     exclusive prefix of their vector counts."""
