@@ -44,17 +44,19 @@ struct gz_cc_pull_op {
     uint64_t vote;
     __device__ __forceinline__ uint64_t identity() const { return ~0ull; }
     __device__ __forceinline__ uint64_t combine(uint64_t a, uint64_t b) const { return a < b ? a : b; }
-    __device__ __forceinline__ uint64_t gather(const gz_lanes& v, uint32_t& edges) const
+    __device__ __forceinline__ uint64_t gather(const gz_lanes& v, bool in_range, uint32_t& edges) const
     {
-        uint64_t m = ~0ull;
+        /* branch-free gathers: invalid lanes read element 0 and are ignored */
         const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
+        uint64_t x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = labels[(in_range && gz_lane_valid(l[k])) ? gz_lane_id(l[k]) : 0];
+        uint64_t m = ~0ull;
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-            if (gz_lane_valid(l[k])) {
-                const uint64_t x = labels[gz_lane_id(l[k])];
-                m = (x < m) ? x : m;
-                edges += 1;
-            }
+            const bool ok = in_range && gz_lane_valid(l[k]);
+            m = (ok && x[k] < m) ? x[k] : m;
+            edges += ok ? 1u : 0u;
         }
         return m;
     }

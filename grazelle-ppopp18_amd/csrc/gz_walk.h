@@ -21,8 +21,9 @@
  * The run open when the chunk ends is returned to the caller (merge record or
  * direct atomic), together with the chunk's first destination.
  *
- * Op supplies: T identity(); T combine(T, T); T gather(const gz_lanes&,
- * uint32_t& edges); void store(uint64_t d, T value) for a complete run.
+ * Op supplies: T identity(); T combine(T, T); T gather(const gz_lanes&, bool
+ * in_range, uint32_t& edges) (identity when !in_range); void store(uint64_t
+ * d, T value) for a complete run.
  */
 #ifndef GZ_WALK_H
 #define GZ_WALK_H
@@ -62,15 +63,88 @@ template <int VPL>
 __device__ __forceinline__ void gz_load_slice(gz_slice<VPL>& s, const gz_vec_t* __restrict__ vecs, uint64_t first,
                                               uint64_t e)
 {
+    /* unconditional loads (indices past the chunk end re-read its last vector
+     * and are masked by the caller): no branch, so the wait counts after the
+     * loads stay exact */
 #pragma unroll
     for (int k = 0; k < VPL; ++k) {
         const uint64_t i = first + (uint64_t)k;
-        if (i < e) {
-            s.v[k] = gz_load_vector(vecs, i);
-        } else {
-            s.v[k].l0 = s.v[k].l1 = s.v[k].l2 = s.v[k].l3 = 0;
-        }
+        s.v[k] = gz_load_vector(vecs, (i < e) ? i : e - 1);
     }
+}
+
+/* One step of the walk: reduce the slice `cur` (vectors base + lane*VPL ...)
+ * and prefetch the following step's slice into `nxt`. */
+template <int VPL, typename T, typename Op>
+__device__ __forceinline__ void gz_walk_step(const gz_vec_t* __restrict__ vecs, uint64_t b, uint64_t e, uint64_t base,
+                                             Op& op, const gz_slice<VPL>& cur, gz_slice<VPL>& nxt, uint64_t& carry_d,
+                                             T& carry_v, uint32_t& edges)
+{
+    const int lane = gz_lane();
+    const uint64_t step = (uint64_t)GZ_WAVE * VPL;
+    const uint64_t my_first = base + (uint64_t)lane * VPL;
+    /* issue every gather of the step first, then the next slice's loads:
+     * waiting for the gathers then leaves the prefetch in flight */
+    T gv[VPL];
+#pragma unroll
+    for (int k = 0; k < VPL; ++k) gv[k] = op.gather(cur.v[k], my_first + (uint64_t)k < e, edges);
+    gz_load_slice<VPL>(nxt, vecs, my_first + step, e);
+    /* 1. lane-local runs */
+    int nvalid = 0;
+    uint64_t h_d = ~0ull, t_d = ~0ull;
+    T h_s = op.identity(), t_s = op.identity();
+#pragma unroll
+    for (int k = 0; k < VPL; ++k) {
+        if (my_first + (uint64_t)k >= e) break;
+        const uint64_t d = gz_shared_vertex(cur.v[k]);
+        const T g = gv[k];
+        if (nvalid == 0) {
+            h_d = t_d = d;
+            h_s = t_s = g;
+        } else if (d == t_d) {
+            t_s = op.combine(t_s, g);
+            if (t_d == h_d) h_s = t_s;
+        } else {
+            if (t_d != h_d) op.store(t_d, t_s); /* interior run, complete */
+            t_d = d;
+            t_s = g;
+        }
+        ++nvalid;
+    }
+    const bool has = nvalid > 0;
+    const bool single = has && (h_d == t_d);
+    /* a carried run the step does not continue is complete */
+    const uint64_t h0 = gz_shfl_t<uint64_t>(h_d, 0);
+    if (lane == 0 && base != b && h0 != carry_d) op.store(carry_d, carry_v);
+    /* 2. cross-lane segmented scan of tails keyed by t_d */
+    const uint64_t prev_t = gz_shfl_up_t<uint64_t>(t_d, 1);
+    const uint64_t heads = __ballot(lane == 0 || t_d != prev_t);
+    const int seg = gz_segment_start(heads, lane);
+    T S = t_s;
+#pragma unroll
+    for (int o = 1; o < GZ_WAVE; o <<= 1) {
+        const T x = gz_shfl_up_t<T>(S, o);
+        if (lane - o >= seg) S = op.combine(S, x);
+    }
+    /* lanes whose scan segment starts at lane 0 and continues the carry */
+    if (seg == 0 && t_d == carry_d && has) S = op.combine(S, carry_v);
+    const T S_prev = gz_shfl_up_t<T>(S, 1);
+    /* the head of a multi-run lane completes inside the lane */
+    if (has && !single) {
+        T head = h_s;
+        if (lane == 0) {
+            if (h_d == carry_d) head = op.combine(head, carry_v);
+        } else if (prev_t == h_d) {
+            head = op.combine(head, S_prev);
+        }
+        op.store(h_d, head);
+    }
+    /* 3. tails: complete when the next lane starts another destination */
+    const uint64_t next_h = gz_shfl_t<uint64_t>(h_d, (lane + 1) & (GZ_WAVE - 1));
+    const int last = __popcll(__ballot(has)) - 1; /* valid lanes form a prefix */
+    if (has && lane != last && next_h != t_d) op.store(t_d, S);
+    carry_d = gz_shfl_t<uint64_t>(t_d, last);
+    carry_v = gz_shfl_t<T>(S, last);
 }
 
 template <int VPL, typename T, typename Op>
@@ -82,68 +156,18 @@ __device__ __forceinline__ void gz_walk_chunk(const gz_vec_t* __restrict__ vecs,
     first_d = gz_shared_vertex(gz_load_vector(vecs, b));
     carry_d = first_d;
     carry_v = op.identity();
-    gz_slice<VPL> cur, nxt;
-    gz_load_slice<VPL>(cur, vecs, b + (uint64_t)lane * VPL, e);
-    for (uint64_t base = b; base < e; base += step) {
-        const uint64_t my_first = base + (uint64_t)lane * VPL;
-        if (base + step < e) gz_load_slice<VPL>(nxt, vecs, my_first + step, e);
-        /* 1. lane-local runs */
-        int nvalid = 0;
-        uint64_t h_d = ~0ull, t_d = ~0ull;
-        T h_s = op.identity(), t_s = op.identity();
-#pragma unroll
-        for (int k = 0; k < VPL; ++k) {
-            if (my_first + (uint64_t)k >= e) break;
-            const uint64_t d = gz_shared_vertex(cur.v[k]);
-            const T g = op.gather(cur.v[k], edges);
-            if (nvalid == 0) {
-                h_d = t_d = d;
-                h_s = t_s = g;
-            } else if (d == t_d) {
-                t_s = op.combine(t_s, g);
-                if (t_d == h_d) h_s = t_s;
-            } else {
-                if (t_d != h_d) op.store(t_d, t_s); /* interior run, complete */
-                t_d = d;
-                t_s = g;
-            }
-            ++nvalid;
-        }
-        const bool has = nvalid > 0;
-        const bool single = has && (h_d == t_d);
-        /* a carried run the step does not continue is complete */
-        const uint64_t h0 = gz_shfl_t<uint64_t>(h_d, 0);
-        if (lane == 0 && base != b && h0 != carry_d) op.store(carry_d, carry_v);
-        /* 2. cross-lane segmented scan of tails keyed by t_d */
-        const uint64_t prev_t = gz_shfl_up_t<uint64_t>(t_d, 1);
-        const uint64_t heads = __ballot(lane == 0 || t_d != prev_t);
-        const int seg = gz_segment_start(heads, lane);
-        T S = t_s;
-#pragma unroll
-        for (int o = 1; o < GZ_WAVE; o <<= 1) {
-            const T x = gz_shfl_up_t<T>(S, o);
-            if (lane - o >= seg) S = op.combine(S, x);
-        }
-        /* lanes whose scan segment starts at lane 0 and continues the carry */
-        if (seg == 0 && t_d == carry_d && has) S = op.combine(S, carry_v);
-        const T S_prev = gz_shfl_up_t<T>(S, 1);
-        /* the head of a multi-run lane completes inside the lane */
-        if (has && !single) {
-            T head = h_s;
-            if (lane == 0) {
-                if (h_d == carry_d) head = op.combine(head, carry_v);
-            } else if (prev_t == h_d) {
-                head = op.combine(head, S_prev);
-            }
-            op.store(h_d, head);
-        }
-        /* 3. tails: complete when the next lane starts another destination */
-        const uint64_t next_h = gz_shfl_t<uint64_t>(h_d, (lane + 1) & (GZ_WAVE - 1));
-        const int last = __popcll(__ballot(has)) - 1; /* valid lanes form a prefix */
-        if (has && lane != last && next_h != t_d) op.store(t_d, S);
-        carry_d = gz_shfl_t<uint64_t>(t_d, last);
-        carry_v = gz_shfl_t<T>(S, last);
-        if (base + step < e) cur = nxt;
+    /* two slice buffers used in turn, so that no register copy forces the
+     * prefetched loads to complete early */
+    gz_slice<VPL> sa, sb;
+    gz_load_slice<VPL>(sa, vecs, b + (uint64_t)lane * VPL, e);
+    uint64_t base = b;
+    while (true) {
+        gz_walk_step<VPL, T>(vecs, b, e, base, op, sa, sb, carry_d, carry_v, edges);
+        base += step;
+        if (base >= e) break;
+        gz_walk_step<VPL, T>(vecs, b, e, base, op, sb, sa, carry_d, carry_v, edges);
+        base += step;
+        if (base >= e) break;
     }
 }
 
