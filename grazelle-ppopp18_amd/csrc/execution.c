@@ -83,7 +83,11 @@ void gz_execution_prepare(const gz_run_options_t* o)
         gz_dev_fill_u64(gz_graph.frontier_wants_info, ~0ull, fwords);
         break;
     case GZ_ALG_CC:
-        gz_dev_iota_u64((uint64_t*)gz_graph.vertex_props, V);
+        /* labels start at the original ids, also on a renumbered graph */
+        if (gz_graph_original_ids_device() != NULL)
+            gz_device_copy(gz_graph.vertex_props, gz_graph_original_ids_device(), V * sizeof(uint64_t));
+        else
+            gz_dev_iota_u64((uint64_t*)gz_graph.vertex_props, V);
         gz_dev_fill_u64((uint64_t*)gz_graph.vertex_accumulators, 0, fwords);
         gz_dev_fill_u64(gz_graph.frontier_has_info, ~0ull, fwords);
         gz_dev_fill_u64(gz_graph.frontier_wants_info, ~0ull, fwords);
@@ -94,7 +98,7 @@ void gz_execution_prepare(const gz_run_options_t* o)
                     (unsigned long long)o->root, (unsigned long long)V);
             exit(66);
         }
-        const uint64_t r = o->root;
+        const uint64_t r = gz_graph_device_vertex(o->root);
         gz_dev_fill_u64((uint64_t*)gz_graph.vertex_props, GZ_BFS_UNVISITED, V);
         gz_dev_store_u64((uint64_t*)gz_graph.vertex_props, r, GZ_BFS_PARENT_NONE);
         gz_dev_fill_u64((uint64_t*)gz_graph.vertex_accumulators, 0, fwords);
@@ -168,7 +172,7 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
     uint64_t vote = 0;
     if (bfs) {
         const double* od = gz_graph_outdegrees_host();
-        if (o->stat_mode != GZ_STAT_COUNT_ONLY) vote += (uint64_t)od[o->root];
+        if (o->stat_mode != GZ_STAT_COUNT_ONLY) vote += (uint64_t)od[gz_graph_device_vertex(o->root)];
         if (o->stat_mode != GZ_STAT_OUTDEG_ONLY) vote += 1;
     } else {
         if (o->stat_mode != GZ_STAT_COUNT_ONLY) vote += E;

@@ -25,6 +25,7 @@
 
 #include "grazelle_host.h"
 #include "grazelle_phases.h"
+#include "gz_host_internal.h"
 
 /* ---- edge-list files ------------------------------------------------------ */
 
@@ -249,6 +250,10 @@ typedef struct gz_host_graph_t {
     void* d_acc;
     uint64_t* d_has;
     uint64_t* d_wants;
+    /* renumbered graphs only (else NULL) */
+    uint64_t* d_original_ids;    /* original id by device vertex */
+    uint64_t* original_ids;      /* host mirror of the above */
+    uint64_t* device_ids;        /* device vertex by original id */
 } gz_host_graph_t;
 
 static gz_host_graph_t G;
@@ -282,10 +287,15 @@ void gz_graph_release(void)
     gz_device_free(G.d_acc);
     gz_device_free(G.d_has);
     gz_device_free(G.d_wants);
+    gz_device_free(G.d_original_ids);
+    free(G.original_ids);
+    free(G.device_ids);
     free(G.outdeg);
     memset(&G, 0, sizeof(G));
     memset(&gz_graph, 0, sizeof(gz_graph));
 }
+
+static void gz_graph_finish(void);
 
 static void gz_upload_layout(const uint64_t* pairs, uint64_t num_edges, uint64_t num_vertices, int by_dst,
                              gz_vec_t** d_vecs, uint64_t** d_offsets, uint64_t* count)
@@ -322,9 +332,61 @@ int gz_graph_load_pairs(uint64_t num_vertices, uint64_t pull_edges, const uint64
     if (pull_pairs) gz_upload_layout(pull_pairs, pull_edges, num_vertices, 1, &G.d_pull, &G.d_pull_offsets, &G.pull_vectors);
     if (push_pairs) gz_upload_layout(push_pairs, push_edges, num_vertices, 0, &G.d_push, &G.d_push_offsets, &G.push_vectors);
     const size_t vbytes = (num_vertices + 8) * sizeof(uint64_t);
-    const size_t fwords = (num_vertices + 63) / 64 + 8;
     G.d_outdeg = (double*)gz_device_malloc(vbytes);
     gz_copy_to_device(G.d_outdeg, G.outdeg, vbytes);
+    gz_graph_finish();
+    return 0;
+}
+
+void gz_graph_adopt_device(const gz_device_layouts_t* L)
+{
+    G.num_vertices = L->num_vertices;
+    G.num_edges = L->num_edges;
+    G.d_pull = L->pull;
+    G.d_pull_offsets = L->pull_offsets;
+    G.pull_vectors = L->pull_count;
+    G.d_push = L->push;
+    G.d_push_offsets = L->push_offsets;
+    G.push_vectors = L->push_count;
+    G.d_outdeg = L->outdeg;
+    G.outdeg = (double*)malloc((L->num_vertices + 8) * sizeof(double));
+    gz_copy_to_host(G.outdeg, G.d_outdeg, (L->num_vertices + 8) * sizeof(double));
+    if (L->original_ids != NULL) {
+        const uint64_t n = L->num_vertices;
+        G.d_original_ids = L->original_ids;
+        G.original_ids = (uint64_t*)malloc(n * sizeof(uint64_t));
+        G.device_ids = (uint64_t*)malloc(n * sizeof(uint64_t));
+        gz_copy_to_host(G.original_ids, G.d_original_ids, n * sizeof(uint64_t));
+        for (uint64_t i = 0; i < n; ++i) G.device_ids[G.original_ids[i]] = i;
+    }
+    gz_graph_finish();
+}
+
+const uint64_t* gz_graph_original_ids_device(void) { return G.d_original_ids; }
+
+uint64_t gz_graph_device_vertex(uint64_t original)
+{
+    if (G.device_ids == NULL || original >= G.num_vertices) return original;
+    return G.device_ids[original];
+}
+
+/* out[original id] = in[device vertex] for V words (identity copy when the
+ * graph is not renumbered). */
+static void gz_to_original_order(uint64_t* out, const uint64_t* in)
+{
+    if (G.device_ids == NULL) {
+        if (out != in) memcpy(out, in, G.num_vertices * sizeof(uint64_t));
+        return;
+    }
+    for (uint64_t v = 0; v < G.num_vertices; ++v) out[v] = in[G.device_ids[v]];
+}
+
+/* Allocate the vertex arrays and frontiers and publish the graph record. */
+static void gz_graph_finish(void)
+{
+    const uint64_t num_vertices = G.num_vertices;
+    const size_t vbytes = (num_vertices + 8) * sizeof(uint64_t);
+    const size_t fwords = (num_vertices + 63) / 64 + 8;
     G.d_props = gz_device_malloc(vbytes);
     G.d_acc = gz_device_malloc(vbytes);
     G.d_has = (uint64_t*)gz_device_malloc(fwords * sizeof(uint64_t));
@@ -343,7 +405,6 @@ int gz_graph_load_pairs(uint64_t num_vertices, uint64_t pull_edges, const uint64
     gz_graph.vertex_accumulators = G.d_acc;
     gz_graph.frontier_has_info = G.d_has;
     gz_graph.frontier_wants_info = G.d_wants;
-    return 0;
 }
 
 void graph_data_read_from_file(const char* filename_gather, const char* filename_scatter, uint32_t num_numa_nodes,
@@ -394,29 +455,55 @@ void gz_graph_download_offsets(int pull, uint64_t* out)
 
 /* ---- results -------------------------------------------------------------- */
 
+/* Device vertex words in original-id order (a fresh malloc'd array). */
+static uint64_t* gz_download_vertex_words(void)
+{
+    const uint64_t n = G.num_vertices;
+    uint64_t* w = (uint64_t*)malloc((n ? n : 1) * sizeof(uint64_t));
+    gz_copy_to_host(w, gz_graph.vertex_props, n * sizeof(uint64_t));
+    if (G.device_ids != NULL) {
+        uint64_t* o = (uint64_t*)malloc((n ? n : 1) * sizeof(uint64_t));
+        gz_to_original_order(o, w);
+        free(w);
+        w = o;
+    }
+    return w;
+}
+
 void gz_result_pr_props(double* out)
 {
-    gz_copy_to_host(out, gz_graph.vertex_props, G.num_vertices * sizeof(double));
+    uint64_t* w = gz_download_vertex_words();
+    memcpy(out, w, G.num_vertices * sizeof(double));
+    free(w);
 }
 
 void gz_result_pr_ranks(double* out)
 {
     gz_result_pr_props(out);
     const double nv = (double)G.num_vertices;
-    for (uint64_t i = 0; i < G.num_vertices; ++i) out[i] = out[i] * (0.0 == G.outdeg[i] ? nv : G.outdeg[i]);
+    for (uint64_t i = 0; i < G.num_vertices; ++i) {
+        const double od = G.outdeg[gz_graph_device_vertex(i)];
+        out[i] = out[i] * (0.0 == od ? nv : od);
+    }
 }
 
 void gz_result_cc_labels(uint64_t* out)
 {
-    gz_copy_to_host(out, gz_graph.vertex_props, G.num_vertices * sizeof(uint64_t));
+    uint64_t* w = gz_download_vertex_words();
+    memcpy(out, w, G.num_vertices * sizeof(uint64_t));
+    free(w);
 }
 
 void gz_result_bfs(int64_t* parents, uint64_t* depths)
 {
-    uint64_t* w = (uint64_t*)malloc(G.num_vertices * sizeof(uint64_t));
-    gz_copy_to_host(w, gz_graph.vertex_props, G.num_vertices * sizeof(uint64_t));
+    uint64_t* w = gz_download_vertex_words();
     for (uint64_t i = 0; i < G.num_vertices; ++i) {
-        if (parents) parents[i] = (w[i] == GZ_BFS_UNVISITED) ? -1 : gz_bfs_parent(w[i]);
+        int64_t p = -1;
+        if (w[i] != GZ_BFS_UNVISITED && gz_bfs_parent(w[i]) >= 0) {
+            p = gz_bfs_parent(w[i]);
+            if (G.original_ids != NULL) p = (int64_t)G.original_ids[p];
+        }
+        if (parents) parents[i] = p;
         if (depths) depths[i] = (w[i] == GZ_BFS_UNVISITED) ? UINT64_MAX : gz_bfs_depth(w[i]);
     }
     free(w);

@@ -1,0 +1,436 @@
+/*
+ * gz_ingress.hip -- device Vector-Sparse ingress: the edge lists are built in
+ * HBM instead of by the host packer.
+ *
+ * The reference loads -pull / -push files with a producer thread and one
+ * packing consumer per layout (source/graphdata.c:595-951), which at a
+ * billion edges is minutes of host time.  Here the same result is computed
+ * as a device pipeline over 64-bit keys (group << b | neighbour), b = bits of
+ * V-1:
+ *   1. keys: drawn by the R-MAT kernel (the host generator's exact stream) or
+ *      converted from uploaded (src, dst) pairs, optionally both directions;
+ *   2. LSD radix sort over the 2b key bits (pull: group = dst);
+ *   3. drop self-loops and duplicates (a key equal to its predecessor);
+ *   4. per-vertex edge offsets by binary search in the sorted keys, vector
+ *      counts ceil(deg / 4) and their exclusive scan = the CSR vector offsets;
+ *   5. pack: edge i of group g at rank r goes to lane r % 4 of vector
+ *      offsets[g] + r / 4, which is the flush rule of graphdata.c:679-686 on a
+ *      group-sorted list (a vector closes when the shared vertex changes or it
+ *      holds four edges);
+ *   6. the push layout: the kept keys with group and neighbour swapped,
+ *      sorted again; FP64 outdegrees are its group sizes (graphdata.c:814
+ *      counts sources of the pull file, which is the same number).
+ * The result equals gz_edges_sort(..., dedupe) + gz_pack_vectors on the host
+ * bit for bit, invalid lanes included (shared-vertex piece, neighbour 0).
+ */
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_select.hpp>
+
+#include "grazelle_host.h"
+#include "gz_host_internal.h"
+#include "gz_internal.h"
+
+namespace {
+
+__device__ __forceinline__ uint64_t gz_splitmix64_dev(uint64_t x)
+{
+    x += 0x9e3779b97f4a7c15ull;
+    x = (x ^ (x >> 30)) * 0xbf58476d1ce4e5b9ull;
+    x = (x ^ (x >> 27)) * 0x94d049bb133111ebull;
+    return x ^ (x >> 31);
+}
+
+/* The R-MAT draw of rmat.c for edges [0, n): key (dst << b | src), and with
+ * `both` also (src << b | dst) at the odd slot. */
+__global__ void gz_k_rmat_keys(uint64_t seed, uint64_t edge_offset, uint32_t scale, uint64_t nv, uint32_t ta,
+                               uint32_t tb, uint32_t tc, uint64_t n, uint32_t b, int both, uint64_t* __restrict__ keys)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+        uint64_t x = gz_splitmix64_dev(seed ^ gz_splitmix64_dev(edge_offset + e));
+        uint64_t s = 0, d = 0;
+        for (uint32_t l = 0; l < scale; ++l) {
+            if (l && (l & 3) == 0) x = gz_splitmix64_dev(x);
+            const uint32_t u = (uint32_t)((x >> (16 * (l & 3))) & 0xffff);
+            const uint32_t sb = (u >= tb) ? 1u : 0u;
+            const uint32_t db = ((u >= ta && u < tb) || (u >= tc)) ? 1u : 0u;
+            s = (s << 1) | sb;
+            d = (d << 1) | db;
+        }
+        if (s >= nv) s %= nv;
+        if (d >= nv) d %= nv;
+        if (both) {
+            keys[2 * e] = (d << b) | s;
+            keys[2 * e + 1] = (s << b) | d;
+        } else {
+            keys[e] = (d << b) | s;
+        }
+    }
+}
+
+__global__ void gz_k_pairs_keys(const uint64_t* __restrict__ pairs, uint64_t n, uint32_t b, int both,
+                                uint64_t* __restrict__ keys)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+        const uint64_t s = pairs[2 * e], d = pairs[2 * e + 1];
+        if (both) {
+            keys[2 * e] = (d << b) | s;
+            keys[2 * e + 1] = (s << b) | d;
+        } else {
+            keys[e] = (d << b) | s;
+        }
+    }
+}
+
+/* keep = first of its value and not a self-loop */
+__global__ void gz_k_mark_kept(const uint64_t* __restrict__ keys, uint64_t n, uint32_t b, uint8_t* __restrict__ keep)
+{
+    const uint64_t mask = (1ull << b) - 1ull;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t k = keys[i];
+        const bool dup = (i > 0) && keys[i - 1] == k;
+        keep[i] = (!dup && (k >> b) != (k & mask)) ? 1 : 0;
+    }
+}
+
+__global__ void gz_k_swap_keys(const uint64_t* __restrict__ in, uint64_t n, uint32_t b, uint64_t* __restrict__ out)
+{
+    const uint64_t mask = (1ull << b) - 1ull;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t k = in[i];
+        out[i] = ((k & mask) << b) | (k >> b);
+    }
+}
+
+/* eoff[v] = first i with keys[i] >> b >= v, for v in [0, V] (eoff[V] = m). */
+__global__ void gz_k_group_offsets(const uint64_t* __restrict__ keys, uint64_t m, uint32_t b, uint64_t nv,
+                                   uint64_t* __restrict__ eoff)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v <= nv; v += stride) {
+        if (v == nv) {
+            eoff[v] = m;
+            continue;
+        }
+        const uint64_t target = v << b;
+        uint64_t lo = 0, hi = m;
+        while (lo < hi) {
+            const uint64_t mid = lo + ((hi - lo) >> 1);
+            if (keys[mid] < target)
+                lo = mid + 1;
+            else
+                hi = mid;
+        }
+        eoff[v] = lo;
+    }
+}
+
+__global__ void gz_k_vector_counts(const uint64_t* __restrict__ eoff, uint64_t nv, uint64_t* __restrict__ cnt)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v <= nv; v += stride)
+        cnt[v] = (v < nv) ? (eoff[v + 1] - eoff[v] + 3) / 4 : 0;
+}
+
+__device__ __forceinline__ uint64_t gz_piece(uint64_t g, int l)
+{
+    const int shift = 15 * l;
+    const uint64_t mask = (l == 3) ? 0x7ull : 0x7fffull;
+    return ((g >> shift) & mask) << 48;
+}
+
+__global__ void gz_k_pack(const uint64_t* __restrict__ keys, uint64_t m, uint32_t b, const uint64_t* __restrict__ eoff,
+                          const uint64_t* __restrict__ voff, uint64_t* __restrict__ lanes)
+{
+    const uint64_t mask = (1ull << b) - 1ull;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += stride) {
+        const uint64_t k = keys[i];
+        const uint64_t g = k >> b;
+        const uint64_t r = i - eoff[g];
+        const uint64_t j = voff[g] + r / 4;
+        const int l = (int)(r & 3);
+        lanes[4 * j + l] = GZ_LANE_VALID_BIT | gz_piece(g, l) | (k & mask);
+        if (l == 0) {
+            const uint64_t left = eoff[g + 1] - eoff[g] - r;
+            for (int q = (left < 4 ? (int)left : 4); q < 4; ++q) lanes[4 * j + q] = gz_piece(g, q);
+        }
+    }
+}
+
+__global__ void gz_k_outdegrees(const uint64_t* __restrict__ eoff, uint64_t nv, double* __restrict__ outdeg)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv + 8; v += stride)
+        outdeg[v] = (v < nv) ? (double)(eoff[v + 1] - eoff[v]) : 0.0;
+}
+
+unsigned gz_grid_for(uint64_t n)
+{
+    const uint64_t blocks = (n + GZ_BLOCK - 1) / GZ_BLOCK;
+    return (unsigned)(blocks < 65536 ? (blocks ? blocks : 1) : 65536);
+}
+
+uint32_t gz_id_bits(uint64_t nv)
+{
+    uint32_t b = 1;
+    while (b < 64 && (nv - 1) >> b) ++b;
+    return b;
+}
+
+/* Sort keys[0, n) over bits [0, 2b); the sorted keys end in *sorted (one of a
+ * or b_buf). */
+void gz_sort_keys(uint64_t* a, uint64_t* b_buf, uint64_t n, uint32_t bits, uint64_t** sorted)
+{
+    rocprim::double_buffer<uint64_t> db(a, b_buf);
+    size_t tmp_bytes = 0;
+    GZ_CHECK(rocprim::radix_sort_keys(nullptr, tmp_bytes, db, (size_t)n, 0, bits, gz_stream));
+    void* tmp = gz_device_malloc(tmp_bytes ? tmp_bytes : 1);
+    GZ_CHECK(rocprim::radix_sort_keys(tmp, tmp_bytes, db, (size_t)n, 0, bits, gz_stream));
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    gz_device_free(tmp);
+    *sorted = db.current();
+}
+
+/* Build one layout from sorted, duplicate-free keys[0, m). */
+void gz_build_layout(const uint64_t* keys, uint64_t m, uint32_t b, uint64_t nv, gz_vec_t** d_vecs,
+                     uint64_t** d_offsets, uint64_t* vector_count, uint64_t* eoff)
+{
+    const unsigned tb = GZ_BLOCK;
+    hipLaunchKernelGGL(gz_k_group_offsets, dim3(gz_grid_for(nv + 1)), dim3(tb), 0, gz_stream, keys, m, b, nv, eoff);
+    GZ_LAUNCH_CHECK();
+    uint64_t* cnt = (uint64_t*)gz_device_malloc((nv + 1) * sizeof(uint64_t));
+    uint64_t* voff = (uint64_t*)gz_device_malloc((nv + 1) * sizeof(uint64_t));
+    hipLaunchKernelGGL(gz_k_vector_counts, dim3(gz_grid_for(nv + 1)), dim3(tb), 0, gz_stream, eoff, nv, cnt);
+    GZ_LAUNCH_CHECK();
+    size_t tmp_bytes = 0;
+    GZ_CHECK(rocprim::exclusive_scan(nullptr, tmp_bytes, cnt, voff, (uint64_t)0, (size_t)(nv + 1),
+                                     rocprim::plus<uint64_t>(), gz_stream));
+    void* tmp = gz_device_malloc(tmp_bytes ? tmp_bytes : 1);
+    GZ_CHECK(rocprim::exclusive_scan(tmp, tmp_bytes, cnt, voff, (uint64_t)0, (size_t)(nv + 1),
+                                     rocprim::plus<uint64_t>(), gz_stream));
+    uint64_t total = 0;
+    GZ_CHECK(hipMemcpyAsync(&total, voff + nv, sizeof(uint64_t), hipMemcpyDeviceToHost, gz_stream));
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    gz_device_free(tmp);
+    gz_device_free(cnt);
+    gz_vec_t* vecs = (gz_vec_t*)gz_device_malloc((total ? total : 1) * sizeof(gz_vec_t));
+    if (m) {
+        hipLaunchKernelGGL(gz_k_pack, dim3(gz_grid_for(m)), dim3(tb), 0, gz_stream, keys, m, b, eoff, voff,
+                           (uint64_t*)vecs);
+        GZ_LAUNCH_CHECK();
+    }
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    *d_vecs = vecs;
+    *d_offsets = voff;
+    *vector_count = total;
+}
+
+__global__ void gz_k_count_sources(const uint64_t* __restrict__ keys, uint64_t n, uint32_t b, uint32_t* __restrict__ cnt)
+{
+    const uint64_t mask = (1ull << b) - 1ull;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+        atomicAdd(&cnt[keys[i] & mask], 1u);
+}
+
+/* sort key: out-degree descending, then id ascending */
+__global__ void gz_k_degree_order(const uint32_t* __restrict__ cnt, uint64_t nv, uint64_t* __restrict__ order)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride)
+        order[v] = ((uint64_t)(0xffffffffu - cnt[v]) << 32) | v;
+}
+
+__global__ void gz_k_permutation(const uint64_t* __restrict__ order, uint64_t nv, uint64_t* __restrict__ old_of_new,
+                                 uint32_t* __restrict__ new_of_old)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
+        const uint64_t old = order[i] & 0xffffffffull;
+        old_of_new[i] = old;
+        new_of_old[old] = (uint32_t)i;
+    }
+}
+
+__global__ void gz_k_remap_keys(uint64_t* __restrict__ keys, uint64_t n, uint32_t b,
+                                const uint32_t* __restrict__ new_of_old)
+{
+    const uint64_t mask = (1ull << b) - 1ull;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint64_t k = keys[i];
+        keys[i] = ((uint64_t)new_of_old[k >> b] << b) | (uint64_t)new_of_old[k & mask];
+    }
+}
+
+/* Renumber the vertices by raw out-degree, descending (ties by id), and
+ * rewrite the keys in place.  Returns the device array of original ids,
+ * indexed by new id.  The hot sources of a power-law graph then share cache
+ * lines, which is what the pull gathers need once the vertex array no longer
+ * fits in the caches. */
+uint64_t* gz_relabel_by_outdegree(uint64_t* keys, uint64_t n, uint32_t b, uint64_t nv)
+{
+    uint32_t* cnt = (uint32_t*)gz_device_malloc(nv * sizeof(uint32_t));
+    GZ_CHECK(hipMemsetAsync(cnt, 0, nv * sizeof(uint32_t), gz_stream));
+    if (n) {
+        hipLaunchKernelGGL(gz_k_count_sources, dim3(gz_grid_for(n)), dim3(GZ_BLOCK), 0, gz_stream, keys, n, b, cnt);
+        GZ_LAUNCH_CHECK();
+    }
+    uint64_t* order = (uint64_t*)gz_device_malloc(nv * sizeof(uint64_t));
+    uint64_t* order_b = (uint64_t*)gz_device_malloc(nv * sizeof(uint64_t));
+    hipLaunchKernelGGL(gz_k_degree_order, dim3(gz_grid_for(nv)), dim3(GZ_BLOCK), 0, gz_stream, cnt, nv, order);
+    GZ_LAUNCH_CHECK();
+    uint64_t* sorted = order;
+    if (nv > 1) gz_sort_keys(order, order_b, nv, 64, &sorted);
+    uint64_t* old_of_new = (uint64_t*)gz_device_malloc((nv + 8) * sizeof(uint64_t));
+    uint32_t* new_of_old = cnt; /* counts are no longer needed */
+    hipLaunchKernelGGL(gz_k_permutation, dim3(gz_grid_for(nv)), dim3(GZ_BLOCK), 0, gz_stream, sorted, nv, old_of_new,
+                       new_of_old);
+    GZ_LAUNCH_CHECK();
+    if (n) {
+        hipLaunchKernelGGL(gz_k_remap_keys, dim3(gz_grid_for(n)), dim3(GZ_BLOCK), 0, gz_stream, keys, n, b, new_of_old);
+        GZ_LAUNCH_CHECK();
+    }
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    gz_device_free(order);
+    gz_device_free(order_b);
+    gz_device_free(cnt);
+    return old_of_new;
+}
+
+/* Keys already in a[0, n): optionally renumber, sort, drop self-loops and
+ * duplicates, build both layouts and hand them to the graph record. */
+void gz_ingress_from_keys(uint64_t nv, uint64_t* a, uint64_t* b_buf, uint64_t n, uint32_t b, uint32_t flags)
+{
+    uint64_t* old_of_new = nullptr;
+    if (flags & GZ_INGRESS_RELABEL_BY_OUTDEGREE) old_of_new = gz_relabel_by_outdegree(a, n, b, nv);
+    uint64_t* sorted = a;
+    if (n > 1) gz_sort_keys(a, b_buf, n, 2 * b, &sorted);
+    uint64_t* other = (sorted == a) ? b_buf : a;
+    uint64_t m = 0;
+    if (n) {
+        uint8_t* keep = (uint8_t*)gz_device_malloc(n);
+        uint64_t* d_count = (uint64_t*)gz_device_malloc(sizeof(uint64_t));
+        hipLaunchKernelGGL(gz_k_mark_kept, dim3(gz_grid_for(n)), dim3(GZ_BLOCK), 0, gz_stream, sorted, n, b, keep);
+        GZ_LAUNCH_CHECK();
+        size_t tmp_bytes = 0;
+        GZ_CHECK(rocprim::select(nullptr, tmp_bytes, sorted, keep, other, d_count, (size_t)n, gz_stream));
+        void* tmp = gz_device_malloc(tmp_bytes ? tmp_bytes : 1);
+        GZ_CHECK(rocprim::select(tmp, tmp_bytes, sorted, keep, other, d_count, (size_t)n, gz_stream));
+        GZ_CHECK(hipMemcpyAsync(&m, d_count, sizeof(uint64_t), hipMemcpyDeviceToHost, gz_stream));
+        GZ_CHECK(hipStreamSynchronize(gz_stream));
+        gz_device_free(tmp);
+        gz_device_free(keep);
+        gz_device_free(d_count);
+    }
+
+    gz_device_layouts_t L;
+    L.num_vertices = nv;
+    L.num_edges = m;
+    L.original_ids = old_of_new;
+    uint64_t* eoff = (uint64_t*)gz_device_malloc((nv + 1) * sizeof(uint64_t));
+    gz_build_layout(other, m, b, nv, &L.pull, &L.pull_offsets, &L.pull_count, eoff);
+    /* push layout: swap group and neighbour, sort again (still unique) */
+    if (m) {
+        hipLaunchKernelGGL(gz_k_swap_keys, dim3(gz_grid_for(m)), dim3(GZ_BLOCK), 0, gz_stream, other, m, b, sorted);
+        GZ_LAUNCH_CHECK();
+    }
+    uint64_t* push_sorted = sorted;
+    if (m > 1) gz_sort_keys(sorted, other, m, 2 * b, &push_sorted);
+    gz_build_layout(push_sorted, m, b, nv, &L.push, &L.push_offsets, &L.push_count, eoff);
+    L.outdeg = (double*)gz_device_malloc((nv + 8) * sizeof(double));
+    hipLaunchKernelGGL(gz_k_outdegrees, dim3(gz_grid_for(nv + 8)), dim3(GZ_BLOCK), 0, gz_stream, eoff, nv, L.outdeg);
+    GZ_LAUNCH_CHECK();
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    gz_device_free(eoff);
+    gz_graph_adopt_device(&L);
+}
+
+void gz_check_vertex_count(uint64_t nv)
+{
+    if (nv == 0 || nv > 0xffffffffull) {
+        fprintf(stderr, "grazelle: device ingress needs 1 <= V <= 2^32 (got %llu)\n", (unsigned long long)nv);
+        exit(65);
+    }
+}
+
+}  // namespace
+
+extern "C" int gz_graph_build_pairs_device(uint64_t num_vertices, uint64_t num_edges, const uint64_t* pairs,
+                                           int symmetrize, uint32_t flags)
+{
+    gz_runtime_init(0);
+    gz_graph_release();
+    gz_check_vertex_count(num_vertices);
+    for (uint64_t i = 0; i < 2 * num_edges; ++i)
+        if (pairs[i] >= num_vertices) {
+            fprintf(stderr, "grazelle: edge endpoint %llu is not below V = %llu\n", (unsigned long long)pairs[i],
+                    (unsigned long long)num_vertices);
+            exit(65);
+        }
+    const uint32_t b = gz_id_bits(num_vertices);
+    const uint64_t n = symmetrize ? 2 * num_edges : num_edges;
+    uint64_t* d_pairs = (uint64_t*)gz_device_malloc((num_edges ? 2 * num_edges : 2) * sizeof(uint64_t));
+    if (num_edges) gz_copy_to_device(d_pairs, pairs, 2 * num_edges * sizeof(uint64_t));
+    uint64_t* a = (uint64_t*)gz_device_malloc((n ? n : 1) * sizeof(uint64_t));
+    uint64_t* bb = (uint64_t*)gz_device_malloc((n ? n : 1) * sizeof(uint64_t));
+    if (num_edges) {
+        hipLaunchKernelGGL(gz_k_pairs_keys, dim3(gz_grid_for(num_edges)), dim3(GZ_BLOCK), 0, gz_stream, d_pairs,
+                           num_edges, b, symmetrize, a);
+        GZ_LAUNCH_CHECK();
+    }
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    gz_device_free(d_pairs);
+    gz_ingress_from_keys(num_vertices, a, bb, n, b, flags);
+    gz_device_free(a);
+    gz_device_free(bb);
+    return 0;
+}
+
+extern "C" int gz_graph_build_rmat_device(const gz_rmat_params_t* p, uint64_t num_vertices, int symmetrize,
+                                          uint64_t extra_edges, const uint64_t* extra_pairs, uint32_t flags)
+{
+    gz_runtime_init(0);
+    gz_graph_release();
+    const uint64_t rmat_nv = p->num_vertices ? p->num_vertices : (1ull << p->scale);
+    uint64_t nv = num_vertices > rmat_nv ? num_vertices : rmat_nv;
+    for (uint64_t i = 0; i < 2 * extra_edges; ++i)
+        if (extra_pairs[i] >= nv) {
+            fprintf(stderr, "grazelle: extra edge endpoint %llu is not below V = %llu\n",
+                    (unsigned long long)extra_pairs[i], (unsigned long long)nv);
+            exit(65);
+        }
+    gz_check_vertex_count(nv);
+    const uint32_t b = gz_id_bits(nv);
+    const uint64_t per = symmetrize ? 2 : 1;
+    const uint64_t n = per * (p->num_edges + extra_edges);
+    uint64_t* a = (uint64_t*)gz_device_malloc((n ? n : 1) * sizeof(uint64_t));
+    uint64_t* bb = (uint64_t*)gz_device_malloc((n ? n : 1) * sizeof(uint64_t));
+    if (p->num_edges) {
+        const uint32_t ta = (uint32_t)(p->a * 65536.0);
+        const uint32_t tb = (uint32_t)((p->a + p->b) * 65536.0);
+        const uint32_t tc = (uint32_t)((p->a + p->b + p->c) * 65536.0);
+        hipLaunchKernelGGL(gz_k_rmat_keys, dim3(gz_grid_for(p->num_edges)), dim3(GZ_BLOCK), 0, gz_stream, p->seed,
+                           p->edge_offset, p->scale, rmat_nv, ta, tb, tc, p->num_edges, b, symmetrize, a);
+        GZ_LAUNCH_CHECK();
+    }
+    if (extra_edges) {
+        uint64_t* d_pairs = (uint64_t*)gz_device_malloc(2 * extra_edges * sizeof(uint64_t));
+        gz_copy_to_device(d_pairs, extra_pairs, 2 * extra_edges * sizeof(uint64_t));
+        hipLaunchKernelGGL(gz_k_pairs_keys, dim3(gz_grid_for(extra_edges)), dim3(GZ_BLOCK), 0, gz_stream, d_pairs,
+                           extra_edges, b, symmetrize, a + per * p->num_edges);
+        GZ_LAUNCH_CHECK();
+        GZ_CHECK(hipStreamSynchronize(gz_stream));
+        gz_device_free(d_pairs);
+    }
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    gz_ingress_from_keys(nv, a, bb, n, b, flags);
+    gz_device_free(a);
+    gz_device_free(bb);
+    return 0;
+}

@@ -91,6 +91,9 @@ _sig("gz_count_outdegrees", None, vp, u64, u64, vp)
 _sig("gz_rmat_draw", None, P(RmatParams), vp)
 _sig("gz_splitmix64", u64, u64)
 _sig("gz_graph_load_pairs", C.c_int, u64, u64, vp, u64, vp)
+_sig("gz_graph_build_pairs_device", C.c_int, u64, u64, vp, C.c_int, u32)
+_sig("gz_graph_build_rmat_device", C.c_int, P(RmatParams), u64, C.c_int, u64, vp, u32)
+_sig("gz_graph_device_vertex", u64, u64)
 _sig("gz_graph_release", None)
 _sig("gz_graph_num_vertices", u64)
 _sig("gz_graph_num_edges", u64)
@@ -98,6 +101,7 @@ _sig("gz_graph_pull_vector_count", u64)
 _sig("gz_graph_push_vector_count", u64)
 _sig("gz_graph_download_vectors", None, C.c_int, vp)
 _sig("gz_graph_download_offsets", None, C.c_int, vp)
+_sig("gz_graph_outdegrees_host", P(C.c_double))
 _sig("gz_run_options_default", None, P(RunOptions))
 _sig("gz_execution_prepare", None, P(RunOptions))
 _sig("gz_execution_run", None, P(RunOptions), P(RunResult))
@@ -195,7 +199,16 @@ WORKLOADS = {
     "bfs-rmat24": "BFS graph: R-MAT scale 24, edge factor 16",
     "pr-rmat20": "R-MAT scale 20, edge factor 16",
     "cc-rmat16-sym-trees": "symmetrised R-MAT scale 16 plus 64 planted 100-vertex trees (reduced CC shape)",
+    "full-rmat26-fold41.7m": "R-MAT scale 26 with ids folded by modulo into 41.7M vertices, 1.494B raw edges "
+                             "(about 1.47B after self-loop and duplicate removal); device ingress only",
 }
+
+# The full-scale power-law graph: scale-26 R-MAT draws folded into 41.7M ids.
+# The raw count is chosen so that about 1.47B edges survive deduplication
+# (kept fraction 0.949 / 0.958 / 0.965 measured at the same shape scaled down
+# to scales 20 / 21 / 22, extrapolated to about 0.984 at scale 26).
+FULL_VERTICES = 41_700_000
+FULL_RAW_EDGES = 1_494_000_000
 
 
 def workload_graph(name: str, seed: int = 1) -> EdgeLists:
@@ -261,6 +274,50 @@ def load(g: EdgeLists, pull: bool = True, push: bool = True) -> None:
     ps = np.ascontiguousarray(g.push, dtype=np.uint64)
     lib.gz_graph_load_pairs(g.num_vertices, pl.shape[0], _ptr(pl) if pull else None, ps.shape[0],
                             _ptr(ps) if push else None)
+
+
+RELABEL_BY_OUTDEGREE = 1
+
+
+def load_pairs_device(num_vertices: int, pairs: np.ndarray, symmetrize: bool = False, relabel: bool = False) -> None:
+    """Device ingress from raw (src, dst) pairs: sort, dedupe, pack in HBM."""
+    init()
+    p = np.ascontiguousarray(pairs, dtype=np.uint64).reshape(-1, 2)
+    lib.gz_graph_build_pairs_device(num_vertices, p.shape[0], _ptr(p) if p.shape[0] else None, int(symmetrize),
+                                    RELABEL_BY_OUTDEGREE if relabel else 0)
+
+
+def load_rmat_device(scale: int, num_edges: int, seed: int, symmetrize: bool = False, num_vertices: int = 0,
+                     extra: np.ndarray | None = None, rmat_vertices: int = 0, a=0.57, b=0.19, c=0.19,
+                     relabel: bool = False) -> None:
+    """Device ingress of a seeded R-MAT graph (the stream of rmat_edges) plus
+    optional extra host pairs; V = max(num_vertices, R-MAT vertex count).
+    relabel renumbers vertices by out-degree on the device (results keep
+    original ids)."""
+    init()
+    p = RmatParams(scale, rmat_vertices, num_edges, a, b, c, seed, 0)
+    ex = np.ascontiguousarray(extra if extra is not None else np.zeros((0, 2), np.uint64), dtype=np.uint64)
+    lib.gz_graph_build_rmat_device(C.byref(p), num_vertices, int(symmetrize), ex.shape[0],
+                                   _ptr(ex) if ex.shape[0] else None, RELABEL_BY_OUTDEGREE if relabel else 0)
+
+
+def load_workload_device(name: str, seed: int = 1, relabel: bool = False) -> None:
+    """Build a named workload with device ingress (same graph as workload_graph)."""
+    if name in ("pr-rmat16", "pr-rmat20", "bfs-rmat24"):
+        scale = {"pr-rmat16": 16, "pr-rmat20": 20, "bfs-rmat24": 24}[name]
+        load_rmat_device(scale, (1 << scale) * 16, seed, relabel=relabel)
+        return
+    if name in ("cc-rmat22-sym-trees", "cc-rmat16-sym-trees"):
+        scale, tsize = (22, 1000) if name.startswith("cc-rmat22") else (16, 100)
+        nv0 = 1 << scale
+        tr = planted_trees(nv0, 64, tsize, seed + 1000)
+        load_rmat_device(scale, nv0 * 16, seed, symmetrize=True, num_vertices=nv0 + 64 * tsize, extra=tr,
+                         relabel=relabel)
+        return
+    if name == "full-rmat26-fold41.7m":
+        load_rmat_device(26, FULL_RAW_EDGES, seed, rmat_vertices=FULL_VERTICES, relabel=relabel)
+        return
+    raise KeyError(name)
 
 
 def set_chunk_vectors(n: int) -> None:
@@ -350,6 +407,16 @@ def device_offsets(pull: bool) -> np.ndarray:
     out = np.zeros(num_vertices() + 1, dtype=np.uint64)
     lib.gz_graph_download_offsets(int(pull), _ptr(out))
     return out
+
+
+def outdegrees() -> np.ndarray:
+    """FP64 outdegrees of the loaded graph (host mirror, V entries)."""
+    n = num_vertices()
+    return np.ctypeslib.as_array(lib.gz_graph_outdegrees_host(), shape=(n,)).copy()
+
+
+def num_edges() -> int:
+    return int(lib.gz_graph_num_edges())
 
 
 def release() -> None:

@@ -74,6 +74,33 @@ int gz_graph_load_pairs(uint64_t num_vertices, uint64_t pull_edges, const uint64
 void graph_data_read_from_file(const char* filename_gather, const char* filename_scatter,
                                uint32_t num_numa_nodes, const uint32_t* numa_nodes);
 void graph_data_allocate_merge_buffers(uint64_t num_threads, uint32_t num_numa_nodes, const uint32_t* numa_nodes);
+
+/* Device ingress (replaces the packing consumers of graphdata.c:595-951 with
+ * a device pipeline; no reference counterpart as an entry point).  Both
+ * layouts are built in HBM from raw (src, dst) pairs: grouped by dst (pull)
+ * and src (push) ascending, neighbours ascending inside a group, self-loops
+ * and duplicates dropped, symmetrised first when `symmetrize`.  The result is
+ * bit-identical to gz_edges_sort(dedupe) + gz_pack_vectors on the host.
+ * Needs 1 <= V <= 2^32; dies with exit 65 otherwise.
+ *
+ * Flag GZ_INGRESS_RELABEL_BY_OUTDEGREE renumbers the vertices by raw
+ * out-degree (descending, ties by id) before packing, so that the hot sources
+ * of a power-law graph share cache lines.  The device arrays then live in the
+ * new numbering; every result getter, the -o writer and the BFS root keep
+ * using original ids, CC labels start from the original ids (so they are the
+ * smallest original id of each component, as without renumbering), and BFS
+ * parents are mapped back (a valid BFS tree; ties between equal-depth parents
+ * may resolve differently). */
+#define GZ_INGRESS_RELABEL_BY_OUTDEGREE 1u
+int gz_graph_build_pairs_device(uint64_t num_vertices, uint64_t num_edges, const uint64_t* pairs, int symmetrize,
+                                uint32_t flags);
+/* As above with the raw edges drawn on the device by the generator of
+ * gz_rmat_draw (same stream), plus `extra_edges` host pairs appended (planted
+ * components).  V = max(num_vertices, R-MAT vertex count). */
+int gz_graph_build_rmat_device(const gz_rmat_params_t* p, uint64_t num_vertices, int symmetrize,
+                               uint64_t extra_edges, const uint64_t* extra_pairs, uint32_t flags);
+/* Device id of an original vertex id (identity unless renumbered). */
+uint64_t gz_graph_device_vertex(uint64_t original);
 void gz_graph_release(void);
 
 /* Host mirrors filled at load time (for output, statistics and tests). */
