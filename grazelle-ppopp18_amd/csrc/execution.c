@@ -115,6 +115,101 @@ void gz_execution_prepare(const gz_run_options_t* o)
     gz_records_count = 0;
 }
 
+/* ---- CC / BFS helpers ------------------------------------------------------ */
+
+static void gz_edge_phase(int bfs, int pull)
+{
+    if (pull) {
+        if (bfs)
+            perform_edge_pull_phase_bfs(gz_graph.pull_vectors, gz_graph.pull_vector_count);
+        else
+            perform_edge_pull_phase_cc(gz_graph.pull_vectors, gz_graph.pull_vector_count);
+    } else {
+        if (bfs)
+            perform_edge_push_phase_bfs(gz_graph.push_vectors, gz_graph.push_vector_count);
+        else
+            perform_edge_push_phase_cc(gz_graph.push_vectors, gz_graph.push_vector_count);
+    }
+}
+
+/* What the selector left next to the vote (gz_sel_dev_step). */
+typedef struct gz_selector_reading_t {
+    uint32_t choice;            /* 1 pull, 2 push, 0 no model */
+    float features[8];
+    float predicted[2];
+} gz_selector_reading_t;
+
+static void gz_selector_read(gz_selector_reading_t* s)
+{
+    const uint64_t* w = gz_reduce_words();
+    s->choice = (uint32_t)w[1];
+    memcpy(s->features, &w[2], sizeof(s->features));
+    memcpy(s->predicted, &w[6], sizeof(s->predicted));
+}
+
+/* Everything a CC / BFS edge phase writes: vertex words and the three
+ * bitmaps (HasInfo may be updated by CC push, WantsInfo by BFS, HasInfo* by
+ * both). */
+typedef struct gz_snapshot_t {
+    void* props;
+    void* has;
+    void* wants;
+    void* next;
+    size_t vbytes, fbytes;
+} gz_snapshot_t;
+
+static void gz_snapshot_alloc(gz_snapshot_t* s)
+{
+    const uint64_t V = gz_graph.num_vertices;
+    s->vbytes = V * sizeof(uint64_t);
+    s->fbytes = ((V + 63) / 64) * sizeof(uint64_t);
+    s->props = gz_device_malloc(s->vbytes ? s->vbytes : 8);
+    s->has = gz_device_malloc(s->fbytes ? s->fbytes : 8);
+    s->wants = gz_device_malloc(s->fbytes ? s->fbytes : 8);
+    s->next = gz_device_malloc(s->fbytes ? s->fbytes : 8);
+}
+
+static void gz_snapshot_free(gz_snapshot_t* s)
+{
+    gz_device_free(s->props);
+    gz_device_free(s->has);
+    gz_device_free(s->wants);
+    gz_device_free(s->next);
+}
+
+static void gz_snapshot_take(gz_snapshot_t* s)
+{
+    gz_device_copy(s->props, gz_graph.vertex_props, s->vbytes);
+    gz_device_copy(s->has, gz_graph.frontier_has_info, s->fbytes);
+    gz_device_copy(s->wants, gz_graph.frontier_wants_info, s->fbytes);
+    gz_device_copy(s->next, gz_graph.vertex_accumulators, s->fbytes);
+}
+
+static void gz_snapshot_restore(const gz_snapshot_t* s)
+{
+    gz_device_copy(gz_graph.vertex_props, s->props, s->vbytes);
+    gz_device_copy(gz_graph.frontier_has_info, s->has, s->fbytes);
+    gz_device_copy(gz_graph.frontier_wants_info, s->wants, s->fbytes);
+    gz_device_copy(gz_graph.vertex_accumulators, s->next, s->fbytes);
+}
+
+/* Best of three timed runs of one engine's edge phase, each from the
+ * snapshot; the state afterwards is that engine's result. */
+static double gz_probe_engine(int bfs, int pull, const gz_snapshot_t* s, gz_timer_t* t)
+{
+    double best = 0.0;
+    for (int rep = 0; rep < 3; ++rep) {
+        gz_snapshot_restore(s);
+        gz_timer_start(t);
+        phase_op_reset_global_accum();
+        gz_edge_phase(bfs, pull);
+        gz_timer_stop(t);
+        const double ms = gz_timer_elapsed_ms(t);
+        if (rep == 0 || ms < best) best = ms;
+    }
+    return best;
+}
+
 static void gz_swap_frontiers(void)
 {
     /* execution_cc.c:245-250 / execution_bfs.c:253-258 */
@@ -179,12 +274,24 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
         if (o->stat_mode != GZ_STAT_OUTDEG_ONLY) vote += V;
     }
     const uint64_t threshold = (o->stat_mode == GZ_STAT_COUNT_ONLY) ? V / 2 : E / 5;
-    gz_timer_t* t = (o->profile || o->engine == GZ_ENGINE_LEARNED) ? gz_timer_create() : NULL;
-    gz_selector_state_t sel;
-    gz_selector_begin(&sel, V, E);
+    const int learned = (o->engine == GZ_ENGINE_LEARNED) && gz_selector_loaded();
+    const int probe = (o->engine == GZ_ENGINE_PROBE) && gz_graph.pull_vectors != NULL && gz_graph.push_vectors != NULL;
+    const int features = learned || probe;
+    gz_timer_t* t = (o->profile || probe) ? gz_timer_create() : NULL;
+    const uint32_t* indeg = features ? gz_graph_indegrees_device() : NULL;
+    gz_snapshot_t snap;
+    memset(&snap, 0, sizeof(snap));
+    if (probe) gz_snapshot_alloc(&snap);
+    gz_selector_reading_t sel;
+    memset(&sel, 0, sizeof(sel));
+    if (features) {
+        /* features of the first iteration: the initial frontier */
+        gz_sel_dev_begin(V, E);
+        gz_sel_dev_step(gz_graph.frontier_has_info, indeg, gz_reduce_buffer, learned, bfs, 1, 0, 0.0f);
+        (void)phase_op_combine_global_var_from_buf(gz_reduce_buffer);
+        gz_selector_read(&sel);
+    }
     uint64_t ctr = 0;
-    uint32_t prev_engine = 0;
-    double prev_ms = 0.0;
     while (1) {
         ctr += 1;
         if (o->max_iterations && ctr > o->max_iterations) {
@@ -196,8 +303,8 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
             use_pull = 1;
         else if (o->engine == GZ_ENGINE_PUSH)
             use_pull = 0;
-        else if (o->engine == GZ_ENGINE_LEARNED && gz_selector_loaded())
-            use_pull = gz_selector_choose_pull(&sel, ctr, vote, prev_engine, prev_ms);
+        else if (learned)
+            use_pull = (sel.choice == 1);
         else
             use_pull = vote > threshold;
         if (use_pull && gz_graph.pull_vectors == NULL) use_pull = 0;
@@ -208,31 +315,41 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
         rec.iteration = ctr;
         rec.engine = use_pull ? 1 : 2;
         rec.frontier_stat = vote;
-        if (t) gz_timer_start(t);
+        memcpy(rec.features, sel.features, sizeof(rec.features));
+        memcpy(rec.predicted, sel.predicted, sizeof(rec.predicted));
         gz_phase_config.bfs_level = (uint32_t)ctr;
-        phase_op_reset_global_accum();
-        if (use_pull) {
-            r->pull_iterations += 1;
-            if (bfs)
-                perform_edge_pull_phase_bfs(gz_graph.pull_vectors, gz_graph.pull_vector_count);
-            else
-                perform_edge_pull_phase_cc(gz_graph.pull_vectors, gz_graph.pull_vector_count);
+        float prev_ms = -1.0f;
+        if (probe) {
+            /* the chosen engine runs last, so its result is what continues */
+            gz_snapshot_take(&snap);
+            const double other = gz_probe_engine(bfs, !use_pull, &snap, t);
+            const double chosen = gz_probe_engine(bfs, use_pull, &snap, t);
+            rec.pull_ms = use_pull ? chosen : other;
+            rec.push_ms = use_pull ? other : chosen;
+            rec.edge_ms = chosen;
+            prev_ms = (float)chosen;
         } else {
-            r->push_iterations += 1;
-            if (bfs)
-                perform_edge_push_phase_bfs(gz_graph.push_vectors, gz_graph.push_vector_count);
-            else
-                perform_edge_push_phase_cc(gz_graph.push_vectors, gz_graph.push_vector_count);
+            if (t) gz_timer_start(t);
+            if (learned) gz_sel_dev_stamp();
+            phase_op_reset_global_accum();
+            gz_edge_phase(bfs, use_pull);
         }
+        if (use_pull)
+            r->pull_iterations += 1;
+        else
+            r->push_iterations += 1;
         phase_op_write_global_accum_to_buf(gz_reduce_buffer);
-        if (t) gz_timer_stop(t);
+        if (t && !probe) gz_timer_stop(t);
+        if (features) {
+            /* the vertices this phase activated are the next frontier */
+            gz_sel_dev_step((const uint64_t*)gz_graph.vertex_accumulators, indeg, gz_reduce_buffer, learned, bfs,
+                            ctr + 1, rec.engine, prev_ms);
+        }
         const uint64_t next_vote = phase_op_combine_global_var_from_buf(gz_reduce_buffer);
-        if (t) rec.edge_ms = gz_timer_elapsed_ms(t);
-        prev_engine = rec.engine;
-        prev_ms = rec.edge_ms;
+        if (features) gz_selector_read(&sel);
+        if (t && !probe) rec.edge_ms = gz_timer_elapsed_ms(t);
         if (o->collect_stats) gz_phase_stats_read(&rec.vectors, &rec.edges);
-        if (o->profile || o->collect_stats) gz_record(&rec);
-        gz_selector_observe(&sel, vote, next_vote);
+        if (o->profile || o->collect_stats || features) gz_record(&rec);
         vote = next_vote;
         if (0 == vote) break;
         gz_swap_frontiers();
@@ -243,6 +360,7 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
     }
     r->iterations = ctr;
     if (t) gz_timer_destroy(t);
+    if (probe) gz_snapshot_free(&snap);
 }
 
 void gz_execution_run(const gz_run_options_t* o, gz_run_result_t* r)

@@ -254,6 +254,7 @@ typedef struct gz_host_graph_t {
     uint64_t* d_original_ids;    /* original id by device vertex */
     uint64_t* original_ids;      /* host mirror of the above */
     uint64_t* device_ids;        /* device vertex by original id */
+    uint32_t* d_indeg;           /* in-degrees, built on first use */
 } gz_host_graph_t;
 
 static gz_host_graph_t G;
@@ -288,6 +289,7 @@ void gz_graph_release(void)
     gz_device_free(G.d_has);
     gz_device_free(G.d_wants);
     gz_device_free(G.d_original_ids);
+    gz_device_free(G.d_indeg);
     free(G.original_ids);
     free(G.device_ids);
     free(G.outdeg);
@@ -363,6 +365,18 @@ void gz_graph_adopt_device(const gz_device_layouts_t* L)
 }
 
 const uint64_t* gz_graph_original_ids_device(void) { return G.d_original_ids; }
+
+const uint32_t* gz_graph_indegrees_device(void)
+{
+    if (G.d_indeg == NULL && G.num_vertices) {
+        G.d_indeg = (uint32_t*)gz_device_malloc((G.num_vertices + 8) * sizeof(uint32_t));
+        if (G.d_pull != NULL)
+            gz_dev_indegrees(G.d_pull, G.pull_vectors, 1, G.d_indeg, G.num_vertices);
+        else
+            gz_dev_indegrees(G.d_push, G.push_vectors, 0, G.d_indeg, G.num_vertices);
+    }
+    return G.d_indeg;
+}
 
 uint64_t gz_graph_device_vertex(uint64_t original)
 {

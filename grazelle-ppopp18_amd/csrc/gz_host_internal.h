@@ -36,6 +36,19 @@ void gz_dev_fill_u64(uint64_t* p, uint64_t value, uint64_t n);
 void gz_dev_iota_u64(uint64_t* p, uint64_t n);
 void gz_dev_store_u64(uint64_t* p, uint64_t index, uint64_t value);
 void gz_graph_reset_roles(void);
+void gz_dev_indegrees(const gz_vec_t* vecs, uint64_t count, int pull_layout, uint32_t* indeg, uint64_t n);
+/* In-degrees of the loaded graph (u32[V+8]), built on first use. */
+const uint32_t* gz_graph_indegrees_device(void);
+
+/* The 8 reduce-buffer words as read by the last
+ * phase_op_combine_global_var_from_buf (pinned host memory). */
+const uint64_t* gz_reduce_words(void);
+
+/* Learned selector on the device (selector_kernels.hip). */
+void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges);
+void gz_sel_dev_stamp(void);
+void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg, uint64_t* reduce_buffer, int have_model,
+                     int bfs, uint64_t iteration, uint32_t prev_engine, float prev_ms);
 void gz_clear_bitmap_range(uint64_t* bitmap, uint64_t vertex_start, uint64_t vertex_count);
 
 #ifdef __cplusplus

@@ -71,3 +71,34 @@ extern "C" void gz_dev_store_u64(uint64_t* p, uint64_t index, uint64_t value)
     hipLaunchKernelGGL(gz_k_store, dim3(1), dim3(1), 0, gz_stream, p, index, value);
     GZ_LAUNCH_CHECK();
 }
+
+/* In-degrees from either layout: the pull list holds each destination's
+ * in-edges as its vectors' valid lanes; in the push list every valid lane
+ * names one destination. */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_indegrees(const gz_vec_t* __restrict__ vecs, uint64_t count,
+                                                           int pull_layout, uint32_t* __restrict__ indeg)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) {
+        const gz_lanes v = gz_load_vector(vecs, i);
+        const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
+        if (pull_layout) {
+            uint32_t c = 0;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) c += gz_lane_valid(l[k]) ? 1u : 0u;
+            atomicAdd(&indeg[gz_shared_vertex(v)], c);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (gz_lane_valid(l[k])) atomicAdd(&indeg[gz_lane_id(l[k])], 1u);
+        }
+    }
+}
+
+extern "C" void gz_dev_indegrees(const gz_vec_t* vecs, uint64_t count, int pull_layout, uint32_t* indeg, uint64_t n)
+{
+    gz_device_zero(indeg, (n + 8) * sizeof(uint32_t));
+    if (count == 0) return;
+    hipLaunchKernelGGL(gz_k_indegrees, dim3(gz_blocks_for(count)), dim3(GZ_BLOCK), 0, gz_stream, vecs, count,
+                       pull_layout, indeg);
+    GZ_LAUNCH_CHECK();
+}

@@ -436,11 +436,23 @@ extern "C" void phase_op_write_global_accum_to_buf(uint64_t* reduce_buffer)
     gz_reduce_partials(reduce_buffer);
 }
 
+/* Pinned host copy of the whole 8-word reduce buffer: the vote is word 0,
+ * and the learned selector leaves its decision and features in words 1-6,
+ * so one small copy serves both. */
+static uint64_t* gz_reduce_mirror = nullptr;
+
 extern "C" uint64_t phase_op_combine_global_var_from_buf(uint64_t* reduce_buffer)
 {
-    uint64_t v = 0;
-    gz_copy_to_host(&v, reduce_buffer, sizeof(v));
-    return v;
+    gz_require_runtime();
+    if (gz_reduce_mirror == nullptr) GZ_CHECK(hipHostMalloc((void**)&gz_reduce_mirror, 8 * sizeof(uint64_t)));
+    GZ_CHECK(hipMemcpyAsync(gz_reduce_mirror, reduce_buffer, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, gz_stream));
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    return gz_reduce_mirror[0];
+}
+
+extern "C" const uint64_t* gz_reduce_words(void)
+{
+    return gz_reduce_mirror;
 }
 
 /* Parallel form of the serial merge in phases.c:41-70.  Records sharing a
@@ -515,6 +527,8 @@ extern "C" void gz_runtime_shutdown(void)
     gz_merge_buffer = nullptr;
     gz_merge_buffer_count = 0;
     memset(&gz_scratch, 0, sizeof(gz_scratch));
+    if (gz_reduce_mirror != nullptr) GZ_CHECK(hipHostFree(gz_reduce_mirror));
+    gz_reduce_mirror = nullptr;
     GZ_CHECK(hipStreamDestroy(gz_stream));
     gz_stream = nullptr;
     gz_device_id = -1;

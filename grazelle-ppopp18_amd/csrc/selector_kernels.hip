@@ -11,6 +11,8 @@
  * writes one partial per block, and the Adam kernel sums the block partials
  * in index order.
  */
+#include <string.h>
+
 #include "gz_internal.h"
 #include "grazelle_selector.h"
 
@@ -134,27 +136,209 @@ __global__ void gz_k_sel_adam(float* __restrict__ p, float* __restrict__ m, floa
     p[i] -= lr * (mi / bc1) / (sqrtf(vi / bc2) + eps);
 }
 
+/* ---- per-iteration decision on the device ---------------------------------
+ * The drivers read one block of words from the reduce buffer after every
+ * edge phase (the converge vote).  The selector rides on that read: right
+ * after the edge phase one kernel folds the statistics of the next frontier
+ * (the vertices the phase activated) and its last block to finish builds the
+ * feature vector, runs the MLP and writes the decision and the features next
+ * to the vote, so choosing the engine costs one extra launch and no extra
+ * host synchronisation. */
+
+struct gz_sel_dev_state {
+    uint64_t frontier_vertices;  /* |frontier|, accumulated by the step kernel */
+    uint64_t frontier_edges;     /* sum of out-degrees over the frontier */
+    uint64_t frontier_in_edges;  /* sum of in-degrees over the frontier */
+    uint64_t unvisited;          /* BFS: vertices never in a frontier */
+    uint64_t unvisited_in_edges; /* BFS: their in-degrees */
+    uint64_t t_begin, t_end;     /* wall clock around the last edge phase */
+    uint32_t blocks_done;        /* last-block detection of the step kernel */
+    uint32_t pad;
+};
+
+__global__ void gz_k_sel_stamp(gz_sel_dev_state* st)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) st->t_begin = (uint64_t)wall_clock64();
+}
+
+/* Statistics of `frontier` into the state; the last block then writes
+ * out[1] = decision (1 pull, 2 push, 0 no model), out[2..5] = the raw
+ * features as 8 floats, out[6] = the two outputs as floats, and clears the
+ * accumulators for the next call.  The unvisited counts advance by the
+ * frontier (BFS marks each vertex once). */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_step(const uint64_t* __restrict__ frontier,
+                                                          const double* __restrict__ outdeg,
+                                                          const uint32_t* __restrict__ indeg, uint64_t num_vertices,
+                                                          uint64_t num_edges, uint64_t words, gz_sel_dev_state* st,
+                                                          const float* __restrict__ p, const float* __restrict__ norm,
+                                                          int have_model, int bfs, float iteration, float prev_engine,
+                                                          float prev_ms, float clock_khz, uint64_t* __restrict__ out)
+{
+    __shared__ float x[GZ_SEL_IN], xn[GZ_SEL_IN], h1[GZ_SEL_H], h2[GZ_SEL_H], y[GZ_SEL_OUT];
+    __shared__ int last;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && prev_ms < 0.f) st->t_end = (uint64_t)wall_clock64();
+    uint64_t fv = 0, fe = 0, fi = 0;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words;
+         w += (uint64_t)gridDim.x * blockDim.x) {
+        uint64_t bits = frontier[w];
+        const uint64_t base = w * 64;
+        if (base + 64 > num_vertices) {
+            const uint64_t keep = num_vertices - base;
+            bits &= (keep >= 64) ? ~0ull : ((1ull << keep) - 1ull);
+        }
+        fv += (uint64_t)__popcll(bits);
+        while (bits) {
+            const int b = __ffsll((unsigned long long)bits) - 1;
+            bits &= bits - 1;
+            fe += (uint64_t)outdeg[base + (uint64_t)b];
+            fi += (uint64_t)indeg[base + (uint64_t)b];
+        }
+    }
+    fv = gz_wave_sum_u64(fv);
+    fe = gz_wave_sum_u64(fe);
+    fi = gz_wave_sum_u64(fi);
+    if (gz_lane() == 0 && (fv | fe | fi)) {
+        atomicAdd((unsigned long long*)&st->frontier_vertices, (unsigned long long)fv);
+        atomicAdd((unsigned long long*)&st->frontier_edges, (unsigned long long)fe);
+        atomicAdd((unsigned long long*)&st->frontier_in_edges, (unsigned long long)fi);
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) last = (atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1);
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    const int j = threadIdx.x;
+    if (j == 0) {
+        const uint64_t cv = atomicAdd((unsigned long long*)&st->frontier_vertices, 0ull);
+        const uint64_t ce = atomicAdd((unsigned long long*)&st->frontier_edges, 0ull);
+        const uint64_t ci = atomicAdd((unsigned long long*)&st->frontier_in_edges, 0ull);
+        uint64_t uv = num_vertices, ue = num_edges;
+        if (bfs) {
+            st->unvisited -= cv;
+            st->unvisited_in_edges -= ci;
+            uv = st->unvisited;
+            ue = st->unvisited_in_edges;
+        }
+        float ms = prev_ms;
+        if (ms < 0.f) ms = (st->t_end > st->t_begin) ? (float)(st->t_end - st->t_begin) / clock_khz : 0.f;
+        x[0] = logf(1.f + (float)cv);
+        x[1] = logf(1.f + (float)ce);
+        x[2] = logf(1.f + (float)uv);
+        x[3] = logf(1.f + (float)ue);
+        x[4] = iteration;
+        x[5] = prev_engine;
+        x[6] = logf(1.f + ms);
+        x[7] = num_vertices ? (float)num_edges / (float)num_vertices : 0.f;
+        for (int i = 0; i < GZ_SEL_IN; ++i) {
+            const float sd = norm[GZ_SEL_IN + i];
+            xn[i] = (x[i] - norm[i]) / (sd > 0.f ? sd : 1.f);
+        }
+        st->frontier_vertices = 0;
+        st->frontier_edges = 0;
+        st->frontier_in_edges = 0;
+        st->blocks_done = 0;
+    }
+    __syncthreads();
+    if (have_model) {
+        if (j < GZ_SEL_H) {
+            float a = p[O_B1 + j];
+            for (int i = 0; i < GZ_SEL_IN; ++i) a += p[O_W1 + j * GZ_SEL_IN + i] * xn[i];
+            h1[j] = a > 0.f ? a : 0.f;
+        }
+        __syncthreads();
+        if (j < GZ_SEL_H) {
+            float a = p[O_B2 + j];
+            for (int i = 0; i < GZ_SEL_H; ++i) a += p[O_W2 + j * GZ_SEL_H + i] * h1[i];
+            h2[j] = a > 0.f ? a : 0.f;
+        }
+        __syncthreads();
+        if (j < GZ_SEL_OUT) {
+            float a = p[O_B3 + j];
+            for (int i = 0; i < GZ_SEL_H; ++i) a += p[O_W3 + j * GZ_SEL_H + i] * h2[i];
+            y[j] = a;
+        }
+        __syncthreads();
+    }
+    if (j == 0) {
+        float* fo = (float*)&out[2];
+        for (int i = 0; i < GZ_SEL_IN; ++i) fo[i] = x[i];
+        float* yo = (float*)&out[6];
+        yo[0] = have_model ? y[0] : 0.f;
+        yo[1] = have_model ? y[1] : 0.f;
+        out[1] = have_model ? ((y[0] < y[1]) ? 1u : 2u) : 0u;
+    }
+}
+
 /* ---- host entry points ---------------------------------------------------- */
 
 static float* gz_sel_dev_params = nullptr;
-static float* gz_sel_dev_io = nullptr;
+static float* gz_sel_dev_norm = nullptr;      /* mean[8], std[8] */
+static gz_sel_dev_state* gz_sel_dev_st = nullptr;
+static float gz_sel_clock_khz = 0.f;
+
+extern "C" int gz_selector_device_upload_norm(const float* mean, const float* stdev)
+{
+    if (gz_sel_dev_norm == nullptr) gz_sel_dev_norm = (float*)gz_device_malloc(2 * GZ_SEL_IN * sizeof(float));
+    gz_copy_to_device(gz_sel_dev_norm, mean, GZ_SEL_IN * sizeof(float));
+    gz_copy_to_device(gz_sel_dev_norm + GZ_SEL_IN, stdev, GZ_SEL_IN * sizeof(float));
+    return 0;
+}
+
+/* Start a run: unvisited = V / E (the first step removes the initial
+ * frontier). */
+extern "C" void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges)
+{
+    if (gz_sel_dev_st == nullptr) {
+        gz_sel_dev_st = (gz_sel_dev_state*)gz_device_malloc(sizeof(gz_sel_dev_state));
+        int rate = 0;
+        int dev = 0;
+        GZ_CHECK(hipGetDevice(&dev));
+        GZ_CHECK(hipDeviceGetAttribute(&rate, hipDeviceAttributeWallClockRate, dev));
+        gz_sel_clock_khz = (float)(rate > 0 ? rate : 100000);
+    }
+    if (gz_sel_dev_norm == nullptr) {
+        const float zero[GZ_SEL_IN] = {0, 0, 0, 0, 0, 0, 0, 0};
+        const float one[GZ_SEL_IN] = {1, 1, 1, 1, 1, 1, 1, 1};
+        gz_selector_device_upload_norm(zero, one);
+    }
+    gz_sel_dev_state s;
+    memset(&s, 0, sizeof(s));
+    s.unvisited = num_vertices;
+    s.unvisited_in_edges = num_edges;
+    gz_copy_to_device(gz_sel_dev_st, &s, sizeof(s));
+}
+
+/* Wall-clock stamp taken before an edge phase (learned engine). */
+extern "C" void gz_sel_dev_stamp(void)
+{
+    hipLaunchKernelGGL(gz_k_sel_stamp, dim3(1), dim3(64), 0, gz_stream, gz_sel_dev_st);
+    GZ_LAUNCH_CHECK();
+}
+
+/* Statistics of `frontier` (a bitmap over V vertices), features and (with a
+ * model) the decision into reduce_buffer[1..6].  prev_ms < 0 takes the
+ * previous edge phase's time from the device clock (stamp to this launch). */
+extern "C" void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg, uint64_t* reduce_buffer,
+                                int have_model, int bfs, uint64_t iteration, uint32_t prev_engine, float prev_ms)
+{
+    if (have_model && gz_sel_dev_params == nullptr) have_model = 0;
+    const uint64_t V = gz_graph.num_vertices;
+    const uint64_t words = (V + 63) / 64;
+    uint64_t blocks = (words + GZ_BLOCK - 1) / GZ_BLOCK;
+    if (blocks > 1024) blocks = 1024;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(gz_k_sel_step, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, frontier,
+                       gz_graph.vertex_outdegrees, indeg, V, gz_graph.num_edges, words, gz_sel_dev_st,
+                       gz_sel_dev_params, gz_sel_dev_norm, have_model, bfs, (float)iteration, (float)prev_engine,
+                       prev_ms, gz_sel_clock_khz, reduce_buffer);
+    GZ_LAUNCH_CHECK();
+}
 
 extern "C" int gz_selector_device_upload(const float* params)
 {
     if (gz_sel_dev_params == nullptr) gz_sel_dev_params = (float*)gz_device_malloc(GZ_SEL_PARAMS * sizeof(float));
     gz_copy_to_device(gz_sel_dev_params, params, GZ_SEL_PARAMS * sizeof(float));
-    return 0;
-}
-
-extern "C" int gz_selector_device_forward(const float* x, float* y)
-{
-    if (gz_sel_dev_params == nullptr) return -1;
-    if (gz_sel_dev_io == nullptr) gz_sel_dev_io = (float*)gz_device_malloc(64 * sizeof(float));
-    gz_copy_to_device(gz_sel_dev_io, x, GZ_SEL_IN * sizeof(float));
-    hipLaunchKernelGGL(gz_k_sel_forward, dim3(1), dim3(64), 0, gz_stream, gz_sel_dev_params, gz_sel_dev_io,
-                       gz_sel_dev_io + 32, (uint64_t)1);
-    GZ_LAUNCH_CHECK();
-    gz_copy_to_host(y, gz_sel_dev_io + 32, GZ_SEL_OUT * sizeof(float));
     return 0;
 }
 

@@ -17,7 +17,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "_build", "libgrazelle.so")
 
 ALG = {"pr": 0, "cc": 1, "bfs": 2}
-ENGINE = {"hybrid": 0, "pull": 1, "push": 2, "learned": 3}
+ENGINE = {"hybrid": 0, "pull": 1, "push": 2, "learned": 3, "probe": 4}
 STAT = {"both": 0, "outdeg": 1, "count": 2}
 PULL_VARIANT = {"sched_aware": 0, "traditional": 1}
 
@@ -46,7 +46,8 @@ class RunResult(C.Structure):
 
 class IterationRecord(C.Structure):
     _fields_ = [("iteration", u64), ("engine", u32), ("pad", u32), ("frontier_stat", u64),
-                ("vectors", u64), ("edges", u64), ("edge_ms", C.c_double)]
+                ("vectors", u64), ("edges", u64), ("edge_ms", C.c_double), ("features", C.c_float * 8),
+                ("predicted", C.c_float * 2), ("pull_ms", C.c_double), ("push_ms", C.c_double)]
 
 
 class RmatParams(C.Structure):
@@ -117,6 +118,9 @@ _sig("gz_kernel_timing_reset", None)
 _sig("gz_kernel_timing_read", None, C.c_int, P(C.c_double), P(u64))
 _sig("gz_selector_set_weights", C.c_int, vp, vp, vp)
 _sig("gz_selector_clear", None)
+_sig("gz_selector_load_file", C.c_int, C.c_char_p)
+_sig("gz_selector_save_file", C.c_int, C.c_char_p)
+_sig("gz_selector_loaded", C.c_int)
 _sig("gz_selector_forward_batch", C.c_int, vp, vp, vp, u64)
 _sig("gz_selector_gradient", C.c_float, vp, vp, vp, u64, vp)
 _sig("gz_selector_train", C.c_int, vp, vp, vp, u64, u32, C.c_float, vp)
@@ -492,3 +496,19 @@ def selector_set(params: np.ndarray, mean: np.ndarray, std: np.ndarray) -> None:
 
 def selector_clear() -> None:
     lib.gz_selector_clear()
+
+
+def selector_load(path: str) -> None:
+    rc = lib.gz_selector_load_file(path.encode())
+    if rc != 0:
+        raise OSError(f"cannot load selector weights from {path} ({rc})")
+
+
+def selector_save(path: str) -> None:
+    rc = lib.gz_selector_save_file(path.encode())
+    if rc != 0:
+        raise OSError(f"cannot write selector weights to {path} ({rc})")
+
+
+def selector_loaded() -> bool:
+    return bool(lib.gz_selector_loaded())

@@ -1,0 +1,205 @@
+"""Training data, training and evaluation of the learned engine selector.
+
+The selector (an 8->32->32->2 FP32 MLP, include/grazelle_selector.h) predicts
+the log edge-phase time of the pull and push engines from per-iteration
+frontier statistics.  Its training data comes from the engine itself: the
+probe engine (GZ_ENGINE_PROBE) runs every iteration of a search twice from the
+same state, once per engine, records both times with the features the
+selector would see, and continues with the fixed-threshold choice.  A BFS
+visits the same frontiers whichever engine runs each level, so one probed
+search gives the per-level cost of both engines along the only trajectory
+there is, and any policy's total can be read off it.
+
+Training minimises the squared error of the predicted log times with the
+library's own device kernels (gz_selector_train: forward, backward and Adam
+on the GPU).
+
+    python selector_train.py collect --graphs 200 --out data.npz
+    python selector_train.py fit --data data.npz --out selector_bfs.bin
+    python selector_train.py evaluate --data held.npz --weights selector_bfs.bin
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+if HERE not in sys.path:
+    sys.path.insert(0, HERE)
+
+import grazelle as gz  # noqa: E402
+
+DEFAULT_WEIGHTS = os.path.join(HERE, "selector_bfs.bin")
+
+
+def training_graphs(count: int, first_seed: int = 1000):
+    """(scale, edge factor, seed) of the training graphs: scales 18..22 and
+    edge factors 8 / 16 / 24 / 32 in turn, seeds first_seed, first_seed+1, ..."""
+    efs = (8, 16, 24, 32)
+    return [(18 + i % 5, efs[(i // 5) % 4], first_seed + i) for i in range(count)]
+
+
+def pick_roots(count: int, seed: int) -> list[int]:
+    """Seeded roots among the loaded graph's vertices with out-degree > 0
+    (original ids)."""
+    od = gz.outdegrees()
+    cand_dev = np.nonzero(od > 0)[0]
+    rng = np.random.default_rng(seed)
+    picks = rng.choice(cand_dev.shape[0], size=min(count, cand_dev.shape[0]), replace=False)
+    # outdegrees() is indexed by device vertex; map back to original ids
+    n = gz.num_vertices()
+    if gz.lib.gz_graph_device_vertex(0) == 0 and gz.lib.gz_graph_device_vertex(n - 1) == n - 1:
+        return [int(cand_dev[p]) for p in picks]
+    orig = np.empty(n, dtype=np.int64)
+    for v in range(n):
+        orig[gz.lib.gz_graph_device_vertex(v)] = v
+    return [int(orig[cand_dev[p]]) for p in picks]
+
+
+def probe_bfs(root: int) -> dict:
+    """One probed BFS: per-level features, both engines' times, fixed choice."""
+    gz.run(gz.options("bfs", engine="probe", root=root))
+    recs = gz.records()
+    return {
+        "features": np.array([list(r.features) for r in recs], dtype=np.float32).reshape(-1, 8),
+        "pull_ms": np.array([r.pull_ms for r in recs]),
+        "push_ms": np.array([r.push_ms for r in recs]),
+        "fixed": np.array([r.engine for r in recs], dtype=np.int32),
+    }
+
+
+def collect(specs, roots_per_graph: int, log=print) -> dict:
+    runs = []
+    t0 = time.time()
+    for k, (scale, ef, seed) in enumerate(specs):
+        gz.load_rmat_device(scale, (1 << scale) * ef, seed)
+        for root in pick_roots(roots_per_graph, seed):
+            run = probe_bfs(root)
+            run["graph"] = (scale, ef, seed)
+            run["root"] = root
+            runs.append(run)
+        log(f"[collect] graph {k + 1}/{len(specs)} s{scale} ef{ef} seed {seed}: "
+            f"{sum(r['features'].shape[0] for r in runs)} samples, {time.time() - t0:.1f} s")
+    return pack_runs(runs)
+
+
+def pack_runs(runs: list[dict]) -> dict:
+    lens = np.array([r["features"].shape[0] for r in runs], dtype=np.int64)
+    return {
+        "features": np.concatenate([r["features"] for r in runs]) if runs else np.zeros((0, 8), np.float32),
+        "pull_ms": np.concatenate([r["pull_ms"] for r in runs]) if runs else np.zeros(0),
+        "push_ms": np.concatenate([r["push_ms"] for r in runs]) if runs else np.zeros(0),
+        "fixed": np.concatenate([r["fixed"] for r in runs]) if runs else np.zeros(0, np.int32),
+        "run_lengths": lens,
+        "graphs": np.array([r["graph"] for r in runs], dtype=np.int64).reshape(-1, 3),
+        "roots": np.array([r["root"] for r in runs], dtype=np.int64),
+    }
+
+
+def targets(data: dict) -> np.ndarray:
+    return np.log(np.stack([data["pull_ms"], data["push_ms"]], axis=1)).astype(np.float32)
+
+
+def init_params(seed: int) -> np.ndarray:
+    """He-uniform weights, zero biases, in the W1 b1 W2 b2 W3 b3 layout."""
+    rng = np.random.default_rng(seed)
+    parts = []
+    for fan_in, fan_out in ((8, 32), (32, 32), (32, 2)):
+        lim = np.sqrt(6.0 / fan_in)
+        parts.append(rng.uniform(-lim, lim, size=fan_out * fan_in))
+        parts.append(np.zeros(fan_out))
+    return np.concatenate(parts).astype(np.float32)
+
+
+def fit(data: dict, steps: int = 3000, lr: float = 3e-3, seed: int = 7):
+    X = data["features"].astype(np.float32)
+    mean = X.mean(axis=0)
+    std = X.std(axis=0)
+    std[std < 1e-6] = 1.0
+    Xn = ((X - mean) / std).astype(np.float32)
+    params, hist = gz.selector_train(init_params(seed), Xn, targets(data), steps, lr)
+    return params, mean.astype(np.float32), std.astype(np.float32), hist
+
+
+def policy_times(data: dict, params=None, mean=None, std=None) -> dict:
+    """Total edge-phase ms of the fixed threshold, the learned selector and
+    the per-level best, summed over the probed runs.  The learned policy sees
+    its own previous choice and time (features 5 and 6)."""
+    X = data["features"].astype(np.float32)
+    tp, ts = data["pull_ms"], data["push_ms"]
+    fixed = np.where(data["fixed"] == 1, tp, ts)
+    out = {"fixed": float(fixed.sum()), "best": float(np.minimum(tp, ts).sum())}
+    if params is not None:
+        start = 0
+        total = 0.0
+        for n in data["run_lengths"]:
+            prev_engine, prev_ms = 0.0, 0.0
+            for i in range(start, start + int(n)):
+                x = X[i].copy()
+                x[5] = prev_engine
+                x[6] = np.log1p(prev_ms)
+                y = gz.selector_forward(params, ((x - mean) / std)[None, :].astype(np.float32))[0]
+                pull = y[0] < y[1]
+                t = tp[i] if pull else ts[i]
+                total += t
+                prev_engine, prev_ms = (1.0 if pull else 2.0), t
+            start += int(n)
+        out["learned"] = total
+    return out
+
+
+def save_weights(path: str, params, mean, std) -> None:
+    with open(path, "wb") as f:
+        f.write(np.ascontiguousarray(params, dtype="<f4").tobytes())
+        f.write(np.ascontiguousarray(mean, dtype="<f4").tobytes())
+        f.write(np.ascontiguousarray(std, dtype="<f4").tobytes())
+
+
+def load_weights(path: str):
+    raw = np.fromfile(path, dtype="<f4")
+    p = raw[:gz.SEL_PARAMS]
+    return p, raw[gz.SEL_PARAMS:gz.SEL_PARAMS + 8], raw[gz.SEL_PARAMS + 8:gz.SEL_PARAMS + 16]
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("collect")
+    c.add_argument("--graphs", type=int, default=200)
+    c.add_argument("--first-seed", type=int, default=1000)
+    c.add_argument("--roots", type=int, default=2)
+    c.add_argument("--out", required=True)
+    f = sub.add_parser("fit")
+    f.add_argument("--data", required=True)
+    f.add_argument("--out", default=DEFAULT_WEIGHTS)
+    f.add_argument("--steps", type=int, default=3000)
+    f.add_argument("--lr", type=float, default=3e-3)
+    e = sub.add_parser("evaluate")
+    e.add_argument("--data", required=True)
+    e.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    a = ap.parse_args(argv)
+    gz.init(0)
+    if a.cmd == "collect":
+        data = collect(training_graphs(a.graphs, a.first_seed), a.roots)
+        np.savez(a.out, **data)
+        print(json.dumps({"samples": int(data["features"].shape[0]), "runs": int(data["run_lengths"].shape[0])}))
+    elif a.cmd == "fit":
+        data = dict(np.load(a.data))
+        params, mean, std, hist = fit(data, a.steps, a.lr)
+        save_weights(a.out, params, mean, std)
+        print(json.dumps({"samples": int(data["features"].shape[0]), "loss_first": float(hist[0]),
+                          "loss_last": float(hist[-1]), **policy_times(data, params, mean, std)}))
+    else:
+        data = dict(np.load(a.data))
+        params, mean, std = load_weights(a.weights)
+        print(json.dumps(policy_times(data, params, mean, std)))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

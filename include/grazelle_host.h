@@ -120,7 +120,10 @@ enum gz_engine_mode {
     GZ_ENGINE_HYBRID = 0,       /* fixed threshold (execution_cc.c:124-142) */
     GZ_ENGINE_PULL = 1,         /* EXPERIMENT_EDGE_FORCE_PULL */
     GZ_ENGINE_PUSH = 2,         /* EXPERIMENT_EDGE_FORCE_PUSH */
-    GZ_ENGINE_LEARNED = 3       /* MLP selector (gz_selector_*) */
+    GZ_ENGINE_LEARNED = 3,      /* MLP selector (gz_selector_*), fixed threshold when no weights */
+    GZ_ENGINE_PROBE = 4         /* every iteration: time both engines from the same state (best of
+                                 * 3 runs each), then continue with the fixed-threshold choice;
+                                 * records carry both times and the selector features */
 };
 
 typedef struct gz_run_options_t {
@@ -148,6 +151,9 @@ typedef struct gz_iteration_record_t {
     uint64_t vectors;           /* vectors examined (collect_stats) */
     uint64_t edges;             /* valid lanes examined (collect_stats) */
     double edge_ms;             /* edge phase time (profile) */
+    float features[8];          /* learned / probe engines: selector features (before normalisation) */
+    float predicted[2];         /* learned engine: predicted log ms of pull, push */
+    double pull_ms, push_ms;    /* probe engine: edge-phase time of each engine */
 } gz_iteration_record_t;
 
 typedef struct gz_run_result_t {

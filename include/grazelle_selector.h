@@ -4,10 +4,19 @@
  * No counterpart in the reference, which selects the engine with a fixed
  * threshold (execution_cc.c:124-142, execution_bfs.c:150-168).  The selector
  * is an 8->32->32->2 FP32 MLP over per-iteration frontier statistics; its
- * two outputs predict log edge-phase time of the pull and push engines and
- * the faster one is chosen.  Both engines compute identical results, so the
- * choice changes time only.  When no weights are loaded the drivers fall back
- * to the fixed threshold.
+ * two outputs predict the log edge-phase time (ms) of the pull and push
+ * engines and the faster one is chosen.  Both engines compute identical
+ * results, so the choice changes time only.  When no weights are loaded the
+ * drivers fall back to the fixed threshold.
+ *
+ * Features of iteration i (computed on the device after iteration i-1's
+ * edge phase, from the bitmap of the vertices it activated):
+ *   0 log(1 + frontier vertices)      1 log(1 + frontier out-edges)
+ *   2 log(1 + unvisited vertices)     3 log(1 + unvisited in-edges)
+ *   4 iteration index (1-based)       5 previous engine (0 none, 1 pull, 2 push)
+ *   6 log(1 + previous edge-phase ms) 7 average degree E / V
+ * "Unvisited" counts vertices never in a frontier (BFS); for CC, which
+ * revisits vertices, it is V and E.  The network sees (x - mean) / std.
  */
 #ifndef GRAZELLE_SELECTOR_H
 #define GRAZELLE_SELECTOR_H
@@ -24,33 +33,23 @@ extern "C" {
 /* W1[32][8] b1[32] W2[32][32] b2[32] W3[2][32] b3[2] */
 #define GZ_SEL_PARAMS (GZ_SEL_H * GZ_SEL_IN + GZ_SEL_H + GZ_SEL_H * GZ_SEL_H + GZ_SEL_H + GZ_SEL_OUT * GZ_SEL_H + GZ_SEL_OUT)
 
-typedef struct gz_selector_state_t {
-    uint64_t num_vertices, num_edges;
-    uint64_t visited;          /* vertices that have had information (BFS: reached) */
-    uint64_t last_features_valid;
-    float features[GZ_SEL_IN];
-} gz_selector_state_t;
-
-/* Feature vector of one iteration (all log1p-scaled except the flags):
- *   0 log frontier vertices       1 log frontier out-edges
- *   2 log unvisited vertices      3 log unvisited in-edges (estimated E*|U|/V)
- *   4 iteration index             5 previous engine (0 none, 1 pull, 2 push)
- *   6 log previous edge-phase ms  7 graph average degree */
-void gz_selector_features(float* out, uint64_t frontier_vertices, uint64_t frontier_edges, uint64_t unvisited,
-                          uint64_t num_vertices, uint64_t num_edges, uint64_t iteration, uint32_t prev_engine,
-                          double prev_ms);
-
-void gz_selector_begin(gz_selector_state_t* s, uint64_t num_vertices, uint64_t num_edges);
-void gz_selector_observe(gz_selector_state_t* s, uint64_t vote_before, uint64_t vote_after);
 int gz_selector_loaded(void);
-int gz_selector_choose_pull(gz_selector_state_t* s, uint64_t iteration, uint64_t vote, uint32_t prev_engine,
-                            double prev_ms);
 
 /* Weights: GZ_SEL_PARAMS floats plus feature normalisation (mean[8], std[8]). */
 int gz_selector_set_weights(const float* params, const float* feat_mean, const float* feat_std);
 void gz_selector_clear(void);
+/* File: params, mean, std as little-endian FP32, nothing else. */
 int gz_selector_load_file(const char* path);
 int gz_selector_save_file(const char* path);
+
+/* Device kernels (hand-written, selector_kernels.hip), host arrays in/out. */
+int gz_selector_forward_batch(const float* params, const float* X, float* Y, uint64_t n);
+/* Full-batch gradient of the mean squared error over n samples and both
+ * outputs; returns the loss. */
+float gz_selector_gradient(const float* params, const float* X, const float* T, uint64_t n, float* grad);
+/* Full-batch Adam, `steps` steps on the device; params updated in place. */
+int gz_selector_train(float* params, const float* X, const float* T, uint64_t n, uint32_t steps, float lr,
+                      float* loss_history);
 
 #ifdef __cplusplus
 }

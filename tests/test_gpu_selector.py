@@ -1,0 +1,131 @@
+"""Learned engine selector on the device.
+
+The MLP kernels (forward, full-batch gradient, Adam) are checked against the
+oracle's double-precision MLP (itself pinned by finite differences in
+test_oracle.py).  The device feature pass is checked against what BFS depths
+fix exactly (frontier sizes per level, unvisited counts).  The probe engine
+must leave results and the engine sequence of the fixed-threshold hybrid
+unchanged, and the learned engine must keep depths and labels exact.  The
+last test measures what the committed network gains over the fixed
+threshold on held-out graphs.
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "grazelle-ppopp18_amd",
+                       "selector_bfs.bin")
+
+
+def _params(seed):
+    import selector_train as st
+    p = st.init_params(seed)
+    rng = np.random.default_rng(seed)
+    p += rng.normal(0, 0.05, size=p.shape).astype(np.float32)  # non-zero biases
+    return p
+
+
+def test_forward_matches_oracle(gz, orc, device):
+    p = _params(3)
+    X = np.random.default_rng(4).normal(size=(300, 8)).astype(np.float32)
+    Y = gz.selector_forward(p, X)
+    ref = np.stack([orc.mlp_forward(p, x) for x in X])
+    assert np.max(np.abs(Y - ref)) <= 1e-4 * max(1.0, float(np.max(np.abs(ref))))
+
+
+def test_gradient_matches_oracle(gz, orc, device):
+    p = _params(5)
+    rng = np.random.default_rng(6)
+    X = rng.normal(size=(1000, 8)).astype(np.float32)
+    T = rng.normal(size=(1000, 2)).astype(np.float32)
+    loss, g = gz.selector_gradient(p, X, T)
+    rloss, rg = orc.mlp_gradient(p, X, T)
+    assert abs(loss - rloss) <= 1e-4 * max(1.0, abs(rloss))
+    # float atomics in a different order: compare against the gradient scale
+    assert np.max(np.abs(g - rg)) <= 1e-4 * max(1e-3, float(np.max(np.abs(rg))))
+    assert np.dot(g, rg) / (np.linalg.norm(g) * np.linalg.norm(rg)) > 0.99999
+
+
+def test_training_fits_a_known_function(gz, device):
+    rng = np.random.default_rng(8)
+    X = rng.normal(size=(2000, 8)).astype(np.float32)
+    T = np.stack([X[:, 0] - 0.5 * X[:, 1], np.maximum(X[:, 2], 0) + 0.1 * X[:, 3]], axis=1).astype(np.float32)
+    import selector_train as st
+    p, hist = gz.selector_train(st.init_params(1), X, T, 1500, 3e-3)
+    assert hist[-1] < 0.05 * hist[0]
+    Y = gz.selector_forward(p, X)
+    assert float(np.mean((Y - T) ** 2)) == pytest.approx(float(hist[-1]), rel=0.2, abs=1e-3)
+
+
+@pytest.fixture(scope="module")
+def rmat13(gz):
+    return gz.rmat_graph(13, 16, seed=17)
+
+
+def test_probe_features_and_sequence(gz, orc, device, rmat13):
+    g = rmat13
+    gz.load(g)
+    G = orc.Graph(g.num_vertices, g.pull, g.push)
+    root = 3
+    gz.run(gz.options("bfs", engine="hybrid", root=root, profile=True))
+    hybrid = [r.engine for r in gz.records()]
+    gz.run(gz.options("bfs", engine="probe", root=root))
+    recs = gz.records()
+    p, d = gz.bfs_result()
+    _, dep, _, _ = orc.bfs(G, orc.options(root=root))
+    assert np.array_equal(d, dep)
+    assert [r.engine for r in recs] == hybrid
+    assert all(r.pull_ms > 0 and r.push_ms > 0 for r in recs)
+    # features pinned by the depths: level i's frontier is depth i-1
+    V, E = g.num_vertices, g.num_edges
+    od = np.bincount(g.pull[:, 0].astype(np.int64), minlength=V)
+    ind = np.bincount(g.pull[:, 1].astype(np.int64), minlength=V)
+    reached = dep != np.iinfo(np.uint64).max
+    for i, r in enumerate(recs):
+        f = np.array(r.features, dtype=np.float64)
+        level = reached & (dep == i)
+        seen = reached & (dep <= i)
+        exp = [np.log1p(level.sum()), np.log1p(od[level].sum()), np.log1p(V - seen.sum()),
+               np.log1p(E - ind[seen].sum()), i + 1, 0 if i == 0 else hybrid[i - 1]]
+        assert np.allclose(f[:6], exp, rtol=1e-5, atol=1e-5), (i, f, exp)
+        assert f[7] == pytest.approx(E / V, rel=1e-6)
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="selector weights not trained yet")
+def test_learned_engine_keeps_results_exact(gz, orc, device, rmat13):
+    g = rmat13
+    gz.selector_load(WEIGHTS)
+    try:
+        gz.load(g)
+        G = orc.Graph(g.num_vertices, g.pull, g.push)
+        for root in (0, 3, 99):
+            gz.run(gz.options("bfs", engine="learned", root=root, profile=True))
+            p, d = gz.bfs_result()
+            _, dep, _, _ = orc.bfs(G, orc.options(root=root))
+            assert np.array_equal(d, dep)
+            assert all(r.engine in (1, 2) for r in gz.records())
+        sym = gz.make_graph(gz.rmat_edges(12, 1 << 15, 5), 1 << 12, symmetrize=True)
+        gz.load(sym)
+        gz.run(gz.options("cc", engine="learned"))
+        labels, _, _ = orc.cc(orc.Graph(sym.num_vertices, sym.pull, sym.push), orc.options())
+        assert np.array_equal(gz.cc_labels(), labels)
+    finally:
+        gz.selector_clear()
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="selector weights not trained yet")
+def test_learned_gain_on_held_out_graphs(gz, device):
+    """Probe both engines on graphs the network never saw and compare the
+    edge-phase total of the fixed threshold with the network's choices (both
+    read off the same measured per-level times)."""
+    import selector_train as st
+    params, mean, std = st.load_weights(WEIGHTS)
+    data = st.collect(st.training_graphs(10, 90000), 2, log=lambda s: None)
+    pt = st.policy_times(data, params, mean, std)
+    print(f"held-out: fixed {pt['fixed']:.3f} ms, learned {pt['learned']:.3f} ms, per-level best {pt['best']:.3f} ms,"
+          f" gain {pt['fixed'] / pt['learned']:.3f}x of at most {pt['fixed'] / pt['best']:.3f}x")
+    assert pt["learned"] <= pt["fixed"]
+    assert pt["learned"] <= 1.03 * pt["best"]
