@@ -42,9 +42,11 @@ __device__ __forceinline__ uint64_t gz_splitmix64_dev(uint64_t x)
 }
 
 /* The R-MAT draw of rmat.c for edges [0, n): key (dst << b | src), and with
- * `both` also (src << b | dst) at the odd slot. */
+ * `both` also (src << b | dst) at the odd slot.  Ids are shifted by id_base
+ * (one graph of a disjoint union). */
 __global__ void gz_k_rmat_keys(uint64_t seed, uint64_t edge_offset, uint32_t scale, uint64_t nv, uint32_t ta,
-                               uint32_t tb, uint32_t tc, uint64_t n, uint32_t b, int both, uint64_t* __restrict__ keys)
+                               uint32_t tb, uint32_t tc, uint64_t n, uint32_t b, int both, uint64_t id_base,
+                               uint64_t* __restrict__ keys)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
@@ -60,6 +62,8 @@ __global__ void gz_k_rmat_keys(uint64_t seed, uint64_t edge_offset, uint32_t sca
         }
         if (s >= nv) s %= nv;
         if (d >= nv) d %= nv;
+        s += id_base;
+        d += id_base;
         if (both) {
             keys[2 * e] = (d << b) | s;
             keys[2 * e + 1] = (s << b) | d;
@@ -416,7 +420,7 @@ extern "C" int gz_graph_build_rmat_device(const gz_rmat_params_t* p, uint64_t nu
         const uint32_t tb = (uint32_t)((p->a + p->b) * 65536.0);
         const uint32_t tc = (uint32_t)((p->a + p->b + p->c) * 65536.0);
         hipLaunchKernelGGL(gz_k_rmat_keys, dim3(gz_grid_for(p->num_edges)), dim3(GZ_BLOCK), 0, gz_stream, p->seed,
-                           p->edge_offset, p->scale, rmat_nv, ta, tb, tc, p->num_edges, b, symmetrize, a);
+                           p->edge_offset, p->scale, rmat_nv, ta, tb, tc, p->num_edges, b, symmetrize, (uint64_t)0, a);
         GZ_LAUNCH_CHECK();
     }
     if (extra_edges) {
@@ -430,6 +434,44 @@ extern "C" int gz_graph_build_rmat_device(const gz_rmat_params_t* p, uint64_t nu
     }
     GZ_CHECK(hipStreamSynchronize(gz_stream));
     gz_ingress_from_keys(nv, a, bb, n, b, flags);
+    gz_device_free(a);
+    gz_device_free(bb);
+    return 0;
+}
+
+extern "C" int gz_graph_build_rmat_union_device(const gz_rmat_params_t* params, uint32_t count, int symmetrize,
+                                                uint64_t* vertex_bases)
+{
+    gz_runtime_init(0);
+    gz_graph_release();
+    uint64_t nv = 0, n = 0;
+    const uint64_t per = symmetrize ? 2 : 1;
+    for (uint32_t g = 0; g < count; ++g) {
+        vertex_bases[g] = nv;
+        nv += params[g].num_vertices ? params[g].num_vertices : (1ull << params[g].scale);
+        n += per * params[g].num_edges;
+    }
+    vertex_bases[count] = nv;
+    gz_check_vertex_count(nv);
+    const uint32_t b = gz_id_bits(nv);
+    uint64_t* a = (uint64_t*)gz_device_malloc((n ? n : 1) * sizeof(uint64_t));
+    uint64_t* bb = (uint64_t*)gz_device_malloc((n ? n : 1) * sizeof(uint64_t));
+    uint64_t at = 0;
+    for (uint32_t g = 0; g < count; ++g) {
+        const gz_rmat_params_t* p = &params[g];
+        if (p->num_edges == 0) continue;
+        const uint32_t ta = (uint32_t)(p->a * 65536.0);
+        const uint32_t tb = (uint32_t)((p->a + p->b) * 65536.0);
+        const uint32_t tc = (uint32_t)((p->a + p->b + p->c) * 65536.0);
+        const uint64_t gnv = vertex_bases[g + 1] - vertex_bases[g];
+        hipLaunchKernelGGL(gz_k_rmat_keys, dim3(gz_grid_for(p->num_edges)), dim3(GZ_BLOCK), 0, gz_stream, p->seed,
+                           p->edge_offset, p->scale, gnv, ta, tb, tc, p->num_edges, b, symmetrize, vertex_bases[g],
+                           a + at);
+        GZ_LAUNCH_CHECK();
+        at += per * p->num_edges;
+    }
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    gz_ingress_from_keys(nv, a, bb, n, b, 0);
     gz_device_free(a);
     gz_device_free(bb);
     return 0;

@@ -17,6 +17,7 @@
  * of all gathered values; each wavefront writes its share to the partials.
  * No floating-point atomics, so results are reproducible run to run.
  */
+#include "gz_pr_ops.h"
 #include "gz_walk.h"
 
 /* Set by the edge phases: the scheduler-aware pull overwrites every
@@ -24,49 +25,6 @@
  * atomics, so the following vertex phase must zero them
  * (phase_vertex_pr.asm:101-113). */
 static uint32_t gz_pr_acc_needs_zero = 0;
-
-/* Sum of the gathered properties of one vector, in the lane order of the
- * asm's vhaddpd / vextractf128 / vaddpd sequence: (g2+g3)+(g0+g1). */
-__device__ __forceinline__ double gz_pr_vector_sum(const gz_lanes& v, const double* __restrict__ props, uint32_t& edges,
-                                                   bool in_range = true)
-{
-    /* branch-free: invalid lanes read element 0 and are zeroed, so the four
-     * loads issue back to back */
-    const bool v0 = in_range && gz_lane_valid(v.l0), v1 = in_range && gz_lane_valid(v.l1),
-               v2 = in_range && gz_lane_valid(v.l2), v3 = in_range && gz_lane_valid(v.l3);
-    double g0 = props[v0 ? gz_lane_id(v.l0) : 0];
-    double g1 = props[v1 ? gz_lane_id(v.l1) : 0];
-    double g2 = props[v2 ? gz_lane_id(v.l2) : 0];
-    double g3 = props[v3 ? gz_lane_id(v.l3) : 0];
-    g0 = v0 ? g0 : 0.0;
-    g1 = v1 ? g1 : 0.0;
-    g2 = v2 ? g2 : 0.0;
-    g3 = v3 ? g3 : 0.0;
-    edges += (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2 + (uint32_t)v3;
-    return (g2 + g3) + (g0 + g1);
-}
-
-struct gz_pr_pull_op {
-    const double* __restrict__ props;
-    double* __restrict__ acc;
-    bool sched_aware;
-    double sum;
-    __device__ __forceinline__ double identity() const { return 0.0; }
-    __device__ __forceinline__ double combine(double a, double b) const { return a + b; }
-    __device__ __forceinline__ double gather(const gz_lanes& v, bool in_range, uint32_t& edges)
-    {
-        const double g = gz_pr_vector_sum(v, props, edges, in_range);
-        sum += g;
-        return g;
-    }
-    __device__ __forceinline__ void store(uint64_t d, double s)
-    {
-        if (sched_aware)
-            acc[d] = s;
-        else
-            unsafeAtomicAdd(&acc[d], s);
-    }
-};
 
 template <bool kSchedAware, int VPL>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr(const gz_vec_t* __restrict__ vecs, uint64_t nvec,

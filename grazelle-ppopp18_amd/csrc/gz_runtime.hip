@@ -240,6 +240,8 @@ static void gz_kt_flush(void)
     gz_kt_used = 0;
 }
 
+extern "C" int gz_kernel_timing_enabled(void) { return gz_kt_on; }
+
 extern "C" void gz_kernel_timing_enable(int on)
 {
     gz_require_runtime();

@@ -512,3 +512,84 @@ def selector_save(path: str) -> None:
 
 def selector_loaded() -> bool:
     return bool(lib.gz_selector_loaded())
+
+
+# ---- batches of independent graphs ------------------------------------------
+
+BATCH_BUCKETS = 4
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("vectors_per_lane", u32), ("workgroup_size", u32), ("chunk", u32 * BATCH_BUCKETS)]
+
+    def as_dict(self) -> dict:
+        return {"vectors_per_lane": self.vectors_per_lane, "workgroup_size": self.workgroup_size,
+                "chunk": list(self.chunk)}
+
+
+_sig("gz_graph_build_rmat_union_device", C.c_int, P(RmatParams), u32, C.c_int, vp)
+_sig("gz_batch_attach", C.c_int, vp, u32)
+_sig("gz_batch_release", None)
+_sig("gz_batch_bucket_of_graph", u32, u32)
+_sig("gz_batch_set_config", C.c_int, P(BatchConfig))
+_sig("gz_batch_get_config", None, P(BatchConfig))
+_sig("gz_batch_pr_prepare", None, C.c_double)
+_sig("gz_batch_pr_run", C.c_double, u32, C.c_int)
+_sig("gz_batch_time_pull", C.c_double, P(BatchConfig), u32, u32)
+_sig("gz_batch_pr_ranks", None, vp)
+
+
+def batch_specs(count: int = 1024, seed: int = 4, smin: int = 10, smax: int = 17, efmin: int = 4, efmax: int = 32):
+    """(scale, edge factor, seed) of a batch: scale s in [smin, smax] drawn
+    with P(s) proportional to 1/s ("log-uniform" over the integers), edge
+    factor uniform over the integers [efmin, efmax], graph seeds seed*100000+i."""
+    rng = np.random.default_rng(seed)
+    scales = np.arange(smin, smax + 1)
+    p = 1.0 / scales
+    p /= p.sum()
+    s = rng.choice(scales, size=count, p=p)
+    ef = rng.integers(efmin, efmax + 1, size=count)
+    return [(int(a), int(b), seed * 100000 + i) for i, (a, b) in enumerate(zip(s, ef))]
+
+
+def load_batch_device(specs, symmetrize: bool = False) -> np.ndarray:
+    """Device ingress of the disjoint union of R-MAT graphs (scale, ef, seed);
+    returns the vertex bases (len(specs)+1) and attaches the batch."""
+    init()
+    params = (RmatParams * len(specs))()
+    for i, (scale, ef, seed) in enumerate(specs):
+        params[i] = RmatParams(scale, 0, (1 << scale) * ef, 0.57, 0.19, 0.19, seed, 0)
+    bases = np.zeros(len(specs) + 1, dtype=np.uint64)
+    lib.gz_graph_build_rmat_union_device(params, len(specs), int(symmetrize), _ptr(bases))
+    if lib.gz_batch_attach(_ptr(bases), len(specs)) != 0:
+        raise RuntimeError("batch attach failed")
+    return bases
+
+
+def batch_set_config(vectors_per_lane=1, workgroup_size=256, chunk=(512, 512, 512, 512)) -> None:
+    c = BatchConfig(vectors_per_lane, workgroup_size, (u32 * BATCH_BUCKETS)(*chunk))
+    if lib.gz_batch_set_config(C.byref(c)) != 0:
+        raise ValueError(f"invalid batch configuration {c.as_dict()}")
+
+
+def batch_get_config() -> dict:
+    c = BatchConfig()
+    lib.gz_batch_get_config(C.byref(c))
+    return c.as_dict()
+
+
+def batch_pr(iterations: int = 10, damping: float = 0.875, use_graph: bool = True) -> float:
+    """Batched PageRank from the initial ranks; returns the device ms of the iterations."""
+    lib.gz_batch_pr_prepare(damping)
+    return float(lib.gz_batch_pr_run(iterations, int(use_graph)))
+
+
+def batch_pr_ranks() -> np.ndarray:
+    out = np.empty(num_vertices(), dtype=np.float64)
+    lib.gz_batch_pr_ranks(_ptr(out))
+    return out
+
+
+def batch_time_pull(vectors_per_lane: int, workgroup_size: int, chunk, bucket: int, reps: int = 3) -> float:
+    c = BatchConfig(vectors_per_lane, workgroup_size, (u32 * BATCH_BUCKETS)(*chunk))
+    return float(lib.gz_batch_time_pull(C.byref(c), bucket, reps))

@@ -101,6 +101,39 @@ int gz_graph_build_rmat_device(const gz_rmat_params_t* p, uint64_t num_vertices,
                                uint64_t extra_edges, const uint64_t* extra_pairs, uint32_t flags);
 /* Device id of an original vertex id (identity unless renumbered). */
 uint64_t gz_graph_device_vertex(uint64_t original);
+/* Disjoint union of `count` R-MAT graphs (params[g] as for
+ * gz_graph_build_rmat_device); graph g gets the ids [bases[g], bases[g+1]),
+ * bases has count+1 entries and is filled here. */
+int gz_graph_build_rmat_union_device(const gz_rmat_params_t* params, uint32_t count, int symmetrize,
+                                     uint64_t* vertex_bases);
+
+/* -------- batches of independent graphs (no reference counterpart) -------
+ * The loaded graph is a disjoint union; gz_batch_attach names the graphs.
+ * CC and BFS need nothing more (the ordinary drivers work on the union).
+ * PageRank keeps V and the sink mass per graph: gz_batch_pr_prepare
+ * initialises it, gz_batch_pr_run runs the iterations (captured into a HIP
+ * graph when use_graph) and returns their device time in ms. */
+#define GZ_BATCH_BUCKETS 4
+typedef struct gz_batch_config_t {
+    uint32_t vectors_per_lane;          /* 1, 2 or 4 */
+    uint32_t workgroup_size;            /* 64..1024, a multiple of 64 */
+    uint32_t chunk[GZ_BATCH_BUCKETS];   /* pull chunk length (vectors) per size bucket,
+                                         * a multiple of 64 x vectors_per_lane; bucket
+                                         * b holds graphs of 2^(10+2b) .. 2^(12+2b) vertices */
+} gz_batch_config_t;
+int gz_batch_attach(const uint64_t* vertex_bases, uint32_t count);
+void gz_batch_release(void);
+uint32_t gz_batch_bucket_of_graph(uint32_t graph);
+int gz_batch_set_config(const gz_batch_config_t* cfg);
+void gz_batch_get_config(gz_batch_config_t* cfg);
+void gz_batch_pr_prepare(double damping);
+double gz_batch_pr_run(uint32_t iterations, int use_graph);
+/* Best-of-reps device time (ms) of the batched pull sweep over one bucket's
+ * graphs under cfg (the autotuner's probe); -1 for an invalid cfg. */
+double gz_batch_time_pull(const gz_batch_config_t* cfg, uint32_t bucket, uint32_t reps);
+/* PageRank ranks of the whole union (rank = props * outdeg', outdeg' = V of
+ * the vertex's graph for sinks), V entries. */
+void gz_batch_pr_ranks(double* out);
 void gz_graph_release(void);
 
 /* Host mirrors filled at load time (for output, statistics and tests). */

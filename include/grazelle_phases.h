@@ -171,6 +171,7 @@ enum gz_kernel_id {
     GZ_K_PULL_BFS_FIND, GZ_K_PULL_BFS_VISIT, GZ_K_PUSH_BFS, GZ_K_COMPACT, GZ_K_REDUCE, GZ_K_COUNT
 };
 void gz_kernel_timing_enable(int on);
+int gz_kernel_timing_enabled(void);
 void gz_kernel_timing_reset(void);
 /* Synchronises; total milliseconds and launches recorded for `id`. */
 void gz_kernel_timing_read(int id, double* total_ms, uint64_t* launches);
