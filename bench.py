@@ -1,24 +1,32 @@
 #!/usr/bin/env python3
 """Benchmark of the hybrid push/pull engine on one MI355X.
 
-Default workload (the connected-components configuration): symmetrised R-MAT
-scale 22, edge factor 16, a/b/c/d = 0.57/0.19/0.19/0.05, plus 64 planted
-1000-vertex random trees, uint64 labels, hybrid push/pull with the reference's
-fixed E/5 threshold.  One step = one complete CC run from initial labels to
-convergence (label initialisation + every edge and vertex phase, with the
-per-iteration vote read-back that drives the engine switch).  Inputs are
-resident in HBM before timing starts.
+Headline (value): edge-pull PageRank on the full-scale power-law graph --
+R-MAT scale 26 (a/b/c/d = 0.57/0.19/0.19/0.05) folded by modulo into 41.7M
+vertices, 1.494B raw edges, 1.47B after self-loop and duplicate removal,
+built in HBM by device ingress with vertices renumbered by out-degree (results
+are reported in original ids).  FP64 ranks, damping 0.85.  One step = one
+PageRank iteration (pull sweep, merge, sink-mass reduction, vertex phase);
+value = E / (time per iteration), GTEPS per iteration (the reference's rate,
+main.c:127-129, per iteration).  Inputs are resident in HBM before timing.
 
-value = GTEPS, E * iterations / time (the reference's "Effective Processing
-Rate", main.c:127-129), i.e. the mean per-iteration GTEPS of the run.
+roofline: the pull sweep kernel's algorithmic bytes per launch, 32*Nvp
+(vectors) + 8*E (one FP64 gather per edge) + 8*V (one accumulator store per
+vertex), over its average launch time measured live with HIP events on the
+library's stream.  traffic: HBM bytes per launch from the committed rocprofv3
+counter summary (profiles/).
 
-The same graph also gets 20 PageRank iterations (scheduler-aware pull, FP64)
-reported under "pagerank_pull", with its own roofline figure.
-
-roofline: algorithmic bytes of the dominant kernel's launch divided by its
-average launch time, measured live with HIP events on the library's stream.
-cpu_baseline: the CPU oracle (a pthreads restatement of the reference
-engine) on the same graph on the host cores.
+Sub-objects, each one configuration of the workload set:
+  cc_rmat22   CC, symmetrised R-MAT s22 + 64 planted 1000-vertex trees,
+              hybrid with the fixed E/5 threshold, time to solution
+  bfs_rmat24  BFS from 64 seeded roots on R-MAT s24: fixed threshold versus
+              the learned selector (selector_bfs.bin), depths checked equal
+  batch_1024  1024 R-MAT graphs (scale 10..17 with P(s) ~ 1/s, edge factor
+              4..32): batched PageRank (10 iterations, captured HIP graph)
+              and CC versus the same graphs run one after another
+  pr_rmat16   PageRank 20 iterations on R-MAT s16 (the reference's own size)
+cpu_baseline: the CPU oracle (pthreads restatement of the reference engine)
+on a bounded sample of the headline workload.
 """
 from __future__ import annotations
 
@@ -30,11 +38,15 @@ import statistics
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "grazelle-ppopp18_amd"))
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "GTEPS/iter + % HBM peak, 1 MI355X; learned-vs-fixed switch & batched-vs-seq"
+FULL = "full-rmat26-fold41.7m"
+DAMPING = 0.85
 
 
 def log(*a):
@@ -50,40 +62,185 @@ def read_traffic(kernel: str, workload: str):
     return int(m.group(1)) if m else None
 
 
-def cpu_baseline_cc(g, iters_gpu, threads, runs):
+def pull_bytes(gz) -> int:
+    return 32 * gz.pull_vector_count() + 8 * gz.num_edges() + 8 * gz.num_vertices()
+
+
+def headline(gz, args) -> dict:
+    t0 = time.time()
+    gz.load_workload_device(FULL, relabel=not args.no_relabel)
+    gz.lib.gz_synchronize()
+    V, E, nvp = gz.num_vertices(), gz.num_edges(), gz.pull_vector_count()
+    log(f"[headline] {FULL}: V={V} E={E} Nvp={nvp} ingress {time.time() - t0:.2f}s")
+    if args.warmup:
+        gz.run(gz.options("pr", iterations=args.warmup, damping=DAMPING))
+    gz.lib.gz_synchronize()
+    w0 = time.perf_counter()
+    r = gz.run(gz.options("pr", iterations=args.steps, damping=DAMPING))
+    gz.lib.gz_synchronize()
+    wall_ms = (time.perf_counter() - w0) * 1e3
+    ranks = gz.pr_ranks()
+    # per-kernel times from a second pass with HIP events around every launch
+    gz.kernel_timing(True)
+    gz.run(gz.options("pr", iterations=args.steps, damping=DAMPING))
+    kt = gz.kernel_times()
+    gz.kernel_timing(False)
+    ms_step = r.elapsed_ms / args.steps
+    pms, pn = kt["pull_pr"]
+    avg = pms / pn
+    B = pull_bytes(gz)
+    achieved = B / (avg * 1e-3) / 1e9
+    log(f"[headline] {args.steps} iterations {r.elapsed_ms:.2f} ms ({ms_step:.3f} ms/it), pull {avg:.3f} ms "
+        f"{achieved:.0f} GB/s, rank sum {ranks.sum():.12f}")
+    return {
+        "V": V, "E": E, "nvp": nvp, "ms_step": ms_step, "wall_ms": wall_ms,
+        "value": E / (ms_step * 1e-3) / 1e9,
+        "phases_ms_per_iteration": {k: round(ms / n, 4) for k, (ms, n) in kt.items() if n},
+        "roofline": {"bound": "hbm", "kernel": "gz_k_edge_pull_pr", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": read_traffic("gz_k_edge_pull_pr", FULL), "bytes_per_launch": B,
+                     "avg_launch_us": round(avg * 1e3, 2), "launches": pn,
+                     "kernel_gteps": round(E / (avg * 1e-3) / 1e9, 2)},
+        "rank_sum": float(ranks.sum()),
+    }
+
+
+def cc_rmat22(gz) -> dict:
+    gz.load_workload_device("cc-rmat22-sym-trees")
+    V, E = gz.num_vertices(), gz.num_edges()
+    gz.run(gz.options("cc"))
+    times, res = [], None
+    gz.kernel_timing(True)
+    for _ in range(5):
+        res = gz.run(gz.options("cc"))
+        times.append(res.elapsed_ms)
+    kt = gz.kernel_times()
+    gz.kernel_timing(False)
+    ms = statistics.median(times)
+    pms, pn = kt["pull_cc"]
+    out = {"workload": "cc-rmat22-sym-trees", "V": V, "E": E, "time_to_solution_ms": round(ms, 3),
+           "iterations": res.iterations, "pull_iterations": res.pull_iterations,
+           "push_iterations": res.push_iterations, "gteps": round(E * res.iterations / (ms * 1e-3) / 1e9, 2)}
+    if pn:
+        avg = pms / pn
+        B = pull_bytes(gz)
+        out["pull_cc_kernel"] = {"avg_launch_us": round(avg * 1e3, 2), "bytes_per_launch": B,
+                                 "achieved_GBs": round(B / (avg * 1e-3) / 1e9, 1),
+                                 "frac_hbm_peak": round(B / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                 "traffic": read_traffic("gz_k_edge_pull_cc", "cc-rmat22-sym-trees")}
+    return out
+
+
+def bfs_rmat24(gz) -> dict:
+    import selector_train as st
+
+    gz.load_workload_device("bfs-rmat24")
+    V, E = gz.num_vertices(), gz.num_edges()
+    roots = st.pick_roots(64, 24)
+    gz.run(gz.options("bfs", root=roots[0]))
+    out = {"workload": "bfs-rmat24", "V": V, "E": E, "roots": len(roots)}
+    totals = {}
+    for engine in ("hybrid", "learned"):
+        if engine == "learned":
+            gz.selector_load(st.DEFAULT_WEIGHTS)
+            gz.run(gz.options("bfs", engine=engine, root=roots[0]))  # builds the in-degree array once
+        t = 0.0
+        for root in roots:
+            t += gz.run(gz.options("bfs", engine=engine, root=root)).elapsed_ms
+        totals[engine] = t
+    # depths agree between the engines (spot check on four roots)
+    for root in roots[:4]:
+        gz.run(gz.options("bfs", engine="learned", root=root))
+        d_l = gz.bfs_result()[1]
+        gz.selector_clear()
+        gz.run(gz.options("bfs", engine="hybrid", root=root))
+        d_f = gz.bfs_result()[1]
+        gz.selector_load(st.DEFAULT_WEIGHTS)
+        if not np.array_equal(d_l, d_f):
+            raise AssertionError(f"BFS depths differ between engines at root {root}")
+    gz.selector_clear()
+    out.update({"fixed_threshold_ms": round(totals["hybrid"], 3), "learned_ms": round(totals["learned"], 3),
+                "learned_speedup": round(totals["hybrid"] / totals["learned"], 4),
+                "fixed_ms_per_search": round(totals["hybrid"] / len(roots), 4),
+                "fixed_gteps_per_search": round(E / (totals["hybrid"] / len(roots) * 1e-3) / 1e9, 2)})
+    return out
+
+
+def batch_1024(gz) -> dict:
+    import batch_tune
+
+    specs = gz.batch_specs()
+    gz.load_batch_device(specs)
+    E = gz.num_edges()
+    tune = batch_tune.tune(reps=2)
+    gz.batch_pr(10, damping=DAMPING, use_graph=True)
+    pr_b = min(gz.batch_pr(10, damping=DAMPING, use_graph=True) for _ in range(3))
+    gz.run(gz.options("cc"))
+    cc_b = min(gz.run(gz.options("cc")).elapsed_ms for _ in range(2))
+    pr_s = cc_s = 0.0
+    for s, ef, seed in specs:
+        gz.load_rmat_device(s, (1 << s) * ef, seed)
+        pr_s += gz.run(gz.options("pr", iterations=10, damping=DAMPING)).elapsed_ms
+        cc_s += gz.run(gz.options("cc")).elapsed_ms
+    return {"graphs": len(specs), "E_total": E, "tuned_config": tune["config"],
+            "pagerank_10it": {"batched_ms": round(pr_b, 3), "sequential_ms": round(pr_s, 3),
+                              "speedup": round(pr_s / pr_b, 2),
+                              "batched_gteps_per_iteration": round(E * 10 / (pr_b * 1e-3) / 1e9, 2)},
+            "cc": {"batched_ms": round(cc_b, 3), "sequential_ms": round(cc_s, 3), "speedup": round(cc_s / cc_b, 2)}}
+
+
+def pr_rmat16(gz) -> dict:
+    gz.load_workload_device("pr-rmat16")
+    gz.run(gz.options("pr", iterations=20, damping=DAMPING))
+    ms = min(gz.run(gz.options("pr", iterations=20, damping=DAMPING)).elapsed_ms for _ in range(5))
+    E = gz.num_edges()
+    return {"workload": "pr-rmat16", "V": gz.num_vertices(), "E": E, "iterations": 20, "ms": round(ms, 4),
+            "gteps_per_iteration": round(E * 20 / (ms * 1e-3) / 1e9, 2)}
+
+
+def cpu_baseline(gz, threads: int, runs: int) -> dict:
+    """The oracle's PageRank on a bounded sample of the headline workload:
+    the same generator at scale 20 (1/64 of the vertices and edges)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
+    scale = 20
+    nv = gz.FULL_VERTICES >> 6
+    raw = gz.rmat_edges(scale, gz.FULL_RAW_EDGES >> 6, 1, num_vertices=nv)
+    g = gz.make_graph(raw, nv)
     G = orc.Graph(g.num_vertices, g.pull, g.push)
-    times, iters = [], []
+    times = []
     for _ in range(runs):
-        _, _, r = orc.cc(G, orc.options(threads=threads), log_cap=0)
-        times.append(r.elapsed_ms)
-        iters.append(r.iterations)
+        _, _, r = orc.pagerank(G, orc.options(threads=threads, iterations=5, damping=DAMPING))
+        times.append(r.elapsed_ms / 5)
     ms = statistics.median(times)
-    it = iters[times.index(min(times, key=lambda x: abs(x - ms)))]
-    return {"value": round(g.num_edges * it / (ms * 1e-3) / 1e9, 3), "unit": "GTEPS", "cores": threads,
-            "kind": "port",
-            "sample": f"full workload (V={g.num_vertices}, E={g.num_edges}), CC hybrid E/5 threshold, "
-                      f"{it} iterations, median of {runs} runs of the pthreads oracle, {ms:.1f} ms per run"}
+    return {"value": round(g.num_edges / (ms * 1e-3) / 1e9, 3), "unit": "GTEPS", "cores": threads, "kind": "port",
+            "sample": f"PageRank pull, 5 iterations, the headline generator at scale 20 folded into {nv} vertices "
+                      f"({g.num_edges} edges, 1/64 of the headline graph); median of {runs} runs of the pthreads "
+                      f"oracle, {ms:.1f} ms per iteration"}
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="cc-rmat22-sym-trees")
+    ap.add_argument("--only-headline", action="store_true", help="skip the other configurations (profiling)")
+    ap.add_argument("--no-relabel", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-runs", type=int, default=5)
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--chunk", type=int, default=512)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the device copy kernel on MIB mebibytes (rocprof counter calibration)")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
+              "The per-GPU shard design is described in DESIGN.md.", file=sys.stderr)
+        return 2
+    import grazelle as gz
+
     if args.calibrate_copy:
         import ctypes
-        import grazelle as gz
 
         gz.init(0)
         nbytes = args.calibrate_copy << 20
@@ -97,106 +254,39 @@ def main() -> int:
         gz.lib.gz_synchronize()
         print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": max(1, args.steps)}), flush=True)
         return 0
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
-              "The per-GPU shard design is described in DESIGN.md.", file=sys.stderr)
-        return 2
-
-    import grazelle as gz
-
     if gz.device_count() < 1:
         print("bench.py needs a HIP device", file=sys.stderr)
         return 1
     dev = gz.init(0)
-    t0 = time.time()
-    g = gz.workload_graph(args.workload, seed=1)
-    log(f"graph {args.workload}: V={g.num_vertices} E={g.num_edges} built in {time.time() - t0:.1f}s")
-    gz.set_chunk_vectors(args.chunk)
-    gz.load(g)
-    V, E = g.num_vertices, g.num_edges
-    nvp = gz.pull_vector_count()
-    log(f"device {dev}: pull vectors {nvp}, push vectors {gz.push_vector_count()}")
-
-    opts = gz.options("cc", engine="hybrid")
-    for _ in range(args.warmup):
-        gz.run(opts)
-    gz.lib.gz_synchronize()
-    gz.kernel_timing(True)
-    results = []
-    gz.lib.gz_synchronize()
-    t_start = time.perf_counter()
-    for _ in range(args.steps):
-        results.append(gz.run(opts))
-    gz.lib.gz_synchronize()
-    t_total = time.perf_counter() - t_start
-    kt = gz.kernel_times()
-    gz.kernel_timing(False)
-    ms_per_step = t_total * 1e3 / args.steps
-    iters = results[-1].iterations
-    value = E * iters / (ms_per_step * 1e-3) / 1e9
-    log(f"CC: {ms_per_step:.3f} ms/step, {iters} iterations ({results[-1].pull_iterations} pull / "
-        f"{results[-1].push_iterations} push), {value:.1f} GTEPS")
-    for k, (ms, n) in kt.items():
-        if n:
-            log(f"  kernel {k:15s} launches {n:5d} avg {ms / n * 1e3:9.1f} us  total {ms:8.2f} ms")
-
-    # dominant kernel of the CC step
-    dom = max(("pull_cc", "push_cc"), key=lambda k: kt[k][0])
-    dom_ms, dom_n = kt[dom]
-    if dom == "pull_cc":
-        bytes_launch = 32 * nvp + 8 * E + 8 * V
-    else:
-        bytes_launch = None
-    roofline = None
-    if bytes_launch and dom_n:
-        avg_s = dom_ms / dom_n * 1e-3
-        achieved = bytes_launch / avg_s / 1e9
-        roofline = {"bound": "hbm", "kernel": "gz_k_edge_pull_cc", "achieved": round(achieved, 1),
-                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": read_traffic("gz_k_edge_pull_cc", args.workload),
-                    "bytes_per_launch": bytes_launch, "avg_launch_us": round(avg_s * 1e6, 2),
-                    "launches": dom_n}
-
-    # PageRank pull on the same graph
-    gz.run(gz.options("pr", iterations=5))
-    gz.kernel_timing(True)
-    gz.lib.gz_synchronize()
-    tp = time.perf_counter()
-    rp = gz.run(gz.options("pr", iterations=20))
-    tp = time.perf_counter() - tp
-    ktp = gz.kernel_times()
-    gz.kernel_timing(False)
-    pms, pn = ktp["pull_pr"]
-    pr_bytes = 32 * nvp + 8 * E + 8 * V
-    pr_avg = pms / pn * 1e-3
-    pr = {"iterations": 20, "ms_per_iteration": round(tp * 1e3 / 20, 4),
-          "gteps_per_iteration": round(E * 20 / tp / 1e9, 2),
-          "edge_kernel": "gz_k_edge_pull_pr", "avg_launch_us": round(pr_avg * 1e6, 2),
-          "bytes_per_launch": pr_bytes, "achieved_GBs": round(pr_bytes / pr_avg / 1e9, 1),
-          "frac_hbm_peak": round(pr_bytes / pr_avg / 1e9 / HBM_PEAK_GBS, 4),
-          "kernel_gteps": round(E / pr_avg / 1e9, 2),
-          "traffic": read_traffic("gz_k_edge_pull_pr", args.workload)}
-    log(f"PR pull: {pr}")
-
+    h = headline(gz, args)
+    extra = {}
+    if not args.only_headline:
+        for name, fn in (("cc_rmat22", cc_rmat22), ("bfs_rmat24", bfs_rmat24), ("batch_1024", batch_1024),
+                         ("pr_rmat16", pr_rmat16)):
+            t0 = time.time()
+            try:
+                extra[name] = fn(gz)
+            except Exception as exc:  # a failing configuration is reported, not hidden
+                extra[name] = {"error": f"{type(exc).__name__}: {exc}"}
+            log(f"[{name}] {time.time() - t0:.1f}s {json.dumps(extra[name])}")
     cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         t0 = time.time()
-        cpu = cpu_baseline_cc(g, iters, threads, args.cpu_runs)
-        log(f"cpu baseline ({time.time() - t0:.1f}s): {cpu}")
-
+        cpu = cpu_baseline(gz, threads, args.cpu_runs)
+        log(f"[cpu_baseline] {time.time() - t0:.1f}s {cpu}")
     line = {
-        "metric": METRIC, "value": round(value, 3), "unit": "GTEPS", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic",
-        "config": {"workload": args.workload, "algorithm": "cc", "engine": "hybrid (fixed E/5 threshold)",
-                   "num_vertices": V, "num_edges": E, "pull_vectors": nvp, "iterations": iters,
-                   "pull_iterations": results[-1].pull_iterations,
-                   "push_iterations": results[-1].push_iterations, "chunk_vectors": args.chunk,
+        "metric": METRIC, "value": round(h["value"], 3), "unit": "GTEPS/iter", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(h["ms_step"], 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": FULL + " PageRank pull (FP64, damping 0.85)", "num_vertices": h["V"],
+                   "num_edges": h["E"], "pull_vectors": h["nvp"],
+                   "ingress": "device, renumbered by out-degree" if not args.no_relabel else "device",
                    "device": dev},
-        "roofline": roofline,
+        "roofline": h["roofline"],
         "cpu_baseline": cpu,
-        "pagerank_pull": pr,
+        "phases_ms_per_iteration": h["phases_ms_per_iteration"],
+        **extra,
     }
     print(json.dumps(line), flush=True)
     gz.release()

@@ -147,6 +147,12 @@ __device__ __forceinline__ uint64_t gz_wave_sum_u64(uint64_t x)
     return x;
 }
 
+/* Lane `src`'s value of x in every lane. */
+__device__ __forceinline__ uint64_t gz_shfl_u64(uint64_t x, int src)
+{
+    return (uint64_t)__shfl((unsigned long long)x, src, GZ_WAVE);
+}
+
 /* Bitmaps are arrays of uint64 words, bit v%64 of word v/64 (LSB first).  The
  * device updates them through 32-bit atomics on the same bytes. */
 __device__ __forceinline__ bool gz_bit_test(const uint64_t* __restrict__ bm, uint64_t v)

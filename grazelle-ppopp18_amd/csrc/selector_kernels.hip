@@ -139,21 +139,21 @@ __global__ void gz_k_sel_adam(float* __restrict__ p, float* __restrict__ m, floa
 /* ---- per-iteration decision on the device ---------------------------------
  * The drivers read one block of words from the reduce buffer after every
  * edge phase (the converge vote).  The selector rides on that read: right
- * after the edge phase one kernel folds the statistics of the next frontier
- * (the vertices the phase activated) and its last block to finish builds the
- * feature vector, runs the MLP and writes the decision and the features next
- * to the vote, so choosing the engine costs one extra launch and no extra
- * host synchronisation. */
+ * after the edge phase, gz_k_sel_stats folds the statistics of the next
+ * frontier (the vertices the phase activated) into per-block partials, and
+ * gz_k_sel_decide (one block) sums them in a fixed order, builds the feature
+ * vector, runs the MLP and writes the decision and the features next to the
+ * vote.  Choosing the engine costs two small launches and no extra host
+ * synchronisation; there are no atomics, so the features are exact and the
+ * same on every run. */
+
+#define GZ_SEL_WORDS_PER_WAVE 64
 
 struct gz_sel_dev_state {
-    uint64_t frontier_vertices;  /* |frontier|, accumulated by the step kernel */
-    uint64_t frontier_edges;     /* sum of out-degrees over the frontier */
-    uint64_t frontier_in_edges;  /* sum of in-degrees over the frontier */
     uint64_t unvisited;          /* BFS: vertices never in a frontier */
     uint64_t unvisited_in_edges; /* BFS: their in-degrees */
-    uint64_t t_begin, t_end;     /* wall clock around the last edge phase */
-    uint32_t blocks_done;        /* last-block detection of the step kernel */
-    uint32_t pad;
+    uint64_t t_begin;            /* wall clock at the start of the last edge phase */
+    uint64_t pad;
 };
 
 __global__ void gz_k_sel_stamp(gz_sel_dev_state* st)
@@ -161,58 +161,81 @@ __global__ void gz_k_sel_stamp(gz_sel_dev_state* st)
     if (threadIdx.x == 0 && blockIdx.x == 0) st->t_begin = (uint64_t)wall_clock64();
 }
 
-/* Statistics of `frontier` into the state; the last block then writes
- * out[1] = decision (1 pull, 2 push, 0 no model), out[2..5] = the raw
- * features as 8 floats, out[6] = the two outputs as floats, and clears the
- * accumulators for the next call.  The unvisited counts advance by the
- * frontier (BFS marks each vertex once). */
-__global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_step(const uint64_t* __restrict__ frontier,
-                                                          const double* __restrict__ outdeg,
-                                                          const uint32_t* __restrict__ indeg, uint64_t num_vertices,
-                                                          uint64_t num_edges, uint64_t words, gz_sel_dev_state* st,
-                                                          const float* __restrict__ p, const float* __restrict__ norm,
-                                                          int have_model, int bfs, float iteration, float prev_engine,
-                                                          float prev_ms, float clock_khz, uint64_t* __restrict__ out)
+/* One wavefront per 64 bitmap words: lane l holds word l of the wave's range
+ * for the population count, then the wave walks the words one by one with
+ * lane l testing vertex 64w + l, so the degree reads of a dense frontier are
+ * coalesced.  partials[3b .. 3b+2] = (vertices, out-edges, in-edges) of
+ * block b; partials[3 * gridDim.x] = wall clock at the kernel start. */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_stats(const uint64_t* __restrict__ frontier,
+                                                           const double* __restrict__ outdeg,
+                                                           const uint32_t* __restrict__ indeg, uint64_t num_vertices,
+                                                           uint64_t words, uint64_t* __restrict__ partials)
 {
-    __shared__ float x[GZ_SEL_IN], xn[GZ_SEL_IN], h1[GZ_SEL_H], h2[GZ_SEL_H], y[GZ_SEL_OUT];
-    __shared__ int last;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && prev_ms < 0.f) st->t_end = (uint64_t)wall_clock64();
-    uint64_t fv = 0, fe = 0, fi = 0;
-    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < words;
-         w += (uint64_t)gridDim.x * blockDim.x) {
-        uint64_t bits = frontier[w];
-        const uint64_t base = w * 64;
-        if (base + 64 > num_vertices) {
-            const uint64_t keep = num_vertices - base;
-            bits &= (keep >= 64) ? ~0ull : ((1ull << keep) - 1ull);
-        }
-        fv += (uint64_t)__popcll(bits);
-        while (bits) {
-            const int b = __ffsll((unsigned long long)bits) - 1;
-            bits &= bits - 1;
-            fe += (uint64_t)outdeg[base + (uint64_t)b];
-            fi += (uint64_t)indeg[base + (uint64_t)b];
+    __shared__ uint64_t red[3][GZ_WAVES_PER_BLOCK];
+    if (blockIdx.x == 0 && threadIdx.x == 0) partials[3 * (uint64_t)gridDim.x] = (uint64_t)wall_clock64();
+    const int lane = gz_lane();
+    const int wib = threadIdx.x / GZ_WAVE;
+    const uint64_t w0 = ((uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + wib) * GZ_SEL_WORDS_PER_WAVE;
+    uint64_t mine = (w0 + lane < words) ? frontier[w0 + lane] : 0ull;
+    if (w0 + lane == words - 1 && (num_vertices & 63)) mine &= (1ull << (num_vertices & 63)) - 1ull;
+    uint64_t fv = (uint64_t)__popcll(mine), fe = 0, fi = 0;
+    uint64_t any = __ballot(mine != 0);
+    while (any) {
+        const int k = __ffsll((unsigned long long)any) - 1;
+        any &= any - 1;
+        const uint64_t word = gz_shfl_u64(mine, k);
+        if ((word >> lane) & 1ull) {
+            const uint64_t v = (w0 + (uint64_t)k) * 64 + (uint64_t)lane;
+            fe += (uint64_t)outdeg[v];
+            fi += (uint64_t)indeg[v];
         }
     }
     fv = gz_wave_sum_u64(fv);
     fe = gz_wave_sum_u64(fe);
     fi = gz_wave_sum_u64(fi);
-    if (gz_lane() == 0 && (fv | fe | fi)) {
-        atomicAdd((unsigned long long*)&st->frontier_vertices, (unsigned long long)fv);
-        atomicAdd((unsigned long long*)&st->frontier_edges, (unsigned long long)fe);
-        atomicAdd((unsigned long long*)&st->frontier_in_edges, (unsigned long long)fi);
+    if (lane == 0) {
+        red[0][wib] = fv;
+        red[1][wib] = fe;
+        red[2][wib] = fi;
     }
-    __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) last = (atomicAdd(&st->blocks_done, 1u) == gridDim.x - 1);
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
+    if (threadIdx.x < 3) {
+        uint64_t s = 0;
+        for (int i = 0; i < GZ_WAVES_PER_BLOCK; ++i) s += red[threadIdx.x][i];
+        partials[3 * (uint64_t)blockIdx.x + threadIdx.x] = s;
+    }
+}
+
+/* out[1] = decision (1 pull, 2 push, 0 no model), out[2..5] = the raw
+ * features as 8 floats, out[6] = the two outputs as floats.  The unvisited
+ * counts advance by the frontier (BFS marks each vertex once). */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_decide(const uint64_t* __restrict__ partials, uint32_t blocks,
+                                                            gz_sel_dev_state* st, const float* __restrict__ p,
+                                                            const float* __restrict__ norm, int have_model, int bfs,
+                                                            uint64_t num_vertices, uint64_t num_edges,
+                                                            float iteration, float prev_engine, float prev_ms,
+                                                            float clock_khz, uint64_t* __restrict__ out)
+{
+    __shared__ uint64_t red[3][GZ_BLOCK];
+    __shared__ float x[GZ_SEL_IN], xn[GZ_SEL_IN], h1[GZ_SEL_H], h2[GZ_SEL_H], y[GZ_SEL_OUT];
     const int j = threadIdx.x;
+    uint64_t s0 = 0, s1 = 0, s2 = 0;
+    for (uint32_t b = j; b < blocks; b += blockDim.x) {
+        s0 += partials[3 * (uint64_t)b];
+        s1 += partials[3 * (uint64_t)b + 1];
+        s2 += partials[3 * (uint64_t)b + 2];
+    }
+    red[0][j] = s0;
+    red[1][j] = s1;
+    red[2][j] = s2;
+    __syncthreads();
     if (j == 0) {
-        const uint64_t cv = atomicAdd((unsigned long long*)&st->frontier_vertices, 0ull);
-        const uint64_t ce = atomicAdd((unsigned long long*)&st->frontier_edges, 0ull);
-        const uint64_t ci = atomicAdd((unsigned long long*)&st->frontier_in_edges, 0ull);
+        uint64_t cv = 0, ce = 0, ci = 0;
+        for (int i = 0; i < (int)blockDim.x; ++i) {
+            cv += red[0][i];
+            ce += red[1][i];
+            ci += red[2][i];
+        }
         uint64_t uv = num_vertices, ue = num_edges;
         if (bfs) {
             st->unvisited -= cv;
@@ -221,7 +244,10 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_step(const uint64_t* __rest
             ue = st->unvisited_in_edges;
         }
         float ms = prev_ms;
-        if (ms < 0.f) ms = (st->t_end > st->t_begin) ? (float)(st->t_end - st->t_begin) / clock_khz : 0.f;
+        if (ms < 0.f) {
+            const uint64_t t_end = partials[3 * (uint64_t)blocks];
+            ms = (t_end > st->t_begin) ? (float)(t_end - st->t_begin) / clock_khz : 0.f;
+        }
         x[0] = logf(1.f + (float)cv);
         x[1] = logf(1.f + (float)ce);
         x[2] = logf(1.f + (float)uv);
@@ -234,10 +260,6 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_step(const uint64_t* __rest
             const float sd = norm[GZ_SEL_IN + i];
             xn[i] = (x[i] - norm[i]) / (sd > 0.f ? sd : 1.f);
         }
-        st->frontier_vertices = 0;
-        st->frontier_edges = 0;
-        st->frontier_in_edges = 0;
-        st->blocks_done = 0;
     }
     __syncthreads();
     if (have_model) {
@@ -275,6 +297,8 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_step(const uint64_t* __rest
 static float* gz_sel_dev_params = nullptr;
 static float* gz_sel_dev_norm = nullptr;      /* mean[8], std[8] */
 static gz_sel_dev_state* gz_sel_dev_st = nullptr;
+static uint64_t* gz_sel_dev_partials = nullptr;
+static uint64_t gz_sel_partials_blocks = 0;
 static float gz_sel_clock_khz = 0.f;
 
 extern "C" int gz_selector_device_upload_norm(const float* mean, const float* stdev)
@@ -283,6 +307,13 @@ extern "C" int gz_selector_device_upload_norm(const float* mean, const float* st
     gz_copy_to_device(gz_sel_dev_norm, mean, GZ_SEL_IN * sizeof(float));
     gz_copy_to_device(gz_sel_dev_norm + GZ_SEL_IN, stdev, GZ_SEL_IN * sizeof(float));
     return 0;
+}
+
+static uint64_t gz_sel_stats_blocks(uint64_t words)
+{
+    const uint64_t per_block = (uint64_t)GZ_WAVES_PER_BLOCK * GZ_SEL_WORDS_PER_WAVE;
+    const uint64_t b = (words + per_block - 1) / per_block;
+    return b ? b : 1;
 }
 
 /* Start a run: unvisited = V / E (the first step removes the initial
@@ -302,6 +333,12 @@ extern "C" void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges)
         const float one[GZ_SEL_IN] = {1, 1, 1, 1, 1, 1, 1, 1};
         gz_selector_device_upload_norm(zero, one);
     }
+    const uint64_t blocks = gz_sel_stats_blocks((num_vertices + 63) / 64);
+    if (blocks > gz_sel_partials_blocks) {
+        gz_device_free(gz_sel_dev_partials);
+        gz_sel_dev_partials = (uint64_t*)gz_device_malloc((3 * blocks + 1) * sizeof(uint64_t));
+        gz_sel_partials_blocks = blocks;
+    }
     gz_sel_dev_state s;
     memset(&s, 0, sizeof(s));
     s.unvisited = num_vertices;
@@ -318,20 +355,21 @@ extern "C" void gz_sel_dev_stamp(void)
 
 /* Statistics of `frontier` (a bitmap over V vertices), features and (with a
  * model) the decision into reduce_buffer[1..6].  prev_ms < 0 takes the
- * previous edge phase's time from the device clock (stamp to this launch). */
+ * previous edge phase's time from the device clock (stamp to this call). */
 extern "C" void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg, uint64_t* reduce_buffer,
                                 int have_model, int bfs, uint64_t iteration, uint32_t prev_engine, float prev_ms)
 {
     if (have_model && gz_sel_dev_params == nullptr) have_model = 0;
     const uint64_t V = gz_graph.num_vertices;
     const uint64_t words = (V + 63) / 64;
-    uint64_t blocks = (words + GZ_BLOCK - 1) / GZ_BLOCK;
-    if (blocks > 1024) blocks = 1024;
-    if (blocks == 0) blocks = 1;
-    hipLaunchKernelGGL(gz_k_sel_step, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, frontier,
-                       gz_graph.vertex_outdegrees, indeg, V, gz_graph.num_edges, words, gz_sel_dev_st,
-                       gz_sel_dev_params, gz_sel_dev_norm, have_model, bfs, (float)iteration, (float)prev_engine,
-                       prev_ms, gz_sel_clock_khz, reduce_buffer);
+    const uint64_t blocks = gz_sel_stats_blocks(words);
+    hipLaunchKernelGGL(gz_k_sel_stats, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, frontier,
+                       gz_graph.vertex_outdegrees, indeg, V, words, gz_sel_dev_partials);
+    GZ_LAUNCH_CHECK();
+    hipLaunchKernelGGL(gz_k_sel_decide, dim3(1), dim3(GZ_BLOCK), 0, gz_stream, (const uint64_t*)gz_sel_dev_partials,
+                       (uint32_t)blocks, gz_sel_dev_st, gz_sel_dev_params, gz_sel_dev_norm, have_model, bfs, V,
+                       gz_graph.num_edges, (float)iteration, (float)prev_engine, prev_ms, gz_sel_clock_khz,
+                       reduce_buffer);
     GZ_LAUNCH_CHECK();
 }
 
