@@ -26,6 +26,9 @@
  *   -S                          per-iteration vector statistics on stderr (ITERATION_STATS)
  *   -w file                     selector weights for -e learned
  *   -D index                    HIP device
+ *   -R                          device ingress of base-pull only, vertices renumbered
+ *                               by out-degree (self-loops and duplicate edges dropped;
+ *                               results and -r keep original ids)
  */
 #include <ctype.h>
 #include <stdio.h>
@@ -48,12 +51,13 @@ typedef struct gz_cli_t {
     uint32_t iterations;
     uint64_t sched_granularity;
     int device;
+    int renumber;
     gz_run_options_t run;
 } gz_cli_t;
 
 static int gz_is_switch(char c) { return c == '-'; }
 
-static int gz_recognized(char c) { return strchr("hinNosVuaedrtTEXpSwD", c) != NULL; }
+static int gz_recognized(char c) { return strchr("hinNosVuaedrtTEXpSwDR", c) != NULL; }
 
 static int gz_requires_value(char c) { return strchr("inNousaedrtwD", c) != NULL; }
 
@@ -87,6 +91,8 @@ static void gz_usage(const char* argv0)
     printf("  -S\n        Per-iteration vector statistics CSV on stderr.\n");
     printf("  -w weights\n        Selector weights for -e learned.\n");
     printf("  -D device\n        HIP device index. Defaults to 0.\n");
+    printf("  -R\n        Build the graph on the device from input-graph-pull alone, vertices renumbered\n"
+           "        by out-degree (drops self-loops and duplicate edges; output keeps original ids).\n");
     printf("  -V\n        Prints version information and exits.\n");
     exit(0);
 }
@@ -205,6 +211,9 @@ static void gz_parse_option(gz_cli_t* c, const char* argv0, const char* opt, con
     case 'D':
         c->device = (int)gz_parse_u64(argv0, opt, val, 0);
         break;
+    case 'R':
+        c->renumber = 1;
+        break;
     default:
         gz_err(argv0, "%s: Unrecognized option `%s'.%s\n", opt, "", 1);
     }
@@ -233,7 +242,7 @@ int gz_main(int argc, char** argv)
     c.run.iterations = c.iterations;
 
     /* unreadable inputs exit 255 before the device is touched (graphdata.c:963-967) */
-    const int need_push = !(c.run.algorithm == GZ_ALG_PR && c.run.engine != GZ_ENGINE_PUSH);
+    const int need_push = !c.renumber && !(c.run.algorithm == GZ_ALG_PR && c.run.engine != GZ_ENGINE_PUSH);
     const char* files[2] = {c.gather, need_push ? c.scatter : NULL};
     for (int k = 0; k < 2; ++k) {
         if (files[k] == NULL) continue;
@@ -248,7 +257,20 @@ int gz_main(int argc, char** argv)
     }
     gz_runtime_init(c.device);
     if (c.sched_granularity) gz_phase_config.chunk_vectors = (uint32_t)((c.sched_granularity + 63) / 64 * 64);
-    graph_data_read_from_file(c.gather, need_push ? c.scatter : NULL, 1, NULL);
+    if (c.renumber) {
+        uint64_t nv = 0, ne = 0, *pairs = NULL;
+        if (gz_edge_list_read(c.gather, &nv, &ne, &pairs) != 0) {
+            fprintf(stderr, "Error: unable to read file \"%s\"\n", c.gather);
+            exit(255);
+        }
+        gz_graph_build_pairs_device(nv, ne, pairs, 0, GZ_INGRESS_RELABEL_BY_OUTDEGREE);
+        free(pairs);
+        printf("Device ingress: %llu vertices, %llu edges, %llu pull / %llu push vectors, renumbered by out-degree\n",
+               (unsigned long long)gz_graph_num_vertices(), (unsigned long long)gz_graph_num_edges(),
+               (unsigned long long)gz_graph_pull_vector_count(), (unsigned long long)gz_graph_push_vector_count());
+    } else {
+        graph_data_read_from_file(c.gather, need_push ? c.scatter : NULL, 1, NULL);
+    }
     graph_data_allocate_merge_buffers(c.threads, 1, NULL);
     printf("Scheduler: total units = %llu, vectors per unit = %u\n",
            (unsigned long long)gz_pull_units_for(gz_graph_pull_vector_count()), gz_phase_config.chunk_vectors);
