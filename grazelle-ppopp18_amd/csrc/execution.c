@@ -286,10 +286,15 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
     memset(&sel, 0, sizeof(sel));
     if (features) {
         /* features of the first iteration: the initial frontier */
-        gz_sel_dev_begin(V, E);
-        gz_sel_dev_step(gz_graph.frontier_has_info, indeg, gz_reduce_buffer, learned, bfs, 1, 0, 0.0f);
+        const uint64_t root = bfs ? gz_graph_device_vertex(o->root) : V;
+        gz_sel_dev_begin(V, E, root, indeg);
+        gz_sel_dev_step(gz_graph.frontier_has_info, indeg, gz_reduce_buffer, learned, bfs, 1, 0, 0.0f,
+                        bfs ? GZ_SEL_FROM_ROOT : GZ_SEL_FROM_BITMAP);
         (void)phase_op_combine_global_var_from_buf(gz_reduce_buffer);
         gz_selector_read(&sel);
+        /* BFS activates each vertex once: its edge phases fold the statistics
+         * of the next frontier into their vote pass */
+        if (bfs) gz_features_fold(indeg);
     }
     uint64_t ctr = 0;
     while (1) {
@@ -343,7 +348,7 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
         if (features) {
             /* the vertices this phase activated are the next frontier */
             gz_sel_dev_step((const uint64_t*)gz_graph.vertex_accumulators, indeg, gz_reduce_buffer, learned, bfs,
-                            ctr + 1, rec.engine, prev_ms);
+                            ctr + 1, rec.engine, prev_ms, bfs ? GZ_SEL_FROM_EDGE_PHASE : GZ_SEL_FROM_BITMAP);
         }
         const uint64_t next_vote = phase_op_combine_global_var_from_buf(gz_reduce_buffer);
         if (features) gz_selector_read(&sel);
@@ -359,6 +364,7 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
             perform_vertex_phase_cc(0, V, NULL);
     }
     r->iterations = ctr;
+    gz_features_fold(NULL);
     if (t) gz_timer_destroy(t);
     if (probe) gz_snapshot_free(&snap);
 }

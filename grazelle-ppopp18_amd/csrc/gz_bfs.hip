@@ -58,13 +58,15 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_bfs(const gz_vec_t* _
                                                                uint64_t* __restrict__ next_frontier,
                                                                const double* __restrict__ outdeg, uint32_t stat_mode,
                                                                uint64_t level, uint64_t* __restrict__ partials,
-                                                               uint64_t* __restrict__ stats, uint32_t collect)
+                                                               uint64_t* __restrict__ stats, uint32_t collect,
+                                                               gz_feature_sink sink)
 {
     const int lane = gz_lane();
     const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
     const uint64_t W = (uint64_t)gridDim.x * GZ_WAVES_PER_BLOCK;
     const uint64_t T = gz_slot_total<kActive>(src);
     uint64_t vote = 0;
+    gz_feature_acc feat{0, 0, 0};
     uint32_t vectors = 0, edges = 0;
     for (uint64_t chunk = gw; chunk * chunk_len < T; chunk += W) {
         const uint64_t j0 = chunk * chunk_len;
@@ -91,12 +93,14 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_bfs(const gz_vec_t* _
                 if (old == GZ_BFS_UNVISITED) {
                     gz_bit_set(next_frontier, d);
                     vote += gz_frontier_stat(outdeg, d, stat_mode);
+                    gz_feature_add(feat, sink, outdeg, d);
                 }
             }
         }
     }
     const uint64_t total = gz_wave_sum_u64(vote);
     if (lane == 0) partials[gw] = total;
+    gz_feature_flush(feat, sink, gw);
     if (collect) {
         const uint64_t tv = gz_wave_sum_u64(vectors), te = gz_wave_sum_u64(edges);
         if (lane == 0) {
@@ -190,12 +194,14 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_visit(const gz_ve
                                                                      uint64_t* __restrict__ next_frontier,
                                                                      const double* __restrict__ outdeg,
                                                                      uint32_t stat_mode, uint64_t level,
-                                                                     uint64_t* __restrict__ partials)
+                                                                     uint64_t* __restrict__ partials,
+                                                                     gz_feature_sink sink)
 {
     const int lane = gz_lane();
     const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
     const uint64_t n = kActive ? totals[0] : num_vertices;
     uint64_t vote = 0;
+    gz_feature_acc feat{0, 0, 0};
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t d = kActive ? list[i] : i;
         const uint64_t w = first_vector[d];
@@ -215,9 +221,11 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_visit(const gz_ve
         gz_bit_clear(wants_info, d);
         gz_bit_set(next_frontier, d);
         vote += gz_frontier_stat(outdeg, d, stat_mode);
+        gz_feature_add(feat, sink, outdeg, d);
     }
     const uint64_t total = gz_wave_sum_u64(vote);
     if (lane == 0) partials[gw] = total;
+    gz_feature_flush(feat, sink, gw);
 }
 
 static gz_slot_source gz_bfs_source(const uint64_t* offsets, uint64_t count)
@@ -251,19 +259,20 @@ extern "C" void perform_edge_push_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     gz_scratch.partials_count = waves;
     gz_scratch.stats_count = gz_phase_config.collect_stats ? waves : 0;
     const uint64_t level = gz_phase_config.bfs_level;
+    const gz_feature_sink sink{gz_scratch.features_indeg ? gz_scratch.features : nullptr, gz_scratch.features_indeg};
     gz_ktimer_begin(GZ_K_PUSH_BFS);
     if (active)
         hipLaunchKernelGGL(gz_k_edge_push_bfs<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
                            src, chunk_len, gz_graph.frontier_has_info, gz_graph.frontier_wants_info,
                            (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
                            gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, level,
-                           (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats);
+                           (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats, sink);
     else
         hipLaunchKernelGGL(gz_k_edge_push_bfs<false>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
                            src, chunk_len, gz_graph.frontier_has_info, gz_graph.frontier_wants_info,
                            (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
                            gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, level,
-                           (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats);
+                           (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats, sink);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_PUSH_BFS);
     const uint64_t words = (gz_graph.num_vertices + 63) / 64;
@@ -295,6 +304,7 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     gz_scratch.partials_count = visit_blocks * GZ_WAVES_PER_BLOCK;
     gz_scratch.stats_count = gz_phase_config.collect_stats ? blocks * GZ_WAVES_PER_BLOCK : 0;
     const uint64_t level = gz_phase_config.bfs_level;
+    const gz_feature_sink sink{gz_scratch.features_indeg ? gz_scratch.features : nullptr, gz_scratch.features_indeg};
     gz_ktimer_begin(GZ_K_PULL_BFS_FIND);
     if (active)
         hipLaunchKernelGGL(gz_k_edge_pull_bfs_find<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
@@ -313,14 +323,14 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
                            gz_graph.frontier_has_info, gz_graph.frontier_wants_info, gz_scratch.bfs_first_vector,
                            (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
                            gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, level,
-                           (uint64_t*)gz_scratch.partials);
+                           (uint64_t*)gz_scratch.partials, sink);
     else
         hipLaunchKernelGGL(gz_k_edge_pull_bfs_visit<false>, dim3((unsigned)visit_blocks), dim3(GZ_BLOCK), 0, gz_stream,
                            edge_list, gz_scratch.active_vertices, gz_scratch.active_totals, gz_graph.num_vertices,
                            gz_graph.frontier_has_info, gz_graph.frontier_wants_info, gz_scratch.bfs_first_vector,
                            (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
                            gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, level,
-                           (uint64_t*)gz_scratch.partials);
+                           (uint64_t*)gz_scratch.partials, sink);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_PULL_BFS_VISIT);
 }

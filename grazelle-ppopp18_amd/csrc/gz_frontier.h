@@ -20,6 +20,15 @@
 
 #include "gz_internal.h"
 
+/* Vertices per thread and per block of the compaction passes. */
+#define GZ_COMPACT_VPT 8
+#define GZ_COMPACT_VPB (GZ_COMPACT_VPT * GZ_BLOCK)
+
+static inline uint64_t gz_compact_blocks(uint64_t num_vertices)
+{
+    return (num_vertices + GZ_COMPACT_VPB - 1) / GZ_COMPACT_VPB;
+}
+
 __device__ __forceinline__ uint64_t gz_wave_or_u64(uint64_t x)
 {
 #pragma unroll

@@ -2,34 +2,34 @@
  * gz_frontier.hip -- compaction of a frontier bitmap into the active-vertex
  * list consumed by the frontier-driven edge kernels (see gz_frontier.h).
  * Three passes: per-block counts, a single-block exclusive scan of the block
- * counts, and a write pass that re-derives each thread's offsets.
+ * counts, and a write pass that re-derives each vertex's position.
+ *
+ * Layout: a block covers GZ_COMPACT_VPB = GZ_COMPACT_VPT x GZ_BLOCK
+ * consecutive vertices, GZ_COMPACT_VPT per thread, so a wavefront covers a
+ * few bitmap words and one contiguous range of offsets; list positions come
+ * from one block scan of the per-thread counts, in vertex order.
  */
 #include "gz_frontier.h"
 
-/* Each thread owns one 64-vertex bitmap word. */
-__device__ __forceinline__ void gz_word_counts(const uint64_t* __restrict__ bitmap, const uint64_t* __restrict__ offsets,
-                                               uint64_t num_vertices, uint64_t w, uint64_t words, uint64_t& nv,
-                                               uint64_t& nvec)
+/* Vertices v0 .. v0+GZ_COMPACT_VPT-1 (v0 a multiple of GZ_COMPACT_VPT):
+ * vector counts c[k] of the vertices whose bit is set, 0 otherwise; a vertex
+ * is active when c[k] > 0.  Neighbouring lanes read neighbouring bits of one
+ * bitmap word and neighbouring ranges of the offsets. */
+__device__ __forceinline__ void gz_run_counts(const uint64_t* __restrict__ bitmap, const uint64_t* __restrict__ offsets,
+                                              uint64_t num_vertices, uint64_t v0, uint64_t c[GZ_COMPACT_VPT])
 {
-    nv = 0;
-    nvec = 0;
-    if (w >= words) return;
-    uint64_t bits = bitmap[w];
-    const uint64_t base = w * 64;
-    if (base + 64 > num_vertices) {
-        const uint64_t keep = num_vertices - base;
-        bits &= (keep >= 64) ? ~0ull : ((1ull << keep) - 1ull);
-    }
-    while (bits) {
-        const int b = __ffsll((unsigned long long)bits) - 1;
-        bits &= bits - 1;
-        const uint64_t v = base + (uint64_t)b;
-        const uint64_t c = offsets[v + 1] - offsets[v];
-        if (c) {
-            nv += 1;
-            nvec += c;
-        }
-    }
+#pragma unroll
+    for (int k = 0; k < GZ_COMPACT_VPT; ++k) c[k] = 0;
+    if (v0 >= num_vertices) return;
+    uint64_t bits = (bitmap[v0 >> 6] >> (v0 & 63)) & ((1ull << GZ_COMPACT_VPT) - 1ull);
+    const uint64_t left = num_vertices - v0;
+    if (left < GZ_COMPACT_VPT) bits &= (1ull << left) - 1ull;
+    if (bits == 0) return;
+    uint64_t o[GZ_COMPACT_VPT + 1];
+#pragma unroll
+    for (int k = 0; k <= GZ_COMPACT_VPT; ++k) o[k] = offsets[(v0 + k < num_vertices) ? v0 + k : num_vertices];
+#pragma unroll
+    for (int k = 0; k < GZ_COMPACT_VPT; ++k) c[k] = ((bits >> k) & 1ull) ? o[k + 1] - o[k] : 0;
 }
 
 template <typename T>
@@ -58,13 +58,19 @@ __device__ __forceinline__ T gz_block_exclusive_scan(T x, T* lds, T& block_total
 
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_count(const uint64_t* __restrict__ bitmap,
                                                                  const uint64_t* __restrict__ offsets,
-                                                                 uint64_t num_vertices, uint64_t words,
+                                                                 uint64_t num_vertices,
                                                                  uint64_t* __restrict__ block_sums)
 {
     __shared__ unsigned long long lds[GZ_WAVES_PER_BLOCK];
-    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t nv, nvec;
-    gz_word_counts(bitmap, offsets, num_vertices, w, words, nv, nvec);
+    const uint64_t v0 = (uint64_t)blockIdx.x * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
+    uint64_t c[GZ_COMPACT_VPT];
+    gz_run_counts(bitmap, offsets, num_vertices, v0, c);
+    unsigned long long nv = 0, nvec = 0;
+#pragma unroll
+    for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
+        nv += c[k] ? 1 : 0;
+        nvec += c[k];
+    }
     unsigned long long tv, tvec;
     gz_block_exclusive_scan<unsigned long long>(nv, lds, tv);
     gz_block_exclusive_scan<unsigned long long>(nvec, lds, tvec);
@@ -75,43 +81,78 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_count(const uint64_t* 
 }
 
 /* Single block: exclusive scan of the per-block pairs, totals to totals[0..1],
- * and the sentinel prefix[total vertices] = total vectors. */
+ * and the sentinel prefix[total vertices] = total vectors.  A pass stages up
+ * to GZ_SCAN_PASS pairs in LDS with coalesced loads; each thread then scans
+ * GZ_SCAN_PASS / 1024 consecutive pairs serially around one block scan of
+ * the thread sums, and the pass is written back coalesced. */
+#define GZ_SCAN_PASS 4096
 __global__ __launch_bounds__(1024) void gz_k_frontier_scan(uint64_t* __restrict__ block_sums, uint64_t blocks,
                                                            uint64_t* __restrict__ totals, uint64_t* __restrict__ prefix)
 {
-    __shared__ unsigned long long lds_v[1024 / GZ_WAVE];
+    __shared__ unsigned long long stage[2 * GZ_SCAN_PASS];
+    __shared__ unsigned long long lds_v[2][1024 / GZ_WAVE];
     __shared__ unsigned long long carry[2];
+    const int per = GZ_SCAN_PASS / 1024;
     if (threadIdx.x == 0) {
         carry[0] = 0;
         carry[1] = 0;
     }
-    __syncthreads();
     const int lane = gz_lane();
     const int w = threadIdx.x / GZ_WAVE;
     const int nw = blockDim.x / GZ_WAVE;
-    for (uint64_t base = 0; base < blocks; base += blockDim.x) {
-        const uint64_t i = base + threadIdx.x;
-        for (int field = 0; field < 2; ++field) {
-            unsigned long long x = (i < blocks) ? block_sums[2 * i + field] : 0ull;
-            unsigned long long incl = x;
+    for (uint64_t base = 0; base < blocks; base += GZ_SCAN_PASS) {
+        const uint64_t n = (blocks - base < GZ_SCAN_PASS) ? blocks - base : GZ_SCAN_PASS;
+        __syncthreads();
+        for (uint64_t i = threadIdx.x; i < 2 * GZ_SCAN_PASS; i += blockDim.x)
+            stage[i] = (i < 2 * n) ? block_sums[2 * base + i] : 0ull;
+        __syncthreads();
+        unsigned long long x[2] = {0, 0};
+        for (int k = 0; k < per; ++k) {
+            x[0] += stage[2 * (threadIdx.x * per + k)];
+            x[1] += stage[2 * (threadIdx.x * per + k) + 1];
+        }
+        unsigned long long incl[2] = {x[0], x[1]};
 #pragma unroll
-            for (int o = 1; o < GZ_WAVE; o <<= 1) {
-                const unsigned long long t = __shfl_up(incl, o, GZ_WAVE);
-                if (lane >= o) incl += t;
+        for (int o = 1; o < GZ_WAVE; o <<= 1) {
+            const unsigned long long t0 = __shfl_up(incl[0], o, GZ_WAVE);
+            const unsigned long long t1 = __shfl_up(incl[1], o, GZ_WAVE);
+            if (lane >= o) {
+                incl[0] += t0;
+                incl[1] += t1;
             }
-            if (lane == GZ_WAVE - 1) lds_v[w] = incl;
-            __syncthreads();
-            unsigned long long off = carry[field], tot = 0;
-            for (int k = 0; k < nw; ++k) {
-                if (k < w) off += lds_v[k];
-                tot += lds_v[k];
+        }
+        if (lane == GZ_WAVE - 1) {
+            lds_v[0][w] = incl[0];
+            lds_v[1][w] = incl[1];
+        }
+        __syncthreads();
+        unsigned long long off[2] = {carry[0], carry[1]}, tot[2] = {0, 0};
+        for (int k = 0; k < nw; ++k) {
+            if (k < w) {
+                off[0] += lds_v[0][k];
+                off[1] += lds_v[1][k];
             }
-            if (i < blocks) block_sums[2 * i + field] = off + incl - x;
-            __syncthreads();
-            if (threadIdx.x == 0) carry[field] += tot;
-            __syncthreads();
+            tot[0] += lds_v[0][k];
+            tot[1] += lds_v[1][k];
+        }
+        off[0] += incl[0] - x[0];
+        off[1] += incl[1] - x[1];
+        for (int k = 0; k < per; ++k) {
+            const int i = 2 * (threadIdx.x * per + k);
+            const unsigned long long y0 = stage[i], y1 = stage[i + 1];
+            stage[i] = off[0];
+            stage[i + 1] = off[1];
+            off[0] += y0;
+            off[1] += y1;
+        }
+        __syncthreads();
+        for (uint64_t i = threadIdx.x; i < 2 * n; i += blockDim.x) block_sums[2 * base + i] = stage[i];
+        if (threadIdx.x == 0) {
+            carry[0] += tot[0];
+            carry[1] += tot[1];
         }
     }
+    __syncthreads();
     if (threadIdx.x == 0) {
         totals[0] = carry[0];
         totals[1] = carry[1];
@@ -121,43 +162,38 @@ __global__ __launch_bounds__(1024) void gz_k_frontier_scan(uint64_t* __restrict_
 
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_write(const uint64_t* __restrict__ bitmap,
                                                                  const uint64_t* __restrict__ offsets,
-                                                                 uint64_t num_vertices, uint64_t words,
+                                                                 uint64_t num_vertices,
                                                                  const uint64_t* __restrict__ block_sums,
                                                                  uint64_t* __restrict__ list,
                                                                  uint64_t* __restrict__ prefix)
 {
     __shared__ unsigned long long lds[GZ_WAVES_PER_BLOCK];
-    const uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    uint64_t nv, nvec;
-    gz_word_counts(bitmap, offsets, num_vertices, w, words, nv, nvec);
+    const uint64_t v0 = (uint64_t)blockIdx.x * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
+    uint64_t c[GZ_COMPACT_VPT];
+    gz_run_counts(bitmap, offsets, num_vertices, v0, c);
+    unsigned long long nv = 0, nvec = 0;
+#pragma unroll
+    for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
+        nv += c[k] ? 1 : 0;
+        nvec += c[k];
+    }
     unsigned long long tv, tvec;
     uint64_t pos = block_sums[2 * blockIdx.x] + gz_block_exclusive_scan<unsigned long long>(nv, lds, tv);
     uint64_t vpos = block_sums[2 * blockIdx.x + 1] + gz_block_exclusive_scan<unsigned long long>(nvec, lds, tvec);
-    if (w >= words || nv == 0) return;
-    uint64_t bits = bitmap[w];
-    const uint64_t base = w * 64;
-    if (base + 64 > num_vertices) {
-        const uint64_t keep = num_vertices - base;
-        bits &= (keep >= 64) ? ~0ull : ((1ull << keep) - 1ull);
-    }
-    while (bits) {
-        const int b = __ffsll((unsigned long long)bits) - 1;
-        bits &= bits - 1;
-        const uint64_t v = base + (uint64_t)b;
-        const uint64_t c = offsets[v + 1] - offsets[v];
-        if (c) {
-            list[pos] = v;
+#pragma unroll
+    for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
+        if (c[k]) {
+            list[pos] = v0 + (uint64_t)k;
             prefix[pos] = vpos;
             pos += 1;
-            vpos += c;
+            vpos += c[k];
         }
     }
 }
 
 void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices)
 {
-    const uint64_t words = (num_vertices + 63) / 64;
-    const uint64_t blocks = (words + GZ_BLOCK - 1) / GZ_BLOCK;
+    const uint64_t blocks = gz_compact_blocks(num_vertices);
     if (blocks == 0) {
         gz_device_zero(gz_scratch.active_totals, 2 * sizeof(uint64_t));
         gz_device_zero(gz_scratch.active_prefix, sizeof(uint64_t));
@@ -165,13 +201,13 @@ void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64
     }
     gz_ktimer_begin(GZ_K_COMPACT);
     hipLaunchKernelGGL(gz_k_frontier_count, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, offsets,
-                       num_vertices, words, gz_scratch.active_block_sums);
+                       num_vertices, gz_scratch.active_block_sums);
     GZ_LAUNCH_CHECK();
     hipLaunchKernelGGL(gz_k_frontier_scan, dim3(1), dim3(1024), 0, gz_stream, gz_scratch.active_block_sums, blocks,
                        gz_scratch.active_totals, gz_scratch.active_prefix);
     GZ_LAUNCH_CHECK();
     hipLaunchKernelGGL(gz_k_frontier_write, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, offsets,
-                       num_vertices, words, gz_scratch.active_block_sums, gz_scratch.active_vertices,
+                       num_vertices, gz_scratch.active_block_sums, gz_scratch.active_vertices,
                        gz_scratch.active_prefix);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_COMPACT);

@@ -45,10 +45,15 @@ const uint32_t* gz_graph_indegrees_device(void);
 const uint64_t* gz_reduce_words(void);
 
 /* Learned selector on the device (selector_kernels.hip). */
-void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges);
+enum { GZ_SEL_FROM_BITMAP = 0, GZ_SEL_FROM_EDGE_PHASE = 1, GZ_SEL_FROM_ROOT = 2 };
+/* root < V stages a one-vertex initial frontier (BFS); pass V for none. */
+void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges, uint64_t root, const uint32_t* indeg);
 void gz_sel_dev_stamp(void);
 void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg, uint64_t* reduce_buffer, int have_model,
-                     int bfs, uint64_t iteration, uint32_t prev_engine, float prev_ms);
+                     int bfs, uint64_t iteration, uint32_t prev_engine, float prev_ms, int source);
+/* BFS edge phases fold the selector's frontier statistics into their vote
+ * pass while indeg != NULL (NULL turns it off). */
+void gz_features_fold(const uint32_t* indeg);
 void gz_clear_bitmap_range(uint64_t* bitmap, uint64_t vertex_start, uint64_t vertex_count);
 
 #ifdef __cplusplus

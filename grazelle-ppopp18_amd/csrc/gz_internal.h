@@ -50,6 +50,8 @@ typedef struct gz_scratch_t {
     uint64_t* active_totals;     /* [0] = active vertices, [1] = active vectors */
     uint64_t active_capacity;
     uint64_t* bfs_first_vector;  /* [V] first qualifying pull vector of each dst, ~0 when none */
+    uint64_t* features;          /* [3 * partials_capacity] selector features per wave (BFS edge phases) */
+    const uint32_t* features_indeg; /* in-degrees when BFS edge phases fold the features, else nullptr */
     uint64_t scratch_vertices;   /* V the vertex-sized buffers were sized for */
 } gz_scratch_t;
 
@@ -147,10 +149,46 @@ __device__ __forceinline__ uint64_t gz_wave_sum_u64(uint64_t x)
     return x;
 }
 
+/* Selector features folded into a BFS edge phase: every vertex the phase
+ * activates (exactly once in BFS) adds (1, outdeg, indeg) to the lane's
+ * counters, and each wavefront writes its sums next to its vote partial at
+ * out[3 * wave + 0..2].  out == nullptr turns it off. */
+struct gz_feature_sink {
+    uint64_t* out;
+    const uint32_t* indeg;
+};
+
+struct gz_feature_acc {
+    uint64_t count, out_edges, in_edges;
+};
+
+__device__ __forceinline__ void gz_feature_add(gz_feature_acc& f, const gz_feature_sink& s,
+                                               const double* __restrict__ outdeg, uint64_t v)
+{
+    if (s.out) {
+        f.count += 1;
+        f.out_edges += (uint64_t)outdeg[v];
+        f.in_edges += (uint64_t)s.indeg[v];
+    }
+}
+
 /* Lane `src`'s value of x in every lane. */
 __device__ __forceinline__ uint64_t gz_shfl_u64(uint64_t x, int src)
 {
     return (uint64_t)__shfl((unsigned long long)x, src, GZ_WAVE);
+}
+
+__device__ __forceinline__ void gz_feature_flush(const gz_feature_acc& f, const gz_feature_sink& s, uint64_t wave)
+{
+    if (s.out) {
+        const uint64_t c = gz_wave_sum_u64(f.count), o = gz_wave_sum_u64(f.out_edges),
+                       i = gz_wave_sum_u64(f.in_edges);
+        if ((threadIdx.x & (GZ_WAVE - 1)) == 0) {
+            s.out[3 * wave] = c;
+            s.out[3 * wave + 1] = o;
+            s.out[3 * wave + 2] = i;
+        }
+    }
 }
 
 /* Bitmaps are arrays of uint64 words, bit v%64 of word v/64 (LSB first).  The

@@ -10,6 +10,7 @@
  * reference split over threads is split over wavefronts.
  */
 #include "gz_internal.h"
+#include "gz_frontier.h"
 
 #include <string.h>
 
@@ -242,6 +243,8 @@ static void gz_kt_flush(void)
 
 extern "C" int gz_kernel_timing_enabled(void) { return gz_kt_on; }
 
+extern "C" void gz_features_fold(const uint32_t* indeg) { gz_scratch.features_indeg = indeg; }
+
 extern "C" void gz_kernel_timing_enable(int on)
 {
     gz_require_runtime();
@@ -317,8 +320,10 @@ void gz_scratch_reserve(uint64_t waves, uint64_t num_vertices)
     if (waves > gz_scratch.partials_capacity) {
         gz_device_free(gz_scratch.partials);
         gz_device_free(gz_scratch.stats);
+        gz_device_free(gz_scratch.features);
         gz_scratch.partials = gz_device_malloc(waves * sizeof(uint64_t));
         gz_scratch.stats = (uint64_t*)gz_device_malloc(waves * 2 * sizeof(uint64_t));
+        gz_scratch.features = (uint64_t*)gz_device_malloc(waves * 3 * sizeof(uint64_t));
         gz_scratch.partials_capacity = waves;
     }
     if (gz_scratch.reduce_tmp == nullptr) gz_scratch.reduce_tmp = (uint64_t*)gz_device_malloc(2048 * sizeof(uint64_t));
@@ -330,8 +335,7 @@ void gz_scratch_reserve(uint64_t waves, uint64_t num_vertices)
         gz_scratch.active_prefix = (uint64_t*)gz_device_malloc((num_vertices + 1) * sizeof(uint64_t));
         gz_scratch.bfs_first_vector = (uint64_t*)gz_device_malloc(num_vertices * sizeof(uint64_t));
         GZ_CHECK(hipMemsetAsync(gz_scratch.bfs_first_vector, 0xff, num_vertices * sizeof(uint64_t), gz_stream));
-        const uint64_t words = (num_vertices + 63) / 64;
-        const uint64_t blocks = (words + GZ_BLOCK - 1) / GZ_BLOCK + 1;
+        const uint64_t blocks = gz_compact_blocks(num_vertices) + 1;
         gz_device_free(gz_scratch.active_block_sums);
         gz_scratch.active_block_sums = (uint64_t*)gz_device_malloc((2 * blocks + 8) * sizeof(uint64_t));
         if (gz_scratch.active_totals == nullptr) gz_scratch.active_totals = (uint64_t*)gz_device_malloc(8 * sizeof(uint64_t));
@@ -519,6 +523,7 @@ extern "C" void gz_runtime_shutdown(void)
     GZ_CHECK(hipStreamSynchronize(gz_stream));
     gz_device_free(gz_scratch.partials);
     gz_device_free(gz_scratch.stats);
+    gz_device_free(gz_scratch.features);
     gz_device_free(gz_scratch.reduce_tmp);
     gz_device_free(gz_scratch.active_vertices);
     gz_device_free(gz_scratch.active_prefix);

@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include "gz_internal.h"
+#include "gz_host_internal.h"
 #include "grazelle_selector.h"
 
 #define O_W1 0
@@ -165,14 +166,13 @@ __global__ void gz_k_sel_stamp(gz_sel_dev_state* st)
  * for the population count, then the wave walks the words one by one with
  * lane l testing vertex 64w + l, so the degree reads of a dense frontier are
  * coalesced.  partials[3b .. 3b+2] = (vertices, out-edges, in-edges) of
- * block b; partials[3 * gridDim.x] = wall clock at the kernel start. */
+ * block b. */
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_stats(const uint64_t* __restrict__ frontier,
                                                            const double* __restrict__ outdeg,
                                                            const uint32_t* __restrict__ indeg, uint64_t num_vertices,
                                                            uint64_t words, uint64_t* __restrict__ partials)
 {
     __shared__ uint64_t red[3][GZ_WAVES_PER_BLOCK];
-    if (blockIdx.x == 0 && threadIdx.x == 0) partials[3 * (uint64_t)gridDim.x] = (uint64_t)wall_clock64();
     const int lane = gz_lane();
     const int wib = threadIdx.x / GZ_WAVE;
     const uint64_t w0 = ((uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + wib) * GZ_SEL_WORDS_PER_WAVE;
@@ -209,29 +209,35 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_stats(const uint64_t* __res
 /* out[1] = decision (1 pull, 2 push, 0 no model), out[2..5] = the raw
  * features as 8 floats, out[6] = the two outputs as floats.  The unvisited
  * counts advance by the frontier (BFS marks each vertex once). */
-__global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_decide(const uint64_t* __restrict__ partials, uint32_t blocks,
+__global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restrict__ partials, uint64_t blocks,
                                                             gz_sel_dev_state* st, const float* __restrict__ p,
                                                             const float* __restrict__ norm, int have_model, int bfs,
                                                             uint64_t num_vertices, uint64_t num_edges,
                                                             float iteration, float prev_engine, float prev_ms,
                                                             float clock_khz, uint64_t* __restrict__ out)
 {
-    __shared__ uint64_t red[3][GZ_BLOCK];
+    __shared__ uint64_t red[3][1024 / GZ_WAVE];
     __shared__ float x[GZ_SEL_IN], xn[GZ_SEL_IN], h1[GZ_SEL_H], h2[GZ_SEL_H], y[GZ_SEL_OUT];
     const int j = threadIdx.x;
+    const uint64_t t_end = (uint64_t)wall_clock64();
     uint64_t s0 = 0, s1 = 0, s2 = 0;
-    for (uint32_t b = j; b < blocks; b += blockDim.x) {
+    for (uint64_t b = j; b < blocks; b += blockDim.x) {
         s0 += partials[3 * (uint64_t)b];
         s1 += partials[3 * (uint64_t)b + 1];
         s2 += partials[3 * (uint64_t)b + 2];
     }
-    red[0][j] = s0;
-    red[1][j] = s1;
-    red[2][j] = s2;
+    s0 = gz_wave_sum_u64(s0);
+    s1 = gz_wave_sum_u64(s1);
+    s2 = gz_wave_sum_u64(s2);
+    if (gz_lane() == 0) {
+        red[0][j / GZ_WAVE] = s0;
+        red[1][j / GZ_WAVE] = s1;
+        red[2][j / GZ_WAVE] = s2;
+    }
     __syncthreads();
     if (j == 0) {
         uint64_t cv = 0, ce = 0, ci = 0;
-        for (int i = 0; i < (int)blockDim.x; ++i) {
+        for (int i = 0; i < (int)blockDim.x / GZ_WAVE; ++i) {
             cv += red[0][i];
             ce += red[1][i];
             ci += red[2][i];
@@ -244,10 +250,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_decide(const uint64_t* __re
             ue = st->unvisited_in_edges;
         }
         float ms = prev_ms;
-        if (ms < 0.f) {
-            const uint64_t t_end = partials[3 * (uint64_t)blocks];
-            ms = (t_end > st->t_begin) ? (float)(t_end - st->t_begin) / clock_khz : 0.f;
-        }
+        if (ms < 0.f) ms = (t_end > st->t_begin) ? (float)(t_end - st->t_begin) / clock_khz : 0.f;
         x[0] = logf(1.f + (float)cv);
         x[1] = logf(1.f + (float)ce);
         x[2] = logf(1.f + (float)uv);
@@ -316,9 +319,26 @@ static uint64_t gz_sel_stats_blocks(uint64_t words)
     return b ? b : 1;
 }
 
+__global__ void gz_k_sel_init(gz_sel_dev_state* st, uint64_t num_vertices, uint64_t num_edges, uint64_t root,
+                              const double* __restrict__ outdeg, const uint32_t* __restrict__ indeg,
+                              uint64_t* __restrict__ partials)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        st->unvisited = num_vertices;
+        st->unvisited_in_edges = num_edges;
+        st->t_begin = 0;
+        if (root < num_vertices) {
+            partials[0] = 1;
+            partials[1] = (uint64_t)outdeg[root];
+            partials[2] = (uint64_t)indeg[root];
+        }
+    }
+}
+
 /* Start a run: unvisited = V / E (the first step removes the initial
- * frontier). */
-extern "C" void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges)
+ * frontier).  With root < V the initial frontier is that one vertex and its
+ * statistics are staged for gz_sel_dev_step(..., GZ_SEL_FROM_ROOT). */
+extern "C" void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges, uint64_t root, const uint32_t* indeg)
 {
     if (gz_sel_dev_st == nullptr) {
         gz_sel_dev_st = (gz_sel_dev_state*)gz_device_malloc(sizeof(gz_sel_dev_state));
@@ -339,11 +359,9 @@ extern "C" void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges)
         gz_sel_dev_partials = (uint64_t*)gz_device_malloc((3 * blocks + 1) * sizeof(uint64_t));
         gz_sel_partials_blocks = blocks;
     }
-    gz_sel_dev_state s;
-    memset(&s, 0, sizeof(s));
-    s.unvisited = num_vertices;
-    s.unvisited_in_edges = num_edges;
-    gz_copy_to_device(gz_sel_dev_st, &s, sizeof(s));
+    hipLaunchKernelGGL(gz_k_sel_init, dim3(1), dim3(64), 0, gz_stream, gz_sel_dev_st, num_vertices, num_edges, root,
+                       gz_graph.vertex_outdegrees, indeg, gz_sel_dev_partials);
+    GZ_LAUNCH_CHECK();
 }
 
 /* Wall-clock stamp taken before an edge phase (learned engine). */
@@ -353,23 +371,34 @@ extern "C" void gz_sel_dev_stamp(void)
     GZ_LAUNCH_CHECK();
 }
 
-/* Statistics of `frontier` (a bitmap over V vertices), features and (with a
- * model) the decision into reduce_buffer[1..6].  prev_ms < 0 takes the
- * previous edge phase's time from the device clock (stamp to this call). */
+/* Features and (with a model) the decision into reduce_buffer[1..6].  The
+ * frontier statistics come from: GZ_SEL_FROM_EDGE_PHASE, the last BFS edge
+ * phase's per-wave sums (gz_scratch.features, folded by the kernels);
+ * GZ_SEL_FROM_ROOT, the root staged by gz_sel_dev_begin; GZ_SEL_FROM_BITMAP,
+ * a pass over `frontier` (a bitmap over V vertices).  prev_ms < 0 takes the
+ * previous edge phase's time from the device clock (stamp to the decision). */
 extern "C" void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg, uint64_t* reduce_buffer,
-                                int have_model, int bfs, uint64_t iteration, uint32_t prev_engine, float prev_ms)
+                                int have_model, int bfs, uint64_t iteration, uint32_t prev_engine, float prev_ms,
+                                int source)
 {
     if (have_model && gz_sel_dev_params == nullptr) have_model = 0;
     const uint64_t V = gz_graph.num_vertices;
-    const uint64_t words = (V + 63) / 64;
-    const uint64_t blocks = gz_sel_stats_blocks(words);
-    hipLaunchKernelGGL(gz_k_sel_stats, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, frontier,
-                       gz_graph.vertex_outdegrees, indeg, V, words, gz_sel_dev_partials);
-    GZ_LAUNCH_CHECK();
-    hipLaunchKernelGGL(gz_k_sel_decide, dim3(1), dim3(GZ_BLOCK), 0, gz_stream, (const uint64_t*)gz_sel_dev_partials,
-                       (uint32_t)blocks, gz_sel_dev_st, gz_sel_dev_params, gz_sel_dev_norm, have_model, bfs, V,
-                       gz_graph.num_edges, (float)iteration, (float)prev_engine, prev_ms, gz_sel_clock_khz,
-                       reduce_buffer);
+    const uint64_t* feat = gz_scratch.features;
+    uint64_t blocks = gz_scratch.partials_count;
+    if (source == GZ_SEL_FROM_ROOT) {
+        feat = gz_sel_dev_partials;
+        blocks = 1;
+    } else if (source == GZ_SEL_FROM_BITMAP) {
+        const uint64_t words = (V + 63) / 64;
+        blocks = gz_sel_stats_blocks(words);
+        hipLaunchKernelGGL(gz_k_sel_stats, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, frontier,
+                           gz_graph.vertex_outdegrees, indeg, V, words, gz_sel_dev_partials);
+        GZ_LAUNCH_CHECK();
+        feat = gz_sel_dev_partials;
+    }
+    hipLaunchKernelGGL(gz_k_sel_decide, dim3(1), dim3(1024), 0, gz_stream, feat, blocks, gz_sel_dev_st,
+                       gz_sel_dev_params, gz_sel_dev_norm, have_model, bfs, V, gz_graph.num_edges, (float)iteration,
+                       (float)prev_engine, prev_ms, gz_sel_clock_khz, reduce_buffer);
     GZ_LAUNCH_CHECK();
 }
 

@@ -10,9 +10,12 @@ visits the same frontiers whichever engine runs each level, so one probed
 search gives the per-level cost of both engines along the only trajectory
 there is, and any policy's total can be read off it.
 
-Training minimises the squared error of the predicted log times with the
-library's own device kernels (gz_selector_train: forward, backward and Adam
-on the GPU).
+Training minimises the squared error of the predicted log times per edge
+(log ms - log E: the choice depends only on the difference of the two
+outputs, and the per-edge cost varies far less with graph size than the
+time, which lets a network trained on scales 18-22 choose well at scale 24)
+with the library's own device kernels (gz_selector_train: forward, backward
+and Adam on the GPU).
 
     python selector_train.py collect --graphs 200 --out data.npz
     python selector_train.py fit --data data.npz --out selector_bfs.bin
@@ -82,6 +85,7 @@ def collect(specs, roots_per_graph: int, log=print) -> dict:
             run = probe_bfs(root)
             run["graph"] = (scale, ef, seed)
             run["root"] = root
+            run["edges"] = gz.num_edges()
             runs.append(run)
         log(f"[collect] graph {k + 1}/{len(specs)} s{scale} ef{ef} seed {seed}: "
             f"{sum(r['features'].shape[0] for r in runs)} samples, {time.time() - t0:.1f} s")
@@ -98,11 +102,19 @@ def pack_runs(runs: list[dict]) -> dict:
         "run_lengths": lens,
         "graphs": np.array([r["graph"] for r in runs], dtype=np.int64).reshape(-1, 3),
         "roots": np.array([r["root"] for r in runs], dtype=np.int64),
+        "edges": np.concatenate([np.full(r["features"].shape[0], r.get("edges", 1), dtype=np.float64)
+                                 for r in runs]) if runs else np.zeros(0),
     }
 
 
-def targets(data: dict) -> np.ndarray:
-    return np.log(np.stack([data["pull_ms"], data["push_ms"]], axis=1)).astype(np.float32)
+def targets(data: dict, per_edge: bool = True) -> np.ndarray:
+    """Regression targets: log edge-phase ms of (pull, push), less log E when
+    per_edge (the choice only depends on their difference, and the per-edge
+    cost varies far less with the graph's size than the time does)."""
+    t = np.log(np.stack([data["pull_ms"], data["push_ms"]], axis=1))
+    if per_edge:
+        t = t - np.log(data["edges"])[:, None]
+    return t.astype(np.float32)
 
 
 def init_params(seed: int) -> np.ndarray:
@@ -116,13 +128,13 @@ def init_params(seed: int) -> np.ndarray:
     return np.concatenate(parts).astype(np.float32)
 
 
-def fit(data: dict, steps: int = 3000, lr: float = 3e-3, seed: int = 7):
+def fit(data: dict, steps: int = 6000, lr: float = 1e-3, seed: int = 7, per_edge: bool = True):
     X = data["features"].astype(np.float32)
     mean = X.mean(axis=0)
     std = X.std(axis=0)
     std[std < 1e-6] = 1.0
     Xn = ((X - mean) / std).astype(np.float32)
-    params, hist = gz.selector_train(init_params(seed), Xn, targets(data), steps, lr)
+    params, hist = gz.selector_train(init_params(seed), Xn, targets(data, per_edge), steps, lr)
     return params, mean.astype(np.float32), std.astype(np.float32), hist
 
 
@@ -177,8 +189,8 @@ def main(argv=None) -> int:
     f = sub.add_parser("fit")
     f.add_argument("--data", required=True)
     f.add_argument("--out", default=DEFAULT_WEIGHTS)
-    f.add_argument("--steps", type=int, default=3000)
-    f.add_argument("--lr", type=float, default=3e-3)
+    f.add_argument("--steps", type=int, default=6000)
+    f.add_argument("--lr", type=float, default=1e-3)
     e = sub.add_parser("evaluate")
     e.add_argument("--data", required=True)
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)

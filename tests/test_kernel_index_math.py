@@ -295,3 +295,60 @@ def test_active_slot_mapping(gz, orc, chunk, density):
                     seen[j] = vi
             a = owners[WAVE - 1]
     assert np.array_equal(seen, want)
+
+
+def emulate_compact_blocks(bitmap_bits, offsets, vpb=2048, block=256):
+    """Replay of the three compaction kernels' index arithmetic: a block
+    covers vpb consecutive vertices, vpb / block per thread; per-block counts, an
+    exclusive scan of the block counts, then each thread's exclusive offset
+    from a block scan of the per-thread counts, written in vertex order."""
+    V = offsets.shape[0] - 1
+    blocks = (V + vpb - 1) // vpb
+    per = vpb // block
+
+    def quad(v0):
+        c = [0] * per
+        if v0 >= V:
+            return c
+        for k in range(per):
+            v = v0 + k
+            if v < V and bitmap_bits[v]:
+                c[k] = int(offsets[min(v + 1, V)] - offsets[min(v, V)])
+        return c
+
+    quads = [[quad(b * vpb + per * t) for t in range(block)] for b in range(blocks)]
+    sums = [(sum(sum(1 for x in q if x) for q in qs), sum(sum(q) for q in qs)) for qs in quads]
+    base_v = np.concatenate([[0], np.cumsum([s[0] for s in sums])]).astype(np.int64)
+    base_c = np.concatenate([[0], np.cumsum([s[1] for s in sums])]).astype(np.int64)
+    n = int(base_v[-1])
+    lst = np.full(n, -1, dtype=np.int64)
+    prefix = np.full(n + 1, -1, dtype=np.int64)
+    prefix[n] = base_c[-1]  # the scan kernel's sentinel
+    for b, qs in enumerate(quads):
+        na = [sum(1 for x in q if x) for q in qs]
+        nc = [sum(q) for q in qs]
+        ea = np.concatenate([[0], np.cumsum(na)])
+        ec = np.concatenate([[0], np.cumsum(nc)])
+        for t, q in enumerate(qs):
+            pos, vpos = int(base_v[b] + ea[t]), int(base_c[b] + ec[t])
+            for k, c in enumerate(q):
+                if c:
+                    assert lst[pos] == -1
+                    lst[pos] = b * vpb + per * t + k
+                    prefix[pos] = vpos
+                    pos += 1
+                    vpos += c
+    return lst, prefix
+
+
+@pytest.mark.parametrize("V,density", [(1, 1.0), (700, 0.3), (1024, 1.0), (2500, 0.05), (3333, 0.9)])
+def test_compaction_blocks_match_definition(gz, orc, V, density):
+    rng = np.random.default_rng(V)
+    pairs = rng.integers(0, V, size=(3 * V, 2), dtype=np.uint64)
+    el = graphs.edge_lists(gz, pairs, V)
+    off = gz.vertex_offsets(orc.pack(el.push, False), V).astype(np.int64)
+    bits = rng.random(V) < density
+    act, prefix = compact(bits, off)
+    lst, pre = emulate_compact_blocks(bits, off)
+    assert np.array_equal(lst, act)
+    assert np.array_equal(pre, prefix)
