@@ -85,6 +85,7 @@ def headline(gz, args) -> dict:
     gz.run(gz.options("pr", iterations=args.steps, damping=DAMPING))
     kt = gz.kernel_times()
     gz.kernel_timing(False)
+    full_other = None if args.only_headline else full_cc_bfs(gz)
     ms_step = r.elapsed_ms / args.steps
     pms, pn = kt["pull_pr"]
     avg = pms / pn
@@ -102,7 +103,37 @@ def headline(gz, args) -> dict:
                      "avg_launch_us": round(avg * 1e3, 2), "launches": pn,
                      "kernel_gteps": round(E / (avg * 1e-3) / 1e9, 2)},
         "rank_sum": float(ranks.sum()),
+        "full_cc_bfs": full_other,
     }
+
+
+def full_cc_bfs(gz) -> dict:
+    """CC and BFS (16 roots, fixed threshold and learned selector) on the
+    already loaded full-scale graph."""
+    import selector_train as st
+
+    E = gz.num_edges()
+    out = {}
+    try:
+        gz.run(gz.options("cc"))
+        r = gz.run(gz.options("cc"))
+        out["cc"] = {"time_to_solution_ms": round(r.elapsed_ms, 2), "iterations": r.iterations,
+                     "pull_iterations": r.pull_iterations, "push_iterations": r.push_iterations,
+                     "gteps": round(E * r.iterations / (r.elapsed_ms * 1e-3) / 1e9, 2)}
+        roots = st.pick_roots(16, 26)
+        gz.run(gz.options("bfs", root=roots[0]))
+        fixed = sum(gz.run(gz.options("bfs", root=x)).elapsed_ms for x in roots)
+        gz.selector_load(st.DEFAULT_WEIGHTS)
+        gz.run(gz.options("bfs", engine="learned", root=roots[0]))
+        learned = sum(gz.run(gz.options("bfs", engine="learned", root=x)).elapsed_ms for x in roots)
+        gz.selector_clear()
+        out["bfs_16_roots"] = {"fixed_threshold_ms": round(fixed, 3), "learned_ms": round(learned, 3),
+                               "learned_speedup": round(fixed / learned, 4),
+                               "fixed_gteps_per_search": round(E / (fixed / len(roots) * 1e-3) / 1e9, 2)}
+    except Exception as exc:  # reported, not hidden
+        out["error"] = f"{type(exc).__name__}: {exc}"
+    log(f"[full_cc_bfs] {json.dumps(out)}")
+    return out
 
 
 def cc_rmat22(gz) -> dict:
@@ -286,6 +317,7 @@ def main() -> int:
         "roofline": h["roofline"],
         "cpu_baseline": cpu,
         "phases_ms_per_iteration": h["phases_ms_per_iteration"],
+        "full_scale_cc_bfs": h["full_cc_bfs"],
         **extra,
     }
     print(json.dumps(line), flush=True)
