@@ -89,6 +89,10 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_bfs(const gz_vec_t* _
                 edges += 1;
                 const uint64_t d = gz_lane_id(l[k]);
                 if (!gz_bit_test(wants_info, d)) continue;
+                /* words only decrease: a value already <= key means a source
+                 * no larger than s reached d at this level, and the atomic
+                 * would change nothing */
+                if (__hip_atomic_load(&words[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= key) continue;
                 const unsigned long long old = atomicMin((unsigned long long*)&words[d], key);
                 if (old == GZ_BFS_UNVISITED) {
                     gz_bit_set(next_frontier, d);
@@ -153,7 +157,11 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_find(const gz_vec
                 d = gz_shared_vertex(v);
                 if (kActive || gz_bit_test(wants_info, d)) {
                     vectors += 1;
-                    cand = gz_vec_has_frontier_source(v, has_info, edges);
+                    /* a smaller qualifying vector of d is already known (from
+                     * another chunk): this one cannot win, skip its gathers */
+                    const uint64_t known = __hip_atomic_load(&first_vector[d], __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT);
+                    if (known > vi) cand = gz_vec_has_frontier_source(v, has_info, edges);
                 }
             }
             const uint64_t d_prev = __shfl_up((unsigned long long)d, 1, GZ_WAVE);
@@ -202,26 +210,45 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_visit(const gz_ve
     const uint64_t n = kActive ? totals[0] : num_vertices;
     uint64_t vote = 0;
     gz_feature_acc feat{0, 0, 0};
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t d = kActive ? list[i] : i;
-        const uint64_t w = first_vector[d];
-        if (w == ~0ull) continue;
-        first_vector[d] = ~0ull;
-        const gz_lanes v = gz_load_vector(vecs, w);
-        uint64_t parent = 0;
-        const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
+    /* wave-uniform loop: the list is in vertex order, so visited vertices of
+     * neighbouring lanes share bitmap words; one lane per run of equal words
+     * applies the whole run's bits with a single atomic per bitmap */
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(uint64_t)(GZ_WAVE - 1)); i0 < n;
+         i0 += stride) {
+        const uint64_t i = i0 + (uint64_t)lane;
+        const uint64_t d = (i < n) ? (kActive ? list[i] : i) : 0;
+        const uint64_t fv = (i < n) ? first_vector[d] : ~0ull;
+        const bool found = fv != ~0ull;
+        if (found) {
+            first_vector[d] = ~0ull;
+            const gz_lanes v = gz_load_vector(vecs, fv);
+            uint64_t parent = 0;
+            const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (gz_lane_valid(l[k])) {
-                const uint64_t s = gz_lane_id(l[k]);
-                if (gz_bit_test(has_info, s) && s > parent) parent = s;
+            for (int k = 0; k < 4; ++k) {
+                if (gz_lane_valid(l[k])) {
+                    const uint64_t s = gz_lane_id(l[k]);
+                    if (gz_bit_test(has_info, s) && s > parent) parent = s;
+                }
             }
+            words[d] = ((uint64_t)level << 48) | parent;
+            vote += gz_frontier_stat(outdeg, d, stat_mode);
+            gz_feature_add(feat, sink, outdeg, d);
         }
-        words[d] = ((uint64_t)level << 48) | parent;
-        gz_bit_clear(wants_info, d);
-        gz_bit_set(next_frontier, d);
-        vote += gz_frontier_stat(outdeg, d, stat_mode);
-        gz_feature_add(feat, sink, outdeg, d);
+        const uint64_t word = found ? (d >> 5) : ~0ull;
+        uint32_t bits = found ? (1u << (d & 31)) : 0u;
+#pragma unroll
+        for (int o = 1; o < GZ_WAVE; o <<= 1) {
+            const uint32_t xb = __shfl_down(bits, o, GZ_WAVE);
+            const uint64_t xw = (uint64_t)__shfl_down((unsigned long long)word, o, GZ_WAVE);
+            if (lane + o < GZ_WAVE && xw == word) bits |= xb;
+        }
+        const uint64_t prev_word = (uint64_t)__shfl_up((unsigned long long)word, 1, GZ_WAVE);
+        if (found && (lane == 0 || prev_word != word)) {
+            atomicAnd(reinterpret_cast<unsigned int*>(wants_info) + word, ~bits);
+            atomicOr(reinterpret_cast<unsigned int*>(next_frontier) + word, bits);
+        }
     }
     const uint64_t total = gz_wave_sum_u64(vote);
     if (lane == 0) partials[gw] = total;
