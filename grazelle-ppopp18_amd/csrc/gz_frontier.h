@@ -21,7 +21,7 @@
 #include "gz_internal.h"
 
 /* Vertices per thread and per block of the compaction passes. */
-#define GZ_COMPACT_VPT 8
+#define GZ_COMPACT_VPT 16
 #define GZ_COMPACT_VPB (GZ_COMPACT_VPT * GZ_BLOCK)
 
 static inline uint64_t gz_compact_blocks(uint64_t num_vertices)

@@ -103,8 +103,11 @@ __global__ __launch_bounds__(1024) void gz_k_frontier_scan(uint64_t* __restrict_
     for (uint64_t base = 0; base < blocks; base += GZ_SCAN_PASS) {
         const uint64_t n = (blocks - base < GZ_SCAN_PASS) ? blocks - base : GZ_SCAN_PASS;
         __syncthreads();
-        for (uint64_t i = threadIdx.x; i < 2 * GZ_SCAN_PASS; i += blockDim.x)
+#pragma unroll
+        for (int r = 0; r < 2 * GZ_SCAN_PASS / 1024; ++r) {
+            const uint64_t i = (uint64_t)r * 1024 + threadIdx.x;
             stage[i] = (i < 2 * n) ? block_sums[2 * base + i] : 0ull;
+        }
         __syncthreads();
         unsigned long long x[2] = {0, 0};
         for (int k = 0; k < per; ++k) {
@@ -146,7 +149,11 @@ __global__ __launch_bounds__(1024) void gz_k_frontier_scan(uint64_t* __restrict_
             off[1] += y1;
         }
         __syncthreads();
-        for (uint64_t i = threadIdx.x; i < 2 * n; i += blockDim.x) block_sums[2 * base + i] = stage[i];
+#pragma unroll
+        for (int r = 0; r < 2 * GZ_SCAN_PASS / 1024; ++r) {
+            const uint64_t i = (uint64_t)r * 1024 + threadIdx.x;
+            if (i < 2 * n) block_sums[2 * base + i] = stage[i];
+        }
         if (threadIdx.x == 0) {
             carry[0] += tot[0];
             carry[1] += tot[1];

@@ -297,7 +297,7 @@ def test_active_slot_mapping(gz, orc, chunk, density):
     assert np.array_equal(seen, want)
 
 
-def emulate_compact_blocks(bitmap_bits, offsets, vpb=2048, block=256):
+def emulate_compact_blocks(bitmap_bits, offsets, vpb=4096, block=256):
     """Replay of the three compaction kernels' index arithmetic: a block
     covers vpb consecutive vertices, vpb / block per thread; per-block counts, an
     exclusive scan of the block counts, then each thread's exclusive offset
