@@ -194,12 +194,16 @@ static void gz_snapshot_restore(const gz_snapshot_t* s)
 }
 
 /* Best of three timed runs of one engine's edge phase, each from the
- * snapshot; the state afterwards is that engine's result. */
-static double gz_probe_engine(int bfs, int pull, const gz_snapshot_t* s, gz_timer_t* t)
+ * snapshot; the state afterwards is that engine's result.  With `stamp` the
+ * last run is also bracketed by the selector's device clock, so the next
+ * iteration's "previous edge-phase time" feature is measured exactly as the
+ * learned engine measures it. */
+static double gz_probe_engine(int bfs, int pull, const gz_snapshot_t* s, gz_timer_t* t, int stamp)
 {
     double best = 0.0;
     for (int rep = 0; rep < 3; ++rep) {
         gz_snapshot_restore(s);
+        if (stamp && rep == 2) gz_sel_dev_stamp(bfs && gz_phase_config.push_frontier_skip);
         gz_timer_start(t);
         phase_op_reset_global_accum();
         gz_edge_phase(bfs, pull);
@@ -289,7 +293,7 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
         const uint64_t root = bfs ? gz_graph_device_vertex(o->root) : V;
         gz_sel_dev_begin(V, E, root, indeg);
         gz_sel_dev_step(gz_graph.frontier_has_info, indeg, gz_reduce_buffer, learned, bfs, 1, 0, 0.0f,
-                        bfs ? GZ_SEL_FROM_ROOT : GZ_SEL_FROM_BITMAP);
+                        bfs ? GZ_SEL_FROM_ROOT : GZ_SEL_FROM_BITMAP, 0);
         (void)phase_op_combine_global_var_from_buf(gz_reduce_buffer);
         gz_selector_read(&sel);
         /* BFS activates each vertex once: its edge phases fold the statistics
@@ -327,15 +331,14 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
         if (probe) {
             /* the chosen engine runs last, so its result is what continues */
             gz_snapshot_take(&snap);
-            const double other = gz_probe_engine(bfs, !use_pull, &snap, t);
-            const double chosen = gz_probe_engine(bfs, use_pull, &snap, t);
+            const double other = gz_probe_engine(bfs, !use_pull, &snap, t, 0);
+            const double chosen = gz_probe_engine(bfs, use_pull, &snap, t, 1);
             rec.pull_ms = use_pull ? chosen : other;
             rec.push_ms = use_pull ? other : chosen;
             rec.edge_ms = chosen;
-            prev_ms = (float)chosen;
         } else {
             if (t) gz_timer_start(t);
-            if (learned) gz_sel_dev_stamp();
+            if (learned) gz_sel_dev_stamp(bfs && gz_phase_config.push_frontier_skip);
             phase_op_reset_global_accum();
             gz_edge_phase(bfs, use_pull);
         }
@@ -343,12 +346,16 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
             r->pull_iterations += 1;
         else
             r->push_iterations += 1;
-        phase_op_write_global_accum_to_buf(gz_reduce_buffer);
+        /* with BFS statistics folded into the edge phase, the selector's
+         * decision kernel also sums the vote */
+        const int vote_in_step = features && bfs;
+        if (!vote_in_step) phase_op_write_global_accum_to_buf(gz_reduce_buffer);
         if (t && !probe) gz_timer_stop(t);
         if (features) {
             /* the vertices this phase activated are the next frontier */
             gz_sel_dev_step((const uint64_t*)gz_graph.vertex_accumulators, indeg, gz_reduce_buffer, learned, bfs,
-                            ctr + 1, rec.engine, prev_ms, bfs ? GZ_SEL_FROM_EDGE_PHASE : GZ_SEL_FROM_BITMAP);
+                            ctr + 1, rec.engine, prev_ms, bfs ? GZ_SEL_FROM_EDGE_PHASE : GZ_SEL_FROM_BITMAP,
+                            vote_in_step);
         }
         const uint64_t next_vote = phase_op_combine_global_var_from_buf(gz_reduce_buffer);
         if (features) gz_selector_read(&sel);

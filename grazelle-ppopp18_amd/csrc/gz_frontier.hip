@@ -59,9 +59,11 @@ __device__ __forceinline__ T gz_block_exclusive_scan(T x, T* lds, T& block_total
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_count(const uint64_t* __restrict__ bitmap,
                                                                  const uint64_t* __restrict__ offsets,
                                                                  uint64_t num_vertices,
-                                                                 uint64_t* __restrict__ block_sums)
+                                                                 uint64_t* __restrict__ block_sums,
+                                                                 uint64_t* __restrict__ stamp)
 {
     __shared__ unsigned long long lds[GZ_WAVES_PER_BLOCK];
+    if (stamp != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *stamp = (uint64_t)wall_clock64();
     const uint64_t v0 = (uint64_t)blockIdx.x * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
     uint64_t c[GZ_COMPACT_VPT];
     gz_run_counts(bitmap, offsets, num_vertices, v0, c);
@@ -208,7 +210,8 @@ void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64
     }
     gz_ktimer_begin(GZ_K_COMPACT);
     hipLaunchKernelGGL(gz_k_frontier_count, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, offsets,
-                       num_vertices, gz_scratch.active_block_sums);
+                       num_vertices, gz_scratch.active_block_sums, gz_scratch.stamp_target);
+    gz_scratch.stamp_target = nullptr;
     GZ_LAUNCH_CHECK();
     hipLaunchKernelGGL(gz_k_frontier_scan, dim3(1), dim3(1024), 0, gz_stream, gz_scratch.active_block_sums, blocks,
                        gz_scratch.active_totals, gz_scratch.active_prefix);

@@ -52,6 +52,7 @@ typedef struct gz_scratch_t {
     uint64_t* bfs_first_vector;  /* [V] first qualifying pull vector of each dst, ~0 when none */
     uint64_t* features;          /* [3 * partials_capacity] selector features per wave (BFS edge phases) */
     const uint32_t* features_indeg; /* in-degrees when BFS edge phases fold the features, else nullptr */
+    uint64_t* stamp_target;      /* when set, the next compaction writes the wall clock there (once) */
     uint64_t scratch_vertices;   /* V the vertex-sized buffers were sized for */
 } gz_scratch_t;
 

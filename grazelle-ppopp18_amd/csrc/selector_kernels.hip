@@ -210,38 +210,44 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_stats(const uint64_t* __res
  * features as 8 floats, out[6] = the two outputs as floats.  The unvisited
  * counts advance by the frontier (BFS marks each vertex once). */
 __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restrict__ partials, uint64_t blocks,
+                                                            const uint64_t* __restrict__ votes,
                                                             gz_sel_dev_state* st, const float* __restrict__ p,
                                                             const float* __restrict__ norm, int have_model, int bfs,
                                                             uint64_t num_vertices, uint64_t num_edges,
                                                             float iteration, float prev_engine, float prev_ms,
                                                             float clock_khz, uint64_t* __restrict__ out)
 {
-    __shared__ uint64_t red[3][1024 / GZ_WAVE];
+    __shared__ uint64_t red[4][1024 / GZ_WAVE];
     __shared__ float x[GZ_SEL_IN], xn[GZ_SEL_IN], h1[GZ_SEL_H], h2[GZ_SEL_H], y[GZ_SEL_OUT];
     const int j = threadIdx.x;
     const uint64_t t_end = (uint64_t)wall_clock64();
-    uint64_t s0 = 0, s1 = 0, s2 = 0;
+    uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
     for (uint64_t b = j; b < blocks; b += blockDim.x) {
         s0 += partials[3 * (uint64_t)b];
         s1 += partials[3 * (uint64_t)b + 1];
         s2 += partials[3 * (uint64_t)b + 2];
+        if (votes) s3 += votes[b];
     }
     s0 = gz_wave_sum_u64(s0);
     s1 = gz_wave_sum_u64(s1);
     s2 = gz_wave_sum_u64(s2);
+    s3 = gz_wave_sum_u64(s3);
     if (gz_lane() == 0) {
         red[0][j / GZ_WAVE] = s0;
         red[1][j / GZ_WAVE] = s1;
         red[2][j / GZ_WAVE] = s2;
+        red[3][j / GZ_WAVE] = s3;
     }
     __syncthreads();
     if (j == 0) {
-        uint64_t cv = 0, ce = 0, ci = 0;
+        uint64_t cv = 0, ce = 0, ci = 0, vote = 0;
         for (int i = 0; i < (int)blockDim.x / GZ_WAVE; ++i) {
             cv += red[0][i];
             ce += red[1][i];
             ci += red[2][i];
+            vote += red[3][i];
         }
+        if (votes) out[0] = vote;
         uint64_t uv = num_vertices, ue = num_edges;
         if (bfs) {
             st->unvisited -= cv;
@@ -364,14 +370,23 @@ extern "C" void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges, uint
     GZ_LAUNCH_CHECK();
 }
 
-/* Wall-clock stamp taken before an edge phase (learned engine). */
-extern "C" void gz_sel_dev_stamp(void)
+/* Wall-clock stamp taken before an edge phase (learned engine).  With
+ * `by_compaction` the edge phase's first kernel (the frontier compaction)
+ * takes it, which saves a launch; otherwise a one-thread kernel does. */
+extern "C" void gz_sel_dev_stamp(int by_compaction)
 {
+    if (by_compaction) {
+        gz_scratch.stamp_target = &gz_sel_dev_st->t_begin;
+        return;
+    }
     hipLaunchKernelGGL(gz_k_sel_stamp, dim3(1), dim3(64), 0, gz_stream, gz_sel_dev_st);
     GZ_LAUNCH_CHECK();
 }
 
-/* Features and (with a model) the decision into reduce_buffer[1..6].  The
+/* Features and (with a model) the decision into reduce_buffer[1..6]; with
+ * `with_vote` (edge-phase statistics only) also the converge vote into
+ * reduce_buffer[0], summed from the edge phase's per-wave vote partials in
+ * place of phase_op_write_global_accum_to_buf.  The
  * frontier statistics come from: GZ_SEL_FROM_EDGE_PHASE, the last BFS edge
  * phase's per-wave sums (gz_scratch.features, folded by the kernels);
  * GZ_SEL_FROM_ROOT, the root staged by gz_sel_dev_begin; GZ_SEL_FROM_BITMAP,
@@ -379,7 +394,7 @@ extern "C" void gz_sel_dev_stamp(void)
  * previous edge phase's time from the device clock (stamp to the decision). */
 extern "C" void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg, uint64_t* reduce_buffer,
                                 int have_model, int bfs, uint64_t iteration, uint32_t prev_engine, float prev_ms,
-                                int source)
+                                int source, int with_vote)
 {
     if (have_model && gz_sel_dev_params == nullptr) have_model = 0;
     const uint64_t V = gz_graph.num_vertices;
@@ -396,7 +411,10 @@ extern "C" void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg,
         GZ_LAUNCH_CHECK();
         feat = gz_sel_dev_partials;
     }
-    hipLaunchKernelGGL(gz_k_sel_decide, dim3(1), dim3(1024), 0, gz_stream, feat, blocks, gz_sel_dev_st,
+    const uint64_t* votes = (with_vote && source == GZ_SEL_FROM_EDGE_PHASE && !gz_scratch.partials_is_double)
+                                ? (const uint64_t*)gz_scratch.partials
+                                : nullptr;
+    hipLaunchKernelGGL(gz_k_sel_decide, dim3(1), dim3(1024), 0, gz_stream, feat, blocks, votes, gz_sel_dev_st,
                        gz_sel_dev_params, gz_sel_dev_norm, have_model, bfs, V, gz_graph.num_edges, (float)iteration,
                        (float)prev_engine, prev_ms, gz_sel_clock_khz, reduce_buffer);
     GZ_LAUNCH_CHECK();

@@ -10,12 +10,11 @@ visits the same frontiers whichever engine runs each level, so one probed
 search gives the per-level cost of both engines along the only trajectory
 there is, and any policy's total can be read off it.
 
-Training minimises the squared error of the predicted log times per edge
-(log ms - log E: the choice depends only on the difference of the two
-outputs, and the per-edge cost varies far less with graph size than the
-time, which lets a network trained on scales 18-22 choose well at scale 24)
-with the library's own device kernels (gz_selector_train: forward, backward
-and Adam on the GPU).
+Training minimises the squared error of the predicted log times with the
+library's own device kernels (gz_selector_train: forward, backward and Adam
+on the GPU).  `fit --per-edge` regresses log ms - log E instead (the choice
+depends only on the difference of the two outputs); which of the two is
+committed is decided on held-out graphs of the training distribution.
 
     python selector_train.py collect --graphs 200 --out data.npz
     python selector_train.py fit --data data.npz --out selector_bfs.bin
@@ -107,7 +106,7 @@ def pack_runs(runs: list[dict]) -> dict:
     }
 
 
-def targets(data: dict, per_edge: bool = True) -> np.ndarray:
+def targets(data: dict, per_edge: bool = False) -> np.ndarray:
     """Regression targets: log edge-phase ms of (pull, push), less log E when
     per_edge (the choice only depends on their difference, and the per-edge
     cost varies far less with the graph's size than the time does)."""
@@ -128,7 +127,7 @@ def init_params(seed: int) -> np.ndarray:
     return np.concatenate(parts).astype(np.float32)
 
 
-def fit(data: dict, steps: int = 6000, lr: float = 1e-3, seed: int = 7, per_edge: bool = True):
+def fit(data: dict, steps: int = 6000, lr: float = 1e-3, seed: int = 7, per_edge: bool = False):
     X = data["features"].astype(np.float32)
     mean = X.mean(axis=0)
     std = X.std(axis=0)
@@ -141,7 +140,10 @@ def fit(data: dict, steps: int = 6000, lr: float = 1e-3, seed: int = 7, per_edge
 def policy_times(data: dict, params=None, mean=None, std=None) -> dict:
     """Total edge-phase ms of the fixed threshold, the learned selector and
     the per-level best, summed over the probed runs.  The learned policy sees
-    its own previous choice and time (features 5 and 6)."""
+    its own previous choice and time (features 5 and 6): where it chose what
+    the fixed threshold chose, the recorded feature (the device-clock time of
+    that run) is exact; otherwise the other engine's time plus the same
+    clock overhead stands in for it."""
     X = data["features"].astype(np.float32)
     tp, ts = data["pull_ms"], data["push_ms"]
     fixed = np.where(data["fixed"] == 1, tp, ts)
@@ -159,7 +161,11 @@ def policy_times(data: dict, params=None, mean=None, std=None) -> dict:
                 pull = y[0] < y[1]
                 t = tp[i] if pull else ts[i]
                 total += t
-                prev_engine, prev_ms = (1.0 if pull else 2.0), t
+                chose_fixed = (1 if pull else 2) == int(data["fixed"][i])
+                if i + 1 < start + int(n):
+                    recorded = float(np.expm1(X[i + 1][6]))
+                    prev_ms = recorded if chose_fixed else max(0.0, recorded - float(fixed[i]) + t)
+                prev_engine = 1.0 if pull else 2.0
             start += int(n)
         out["learned"] = total
     return out
@@ -191,6 +197,7 @@ def main(argv=None) -> int:
     f.add_argument("--out", default=DEFAULT_WEIGHTS)
     f.add_argument("--steps", type=int, default=6000)
     f.add_argument("--lr", type=float, default=1e-3)
+    f.add_argument("--per-edge", action="store_true", help="regress log ms - log E")
     e = sub.add_parser("evaluate")
     e.add_argument("--data", required=True)
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
@@ -202,7 +209,7 @@ def main(argv=None) -> int:
         print(json.dumps({"samples": int(data["features"].shape[0]), "runs": int(data["run_lengths"].shape[0])}))
     elif a.cmd == "fit":
         data = dict(np.load(a.data))
-        params, mean, std, hist = fit(data, a.steps, a.lr)
+        params, mean, std, hist = fit(data, a.steps, a.lr, per_edge=a.per_edge)
         save_weights(a.out, params, mean, std)
         print(json.dumps({"samples": int(data["features"].shape[0]), "loss_first": float(hist[0]),
                           "loss_last": float(hist[-1]), **policy_times(data, params, mean, std)}))
