@@ -54,6 +54,8 @@ void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg, uint64_t* 
 /* BFS edge phases fold the selector's frontier statistics into their vote
  * pass while indeg != NULL (NULL turns it off). */
 void gz_features_fold(const uint32_t* indeg);
+/* Free the source-sliced layout (graph release). */
+void gz_source_slices_release(void);
 void gz_clear_bitmap_range(uint64_t* bitmap, uint64_t vertex_start, uint64_t vertex_count);
 
 #ifdef __cplusplus

@@ -179,6 +179,13 @@ unsigned gz_grid_for(uint64_t n)
     return (unsigned)(blocks < 65536 ? (blocks ? blocks : 1) : 65536);
 }
 
+/* Slice size actually used: at most V / 8, so the eight windows fit. */
+uint64_t gz_slices_effective(uint64_t slice_vertices, uint64_t nv)
+{
+    const uint64_t cap = nv / GZ_XCDS;
+    return slice_vertices < cap ? slice_vertices : cap;
+}
+
 uint32_t gz_id_bits(uint64_t nv)
 {
     uint32_t b = 1;
@@ -250,14 +257,18 @@ __global__ void gz_k_degree_order(const uint32_t* __restrict__ cnt, uint64_t nv,
         order[v] = ((uint64_t)(0xffffffffu - cnt[v]) << 32) | v;
 }
 
-__global__ void gz_k_permutation(const uint64_t* __restrict__ order, uint64_t nv, uint64_t* __restrict__ old_of_new,
-                                 uint32_t* __restrict__ new_of_old)
+/* Rank i of the degree order becomes id i, except that the first 8*S ranks
+ * are dealt round robin into eight S-id windows: rank i -> (i % 8) * S + i / 8. */
+__global__ void gz_k_permutation(const uint64_t* __restrict__ order, uint64_t nv, uint64_t slice_vertices,
+                                 uint64_t* __restrict__ old_of_new, uint32_t* __restrict__ new_of_old)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t hot = (uint64_t)GZ_XCDS * slice_vertices;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
         const uint64_t old = order[i] & 0xffffffffull;
-        old_of_new[i] = old;
-        new_of_old[old] = (uint32_t)i;
+        const uint64_t id = (i < hot) ? (i % GZ_XCDS) * slice_vertices + i / GZ_XCDS : i;
+        old_of_new[id] = old;
+        new_of_old[old] = (uint32_t)id;
     }
 }
 
@@ -277,7 +288,7 @@ __global__ void gz_k_remap_keys(uint64_t* __restrict__ keys, uint64_t n, uint32_
  * indexed by new id.  The hot sources of a power-law graph then share cache
  * lines, which is what the pull gathers need once the vertex array no longer
  * fits in the caches. */
-uint64_t* gz_relabel_by_outdegree(uint64_t* keys, uint64_t n, uint32_t b, uint64_t nv)
+uint64_t* gz_relabel_by_outdegree(uint64_t* keys, uint64_t n, uint32_t b, uint64_t nv, uint64_t slice_vertices)
 {
     uint32_t* cnt = (uint32_t*)gz_device_malloc(nv * sizeof(uint32_t));
     GZ_CHECK(hipMemsetAsync(cnt, 0, nv * sizeof(uint32_t), gz_stream));
@@ -293,8 +304,8 @@ uint64_t* gz_relabel_by_outdegree(uint64_t* keys, uint64_t n, uint32_t b, uint64
     if (nv > 1) gz_sort_keys(order, order_b, nv, 64, &sorted);
     uint64_t* old_of_new = (uint64_t*)gz_device_malloc((nv + 8) * sizeof(uint64_t));
     uint32_t* new_of_old = cnt; /* counts are no longer needed */
-    hipLaunchKernelGGL(gz_k_permutation, dim3(gz_grid_for(nv)), dim3(GZ_BLOCK), 0, gz_stream, sorted, nv, old_of_new,
-                       new_of_old);
+    hipLaunchKernelGGL(gz_k_permutation, dim3(gz_grid_for(nv)), dim3(GZ_BLOCK), 0, gz_stream, sorted, nv,
+                       slice_vertices, old_of_new, new_of_old);
     GZ_LAUNCH_CHECK();
     if (n) {
         hipLaunchKernelGGL(gz_k_remap_keys, dim3(gz_grid_for(n)), dim3(GZ_BLOCK), 0, gz_stream, keys, n, b, new_of_old);
@@ -311,8 +322,10 @@ uint64_t* gz_relabel_by_outdegree(uint64_t* keys, uint64_t n, uint32_t b, uint64
  * duplicates, build both layouts and hand them to the graph record. */
 void gz_ingress_from_keys(uint64_t nv, uint64_t* a, uint64_t* b_buf, uint64_t n, uint32_t b, uint32_t flags)
 {
+    const uint32_t slice_log2 = (flags >> 8) & 0x3fu;
+    const uint64_t slice_vertices = slice_log2 ? gz_slices_effective(1ull << slice_log2, nv) : 0;
     uint64_t* old_of_new = nullptr;
-    if (flags & GZ_INGRESS_RELABEL_BY_OUTDEGREE) old_of_new = gz_relabel_by_outdegree(a, n, b, nv);
+    if (flags & GZ_INGRESS_RELABEL_BY_OUTDEGREE) old_of_new = gz_relabel_by_outdegree(a, n, b, nv, slice_vertices);
     uint64_t* sorted = a;
     if (n > 1) gz_sort_keys(a, b_buf, n, 2 * b, &sorted);
     uint64_t* other = (sorted == a) ? b_buf : a;
@@ -353,6 +366,7 @@ void gz_ingress_from_keys(uint64_t nv, uint64_t* a, uint64_t* b_buf, uint64_t n,
     GZ_CHECK(hipStreamSynchronize(gz_stream));
     gz_device_free(eoff);
     gz_graph_adopt_device(&L);
+    if (slice_log2) gz_graph_build_source_slices(1ull << slice_log2);
 }
 
 void gz_check_vertex_count(uint64_t nv)
@@ -474,5 +488,144 @@ extern "C" int gz_graph_build_rmat_union_device(const gz_rmat_params_t* params, 
     gz_ingress_from_keys(nv, a, bb, n, b, 0);
     gz_device_free(a);
     gz_device_free(bb);
+    return 0;
+}
+
+/* ---- source-sliced pull layout ---------------------------------------------
+ * From the loaded pull vectors: every valid lane becomes the key
+ * (slice << 2b | dst << b | src) (invalid lanes all ones), a stable radix pass
+ * over the 4 slice bits groups the keys by slice while keeping the (dst, src)
+ * order, and each slice's keys are packed by the layout builder above. */
+namespace {
+
+__global__ void gz_k_slice_keys(const gz_vec_t* __restrict__ vecs, uint64_t nvec, uint32_t b, uint64_t slice_vertices,
+                                uint64_t* __restrict__ keys)
+{
+    const uint64_t hot = (uint64_t)GZ_XCDS * slice_vertices;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 4 * nvec; i += stride) {
+        const gz_vec_t* v = vecs + i / 4;
+        const uint64_t l = v->lane[i % 4];
+        if (!gz_lane_valid(l)) {
+            keys[i] = ~0ull;
+            continue;
+        }
+        gz_lanes w;
+        w.l0 = v->lane[0];
+        w.l1 = v->lane[1];
+        w.l2 = v->lane[2];
+        w.l3 = v->lane[3];
+        const uint64_t d = gz_shared_vertex(w);
+        const uint64_t s = gz_lane_id(l);
+        const uint64_t x = (s < hot) ? s / slice_vertices : (uint64_t)GZ_XCDS;
+        keys[i] = (x << (2 * b)) | (d << b) | s;
+    }
+}
+
+/* first[x] = first key whose slice field is >= x, x in [0, GZ_SLICES] */
+__global__ void gz_k_slice_bounds(const uint64_t* __restrict__ keys, uint64_t n, uint32_t b, uint64_t* __restrict__ first)
+{
+    const uint64_t x = threadIdx.x;
+    if (x > GZ_SLICES) return;
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = lo + ((hi - lo) >> 1);
+        if ((keys[mid] >> (2 * b)) < x)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    first[x] = lo;
+}
+
+__global__ void gz_k_strip_slice(uint64_t* __restrict__ keys, uint64_t n, uint32_t b)
+{
+    const uint64_t mask = (1ull << (2 * b)) - 1ull;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) keys[i] &= mask;
+}
+
+__global__ void gz_k_slice_mask(const uint64_t* __restrict__ voff, uint64_t nv, uint32_t bit, uint16_t* __restrict__ mask)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride)
+        if (voff[v + 1] > voff[v]) mask[v] |= (uint16_t)(1u << bit);
+}
+
+}  // namespace
+
+extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices)
+{
+    gz_runtime_init(0);
+    if (gz_graph.pull_vectors == nullptr) {
+        fprintf(stderr, "grazelle: the source-sliced layout needs a pull layout\n");
+        exit(65);
+    }
+    const uint64_t nv = gz_graph.num_vertices, nvec = gz_graph.pull_vector_count;
+    const uint32_t b = gz_id_bits(nv);
+    if (2 * b + 4 > 64) {
+        fprintf(stderr, "grazelle: the source-sliced layout needs V <= 2^30 (V = %llu)\n", (unsigned long long)nv);
+        exit(65);
+    }
+    gz_source_slices_release();
+    const uint64_t S = gz_slices_effective(slice_vertices, nv);
+    const uint64_t n = 4 * nvec;
+    uint64_t* a = (uint64_t*)gz_device_malloc((n ? n : 1) * sizeof(uint64_t));
+    uint64_t* bb = (uint64_t*)gz_device_malloc((n ? n : 1) * sizeof(uint64_t));
+    uint64_t* first = (uint64_t*)gz_device_malloc((GZ_SLICES + 1) * sizeof(uint64_t));
+    uint64_t lo[GZ_SLICES + 1];
+    uint64_t* sorted = a;
+    if (n) {
+        hipLaunchKernelGGL(gz_k_slice_keys, dim3(gz_grid_for(n)), dim3(GZ_BLOCK), 0, gz_stream, gz_graph.pull_vectors,
+                           nvec, b, S, a);
+        GZ_LAUNCH_CHECK();
+        rocprim::double_buffer<uint64_t> db(a, bb);
+        size_t tmp_bytes = 0;
+        GZ_CHECK(rocprim::radix_sort_keys(nullptr, tmp_bytes, db, (size_t)n, 2 * b, 2 * b + 4, gz_stream));
+        void* tmp = gz_device_malloc(tmp_bytes ? tmp_bytes : 1);
+        GZ_CHECK(rocprim::radix_sort_keys(tmp, tmp_bytes, db, (size_t)n, 2 * b, 2 * b + 4, gz_stream));
+        GZ_CHECK(hipStreamSynchronize(gz_stream));
+        gz_device_free(tmp);
+        sorted = db.current();
+        hipLaunchKernelGGL(gz_k_slice_bounds, dim3(1), dim3(64), 0, gz_stream, sorted, n, b, first);
+        GZ_LAUNCH_CHECK();
+        gz_copy_to_host(lo, first, sizeof(lo));
+        hipLaunchKernelGGL(gz_k_strip_slice, dim3(gz_grid_for(lo[GZ_SLICES])), dim3(GZ_BLOCK), 0, gz_stream, sorted,
+                           lo[GZ_SLICES], b);
+        GZ_LAUNCH_CHECK();
+    } else {
+        for (int x = 0; x <= GZ_SLICES; ++x) lo[x] = 0;
+    }
+    gz_slices.mask = (uint16_t*)gz_device_malloc((nv + 8) * sizeof(uint16_t));
+    uint64_t* eoff = (uint64_t*)gz_device_malloc((nv + 1) * sizeof(uint64_t));
+    gz_vec_t* part_vecs[GZ_SLICES];
+    uint64_t counts[GZ_SLICES];
+    for (int x = 0; x < GZ_SLICES; ++x) {
+        uint64_t* voff = nullptr;
+        gz_build_layout(sorted + lo[x], lo[x + 1] - lo[x], b, nv, &part_vecs[x], &voff, &counts[x], eoff);
+        hipLaunchKernelGGL(gz_k_slice_mask, dim3(gz_grid_for(nv)), dim3(GZ_BLOCK), 0, gz_stream, voff, nv, (uint32_t)x,
+                           gz_slices.mask);
+        GZ_LAUNCH_CHECK();
+        GZ_CHECK(hipStreamSynchronize(gz_stream));
+        gz_device_free(voff);
+    }
+    gz_device_free(eoff);
+    gz_device_free(first);
+    gz_device_free(a);
+    gz_device_free(bb);
+    gz_slices.begin[0] = 0;
+    for (int x = 0; x < GZ_SLICES; ++x) gz_slices.begin[x + 1] = gz_slices.begin[x] + counts[x];
+    const uint64_t total = gz_slices.begin[GZ_SLICES];
+    gz_slices.vecs = (gz_vec_t*)gz_device_malloc((total ? total : 1) * sizeof(gz_vec_t));
+    for (int x = 0; x < GZ_SLICES; ++x) {
+        if (counts[x]) gz_device_copy(gz_slices.vecs + gz_slices.begin[x], part_vecs[x], counts[x] * sizeof(gz_vec_t));
+        GZ_CHECK(hipStreamSynchronize(gz_stream));
+        gz_device_free(part_vecs[x]);
+    }
+    gz_slices.part_stride = nv + 8;
+    gz_slices.parts = (double*)gz_device_malloc(GZ_SLICES * gz_slices.part_stride * sizeof(double));
+    gz_slices.base_list = gz_graph.pull_vectors;
+    gz_slices.base_count = nvec;
+    gz_slices.slice_vertices = S;
     return 0;
 }

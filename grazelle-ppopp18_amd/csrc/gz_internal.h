@@ -58,6 +58,42 @@ typedef struct gz_scratch_t {
 
 extern gz_scratch_t gz_scratch;
 extern hipStream_t gz_stream;
+extern uint32_t gz_cu_count; /* compute units of the device */
+
+/* ---- source-sliced pull layout, owned by gz_slices.hip -------------------
+ * The pull vectors split by source: hot slice x < GZ_XCDS holds the edges
+ * whose source lies in [x*S, (x+1)*S), the last slice every other edge.  Each
+ * slice is an ordinary pull list (dst-grouped), and the slices are stored one
+ * after another.  Workgroups are dispatched to the XCDs round robin, so a
+ * chunk table that gives XCD x only slice x's chunks (plus a share of the
+ * last slice) keeps each XCD's gathers inside one S-vertex window that its
+ * own L2 can hold. */
+#define GZ_XCDS 8
+#define GZ_SLICES (GZ_XCDS + 1)
+
+typedef struct gz_source_slices_t {
+    const gz_vec_t* base_list;        /* the pull list the layout was built from */
+    uint64_t base_count;
+    uint64_t slice_vertices;          /* S */
+    gz_vec_t* vecs;                   /* all slices, concatenated */
+    uint64_t begin[GZ_SLICES + 1];    /* first vector of each slice in vecs */
+    uint16_t* mask;                   /* [V] bit x: the vertex has vectors in slice x */
+    double* parts;                    /* slice x's accumulators at parts + x * part_stride */
+    uint64_t part_stride;
+    gz_slice_chunk_t* table;          /* chunk slots, built for table_chunk_len */
+    uint64_t slots;
+    uint64_t table_chunk_len;
+    uint32_t table_cold_spread;
+    mergeaccum_t* merge;              /* one record per chunk, in slice order */
+    uint64_t records;
+    uint64_t xcd_chunks[GZ_XCDS];     /* chunks in each XCD's slot sequence */
+    uint32_t* counters;               /* per-XCD work counters of the persistent pull */
+} gz_source_slices_t;
+
+extern gz_source_slices_t gz_slices;
+/* PageRank pull over gz_slices (pull and per-slice merge; the vertex phase
+ * sums the slices). */
+void gz_pull_pr_sliced(const gz_vec_t* edge_list, uint64_t edge_list_count);
 
 /* Bracket a kernel launch for gz_kernel_timing_*; no-ops when disabled. */
 void gz_ktimer_begin(int id);

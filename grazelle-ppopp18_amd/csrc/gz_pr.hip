@@ -25,6 +25,9 @@
  * atomics, so the following vertex phase must zero them
  * (phase_vertex_pr.asm:101-113). */
 static uint32_t gz_pr_acc_needs_zero = 0;
+/* Set by the source-sliced pull: the vertex phase sums the slices'
+ * accumulators instead of reading the accumulator array. */
+static uint32_t gz_pr_acc_in_slices = 0;
 
 template <bool kSchedAware, int VPL>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr(const gz_vec_t* __restrict__ vecs, uint64_t nvec,
@@ -94,6 +97,13 @@ extern "C" void perform_edge_pull_phase_pr(const gz_vec_t* edge_list, uint64_t e
     gz_scratch.partials_count = nchunks;
     gz_scratch.stats_count = gz_phase_config.collect_stats ? nchunks : 0;
     if (nchunks == 0) return;
+    if (gz_phase_config.pull_variant == GZ_PULL_SOURCE_SLICED) {
+        gz_pr_acc_needs_zero = 0;
+        gz_pr_acc_in_slices = 1;
+        gz_pull_pr_sliced(edge_list, edge_list_count);
+        return;
+    }
+    gz_pr_acc_in_slices = 0;
     const bool sa = gz_phase_config.pull_variant == GZ_PULL_SCHED_AWARE;
     gz_pr_acc_needs_zero = sa ? 0u : 1u;
     if (sa && gz_merge_buffer_count < nchunks) {
@@ -155,6 +165,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_pr(const gz_vec_t* __
 extern "C" void perform_edge_push_phase_pr(const gz_vec_t* edge_list, uint64_t edge_list_count)
 {
     gz_pr_acc_needs_zero = 1;
+    gz_pr_acc_in_slices = 0;
     const uint64_t nchunks = gz_chunks_for(edge_list_count);
     gz_scratch_reserve(nchunks, gz_graph.num_vertices);
     gz_scratch.partials_is_double = 1;
@@ -172,14 +183,37 @@ extern "C" void perform_edge_push_phase_pr(const gz_vec_t* edge_list, uint64_t e
     gz_ktimer_end(GZ_K_PUSH_PR);
 }
 
+/* acc[v] after a source-sliced pull: the sums of the slices holding v, added
+ * in slice order (all loads issued before the adds); 0 with no in-edges. */
+__device__ __forceinline__ double gz_slices_sum(const uint16_t* __restrict__ mask, const double* __restrict__ parts,
+                                                uint64_t part_stride, uint64_t v)
+{
+    const uint32_t m = mask[v];
+    double p[GZ_SLICES];
+#pragma unroll
+    for (int x = 0; x < GZ_SLICES; ++x) p[x] = ((m >> x) & 1u) ? parts[(uint64_t)x * part_stride + v] : 0.0;
+    double s = 0.0;
+    bool first = true;
+#pragma unroll
+    for (int x = 0; x < GZ_SLICES; ++x)
+        if ((m >> x) & 1u) {
+            s = first ? p[x] : s + p[x];
+            first = false;
+        }
+    return s;
+}
+
 /* props[v] = ((acc[v] + (1-S)/V) * d + (1-d)/V) / outdeg'(v), outdeg' = V
  * for sinks, as separate IEEE mul/add/div (this file is built with
  * -ffp-contract=off, matching the asm's vaddpd/vmulpd/vaddpd/vdivpd). */
+template <bool kSliced>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr(double* __restrict__ props, double* __restrict__ acc,
                                                            const double* __restrict__ outdeg, uint64_t start,
                                                            uint64_t end, double num_vertices,
                                                            const uint64_t* __restrict__ reduce_buffer,
-                                                           double damping, uint32_t zero_acc)
+                                                           double damping, uint32_t zero_acc,
+                                                           const uint16_t* __restrict__ mask,
+                                                           const double* __restrict__ parts, uint64_t part_stride)
 {
     const uint64_t sbits = reduce_buffer[0];
     const double S = __longlong_as_double((long long)sbits);
@@ -189,8 +223,13 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr(double* __restrict__ 
          v += (uint64_t)gridDim.x * blockDim.x) {
         double od = outdeg[v];
         if (od == 0.0) od = num_vertices;
-        const double a = acc[v];
-        if (zero_acc) acc[v] = 0.0;
+        double a;
+        if (kSliced) {
+            a = gz_slices_sum(mask, parts, part_stride, v);
+        } else {
+            a = acc[v];
+            if (zero_acc) acc[v] = 0.0;
+        }
         double x = a + c;
         x = x * damping;
         x = x + k;
@@ -205,10 +244,16 @@ extern "C" void perform_vertex_phase_pr(uint64_t vertex_start, uint64_t vertex_c
     uint64_t blocks = (vertex_count + GZ_BLOCK - 1) / GZ_BLOCK;
     if (blocks > 4096) blocks = 4096;
     gz_ktimer_begin(GZ_K_VERTEX_PR);
-    hipLaunchKernelGGL(gz_k_vertex_pr, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
-                       (double*)gz_graph.vertex_props, (double*)gz_graph.vertex_accumulators, gz_graph.vertex_outdegrees,
-                       vertex_start, end, (double)gz_graph.num_vertices, reduce_buffer, gz_phase_config.damping,
-                       gz_pr_acc_needs_zero);
+#define GZ_VERTEX_PR_ARGS                                                                                      \
+    dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, (double*)gz_graph.vertex_props,                        \
+        (double*)gz_graph.vertex_accumulators, gz_graph.vertex_outdegrees, vertex_start, end,                    \
+        (double)gz_graph.num_vertices, reduce_buffer, gz_phase_config.damping, gz_pr_acc_needs_zero,             \
+        (const uint16_t*)gz_slices.mask, (const double*)gz_slices.parts, gz_slices.part_stride
+    if (gz_pr_acc_in_slices && gz_slices.mask != nullptr)
+        hipLaunchKernelGGL(gz_k_vertex_pr<true>, GZ_VERTEX_PR_ARGS);
+    else
+        hipLaunchKernelGGL(gz_k_vertex_pr<false>, GZ_VERTEX_PR_ARGS);
+#undef GZ_VERTEX_PR_ARGS
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_VERTEX_PR);
 }

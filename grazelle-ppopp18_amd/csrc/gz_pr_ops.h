@@ -7,8 +7,18 @@
 
 #include "gz_internal.h"
 
+/* One property gather; kStream marks it nontemporal, so that gathers known
+ * to miss do not evict lines that other gathers reuse. */
+template <bool kStream>
+__device__ __forceinline__ double gz_gather_prop(const double* __restrict__ props, uint64_t i)
+{
+    if (kStream) return __builtin_nontemporal_load(props + i);
+    return props[i];
+}
+
 /* Sum of the gathered properties of one vector, in the lane order of the
  * asm's vhaddpd / vextractf128 / vaddpd sequence: (g2+g3)+(g0+g1). */
+template <bool kStream = false>
 __device__ __forceinline__ double gz_pr_vector_sum(const gz_lanes& v, const double* __restrict__ props, uint32_t& edges,
                                                    bool in_range = true)
 {
@@ -16,10 +26,10 @@ __device__ __forceinline__ double gz_pr_vector_sum(const gz_lanes& v, const doub
      * loads issue back to back */
     const bool v0 = in_range && gz_lane_valid(v.l0), v1 = in_range && gz_lane_valid(v.l1),
                v2 = in_range && gz_lane_valid(v.l2), v3 = in_range && gz_lane_valid(v.l3);
-    double g0 = props[v0 ? gz_lane_id(v.l0) : 0];
-    double g1 = props[v1 ? gz_lane_id(v.l1) : 0];
-    double g2 = props[v2 ? gz_lane_id(v.l2) : 0];
-    double g3 = props[v3 ? gz_lane_id(v.l3) : 0];
+    double g0 = gz_gather_prop<kStream>(props, v0 ? gz_lane_id(v.l0) : 0);
+    double g1 = gz_gather_prop<kStream>(props, v1 ? gz_lane_id(v.l1) : 0);
+    double g2 = gz_gather_prop<kStream>(props, v2 ? gz_lane_id(v.l2) : 0);
+    double g3 = gz_gather_prop<kStream>(props, v3 ? gz_lane_id(v.l3) : 0);
     g0 = v0 ? g0 : 0.0;
     g1 = v1 ? g1 : 0.0;
     g2 = v2 ? g2 : 0.0;
@@ -28,7 +38,8 @@ __device__ __forceinline__ double gz_pr_vector_sum(const gz_lanes& v, const doub
     return (g2 + g3) + (g0 + g1);
 }
 
-struct gz_pr_pull_op {
+template <bool kStream>
+struct gz_pr_pull_op_t {
     const double* __restrict__ props;
     double* __restrict__ acc;
     bool sched_aware;
@@ -37,7 +48,7 @@ struct gz_pr_pull_op {
     __device__ __forceinline__ double combine(double a, double b) const { return a + b; }
     __device__ __forceinline__ double gather(const gz_lanes& v, bool in_range, uint32_t& edges)
     {
-        const double g = gz_pr_vector_sum(v, props, edges, in_range);
+        const double g = gz_pr_vector_sum<kStream>(v, props, edges, in_range);
         sum += g;
         return g;
     }
@@ -49,5 +60,7 @@ struct gz_pr_pull_op {
             unsafeAtomicAdd(&acc[d], s);
     }
 };
+
+typedef gz_pr_pull_op_t<false> gz_pr_pull_op;
 
 #endif /* GZ_PR_OPS_H */

@@ -29,6 +29,7 @@ mergeaccum_t* gz_merge_buffer = nullptr;
 uint64_t gz_merge_buffer_count = 0;
 
 gz_scratch_t gz_scratch;
+uint32_t gz_cu_count = 1;
 hipStream_t gz_stream = nullptr;
 static int gz_device_id = -1;
 static char gz_device_name_buf[256];
@@ -56,6 +57,7 @@ extern "C" int gz_runtime_init(int device)
     hipDeviceProp_t prop;
     GZ_CHECK(hipGetDeviceProperties(&prop, device));
     snprintf(gz_device_name_buf, sizeof(gz_device_name_buf), "%s (%s)", prop.name, prop.gcnArchName);
+    gz_cu_count = prop.multiProcessorCount > 0 ? (uint32_t)prop.multiProcessorCount : 1u;
     gz_device_id = device;
     memset(&gz_scratch, 0, sizeof(gz_scratch));
     return 0;

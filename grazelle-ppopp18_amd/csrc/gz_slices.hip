@@ -1,0 +1,477 @@
+/*
+ * gz_slices.hip -- PageRank pull over the source-sliced layout.
+ *
+ * The scheduler-aware pull (phase_edge_pull_pr.asm:189-307) gathers one FP64
+ * property per edge from anywhere in the vertex array.  On a power-law graph
+ * renumbered by degree, the few million hottest sources carry most edges, but
+ * their properties exceed one XCD's L2, and every XCD caches the same hot
+ * lines.  Here the pull list is split by source into eight hot slices of S
+ * vertices and a cold rest; the chunk table hands XCD x (workgroups w with
+ * w % 8 == x, the dispatcher's round robin) only slice x's chunks plus an even
+ * share of the cold slice, so each XCD gathers from one S-vertex window.
+ *
+ * Each slice is an ordinary pull list, walked exactly as in gz_pr.hip into
+ * its own accumulator array (complete runs stored directly, the chunk's last
+ * run into its merge record).  The merge then runs per slice
+ * (edge_pull_op_merge_with_merge_buffer's rule, phases.c:41-70), and the
+ * vertex phase takes acc[v] = the sum over the slices holding v, in slice
+ * order (gz_k_vertex_pr reads the slices' accumulators directly, so the
+ * accumulator array itself is not written).  No atomics, so the
+ * result is reproducible run to run; the per-vertex sum is regrouped by slice,
+ * which moves it by rounding only.
+ */
+#include "grazelle_host.h"
+#include "gz_host_internal.h"
+#include "gz_pr_ops.h"
+#include "gz_walk.h"
+
+#include <string.h>
+
+gz_source_slices_t gz_slices;
+/* the per-XCD work counters sit 256 bytes apart, each in a line of its own */
+#define GZ_COUNTER_STRIDE 64
+static uint32_t gz_slices_cold_spread = 1;
+static uint32_t gz_slices_cold_stream = 0;
+static uint32_t gz_slices_hot_cache = 0;
+
+/* ---- chunk table (host) ---------------------------------------------------- */
+
+extern "C" uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint64_t chunk_len, uint32_t cold_spread,
+                                          gz_slice_chunk_t* out, uint64_t capacity)
+{
+    if (chunk_len == 0) return 0;
+    uint64_t base[GZ_SLICES + 1], chunks[GZ_SLICES], rec[GZ_SLICES + 1];
+    base[0] = 0;
+    rec[0] = 0;
+    for (int x = 0; x < GZ_SLICES; ++x) {
+        chunks[x] = (vectors[x] + chunk_len - 1) / chunk_len;
+        base[x + 1] = base[x] + vectors[x];
+        rec[x + 1] = rec[x] + chunks[x];
+    }
+    if (rec[GZ_SLICES] >= (1ull << GZ_SLICE_RECORD_BITS)) {
+        fprintf(stderr, "grazelle: %llu chunks exceed the source-sliced record field\n",
+                (unsigned long long)rec[GZ_SLICES]);
+        exit(72);
+    }
+    /* chunks per XCD: an even share of all chunks, at least its own slice's */
+    uint64_t target = (rec[GZ_SLICES] + GZ_XCDS - 1) / GZ_XCDS;
+    for (int x = 0; x < GZ_XCDS; ++x)
+        if (chunks[x] > target) target = chunks[x];
+    uint64_t give[GZ_XCDS], offered = 0;
+    for (int x = 0; x < GZ_XCDS; ++x) {
+        give[x] = target - chunks[x];
+        offered += give[x];
+    }
+    /* take back what the cold slice cannot fill, one chunk per XCD in turn
+     * from the XCDs that were offered the most */
+    uint64_t excess = offered - chunks[GZ_XCDS];
+    while (excess > 0) {
+        uint64_t most = 0;
+        for (int x = 0; x < GZ_XCDS; ++x)
+            if (give[x] > most) most = give[x];
+        for (int x = 0; x < GZ_XCDS && excess > 0; ++x)
+            if (give[x] == most) {
+                give[x] -= 1;
+                excess -= 1;
+            }
+    }
+    uint64_t len[GZ_XCDS], blocks = 0;
+    for (int x = 0; x < GZ_XCDS; ++x) {
+        len[x] = chunks[x] + give[x];
+        const uint64_t bx = (len[x] + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+        if (bx > blocks) blocks = bx;
+    }
+    const uint64_t slots = blocks * GZ_XCDS * GZ_WAVES_PER_BLOCK;
+    if (out == nullptr) return slots;
+    if (capacity < slots) {
+        fprintf(stderr, "grazelle: chunk table needs %llu slots, %llu given\n", (unsigned long long)slots,
+                (unsigned long long)capacity);
+        exit(72);
+    }
+    memset(out, 0, slots * sizeof(gz_slice_chunk_t));
+    uint64_t cold_next = 0;
+    for (int x = 0; x < GZ_XCDS; ++x) {
+        uint64_t hot_next = 0;
+        const uint64_t n = len[x], g = give[x];
+        for (uint64_t i = 0; i < n; ++i) {
+            /* cold positions: where floor(i * g / n) steps (spread), or the
+             * last g positions */
+            const bool cold = cold_spread ? ((i + 1) * g / n != i * g / n) : (i >= n - g);
+            int sl;
+            uint64_t k;
+            if (cold) {
+                sl = GZ_XCDS;
+                k = cold_next++;
+            } else {
+                sl = x;
+                k = hot_next++;
+            }
+            gz_slice_chunk_t c;
+            c.begin = base[sl] + k * chunk_len;
+            const uint64_t left = vectors[sl] - k * chunk_len;
+            c.count = (uint32_t)(left < chunk_len ? left : chunk_len);
+            c.record = ((uint32_t)sl << GZ_SLICE_RECORD_BITS) | (uint32_t)(rec[sl] + k);
+            /* the XCD's i-th chunk: workgroup 8 * (i / 4) + x, wave i % 4 */
+            const uint64_t wg = (uint64_t)GZ_XCDS * (i / GZ_WAVES_PER_BLOCK) + (uint64_t)x;
+            out[wg * GZ_WAVES_PER_BLOCK + i % GZ_WAVES_PER_BLOCK] = c;
+        }
+    }
+    return slots;
+}
+
+extern "C" void gz_slices_set_cold_spread(uint32_t cold_spread) { gz_slices_cold_spread = cold_spread ? 1u : 0u; }
+extern "C" void gz_slices_set_cold_stream(uint32_t cold_stream) { gz_slices_cold_stream = cold_stream ? 1u : 0u; }
+
+extern "C" int gz_slices_set_hot_cache(uint32_t vertices)
+{
+    if (vertices != 0 && vertices != 4096 && vertices != 8192 && vertices != 16384 && vertices != 20480) return -1;
+    gz_slices_hot_cache = vertices;
+    return 0;
+}
+
+extern "C" void gz_source_slices_release(void)
+{
+    gz_device_free(gz_slices.vecs);
+    gz_device_free(gz_slices.mask);
+    gz_device_free(gz_slices.parts);
+    gz_device_free(gz_slices.table);
+    gz_device_free(gz_slices.merge);
+    gz_device_free(gz_slices.counters);
+    memset(&gz_slices, 0, sizeof(gz_slices));
+}
+
+extern "C" int gz_graph_source_slices_info(uint64_t* vectors, uint64_t* slice_vertices)
+{
+    if (gz_slices.vecs == nullptr) return 0;
+    for (int x = 0; x < GZ_SLICES; ++x) vectors[x] = gz_slices.begin[x + 1] - gz_slices.begin[x];
+    *slice_vertices = gz_slices.slice_vertices;
+    return 1;
+}
+
+/* Build (or rebuild for a new chunk length) the chunk table and the merge
+ * records it points into. */
+static void gz_slices_prepare_table(uint64_t chunk_len)
+{
+    if (gz_slices.table != nullptr && gz_slices.table_chunk_len == chunk_len &&
+        gz_slices.table_cold_spread == gz_slices_cold_spread)
+        return;
+    uint64_t vectors[GZ_SLICES];
+    for (int x = 0; x < GZ_SLICES; ++x) vectors[x] = gz_slices.begin[x + 1] - gz_slices.begin[x];
+    const uint64_t slots = gz_slices_chunk_table(vectors, chunk_len, gz_slices_cold_spread, nullptr, 0);
+    gz_slice_chunk_t* h = (gz_slice_chunk_t*)malloc((slots ? slots : 1) * sizeof(gz_slice_chunk_t));
+    gz_slices_chunk_table(vectors, chunk_len, gz_slices_cold_spread, h, slots);
+    uint64_t records = 0;
+    for (int x = 0; x < GZ_SLICES; ++x) records += (vectors[x] + chunk_len - 1) / chunk_len;
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    gz_device_free(gz_slices.table);
+    gz_device_free(gz_slices.merge);
+    gz_slices.table = (gz_slice_chunk_t*)gz_device_malloc((slots ? slots : 1) * sizeof(gz_slice_chunk_t));
+    gz_copy_to_device(gz_slices.table, h, slots * sizeof(gz_slice_chunk_t));
+    gz_slices.merge = (mergeaccum_t*)gz_device_malloc((records ? records : 1) * sizeof(mergeaccum_t));
+    gz_slices.records = records;
+    for (int x = 0; x < GZ_XCDS; ++x) gz_slices.xcd_chunks[x] = 0;
+    for (uint64_t i = 0; i < slots; ++i)
+        if (h[i].count) gz_slices.xcd_chunks[(i / GZ_WAVES_PER_BLOCK) % GZ_XCDS] += 1;
+    if (gz_slices.counters == nullptr)
+        gz_slices.counters = (uint32_t*)gz_device_malloc(GZ_XCDS * GZ_COUNTER_STRIDE * sizeof(uint32_t));
+    free(h);
+    gz_slices.slots = slots;
+    gz_slices.table_chunk_len = chunk_len;
+    gz_slices.table_cold_spread = gz_slices_cold_spread;
+}
+
+/* ---- kernels ---------------------------------------------------------------- */
+
+struct gz_slice_bounds {
+    uint64_t rec[GZ_SLICES + 1];
+};
+
+/* Walk one chunk into slice accumulators `acc`; returns the wave's share of S. */
+template <int VPL, bool kStream>
+__device__ __forceinline__ double gz_sliced_walk(const gz_vec_t* __restrict__ vecs, uint64_t b, uint64_t e,
+                                                 const double* __restrict__ props, double* __restrict__ acc,
+                                                 mergeaccum_t* __restrict__ record, uint32_t& edges)
+{
+    gz_pr_pull_op_t<kStream> op{props, acc, true, 0.0};
+    uint64_t first_d, carry_d;
+    double carry_v;
+    gz_walk_chunk<VPL, double>(vecs, b, e, op, first_d, carry_d, carry_v, edges);
+    if (gz_lane() == 0) {
+        mergeaccum_t r;
+        r.initial_vertex_id = first_d;
+        r.final_vertex_id = carry_d;
+        r.final_partial_value = carry_v;
+        r.__padding__ = 0;
+        *record = r;
+    }
+    return gz_wave_sum(op.sum);
+}
+
+/* kColdStream: the cold slice's gathers, which mostly miss, are nontemporal. */
+template <int VPL, bool kColdStream>
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr_sliced(const gz_vec_t* __restrict__ vecs,
+                                                                     const gz_slice_chunk_t* __restrict__ table,
+                                                                     const double* __restrict__ props,
+                                                                     double* __restrict__ parts, uint64_t part_stride,
+                                                                     mergeaccum_t* __restrict__ merge,
+                                                                     double* __restrict__ partials,
+                                                                     uint64_t* __restrict__ stats, uint32_t collect)
+{
+    const int lane = gz_lane();
+    const uint64_t slot = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
+    const gz_slice_chunk_t c = table[slot];
+    if (c.count == 0) {
+        if (lane == 0) {
+            partials[slot] = 0.0;
+            if (collect) {
+                stats[2 * slot] = 0;
+                stats[2 * slot + 1] = 0;
+            }
+        }
+        return;
+    }
+    const uint32_t x = c.record >> GZ_SLICE_RECORD_BITS;
+    const uint64_t rec = c.record & ((1u << GZ_SLICE_RECORD_BITS) - 1u);
+    const uint64_t b = c.begin, e = c.begin + c.count;
+    double* acc = parts + (uint64_t)x * part_stride;
+    uint32_t edges = 0;
+    const double S = (kColdStream && x == GZ_XCDS)
+                         ? gz_sliced_walk<VPL, true>(vecs, b, e, props, acc, merge + rec, edges)
+                         : gz_sliced_walk<VPL, false>(vecs, b, e, props, acc, merge + rec, edges);
+    if (lane == 0) partials[slot] = S;
+    if (collect) {
+        const uint64_t tot_edges = gz_wave_sum_u64(edges);
+        if (lane == 0) {
+            stats[2 * slot] = c.count;
+            stats[2 * slot + 1] = tot_edges;
+        }
+    }
+}
+
+/* The merge of gz_k_merge_sum applied to each slice's own records and
+ * accumulators: runs never cross a slice boundary. */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_merge_sliced(const mergeaccum_t* __restrict__ mb,
+                                                              gz_slice_bounds bounds, double* __restrict__ parts,
+                                                              uint64_t part_stride)
+{
+    const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= bounds.rec[GZ_SLICES]) return;
+    int x = 0;
+    while (u >= bounds.rec[x + 1]) ++x;
+    const uint64_t lo = bounds.rec[x], hi = bounds.rec[x + 1];
+    double* acc = parts + (uint64_t)x * part_stride;
+    const mergeaccum_t r = mb[u];
+    if (u > lo && mb[u - 1].final_vertex_id == r.final_vertex_id) return;
+    double proposed = r.final_partial_value;
+    uint64_t j = u + 1;
+    while (j < hi && mb[j].final_vertex_id == r.final_vertex_id) {
+        proposed = proposed + mb[j].final_partial_value;
+        ++j;
+    }
+    if (j < hi && mb[j].initial_vertex_id == r.final_vertex_id) proposed = proposed + acc[r.final_vertex_id];
+    acc[r.final_vertex_id] = proposed;
+}
+
+/* ---- persistent pull with the hottest sources of each slice in LDS -------------
+ * As many workgroups as fit on the device at once (LDS and registers
+ * permitting) each copy the first `hot` properties of their XCD's slice
+ * window into LDS; their wavefronts then take that XCD's chunks from a
+ * per-XCD counter.  Gathers that fall in the cached range read LDS instead of
+ * issuing an L2 request.  A cache of 128 KiB or more leaves room for one
+ * workgroup per CU, which then has 1024 threads so that 16 wavefronts share
+ * it; smaller caches use 512-thread workgroups, several per CU. */
+template <int HOT>
+struct gz_hot_wg {
+    static constexpr int value = HOT >= 16384 ? 1024 : 512;
+};
+
+struct gz_xcd_lens {
+    uint64_t len[GZ_XCDS];
+};
+
+struct gz_pr_pull_op_hot {
+    const double* __restrict__ props;
+    const double* hot; /* LDS */
+    uint64_t wlo, wn;
+    double* __restrict__ acc;
+    double sum;
+    __device__ __forceinline__ double identity() const { return 0.0; }
+    __device__ __forceinline__ double combine(double a, double b) const { return a + b; }
+    __device__ __forceinline__ double fetch(uint64_t i) const
+    {
+        const uint64_t r = i - wlo;
+        return (r < wn) ? hot[r] : props[i];
+    }
+    __device__ __forceinline__ double gather(const gz_lanes& v, bool in_range, uint32_t& edges)
+    {
+        /* gz_pr_vector_sum with the cache in front; invalid lanes read the
+         * window's first element */
+        const bool v0 = in_range && gz_lane_valid(v.l0), v1 = in_range && gz_lane_valid(v.l1),
+                   v2 = in_range && gz_lane_valid(v.l2), v3 = in_range && gz_lane_valid(v.l3);
+        double g0 = fetch(v0 ? gz_lane_id(v.l0) : wlo);
+        double g1 = fetch(v1 ? gz_lane_id(v.l1) : wlo);
+        double g2 = fetch(v2 ? gz_lane_id(v.l2) : wlo);
+        double g3 = fetch(v3 ? gz_lane_id(v.l3) : wlo);
+        g0 = v0 ? g0 : 0.0;
+        g1 = v1 ? g1 : 0.0;
+        g2 = v2 ? g2 : 0.0;
+        g3 = v3 ? g3 : 0.0;
+        edges += (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2 + (uint32_t)v3;
+        const double g = (g2 + g3) + (g0 + g1);
+        sum += g;
+        return g;
+    }
+    __device__ __forceinline__ void store(uint64_t d, double s) { acc[d] = s; }
+};
+
+template <int HOT>
+__global__ __launch_bounds__(gz_hot_wg<HOT>::value) void gz_k_edge_pull_pr_sliced_hot(
+    const gz_vec_t* __restrict__ vecs, const gz_slice_chunk_t* __restrict__ table, gz_xcd_lens lens,
+    const double* __restrict__ props, double* __restrict__ parts, uint64_t part_stride,
+    mergeaccum_t* __restrict__ merge, double* __restrict__ partials, uint64_t* __restrict__ stats, uint32_t collect,
+    uint32_t* __restrict__ counters, uint64_t slice_vertices, uint64_t hot_count)
+{
+    __shared__ double hot[HOT];
+    const uint32_t x = blockIdx.x % GZ_XCDS;
+    const uint64_t lo = (uint64_t)x * slice_vertices;
+    for (uint64_t i = threadIdx.x; i < hot_count; i += blockDim.x) hot[i] = props[lo + i];
+    __syncthreads();
+    const int lane = gz_lane();
+    const uint64_t n = lens.len[x];
+    while (true) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = atomicAdd(&counters[x * GZ_COUNTER_STRIDE], 1u);
+        idx = (uint32_t)__shfl((int)idx, 0, GZ_WAVE);
+        if ((uint64_t)idx >= n) break;
+        const uint64_t slot =
+            ((uint64_t)GZ_XCDS * (idx / GZ_WAVES_PER_BLOCK) + x) * GZ_WAVES_PER_BLOCK + idx % GZ_WAVES_PER_BLOCK;
+        const gz_slice_chunk_t c = table[slot];
+        const uint32_t sl = c.record >> GZ_SLICE_RECORD_BITS;
+        const uint64_t rec = c.record & ((1u << GZ_SLICE_RECORD_BITS) - 1u);
+        gz_pr_pull_op_hot op{props, hot, (sl == x) ? lo : 0, (sl == x) ? hot_count : 0,
+                             parts + (uint64_t)sl * part_stride, 0.0};
+        uint64_t first_d, carry_d;
+        double carry_v;
+        uint32_t edges = 0;
+        gz_walk_chunk<1, double>(vecs, c.begin, c.begin + c.count, op, first_d, carry_d, carry_v, edges);
+        const double S = gz_wave_sum(op.sum);
+        if (lane == 0) {
+            mergeaccum_t r;
+            r.initial_vertex_id = first_d;
+            r.final_vertex_id = carry_d;
+            r.final_partial_value = carry_v;
+            r.__padding__ = 0;
+            merge[rec] = r;
+            partials[rec] = S;
+        }
+        if (collect) {
+            const uint64_t tot_edges = gz_wave_sum_u64(edges);
+            if (lane == 0) {
+                stats[2 * rec] = c.count;
+                stats[2 * rec + 1] = tot_edges;
+            }
+        }
+    }
+}
+
+/* One workgroup per XCD and resident slot: the grid is exactly what fits. */
+template <int HOT>
+static void gz_launch_hot(const gz_xcd_lens& lens, uint64_t hot_count)
+{
+    constexpr int WG = gz_hot_wg<HOT>::value;
+    int per_cu = 0;
+    GZ_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gz_k_edge_pull_pr_sliced_hot<HOT>, WG, 0));
+    if (per_cu < 1) per_cu = 1;
+    uint64_t per_xcd = gz_cu_count / GZ_XCDS;
+    if (per_xcd == 0) per_xcd = 1;
+    const uint64_t blocks = (uint64_t)GZ_XCDS * per_xcd * (uint64_t)per_cu;
+    hipLaunchKernelGGL(gz_k_edge_pull_pr_sliced_hot<HOT>, dim3((unsigned)blocks), dim3(WG), 0, gz_stream,
+                       gz_slices.vecs, gz_slices.table, lens, (const double*)gz_graph.vertex_props, gz_slices.parts,
+                       gz_slices.part_stride, gz_slices.merge, (double*)gz_scratch.partials, gz_scratch.stats,
+                       gz_phase_config.collect_stats, gz_slices.counters, gz_slices.slice_vertices, hot_count);
+    GZ_LAUNCH_CHECK();
+}
+
+/* ---- the phase ---------------------------------------------------------------- */
+
+void gz_pull_pr_sliced(const gz_vec_t* edge_list, uint64_t edge_list_count)
+{
+    if (gz_slices.vecs == nullptr || gz_slices.base_list != edge_list || gz_slices.base_count != edge_list_count) {
+        fprintf(stderr,
+                "grazelle: the source-sliced pull needs a sliced layout of this edge list "
+                "(gz_graph_build_source_slices)\n");
+        exit(72);
+    }
+    const uint64_t chunk_len = gz_chunk_len();
+    gz_slices_prepare_table(chunk_len);
+    const uint64_t slots = gz_slices.slots;
+    const uint32_t hot = gz_slices_hot_cache;
+    const uint64_t waves = hot ? gz_slices.records : slots;
+    gz_scratch_reserve(waves, gz_graph.num_vertices);
+    gz_scratch.partials_is_double = 1;
+    gz_scratch.partials_count = waves;
+    gz_scratch.stats_count = gz_phase_config.collect_stats ? waves : 0;
+    if (hot && gz_slices.records > 0) {
+        const uint64_t hot_count = hot < gz_slices.slice_vertices ? hot : gz_slices.slice_vertices;
+        gz_xcd_lens lens;
+        for (int x = 0; x < GZ_XCDS; ++x) lens.len[x] = gz_slices.xcd_chunks[x];
+        gz_device_zero(gz_slices.counters, GZ_XCDS * GZ_COUNTER_STRIDE * sizeof(uint32_t));
+        gz_ktimer_begin(GZ_K_PULL_PR);
+        if (hot == 20480)
+            gz_launch_hot<20480>(lens, hot_count);
+        else if (hot == 16384)
+            gz_launch_hot<16384>(lens, hot_count);
+        else if (hot == 8192)
+            gz_launch_hot<8192>(lens, hot_count);
+        else
+            gz_launch_hot<4096>(lens, hot_count);
+        gz_ktimer_end(GZ_K_PULL_PR);
+    } else if (!hot && slots > 0) {
+        const uint64_t blocks = slots / GZ_WAVES_PER_BLOCK;
+        const uint32_t vpl = gz_vectors_per_lane();
+        gz_ktimer_begin(GZ_K_PULL_PR);
+#define GZ_SLICED_ARGS                                                                                          \
+    dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, gz_slices.vecs, gz_slices.table,                       \
+        (const double*)gz_graph.vertex_props, gz_slices.parts, gz_slices.part_stride, gz_slices.merge,           \
+        (double*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats
+        const bool cs = gz_slices_cold_stream != 0;
+        if (vpl == 4 && cs)
+            hipLaunchKernelGGL((gz_k_edge_pull_pr_sliced<4, true>), GZ_SLICED_ARGS);
+        else if (vpl == 4)
+            hipLaunchKernelGGL((gz_k_edge_pull_pr_sliced<4, false>), GZ_SLICED_ARGS);
+        else if (vpl == 2 && cs)
+            hipLaunchKernelGGL((gz_k_edge_pull_pr_sliced<2, true>), GZ_SLICED_ARGS);
+        else if (vpl == 2)
+            hipLaunchKernelGGL((gz_k_edge_pull_pr_sliced<2, false>), GZ_SLICED_ARGS);
+        else if (cs)
+            hipLaunchKernelGGL((gz_k_edge_pull_pr_sliced<1, true>), GZ_SLICED_ARGS);
+        else
+            hipLaunchKernelGGL((gz_k_edge_pull_pr_sliced<1, false>), GZ_SLICED_ARGS);
+#undef GZ_SLICED_ARGS
+        GZ_LAUNCH_CHECK();
+        gz_ktimer_end(GZ_K_PULL_PR);
+    }
+    gz_ktimer_begin(GZ_K_MERGE);
+    if (gz_slices.records > 0) {
+        gz_slice_bounds bounds;
+        bounds.rec[0] = 0;
+        for (int x = 0; x < GZ_SLICES; ++x) {
+            const uint64_t n = gz_slices.begin[x + 1] - gz_slices.begin[x];
+            bounds.rec[x + 1] = bounds.rec[x] + (n + chunk_len - 1) / chunk_len;
+        }
+        const uint64_t mblocks = (gz_slices.records + GZ_BLOCK - 1) / GZ_BLOCK;
+        hipLaunchKernelGGL(gz_k_merge_sliced, dim3((unsigned)mblocks), dim3(GZ_BLOCK), 0, gz_stream, gz_slices.merge,
+                           bounds, gz_slices.parts, gz_slices.part_stride);
+        GZ_LAUNCH_CHECK();
+    }
+    gz_ktimer_end(GZ_K_MERGE);
+}
+
+extern "C" int gz_graph_source_slices_download(gz_vec_t* vectors, uint16_t* mask)
+{
+    if (gz_slices.vecs == nullptr) return 0;
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    if (vectors) gz_copy_to_host(vectors, gz_slices.vecs, gz_slices.begin[GZ_SLICES] * sizeof(gz_vec_t));
+    if (mask) gz_copy_to_host(mask, gz_slices.mask, gz_graph.num_vertices * sizeof(uint16_t));
+    return 1;
+}

@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "_build", "libgrazelle.so")
 ALG = {"pr": 0, "cc": 1, "bfs": 2}
 ENGINE = {"hybrid": 0, "pull": 1, "push": 2, "learned": 3, "probe": 4}
 STAT = {"both": 0, "outdeg": 1, "count": 2}
-PULL_VARIANT = {"sched_aware": 0, "traditional": 1}
+PULL_VARIANT = {"sched_aware": 0, "traditional": 1, "source_sliced": 2}
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"libgrazelle.so is not built ({LIB_PATH}); run __graft_entry__.build()")
@@ -95,6 +95,13 @@ _sig("gz_graph_load_pairs", C.c_int, u64, u64, vp, u64, vp)
 _sig("gz_graph_build_pairs_device", C.c_int, u64, u64, vp, C.c_int, u32)
 _sig("gz_graph_build_rmat_device", C.c_int, P(RmatParams), u64, C.c_int, u64, vp, u32)
 _sig("gz_graph_device_vertex", u64, u64)
+_sig("gz_graph_build_source_slices", C.c_int, u64)
+_sig("gz_graph_source_slices_info", C.c_int, vp, P(u64))
+_sig("gz_slices_chunk_table", u64, vp, u64, u32, vp, u64)
+_sig("gz_slices_set_cold_spread", None, u32)
+_sig("gz_slices_set_cold_stream", None, u32)
+_sig("gz_slices_set_hot_cache", C.c_int, u32)
+_sig("gz_graph_source_slices_download", C.c_int, vp, vp)
 _sig("gz_graph_release", None)
 _sig("gz_graph_num_vertices", u64)
 _sig("gz_graph_num_edges", u64)
@@ -281,47 +288,108 @@ def load(g: EdgeLists, pull: bool = True, push: bool = True) -> None:
 
 
 RELABEL_BY_OUTDEGREE = 1
+SLICES = 9
+SLICE_CHUNK = np.dtype([("begin", np.uint64), ("count", np.uint32), ("record", np.uint32)])
+SLICE_RECORD_BITS = 28
 
 
-def load_pairs_device(num_vertices: int, pairs: np.ndarray, symmetrize: bool = False, relabel: bool = False) -> None:
-    """Device ingress from raw (src, dst) pairs: sort, dedupe, pack in HBM."""
+def _ingress_flags(relabel: bool, slices_log2: int) -> int:
+    return (RELABEL_BY_OUTDEGREE if relabel else 0) | ((slices_log2 & 0x3F) << 8)
+
+
+def load_pairs_device(num_vertices: int, pairs: np.ndarray, symmetrize: bool = False, relabel: bool = False,
+                      slices_log2: int = 0) -> None:
+    """Device ingress from raw (src, dst) pairs: sort, dedupe, pack in HBM.
+    slices_log2 = k > 0 also builds the source-sliced pull layout with
+    S = 2^k (and, with relabel, deals the 8*S hottest vertices into the
+    eight slices)."""
     init()
     p = np.ascontiguousarray(pairs, dtype=np.uint64).reshape(-1, 2)
     lib.gz_graph_build_pairs_device(num_vertices, p.shape[0], _ptr(p) if p.shape[0] else None, int(symmetrize),
-                                    RELABEL_BY_OUTDEGREE if relabel else 0)
+                                    _ingress_flags(relabel, slices_log2))
 
 
 def load_rmat_device(scale: int, num_edges: int, seed: int, symmetrize: bool = False, num_vertices: int = 0,
                      extra: np.ndarray | None = None, rmat_vertices: int = 0, a=0.57, b=0.19, c=0.19,
-                     relabel: bool = False) -> None:
+                     relabel: bool = False, slices_log2: int = 0) -> None:
     """Device ingress of a seeded R-MAT graph (the stream of rmat_edges) plus
     optional extra host pairs; V = max(num_vertices, R-MAT vertex count).
     relabel renumbers vertices by out-degree on the device (results keep
-    original ids)."""
+    original ids); slices_log2 as for load_pairs_device."""
     init()
     p = RmatParams(scale, rmat_vertices, num_edges, a, b, c, seed, 0)
     ex = np.ascontiguousarray(extra if extra is not None else np.zeros((0, 2), np.uint64), dtype=np.uint64)
     lib.gz_graph_build_rmat_device(C.byref(p), num_vertices, int(symmetrize), ex.shape[0],
-                                   _ptr(ex) if ex.shape[0] else None, RELABEL_BY_OUTDEGREE if relabel else 0)
+                                   _ptr(ex) if ex.shape[0] else None, _ingress_flags(relabel, slices_log2))
 
 
-def load_workload_device(name: str, seed: int = 1, relabel: bool = False) -> None:
+def load_workload_device(name: str, seed: int = 1, relabel: bool = False, slices_log2: int = 0) -> None:
     """Build a named workload with device ingress (same graph as workload_graph)."""
+    kw = dict(relabel=relabel, slices_log2=slices_log2)
     if name in ("pr-rmat16", "pr-rmat20", "bfs-rmat24"):
         scale = {"pr-rmat16": 16, "pr-rmat20": 20, "bfs-rmat24": 24}[name]
-        load_rmat_device(scale, (1 << scale) * 16, seed, relabel=relabel)
+        load_rmat_device(scale, (1 << scale) * 16, seed, **kw)
         return
     if name in ("cc-rmat22-sym-trees", "cc-rmat16-sym-trees"):
         scale, tsize = (22, 1000) if name.startswith("cc-rmat22") else (16, 100)
         nv0 = 1 << scale
         tr = planted_trees(nv0, 64, tsize, seed + 1000)
-        load_rmat_device(scale, nv0 * 16, seed, symmetrize=True, num_vertices=nv0 + 64 * tsize, extra=tr,
-                         relabel=relabel)
+        load_rmat_device(scale, nv0 * 16, seed, symmetrize=True, num_vertices=nv0 + 64 * tsize, extra=tr, **kw)
         return
     if name == "full-rmat26-fold41.7m":
-        load_rmat_device(26, FULL_RAW_EDGES, seed, rmat_vertices=FULL_VERTICES, relabel=relabel)
+        load_rmat_device(26, FULL_RAW_EDGES, seed, rmat_vertices=FULL_VERTICES, **kw)
         return
     raise KeyError(name)
+
+
+def build_source_slices(slice_vertices: int) -> None:
+    """Source-sliced pull layout of the loaded graph (pull_variant
+    "source_sliced" runs PageRank over it)."""
+    lib.gz_graph_build_source_slices(slice_vertices)
+
+
+def source_slices_info():
+    """(vectors per slice [9], S), or None without a sliced layout."""
+    v = np.zeros(SLICES, np.uint64)
+    s = u64(0)
+    if not lib.gz_graph_source_slices_info(_ptr(v), C.byref(s)):
+        return None
+    return v, int(s.value)
+
+
+def source_slices_download():
+    """(concatenated slice vectors as (N, 4) uint64, per-vertex slice mask)."""
+    info = source_slices_info()
+    if info is None:
+        return None
+    n = int(info[0].sum())
+    vecs = np.zeros((max(n, 1), 4), np.uint64)
+    mask = np.zeros(max(num_vertices(), 1), np.uint16)
+    lib.gz_graph_source_slices_download(_ptr(vecs), _ptr(mask))
+    return vecs[:n], mask[:num_vertices()]
+
+
+def slices_chunk_table(vectors, chunk_len: int, cold_spread: bool = True) -> np.ndarray:
+    """The chunk slots the source-sliced pull launches (host computation)."""
+    v = np.ascontiguousarray(vectors, dtype=np.uint64)
+    n = lib.gz_slices_chunk_table(_ptr(v), chunk_len, int(cold_spread), None, 0)
+    out = np.zeros(max(int(n), 1), SLICE_CHUNK)
+    lib.gz_slices_chunk_table(_ptr(v), chunk_len, int(cold_spread), _ptr(out), n)
+    return out[:n]
+
+
+def slices_set_cold_spread(on: bool) -> None:
+    lib.gz_slices_set_cold_spread(int(on))
+
+
+def slices_set_cold_stream(on: bool) -> None:
+    lib.gz_slices_set_cold_stream(int(on))
+
+
+def slices_set_hot_cache(vertices: int) -> None:
+    """LDS cache of the hottest `vertices` sources of each slice (0 = off)."""
+    if lib.gz_slices_set_hot_cache(vertices) != 0:
+        raise ValueError(f"hot cache size {vertices} is not 0, 4096, 8192, 16384 or 20480")
 
 
 def set_chunk_vectors(n: int) -> None:

@@ -80,7 +80,10 @@ enum gz_stat_mode {                        /* phase_helper_add_frontier_stat, ph
 
 enum gz_pull_variant {
     GZ_PULL_SCHED_AWARE = 0,               /* merge buffers, plain stores (default) */
-    GZ_PULL_TRADITIONAL = 1                /* EXPERIMENT_EDGE_PULL_WITHOUT_SCHED_AWARE: atomics, no merge */
+    GZ_PULL_TRADITIONAL = 1,               /* EXPERIMENT_EDGE_PULL_WITHOUT_SCHED_AWARE: atomics, no merge */
+    GZ_PULL_SOURCE_SLICED = 2              /* scheduler-aware over the source-sliced layout built by
+                                            * gz_graph_build_source_slices for this edge list; merges
+                                            * inside the phase (no reference counterpart) */
 };
 
 typedef struct gz_phase_config_t {

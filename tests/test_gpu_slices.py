@@ -1,0 +1,174 @@
+"""Source-sliced PageRank pull versus the host packer and the CPU oracle.
+
+Layout: slice x < 8 must be the host packer's pull list of the edges whose
+source is in [x*S, (x+1)*S), slice 8 of all other edges, concatenated in
+slice order, and the mask must name the slices holding each destination.
+PageRank over it must match the oracle within 1e-12 maximum relative error
+per vertex (each destination's sum is regrouped by slice), be bit-identical
+run to run, and the interleaved renumbering must deal the 8*S highest-degree
+vertices round robin into the slices.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+PR_TOL = 1e-12
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+@pytest.fixture(scope="module")
+def rmat14(gz):
+    return gz.rmat_graph(14, 16, seed=11)
+
+
+def _expected_layout(gz, pull, V, S):
+    S = min(S, V // 8)
+    src = pull[:, 0].astype(np.int64)
+    sl = np.where(src < 8 * S, src // max(S, 1), 8) if S else np.full(len(src), 8)
+    vecs, mask = [], np.zeros(V, np.uint16)
+    for x in range(9):
+        part = pull[sl == x]
+        if len(part):
+            vecs.append(gz.pack_vectors(part, True))
+            mask[np.unique(part[:, 1].astype(np.int64))] |= np.uint16(1 << x)
+        else:
+            vecs.append(np.zeros((0, 4), np.uint64))
+    return vecs, mask
+
+
+@pytest.mark.parametrize("S", [1, 64, 700, 1 << 20])
+def test_slices_layout_matches_host_packer(gz, device, rmat14, S):
+    g = rmat14
+    gz.load(g)
+    gz.build_source_slices(S)
+    counts, s_eff = gz.source_slices_info()
+    assert s_eff == min(S, g.num_vertices // 8)
+    vecs, mask = gz.source_slices_download()
+    want, want_mask = _expected_layout(gz, g.pull, g.num_vertices, S)
+    assert [int(c) for c in counts] == [len(w) for w in want]
+    assert np.array_equal(vecs, np.concatenate(want).reshape(-1, 4))
+    assert np.array_equal(mask, want_mask)
+
+
+@pytest.mark.parametrize("spread", [True, False])
+@pytest.mark.parametrize("vpl", [1, 2, 4])
+@pytest.mark.parametrize("chunk", [64, 512])
+@pytest.mark.parametrize("S", [64, 1024])
+def test_sliced_pr_parity(gz, orc, device, rmat14, S, chunk, vpl, spread):
+    g = rmat14
+    gz.set_chunk_vectors(chunk)
+    gz.set_vectors_per_lane(vpl)
+    gz.slices_set_cold_spread(spread)
+    try:
+        gz.load(g)
+        gz.build_source_slices(S)
+        gz.run(gz.options("pr", iterations=20, pull_variant="source_sliced"))
+        ranks = gz.pr_ranks()
+    finally:
+        gz.set_chunk_vectors(512)
+        gz.set_vectors_per_lane(1)
+        gz.slices_set_cold_spread(True)
+    _, ref, _ = orc.pagerank(orc.Graph(g.num_vertices, g.pull, g.push), orc.options(iterations=20))
+    assert _rel(ranks, ref) <= PR_TOL
+    assert abs(ranks.sum() - 1.0) < 1e-9
+
+
+def test_sliced_pr_deterministic_and_close_to_plain(gz, device, rmat14):
+    gz.load(rmat14)
+    gz.build_source_slices(256)
+    gz.run(gz.options("pr", iterations=5, pull_variant="source_sliced"))
+    a = gz.pr_props()
+    gz.run(gz.options("pr", iterations=5, pull_variant="source_sliced"))
+    b = gz.pr_props()
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    gz.run(gz.options("pr", iterations=5))
+    c = gz.pr_props()
+    assert _rel(a, c) <= PR_TOL
+
+
+def test_sliced_pr_sym_trees_and_empty_slices(gz, orc, device):
+    # a graph whose low-id slices hold no sources at all
+    V = 5000
+    rng = np.random.default_rng(3)
+    src = rng.integers(2000, V, 40000)
+    dst = rng.integers(0, V, 40000)
+    g = gz.make_graph(np.stack([src, dst], 1).astype(np.uint64), V)
+    gz.load(g)
+    gz.build_source_slices(100)
+    counts, _ = gz.source_slices_info()
+    assert int(counts[0]) == 0 and int(counts[8]) > 0
+    gz.run(gz.options("pr", iterations=10, pull_variant="source_sliced"))
+    _, ref, _ = orc.pagerank(orc.Graph(V, g.pull, g.push), orc.options(iterations=10))
+    assert _rel(gz.pr_ranks(), ref) <= PR_TOL
+
+
+@pytest.mark.parametrize("k", [4, 8])
+def test_interleaved_renumbering_and_pr(gz, orc, device, k):
+    scale, ef, seed = 13, 16, 23
+    nv = 1 << scale
+    raw = gz.rmat_edges(scale, nv * ef, seed)
+    g = gz.make_graph(raw, nv)
+    gz.load_rmat_device(scale, nv * ef, seed, relabel=True, slices_log2=k)
+    S = min(1 << k, nv // 8)
+    assert gz.source_slices_info()[1] == S
+    # rank r of the raw out-degree order (descending, ties by id) gets id
+    # (r % 8) * S + r // 8 for r < 8S, else r
+    cnt = np.bincount(raw[:, 0].astype(np.int64), minlength=nv)
+    order = np.lexsort((np.arange(nv), -cnt))
+    r = np.arange(nv)
+    want = np.where(r < 8 * S, (r % 8) * S + r // 8, r)
+    got = np.array([gz.lib.gz_graph_device_vertex(int(v)) for v in order])
+    assert np.array_equal(got, want)
+    gz.run(gz.options("pr", iterations=15, pull_variant="source_sliced"))
+    _, ref, _ = orc.pagerank(orc.Graph(nv, g.pull, g.push), orc.options(iterations=15))
+    assert _rel(gz.pr_ranks(), ref) <= PR_TOL
+    # the plain pull on the same renumbered graph
+    gz.run(gz.options("pr", iterations=15))
+    assert _rel(gz.pr_ranks(), ref) <= PR_TOL
+
+
+@pytest.fixture(scope="module")
+def rmat17(gz):
+    return gz.rmat_graph(17, 16, seed=5)
+
+
+@pytest.mark.parametrize("hot", [4096, 8192, 16384, 20480])
+@pytest.mark.parametrize("S", [4096, 16384])
+def test_sliced_pr_hot_cache(gz, orc, device, rmat17, S, hot):
+    g = rmat17
+    gz.load(g)
+    gz.build_source_slices(S)
+    gz.slices_set_hot_cache(hot)
+    try:
+        gz.run(gz.options("pr", iterations=10, pull_variant="source_sliced"))
+        a = gz.pr_props()
+        ranks = gz.pr_ranks()
+        gz.run(gz.options("pr", iterations=10, pull_variant="source_sliced"))
+        b = gz.pr_props()
+    finally:
+        gz.slices_set_hot_cache(0)
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    _, ref, _ = orc.pagerank(orc.Graph(g.num_vertices, g.pull, g.push), orc.options(iterations=10))
+    assert _rel(ranks, ref) <= PR_TOL
+
+
+@pytest.mark.parametrize("chunk", [64, 1024])
+def test_sliced_pr_hot_cache_small_slices(gz, orc, device, rmat14, chunk):
+    # windows smaller than the cache, many chunks per XCD
+    g = rmat14
+    gz.set_chunk_vectors(chunk)
+    gz.load(g)
+    gz.build_source_slices(64)
+    gz.slices_set_hot_cache(4096)
+    try:
+        gz.run(gz.options("pr", iterations=20, pull_variant="source_sliced"))
+        ranks = gz.pr_ranks()
+    finally:
+        gz.slices_set_hot_cache(0)
+        gz.set_chunk_vectors(512)
+    _, ref, _ = orc.pagerank(orc.Graph(g.num_vertices, g.pull, g.push), orc.options(iterations=20))
+    assert _rel(ranks, ref) <= PR_TOL
