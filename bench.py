@@ -9,12 +9,17 @@ are reported in original ids).  FP64 ranks, damping 0.85.  One step = one
 PageRank iteration (pull sweep, merge, sink-mass reduction, vertex phase);
 value = E / (time per iteration), GTEPS per iteration (the reference's rate,
 main.c:127-129, per iteration).  Inputs are resident in HBM before timing.
+The pull runs over the source-sliced layout (eight 2^19-vertex source windows,
+one per XCD, the hottest 20480 sources of each window in LDS, chunks of 2048
+vectors); --plain-pull runs the scheduler-aware pull over the ordinary list.
 
 roofline: the pull sweep kernel's algorithmic bytes per launch, 32*Nvp
-(vectors) + 8*E (one FP64 gather per edge) + 8*V (one accumulator store per
-vertex), over its average launch time measured live with HIP events on the
-library's stream.  traffic: HBM bytes per launch from the committed rocprofv3
-counter summary (profiles/).
+(vectors of the ordinary pull list) + 8*E (one FP64 gather per edge) + 8*V
+(one accumulator store per vertex), over its average launch time measured
+live with HIP events on the library's stream.  The sliced layout streams more
+vectors than Nvp; those extra bytes are not counted as useful work.  traffic:
+HBM bytes per launch from the committed rocprofv3 counter summary
+(profiles/).
 
 Sub-objects, each one configuration of the workload set:
   cc_rmat22   CC, symmetrised R-MAT s22 + 64 planted 1000-vertex trees,
@@ -47,6 +52,9 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "GTEPS/iter + % HBM peak, 1 MI355X; learned-vs-fixed switch & batched-vs-seq"
 FULL = "full-rmat26-fold41.7m"
 DAMPING = 0.85
+SLICES_LOG2 = 19      # source window of 2^19 vertices per XCD
+HOT_CACHE = 20480     # hottest sources of each window kept in LDS (160 KiB)
+SLICED_CHUNK = 2048   # vectors per chunk of the sliced pull
 
 
 def log(*a):
@@ -68,23 +76,32 @@ def pull_bytes(gz) -> int:
 
 def headline(gz, args) -> dict:
     t0 = time.time()
-    gz.load_workload_device(FULL, relabel=not args.no_relabel)
+    sliced = not args.plain_pull
+    gz.load_workload_device(FULL, relabel=not args.no_relabel, slices_log2=SLICES_LOG2 if sliced else 0)
     gz.lib.gz_synchronize()
     V, E, nvp = gz.num_vertices(), gz.num_edges(), gz.pull_vector_count()
     log(f"[headline] {FULL}: V={V} E={E} Nvp={nvp} ingress {time.time() - t0:.2f}s")
+    variant = "source_sliced" if sliced else "sched_aware"
+    if sliced:
+        gz.set_chunk_vectors(SLICED_CHUNK)
+        gz.slices_set_hot_cache(HOT_CACHE)
+    opts = gz.options("pr", iterations=args.steps, damping=DAMPING, pull_variant=variant)
     if args.warmup:
-        gz.run(gz.options("pr", iterations=args.warmup, damping=DAMPING))
+        gz.run(gz.options("pr", iterations=args.warmup, damping=DAMPING, pull_variant=variant))
     gz.lib.gz_synchronize()
     w0 = time.perf_counter()
-    r = gz.run(gz.options("pr", iterations=args.steps, damping=DAMPING))
+    r = gz.run(opts)
     gz.lib.gz_synchronize()
     wall_ms = (time.perf_counter() - w0) * 1e3
     ranks = gz.pr_ranks()
     # per-kernel times from a second pass with HIP events around every launch
     gz.kernel_timing(True)
-    gz.run(gz.options("pr", iterations=args.steps, damping=DAMPING))
+    gz.run(opts)
     kt = gz.kernel_times()
     gz.kernel_timing(False)
+    sliced_vectors = int(gz.source_slices_info()[0].sum()) if sliced else nvp
+    gz.set_chunk_vectors(512)
+    gz.slices_set_hot_cache(0)
     full_other = None if args.only_headline else full_cc_bfs(gz)
     ms_step = r.elapsed_ms / args.steps
     pms, pn = kt["pull_pr"]
@@ -93,13 +110,15 @@ def headline(gz, args) -> dict:
     achieved = B / (avg * 1e-3) / 1e9
     log(f"[headline] {args.steps} iterations {r.elapsed_ms:.2f} ms ({ms_step:.3f} ms/it), pull {avg:.3f} ms "
         f"{achieved:.0f} GB/s, rank sum {ranks.sum():.12f}")
+    kernel = "gz_k_edge_pull_pr_sliced_hot" if sliced else "gz_k_edge_pull_pr"
     return {
-        "V": V, "E": E, "nvp": nvp, "ms_step": ms_step, "wall_ms": wall_ms,
+        "V": V, "E": E, "nvp": nvp, "ms_step": ms_step, "wall_ms": wall_ms, "sliced": sliced,
         "value": E / (ms_step * 1e-3) / 1e9,
         "phases_ms_per_iteration": {k: round(ms / n, 4) for k, (ms, n) in kt.items() if n},
-        "roofline": {"bound": "hbm", "kernel": "gz_k_edge_pull_pr", "achieved": round(achieved, 1),
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": read_traffic("gz_k_edge_pull_pr", FULL), "bytes_per_launch": B,
+                     "traffic": read_traffic(kernel, FULL), "bytes_per_launch": B,
+                     "streamed_vectors": sliced_vectors,
                      "avg_launch_us": round(avg * 1e3, 2), "launches": pn,
                      "kernel_gteps": round(E / (avg * 1e-3) / 1e9, 2)},
         "rank_sum": float(ranks.sum()),
@@ -258,6 +277,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--only-headline", action="store_true", help="skip the other configurations (profiling)")
     ap.add_argument("--no-relabel", action="store_true")
+    ap.add_argument("--plain-pull", action="store_true", help="headline over the ordinary pull list")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
@@ -313,6 +333,8 @@ def main() -> int:
         "config": {"workload": FULL + " PageRank pull (FP64, damping 0.85)", "num_vertices": h["V"],
                    "num_edges": h["E"], "pull_vectors": h["nvp"],
                    "ingress": "device, renumbered by out-degree" if not args.no_relabel else "device",
+                   "pull": (f"source-sliced (S=2^{SLICES_LOG2}, LDS cache {HOT_CACHE}, chunk {SLICED_CHUNK})"
+                            if h["sliced"] else "scheduler-aware, ordinary pull list"),
                    "device": dev},
         "roofline": h["roofline"],
         "cpu_baseline": cpu,
