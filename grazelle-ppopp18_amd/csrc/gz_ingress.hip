@@ -179,10 +179,10 @@ unsigned gz_grid_for(uint64_t n)
     return (unsigned)(blocks < 65536 ? (blocks ? blocks : 1) : 65536);
 }
 
-/* Slice size actually used: at most V / 8, so the eight windows fit. */
-uint64_t gz_slices_effective(uint64_t slice_vertices, uint64_t nv)
+/* Slice size actually used: at most V / (8 T), so the windows fit. */
+uint64_t gz_slices_effective(uint64_t slice_vertices, uint32_t tiers, uint64_t nv)
 {
-    const uint64_t cap = nv / GZ_XCDS;
+    const uint64_t cap = nv / ((uint64_t)GZ_XCDS * tiers);
     return slice_vertices < cap ? slice_vertices : cap;
 }
 
@@ -257,16 +257,22 @@ __global__ void gz_k_degree_order(const uint32_t* __restrict__ cnt, uint64_t nv,
         order[v] = ((uint64_t)(0xffffffffu - cnt[v]) << 32) | v;
 }
 
-/* Rank i of the degree order becomes id i, except that the first 8*S ranks
- * are dealt round robin into eight S-id windows: rank i -> (i % 8) * S + i / 8. */
+/* Rank i of the degree order becomes id i, except that each of the first
+ * `tiers` runs of 8*S ranks is dealt round robin into eight S-id windows:
+ * rank t*8S + q -> t*8S + (q % 8) * S + q / 8. */
 __global__ void gz_k_permutation(const uint64_t* __restrict__ order, uint64_t nv, uint64_t slice_vertices,
-                                 uint64_t* __restrict__ old_of_new, uint32_t* __restrict__ new_of_old)
+                                 uint32_t tiers, uint64_t* __restrict__ old_of_new, uint32_t* __restrict__ new_of_old)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t hot = (uint64_t)GZ_XCDS * slice_vertices;
+    const uint64_t tier = (uint64_t)GZ_XCDS * slice_vertices;
+    const uint64_t hot = tier * tiers;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nv; i += stride) {
         const uint64_t old = order[i] & 0xffffffffull;
-        const uint64_t id = (i < hot) ? (i % GZ_XCDS) * slice_vertices + i / GZ_XCDS : i;
+        uint64_t id = i;
+        if (i < hot) {
+            const uint64_t t = i / tier, q = i - t * tier;
+            id = t * tier + (q % GZ_XCDS) * slice_vertices + q / GZ_XCDS;
+        }
         old_of_new[id] = old;
         new_of_old[old] = (uint32_t)id;
     }
@@ -288,7 +294,8 @@ __global__ void gz_k_remap_keys(uint64_t* __restrict__ keys, uint64_t n, uint32_
  * indexed by new id.  The hot sources of a power-law graph then share cache
  * lines, which is what the pull gathers need once the vertex array no longer
  * fits in the caches. */
-uint64_t* gz_relabel_by_outdegree(uint64_t* keys, uint64_t n, uint32_t b, uint64_t nv, uint64_t slice_vertices)
+uint64_t* gz_relabel_by_outdegree(uint64_t* keys, uint64_t n, uint32_t b, uint64_t nv, uint64_t slice_vertices,
+                                  uint32_t tiers)
 {
     uint32_t* cnt = (uint32_t*)gz_device_malloc(nv * sizeof(uint32_t));
     GZ_CHECK(hipMemsetAsync(cnt, 0, nv * sizeof(uint32_t), gz_stream));
@@ -305,7 +312,7 @@ uint64_t* gz_relabel_by_outdegree(uint64_t* keys, uint64_t n, uint32_t b, uint64
     uint64_t* old_of_new = (uint64_t*)gz_device_malloc((nv + 8) * sizeof(uint64_t));
     uint32_t* new_of_old = cnt; /* counts are no longer needed */
     hipLaunchKernelGGL(gz_k_permutation, dim3(gz_grid_for(nv)), dim3(GZ_BLOCK), 0, gz_stream, sorted, nv,
-                       slice_vertices, old_of_new, new_of_old);
+                       slice_vertices, tiers, old_of_new, new_of_old);
     GZ_LAUNCH_CHECK();
     if (n) {
         hipLaunchKernelGGL(gz_k_remap_keys, dim3(gz_grid_for(n)), dim3(GZ_BLOCK), 0, gz_stream, keys, n, b, new_of_old);
@@ -323,9 +330,12 @@ uint64_t* gz_relabel_by_outdegree(uint64_t* keys, uint64_t n, uint32_t b, uint64
 void gz_ingress_from_keys(uint64_t nv, uint64_t* a, uint64_t* b_buf, uint64_t n, uint32_t b, uint32_t flags)
 {
     const uint32_t slice_log2 = (flags >> 8) & 0x3fu;
-    const uint64_t slice_vertices = slice_log2 ? gz_slices_effective(1ull << slice_log2, nv) : 0;
+    uint32_t tiers = (flags >> 14) & 0x3u;
+    if (tiers == 0) tiers = 1;
+    const uint64_t slice_vertices = slice_log2 ? gz_slices_effective(1ull << slice_log2, tiers, nv) : 0;
     uint64_t* old_of_new = nullptr;
-    if (flags & GZ_INGRESS_RELABEL_BY_OUTDEGREE) old_of_new = gz_relabel_by_outdegree(a, n, b, nv, slice_vertices);
+    if (flags & GZ_INGRESS_RELABEL_BY_OUTDEGREE)
+        old_of_new = gz_relabel_by_outdegree(a, n, b, nv, slice_vertices, tiers);
     uint64_t* sorted = a;
     if (n > 1) gz_sort_keys(a, b_buf, n, 2 * b, &sorted);
     uint64_t* other = (sorted == a) ? b_buf : a;
@@ -366,7 +376,7 @@ void gz_ingress_from_keys(uint64_t nv, uint64_t* a, uint64_t* b_buf, uint64_t n,
     GZ_CHECK(hipStreamSynchronize(gz_stream));
     gz_device_free(eoff);
     gz_graph_adopt_device(&L);
-    if (slice_log2) gz_graph_build_source_slices(1ull << slice_log2);
+    if (slice_log2) gz_graph_build_source_slices(1ull << slice_log2, tiers);
 }
 
 void gz_check_vertex_count(uint64_t nv)
@@ -494,14 +504,14 @@ extern "C" int gz_graph_build_rmat_union_device(const gz_rmat_params_t* params, 
 /* ---- source-sliced pull layout ---------------------------------------------
  * From the loaded pull vectors: every valid lane becomes the key
  * (slice << 2b | dst << b | src) (invalid lanes all ones), a stable radix pass
- * over the 4 slice bits groups the keys by slice while keeping the (dst, src)
+ * over the 5 slice bits groups the keys by slice while keeping the (dst, src)
  * order, and each slice's keys are packed by the layout builder above. */
 namespace {
 
 __global__ void gz_k_slice_keys(const gz_vec_t* __restrict__ vecs, uint64_t nvec, uint32_t b, uint64_t slice_vertices,
-                                uint64_t* __restrict__ keys)
+                                uint32_t tiers, uint64_t* __restrict__ keys)
 {
-    const uint64_t hot = (uint64_t)GZ_XCDS * slice_vertices;
+    const uint64_t hot = (uint64_t)GZ_XCDS * tiers * slice_vertices;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 4 * nvec; i += stride) {
         const gz_vec_t* v = vecs + i / 4;
@@ -517,16 +527,17 @@ __global__ void gz_k_slice_keys(const gz_vec_t* __restrict__ vecs, uint64_t nvec
         w.l3 = v->lane[3];
         const uint64_t d = gz_shared_vertex(w);
         const uint64_t s = gz_lane_id(l);
-        const uint64_t x = (s < hot) ? s / slice_vertices : (uint64_t)GZ_XCDS;
+        const uint64_t x = (s < hot) ? s / slice_vertices : (uint64_t)GZ_XCDS * tiers;
         keys[i] = (x << (2 * b)) | (d << b) | s;
     }
 }
 
-/* first[x] = first key whose slice field is >= x, x in [0, GZ_SLICES] */
-__global__ void gz_k_slice_bounds(const uint64_t* __restrict__ keys, uint64_t n, uint32_t b, uint64_t* __restrict__ first)
+/* first[x] = first key whose slice field is >= x, x in [0, nslices] */
+__global__ void gz_k_slice_bounds(const uint64_t* __restrict__ keys, uint64_t n, uint32_t b, uint32_t nslices,
+                                  uint64_t* __restrict__ first)
 {
     const uint64_t x = threadIdx.x;
-    if (x > GZ_SLICES) return;
+    if (x > nslices) return;
     uint64_t lo = 0, hi = n;
     while (lo < hi) {
         const uint64_t mid = lo + ((hi - lo) >> 1);
@@ -545,16 +556,51 @@ __global__ void gz_k_strip_slice(uint64_t* __restrict__ keys, uint64_t n, uint32
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) keys[i] &= mask;
 }
 
-__global__ void gz_k_slice_mask(const uint64_t* __restrict__ voff, uint64_t nv, uint32_t bit, uint16_t* __restrict__ mask)
+/* mask bit for slice `bit`, and has[v] = 1 when v has vectors in the slice
+ * (has[nv] = 0, so its exclusive scan ends in the slice's vertex count) */
+__global__ void gz_k_slice_mask(const uint64_t* __restrict__ voff, uint64_t nv, uint32_t bit, uint32_t* __restrict__ mask,
+                                uint64_t* __restrict__ has)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride)
-        if (voff[v + 1] > voff[v]) mask[v] |= (uint16_t)(1u << bit);
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v <= nv; v += stride) {
+        const bool in = v < nv && voff[v + 1] > voff[v];
+        if (in) mask[v] |= 1u << bit;
+        has[v] = in ? 1 : 0;
+    }
+}
+
+/* local rank in slice `x` of the first vertex of each 64-vertex block, at
+ * out[block * nslices + x] */
+__global__ void gz_k_rank_base(const uint64_t* __restrict__ rank, uint64_t blocks, uint32_t x, uint32_t nslices,
+                               uint32_t* __restrict__ out)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < blocks; k += stride)
+        out[k * nslices + x] = (uint32_t)rank[k * GZ_WAVE];
+}
+
+/* the shared vertex of every vector becomes the destination's local rank */
+__global__ void gz_k_local_dst(gz_vec_t* __restrict__ vecs, uint64_t n, const uint64_t* __restrict__ rank)
+{
+    const uint64_t clear = ~(0x7fffull << 48);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        gz_lanes w;
+        w.l0 = vecs[j].lane[0];
+        w.l1 = vecs[j].lane[1];
+        w.l2 = vecs[j].lane[2];
+        w.l3 = vecs[j].lane[3];
+        const uint64_t r = rank[gz_shared_vertex(w)];
+        vecs[j].lane[0] = (w.l0 & clear) | gz_piece(r, 0);
+        vecs[j].lane[1] = (w.l1 & clear) | gz_piece(r, 1);
+        vecs[j].lane[2] = (w.l2 & clear) | gz_piece(r, 2);
+        vecs[j].lane[3] = (w.l3 & clear) | gz_piece(r, 3);
+    }
 }
 
 }  // namespace
 
-extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices)
+extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t tiers)
 {
     gz_runtime_init(0);
     if (gz_graph.pull_vectors == nullptr) {
@@ -563,69 +609,99 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices)
     }
     const uint64_t nv = gz_graph.num_vertices, nvec = gz_graph.pull_vector_count;
     const uint32_t b = gz_id_bits(nv);
-    if (2 * b + 4 > 64) {
-        fprintf(stderr, "grazelle: the source-sliced layout needs V <= 2^30 (V = %llu)\n", (unsigned long long)nv);
+    if (2 * b + 5 > 64) {
+        fprintf(stderr, "grazelle: the source-sliced layout needs V <= 2^29 (V = %llu)\n", (unsigned long long)nv);
+        exit(65);
+    }
+    if (tiers < 1 || tiers > GZ_MAX_TIERS) {
+        fprintf(stderr, "grazelle: source-sliced tiers must be 1..%d (got %u)\n", GZ_MAX_TIERS, tiers);
         exit(65);
     }
     gz_source_slices_release();
-    const uint64_t S = gz_slices_effective(slice_vertices, nv);
+    const uint64_t S = gz_slices_effective(slice_vertices, tiers, nv);
+    const uint32_t ns = GZ_XCDS * tiers + 1;
     const uint64_t n = 4 * nvec;
     uint64_t* a = (uint64_t*)gz_device_malloc((n ? n : 1) * sizeof(uint64_t));
     uint64_t* bb = (uint64_t*)gz_device_malloc((n ? n : 1) * sizeof(uint64_t));
-    uint64_t* first = (uint64_t*)gz_device_malloc((GZ_SLICES + 1) * sizeof(uint64_t));
-    uint64_t lo[GZ_SLICES + 1];
+    uint64_t* first = (uint64_t*)gz_device_malloc((GZ_MAX_SLICES + 1) * sizeof(uint64_t));
+    uint64_t lo[GZ_MAX_SLICES + 1];
     uint64_t* sorted = a;
     if (n) {
         hipLaunchKernelGGL(gz_k_slice_keys, dim3(gz_grid_for(n)), dim3(GZ_BLOCK), 0, gz_stream, gz_graph.pull_vectors,
-                           nvec, b, S, a);
+                           nvec, b, S, tiers, a);
         GZ_LAUNCH_CHECK();
         rocprim::double_buffer<uint64_t> db(a, bb);
         size_t tmp_bytes = 0;
-        GZ_CHECK(rocprim::radix_sort_keys(nullptr, tmp_bytes, db, (size_t)n, 2 * b, 2 * b + 4, gz_stream));
+        GZ_CHECK(rocprim::radix_sort_keys(nullptr, tmp_bytes, db, (size_t)n, 2 * b, 2 * b + 5, gz_stream));
         void* tmp = gz_device_malloc(tmp_bytes ? tmp_bytes : 1);
-        GZ_CHECK(rocprim::radix_sort_keys(tmp, tmp_bytes, db, (size_t)n, 2 * b, 2 * b + 4, gz_stream));
+        GZ_CHECK(rocprim::radix_sort_keys(tmp, tmp_bytes, db, (size_t)n, 2 * b, 2 * b + 5, gz_stream));
         GZ_CHECK(hipStreamSynchronize(gz_stream));
         gz_device_free(tmp);
         sorted = db.current();
-        hipLaunchKernelGGL(gz_k_slice_bounds, dim3(1), dim3(64), 0, gz_stream, sorted, n, b, first);
+        hipLaunchKernelGGL(gz_k_slice_bounds, dim3(1), dim3(64), 0, gz_stream, sorted, n, b, ns, first);
         GZ_LAUNCH_CHECK();
-        gz_copy_to_host(lo, first, sizeof(lo));
-        hipLaunchKernelGGL(gz_k_strip_slice, dim3(gz_grid_for(lo[GZ_SLICES])), dim3(GZ_BLOCK), 0, gz_stream, sorted,
-                           lo[GZ_SLICES], b);
+        gz_copy_to_host(lo, first, (ns + 1) * sizeof(uint64_t));
+        hipLaunchKernelGGL(gz_k_strip_slice, dim3(gz_grid_for(lo[ns])), dim3(GZ_BLOCK), 0, gz_stream, sorted, lo[ns],
+                           b);
         GZ_LAUNCH_CHECK();
     } else {
-        for (int x = 0; x <= GZ_SLICES; ++x) lo[x] = 0;
+        for (uint32_t x = 0; x <= ns; ++x) lo[x] = 0;
     }
-    gz_slices.mask = (uint16_t*)gz_device_malloc((nv + 8) * sizeof(uint16_t));
+    gz_slices.mask = (uint32_t*)gz_device_malloc((nv + 8) * sizeof(uint32_t));
+    gz_slices.rank_blocks = (nv + GZ_WAVE - 1) / GZ_WAVE;
+    gz_slices.rank_base = (uint32_t*)gz_device_malloc(ns * gz_slices.rank_blocks * sizeof(uint32_t));
     uint64_t* eoff = (uint64_t*)gz_device_malloc((nv + 1) * sizeof(uint64_t));
-    gz_vec_t* part_vecs[GZ_SLICES];
-    uint64_t counts[GZ_SLICES];
-    for (int x = 0; x < GZ_SLICES; ++x) {
+    uint64_t* has = (uint64_t*)gz_device_malloc((nv + 1) * sizeof(uint64_t));
+    uint64_t* rank = (uint64_t*)gz_device_malloc((nv + 1) * sizeof(uint64_t));
+    size_t scan_bytes = 0;
+    GZ_CHECK(rocprim::exclusive_scan(nullptr, scan_bytes, has, rank, (uint64_t)0, (size_t)(nv + 1),
+                                     rocprim::plus<uint64_t>(), gz_stream));
+    void* scan_tmp = gz_device_malloc(scan_bytes ? scan_bytes : 1);
+    gz_vec_t* part_vecs[GZ_MAX_SLICES];
+    uint64_t counts[GZ_MAX_SLICES], ranked[GZ_MAX_SLICES];
+    for (uint32_t x = 0; x < ns; ++x) {
         uint64_t* voff = nullptr;
         gz_build_layout(sorted + lo[x], lo[x + 1] - lo[x], b, nv, &part_vecs[x], &voff, &counts[x], eoff);
-        hipLaunchKernelGGL(gz_k_slice_mask, dim3(gz_grid_for(nv)), dim3(GZ_BLOCK), 0, gz_stream, voff, nv, (uint32_t)x,
-                           gz_slices.mask);
+        hipLaunchKernelGGL(gz_k_slice_mask, dim3(gz_grid_for(nv + 1)), dim3(GZ_BLOCK), 0, gz_stream, voff, nv,
+                           x, gz_slices.mask, has);
         GZ_LAUNCH_CHECK();
+        GZ_CHECK(rocprim::exclusive_scan(scan_tmp, scan_bytes, has, rank, (uint64_t)0, (size_t)(nv + 1),
+                                         rocprim::plus<uint64_t>(), gz_stream));
+        hipLaunchKernelGGL(gz_k_rank_base, dim3(gz_grid_for(gz_slices.rank_blocks)), dim3(GZ_BLOCK), 0, gz_stream,
+                           rank, gz_slices.rank_blocks, x, ns, gz_slices.rank_base);
+        GZ_LAUNCH_CHECK();
+        if (counts[x]) {
+            hipLaunchKernelGGL(gz_k_local_dst, dim3(gz_grid_for(counts[x])), dim3(GZ_BLOCK), 0, gz_stream,
+                               part_vecs[x], counts[x], rank);
+            GZ_LAUNCH_CHECK();
+        }
+        gz_copy_to_host(&ranked[x], rank + nv, sizeof(uint64_t));
         GZ_CHECK(hipStreamSynchronize(gz_stream));
         gz_device_free(voff);
     }
+    gz_device_free(scan_tmp);
+    gz_device_free(rank);
+    gz_device_free(has);
     gz_device_free(eoff);
     gz_device_free(first);
     gz_device_free(a);
     gz_device_free(bb);
     gz_slices.begin[0] = 0;
-    for (int x = 0; x < GZ_SLICES; ++x) gz_slices.begin[x + 1] = gz_slices.begin[x] + counts[x];
-    const uint64_t total = gz_slices.begin[GZ_SLICES];
+    for (uint32_t x = 0; x < ns; ++x) gz_slices.begin[x + 1] = gz_slices.begin[x] + counts[x];
+    const uint64_t total = gz_slices.begin[ns];
     gz_slices.vecs = (gz_vec_t*)gz_device_malloc((total ? total : 1) * sizeof(gz_vec_t));
-    for (int x = 0; x < GZ_SLICES; ++x) {
+    for (uint32_t x = 0; x < ns; ++x) {
         if (counts[x]) gz_device_copy(gz_slices.vecs + gz_slices.begin[x], part_vecs[x], counts[x] * sizeof(gz_vec_t));
         GZ_CHECK(hipStreamSynchronize(gz_stream));
         gz_device_free(part_vecs[x]);
     }
-    gz_slices.part_stride = nv + 8;
-    gz_slices.parts = (double*)gz_device_malloc(GZ_SLICES * gz_slices.part_stride * sizeof(double));
+    memset(&gz_slices.part_base, 0, sizeof(gz_slices.part_base));
+    for (uint32_t x = 0; x < ns; ++x) gz_slices.part_base.off[x + 1] = gz_slices.part_base.off[x] + ranked[x];
+    gz_slices.parts = (double*)gz_device_malloc((gz_slices.part_base.off[ns] + 8) * sizeof(double));
     gz_slices.base_list = gz_graph.pull_vectors;
     gz_slices.base_count = nvec;
     gz_slices.slice_vertices = S;
+    gz_slices.tiers = tiers;
+    gz_slices.nslices = ns;
     return 0;
 }

@@ -61,25 +61,39 @@ extern hipStream_t gz_stream;
 extern uint32_t gz_cu_count; /* compute units of the device */
 
 /* ---- source-sliced pull layout, owned by gz_slices.hip -------------------
- * The pull vectors split by source: hot slice x < GZ_XCDS holds the edges
- * whose source lies in [x*S, (x+1)*S), the last slice every other edge.  Each
- * slice is an ordinary pull list (dst-grouped), and the slices are stored one
- * after another.  Workgroups are dispatched to the XCDs round robin, so a
- * chunk table that gives XCD x only slice x's chunks (plus a share of the
- * last slice) keeps each XCD's gathers inside one S-vertex window that its
- * own L2 can hold. */
+ * The pull vectors split by source.  Tier t < T holds eight hot slices: slice
+ * 8t + x holds the edges whose source lies in [(8t + x) S, (8t + x + 1) S);
+ * the last slice (8T) every other edge.  Each slice is an ordinary pull list
+ * (dst-grouped), and the slices are stored one after another.  Workgroups are
+ * dispatched to the XCDs round robin, so a chunk table that gives XCD x only
+ * the chunks of slices x, 8 + x, ... (tier by tier) plus a share of the last
+ * slice keeps each XCD's gathers inside one S-vertex window at a time, which
+ * its own L2 can hold. */
 #define GZ_XCDS 8
-#define GZ_SLICES (GZ_XCDS + 1)
+#define GZ_MAX_TIERS 3
+#define GZ_MAX_SLICES (GZ_XCDS * GZ_MAX_TIERS + 1)
+
+/* Offsets of the slices' accumulator runs in one array. */
+typedef struct gz_slice_offsets {
+    uint64_t off[GZ_MAX_SLICES + 1];
+} gz_slice_offsets;
 
 typedef struct gz_source_slices_t {
     const gz_vec_t* base_list;        /* the pull list the layout was built from */
     uint64_t base_count;
     uint64_t slice_vertices;          /* S */
+    uint32_t tiers;                   /* T */
+    uint32_t nslices;                 /* 8T + 1 */
     gz_vec_t* vecs;                   /* all slices, concatenated */
-    uint64_t begin[GZ_SLICES + 1];    /* first vector of each slice in vecs */
-    uint16_t* mask;                   /* [V] bit x: the vertex has vectors in slice x */
-    double* parts;                    /* slice x's accumulators at parts + x * part_stride */
-    uint64_t part_stride;
+    uint64_t begin[GZ_MAX_SLICES + 1]; /* first vector of each slice in vecs */
+    uint32_t* mask;                   /* [V] bit x: the vertex has vectors in slice x */
+    /* Inside slice x, destinations are numbered densely (local rank: the
+     * count of lower vertices that have vectors in slice x), and the shared
+     * vertex of slice x's vectors is that local rank. */
+    double* parts;                    /* slice x's accumulators at parts + part_base.off[x], by local rank */
+    gz_slice_offsets part_base;
+    uint32_t* rank_base;              /* [rank_blocks][nslices] local rank of each 64-vertex block's first vertex */
+    uint64_t rank_blocks;             /* ceil(V / 64) */
     gz_slice_chunk_t* table;          /* chunk slots, built for table_chunk_len */
     uint64_t slots;
     uint64_t table_chunk_len;

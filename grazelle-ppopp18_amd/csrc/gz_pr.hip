@@ -183,37 +183,24 @@ extern "C" void perform_edge_push_phase_pr(const gz_vec_t* edge_list, uint64_t e
     gz_ktimer_end(GZ_K_PUSH_PR);
 }
 
-/* acc[v] after a source-sliced pull: the sums of the slices holding v, added
- * in slice order (all loads issued before the adds); 0 with no in-edges. */
-__device__ __forceinline__ double gz_slices_sum(const uint16_t* __restrict__ mask, const double* __restrict__ parts,
-                                                uint64_t part_stride, uint64_t v)
-{
-    const uint32_t m = mask[v];
-    double p[GZ_SLICES];
-#pragma unroll
-    for (int x = 0; x < GZ_SLICES; ++x) p[x] = ((m >> x) & 1u) ? parts[(uint64_t)x * part_stride + v] : 0.0;
-    double s = 0.0;
-    bool first = true;
-#pragma unroll
-    for (int x = 0; x < GZ_SLICES; ++x)
-        if ((m >> x) & 1u) {
-            s = first ? p[x] : s + p[x];
-            first = false;
-        }
-    return s;
-}
-
 /* props[v] = ((acc[v] + (1-S)/V) * d + (1-d)/V) / outdeg'(v), outdeg' = V
  * for sinks, as separate IEEE mul/add/div (this file is built with
  * -ffp-contract=off, matching the asm's vaddpd/vmulpd/vaddpd/vdivpd). */
-template <bool kSliced>
+__device__ __forceinline__ double gz_pr_vertex_value(double a, double od, double c, double k, double damping,
+                                                     double num_vertices)
+{
+    if (od == 0.0) od = num_vertices;
+    double x = a + c;
+    x = x * damping;
+    x = x + k;
+    return x / od;
+}
+
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr(double* __restrict__ props, double* __restrict__ acc,
                                                            const double* __restrict__ outdeg, uint64_t start,
                                                            uint64_t end, double num_vertices,
                                                            const uint64_t* __restrict__ reduce_buffer,
-                                                           double damping, uint32_t zero_acc,
-                                                           const uint16_t* __restrict__ mask,
-                                                           const double* __restrict__ parts, uint64_t part_stride)
+                                                           double damping, uint32_t zero_acc)
 {
     const uint64_t sbits = reduce_buffer[0];
     const double S = __longlong_as_double((long long)sbits);
@@ -221,19 +208,58 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr(double* __restrict__ 
     const double k = (1.0 - damping) / num_vertices;
     for (uint64_t v = start + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < end;
          v += (uint64_t)gridDim.x * blockDim.x) {
-        double od = outdeg[v];
-        if (od == 0.0) od = num_vertices;
-        double a;
-        if (kSliced) {
-            a = gz_slices_sum(mask, parts, part_stride, v);
-        } else {
-            a = acc[v];
-            if (zero_acc) acc[v] = 0.0;
+        const double a = acc[v];
+        if (zero_acc) acc[v] = 0.0;
+        props[v] = gz_pr_vertex_value(a, outdeg[v], c, k, damping, num_vertices);
+    }
+}
+
+/* The vertex phase after a source-sliced pull: acc[v] is the sum of the
+ * slices holding v (mask bit x), added in slice order, 0 without in-edges.
+ * Slice x keeps its sums densely by local rank; one wavefront takes 64
+ * consecutive vertices, and v's rank in slice x is the rank of the block's
+ * first vertex (rank_base[block][x]) plus the lanes below v that are also in
+ * slice x, so each slice's sums are read as one contiguous run. */
+template <int NS>
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr_sliced(
+    double* __restrict__ props, const double* __restrict__ outdeg, uint64_t start, uint64_t end,
+    uint64_t num_vertices, const uint64_t* __restrict__ reduce_buffer, double damping,
+    const uint32_t* __restrict__ mask, const double* __restrict__ parts, gz_slice_offsets part_base,
+    const uint32_t* __restrict__ rank_base)
+{
+    const double nv = (double)num_vertices;
+    const double S = __longlong_as_double((long long)reduce_buffer[0]);
+    const double c = (1.0 - S) / nv;
+    const double k = (1.0 - damping) / nv;
+    const int lane = gz_lane();
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint64_t last = (end + GZ_WAVE - 1) / GZ_WAVE;
+    const uint64_t waves = (uint64_t)gridDim.x * GZ_WAVES_PER_BLOCK;
+    for (uint64_t blk = start / GZ_WAVE + (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + threadIdx.x / GZ_WAVE;
+         blk < last; blk += waves) {
+        const uint64_t v = blk * GZ_WAVE + (uint64_t)lane;
+        const uint32_t m = (v < num_vertices) ? mask[v] : 0u;
+        const uint32_t* rb = rank_base + blk * (uint64_t)NS;
+        double p[NS];
+#pragma unroll
+        for (int x = 0; x < NS; ++x) {
+            const bool has = (m >> x) & 1u;
+            const uint64_t bal = __ballot(has);
+            p[x] = 0.0;
+            if (bal != 0) { /* wave-uniform: no block vertex in slice x, no load */
+                const uint64_t idx = part_base.off[x] + rb[x] + __popcll(bal & below);
+                if (has) p[x] = parts[idx];
+            }
         }
-        double x = a + c;
-        x = x * damping;
-        x = x + k;
-        props[v] = x / od;
+        double a = 0.0;
+        bool first = true;
+#pragma unroll
+        for (int x = 0; x < NS; ++x)
+            if ((m >> x) & 1u) {
+                a = first ? p[x] : a + p[x];
+                first = false;
+            }
+        if (v >= start && v < end) props[v] = gz_pr_vertex_value(a, outdeg[v], c, k, damping, nv);
     }
 }
 
@@ -244,16 +270,29 @@ extern "C" void perform_vertex_phase_pr(uint64_t vertex_start, uint64_t vertex_c
     uint64_t blocks = (vertex_count + GZ_BLOCK - 1) / GZ_BLOCK;
     if (blocks > 4096) blocks = 4096;
     gz_ktimer_begin(GZ_K_VERTEX_PR);
-#define GZ_VERTEX_PR_ARGS                                                                                      \
-    dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, (double*)gz_graph.vertex_props,                        \
-        (double*)gz_graph.vertex_accumulators, gz_graph.vertex_outdegrees, vertex_start, end,                    \
-        (double)gz_graph.num_vertices, reduce_buffer, gz_phase_config.damping, gz_pr_acc_needs_zero,             \
-        (const uint16_t*)gz_slices.mask, (const double*)gz_slices.parts, gz_slices.part_stride
-    if (gz_pr_acc_in_slices && gz_slices.mask != nullptr)
-        hipLaunchKernelGGL(gz_k_vertex_pr<true>, GZ_VERTEX_PR_ARGS);
-    else
-        hipLaunchKernelGGL(gz_k_vertex_pr<false>, GZ_VERTEX_PR_ARGS);
-#undef GZ_VERTEX_PR_ARGS
+    if (gz_pr_acc_in_slices && gz_slices.mask != nullptr) {
+        /* wavefronts stride over the 64-vertex blocks */
+        const uint64_t vblocks = (end + GZ_WAVE - 1) / GZ_WAVE - vertex_start / GZ_WAVE;
+        uint64_t sblocks = (vblocks + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+        if (sblocks > 4096) sblocks = 4096;
+#define GZ_VERTEX_SLICED_ARGS                                                                                  \
+    dim3((unsigned)sblocks), dim3(GZ_BLOCK), 0, gz_stream, (double*)gz_graph.vertex_props,                        \
+        gz_graph.vertex_outdegrees, vertex_start, end, gz_graph.num_vertices, reduce_buffer,                     \
+        gz_phase_config.damping, (const uint32_t*)gz_slices.mask, (const double*)gz_slices.parts,                \
+        gz_slices.part_base, (const uint32_t*)gz_slices.rank_base
+        if (gz_slices.nslices == 3 * GZ_XCDS + 1)
+            hipLaunchKernelGGL(gz_k_vertex_pr_sliced<3 * GZ_XCDS + 1>, GZ_VERTEX_SLICED_ARGS);
+        else if (gz_slices.nslices == 2 * GZ_XCDS + 1)
+            hipLaunchKernelGGL(gz_k_vertex_pr_sliced<2 * GZ_XCDS + 1>, GZ_VERTEX_SLICED_ARGS);
+        else
+            hipLaunchKernelGGL(gz_k_vertex_pr_sliced<GZ_XCDS + 1>, GZ_VERTEX_SLICED_ARGS);
+#undef GZ_VERTEX_SLICED_ARGS
+    } else {
+        hipLaunchKernelGGL(gz_k_vertex_pr, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
+                           (double*)gz_graph.vertex_props, (double*)gz_graph.vertex_accumulators,
+                           gz_graph.vertex_outdegrees, vertex_start, end, (double)gz_graph.num_vertices,
+                           reduce_buffer, gz_phase_config.damping, gz_pr_acc_needs_zero);
+    }
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_VERTEX_PR);
 }

@@ -5,20 +5,22 @@
  * property per edge from anywhere in the vertex array.  On a power-law graph
  * renumbered by degree, the few million hottest sources carry most edges, but
  * their properties exceed one XCD's L2, and every XCD caches the same hot
- * lines.  Here the pull list is split by source into eight hot slices of S
- * vertices and a cold rest; the chunk table hands XCD x (workgroups w with
- * w % 8 == x, the dispatcher's round robin) only slice x's chunks plus an even
- * share of the cold slice, so each XCD gathers from one S-vertex window.
+ * lines.  Here the pull list is split by source into tiers of eight hot
+ * slices of S vertices and a cold rest; the chunk table hands XCD x
+ * (workgroups w with w % 8 == x, the dispatcher's round robin) only the
+ * chunks of its slices x, 8 + x, ..., tier after tier, plus an even share of
+ * the cold slice, so each XCD gathers from one S-vertex window at a time.
  *
- * Each slice is an ordinary pull list, walked exactly as in gz_pr.hip into
- * its own accumulator array (complete runs stored directly, the chunk's last
- * run into its merge record).  The merge then runs per slice
+ * Each slice is an ordinary pull list whose destinations are numbered
+ * densely inside the slice, walked exactly as in gz_pr.hip into its own
+ * accumulator run (complete runs stored directly, the chunk's last run into
+ * its merge record).  The merge then runs per slice
  * (edge_pull_op_merge_with_merge_buffer's rule, phases.c:41-70), and the
  * vertex phase takes acc[v] = the sum over the slices holding v, in slice
- * order (gz_k_vertex_pr reads the slices' accumulators directly, so the
- * accumulator array itself is not written).  No atomics, so the
- * result is reproducible run to run; the per-vertex sum is regrouped by slice,
- * which moves it by rounding only.
+ * order (gz_k_vertex_pr_sliced reads the slices' accumulators directly, so the
+ * accumulator array itself is not written).  No atomics, so the result is
+ * reproducible run to run; the per-vertex sum is regrouped by slice, which
+ * moves it by rounding only.
  */
 #include "grazelle_host.h"
 #include "gz_host_internal.h"
@@ -36,35 +38,41 @@ static uint32_t gz_slices_hot_cache = 0;
 
 /* ---- chunk table (host) ---------------------------------------------------- */
 
-extern "C" uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint64_t chunk_len, uint32_t cold_spread,
-                                          gz_slice_chunk_t* out, uint64_t capacity)
+extern "C" uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint32_t tiers, uint64_t chunk_len,
+                                          uint32_t cold_spread, gz_slice_chunk_t* out, uint64_t capacity)
 {
-    if (chunk_len == 0) return 0;
-    uint64_t base[GZ_SLICES + 1], chunks[GZ_SLICES], rec[GZ_SLICES + 1];
+    if (chunk_len == 0 || tiers < 1 || tiers > GZ_MAX_TIERS) return 0;
+    const int ns = GZ_XCDS * (int)tiers + 1, cold_slice = ns - 1;
+    uint64_t base[GZ_MAX_SLICES + 1], chunks[GZ_MAX_SLICES], rec[GZ_MAX_SLICES + 1];
     base[0] = 0;
     rec[0] = 0;
-    for (int x = 0; x < GZ_SLICES; ++x) {
+    for (int x = 0; x < ns; ++x) {
         chunks[x] = (vectors[x] + chunk_len - 1) / chunk_len;
         base[x + 1] = base[x] + vectors[x];
         rec[x + 1] = rec[x] + chunks[x];
     }
-    if (rec[GZ_SLICES] >= (1ull << GZ_SLICE_RECORD_BITS)) {
-        fprintf(stderr, "grazelle: %llu chunks exceed the source-sliced record field\n",
-                (unsigned long long)rec[GZ_SLICES]);
+    if (rec[ns] >= (1ull << GZ_SLICE_RECORD_BITS)) {
+        fprintf(stderr, "grazelle: %llu chunks exceed the source-sliced record field\n", (unsigned long long)rec[ns]);
         exit(72);
     }
-    /* chunks per XCD: an even share of all chunks, at least its own slice's */
-    uint64_t target = (rec[GZ_SLICES] + GZ_XCDS - 1) / GZ_XCDS;
+    /* an XCD's own chunks: its slice in every tier */
+    uint64_t own[GZ_XCDS];
+    for (int x = 0; x < GZ_XCDS; ++x) {
+        own[x] = 0;
+        for (uint32_t t = 0; t < tiers; ++t) own[x] += chunks[GZ_XCDS * t + x];
+    }
+    /* chunks per XCD: an even share of all chunks, at least its own */
+    uint64_t target = (rec[ns] + GZ_XCDS - 1) / GZ_XCDS;
     for (int x = 0; x < GZ_XCDS; ++x)
-        if (chunks[x] > target) target = chunks[x];
+        if (own[x] > target) target = own[x];
     uint64_t give[GZ_XCDS], offered = 0;
     for (int x = 0; x < GZ_XCDS; ++x) {
-        give[x] = target - chunks[x];
+        give[x] = target - own[x];
         offered += give[x];
     }
     /* take back what the cold slice cannot fill, one chunk per XCD in turn
      * from the XCDs that were offered the most */
-    uint64_t excess = offered - chunks[GZ_XCDS];
+    uint64_t excess = offered - chunks[cold_slice];
     while (excess > 0) {
         uint64_t most = 0;
         for (int x = 0; x < GZ_XCDS; ++x)
@@ -77,7 +85,7 @@ extern "C" uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint64_t chun
     }
     uint64_t len[GZ_XCDS], blocks = 0;
     for (int x = 0; x < GZ_XCDS; ++x) {
-        len[x] = chunks[x] + give[x];
+        len[x] = own[x] + give[x];
         const uint64_t bx = (len[x] + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
         if (bx > blocks) blocks = bx;
     }
@@ -91,6 +99,7 @@ extern "C" uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint64_t chun
     memset(out, 0, slots * sizeof(gz_slice_chunk_t));
     uint64_t cold_next = 0;
     for (int x = 0; x < GZ_XCDS; ++x) {
+        uint32_t tier = 0;
         uint64_t hot_next = 0;
         const uint64_t n = len[x], g = give[x];
         for (uint64_t i = 0; i < n; ++i) {
@@ -100,10 +109,14 @@ extern "C" uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint64_t chun
             int sl;
             uint64_t k;
             if (cold) {
-                sl = GZ_XCDS;
+                sl = cold_slice;
                 k = cold_next++;
             } else {
-                sl = x;
+                while (hot_next >= chunks[GZ_XCDS * tier + x]) {
+                    hot_next = 0;
+                    ++tier;
+                }
+                sl = GZ_XCDS * (int)tier + x;
                 k = hot_next++;
             }
             gz_slice_chunk_t c;
@@ -134,17 +147,29 @@ extern "C" void gz_source_slices_release(void)
     gz_device_free(gz_slices.vecs);
     gz_device_free(gz_slices.mask);
     gz_device_free(gz_slices.parts);
+    gz_device_free(gz_slices.rank_base);
     gz_device_free(gz_slices.table);
     gz_device_free(gz_slices.merge);
     gz_device_free(gz_slices.counters);
     memset(&gz_slices, 0, sizeof(gz_slices));
 }
 
-extern "C" int gz_graph_source_slices_info(uint64_t* vectors, uint64_t* slice_vertices)
+extern "C" int gz_graph_source_slices_info(uint64_t* vectors, uint64_t* slice_vertices, uint32_t* tiers)
 {
     if (gz_slices.vecs == nullptr) return 0;
-    for (int x = 0; x < GZ_SLICES; ++x) vectors[x] = gz_slices.begin[x + 1] - gz_slices.begin[x];
-    *slice_vertices = gz_slices.slice_vertices;
+    if (vectors)
+        for (uint32_t x = 0; x < gz_slices.nslices; ++x) vectors[x] = gz_slices.begin[x + 1] - gz_slices.begin[x];
+    if (slice_vertices) *slice_vertices = gz_slices.slice_vertices;
+    if (tiers) *tiers = gz_slices.tiers;
+    return 1;
+}
+
+extern "C" int gz_graph_source_slices_download(gz_vec_t* vectors, uint32_t* mask)
+{
+    if (gz_slices.vecs == nullptr) return 0;
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    if (vectors) gz_copy_to_host(vectors, gz_slices.vecs, gz_slices.begin[gz_slices.nslices] * sizeof(gz_vec_t));
+    if (mask) gz_copy_to_host(mask, gz_slices.mask, gz_graph.num_vertices * sizeof(uint32_t));
     return 1;
 }
 
@@ -155,13 +180,15 @@ static void gz_slices_prepare_table(uint64_t chunk_len)
     if (gz_slices.table != nullptr && gz_slices.table_chunk_len == chunk_len &&
         gz_slices.table_cold_spread == gz_slices_cold_spread)
         return;
-    uint64_t vectors[GZ_SLICES];
-    for (int x = 0; x < GZ_SLICES; ++x) vectors[x] = gz_slices.begin[x + 1] - gz_slices.begin[x];
-    const uint64_t slots = gz_slices_chunk_table(vectors, chunk_len, gz_slices_cold_spread, nullptr, 0);
+    const uint32_t ns = gz_slices.nslices;
+    uint64_t vectors[GZ_MAX_SLICES];
+    for (uint32_t x = 0; x < ns; ++x) vectors[x] = gz_slices.begin[x + 1] - gz_slices.begin[x];
+    const uint64_t slots =
+        gz_slices_chunk_table(vectors, gz_slices.tiers, chunk_len, gz_slices_cold_spread, nullptr, 0);
     gz_slice_chunk_t* h = (gz_slice_chunk_t*)malloc((slots ? slots : 1) * sizeof(gz_slice_chunk_t));
-    gz_slices_chunk_table(vectors, chunk_len, gz_slices_cold_spread, h, slots);
+    gz_slices_chunk_table(vectors, gz_slices.tiers, chunk_len, gz_slices_cold_spread, h, slots);
     uint64_t records = 0;
-    for (int x = 0; x < GZ_SLICES; ++x) records += (vectors[x] + chunk_len - 1) / chunk_len;
+    for (uint32_t x = 0; x < ns; ++x) records += (vectors[x] + chunk_len - 1) / chunk_len;
     GZ_CHECK(hipStreamSynchronize(gz_stream));
     gz_device_free(gz_slices.table);
     gz_device_free(gz_slices.merge);
@@ -183,7 +210,7 @@ static void gz_slices_prepare_table(uint64_t chunk_len)
 /* ---- kernels ---------------------------------------------------------------- */
 
 struct gz_slice_bounds {
-    uint64_t rec[GZ_SLICES + 1];
+    uint64_t rec[GZ_MAX_SLICES + 1];
 };
 
 /* Walk one chunk into slice accumulators `acc`; returns the wave's share of S. */
@@ -207,12 +234,14 @@ __device__ __forceinline__ double gz_sliced_walk(const gz_vec_t* __restrict__ ve
     return gz_wave_sum(op.sum);
 }
 
-/* kColdStream: the cold slice's gathers, which mostly miss, are nontemporal. */
+/* One wavefront per chunk slot.  kColdStream: the cold slice's gathers, which
+ * mostly miss, are nontemporal. */
 template <int VPL, bool kColdStream>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr_sliced(const gz_vec_t* __restrict__ vecs,
                                                                      const gz_slice_chunk_t* __restrict__ table,
                                                                      const double* __restrict__ props,
-                                                                     double* __restrict__ parts, uint64_t part_stride,
+                                                                     double* __restrict__ parts, gz_slice_offsets part_base,
+                                                                     uint32_t cold_slice,
                                                                      mergeaccum_t* __restrict__ merge,
                                                                      double* __restrict__ partials,
                                                                      uint64_t* __restrict__ stats, uint32_t collect)
@@ -233,9 +262,9 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr_sliced(const gz_ve
     const uint32_t x = c.record >> GZ_SLICE_RECORD_BITS;
     const uint64_t rec = c.record & ((1u << GZ_SLICE_RECORD_BITS) - 1u);
     const uint64_t b = c.begin, e = c.begin + c.count;
-    double* acc = parts + (uint64_t)x * part_stride;
+    double* acc = parts + part_base.off[x];
     uint32_t edges = 0;
-    const double S = (kColdStream && x == GZ_XCDS)
+    const double S = (kColdStream && x == cold_slice)
                          ? gz_sliced_walk<VPL, true>(vecs, b, e, props, acc, merge + rec, edges)
                          : gz_sliced_walk<VPL, false>(vecs, b, e, props, acc, merge + rec, edges);
     if (lane == 0) partials[slot] = S;
@@ -251,15 +280,15 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr_sliced(const gz_ve
 /* The merge of gz_k_merge_sum applied to each slice's own records and
  * accumulators: runs never cross a slice boundary. */
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_merge_sliced(const mergeaccum_t* __restrict__ mb,
-                                                              gz_slice_bounds bounds, double* __restrict__ parts,
-                                                              uint64_t part_stride)
+                                                              gz_slice_bounds bounds, uint32_t nslices,
+                                                              double* __restrict__ parts, gz_slice_offsets part_base)
 {
     const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= bounds.rec[GZ_SLICES]) return;
-    int x = 0;
+    if (u >= bounds.rec[nslices]) return;
+    uint32_t x = 0;
     while (u >= bounds.rec[x + 1]) ++x;
     const uint64_t lo = bounds.rec[x], hi = bounds.rec[x + 1];
-    double* acc = parts + (uint64_t)x * part_stride;
+    double* acc = parts + part_base.off[x];
     const mergeaccum_t r = mb[u];
     if (u > lo && mb[u - 1].final_vertex_id == r.final_vertex_id) return;
     double proposed = r.final_partial_value;
@@ -272,9 +301,9 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_merge_sliced(const mergeaccum_t
     acc[r.final_vertex_id] = proposed;
 }
 
-/* ---- persistent pull with the hottest sources of each slice in LDS -------------
+/* ---- persistent pull with the hottest sources of each XCD in LDS -------------
  * As many workgroups as fit on the device at once (LDS and registers
- * permitting) each copy the first `hot` properties of their XCD's slice
+ * permitting) each copy the first `hot` properties of their XCD's tier-0
  * window into LDS; their wavefronts then take that XCD's chunks from a
  * per-XCD counter.  Gathers that fall in the cached range read LDS instead of
  * issuing an L2 request.  A cache of 128 KiB or more leaves room for one
@@ -289,9 +318,16 @@ struct gz_xcd_lens {
     uint64_t len[GZ_XCDS];
 };
 
+/* Pointers with their address space spelled out: an LDS read and a global
+ * read stay two instructions (each under its own lane mask) instead of being
+ * folded into one generic (flat) load through a selected pointer. */
+typedef __attribute__((address_space(3))) const double gz_lds_f64;
+typedef __attribute__((address_space(1))) const double gz_global_f64;
+
+template <bool kStream>
 struct gz_pr_pull_op_hot {
-    const double* __restrict__ props;
-    const double* hot; /* LDS */
+    gz_global_f64* __restrict__ props;
+    gz_lds_f64* hot;
     uint64_t wlo, wn;
     double* __restrict__ acc;
     double sum;
@@ -300,7 +336,14 @@ struct gz_pr_pull_op_hot {
     __device__ __forceinline__ double fetch(uint64_t i) const
     {
         const uint64_t r = i - wlo;
-        return (r < wn) ? hot[r] : props[i];
+        double g;
+        if (r < wn)
+            g = hot[r];
+        else if (kStream)
+            g = __builtin_nontemporal_load(props + i);
+        else
+            g = props[i];
+        return g;
     }
     __device__ __forceinline__ double gather(const gz_lanes& v, bool in_range, uint32_t& edges)
     {
@@ -324,12 +367,13 @@ struct gz_pr_pull_op_hot {
     __device__ __forceinline__ void store(uint64_t d, double s) { acc[d] = s; }
 };
 
-template <int HOT>
+template <int HOT, int VPL>
 __global__ __launch_bounds__(gz_hot_wg<HOT>::value) void gz_k_edge_pull_pr_sliced_hot(
     const gz_vec_t* __restrict__ vecs, const gz_slice_chunk_t* __restrict__ table, gz_xcd_lens lens,
-    const double* __restrict__ props, double* __restrict__ parts, uint64_t part_stride,
+    const double* __restrict__ props, double* __restrict__ parts, gz_slice_offsets part_base,
     mergeaccum_t* __restrict__ merge, double* __restrict__ partials, uint64_t* __restrict__ stats, uint32_t collect,
-    uint32_t* __restrict__ counters, uint64_t slice_vertices, uint64_t hot_count)
+    uint32_t* __restrict__ counters, uint64_t slice_vertices, uint64_t hot_count, uint32_t cold_slice,
+    uint32_t cold_stream)
 {
     __shared__ double hot[HOT];
     const uint32_t x = blockIdx.x % GZ_XCDS;
@@ -348,13 +392,23 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value) void gz_k_edge_pull_pr_slice
         const gz_slice_chunk_t c = table[slot];
         const uint32_t sl = c.record >> GZ_SLICE_RECORD_BITS;
         const uint64_t rec = c.record & ((1u << GZ_SLICE_RECORD_BITS) - 1u);
-        gz_pr_pull_op_hot op{props, hot, (sl == x) ? lo : 0, (sl == x) ? hot_count : 0,
-                             parts + (uint64_t)sl * part_stride, 0.0};
         uint64_t first_d, carry_d;
         double carry_v;
         uint32_t edges = 0;
-        gz_walk_chunk<1, double>(vecs, c.begin, c.begin + c.count, op, first_d, carry_d, carry_v, edges);
-        const double S = gz_wave_sum(op.sum);
+        double sum;
+        gz_global_f64* gprops = (gz_global_f64*)props;
+        gz_lds_f64* lds = (gz_lds_f64*)hot;
+        if (cold_stream && sl == cold_slice) {
+            gz_pr_pull_op_hot<true> op{gprops, lds, 0, 0, parts + part_base.off[sl], 0.0};
+            gz_walk_chunk<VPL, double>(vecs, c.begin, c.begin + c.count, op, first_d, carry_d, carry_v, edges);
+            sum = op.sum;
+        } else {
+            gz_pr_pull_op_hot<false> op{gprops, lds, (sl == x) ? lo : 0, (sl == x) ? hot_count : 0,
+                                        parts + part_base.off[sl], 0.0};
+            gz_walk_chunk<VPL, double>(vecs, c.begin, c.begin + c.count, op, first_d, carry_d, carry_v, edges);
+            sum = op.sum;
+        }
+        const double S = gz_wave_sum(sum);
         if (lane == 0) {
             mergeaccum_t r;
             r.initial_vertex_id = first_d;
@@ -375,21 +429,34 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value) void gz_k_edge_pull_pr_slice
 }
 
 /* One workgroup per XCD and resident slot: the grid is exactly what fits. */
-template <int HOT>
-static void gz_launch_hot(const gz_xcd_lens& lens, uint64_t hot_count)
+template <int HOT, int VPL>
+static void gz_launch_hot_vpl(const gz_xcd_lens& lens, uint64_t hot_count)
 {
     constexpr int WG = gz_hot_wg<HOT>::value;
     int per_cu = 0;
-    GZ_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gz_k_edge_pull_pr_sliced_hot<HOT>, WG, 0));
+    GZ_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gz_k_edge_pull_pr_sliced_hot<HOT, VPL>, WG, 0));
     if (per_cu < 1) per_cu = 1;
     uint64_t per_xcd = gz_cu_count / GZ_XCDS;
     if (per_xcd == 0) per_xcd = 1;
     const uint64_t blocks = (uint64_t)GZ_XCDS * per_xcd * (uint64_t)per_cu;
-    hipLaunchKernelGGL(gz_k_edge_pull_pr_sliced_hot<HOT>, dim3((unsigned)blocks), dim3(WG), 0, gz_stream,
+    hipLaunchKernelGGL((gz_k_edge_pull_pr_sliced_hot<HOT, VPL>), dim3((unsigned)blocks), dim3(WG), 0, gz_stream,
                        gz_slices.vecs, gz_slices.table, lens, (const double*)gz_graph.vertex_props, gz_slices.parts,
-                       gz_slices.part_stride, gz_slices.merge, (double*)gz_scratch.partials, gz_scratch.stats,
-                       gz_phase_config.collect_stats, gz_slices.counters, gz_slices.slice_vertices, hot_count);
+                       gz_slices.part_base, gz_slices.merge, (double*)gz_scratch.partials, gz_scratch.stats,
+                       gz_phase_config.collect_stats, gz_slices.counters, gz_slices.slice_vertices, hot_count,
+                       gz_slices.nslices - 1, gz_slices_cold_stream);
     GZ_LAUNCH_CHECK();
+}
+
+template <int HOT>
+static void gz_launch_hot(const gz_xcd_lens& lens, uint64_t hot_count)
+{
+    const uint32_t vpl = gz_vectors_per_lane();
+    if (vpl == 4)
+        gz_launch_hot_vpl<HOT, 4>(lens, hot_count);
+    else if (vpl == 2)
+        gz_launch_hot_vpl<HOT, 2>(lens, hot_count);
+    else
+        gz_launch_hot_vpl<HOT, 1>(lens, hot_count);
 }
 
 /* ---- the phase ---------------------------------------------------------------- */
@@ -429,10 +496,11 @@ void gz_pull_pr_sliced(const gz_vec_t* edge_list, uint64_t edge_list_count)
     } else if (!hot && slots > 0) {
         const uint64_t blocks = slots / GZ_WAVES_PER_BLOCK;
         const uint32_t vpl = gz_vectors_per_lane();
+        const uint32_t cold_slice = gz_slices.nslices - 1;
         gz_ktimer_begin(GZ_K_PULL_PR);
 #define GZ_SLICED_ARGS                                                                                          \
     dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, gz_slices.vecs, gz_slices.table,                       \
-        (const double*)gz_graph.vertex_props, gz_slices.parts, gz_slices.part_stride, gz_slices.merge,           \
+        (const double*)gz_graph.vertex_props, gz_slices.parts, gz_slices.part_base, cold_slice, gz_slices.merge,  \
         (double*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats
         const bool cs = gz_slices_cold_stream != 0;
         if (vpl == 4 && cs)
@@ -454,24 +522,15 @@ void gz_pull_pr_sliced(const gz_vec_t* edge_list, uint64_t edge_list_count)
     gz_ktimer_begin(GZ_K_MERGE);
     if (gz_slices.records > 0) {
         gz_slice_bounds bounds;
-        bounds.rec[0] = 0;
-        for (int x = 0; x < GZ_SLICES; ++x) {
+        memset(&bounds, 0, sizeof(bounds));
+        for (uint32_t x = 0; x < gz_slices.nslices; ++x) {
             const uint64_t n = gz_slices.begin[x + 1] - gz_slices.begin[x];
             bounds.rec[x + 1] = bounds.rec[x] + (n + chunk_len - 1) / chunk_len;
         }
         const uint64_t mblocks = (gz_slices.records + GZ_BLOCK - 1) / GZ_BLOCK;
         hipLaunchKernelGGL(gz_k_merge_sliced, dim3((unsigned)mblocks), dim3(GZ_BLOCK), 0, gz_stream, gz_slices.merge,
-                           bounds, gz_slices.parts, gz_slices.part_stride);
+                           bounds, gz_slices.nslices, gz_slices.parts, gz_slices.part_base);
         GZ_LAUNCH_CHECK();
     }
     gz_ktimer_end(GZ_K_MERGE);
-}
-
-extern "C" int gz_graph_source_slices_download(gz_vec_t* vectors, uint16_t* mask)
-{
-    if (gz_slices.vecs == nullptr) return 0;
-    GZ_CHECK(hipStreamSynchronize(gz_stream));
-    if (vectors) gz_copy_to_host(vectors, gz_slices.vecs, gz_slices.begin[GZ_SLICES] * sizeof(gz_vec_t));
-    if (mask) gz_copy_to_host(mask, gz_slices.mask, gz_graph.num_vertices * sizeof(uint16_t));
-    return 1;
 }

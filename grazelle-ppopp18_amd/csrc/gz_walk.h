@@ -30,28 +30,97 @@
 
 #include "gz_internal.h"
 
+/* Cross-lane moves of 64-bit values through DPP (data-parallel primitives:
+ * VALU moves that read another lane of the same instruction, no LDS round
+ * trip).  Lanes whose source does not exist keep `old`. */
+#define GZ_DPP_ROW_SHR(n) (0x110 + (n)) /* lane i <- lane i-n of its 16-lane row */
+#define GZ_DPP_ROW_BCAST15 0x142        /* rows 1..3 <- lane 15 of the row before */
+#define GZ_DPP_ROW_BCAST31 0x143        /* rows 2..3 <- lane 31 */
+#define GZ_DPP_WAVE_SHR1 0x138          /* lane i <- lane i-1 */
+#define GZ_DPP_WAVE_ROL1 0x134          /* lane i <- lane i+1, lane 63 <- lane 0 */
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t gz_dpp_u64(uint64_t old, uint64_t x)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)old, (int)(uint32_t)x, CTRL, ROWS, 0xf,
+                                                              false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp((int)(uint32_t)(old >> 32), (int)(uint32_t)(x >> 32),
+                                                              CTRL, ROWS, 0xf, false);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+template <int CTRL, int ROWS, typename T>
+__device__ __forceinline__ T gz_dpp(T x);
+
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t gz_dpp_impl(uint64_t x)
+{
+    return gz_dpp_u64<CTRL, ROWS>(x, x);
+}
+
+template <int CTRL, int ROWS, typename T>
+__device__ __forceinline__ T gz_dpp(T x)
+{
+    if constexpr (sizeof(T) == 8 && !__is_same(T, uint64_t)) {
+        return __longlong_as_double((long long)gz_dpp_impl<CTRL, ROWS>((uint64_t)__double_as_longlong(x)));
+    } else {
+        return gz_dpp_impl<CTRL, ROWS>(x);
+    }
+}
+
+/* The value of x in lane `src` (src uniform across the wavefront). */
+__device__ __forceinline__ uint64_t gz_readlane_u64(uint64_t x, int src)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, src);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(x >> 32), src);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 template <typename T>
-__device__ __forceinline__ T gz_shfl_up_t(T x, int o)
+__device__ __forceinline__ T gz_readlane_t(T x, int src)
 {
-    return __shfl_up(x, o, GZ_WAVE);
+    if constexpr (!__is_same(T, uint64_t)) {
+        return __longlong_as_double((long long)gz_readlane_u64((uint64_t)__double_as_longlong(x), src));
+    } else {
+        return gz_readlane_u64(x, src);
+    }
 }
 
-template <>
-__device__ __forceinline__ uint64_t gz_shfl_up_t<uint64_t>(uint64_t x, int o)
+/* Inclusive segmented scan across the wavefront: S_i = combine of the values
+ * of lanes [seg_i, i], seg_i being the first lane of lane i's segment.  Within
+ * each 16-lane row a Hillis-Steele scan over row_shr 1, 2, 4, 8 (a lane only
+ * takes a partner in its own row and segment); then rows 1 and 3 take lane 15
+ * and lane 47, and rows 2 and 3 take lane 31, when their segment reaches back
+ * past their row or half. */
+template <typename T, typename Op>
+__device__ __forceinline__ T gz_segscan_dpp(T S, int lane, int seg, const Op& op)
 {
-    return (uint64_t)__shfl_up((unsigned long long)x, o, GZ_WAVE);
-}
-
-template <typename T>
-__device__ __forceinline__ T gz_shfl_t(T x, int src)
-{
-    return __shfl(x, src, GZ_WAVE);
-}
-
-template <>
-__device__ __forceinline__ uint64_t gz_shfl_t<uint64_t>(uint64_t x, int src)
-{
-    return (uint64_t)__shfl((unsigned long long)x, src, GZ_WAVE);
+    const int rl = lane & 15;
+    {
+        const T x = gz_dpp<GZ_DPP_ROW_SHR(1), 0xf>(S);
+        if (rl >= 1 && lane - 1 >= seg) S = op.combine(S, x);
+    }
+    {
+        const T x = gz_dpp<GZ_DPP_ROW_SHR(2), 0xf>(S);
+        if (rl >= 2 && lane - 2 >= seg) S = op.combine(S, x);
+    }
+    {
+        const T x = gz_dpp<GZ_DPP_ROW_SHR(4), 0xf>(S);
+        if (rl >= 4 && lane - 4 >= seg) S = op.combine(S, x);
+    }
+    {
+        const T x = gz_dpp<GZ_DPP_ROW_SHR(8), 0xf>(S);
+        if (rl >= 8 && lane - 8 >= seg) S = op.combine(S, x);
+    }
+    {
+        const T x = gz_dpp<GZ_DPP_ROW_BCAST15, 0xa>(S);
+        if (((lane >> 4) & 1) && seg < (lane & ~15)) S = op.combine(S, x);
+    }
+    {
+        const T x = gz_dpp<GZ_DPP_ROW_BCAST31, 0xc>(S);
+        if (lane >= 32 && seg < 32) S = op.combine(S, x);
+    }
+    return S;
 }
 
 template <int VPL>
@@ -114,21 +183,16 @@ __device__ __forceinline__ void gz_walk_step(const gz_vec_t* __restrict__ vecs, 
     const bool has = nvalid > 0;
     const bool single = has && (h_d == t_d);
     /* a carried run the step does not continue is complete */
-    const uint64_t h0 = gz_shfl_t<uint64_t>(h_d, 0);
+    const uint64_t h0 = gz_readlane_u64(h_d, 0);
     if (lane == 0 && base != b && h0 != carry_d) op.store(carry_d, carry_v);
     /* 2. cross-lane segmented scan of tails keyed by t_d */
-    const uint64_t prev_t = gz_shfl_up_t<uint64_t>(t_d, 1);
+    const uint64_t prev_t = gz_dpp<GZ_DPP_WAVE_SHR1, 0xf>(t_d);
     const uint64_t heads = __ballot(lane == 0 || t_d != prev_t);
     const int seg = gz_segment_start(heads, lane);
-    T S = t_s;
-#pragma unroll
-    for (int o = 1; o < GZ_WAVE; o <<= 1) {
-        const T x = gz_shfl_up_t<T>(S, o);
-        if (lane - o >= seg) S = op.combine(S, x);
-    }
+    T S = gz_segscan_dpp<T>(t_s, lane, seg, op);
     /* lanes whose scan segment starts at lane 0 and continues the carry */
     if (seg == 0 && t_d == carry_d && has) S = op.combine(S, carry_v);
-    const T S_prev = gz_shfl_up_t<T>(S, 1);
+    const T S_prev = gz_dpp<GZ_DPP_WAVE_SHR1, 0xf>(S);
     /* the head of a multi-run lane completes inside the lane */
     if (has && !single) {
         T head = h_s;
@@ -140,11 +204,11 @@ __device__ __forceinline__ void gz_walk_step(const gz_vec_t* __restrict__ vecs, 
         op.store(h_d, head);
     }
     /* 3. tails: complete when the next lane starts another destination */
-    const uint64_t next_h = gz_shfl_t<uint64_t>(h_d, (lane + 1) & (GZ_WAVE - 1));
+    const uint64_t next_h = gz_dpp<GZ_DPP_WAVE_ROL1, 0xf>(h_d);
     const int last = __popcll(__ballot(has)) - 1; /* valid lanes form a prefix */
     if (has && lane != last && next_h != t_d) op.store(t_d, S);
-    carry_d = gz_shfl_t<uint64_t>(t_d, last);
-    carry_v = gz_shfl_t<T>(S, last);
+    carry_d = gz_readlane_u64(t_d, last);
+    carry_v = gz_readlane_t<T>(S, last);
 }
 
 template <int VPL, typename T, typename Op>

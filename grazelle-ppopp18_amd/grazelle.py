@@ -95,9 +95,9 @@ _sig("gz_graph_load_pairs", C.c_int, u64, u64, vp, u64, vp)
 _sig("gz_graph_build_pairs_device", C.c_int, u64, u64, vp, C.c_int, u32)
 _sig("gz_graph_build_rmat_device", C.c_int, P(RmatParams), u64, C.c_int, u64, vp, u32)
 _sig("gz_graph_device_vertex", u64, u64)
-_sig("gz_graph_build_source_slices", C.c_int, u64)
-_sig("gz_graph_source_slices_info", C.c_int, vp, P(u64))
-_sig("gz_slices_chunk_table", u64, vp, u64, u32, vp, u64)
+_sig("gz_graph_build_source_slices", C.c_int, u64, u32)
+_sig("gz_graph_source_slices_info", C.c_int, vp, P(u64), P(u32))
+_sig("gz_slices_chunk_table", u64, vp, u32, u64, u32, vp, u64)
 _sig("gz_slices_set_cold_spread", None, u32)
 _sig("gz_slices_set_cold_stream", None, u32)
 _sig("gz_slices_set_hot_cache", C.c_int, u32)
@@ -288,44 +288,45 @@ def load(g: EdgeLists, pull: bool = True, push: bool = True) -> None:
 
 
 RELABEL_BY_OUTDEGREE = 1
-SLICES = 9
 SLICE_CHUNK = np.dtype([("begin", np.uint64), ("count", np.uint32), ("record", np.uint32)])
-SLICE_RECORD_BITS = 28
+SLICE_RECORD_BITS = 27
 
 
-def _ingress_flags(relabel: bool, slices_log2: int) -> int:
-    return (RELABEL_BY_OUTDEGREE if relabel else 0) | ((slices_log2 & 0x3F) << 8)
+def _ingress_flags(relabel: bool, slices_log2: int, slice_tiers: int = 1) -> int:
+    return (RELABEL_BY_OUTDEGREE if relabel else 0) | ((slices_log2 & 0x3F) << 8) | ((slice_tiers & 0x3) << 14)
 
 
 def load_pairs_device(num_vertices: int, pairs: np.ndarray, symmetrize: bool = False, relabel: bool = False,
-                      slices_log2: int = 0) -> None:
+                      slices_log2: int = 0, slice_tiers: int = 1) -> None:
     """Device ingress from raw (src, dst) pairs: sort, dedupe, pack in HBM.
     slices_log2 = k > 0 also builds the source-sliced pull layout with
-    S = 2^k (and, with relabel, deals the 8*S hottest vertices into the
-    eight slices)."""
+    S = 2^k and slice_tiers tiers (and, with relabel, deals each tier's 8*S
+    ranks of the degree order into its eight windows)."""
     init()
     p = np.ascontiguousarray(pairs, dtype=np.uint64).reshape(-1, 2)
     lib.gz_graph_build_pairs_device(num_vertices, p.shape[0], _ptr(p) if p.shape[0] else None, int(symmetrize),
-                                    _ingress_flags(relabel, slices_log2))
+                                    _ingress_flags(relabel, slices_log2, slice_tiers))
 
 
 def load_rmat_device(scale: int, num_edges: int, seed: int, symmetrize: bool = False, num_vertices: int = 0,
                      extra: np.ndarray | None = None, rmat_vertices: int = 0, a=0.57, b=0.19, c=0.19,
-                     relabel: bool = False, slices_log2: int = 0) -> None:
+                     relabel: bool = False, slices_log2: int = 0, slice_tiers: int = 1) -> None:
     """Device ingress of a seeded R-MAT graph (the stream of rmat_edges) plus
     optional extra host pairs; V = max(num_vertices, R-MAT vertex count).
     relabel renumbers vertices by out-degree on the device (results keep
-    original ids); slices_log2 as for load_pairs_device."""
+    original ids); slices_log2 and slice_tiers as for load_pairs_device."""
     init()
     p = RmatParams(scale, rmat_vertices, num_edges, a, b, c, seed, 0)
     ex = np.ascontiguousarray(extra if extra is not None else np.zeros((0, 2), np.uint64), dtype=np.uint64)
     lib.gz_graph_build_rmat_device(C.byref(p), num_vertices, int(symmetrize), ex.shape[0],
-                                   _ptr(ex) if ex.shape[0] else None, _ingress_flags(relabel, slices_log2))
+                                   _ptr(ex) if ex.shape[0] else None,
+                                   _ingress_flags(relabel, slices_log2, slice_tiers))
 
 
-def load_workload_device(name: str, seed: int = 1, relabel: bool = False, slices_log2: int = 0) -> None:
+def load_workload_device(name: str, seed: int = 1, relabel: bool = False, slices_log2: int = 0,
+                         slice_tiers: int = 1) -> None:
     """Build a named workload with device ingress (same graph as workload_graph)."""
-    kw = dict(relabel=relabel, slices_log2=slices_log2)
+    kw = dict(relabel=relabel, slices_log2=slices_log2, slice_tiers=slice_tiers)
     if name in ("pr-rmat16", "pr-rmat20", "bfs-rmat24"):
         scale = {"pr-rmat16": 16, "pr-rmat20": 20, "bfs-rmat24": 24}[name]
         load_rmat_device(scale, (1 << scale) * 16, seed, **kw)
@@ -342,18 +343,19 @@ def load_workload_device(name: str, seed: int = 1, relabel: bool = False, slices
     raise KeyError(name)
 
 
-def build_source_slices(slice_vertices: int) -> None:
+def build_source_slices(slice_vertices: int, tiers: int = 1) -> None:
     """Source-sliced pull layout of the loaded graph (pull_variant
     "source_sliced" runs PageRank over it)."""
-    lib.gz_graph_build_source_slices(slice_vertices)
+    lib.gz_graph_build_source_slices(slice_vertices, tiers)
 
 
 def source_slices_info():
-    """(vectors per slice [9], S), or None without a sliced layout."""
-    v = np.zeros(SLICES, np.uint64)
-    s = u64(0)
-    if not lib.gz_graph_source_slices_info(_ptr(v), C.byref(s)):
+    """(vectors per slice [8T + 1], S), or None without a sliced layout."""
+    s, t = u64(0), u32(0)
+    if not lib.gz_graph_source_slices_info(None, C.byref(s), C.byref(t)):
         return None
+    v = np.zeros(8 * t.value + 1, np.uint64)
+    lib.gz_graph_source_slices_info(_ptr(v), None, None)
     return v, int(s.value)
 
 
@@ -364,17 +366,21 @@ def source_slices_download():
         return None
     n = int(info[0].sum())
     vecs = np.zeros((max(n, 1), 4), np.uint64)
-    mask = np.zeros(max(num_vertices(), 1), np.uint16)
+    mask = np.zeros(max(num_vertices(), 1), np.uint32)
     lib.gz_graph_source_slices_download(_ptr(vecs), _ptr(mask))
     return vecs[:n], mask[:num_vertices()]
 
 
 def slices_chunk_table(vectors, chunk_len: int, cold_spread: bool = True) -> np.ndarray:
-    """The chunk slots the source-sliced pull launches (host computation)."""
+    """The chunk slots the source-sliced pull launches for slices of the
+    given vector counts (8T + 1 of them; host computation)."""
     v = np.ascontiguousarray(vectors, dtype=np.uint64)
-    n = lib.gz_slices_chunk_table(_ptr(v), chunk_len, int(cold_spread), None, 0)
+    tiers = (len(v) - 1) // 8
+    if len(v) != 8 * tiers + 1:
+        raise ValueError("vector counts must have 8T + 1 entries")
+    n = lib.gz_slices_chunk_table(_ptr(v), tiers, chunk_len, int(cold_spread), None, 0)
     out = np.zeros(max(int(n), 1), SLICE_CHUNK)
-    lib.gz_slices_chunk_table(_ptr(v), chunk_len, int(cold_spread), _ptr(out), n)
+    lib.gz_slices_chunk_table(_ptr(v), tiers, chunk_len, int(cold_spread), _ptr(out), n)
     return out[:n]
 
 

@@ -93,11 +93,13 @@ void graph_data_allocate_merge_buffers(uint64_t num_threads, uint32_t num_numa_n
  * may resolve differently). */
 #define GZ_INGRESS_RELABEL_BY_OUTDEGREE 1u
 /* GZ_INGRESS_SOURCE_SLICES(k) also builds the source-sliced pull layout
- * (gz_graph_build_source_slices) with S = 2^k.  Together with the
- * renumbering, the 8*S highest-degree vertices are dealt round robin into the
- * eight hot slices (rank r -> id (r % 8) * S + r / 8), so that every slice
- * carries about the same share of the edges. */
+ * (gz_graph_build_source_slices) with S = 2^k and GZ_INGRESS_SLICE_TIERS(t)
+ * tiers (1 when absent).  Together with the renumbering, each run of 8*S
+ * ranks of the degree order, for the first t runs, is dealt round robin into
+ * its tier's eight windows (rank i*8S + q -> id i*8S + (q % 8) * S + q / 8),
+ * so that the slices of a tier carry about the same share of the edges. */
 #define GZ_INGRESS_SOURCE_SLICES(k) (((uint32_t)(k) & 0x3fu) << 8)
+#define GZ_INGRESS_SLICE_TIERS(t) (((uint32_t)(t) & 0x3u) << 14)
 int gz_graph_build_pairs_device(uint64_t num_vertices, uint64_t num_edges, const uint64_t* pairs, int symmetrize,
                                 uint32_t flags);
 /* As above with the raw edges drawn on the device by the generator of
@@ -114,32 +116,36 @@ int gz_graph_build_rmat_union_device(const gz_rmat_params_t* params, uint32_t co
                                      uint64_t* vertex_bases);
 
 /* -------- source-sliced pull layout (no reference counterpart) ------------
- * Splits the loaded graph's pull vectors by source: slice x < 8 holds the
- * edges with source in [x*S, (x+1)*S), slice 8 all others; each slice is an
- * ordinary dst-grouped pull list.  PageRank runs over it with
- * pull_variant GZ_PULL_SOURCE_SLICED, which gives XCD x only slice x's
- * chunks (plus a share of slice 8).  S is capped at V / 8.  Needs
- * V <= 2^30.  Replaces any earlier sliced layout. */
-int gz_graph_build_source_slices(uint64_t slice_vertices);
-/* Slice sizes of the current layout: vectors[9], S; returns 0 when none. */
-int gz_graph_source_slices_info(uint64_t* vectors, uint64_t* slice_vertices);
+ * Splits the loaded graph's pull vectors by source into 8T + 1 slices, T =
+ * `tiers` (1..3): slice x < 8T holds the edges with source in
+ * [x*S, (x+1)*S), slice 8T all others.  Each slice is an ordinary
+ * dst-grouped pull list whose destinations are renumbered densely inside the
+ * slice (in vertex order).  PageRank runs over it with pull_variant
+ * GZ_PULL_SOURCE_SLICED, which gives XCD x the chunks of slices x, 8 + x, ...
+ * (plus a share of slice 8T).  S is capped at V / (8T).  Needs V <= 2^29.
+ * Replaces any earlier sliced layout. */
+int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t tiers);
+/* The current layout: vectors per slice (8T + 1 entries), S and T, each
+ * pointer may be NULL; returns 0 when there is none. */
+int gz_graph_source_slices_info(uint64_t* vectors, uint64_t* slice_vertices, uint32_t* tiers);
 /* Copy the concatenated slice vectors and the per-vertex slice mask (bit x:
  * the vertex has vectors in slice x) to the host; either may be NULL.
  * Returns 0 when there is no sliced layout. */
-int gz_graph_source_slices_download(gz_vec_t* vectors, uint16_t* mask);
-/* How the chunks of slices with vectors[9] vectors are laid out at chunk
- * length chunk_len: XCD x (workgroup w with w % 8 == x) takes slice x's
- * chunks, and slice 8's chunks fill the XCDs up to an even share, spread
- * between the hot chunks (cold_spread) or after them.  Four slots per
- * workgroup.  Writes up to `capacity` slots to `out` (NULL: count only) and
- * returns the slot count.  Host-only computation. */
-uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint64_t chunk_len, uint32_t cold_spread,
+int gz_graph_source_slices_download(gz_vec_t* vectors, uint32_t* mask);
+/* How the chunks of slices with vectors[8T + 1] vectors are laid out at chunk
+ * length chunk_len: XCD x (workgroup w with w % 8 == x) takes the chunks of
+ * slices x, 8 + x, ... in that order, and slice 8T's chunks fill the XCDs up
+ * to an even share, spread between the others (cold_spread) or after them.
+ * Four slots per workgroup.  Writes up to `capacity` slots to `out` (NULL:
+ * count only) and returns the slot count (0 for tiers outside 1..3).
+ * Host-only computation. */
+uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint32_t tiers, uint64_t chunk_len, uint32_t cold_spread,
                                gz_slice_chunk_t* out, uint64_t capacity);
-/* Layout of slice 8's chunks for the next source-sliced pull (default 1). */
+/* Layout of the last slice's chunks for the next source-sliced pull (default 1). */
 void gz_slices_set_cold_spread(uint32_t cold_spread);
-/* Nonzero: slice 8's gathers are nontemporal loads (default 0). */
+/* Nonzero: the last slice's gathers are nontemporal loads (default 0). */
 void gz_slices_set_cold_stream(uint32_t cold_stream);
-/* Keep the first `vertices` properties of each hot slice in LDS (0 = off,
+/* Keep the first `vertices` properties of each tier-0 slice in LDS (0 = off,
  * else 4096, 8192, 16384 or 20480); the pull then runs as a persistent kernel.
  * Returns -1 for another value. */
 int gz_slices_set_hot_cache(uint32_t vertices);

@@ -1,12 +1,13 @@
 """Source-sliced PageRank pull versus the host packer and the CPU oracle.
 
-Layout: slice x < 8 must be the host packer's pull list of the edges whose
-source is in [x*S, (x+1)*S), slice 8 of all other edges, concatenated in
-slice order, and the mask must name the slices holding each destination.
+Layout (T tiers): slice x < 8T must be the host packer's pull list of the
+edges whose source is in [x*S, (x+1)*S), slice 8T of all other edges, with each
+destination replaced by its rank among the slice's destinations, concatenated
+in slice order; the mask must name the slices holding each destination.
 PageRank over it must match the oracle within 1e-12 maximum relative error
 per vertex (each destination's sum is regrouped by slice), be bit-identical
-run to run, and the interleaved renumbering must deal the 8*S highest-degree
-vertices round robin into the slices.
+run to run, and the interleaved renumbering must deal each tier's 8*S ranks
+of the degree order round robin into the tier's slices.
 """
 import numpy as np
 import pytest
@@ -25,30 +26,33 @@ def rmat14(gz):
     return gz.rmat_graph(14, 16, seed=11)
 
 
-def _expected_layout(gz, pull, V, S):
-    S = min(S, V // 8)
+def _expected_layout(gz, pull, V, S, T=1):
+    S = min(S, V // (8 * T))
     src = pull[:, 0].astype(np.int64)
-    sl = np.where(src < 8 * S, src // max(S, 1), 8) if S else np.full(len(src), 8)
-    vecs, mask = [], np.zeros(V, np.uint16)
-    for x in range(9):
+    sl = np.where(src < 8 * T * S, src // max(S, 1), 8 * T) if S else np.full(len(src), 8 * T)
+    vecs, mask = [], np.zeros(V, np.uint32)
+    for x in range(8 * T + 1):
         part = pull[sl == x]
         if len(part):
-            vecs.append(gz.pack_vectors(part, True))
-            mask[np.unique(part[:, 1].astype(np.int64))] |= np.uint16(1 << x)
+            dsts, local = np.unique(part[:, 1], return_inverse=True)
+            ranked = np.stack([part[:, 0], local.astype(np.uint64)], axis=1)
+            vecs.append(gz.pack_vectors(ranked, True))
+            mask[dsts.astype(np.int64)] |= np.uint32(1 << x)
         else:
             vecs.append(np.zeros((0, 4), np.uint64))
     return vecs, mask
 
 
-@pytest.mark.parametrize("S", [1, 64, 700, 1 << 20])
-def test_slices_layout_matches_host_packer(gz, device, rmat14, S):
+@pytest.mark.parametrize("S,T", [(1, 1), (64, 1), (700, 1), (1 << 20, 1), (64, 2), (300, 3), (1 << 20, 3)])
+def test_slices_layout_matches_host_packer(gz, device, rmat14, S, T):
     g = rmat14
     gz.load(g)
-    gz.build_source_slices(S)
+    gz.build_source_slices(S, T)
     counts, s_eff = gz.source_slices_info()
-    assert s_eff == min(S, g.num_vertices // 8)
+    assert len(counts) == 8 * T + 1
+    assert s_eff == min(S, g.num_vertices // (8 * T))
     vecs, mask = gz.source_slices_download()
-    want, want_mask = _expected_layout(gz, g.pull, g.num_vertices, S)
+    want, want_mask = _expected_layout(gz, g.pull, g.num_vertices, S, T)
     assert [int(c) for c in counts] == [len(w) for w in want]
     assert np.array_equal(vecs, np.concatenate(want).reshape(-1, 4))
     assert np.array_equal(mask, want_mask)
@@ -57,15 +61,15 @@ def test_slices_layout_matches_host_packer(gz, device, rmat14, S):
 @pytest.mark.parametrize("spread", [True, False])
 @pytest.mark.parametrize("vpl", [1, 2, 4])
 @pytest.mark.parametrize("chunk", [64, 512])
-@pytest.mark.parametrize("S", [64, 1024])
-def test_sliced_pr_parity(gz, orc, device, rmat14, S, chunk, vpl, spread):
+@pytest.mark.parametrize("S,T", [(64, 1), (1024, 1), (128, 2), (256, 3)])
+def test_sliced_pr_parity(gz, orc, device, rmat14, S, T, chunk, vpl, spread):
     g = rmat14
     gz.set_chunk_vectors(chunk)
     gz.set_vectors_per_lane(vpl)
     gz.slices_set_cold_spread(spread)
     try:
         gz.load(g)
-        gz.build_source_slices(S)
+        gz.build_source_slices(S, T)
         gz.run(gz.options("pr", iterations=20, pull_variant="source_sliced"))
         ranks = gz.pr_ranks()
     finally:
@@ -106,21 +110,23 @@ def test_sliced_pr_sym_trees_and_empty_slices(gz, orc, device):
     assert _rel(gz.pr_ranks(), ref) <= PR_TOL
 
 
-@pytest.mark.parametrize("k", [4, 8])
-def test_interleaved_renumbering_and_pr(gz, orc, device, k):
+@pytest.mark.parametrize("k,T", [(4, 1), (8, 1), (5, 2), (6, 3)])
+def test_interleaved_renumbering_and_pr(gz, orc, device, k, T):
     scale, ef, seed = 13, 16, 23
     nv = 1 << scale
     raw = gz.rmat_edges(scale, nv * ef, seed)
     g = gz.make_graph(raw, nv)
-    gz.load_rmat_device(scale, nv * ef, seed, relabel=True, slices_log2=k)
-    S = min(1 << k, nv // 8)
+    gz.load_rmat_device(scale, nv * ef, seed, relabel=True, slices_log2=k, slice_tiers=T)
+    S = min(1 << k, nv // (8 * T))
     assert gz.source_slices_info()[1] == S
-    # rank r of the raw out-degree order (descending, ties by id) gets id
-    # (r % 8) * S + r // 8 for r < 8S, else r
+    assert len(gz.source_slices_info()[0]) == 8 * T + 1
+    # rank t*8S + q of the raw out-degree order (descending, ties by id) gets
+    # id t*8S + (q % 8) * S + q // 8 for t < T, else keeps its rank
     cnt = np.bincount(raw[:, 0].astype(np.int64), minlength=nv)
     order = np.lexsort((np.arange(nv), -cnt))
     r = np.arange(nv)
-    want = np.where(r < 8 * S, (r % 8) * S + r // 8, r)
+    t, q = r // (8 * S), r % (8 * S)
+    want = np.where(r < 8 * S * T, t * 8 * S + (q % 8) * S + q // 8, r)
     got = np.array([gz.lib.gz_graph_device_vertex(int(v)) for v in order])
     assert np.array_equal(got, want)
     gz.run(gz.options("pr", iterations=15, pull_variant="source_sliced"))
@@ -137,11 +143,11 @@ def rmat17(gz):
 
 
 @pytest.mark.parametrize("hot", [4096, 8192, 16384, 20480])
-@pytest.mark.parametrize("S", [4096, 16384])
-def test_sliced_pr_hot_cache(gz, orc, device, rmat17, S, hot):
+@pytest.mark.parametrize("S,T", [(4096, 1), (16384, 1), (4096, 3)])
+def test_sliced_pr_hot_cache(gz, orc, device, rmat17, S, T, hot):
     g = rmat17
     gz.load(g)
-    gz.build_source_slices(S)
+    gz.build_source_slices(S, T)
     gz.slices_set_hot_cache(hot)
     try:
         gz.run(gz.options("pr", iterations=10, pull_variant="source_sliced"))

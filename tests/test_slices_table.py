@@ -2,9 +2,9 @@
 
 Properties the kernel relies on: every chunk of every slice is in exactly one
 slot, with its merge record numbered in slice order; a workgroup w only holds
-chunks of slice w % 8 or of the cold slice 8; the XCDs get an even number of
-chunks whenever the cold slice can fill them; four slots per workgroup and
-empty slots marked by count 0.
+chunks of slices x = w % 8, 8 + x, ... (tier by tier, in order) or of the cold
+slice 8T; the XCDs get an even number of chunks whenever the cold slice can
+fill them; four slots per workgroup and empty slots marked by count 0.
 """
 import numpy as np
 import pytest
@@ -14,8 +14,20 @@ import grazelle as gz
 XCDS = 8
 
 
+def _xcd_sequences(t):
+    out = []
+    for x in range(XCDS):
+        seq = []
+        for wg in range(x, len(t) // 4, XCDS):
+            seq.extend(t[4 * wg:4 * wg + 4])
+        out.append(seq)
+    return out
+
+
 def _check(vectors, L, spread):
     t = gz.slices_chunk_table(vectors, L, spread)
+    T = (len(vectors) - 1) // 8
+    cold_slice = 8 * T
     vectors = np.asarray(vectors, dtype=np.int64)
     chunks = (vectors + L - 1) // L
     base = np.concatenate([[0], np.cumsum(vectors)])
@@ -26,52 +38,64 @@ def _check(vectors, L, spread):
     sl = (used["record"] >> gz.SLICE_RECORD_BITS).astype(np.int64)
     r = (used["record"] & ((1 << gz.SLICE_RECORD_BITS) - 1)).astype(np.int64)
     assert np.array_equal(np.sort(r), np.arange(chunks.sum()))
-    for x in range(gz.SLICES):
+    for x in range(len(vectors)):
         m = sl == x
         k = r[m] - rec[x]
         assert np.all((k >= 0) & (k < chunks[x]))
         assert np.array_equal(used["begin"][m].astype(np.int64), base[x] + k * L)
         want = np.minimum(L, vectors[x] - k * L)
         assert np.array_equal(used["count"][m].astype(np.int64), want)
-    # workgroup -> XCD round robin
+    # workgroup -> XCD round robin; an XCD holds its own slices in every tier
     slot = np.flatnonzero(t["count"] > 0)
     xcd = (slot // 4) % XCDS
-    assert np.all((sl == xcd) | (sl == XCDS))
+    assert np.all(((sl % XCDS == xcd) & (sl < cold_slice)) | (sl == cold_slice))
     # per-XCD chunk counts
     per = np.bincount(xcd, minlength=XCDS)
-    target = max(int(-(-chunks.sum() // XCDS)), int(chunks[:XCDS].max(initial=0)))
+    own = np.array([sum(int(chunks[8 * k + x]) for k in range(T)) for x in range(XCDS)])
+    target = max(int(-(-chunks.sum() // XCDS)), int(own.max(initial=0)))
     assert per.max() <= target
-    if chunks[:XCDS].max(initial=0) <= -(-chunks.sum() // XCDS):
+    if own.max(initial=0) <= -(-chunks.sum() // XCDS):
         assert per.max() - per.min() <= 1
-    # a XCD's chunks occupy a prefix of its own slot sequence
-    for x in range(XCDS):
-        seq = []
-        for wg in range(x, len(t) // 4, XCDS):
-            seq.extend(t["count"][4 * wg:4 * wg + 4] > 0)
-        seq = np.array(seq, bool)
-        n = int(seq.sum())
-        assert seq[:n].all() and not seq[n:].any()
+    for x, seq in enumerate(_xcd_sequences(t)):
+        used = np.array([c["count"] > 0 for c in seq], bool)
+        n = int(used.sum())
+        # a XCD's chunks occupy a prefix of its own slot sequence
+        assert used[:n].all() and not used[n:].any()
+        slices = np.array([int(c["record"] >> gz.SLICE_RECORD_BITS) for c in seq[:n]], np.int64)
+        hot = slices[slices != cold_slice]
+        # tier by tier, each slice's chunks in order
+        assert np.all(np.diff(hot) >= 0)
         if not spread:
-            own = []
-            for wg in range(x, len(t) // 4, XCDS):
-                for c in t[4 * wg:4 * wg + 4]:
-                    if c["count"]:
-                        own.append(int(c["record"] >> gz.SLICE_RECORD_BITS))
-            cold = np.array(own) == XCDS
+            cold = slices == cold_slice
             if cold.any():
-                first = int(np.argmax(cold))
-                assert cold[first:].all()
+                assert cold[int(np.argmax(cold)):].all()
     return t
 
 
+@pytest.mark.parametrize("tiers", [1, 2, 3])
 @pytest.mark.parametrize("spread", [True, False])
 @pytest.mark.parametrize("L", [64, 512])
-def test_table_random(spread, L):
-    rng = np.random.default_rng(5)
-    for _ in range(60):
-        v = rng.integers(0, 40 * L, size=gz.SLICES)
-        v[rng.random(gz.SLICES) < 0.2] = 0
+def test_table_random(spread, L, tiers):
+    rng = np.random.default_rng(5 + tiers)
+    n = 8 * tiers + 1
+    for _ in range(40):
+        v = rng.integers(0, 40 * L, size=n)
+        v[rng.random(n) < 0.2] = 0
         _check(v, L, spread)
+
+
+def test_table_rejects_bad_sizes():
+    with pytest.raises(ValueError):
+        gz.slices_chunk_table([1] * 10, 64)
+    assert len(gz.slices_chunk_table([1] * 33, 64)) == 0  # four tiers are not supported
+
+
+@pytest.mark.parametrize("spread", [True, False])
+def test_table_edge_cases_tiers(spread):
+    _check([0] * 17, 64, spread)
+    _check([64 * 3] * 8 + [0] * 8 + [100], 64, spread)
+    _check([0] * 8 + [64 * 3] * 8 + [100], 64, spread)
+    _check([64] * 24 + [64 * 40], 64, spread)
 
 
 @pytest.mark.parametrize("spread", [True, False])
