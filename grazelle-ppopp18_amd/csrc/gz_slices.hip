@@ -306,12 +306,16 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_merge_sliced(const mergeaccum_t
  * permitting) each copy the first `hot` properties of their XCD's tier-0
  * window into LDS; their wavefronts then take that XCD's chunks from a
  * per-XCD counter.  Gathers that fall in the cached range read LDS instead of
- * issuing an L2 request.  A cache of 128 KiB or more leaves room for one
- * workgroup per CU, which then has 1024 threads so that 16 wavefronts share
- * it; smaller caches use 512-thread workgroups, several per CU. */
+ * issuing an L2 request.  Workgroups have 1024 threads, so that 16
+ * wavefronts share one cache; a cache of at most 64 KiB leaves room for two
+ * workgroups per CU, and the kernel is then held to the registers that allow
+ * both. */
 template <int HOT>
 struct gz_hot_wg {
-    static constexpr int value = HOT >= 16384 ? 1024 : 512;
+    static constexpr int value = 1024;
+    /* wavefronts per SIMD the registers must allow: two workgroups' worth
+     * when two caches fit in the CU's LDS */
+    static constexpr int waves_per_simd = HOT <= 8192 ? 8 : 4;
 };
 
 struct gz_xcd_lens {
@@ -368,7 +372,7 @@ struct gz_pr_pull_op_hot {
 };
 
 template <int HOT, int VPL>
-__global__ __launch_bounds__(gz_hot_wg<HOT>::value) void gz_k_edge_pull_pr_sliced_hot(
+__global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_simd) void gz_k_edge_pull_pr_sliced_hot(
     const gz_vec_t* __restrict__ vecs, const gz_slice_chunk_t* __restrict__ table, gz_xcd_lens lens,
     const double* __restrict__ props, double* __restrict__ parts, gz_slice_offsets part_base,
     mergeaccum_t* __restrict__ merge, double* __restrict__ partials, uint64_t* __restrict__ stats, uint32_t collect,
@@ -400,12 +404,12 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value) void gz_k_edge_pull_pr_slice
         gz_lds_f64* lds = (gz_lds_f64*)hot;
         if (cold_stream && sl == cold_slice) {
             gz_pr_pull_op_hot<true> op{gprops, lds, 0, 0, parts + part_base.off[sl], 0.0};
-            gz_walk_chunk<VPL, double>(vecs, c.begin, c.begin + c.count, op, first_d, carry_d, carry_v, edges);
+            gz_walk_chunk<VPL, double, true>(vecs, c.begin, c.begin + c.count, op, first_d, carry_d, carry_v, edges);
             sum = op.sum;
         } else {
             gz_pr_pull_op_hot<false> op{gprops, lds, (sl == x) ? lo : 0, (sl == x) ? hot_count : 0,
                                         parts + part_base.off[sl], 0.0};
-            gz_walk_chunk<VPL, double>(vecs, c.begin, c.begin + c.count, op, first_d, carry_d, carry_v, edges);
+            gz_walk_chunk<VPL, double, true>(vecs, c.begin, c.begin + c.count, op, first_d, carry_d, carry_v, edges);
             sum = op.sum;
         }
         const double S = gz_wave_sum(sum);

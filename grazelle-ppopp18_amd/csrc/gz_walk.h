@@ -86,6 +86,34 @@ __device__ __forceinline__ T gz_readlane_t(T x, int src)
     }
 }
 
+/* The same moves through LDS permutes (ds_bpermute): fewer VALU
+ * instructions, a longer dependent latency.  The plain pull sweeps, which run
+ * many wavefronts per SIMD, are a little faster this way; the LDS-cached
+ * sliced sweep, whose LDS port also serves its gathers, uses DPP. */
+template <typename T>
+__device__ __forceinline__ T gz_shfl_up_t(T x, int o)
+{
+    return __shfl_up(x, o, GZ_WAVE);
+}
+
+template <>
+__device__ __forceinline__ uint64_t gz_shfl_up_t<uint64_t>(uint64_t x, int o)
+{
+    return (uint64_t)__shfl_up((unsigned long long)x, o, GZ_WAVE);
+}
+
+template <typename T>
+__device__ __forceinline__ T gz_shfl_t(T x, int src)
+{
+    return __shfl(x, src, GZ_WAVE);
+}
+
+template <>
+__device__ __forceinline__ uint64_t gz_shfl_t<uint64_t>(uint64_t x, int src)
+{
+    return (uint64_t)__shfl((unsigned long long)x, src, GZ_WAVE);
+}
+
 /* Inclusive segmented scan across the wavefront: S_i = combine of the values
  * of lanes [seg_i, i], seg_i being the first lane of lane i's segment.  Within
  * each 16-lane row a Hillis-Steele scan over row_shr 1, 2, 4, 8 (a lane only
@@ -144,7 +172,7 @@ __device__ __forceinline__ void gz_load_slice(gz_slice<VPL>& s, const gz_vec_t* 
 
 /* One step of the walk: reduce the slice `cur` (vectors base + lane*VPL ...)
  * and prefetch the following step's slice into `nxt`. */
-template <int VPL, typename T, typename Op>
+template <int VPL, typename T, bool kDpp, typename Op>
 __device__ __forceinline__ void gz_walk_step(const gz_vec_t* __restrict__ vecs, uint64_t b, uint64_t e, uint64_t base,
                                              Op& op, const gz_slice<VPL>& cur, gz_slice<VPL>& nxt, uint64_t& carry_d,
                                              T& carry_v, uint32_t& edges)
@@ -186,13 +214,30 @@ __device__ __forceinline__ void gz_walk_step(const gz_vec_t* __restrict__ vecs, 
     const uint64_t h0 = gz_readlane_u64(h_d, 0);
     if (lane == 0 && base != b && h0 != carry_d) op.store(carry_d, carry_v);
     /* 2. cross-lane segmented scan of tails keyed by t_d */
-    const uint64_t prev_t = gz_dpp<GZ_DPP_WAVE_SHR1, 0xf>(t_d);
+    uint64_t prev_t;
+    if constexpr (kDpp)
+        prev_t = gz_dpp<GZ_DPP_WAVE_SHR1, 0xf>(t_d);
+    else
+        prev_t = gz_shfl_up_t<uint64_t>(t_d, 1);
     const uint64_t heads = __ballot(lane == 0 || t_d != prev_t);
     const int seg = gz_segment_start(heads, lane);
-    T S = gz_segscan_dpp<T>(t_s, lane, seg, op);
+    T S = t_s;
+    if constexpr (kDpp) {
+        S = gz_segscan_dpp<T>(t_s, lane, seg, op);
+    } else {
+#pragma unroll
+        for (int o = 1; o < GZ_WAVE; o <<= 1) {
+            const T x = gz_shfl_up_t<T>(S, o);
+            if (lane - o >= seg) S = op.combine(S, x);
+        }
+    }
     /* lanes whose scan segment starts at lane 0 and continues the carry */
     if (seg == 0 && t_d == carry_d && has) S = op.combine(S, carry_v);
-    const T S_prev = gz_dpp<GZ_DPP_WAVE_SHR1, 0xf>(S);
+    T S_prev;
+    if constexpr (kDpp)
+        S_prev = gz_dpp<GZ_DPP_WAVE_SHR1, 0xf>(S);
+    else
+        S_prev = gz_shfl_up_t<T>(S, 1);
     /* the head of a multi-run lane completes inside the lane */
     if (has && !single) {
         T head = h_s;
@@ -204,14 +249,18 @@ __device__ __forceinline__ void gz_walk_step(const gz_vec_t* __restrict__ vecs, 
         op.store(h_d, head);
     }
     /* 3. tails: complete when the next lane starts another destination */
-    const uint64_t next_h = gz_dpp<GZ_DPP_WAVE_ROL1, 0xf>(h_d);
+    uint64_t next_h;
+    if constexpr (kDpp)
+        next_h = gz_dpp<GZ_DPP_WAVE_ROL1, 0xf>(h_d);
+    else
+        next_h = gz_shfl_t<uint64_t>(h_d, (lane + 1) & (GZ_WAVE - 1));
     const int last = __popcll(__ballot(has)) - 1; /* valid lanes form a prefix */
     if (has && lane != last && next_h != t_d) op.store(t_d, S);
     carry_d = gz_readlane_u64(t_d, last);
     carry_v = gz_readlane_t<T>(S, last);
 }
 
-template <int VPL, typename T, typename Op>
+template <int VPL, typename T, bool kDpp = false, typename Op>
 __device__ __forceinline__ void gz_walk_chunk(const gz_vec_t* __restrict__ vecs, uint64_t b, uint64_t e, Op& op,
                                               uint64_t& first_d, uint64_t& carry_d, T& carry_v, uint32_t& edges)
 {
@@ -226,10 +275,10 @@ __device__ __forceinline__ void gz_walk_chunk(const gz_vec_t* __restrict__ vecs,
     gz_load_slice<VPL>(sa, vecs, b + (uint64_t)lane * VPL, e);
     uint64_t base = b;
     while (true) {
-        gz_walk_step<VPL, T>(vecs, b, e, base, op, sa, sb, carry_d, carry_v, edges);
+        gz_walk_step<VPL, T, kDpp>(vecs, b, e, base, op, sa, sb, carry_d, carry_v, edges);
         base += step;
         if (base >= e) break;
-        gz_walk_step<VPL, T>(vecs, b, e, base, op, sb, sa, carry_d, carry_v, edges);
+        gz_walk_step<VPL, T, kDpp>(vecs, b, e, base, op, sb, sa, carry_d, carry_v, edges);
         base += step;
         if (base >= e) break;
     }
