@@ -598,6 +598,28 @@ __global__ void gz_k_local_dst(gz_vec_t* __restrict__ vecs, uint64_t n, const ui
     }
 }
 
+/* Vector-Sparse vectors -> packed vectors (32-bit neighbour ids, empty
+ * lanes GZ_PACKED_NONE, and the 32-bit shared vertex). */
+__global__ void gz_k_pack_u32(const gz_vec_t* __restrict__ vecs, uint64_t n, gz_u32x4* __restrict__ nbr,
+                              uint32_t* __restrict__ shared)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        gz_lanes w;
+        w.l0 = vecs[j].lane[0];
+        w.l1 = vecs[j].lane[1];
+        w.l2 = vecs[j].lane[2];
+        w.l3 = vecs[j].lane[3];
+        gz_u32x4 a;
+        a.x = gz_lane_valid(w.l0) ? (uint32_t)gz_lane_id(w.l0) : GZ_PACKED_NONE;
+        a.y = gz_lane_valid(w.l1) ? (uint32_t)gz_lane_id(w.l1) : GZ_PACKED_NONE;
+        a.z = gz_lane_valid(w.l2) ? (uint32_t)gz_lane_id(w.l2) : GZ_PACKED_NONE;
+        a.w = gz_lane_valid(w.l3) ? (uint32_t)gz_lane_id(w.l3) : GZ_PACKED_NONE;
+        nbr[j] = a;
+        shared[j] = (uint32_t)gz_shared_vertex(w);
+    }
+}
+
 }  // namespace
 
 extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t tiers)
@@ -689,9 +711,14 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
     gz_slices.begin[0] = 0;
     for (uint32_t x = 0; x < ns; ++x) gz_slices.begin[x + 1] = gz_slices.begin[x] + counts[x];
     const uint64_t total = gz_slices.begin[ns];
-    gz_slices.vecs = (gz_vec_t*)gz_device_malloc((total ? total : 1) * sizeof(gz_vec_t));
+    gz_slices.nbr = (gz_u32x4*)gz_device_malloc((total ? total : 1) * sizeof(gz_u32x4));
+    gz_slices.shared = (uint32_t*)gz_device_malloc((total ? total : 1) * sizeof(uint32_t));
     for (uint32_t x = 0; x < ns; ++x) {
-        if (counts[x]) gz_device_copy(gz_slices.vecs + gz_slices.begin[x], part_vecs[x], counts[x] * sizeof(gz_vec_t));
+        if (counts[x]) {
+            hipLaunchKernelGGL(gz_k_pack_u32, dim3(gz_grid_for(counts[x])), dim3(GZ_BLOCK), 0, gz_stream, part_vecs[x],
+                               counts[x], gz_slices.nbr + gz_slices.begin[x], gz_slices.shared + gz_slices.begin[x]);
+            GZ_LAUNCH_CHECK();
+        }
         GZ_CHECK(hipStreamSynchronize(gz_stream));
         gz_device_free(part_vecs[x]);
     }

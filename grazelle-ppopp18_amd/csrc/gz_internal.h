@@ -13,6 +13,16 @@
 
 #include "grazelle_phases.h"
 
+typedef unsigned int gz_u32x4 __attribute__((ext_vector_type(4)));
+
+/* A packed vector: the four 32-bit neighbour ids of a Vector-Sparse vector
+ * (GZ_PACKED_NONE for an empty lane; valid lanes form a prefix) and its
+ * 32-bit shared vertex, stored as a 16-byte and a 4-byte array. */
+#define GZ_PACKED_NONE 0xffffffffu
+struct gz_packed_vec {
+    uint32_t s0, s1, s2, s3, d;
+};
+
 #define GZ_WAVE 64
 #define GZ_BLOCK 256
 #define GZ_WAVES_PER_BLOCK (GZ_BLOCK / GZ_WAVE)
@@ -84,8 +94,9 @@ typedef struct gz_source_slices_t {
     uint64_t slice_vertices;          /* S */
     uint32_t tiers;                   /* T */
     uint32_t nslices;                 /* 8T + 1 */
-    gz_vec_t* vecs;                   /* all slices, concatenated */
-    uint64_t begin[GZ_MAX_SLICES + 1]; /* first vector of each slice in vecs */
+    gz_u32x4* nbr;                    /* all slices, concatenated, as packed vectors: neighbour ids */
+    uint32_t* shared;                 /* ... and shared vertices (local ranks) */
+    uint64_t begin[GZ_MAX_SLICES + 1]; /* first vector of each slice */
     uint32_t* mask;                   /* [V] bit x: the vertex has vectors in slice x */
     /* Inside slice x, destinations are numbered densely (local rank: the
      * count of lower vertices that have vectors in slice x), and the shared

@@ -38,6 +38,25 @@ __device__ __forceinline__ double gz_pr_vector_sum(const gz_lanes& v, const doub
     return (g2 + g3) + (g0 + g1);
 }
 
+/* The same sum for a packed vector (empty lanes hold GZ_PACKED_NONE). */
+template <bool kStream = false>
+__device__ __forceinline__ double gz_pr_packed_sum(const gz_packed_vec& v, const double* __restrict__ props,
+                                                   uint32_t& edges, bool in_range = true)
+{
+    const bool v0 = in_range && v.s0 != GZ_PACKED_NONE, v1 = in_range && v.s1 != GZ_PACKED_NONE,
+               v2 = in_range && v.s2 != GZ_PACKED_NONE, v3 = in_range && v.s3 != GZ_PACKED_NONE;
+    double g0 = gz_gather_prop<kStream>(props, v0 ? v.s0 : 0);
+    double g1 = gz_gather_prop<kStream>(props, v1 ? v.s1 : 0);
+    double g2 = gz_gather_prop<kStream>(props, v2 ? v.s2 : 0);
+    double g3 = gz_gather_prop<kStream>(props, v3 ? v.s3 : 0);
+    g0 = v0 ? g0 : 0.0;
+    g1 = v1 ? g1 : 0.0;
+    g2 = v2 ? g2 : 0.0;
+    g3 = v3 ? g3 : 0.0;
+    edges += (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2 + (uint32_t)v3;
+    return (g2 + g3) + (g0 + g1);
+}
+
 template <bool kStream>
 struct gz_pr_pull_op_t {
     const double* __restrict__ props;
@@ -49,6 +68,12 @@ struct gz_pr_pull_op_t {
     __device__ __forceinline__ double gather(const gz_lanes& v, bool in_range, uint32_t& edges)
     {
         const double g = gz_pr_vector_sum<kStream>(v, props, edges, in_range);
+        sum += g;
+        return g;
+    }
+    __device__ __forceinline__ double gather(const gz_packed_vec& v, bool in_range, uint32_t& edges)
+    {
+        const double g = gz_pr_packed_sum<kStream>(v, props, edges, in_range);
         sum += g;
         return g;
     }

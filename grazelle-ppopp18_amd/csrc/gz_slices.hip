@@ -12,8 +12,9 @@
  * the cold slice, so each XCD gathers from one S-vertex window at a time.
  *
  * Each slice is an ordinary pull list whose destinations are numbered
- * densely inside the slice, walked exactly as in gz_pr.hip into its own
- * accumulator run (complete runs stored directly, the chunk's last run into
+ * densely inside the slice, stored packed (32-bit neighbour ids and shared
+ * vertex: 20 bytes per vector instead of 32), walked exactly as in gz_pr.hip
+ * into its own accumulator run (complete runs stored directly, the chunk's last run into
  * its merge record).  The merge then runs per slice
  * (edge_pull_op_merge_with_merge_buffer's rule, phases.c:41-70), and the
  * vertex phase takes acc[v] = the sum over the slices holding v, in slice
@@ -144,7 +145,8 @@ extern "C" int gz_slices_set_hot_cache(uint32_t vertices)
 
 extern "C" void gz_source_slices_release(void)
 {
-    gz_device_free(gz_slices.vecs);
+    gz_device_free(gz_slices.nbr);
+    gz_device_free(gz_slices.shared);
     gz_device_free(gz_slices.mask);
     gz_device_free(gz_slices.parts);
     gz_device_free(gz_slices.rank_base);
@@ -156,7 +158,7 @@ extern "C" void gz_source_slices_release(void)
 
 extern "C" int gz_graph_source_slices_info(uint64_t* vectors, uint64_t* slice_vertices, uint32_t* tiers)
 {
-    if (gz_slices.vecs == nullptr) return 0;
+    if (gz_slices.nbr == nullptr) return 0;
     if (vectors)
         for (uint32_t x = 0; x < gz_slices.nslices; ++x) vectors[x] = gz_slices.begin[x + 1] - gz_slices.begin[x];
     if (slice_vertices) *slice_vertices = gz_slices.slice_vertices;
@@ -166,9 +168,28 @@ extern "C" int gz_graph_source_slices_info(uint64_t* vectors, uint64_t* slice_ve
 
 extern "C" int gz_graph_source_slices_download(gz_vec_t* vectors, uint32_t* mask)
 {
-    if (gz_slices.vecs == nullptr) return 0;
+    if (gz_slices.nbr == nullptr) return 0;
     GZ_CHECK(hipStreamSynchronize(gz_stream));
-    if (vectors) gz_copy_to_host(vectors, gz_slices.vecs, gz_slices.begin[gz_slices.nslices] * sizeof(gz_vec_t));
+    if (vectors) {
+        /* back to Vector-Sparse form: the packed copy keeps the neighbour ids
+         * and the shared vertex, which is all a vector holds */
+        const uint64_t n = gz_slices.begin[gz_slices.nslices];
+        uint32_t* nbr = (uint32_t*)malloc((n ? n : 1) * 4 * sizeof(uint32_t));
+        uint32_t* shared = (uint32_t*)malloc((n ? n : 1) * sizeof(uint32_t));
+        gz_copy_to_host(nbr, gz_slices.nbr, n * 4 * sizeof(uint32_t));
+        gz_copy_to_host(shared, gz_slices.shared, n * sizeof(uint32_t));
+        for (uint64_t j = 0; j < n; ++j) {
+            uint64_t ids[4];
+            unsigned count = 0;
+            while (count < 4 && nbr[4 * j + count] != GZ_PACKED_NONE) {
+                ids[count] = nbr[4 * j + count];
+                ++count;
+            }
+            gz_vec_encode(&vectors[j], shared[j], ids, count);
+        }
+        free(nbr);
+        free(shared);
+    }
     if (mask) gz_copy_to_host(mask, gz_slices.mask, gz_graph.num_vertices * sizeof(uint32_t));
     return 1;
 }
@@ -215,14 +236,14 @@ struct gz_slice_bounds {
 
 /* Walk one chunk into slice accumulators `acc`; returns the wave's share of S. */
 template <int VPL, bool kStream>
-__device__ __forceinline__ double gz_sliced_walk(const gz_vec_t* __restrict__ vecs, uint64_t b, uint64_t e,
+__device__ __forceinline__ double gz_sliced_walk(const gz_packed_list& list, uint64_t b, uint64_t e,
                                                  const double* __restrict__ props, double* __restrict__ acc,
                                                  mergeaccum_t* __restrict__ record, uint32_t& edges)
 {
     gz_pr_pull_op_t<kStream> op{props, acc, true, 0.0};
     uint64_t first_d, carry_d;
     double carry_v;
-    gz_walk_chunk<VPL, double>(vecs, b, e, op, first_d, carry_d, carry_v, edges);
+    gz_walk_list<VPL, double, false>(list, b, e, op, first_d, carry_d, carry_v, edges);
     if (gz_lane() == 0) {
         mergeaccum_t r;
         r.initial_vertex_id = first_d;
@@ -237,7 +258,7 @@ __device__ __forceinline__ double gz_sliced_walk(const gz_vec_t* __restrict__ ve
 /* One wavefront per chunk slot.  kColdStream: the cold slice's gathers, which
  * mostly miss, are nontemporal. */
 template <int VPL, bool kColdStream>
-__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr_sliced(const gz_vec_t* __restrict__ vecs,
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr_sliced(gz_packed_list list,
                                                                      const gz_slice_chunk_t* __restrict__ table,
                                                                      const double* __restrict__ props,
                                                                      double* __restrict__ parts, gz_slice_offsets part_base,
@@ -265,8 +286,8 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr_sliced(const gz_ve
     double* acc = parts + part_base.off[x];
     uint32_t edges = 0;
     const double S = (kColdStream && x == cold_slice)
-                         ? gz_sliced_walk<VPL, true>(vecs, b, e, props, acc, merge + rec, edges)
-                         : gz_sliced_walk<VPL, false>(vecs, b, e, props, acc, merge + rec, edges);
+                         ? gz_sliced_walk<VPL, true>(list, b, e, props, acc, merge + rec, edges)
+                         : gz_sliced_walk<VPL, false>(list, b, e, props, acc, merge + rec, edges);
     if (lane == 0) partials[slot] = S;
     if (collect) {
         const uint64_t tot_edges = gz_wave_sum_u64(edges);
@@ -349,16 +370,16 @@ struct gz_pr_pull_op_hot {
             g = props[i];
         return g;
     }
-    __device__ __forceinline__ double gather(const gz_lanes& v, bool in_range, uint32_t& edges)
+    __device__ __forceinline__ double gather(const gz_packed_vec& v, bool in_range, uint32_t& edges)
     {
-        /* gz_pr_vector_sum with the cache in front; invalid lanes read the
+        /* gz_pr_packed_sum with the cache in front; empty lanes read the
          * window's first element */
-        const bool v0 = in_range && gz_lane_valid(v.l0), v1 = in_range && gz_lane_valid(v.l1),
-                   v2 = in_range && gz_lane_valid(v.l2), v3 = in_range && gz_lane_valid(v.l3);
-        double g0 = fetch(v0 ? gz_lane_id(v.l0) : wlo);
-        double g1 = fetch(v1 ? gz_lane_id(v.l1) : wlo);
-        double g2 = fetch(v2 ? gz_lane_id(v.l2) : wlo);
-        double g3 = fetch(v3 ? gz_lane_id(v.l3) : wlo);
+        const bool v0 = in_range && v.s0 != GZ_PACKED_NONE, v1 = in_range && v.s1 != GZ_PACKED_NONE,
+                   v2 = in_range && v.s2 != GZ_PACKED_NONE, v3 = in_range && v.s3 != GZ_PACKED_NONE;
+        double g0 = fetch(v0 ? v.s0 : wlo);
+        double g1 = fetch(v1 ? v.s1 : wlo);
+        double g2 = fetch(v2 ? v.s2 : wlo);
+        double g3 = fetch(v3 ? v.s3 : wlo);
         g0 = v0 ? g0 : 0.0;
         g1 = v1 ? g1 : 0.0;
         g2 = v2 ? g2 : 0.0;
@@ -373,7 +394,7 @@ struct gz_pr_pull_op_hot {
 
 template <int HOT, int VPL>
 __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_simd) void gz_k_edge_pull_pr_sliced_hot(
-    const gz_vec_t* __restrict__ vecs, const gz_slice_chunk_t* __restrict__ table, gz_xcd_lens lens,
+    gz_packed_list list, const gz_slice_chunk_t* __restrict__ table, gz_xcd_lens lens,
     const double* __restrict__ props, double* __restrict__ parts, gz_slice_offsets part_base,
     mergeaccum_t* __restrict__ merge, double* __restrict__ partials, uint64_t* __restrict__ stats, uint32_t collect,
     uint32_t* __restrict__ counters, uint64_t slice_vertices, uint64_t hot_count, uint32_t cold_slice,
@@ -404,12 +425,12 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
         gz_lds_f64* lds = (gz_lds_f64*)hot;
         if (cold_stream && sl == cold_slice) {
             gz_pr_pull_op_hot<true> op{gprops, lds, 0, 0, parts + part_base.off[sl], 0.0};
-            gz_walk_chunk<VPL, double, true>(vecs, c.begin, c.begin + c.count, op, first_d, carry_d, carry_v, edges);
+            gz_walk_list<VPL, double, true>(list, c.begin, c.begin + c.count, op, first_d, carry_d, carry_v, edges);
             sum = op.sum;
         } else {
             gz_pr_pull_op_hot<false> op{gprops, lds, (sl == x) ? lo : 0, (sl == x) ? hot_count : 0,
                                         parts + part_base.off[sl], 0.0};
-            gz_walk_chunk<VPL, double, true>(vecs, c.begin, c.begin + c.count, op, first_d, carry_d, carry_v, edges);
+            gz_walk_list<VPL, double, true>(list, c.begin, c.begin + c.count, op, first_d, carry_d, carry_v, edges);
             sum = op.sum;
         }
         const double S = gz_wave_sum(sum);
@@ -444,7 +465,8 @@ static void gz_launch_hot_vpl(const gz_xcd_lens& lens, uint64_t hot_count)
     if (per_xcd == 0) per_xcd = 1;
     const uint64_t blocks = (uint64_t)GZ_XCDS * per_xcd * (uint64_t)per_cu;
     hipLaunchKernelGGL((gz_k_edge_pull_pr_sliced_hot<HOT, VPL>), dim3((unsigned)blocks), dim3(WG), 0, gz_stream,
-                       gz_slices.vecs, gz_slices.table, lens, (const double*)gz_graph.vertex_props, gz_slices.parts,
+                       gz_packed_list{gz_slices.nbr, gz_slices.shared}, gz_slices.table, lens,
+                       (const double*)gz_graph.vertex_props, gz_slices.parts,
                        gz_slices.part_base, gz_slices.merge, (double*)gz_scratch.partials, gz_scratch.stats,
                        gz_phase_config.collect_stats, gz_slices.counters, gz_slices.slice_vertices, hot_count,
                        gz_slices.nslices - 1, gz_slices_cold_stream);
@@ -467,7 +489,7 @@ static void gz_launch_hot(const gz_xcd_lens& lens, uint64_t hot_count)
 
 void gz_pull_pr_sliced(const gz_vec_t* edge_list, uint64_t edge_list_count)
 {
-    if (gz_slices.vecs == nullptr || gz_slices.base_list != edge_list || gz_slices.base_count != edge_list_count) {
+    if (gz_slices.nbr == nullptr || gz_slices.base_list != edge_list || gz_slices.base_count != edge_list_count) {
         fprintf(stderr,
                 "grazelle: the source-sliced pull needs a sliced layout of this edge list "
                 "(gz_graph_build_source_slices)\n");
@@ -503,7 +525,8 @@ void gz_pull_pr_sliced(const gz_vec_t* edge_list, uint64_t edge_list_count)
         const uint32_t cold_slice = gz_slices.nslices - 1;
         gz_ktimer_begin(GZ_K_PULL_PR);
 #define GZ_SLICED_ARGS                                                                                          \
-    dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, gz_slices.vecs, gz_slices.table,                       \
+    dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, gz_packed_list{gz_slices.nbr, gz_slices.shared},         \
+        gz_slices.table,                                                                                          \
         (const double*)gz_graph.vertex_props, gz_slices.parts, gz_slices.part_base, cold_slice, gz_slices.merge,  \
         (double*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats
         const bool cs = gz_slices_cold_stream != 0;

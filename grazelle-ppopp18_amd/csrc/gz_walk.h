@@ -21,9 +21,10 @@
  * The run open when the chunk ends is returned to the caller (merge record or
  * direct atomic), together with the chunk's first destination.
  *
- * Op supplies: T identity(); T combine(T, T); T gather(const gz_lanes&, bool
- * in_range, uint32_t& edges) (identity when !in_range); void store(uint64_t
- * d, T value) for a complete run.
+ * Op supplies: T identity(); T combine(T, T); T gather(const vec&, bool
+ * in_range, uint32_t& edges) (identity when !in_range) for the list's vector
+ * type; void store(uint64_t d, T value) for a complete run.  The list is a
+ * Vector-Sparse list (gz_vs_list) or a packed copy (gz_packed_list).
  */
 #ifndef GZ_WALK_H
 #define GZ_WALK_H
@@ -151,14 +152,40 @@ __device__ __forceinline__ T gz_segscan_dpp(T S, int lane, int seg, const Op& op
     return S;
 }
 
-template <int VPL>
-struct gz_slice {
-    gz_lanes v[VPL];
+/* Where the walker reads its vectors: a Vector-Sparse list ... */
+struct gz_vs_list {
+    const gz_vec_t* __restrict__ vecs;
+    typedef gz_lanes vec;
+    __device__ __forceinline__ vec load(uint64_t i) const { return gz_load_vector(vecs, i); }
+    __device__ __forceinline__ static uint64_t dst(const vec& v) { return gz_shared_vertex(v); }
 };
 
-template <int VPL>
-__device__ __forceinline__ void gz_load_slice(gz_slice<VPL>& s, const gz_vec_t* __restrict__ vecs, uint64_t first,
-                                              uint64_t e)
+/* ... or a packed list (gz_packed_vec, gz_internal.h). */
+struct gz_packed_list {
+    const gz_u32x4* __restrict__ nbr;
+    const uint32_t* __restrict__ shared;
+    typedef gz_packed_vec vec;
+    __device__ __forceinline__ vec load(uint64_t i) const
+    {
+        const gz_u32x4 a = __builtin_nontemporal_load(nbr + i);
+        vec v;
+        v.s0 = a.x;
+        v.s1 = a.y;
+        v.s2 = a.z;
+        v.s3 = a.w;
+        v.d = __builtin_nontemporal_load(shared + i);
+        return v;
+    }
+    __device__ __forceinline__ static uint64_t dst(const vec& v) { return v.d; }
+};
+
+template <typename L, int VPL>
+struct gz_slice {
+    typename L::vec v[VPL];
+};
+
+template <typename L, int VPL>
+__device__ __forceinline__ void gz_load_slice(gz_slice<L, VPL>& s, const L& list, uint64_t first, uint64_t e)
 {
     /* unconditional loads (indices past the chunk end re-read its last vector
      * and are masked by the caller): no branch, so the wait counts after the
@@ -166,15 +193,15 @@ __device__ __forceinline__ void gz_load_slice(gz_slice<VPL>& s, const gz_vec_t* 
 #pragma unroll
     for (int k = 0; k < VPL; ++k) {
         const uint64_t i = first + (uint64_t)k;
-        s.v[k] = gz_load_vector(vecs, (i < e) ? i : e - 1);
+        s.v[k] = list.load((i < e) ? i : e - 1);
     }
 }
 
 /* One step of the walk: reduce the slice `cur` (vectors base + lane*VPL ...)
  * and prefetch the following step's slice into `nxt`. */
-template <int VPL, typename T, bool kDpp, typename Op>
-__device__ __forceinline__ void gz_walk_step(const gz_vec_t* __restrict__ vecs, uint64_t b, uint64_t e, uint64_t base,
-                                             Op& op, const gz_slice<VPL>& cur, gz_slice<VPL>& nxt, uint64_t& carry_d,
+template <int VPL, typename T, bool kDpp, typename L, typename Op>
+__device__ __forceinline__ void gz_walk_step(const L& list, uint64_t b, uint64_t e, uint64_t base, Op& op,
+                                             const gz_slice<L, VPL>& cur, gz_slice<L, VPL>& nxt, uint64_t& carry_d,
                                              T& carry_v, uint32_t& edges)
 {
     const int lane = gz_lane();
@@ -185,7 +212,7 @@ __device__ __forceinline__ void gz_walk_step(const gz_vec_t* __restrict__ vecs, 
     T gv[VPL];
 #pragma unroll
     for (int k = 0; k < VPL; ++k) gv[k] = op.gather(cur.v[k], my_first + (uint64_t)k < e, edges);
-    gz_load_slice<VPL>(nxt, vecs, my_first + step, e);
+    gz_load_slice<L, VPL>(nxt, list, my_first + step, e);
     /* 1. lane-local runs */
     int nvalid = 0;
     uint64_t h_d = ~0ull, t_d = ~0ull;
@@ -193,7 +220,7 @@ __device__ __forceinline__ void gz_walk_step(const gz_vec_t* __restrict__ vecs, 
 #pragma unroll
     for (int k = 0; k < VPL; ++k) {
         if (my_first + (uint64_t)k >= e) break;
-        const uint64_t d = gz_shared_vertex(cur.v[k]);
+        const uint64_t d = L::dst(cur.v[k]);
         const T g = gv[k];
         if (nvalid == 0) {
             h_d = t_d = d;
@@ -260,28 +287,37 @@ __device__ __forceinline__ void gz_walk_step(const gz_vec_t* __restrict__ vecs, 
     carry_v = gz_readlane_t<T>(S, last);
 }
 
-template <int VPL, typename T, bool kDpp = false, typename Op>
-__device__ __forceinline__ void gz_walk_chunk(const gz_vec_t* __restrict__ vecs, uint64_t b, uint64_t e, Op& op,
-                                              uint64_t& first_d, uint64_t& carry_d, T& carry_v, uint32_t& edges)
+template <int VPL, typename T, bool kDpp, typename L, typename Op>
+__device__ __forceinline__ void gz_walk_list(const L& list, uint64_t b, uint64_t e, Op& op, uint64_t& first_d,
+                                             uint64_t& carry_d, T& carry_v, uint32_t& edges)
 {
     const int lane = gz_lane();
     const uint64_t step = (uint64_t)GZ_WAVE * VPL;
-    first_d = gz_shared_vertex(gz_load_vector(vecs, b));
+    first_d = L::dst(list.load(b));
     carry_d = first_d;
     carry_v = op.identity();
     /* two slice buffers used in turn, so that no register copy forces the
      * prefetched loads to complete early */
-    gz_slice<VPL> sa, sb;
-    gz_load_slice<VPL>(sa, vecs, b + (uint64_t)lane * VPL, e);
+    gz_slice<L, VPL> sa, sb;
+    gz_load_slice<L, VPL>(sa, list, b + (uint64_t)lane * VPL, e);
     uint64_t base = b;
     while (true) {
-        gz_walk_step<VPL, T, kDpp>(vecs, b, e, base, op, sa, sb, carry_d, carry_v, edges);
+        gz_walk_step<VPL, T, kDpp>(list, b, e, base, op, sa, sb, carry_d, carry_v, edges);
         base += step;
         if (base >= e) break;
-        gz_walk_step<VPL, T, kDpp>(vecs, b, e, base, op, sb, sa, carry_d, carry_v, edges);
+        gz_walk_step<VPL, T, kDpp>(list, b, e, base, op, sb, sa, carry_d, carry_v, edges);
         base += step;
         if (base >= e) break;
     }
+}
+
+/* The walk over a Vector-Sparse list. */
+template <int VPL, typename T, bool kDpp = false, typename Op>
+__device__ __forceinline__ void gz_walk_chunk(const gz_vec_t* __restrict__ vecs, uint64_t b, uint64_t e, Op& op,
+                                              uint64_t& first_d, uint64_t& carry_d, T& carry_v, uint32_t& edges)
+{
+    const gz_vs_list list{vecs};
+    gz_walk_list<VPL, T, kDpp>(list, b, e, op, first_d, carry_d, carry_v, edges);
 }
 
 #endif /* GZ_WALK_H */
