@@ -598,8 +598,9 @@ __global__ void gz_k_local_dst(gz_vec_t* __restrict__ vecs, uint64_t n, const ui
     }
 }
 
-/* Vector-Sparse vectors -> packed vectors (32-bit neighbour ids, empty
- * lanes GZ_PACKED_NONE, and the 32-bit shared vertex). */
+/* Vector-Sparse vectors of one slice -> packed vectors: 32-bit neighbour ids
+ * (empty lanes GZ_PACKED_NONE), bit 31 of the first set on the first vector
+ * of each destination, and the 32-bit shared vertex in its own array. */
 __global__ void gz_k_pack_u32(const gz_vec_t* __restrict__ vecs, uint64_t n, gz_u32x4* __restrict__ nbr,
                               uint32_t* __restrict__ shared)
 {
@@ -610,13 +611,23 @@ __global__ void gz_k_pack_u32(const gz_vec_t* __restrict__ vecs, uint64_t n, gz_
         w.l1 = vecs[j].lane[1];
         w.l2 = vecs[j].lane[2];
         w.l3 = vecs[j].lane[3];
+        const uint64_t d = gz_shared_vertex(w);
+        bool first = true;
+        if (j > 0) {
+            gz_lanes p;
+            p.l0 = vecs[j - 1].lane[0];
+            p.l1 = vecs[j - 1].lane[1];
+            p.l2 = vecs[j - 1].lane[2];
+            p.l3 = vecs[j - 1].lane[3];
+            first = gz_shared_vertex(p) != d;
+        }
         gz_u32x4 a;
-        a.x = gz_lane_valid(w.l0) ? (uint32_t)gz_lane_id(w.l0) : GZ_PACKED_NONE;
+        a.x = (uint32_t)gz_lane_id(w.l0) | (first ? 0x80000000u : 0u); /* the first lane holds an edge */
         a.y = gz_lane_valid(w.l1) ? (uint32_t)gz_lane_id(w.l1) : GZ_PACKED_NONE;
         a.z = gz_lane_valid(w.l2) ? (uint32_t)gz_lane_id(w.l2) : GZ_PACKED_NONE;
         a.w = gz_lane_valid(w.l3) ? (uint32_t)gz_lane_id(w.l3) : GZ_PACKED_NONE;
         nbr[j] = a;
-        shared[j] = (uint32_t)gz_shared_vertex(w);
+        shared[j] = (uint32_t)d;
     }
 }
 

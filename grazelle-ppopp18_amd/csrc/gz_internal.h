@@ -15,9 +15,10 @@
 
 typedef unsigned int gz_u32x4 __attribute__((ext_vector_type(4)));
 
-/* A packed vector: the four 32-bit neighbour ids of a Vector-Sparse vector
- * (GZ_PACKED_NONE for an empty lane; valid lanes form a prefix) and its
- * 32-bit shared vertex, stored as a 16-byte and a 4-byte array. */
+/* A packed vector as the walker sees it: the four 32-bit neighbour ids of a
+ * Vector-Sparse vector (GZ_PACKED_NONE for an empty lane; valid lanes form a
+ * prefix) and d, its destination or a flag while it is being loaded
+ * (gz_packed_list, gz_walk.h). */
 #define GZ_PACKED_NONE 0xffffffffu
 struct gz_packed_vec {
     uint32_t s0, s1, s2, s3, d;

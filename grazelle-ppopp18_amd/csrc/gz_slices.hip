@@ -12,8 +12,9 @@
  * the cold slice, so each XCD gathers from one S-vertex window at a time.
  *
  * Each slice is an ordinary pull list whose destinations are numbered
- * densely inside the slice, stored packed (32-bit neighbour ids and shared
- * vertex: 20 bytes per vector instead of 32), walked exactly as in gz_pr.hip
+ * densely inside the slice, stored packed (32-bit neighbour ids, the
+ * destinations implied by flags: 16 bytes per vector instead of 32), walked
+ * exactly as in gz_pr.hip
  * into its own accumulator run (complete runs stored directly, the chunk's last run into
  * its merge record).  The merge then runs per slice
  * (edge_pull_op_merge_with_merge_buffer's rule, phases.c:41-70), and the
@@ -181,8 +182,8 @@ extern "C" int gz_graph_source_slices_download(gz_vec_t* vectors, uint32_t* mask
         for (uint64_t j = 0; j < n; ++j) {
             uint64_t ids[4];
             unsigned count = 0;
-            while (count < 4 && nbr[4 * j + count] != GZ_PACKED_NONE) {
-                ids[count] = nbr[4 * j + count];
+            while (count < 4 && (count == 0 || nbr[4 * j + count] != GZ_PACKED_NONE)) {
+                ids[count] = nbr[4 * j + count] & (count == 0 ? 0x7fffffffu : 0xffffffffu);
                 ++count;
             }
             gz_vec_encode(&vectors[j], shared[j], ids, count);

@@ -152,31 +152,89 @@ __device__ __forceinline__ T gz_segscan_dpp(T S, int lane, int seg, const Op& op
     return S;
 }
 
-/* Where the walker reads its vectors: a Vector-Sparse list ... */
+/* Where the walker reads its vectors.  A list supplies load(i), the
+ * destination of the chunk's first vector, and the destinations of a step's
+ * vectors (dsts), with a cursor carried from step to step.
+ *
+ * A Vector-Sparse list: each vector names its destination. */
 struct gz_vs_list {
     const gz_vec_t* __restrict__ vecs;
     typedef gz_lanes vec;
+    struct cursor {};
     __device__ __forceinline__ vec load(uint64_t i) const { return gz_load_vector(vecs, i); }
-    __device__ __forceinline__ static uint64_t dst(const vec& v) { return gz_shared_vertex(v); }
+    __device__ __forceinline__ uint64_t first_dst(uint64_t b, cursor&) const
+    {
+        return gz_shared_vertex(load(b));
+    }
+    template <int VPL>
+    __device__ __forceinline__ void dsts(const vec (&v)[VPL], uint64_t, uint64_t, uint64_t, cursor&,
+                                         uint64_t (&d)[VPL]) const
+    {
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) d[k] = gz_shared_vertex(v[k]);
+    }
 };
 
-/* ... or a packed list (gz_packed_vec, gz_internal.h). */
+/* A packed list (gz_packed_vec, gz_internal.h) whose destinations are the
+ * dense ranks 0, 1, 2, ... in order: bit 31 of a vector's first neighbour id
+ * flags the first vector of a destination, so a vector's destination is the
+ * chunk's first destination plus the flags after the chunk's first vector up
+ * to it.  `shared` holds every vector's destination but is read only at chunk
+ * starts. */
+#define GZ_PACKED_FIRST 0x80000000u
 struct gz_packed_list {
     const gz_u32x4* __restrict__ nbr;
     const uint32_t* __restrict__ shared;
     typedef gz_packed_vec vec;
+    struct cursor {
+        uint64_t d;
+    };
     __device__ __forceinline__ vec load(uint64_t i) const
     {
         const gz_u32x4 a = __builtin_nontemporal_load(nbr + i);
         vec v;
-        v.s0 = a.x;
+        v.s0 = a.x & ~GZ_PACKED_FIRST; /* the first lane always holds an edge */
         v.s1 = a.y;
         v.s2 = a.z;
         v.s3 = a.w;
-        v.d = __builtin_nontemporal_load(shared + i);
+        v.d = a.x >> 31; /* the flag; the destination comes from dsts() */
         return v;
     }
-    __device__ __forceinline__ static uint64_t dst(const vec& v) { return v.d; }
+    __device__ __forceinline__ uint64_t first_dst(uint64_t b, cursor& c) const
+    {
+        c.d = shared[b];
+        return c.d;
+    }
+    template <int VPL>
+    __device__ __forceinline__ void dsts(const vec (&v)[VPL], uint64_t my_first, uint64_t b, uint64_t e, cursor& c,
+                                         uint64_t (&d)[VPL]) const
+    {
+        /* flags counted: in range and not the chunk's first vector */
+        uint32_t f[VPL], t = 0;
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) {
+            const uint64_t i = my_first + (uint64_t)k;
+            f[k] = (i < e && i != b) ? v[k].d : 0u;
+            t += f[k];
+        }
+        /* exclusive prefix of the lanes' counts (at most VPL <= 4: 3 bits) */
+        const uint64_t below = (1ull << gz_lane()) - 1ull;
+        uint64_t before = 0, total = 0;
+#pragma unroll
+        for (int bit = 0; bit < 3; ++bit) {
+            if ((1 << bit) > VPL) break;
+            const uint64_t bal = __ballot((t >> bit) & 1u);
+            before += (uint64_t)__popcll(bal & below) << bit;
+            total += (uint64_t)__popcll(bal) << bit;
+        }
+        uint64_t x = c.d + before;
+#pragma unroll
+        for (int k = 0; k < VPL; ++k) {
+            x += f[k];
+            d[k] = x;
+        }
+        c.d += total;
+    }
 };
 
 template <typename L, int VPL>
@@ -201,8 +259,9 @@ __device__ __forceinline__ void gz_load_slice(gz_slice<L, VPL>& s, const L& list
  * and prefetch the following step's slice into `nxt`. */
 template <int VPL, typename T, bool kDpp, typename L, typename Op>
 __device__ __forceinline__ void gz_walk_step(const L& list, uint64_t b, uint64_t e, uint64_t base, Op& op,
-                                             const gz_slice<L, VPL>& cur, gz_slice<L, VPL>& nxt, uint64_t& carry_d,
-                                             T& carry_v, uint32_t& edges)
+                                             const gz_slice<L, VPL>& cur, gz_slice<L, VPL>& nxt,
+                                             typename L::cursor& cursor, uint64_t& carry_d, T& carry_v,
+                                             uint32_t& edges)
 {
     const int lane = gz_lane();
     const uint64_t step = (uint64_t)GZ_WAVE * VPL;
@@ -213,6 +272,8 @@ __device__ __forceinline__ void gz_walk_step(const L& list, uint64_t b, uint64_t
 #pragma unroll
     for (int k = 0; k < VPL; ++k) gv[k] = op.gather(cur.v[k], my_first + (uint64_t)k < e, edges);
     gz_load_slice<L, VPL>(nxt, list, my_first + step, e);
+    uint64_t dd[VPL];
+    list.template dsts<VPL>(cur.v, my_first, b, e, cursor, dd);
     /* 1. lane-local runs */
     int nvalid = 0;
     uint64_t h_d = ~0ull, t_d = ~0ull;
@@ -220,7 +281,7 @@ __device__ __forceinline__ void gz_walk_step(const L& list, uint64_t b, uint64_t
 #pragma unroll
     for (int k = 0; k < VPL; ++k) {
         if (my_first + (uint64_t)k >= e) break;
-        const uint64_t d = L::dst(cur.v[k]);
+        const uint64_t d = dd[k];
         const T g = gv[k];
         if (nvalid == 0) {
             h_d = t_d = d;
@@ -293,7 +354,8 @@ __device__ __forceinline__ void gz_walk_list(const L& list, uint64_t b, uint64_t
 {
     const int lane = gz_lane();
     const uint64_t step = (uint64_t)GZ_WAVE * VPL;
-    first_d = L::dst(list.load(b));
+    typename L::cursor cursor;
+    first_d = list.first_dst(b, cursor);
     carry_d = first_d;
     carry_v = op.identity();
     /* two slice buffers used in turn, so that no register copy forces the
@@ -302,10 +364,10 @@ __device__ __forceinline__ void gz_walk_list(const L& list, uint64_t b, uint64_t
     gz_load_slice<L, VPL>(sa, list, b + (uint64_t)lane * VPL, e);
     uint64_t base = b;
     while (true) {
-        gz_walk_step<VPL, T, kDpp>(list, b, e, base, op, sa, sb, carry_d, carry_v, edges);
+        gz_walk_step<VPL, T, kDpp>(list, b, e, base, op, sa, sb, cursor, carry_d, carry_v, edges);
         base += step;
         if (base >= e) break;
-        gz_walk_step<VPL, T, kDpp>(list, b, e, base, op, sb, sa, carry_d, carry_v, edges);
+        gz_walk_step<VPL, T, kDpp>(list, b, e, base, op, sb, sa, cursor, carry_d, carry_v, edges);
         base += step;
         if (base >= e) break;
     }
