@@ -235,10 +235,24 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr_sliced(
     const uint64_t below = (1ull << lane) - 1ull;
     const uint64_t last = (end + GZ_WAVE - 1) / GZ_WAVE;
     const uint64_t waves = (uint64_t)gridDim.x * GZ_WAVES_PER_BLOCK;
-    for (uint64_t blk = start / GZ_WAVE + (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + threadIdx.x / GZ_WAVE;
-         blk < last; blk += waves) {
+    uint64_t blk = start / GZ_WAVE + (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + threadIdx.x / GZ_WAVE;
+    /* the next block's mask is loaded one iteration ahead, so that the slice
+     * sums (whose addresses depend on it) are the only wait of an iteration */
+    uint32_t m_next = 0;
+    if (blk < last) {
+        const uint64_t v0 = blk * GZ_WAVE + (uint64_t)lane;
+        m_next = (v0 < num_vertices) ? mask[v0] : 0u;
+    }
+    for (; blk < last; blk += waves) {
         const uint64_t v = blk * GZ_WAVE + (uint64_t)lane;
-        const uint32_t m = (v < num_vertices) ? mask[v] : 0u;
+        const uint32_t m = m_next;
+        const uint64_t nb = blk + waves;
+        if (nb < last) {
+            const uint64_t vn = nb * GZ_WAVE + (uint64_t)lane;
+            m_next = (vn < num_vertices) ? mask[vn] : 0u;
+        }
+        const bool mine = v >= start && v < end;
+        const double od = mine ? outdeg[v] : 1.0;
         const uint32_t* rb = rank_base + blk * (uint64_t)NS;
         double p[NS];
 #pragma unroll
@@ -259,7 +273,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr_sliced(
                 a = first ? p[x] : a + p[x];
                 first = false;
             }
-        if (v >= start && v < end) props[v] = gz_pr_vertex_value(a, outdeg[v], c, k, damping, nv);
+        if (mine) props[v] = gz_pr_vertex_value(a, od, c, k, damping, nv);
     }
 }
 
