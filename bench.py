@@ -127,8 +127,8 @@ def headline(gz, args) -> dict:
 
 
 def full_cc_bfs(gz) -> dict:
-    """CC and BFS (16 roots, fixed threshold and learned selector) on the
-    already loaded full-scale graph."""
+    """CC (plain and source-sliced pull) and BFS (16 roots, fixed threshold and
+    learned selector) on the already loaded full-scale graph."""
     import selector_train as st
 
     E = gz.num_edges()
@@ -136,9 +136,22 @@ def full_cc_bfs(gz) -> dict:
     try:
         gz.run(gz.options("cc"))
         r = gz.run(gz.options("cc"))
+        labels = gz.cc_labels()
         out["cc"] = {"time_to_solution_ms": round(r.elapsed_ms, 2), "iterations": r.iterations,
                      "pull_iterations": r.pull_iterations, "push_iterations": r.push_iterations,
                      "gteps": round(E * r.iterations / (r.elapsed_ms * 1e-3) / 1e9, 2)}
+        if gz.source_slices_info() is not None:
+            # the pull sweeps over the source-sliced layout (labels must not change)
+            gz.set_chunk_vectors(SLICED_CHUNK)
+            gz.slices_set_hot_cache(HOT_CACHE)
+            gz.run(gz.options("cc", pull_variant="source_sliced"))
+            r = gz.run(gz.options("cc", pull_variant="source_sliced"))
+            same = bool(np.array_equal(gz.cc_labels(), labels))
+            gz.set_chunk_vectors(512)
+            gz.slices_set_hot_cache(0)
+            out["cc_source_sliced"] = {"time_to_solution_ms": round(r.elapsed_ms, 2), "iterations": r.iterations,
+                                       "pull_iterations": r.pull_iterations,
+                                       "push_iterations": r.push_iterations, "labels_equal_plain": same}
         roots = st.pick_roots(16, 26)
         gz.run(gz.options("bfs", root=roots[0]))
         fixed = sum(gz.run(gz.options("bfs", root=x)).elapsed_ms for x in roots)

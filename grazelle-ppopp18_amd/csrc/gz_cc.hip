@@ -98,6 +98,10 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_cc(const gz_vec_t* __
 
 extern "C" void perform_edge_pull_phase_cc(const gz_vec_t* edge_list, uint64_t edge_list_count)
 {
+    if (gz_phase_config.pull_variant == GZ_PULL_SOURCE_SLICED && edge_list_count > 0) {
+        gz_pull_cc_sliced(edge_list, edge_list_count);
+        return;
+    }
     const uint64_t nchunks = gz_chunks_for(edge_list_count);
     gz_scratch_reserve(nchunks, gz_graph.num_vertices);
     gz_scratch.partials_is_double = 0;

@@ -569,6 +569,15 @@ __global__ void gz_k_slice_mask(const uint64_t* __restrict__ voff, uint64_t nv, 
     }
 }
 
+/* gid[rank[v]] = v for the vertices with vectors in the slice */
+__global__ void gz_k_slice_gid(const uint64_t* __restrict__ has, const uint64_t* __restrict__ rank, uint64_t nv,
+                               uint32_t* __restrict__ gid)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += stride)
+        if (has[v]) gid[rank[v]] = (uint32_t)v;
+}
+
 /* local rank in slice `x` of the first vertex of each 64-vertex block, at
  * out[block * nslices + x] */
 __global__ void gz_k_rank_base(const uint64_t* __restrict__ rank, uint64_t blocks, uint32_t x, uint32_t nslices,
@@ -691,6 +700,7 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
                                      rocprim::plus<uint64_t>(), gz_stream));
     void* scan_tmp = gz_device_malloc(scan_bytes ? scan_bytes : 1);
     gz_vec_t* part_vecs[GZ_MAX_SLICES];
+    uint32_t* part_gid[GZ_MAX_SLICES];
     uint64_t counts[GZ_MAX_SLICES], ranked[GZ_MAX_SLICES];
     for (uint32_t x = 0; x < ns; ++x) {
         uint64_t* voff = nullptr;
@@ -709,6 +719,10 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
             GZ_LAUNCH_CHECK();
         }
         gz_copy_to_host(&ranked[x], rank + nv, sizeof(uint64_t));
+        part_gid[x] = (uint32_t*)gz_device_malloc((ranked[x] ? ranked[x] : 1) * sizeof(uint32_t));
+        hipLaunchKernelGGL(gz_k_slice_gid, dim3(gz_grid_for(nv)), dim3(GZ_BLOCK), 0, gz_stream, has, rank, nv,
+                           part_gid[x]);
+        GZ_LAUNCH_CHECK();
         GZ_CHECK(hipStreamSynchronize(gz_stream));
         gz_device_free(voff);
     }
@@ -736,6 +750,12 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
     memset(&gz_slices.part_base, 0, sizeof(gz_slices.part_base));
     for (uint32_t x = 0; x < ns; ++x) gz_slices.part_base.off[x + 1] = gz_slices.part_base.off[x] + ranked[x];
     gz_slices.parts = (double*)gz_device_malloc((gz_slices.part_base.off[ns] + 8) * sizeof(double));
+    gz_slices.gid = (uint32_t*)gz_device_malloc((gz_slices.part_base.off[ns] + 8) * sizeof(uint32_t));
+    for (uint32_t x = 0; x < ns; ++x) {
+        if (ranked[x]) gz_device_copy(gz_slices.gid + gz_slices.part_base.off[x], part_gid[x], ranked[x] * sizeof(uint32_t));
+        GZ_CHECK(hipStreamSynchronize(gz_stream));
+        gz_device_free(part_gid[x]);
+    }
     gz_slices.base_list = gz_graph.pull_vectors;
     gz_slices.base_count = nvec;
     gz_slices.slice_vertices = S;

@@ -104,6 +104,7 @@ typedef struct gz_source_slices_t {
      * vertex of slice x's vectors is that local rank. */
     double* parts;                    /* slice x's accumulators at parts + part_base.off[x], by local rank */
     gz_slice_offsets part_base;
+    uint32_t* gid;                    /* global vertex of each (slice, local rank), same offsets as parts */
     uint32_t* rank_base;              /* [rank_blocks][nslices] local rank of each 64-vertex block's first vertex */
     uint64_t rank_blocks;             /* ceil(V / 64) */
     gz_slice_chunk_t* table;          /* chunk slots, built for table_chunk_len */
@@ -120,6 +121,8 @@ extern gz_source_slices_t gz_slices;
 /* PageRank pull over gz_slices (pull and per-slice merge; the vertex phase
  * sums the slices). */
 void gz_pull_pr_sliced(const gz_vec_t* edge_list, uint64_t edge_list_count);
+/* Connected Components pull over gz_slices. */
+void gz_pull_cc_sliced(const gz_vec_t* edge_list, uint64_t edge_list_count);
 
 /* Bracket a kernel launch for gz_kernel_timing_*; no-ops when disabled. */
 void gz_ktimer_begin(int id);

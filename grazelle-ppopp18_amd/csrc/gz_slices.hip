@@ -151,6 +151,7 @@ extern "C" void gz_source_slices_release(void)
     gz_device_free(gz_slices.mask);
     gz_device_free(gz_slices.parts);
     gz_device_free(gz_slices.rank_base);
+    gz_device_free(gz_slices.gid);
     gz_device_free(gz_slices.table);
     gz_device_free(gz_slices.merge);
     gz_device_free(gz_slices.counters);
@@ -484,6 +485,182 @@ static void gz_launch_hot(const gz_xcd_lens& lens, uint64_t hot_count)
         gz_launch_hot_vpl<HOT, 2>(lens, hot_count);
     else
         gz_launch_hot_vpl<HOT, 1>(lens, hot_count);
+}
+
+/* ---- Connected Components over the same layout ---------------------------------
+ * The min walk of gz_k_edge_pull_cc (phase_edge_pull_cc.asm:50-325) over the
+ * packed slices: each complete run of a slice-local destination applies its
+ * minimum to the destination's global label with an atomic min, as every run
+ * of the plain sweep does, setting HasInfo* and adding to the vote when the
+ * label drops.  The LDS cache holds the window's labels as they were when the
+ * workgroup started: a label lowered later in the sweep reaches these gathers
+ * one sweep later, which changes the number of sweeps, never the labels (a
+ * sweep that lowers no label found every label already final). */
+typedef __attribute__((address_space(3))) const uint64_t gz_lds_u64;
+typedef __attribute__((address_space(1))) const uint64_t gz_global_u64;
+
+struct gz_cc_pull_op_hot {
+    gz_global_u64* __restrict__ in;      /* labels, read */
+    gz_lds_u64* hot;
+    uint64_t wlo, wn;
+    uint64_t* __restrict__ labels;       /* labels, atomic min */
+    uint64_t* __restrict__ next_frontier;
+    const double* __restrict__ outdeg;
+    const uint32_t* __restrict__ gid;    /* this slice's local rank -> vertex */
+    uint32_t stat_mode;
+    uint64_t vote;
+    __device__ __forceinline__ uint64_t identity() const { return ~0ull; }
+    __device__ __forceinline__ uint64_t combine(uint64_t a, uint64_t b) const { return a < b ? a : b; }
+    __device__ __forceinline__ uint64_t fetch(uint64_t i) const
+    {
+        const uint64_t r = i - wlo;
+        uint64_t x;
+        if (r < wn)
+            x = hot[r];
+        else
+            x = in[i];
+        return x;
+    }
+    __device__ __forceinline__ uint64_t gather(const gz_packed_vec& v, bool in_range, uint32_t& edges) const
+    {
+        const bool v0 = in_range && v.s0 != GZ_PACKED_NONE, v1 = in_range && v.s1 != GZ_PACKED_NONE,
+                   v2 = in_range && v.s2 != GZ_PACKED_NONE, v3 = in_range && v.s3 != GZ_PACKED_NONE;
+        const uint64_t x0 = fetch(v0 ? v.s0 : wlo), x1 = fetch(v1 ? v.s1 : wlo), x2 = fetch(v2 ? v.s2 : wlo),
+                       x3 = fetch(v3 ? v.s3 : wlo);
+        uint64_t m = ~0ull;
+        m = (v0 && x0 < m) ? x0 : m;
+        m = (v1 && x1 < m) ? x1 : m;
+        m = (v2 && x2 < m) ? x2 : m;
+        m = (v3 && x3 < m) ? x3 : m;
+        edges += (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2 + (uint32_t)v3;
+        return m;
+    }
+    __device__ __forceinline__ void store(uint64_t d, uint64_t m)
+    {
+        const uint64_t g = gid[d];
+        if (m < labels[g]) {
+            const unsigned long long old = atomicMin((unsigned long long*)&labels[g], (unsigned long long)m);
+            if (old > m) {
+                gz_bit_set(next_frontier, g);
+                vote += gz_frontier_stat(outdeg, g, stat_mode);
+            }
+        }
+    }
+};
+
+template <int HOT>
+__global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_simd) void gz_k_edge_pull_cc_sliced_hot(
+    gz_packed_list list, const gz_slice_chunk_t* __restrict__ table, gz_xcd_lens lens, uint64_t* __restrict__ labels,
+    uint64_t* __restrict__ next_frontier, const double* __restrict__ outdeg, uint32_t stat_mode,
+    const uint32_t* __restrict__ gid, gz_slice_offsets part_base, uint64_t* __restrict__ partials,
+    uint64_t* __restrict__ stats, uint32_t collect, uint32_t* __restrict__ counters, uint64_t slice_vertices,
+    uint64_t hot_count)
+{
+    __shared__ uint64_t hot[HOT];
+    const uint32_t x = blockIdx.x % GZ_XCDS;
+    const uint64_t lo = (uint64_t)x * slice_vertices;
+    for (uint64_t i = threadIdx.x; i < hot_count; i += blockDim.x) hot[i] = labels[lo + i];
+    __syncthreads();
+    const int lane = gz_lane();
+    const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x / GZ_WAVE) + threadIdx.x / GZ_WAVE;
+    const uint64_t n = lens.len[x];
+    uint64_t vote = 0;
+    uint32_t edges = 0, vectors = 0;
+    while (true) {
+        uint32_t idx = 0;
+        if (lane == 0) idx = atomicAdd(&counters[x * GZ_COUNTER_STRIDE], 1u);
+        idx = (uint32_t)__shfl((int)idx, 0, GZ_WAVE);
+        if ((uint64_t)idx >= n) break;
+        const uint64_t slot =
+            ((uint64_t)GZ_XCDS * (idx / GZ_WAVES_PER_BLOCK) + x) * GZ_WAVES_PER_BLOCK + idx % GZ_WAVES_PER_BLOCK;
+        const gz_slice_chunk_t c = table[slot];
+        const uint32_t sl = c.record >> GZ_SLICE_RECORD_BITS;
+        gz_cc_pull_op_hot op{(gz_global_u64*)labels,
+                             (gz_lds_u64*)hot,
+                             (sl == x) ? lo : 0,
+                             (sl == x) ? hot_count : 0,
+                             labels,
+                             next_frontier,
+                             outdeg,
+                             gid + part_base.off[sl],
+                             stat_mode,
+                             0};
+        uint64_t first_d, carry_d, carry_m;
+        gz_walk_list<1, uint64_t, true>(list, c.begin, c.begin + c.count, op, first_d, carry_d, carry_m, edges);
+        if (lane == 0) op.store(carry_d, carry_m);
+        vote += op.vote;
+        vectors += c.count;
+    }
+    const uint64_t total = gz_wave_sum_u64(vote);
+    if (lane == 0) partials[wave] = total;
+    if (collect) {
+        const uint64_t tot_edges = gz_wave_sum_u64(edges);
+        if (lane == 0) {
+            stats[2 * wave] = vectors;
+            stats[2 * wave + 1] = tot_edges;
+        }
+    }
+}
+
+template <int HOT>
+static uint64_t gz_cc_hot_grid()
+{
+    int per_cu = 0;
+    GZ_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, gz_k_edge_pull_cc_sliced_hot<HOT>,
+                                                          gz_hot_wg<HOT>::value, 0));
+    if (per_cu < 1) per_cu = 1;
+    uint64_t per_xcd = gz_cu_count / GZ_XCDS;
+    if (per_xcd == 0) per_xcd = 1;
+    return (uint64_t)GZ_XCDS * per_xcd * (uint64_t)per_cu;
+}
+
+template <int HOT>
+static void gz_launch_cc_hot(const gz_xcd_lens& lens, uint64_t hot_count, uint64_t blocks)
+{
+    hipLaunchKernelGGL(gz_k_edge_pull_cc_sliced_hot<HOT>, dim3((unsigned)blocks), dim3(gz_hot_wg<HOT>::value), 0,
+                       gz_stream, gz_packed_list{gz_slices.nbr, gz_slices.shared}, gz_slices.table, lens,
+                       (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
+                       gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, gz_slices.gid, gz_slices.part_base,
+                       (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats,
+                       gz_slices.counters, gz_slices.slice_vertices, hot_count);
+    GZ_LAUNCH_CHECK();
+}
+
+void gz_pull_cc_sliced(const gz_vec_t* edge_list, uint64_t edge_list_count)
+{
+    if (gz_slices.nbr == nullptr || gz_slices.base_list != edge_list || gz_slices.base_count != edge_list_count) {
+        fprintf(stderr,
+                "grazelle: the source-sliced pull needs a sliced layout of this edge list "
+                "(gz_graph_build_source_slices)\n");
+        exit(72);
+    }
+    gz_slices_prepare_table(gz_chunk_len());
+    /* without a configured cache the 32 KiB variant runs with an empty window */
+    const uint32_t hot = gz_slices_hot_cache ? gz_slices_hot_cache : 4096;
+    const uint64_t hot_count =
+        gz_slices_hot_cache ? (hot < gz_slices.slice_vertices ? hot : gz_slices.slice_vertices) : 0;
+    const uint64_t blocks = hot == 20480   ? gz_cc_hot_grid<20480>()
+                            : hot == 16384 ? gz_cc_hot_grid<16384>()
+                            : hot == 8192  ? gz_cc_hot_grid<8192>()
+                                           : gz_cc_hot_grid<4096>();
+    const uint64_t waves = blocks * (gz_hot_wg<4096>::value / GZ_WAVE);
+    gz_scratch_reserve(waves, gz_graph.num_vertices);
+    gz_scratch.partials_is_double = 0;
+    gz_scratch.partials_count = waves;
+    gz_scratch.stats_count = gz_phase_config.collect_stats ? waves : 0;
+    gz_xcd_lens lens;
+    for (int x = 0; x < GZ_XCDS; ++x) lens.len[x] = gz_slices.xcd_chunks[x];
+    gz_device_zero(gz_slices.counters, GZ_XCDS * GZ_COUNTER_STRIDE * sizeof(uint32_t));
+    gz_ktimer_begin(GZ_K_PULL_CC);
+    if (hot == 20480)
+        gz_launch_cc_hot<20480>(lens, hot_count, blocks);
+    else if (hot == 16384)
+        gz_launch_cc_hot<16384>(lens, hot_count, blocks);
+    else if (hot == 8192)
+        gz_launch_cc_hot<8192>(lens, hot_count, blocks);
+    else
+        gz_launch_cc_hot<4096>(lens, hot_count, blocks);
+    gz_ktimer_end(GZ_K_PULL_CC);
 }
 
 /* ---- the phase ---------------------------------------------------------------- */

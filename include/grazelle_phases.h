@@ -81,15 +81,16 @@ enum gz_stat_mode {                        /* phase_helper_add_frontier_stat, ph
 enum gz_pull_variant {
     GZ_PULL_SCHED_AWARE = 0,               /* merge buffers, plain stores (default) */
     GZ_PULL_TRADITIONAL = 1,               /* EXPERIMENT_EDGE_PULL_WITHOUT_SCHED_AWARE: atomics, no merge */
-    GZ_PULL_SOURCE_SLICED = 2              /* scheduler-aware over the source-sliced layout built by
-                                            * gz_graph_build_source_slices for this edge list; merges
-                                            * inside the phase (no reference counterpart) */
+    GZ_PULL_SOURCE_SLICED = 2              /* over the source-sliced layout built by
+                                            * gz_graph_build_source_slices for this edge list, PR
+                                            * (merges inside the phase) and CC (no reference
+                                            * counterpart) */
 };
 
 typedef struct gz_phase_config_t {
     double damping;                        /* const_damping_factor (constants.asm:75-79), 0.875 in the reference */
     uint32_t stat_mode;                    /* enum gz_stat_mode */
-    uint32_t pull_variant;                 /* enum gz_pull_variant (PR only) */
+    uint32_t pull_variant;                 /* enum gz_pull_variant (PR; GZ_PULL_SOURCE_SLICED also CC) */
     uint32_t chunk_vectors;                /* Edge vectors per scheduling unit (one wavefront), multiple of 64 */
     uint32_t bfs_level;                    /* BFS depth written by the current edge phase (1-based) */
     uint32_t collect_stats;                /* nonzero: count vectors / valid lanes examined (ITERATION_STATS) */

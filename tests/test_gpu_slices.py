@@ -178,3 +178,51 @@ def test_sliced_pr_hot_cache_small_slices(gz, orc, device, rmat14, chunk):
         gz.set_chunk_vectors(512)
     _, ref, _ = orc.pagerank(orc.Graph(g.num_vertices, g.pull, g.push), orc.options(iterations=20))
     assert _rel(ranks, ref) <= PR_TOL
+
+
+# ---- Connected Components over the sliced layout --------------------------
+
+import graphs  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def cc_trees(gz):
+    return graphs.planted_trees(gz, base_scale=12, edge_factor=8, trees=16, tree_size=200)
+
+
+@pytest.mark.parametrize("hot", [0, 4096, 20480])
+@pytest.mark.parametrize("S,T", [(64, 1), (512, 1), (128, 2)])
+@pytest.mark.parametrize("engine", ["pull", "hybrid"])
+def test_sliced_cc_labels_bit_exact(gz, orc, device, cc_trees, S, T, hot, engine):
+    g = cc_trees
+    gz.load(g)
+    gz.build_source_slices(S, T)
+    gz.slices_set_hot_cache(hot)
+    try:
+        gz.run(gz.options("cc", engine=engine, pull_variant="source_sliced"))
+        got = gz.cc_labels()
+    finally:
+        gz.slices_set_hot_cache(0)
+    labels, _, _ = orc.cc(orc.Graph(g.num_vertices, g.pull, g.push), orc.options(engine=engine))
+    assert np.array_equal(got, labels)
+
+
+@pytest.mark.parametrize("hot", [0, 20480])
+def test_sliced_cc_directed_and_renumbered(gz, orc, device, hot):
+    # directed R-MAT: labels are min{u : u reaches v}
+    scale, ef, seed = 12, 8, 31
+    nv = 1 << scale
+    raw = gz.rmat_edges(scale, nv * ef, seed)
+    g = gz.make_graph(raw, nv)
+    want, _, _ = orc.cc(orc.Graph(nv, g.pull, g.push), orc.options(engine="pull"))
+    gz.load(g)
+    gz.build_source_slices(32, 1)
+    gz.slices_set_hot_cache(hot)
+    try:
+        gz.run(gz.options("cc", engine="pull", pull_variant="source_sliced"))
+        assert np.array_equal(gz.cc_labels(), want)
+        gz.load_rmat_device(scale, nv * ef, seed, relabel=True, slices_log2=5)
+        gz.run(gz.options("cc", engine="hybrid", pull_variant="source_sliced"))
+        assert np.array_equal(gz.cc_labels(), want)
+    finally:
+        gz.slices_set_hot_cache(0)
