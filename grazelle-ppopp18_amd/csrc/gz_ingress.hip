@@ -179,10 +179,12 @@ unsigned gz_grid_for(uint64_t n)
     return (unsigned)(blocks < 65536 ? (blocks ? blocks : 1) : 65536);
 }
 
-/* Slice size actually used: at most V / (8 T), so the windows fit. */
+/* Slice size actually used: at most V / (8 T), so the windows fit, and at
+ * most 2^23, so that packed window offsets fit in 23 bits. */
 uint64_t gz_slices_effective(uint64_t slice_vertices, uint32_t tiers, uint64_t nv)
 {
-    const uint64_t cap = nv / ((uint64_t)GZ_XCDS * tiers);
+    uint64_t cap = nv / ((uint64_t)GZ_XCDS * tiers);
+    if (cap > (1ull << 23)) cap = 1ull << 23;
     return slice_vertices < cap ? slice_vertices : cap;
 }
 
@@ -607,9 +609,10 @@ __global__ void gz_k_local_dst(gz_vec_t* __restrict__ vecs, uint64_t n, const ui
     }
 }
 
-/* Vector-Sparse vectors of one slice -> packed vectors: 32-bit neighbour ids
- * (empty lanes GZ_PACKED_NONE), bit 31 of the first set on the first vector
- * of each destination, and the 32-bit shared vertex in its own array. */
+/* Vector-Sparse vectors of one slice -> their 32-bit shared vertices, and
+ * (nbr != nullptr) 16-byte packed vectors: 32-bit neighbour ids (empty lanes
+ * GZ_PACKED_NONE), bit 31 of the first set on the first vector of each
+ * destination. */
 __global__ void gz_k_pack_u32(const gz_vec_t* __restrict__ vecs, uint64_t n, gz_u32x4* __restrict__ nbr,
                               uint32_t* __restrict__ shared)
 {
@@ -635,8 +638,40 @@ __global__ void gz_k_pack_u32(const gz_vec_t* __restrict__ vecs, uint64_t n, gz_
         a.y = gz_lane_valid(w.l1) ? (uint32_t)gz_lane_id(w.l1) : GZ_PACKED_NONE;
         a.z = gz_lane_valid(w.l2) ? (uint32_t)gz_lane_id(w.l2) : GZ_PACKED_NONE;
         a.w = gz_lane_valid(w.l3) ? (uint32_t)gz_lane_id(w.l3) : GZ_PACKED_NONE;
-        nbr[j] = a;
+        if (nbr) nbr[j] = a;
         shared[j] = (uint32_t)d;
+    }
+}
+
+/* Vector-Sparse vectors of one window slice -> 12-byte packed vectors: four
+ * 24-bit offsets from the window's first vertex (empty lanes all ones), bit
+ * 23 of the first set on the first vector of each destination. */
+__global__ void gz_k_pack_u24(const gz_vec_t* __restrict__ vecs, uint64_t n, uint64_t window,
+                              uint32_t* __restrict__ words)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        gz_lanes w;
+        w.l0 = vecs[j].lane[0];
+        w.l1 = vecs[j].lane[1];
+        w.l2 = vecs[j].lane[2];
+        w.l3 = vecs[j].lane[3];
+        bool first = true;
+        if (j > 0) {
+            gz_lanes p;
+            p.l0 = vecs[j - 1].lane[0];
+            p.l1 = vecs[j - 1].lane[1];
+            p.l2 = vecs[j - 1].lane[2];
+            p.l3 = vecs[j - 1].lane[3];
+            first = gz_shared_vertex(p) != gz_shared_vertex(w);
+        }
+        const uint32_t f0 = (uint32_t)(gz_lane_id(w.l0) - window) | (first ? 0x800000u : 0u);
+        const uint32_t f1 = gz_lane_valid(w.l1) ? (uint32_t)(gz_lane_id(w.l1) - window) : 0xffffffu;
+        const uint32_t f2 = gz_lane_valid(w.l2) ? (uint32_t)(gz_lane_id(w.l2) - window) : 0xffffffu;
+        const uint32_t f3 = gz_lane_valid(w.l3) ? (uint32_t)(gz_lane_id(w.l3) - window) : 0xffffffu;
+        words[3 * j] = f0 | (f1 << 24);
+        words[3 * j + 1] = (f1 >> 8) | (f2 << 16);
+        words[3 * j + 2] = (f2 >> 16) | (f3 << 8);
     }
 }
 
@@ -736,13 +771,22 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
     gz_slices.begin[0] = 0;
     for (uint32_t x = 0; x < ns; ++x) gz_slices.begin[x + 1] = gz_slices.begin[x] + counts[x];
     const uint64_t total = gz_slices.begin[ns];
-    gz_slices.nbr = (gz_u32x4*)gz_device_malloc((total ? total : 1) * sizeof(gz_u32x4));
+    const uint64_t hot_total = gz_slices.begin[ns - 1];
+    gz_slices.nbr24 = (uint32_t*)gz_device_malloc((hot_total ? 3 * hot_total : 1) * sizeof(uint32_t));
+    gz_slices.nbr = (gz_u32x4*)gz_device_malloc((counts[ns - 1] ? counts[ns - 1] : 1) * sizeof(gz_u32x4));
     gz_slices.shared = (uint32_t*)gz_device_malloc((total ? total : 1) * sizeof(uint32_t));
     for (uint32_t x = 0; x < ns; ++x) {
         if (counts[x]) {
+            /* the shared vertices of every slice, then the packed neighbours */
             hipLaunchKernelGGL(gz_k_pack_u32, dim3(gz_grid_for(counts[x])), dim3(GZ_BLOCK), 0, gz_stream, part_vecs[x],
-                               counts[x], gz_slices.nbr + gz_slices.begin[x], gz_slices.shared + gz_slices.begin[x]);
+                               counts[x], x + 1 == ns ? gz_slices.nbr : (gz_u32x4*)nullptr,
+                               gz_slices.shared + gz_slices.begin[x]);
             GZ_LAUNCH_CHECK();
+            if (x + 1 < ns) {
+                hipLaunchKernelGGL(gz_k_pack_u24, dim3(gz_grid_for(counts[x])), dim3(GZ_BLOCK), 0, gz_stream,
+                                   part_vecs[x], counts[x], (uint64_t)x * S, gz_slices.nbr24 + 3 * gz_slices.begin[x]);
+                GZ_LAUNCH_CHECK();
+            }
         }
         GZ_CHECK(hipStreamSynchronize(gz_stream));
         gz_device_free(part_vecs[x]);

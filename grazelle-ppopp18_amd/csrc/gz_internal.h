@@ -95,8 +95,11 @@ typedef struct gz_source_slices_t {
     uint64_t slice_vertices;          /* S */
     uint32_t tiers;                   /* T */
     uint32_t nslices;                 /* 8T + 1 */
-    gz_u32x4* nbr;                    /* all slices, concatenated, as packed vectors: neighbour ids */
-    uint32_t* shared;                 /* ... and shared vertices (local ranks) */
+    /* packed vectors (gz_walk.h): the hot slices' vectors [0, begin[nslices-1])
+     * as 24-bit window offsets, 12 bytes each, the last slice's as 32-bit ids */
+    uint32_t* nbr24;                  /* 3 words per hot vector */
+    gz_u32x4* nbr;                    /* last slice: vector begin[nslices-1] + j at nbr[j] */
+    uint32_t* shared;                 /* every vector's shared vertex (local rank) */
     uint64_t begin[GZ_MAX_SLICES + 1]; /* first vector of each slice */
     uint32_t* mask;                   /* [V] bit x: the vertex has vectors in slice x */
     /* Inside slice x, destinations are numbered densely (local rank: the

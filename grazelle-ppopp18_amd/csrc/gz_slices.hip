@@ -12,9 +12,9 @@
  * the cold slice, so each XCD gathers from one S-vertex window at a time.
  *
  * Each slice is an ordinary pull list whose destinations are numbered
- * densely inside the slice, stored packed (32-bit neighbour ids, the
- * destinations implied by flags: 16 bytes per vector instead of 32), walked
- * exactly as in gz_pr.hip
+ * densely inside the slice, stored packed (24-bit window offsets in the
+ * window slices, 32-bit ids in the last; destinations implied by flags: 12
+ * or 16 bytes per vector instead of 32), walked exactly as in gz_pr.hip
  * into its own accumulator run (complete runs stored directly, the chunk's last run into
  * its merge record).  The merge then runs per slice
  * (edge_pull_op_merge_with_merge_buffer's rule, phases.c:41-70), and the
@@ -146,6 +146,7 @@ extern "C" int gz_slices_set_hot_cache(uint32_t vertices)
 
 extern "C" void gz_source_slices_release(void)
 {
+    gz_device_free(gz_slices.nbr24);
     gz_device_free(gz_slices.nbr);
     gz_device_free(gz_slices.shared);
     gz_device_free(gz_slices.mask);
@@ -176,19 +177,36 @@ extern "C" int gz_graph_source_slices_download(gz_vec_t* vectors, uint32_t* mask
         /* back to Vector-Sparse form: the packed copy keeps the neighbour ids
          * and the shared vertex, which is all a vector holds */
         const uint64_t n = gz_slices.begin[gz_slices.nslices];
-        uint32_t* nbr = (uint32_t*)malloc((n ? n : 1) * 4 * sizeof(uint32_t));
+        const uint64_t hot = gz_slices.begin[gz_slices.nslices - 1];
+        uint32_t* w24 = (uint32_t*)malloc((hot ? 3 * hot : 1) * sizeof(uint32_t));
+        uint32_t* nbr = (uint32_t*)malloc((n - hot ? n - hot : 1) * 4 * sizeof(uint32_t));
         uint32_t* shared = (uint32_t*)malloc((n ? n : 1) * sizeof(uint32_t));
-        gz_copy_to_host(nbr, gz_slices.nbr, n * 4 * sizeof(uint32_t));
+        gz_copy_to_host(w24, gz_slices.nbr24, 3 * hot * sizeof(uint32_t));
+        gz_copy_to_host(nbr, gz_slices.nbr, (n - hot) * 4 * sizeof(uint32_t));
         gz_copy_to_host(shared, gz_slices.shared, n * sizeof(uint32_t));
+        uint32_t x = 0;
         for (uint64_t j = 0; j < n; ++j) {
+            while (j >= gz_slices.begin[x + 1]) ++x;
             uint64_t ids[4];
             unsigned count = 0;
-            while (count < 4 && (count == 0 || nbr[4 * j + count] != GZ_PACKED_NONE)) {
-                ids[count] = nbr[4 * j + count] & (count == 0 ? 0x7fffffffu : 0xffffffffu);
-                ++count;
+            if (j < hot) {
+                const uint32_t* q = w24 + 3 * j;
+                const uint32_t f[4] = {q[0] & 0x7fffffu, (q[0] >> 24) | ((q[1] & 0xffffu) << 8),
+                                       (q[1] >> 16) | ((q[2] & 0xffu) << 16), q[2] >> 8};
+                while (count < 4 && (count == 0 || f[count] != 0xffffffu)) {
+                    ids[count] = (uint64_t)x * gz_slices.slice_vertices + f[count];
+                    ++count;
+                }
+            } else {
+                const uint32_t* q = nbr + 4 * (j - hot);
+                while (count < 4 && (count == 0 || q[count] != GZ_PACKED_NONE)) {
+                    ids[count] = q[count] & (count == 0 ? 0x7fffffffu : 0xffffffffu);
+                    ++count;
+                }
             }
             gz_vec_encode(&vectors[j], shared[j], ids, count);
         }
+        free(w24);
         free(nbr);
         free(shared);
     }
@@ -236,9 +254,37 @@ struct gz_slice_bounds {
     uint64_t rec[GZ_MAX_SLICES + 1];
 };
 
+/* The packed slices as kernel arguments: the window slices in 24-bit form,
+ * the last slice in 32-bit form. */
+struct gz_packed_lists {
+    const uint32_t* nbr24;
+    const gz_u32x4* nbr;
+    const uint32_t* shared;
+    uint64_t cold_first;
+    uint64_t slice_vertices;
+    uint32_t cold_slice;
+    __device__ __forceinline__ gz_packed_list cold() const { return gz_packed_list{{shared}, nbr, cold_first}; }
+    __device__ __forceinline__ gz_packed24_list window(uint32_t sl) const
+    {
+        return gz_packed24_list{{shared}, nbr24, (uint32_t)((uint64_t)sl * slice_vertices)};
+    }
+};
+
+static gz_packed_lists gz_slices_lists()
+{
+    gz_packed_lists p;
+    p.nbr24 = gz_slices.nbr24;
+    p.nbr = gz_slices.nbr;
+    p.shared = gz_slices.shared;
+    p.cold_first = gz_slices.begin[gz_slices.nslices - 1];
+    p.slice_vertices = gz_slices.slice_vertices;
+    p.cold_slice = gz_slices.nslices - 1;
+    return p;
+}
+
 /* Walk one chunk into slice accumulators `acc`; returns the wave's share of S. */
-template <int VPL, bool kStream>
-__device__ __forceinline__ double gz_sliced_walk(const gz_packed_list& list, uint64_t b, uint64_t e,
+template <int VPL, bool kStream, typename L>
+__device__ __forceinline__ double gz_sliced_walk(const L& list, uint64_t b, uint64_t e,
                                                  const double* __restrict__ props, double* __restrict__ acc,
                                                  mergeaccum_t* __restrict__ record, uint32_t& edges)
 {
@@ -260,7 +306,7 @@ __device__ __forceinline__ double gz_sliced_walk(const gz_packed_list& list, uin
 /* One wavefront per chunk slot.  kColdStream: the cold slice's gathers, which
  * mostly miss, are nontemporal. */
 template <int VPL, bool kColdStream>
-__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr_sliced(gz_packed_list list,
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr_sliced(gz_packed_lists lists,
                                                                      const gz_slice_chunk_t* __restrict__ table,
                                                                      const double* __restrict__ props,
                                                                      double* __restrict__ parts, gz_slice_offsets part_base,
@@ -287,9 +333,12 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr_sliced(gz_packed_l
     const uint64_t b = c.begin, e = c.begin + c.count;
     double* acc = parts + part_base.off[x];
     uint32_t edges = 0;
-    const double S = (kColdStream && x == cold_slice)
-                         ? gz_sliced_walk<VPL, true>(list, b, e, props, acc, merge + rec, edges)
-                         : gz_sliced_walk<VPL, false>(list, b, e, props, acc, merge + rec, edges);
+    double S;
+    if (x == cold_slice)
+        S = kColdStream ? gz_sliced_walk<VPL, true>(lists.cold(), b, e, props, acc, merge + rec, edges)
+                        : gz_sliced_walk<VPL, false>(lists.cold(), b, e, props, acc, merge + rec, edges);
+    else
+        S = gz_sliced_walk<VPL, false>(lists.window(x), b, e, props, acc, merge + rec, edges);
     if (lane == 0) partials[slot] = S;
     if (collect) {
         const uint64_t tot_edges = gz_wave_sum_u64(edges);
@@ -396,7 +445,7 @@ struct gz_pr_pull_op_hot {
 
 template <int HOT, int VPL>
 __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_simd) void gz_k_edge_pull_pr_sliced_hot(
-    gz_packed_list list, const gz_slice_chunk_t* __restrict__ table, gz_xcd_lens lens,
+    gz_packed_lists lists, const gz_slice_chunk_t* __restrict__ table, gz_xcd_lens lens,
     const double* __restrict__ props, double* __restrict__ parts, gz_slice_offsets part_base,
     mergeaccum_t* __restrict__ merge, double* __restrict__ partials, uint64_t* __restrict__ stats, uint32_t collect,
     uint32_t* __restrict__ counters, uint64_t slice_vertices, uint64_t hot_count, uint32_t cold_slice,
@@ -425,14 +474,23 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
         double sum;
         gz_global_f64* gprops = (gz_global_f64*)props;
         gz_lds_f64* lds = (gz_lds_f64*)hot;
-        if (cold_stream && sl == cold_slice) {
-            gz_pr_pull_op_hot<true> op{gprops, lds, 0, 0, parts + part_base.off[sl], 0.0};
-            gz_walk_list<VPL, double, true>(list, c.begin, c.begin + c.count, op, first_d, carry_d, carry_v, edges);
-            sum = op.sum;
+        if (sl == cold_slice) {
+            if (cold_stream) {
+                gz_pr_pull_op_hot<true> op{gprops, lds, 0, 0, parts + part_base.off[sl], 0.0};
+                gz_walk_list<VPL, double, true>(lists.cold(), c.begin, c.begin + c.count, op, first_d, carry_d,
+                                                carry_v, edges);
+                sum = op.sum;
+            } else {
+                gz_pr_pull_op_hot<false> op{gprops, lds, 0, 0, parts + part_base.off[sl], 0.0};
+                gz_walk_list<VPL, double, true>(lists.cold(), c.begin, c.begin + c.count, op, first_d, carry_d,
+                                                carry_v, edges);
+                sum = op.sum;
+            }
         } else {
             gz_pr_pull_op_hot<false> op{gprops, lds, (sl == x) ? lo : 0, (sl == x) ? hot_count : 0,
                                         parts + part_base.off[sl], 0.0};
-            gz_walk_list<VPL, double, true>(list, c.begin, c.begin + c.count, op, first_d, carry_d, carry_v, edges);
+            gz_walk_list<VPL, double, true>(lists.window(sl), c.begin, c.begin + c.count, op, first_d, carry_d,
+                                            carry_v, edges);
             sum = op.sum;
         }
         const double S = gz_wave_sum(sum);
@@ -467,8 +525,7 @@ static void gz_launch_hot_vpl(const gz_xcd_lens& lens, uint64_t hot_count)
     if (per_xcd == 0) per_xcd = 1;
     const uint64_t blocks = (uint64_t)GZ_XCDS * per_xcd * (uint64_t)per_cu;
     hipLaunchKernelGGL((gz_k_edge_pull_pr_sliced_hot<HOT, VPL>), dim3((unsigned)blocks), dim3(WG), 0, gz_stream,
-                       gz_packed_list{gz_slices.nbr, gz_slices.shared}, gz_slices.table, lens,
-                       (const double*)gz_graph.vertex_props, gz_slices.parts,
+                       gz_slices_lists(), gz_slices.table, lens, (const double*)gz_graph.vertex_props, gz_slices.parts,
                        gz_slices.part_base, gz_slices.merge, (double*)gz_scratch.partials, gz_scratch.stats,
                        gz_phase_config.collect_stats, gz_slices.counters, gz_slices.slice_vertices, hot_count,
                        gz_slices.nslices - 1, gz_slices_cold_stream);
@@ -550,7 +607,7 @@ struct gz_cc_pull_op_hot {
 
 template <int HOT>
 __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_simd) void gz_k_edge_pull_cc_sliced_hot(
-    gz_packed_list list, const gz_slice_chunk_t* __restrict__ table, gz_xcd_lens lens, uint64_t* __restrict__ labels,
+    gz_packed_lists lists, const gz_slice_chunk_t* __restrict__ table, gz_xcd_lens lens, uint64_t* __restrict__ labels,
     uint64_t* __restrict__ next_frontier, const double* __restrict__ outdeg, uint32_t stat_mode,
     const uint32_t* __restrict__ gid, gz_slice_offsets part_base, uint64_t* __restrict__ partials,
     uint64_t* __restrict__ stats, uint32_t collect, uint32_t* __restrict__ counters, uint64_t slice_vertices,
@@ -586,7 +643,12 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
                              stat_mode,
                              0};
         uint64_t first_d, carry_d, carry_m;
-        gz_walk_list<1, uint64_t, true>(list, c.begin, c.begin + c.count, op, first_d, carry_d, carry_m, edges);
+        if (sl == lists.cold_slice)
+            gz_walk_list<1, uint64_t, true>(lists.cold(), c.begin, c.begin + c.count, op, first_d, carry_d, carry_m,
+                                            edges);
+        else
+            gz_walk_list<1, uint64_t, true>(lists.window(sl), c.begin, c.begin + c.count, op, first_d, carry_d,
+                                            carry_m, edges);
         if (lane == 0) op.store(carry_d, carry_m);
         vote += op.vote;
         vectors += c.count;
@@ -618,7 +680,7 @@ template <int HOT>
 static void gz_launch_cc_hot(const gz_xcd_lens& lens, uint64_t hot_count, uint64_t blocks)
 {
     hipLaunchKernelGGL(gz_k_edge_pull_cc_sliced_hot<HOT>, dim3((unsigned)blocks), dim3(gz_hot_wg<HOT>::value), 0,
-                       gz_stream, gz_packed_list{gz_slices.nbr, gz_slices.shared}, gz_slices.table, lens,
+                       gz_stream, gz_slices_lists(), gz_slices.table, lens,
                        (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
                        gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, gz_slices.gid, gz_slices.part_base,
                        (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats,
@@ -703,8 +765,7 @@ void gz_pull_pr_sliced(const gz_vec_t* edge_list, uint64_t edge_list_count)
         const uint32_t cold_slice = gz_slices.nslices - 1;
         gz_ktimer_begin(GZ_K_PULL_PR);
 #define GZ_SLICED_ARGS                                                                                          \
-    dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, gz_packed_list{gz_slices.nbr, gz_slices.shared},         \
-        gz_slices.table,                                                                                          \
+    dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, gz_slices_lists(), gz_slices.table,                     \
         (const double*)gz_graph.vertex_props, gz_slices.parts, gz_slices.part_base, cold_slice, gz_slices.merge,  \
         (double*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats
         const bool cs = gz_slices_cold_stream != 0;

@@ -24,7 +24,8 @@
  * Op supplies: T identity(); T combine(T, T); T gather(const vec&, bool
  * in_range, uint32_t& edges) (identity when !in_range) for the list's vector
  * type; void store(uint64_t d, T value) for a complete run.  The list is a
- * Vector-Sparse list (gz_vs_list) or a packed copy (gz_packed_list).
+ * Vector-Sparse list (gz_vs_list) or a packed copy (gz_packed_list,
+ * gz_packed24_list).
  */
 #ifndef GZ_WALK_H
 #define GZ_WALK_H
@@ -175,39 +176,32 @@ struct gz_vs_list {
     }
 };
 
-/* A packed list (gz_packed_vec, gz_internal.h) whose destinations are the
- * dense ranks 0, 1, 2, ... in order: bit 31 of a vector's first neighbour id
- * flags the first vector of a destination, so a vector's destination is the
+/* Packed lists (gz_packed_vec, gz_internal.h) whose destinations are the
+ * dense ranks 0, 1, 2, ... in order: a flag on a vector's first neighbour
+ * marks the first vector of a destination, so a vector's destination is the
  * chunk's first destination plus the flags after the chunk's first vector up
  * to it.  `shared` holds every vector's destination but is read only at chunk
- * starts. */
+ * starts.  Vector i of the concatenated slices is either 16 bytes (four
+ * 32-bit ids, flag in bit 31 of the first) or, inside a source window, 12
+ * bytes (four 24-bit offsets from the window's first vertex, flag in bit 23 of
+ * the first); the flagged first lane always holds an edge. */
 #define GZ_PACKED_FIRST 0x80000000u
-struct gz_packed_list {
-    const gz_u32x4* __restrict__ nbr;
+#define GZ_PACKED24_NONE 0xffffffu
+#define GZ_PACKED24_FIRST 0x800000u
+
+struct gz_flagged_dsts {
     const uint32_t* __restrict__ shared;
-    typedef gz_packed_vec vec;
     struct cursor {
         uint64_t d;
     };
-    __device__ __forceinline__ vec load(uint64_t i) const
-    {
-        const gz_u32x4 a = __builtin_nontemporal_load(nbr + i);
-        vec v;
-        v.s0 = a.x & ~GZ_PACKED_FIRST; /* the first lane always holds an edge */
-        v.s1 = a.y;
-        v.s2 = a.z;
-        v.s3 = a.w;
-        v.d = a.x >> 31; /* the flag; the destination comes from dsts() */
-        return v;
-    }
     __device__ __forceinline__ uint64_t first_dst(uint64_t b, cursor& c) const
     {
         c.d = shared[b];
         return c.d;
     }
     template <int VPL>
-    __device__ __forceinline__ void dsts(const vec (&v)[VPL], uint64_t my_first, uint64_t b, uint64_t e, cursor& c,
-                                         uint64_t (&d)[VPL]) const
+    __device__ __forceinline__ void dsts(const gz_packed_vec (&v)[VPL], uint64_t my_first, uint64_t b, uint64_t e,
+                                         cursor& c, uint64_t (&d)[VPL]) const
     {
         /* flags counted: in range and not the chunk's first vector */
         uint32_t f[VPL], t = 0;
@@ -234,6 +228,49 @@ struct gz_packed_list {
             d[k] = x;
         }
         c.d += total;
+    }
+};
+
+/* 16-byte vectors; nbr[i - first] holds vector i. */
+struct gz_packed_list : gz_flagged_dsts {
+    const gz_u32x4* __restrict__ nbr;
+    uint64_t first;
+    typedef gz_packed_vec vec;
+    __device__ __forceinline__ vec load(uint64_t i) const
+    {
+        const gz_u32x4 a = __builtin_nontemporal_load(nbr + (i - first));
+        vec v;
+        v.s0 = a.x & ~GZ_PACKED_FIRST;
+        v.s1 = a.y;
+        v.s2 = a.z;
+        v.s3 = a.w;
+        v.d = a.x >> 31; /* the flag; the destination comes from dsts() */
+        return v;
+    }
+};
+
+/* 12-byte vectors of one window: words w[3i .. 3i+2] hold the offsets
+ * f0 | f1 << 24 | f2 << 48 | f3 << 72 (little-endian, 96 bits). */
+struct gz_packed24_list : gz_flagged_dsts {
+    const uint32_t* __restrict__ w;
+    uint32_t window; /* the window's first vertex */
+    typedef gz_packed_vec vec;
+    __device__ __forceinline__ vec load(uint64_t i) const
+    {
+        const uint32_t w0 = __builtin_nontemporal_load(w + 3 * i);
+        const uint32_t w1 = __builtin_nontemporal_load(w + 3 * i + 1);
+        const uint32_t w2 = __builtin_nontemporal_load(w + 3 * i + 2);
+        const uint32_t f0 = w0 & 0xffffffu;
+        const uint32_t f1 = (w0 >> 24) | ((w1 & 0xffffu) << 8);
+        const uint32_t f2 = (w1 >> 16) | ((w2 & 0xffu) << 16);
+        const uint32_t f3 = w2 >> 8;
+        vec v;
+        v.s0 = window + (f0 & ~GZ_PACKED24_FIRST);
+        v.s1 = (f1 == GZ_PACKED24_NONE) ? GZ_PACKED_NONE : window + f1;
+        v.s2 = (f2 == GZ_PACKED24_NONE) ? GZ_PACKED_NONE : window + f2;
+        v.s3 = (f3 == GZ_PACKED24_NONE) ? GZ_PACKED_NONE : window + f3;
+        v.d = f0 >> 23;
+        return v;
     }
 };
 

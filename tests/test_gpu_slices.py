@@ -226,3 +226,28 @@ def test_sliced_cc_directed_and_renumbered(gz, orc, device, hot):
         assert np.array_equal(gz.cc_labels(), want)
     finally:
         gz.slices_set_hot_cache(0)
+
+
+def test_sliced_matches_plain_at_scale_20(gz, device):
+    # size-independent properties at a size the oracle would be slow on:
+    # sliced PageRank equals the plain pull to rounding, ranks sum to 1, and
+    # sliced CC labels equal the plain CC labels
+    scale = 20
+    gz.load_rmat_device(scale, 16 << scale, 77, relabel=True, slices_log2=15)
+    gz.run(gz.options("pr", iterations=10))
+    plain = gz.pr_ranks()
+    gz.set_chunk_vectors(2048)
+    gz.slices_set_hot_cache(20480)
+    try:
+        gz.run(gz.options("pr", iterations=10, pull_variant="source_sliced"))
+        sliced = gz.pr_ranks()
+        gz.run(gz.options("cc"))
+        cc_plain = gz.cc_labels()
+        gz.run(gz.options("cc", pull_variant="source_sliced"))
+        cc_sliced = gz.cc_labels()
+    finally:
+        gz.slices_set_hot_cache(0)
+        gz.set_chunk_vectors(512)
+    assert _rel(sliced, plain) <= PR_TOL
+    assert abs(sliced.sum() - 1.0) < 1e-9
+    assert np.array_equal(cc_sliced, cc_plain)
