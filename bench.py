@@ -9,9 +9,10 @@ are reported in original ids).  FP64 ranks, damping 0.85.  One step = one
 PageRank iteration (pull sweep, merge, sink-mass reduction, vertex phase);
 value = E / (time per iteration), GTEPS per iteration (the reference's rate,
 main.c:127-129, per iteration).  Inputs are resident in HBM before timing.
-The pull runs over the source-sliced layout (eight 2^19-vertex source windows,
-one per XCD, the hottest 20480 sources of each window in LDS, chunks of 2048
-vectors); --plain-pull runs the scheduler-aware pull over the ordinary list.
+The pull runs over the source-sliced layout (two tiers of eight 2^20-vertex
+source windows, one window per XCD and tier, the hottest 20480 sources of each
+tier-0 window in LDS, chunks of 2048 vectors); --plain-pull runs the
+scheduler-aware pull over the ordinary list.
 
 roofline: the pull sweep kernel's algorithmic bytes per launch, 32*Nvp
 (vectors of the ordinary pull list) + 8*E (one FP64 gather per edge) + 8*V
@@ -52,8 +53,9 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "GTEPS/iter + % HBM peak, 1 MI355X; learned-vs-fixed switch & batched-vs-seq"
 FULL = "full-rmat26-fold41.7m"
 DAMPING = 0.85
-SLICES_LOG2 = 19      # source window of 2^19 vertices per XCD
-HOT_CACHE = 20480     # hottest sources of each window kept in LDS (160 KiB)
+SLICES_LOG2 = 20      # source windows of 2^20 vertices, one per XCD and tier
+SLICE_TIERS = 2       # two tiers of eight windows: the top 16.8 M sources
+HOT_CACHE = 20480     # hottest sources of each tier-0 window kept in LDS (160 KiB)
 SLICED_CHUNK = 2048   # vectors per chunk of the sliced pull
 
 
@@ -77,7 +79,8 @@ def pull_bytes(gz) -> int:
 def headline(gz, args) -> dict:
     t0 = time.time()
     sliced = not args.plain_pull
-    gz.load_workload_device(FULL, relabel=not args.no_relabel, slices_log2=SLICES_LOG2 if sliced else 0)
+    gz.load_workload_device(FULL, relabel=not args.no_relabel, slices_log2=SLICES_LOG2 if sliced else 0,
+                            slice_tiers=SLICE_TIERS)
     gz.lib.gz_synchronize()
     V, E, nvp = gz.num_vertices(), gz.num_edges(), gz.pull_vector_count()
     log(f"[headline] {FULL}: V={V} E={E} Nvp={nvp} ingress {time.time() - t0:.2f}s")
@@ -346,7 +349,8 @@ def main() -> int:
         "config": {"workload": FULL + " PageRank pull (FP64, damping 0.85)", "num_vertices": h["V"],
                    "num_edges": h["E"], "pull_vectors": h["nvp"],
                    "ingress": "device, renumbered by out-degree" if not args.no_relabel else "device",
-                   "pull": (f"source-sliced (S=2^{SLICES_LOG2}, LDS cache {HOT_CACHE}, chunk {SLICED_CHUNK})"
+                   "pull": (f"source-sliced (S=2^{SLICES_LOG2}, {SLICE_TIERS} tiers, LDS cache {HOT_CACHE}, "
+                            f"chunk {SLICED_CHUNK})"
                             if h["sliced"] else "scheduler-aware, ordinary pull list"),
                    "device": dev},
         "roofline": h["roofline"],

@@ -236,24 +236,32 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr_sliced(
     const uint64_t last = (end + GZ_WAVE - 1) / GZ_WAVE;
     const uint64_t waves = (uint64_t)gridDim.x * GZ_WAVES_PER_BLOCK;
     uint64_t blk = start / GZ_WAVE + (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + threadIdx.x / GZ_WAVE;
-    /* the next block's mask is loaded one iteration ahead, so that the slice
-     * sums (whose addresses depend on it) are the only wait of an iteration */
+    /* the next block's mask and rank bases are loaded one iteration ahead, so
+     * that the slice sums (whose addresses depend on them) are the only wait
+     * of an iteration */
     uint32_t m_next = 0;
+    uint32_t rb_next[NS];
     if (blk < last) {
         const uint64_t v0 = blk * GZ_WAVE + (uint64_t)lane;
         m_next = (v0 < num_vertices) ? mask[v0] : 0u;
+#pragma unroll
+        for (int x = 0; x < NS; ++x) rb_next[x] = rank_base[blk * (uint64_t)NS + x];
     }
     for (; blk < last; blk += waves) {
         const uint64_t v = blk * GZ_WAVE + (uint64_t)lane;
         const uint32_t m = m_next;
+        uint32_t rb[NS];
+#pragma unroll
+        for (int x = 0; x < NS; ++x) rb[x] = rb_next[x];
         const uint64_t nb = blk + waves;
         if (nb < last) {
             const uint64_t vn = nb * GZ_WAVE + (uint64_t)lane;
             m_next = (vn < num_vertices) ? mask[vn] : 0u;
+#pragma unroll
+            for (int x = 0; x < NS; ++x) rb_next[x] = rank_base[nb * (uint64_t)NS + x];
         }
         const bool mine = v >= start && v < end;
         const double od = mine ? outdeg[v] : 1.0;
-        const uint32_t* rb = rank_base + blk * (uint64_t)NS;
         double p[NS];
 #pragma unroll
         for (int x = 0; x < NS; ++x) {
