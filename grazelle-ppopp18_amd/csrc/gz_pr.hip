@@ -235,7 +235,10 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr_sliced(
     const uint64_t below = (1ull << lane) - 1ull;
     const uint64_t last = (end + GZ_WAVE - 1) / GZ_WAVE;
     const uint64_t waves = (uint64_t)gridDim.x * GZ_WAVES_PER_BLOCK;
-    uint64_t blk = start / GZ_WAVE + (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + threadIdx.x / GZ_WAVE;
+    /* the wavefront's index, made visibly uniform so that the block's rank
+     * bases are scalar loads and the slice-sum addresses scalar bases */
+    const uint32_t wib = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / GZ_WAVE));
+    uint64_t blk = start / GZ_WAVE + (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + wib;
     /* the next block's mask and rank bases are loaded one iteration ahead, so
      * that the slice sums (whose addresses depend on them) are the only wait
      * of an iteration */
@@ -269,8 +272,8 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr_sliced(
             const uint64_t bal = __ballot(has);
             p[x] = 0.0;
             if (bal != 0) { /* wave-uniform: no block vertex in slice x, no load */
-                const uint64_t idx = part_base.off[x] + rb[x] + __popcll(bal & below);
-                if (has) p[x] = parts[idx];
+                const double* run = parts + part_base.off[x] + rb[x];
+                if (has) p[x] = run[(uint32_t)__popcll(bal & below)];
             }
         }
         double a = 0.0;
