@@ -400,45 +400,66 @@ struct gz_xcd_lens {
 typedef __attribute__((address_space(3))) const double gz_lds_f64;
 typedef __attribute__((address_space(1))) const double gz_global_f64;
 
+/* The four sources of one packed vector (ids s[k], GZ_PACKED_NONE for an
+ * empty lane), each read from the LDS copy `hot` when it lies in
+ * [wlo, wlo + wn) and from memory otherwise; `empty` for an empty or
+ * out-of-range lane.  The memory reads are all issued before any LDS read and
+ * land in registers of their own.  (Written as one if/else per source, both
+ * branches share the result register, so each LDS read must wait for the
+ * memory read issued before it and a wavefront holds one gather in flight
+ * instead of four.) */
+template <bool kStream, typename T, typename GP, typename LP>
+__device__ __forceinline__ void gz_fetch4_hot(GP mem, LP hot, uint32_t wlo, uint32_t wn, const gz_packed_vec& v,
+                                              bool in_range, T empty, T (&out)[4], uint32_t& edges)
+{
+    const uint32_t s[4] = {v.s0, v.s1, v.s2, v.s3};
+    bool ok[4], cached[4];
+    uint32_t r[4];
+    T m[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        ok[k] = in_range && s[k] != GZ_PACKED_NONE;
+        r[k] = s[k] - wlo;
+        cached[k] = ok[k] && r[k] < wn;
+        m[k] = empty;
+        edges += (uint32_t)ok[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (ok[k] && !cached[k]) m[k] = kStream ? __builtin_nontemporal_load(mem + s[k]) : mem[s[k]];
+    if (wn != 0) {
+        T h[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) h[k] = hot[cached[k] ? r[k] : 0u];
+        /* every lane reads LDS (source 0 of the window where the source is
+         * not cached): left to itself the compiler sinks each read into a
+         * branch on cached[k] that first waits for the memory reads */
+        asm volatile("" ::"v"(h[0]), "v"(h[1]), "v"(h[2]), "v"(h[3]));
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = cached[k] ? h[k] : m[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) out[k] = m[k];
+    }
+}
+
 template <bool kStream>
 struct gz_pr_pull_op_hot {
     gz_global_f64* __restrict__ props;
     gz_lds_f64* hot;
-    uint64_t wlo, wn;
+    uint32_t wlo, wn;
     double* __restrict__ acc;
     double sum;
     __device__ __forceinline__ double identity() const { return 0.0; }
     __device__ __forceinline__ double combine(double a, double b) const { return a + b; }
-    __device__ __forceinline__ double fetch(uint64_t i) const
-    {
-        const uint64_t r = i - wlo;
-        double g;
-        if (r < wn)
-            g = hot[r];
-        else if (kStream)
-            g = __builtin_nontemporal_load(props + i);
-        else
-            g = props[i];
-        return g;
-    }
+    /* gz_pr_packed_sum with the cache in front */
     __device__ __forceinline__ double gather(const gz_packed_vec& v, bool in_range, uint32_t& edges)
     {
-        /* gz_pr_packed_sum with the cache in front; empty lanes read the
-         * window's first element */
-        const bool v0 = in_range && v.s0 != GZ_PACKED_NONE, v1 = in_range && v.s1 != GZ_PACKED_NONE,
-                   v2 = in_range && v.s2 != GZ_PACKED_NONE, v3 = in_range && v.s3 != GZ_PACKED_NONE;
-        double g0 = fetch(v0 ? v.s0 : wlo);
-        double g1 = fetch(v1 ? v.s1 : wlo);
-        double g2 = fetch(v2 ? v.s2 : wlo);
-        double g3 = fetch(v3 ? v.s3 : wlo);
-        g0 = v0 ? g0 : 0.0;
-        g1 = v1 ? g1 : 0.0;
-        g2 = v2 ? g2 : 0.0;
-        g3 = v3 ? g3 : 0.0;
-        edges += (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2 + (uint32_t)v3;
-        const double g = (g2 + g3) + (g0 + g1);
-        sum += g;
-        return g;
+        double g[4];
+        gz_fetch4_hot<kStream>(props, hot, wlo, wn, v, in_range, 0.0, g, edges);
+        const double t = (g[2] + g[3]) + (g[0] + g[1]);
+        sum += t;
+        return t;
     }
     __device__ __forceinline__ void store(uint64_t d, double s) { acc[d] = s; }
 };
@@ -461,7 +482,9 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
     while (true) {
         uint32_t idx = 0;
         if (lane == 0) idx = atomicAdd(&counters[x * GZ_COUNTER_STRIDE], 1u);
-        idx = (uint32_t)__shfl((int)idx, 0, GZ_WAVE);
+        /* lane 0's ticket in every lane, and known uniform, so that the chunk's
+         * table entry and bounds are scalar */
+        idx = (uint32_t)__builtin_amdgcn_readfirstlane((int)idx);
         if ((uint64_t)idx >= n) break;
         const uint64_t slot =
             ((uint64_t)GZ_XCDS * (idx / GZ_WAVES_PER_BLOCK) + x) * GZ_WAVES_PER_BLOCK + idx % GZ_WAVES_PER_BLOCK;
@@ -477,19 +500,19 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
         if (sl == cold_slice) {
             if (cold_stream) {
                 gz_pr_pull_op_hot<true> op{gprops, lds, 0, 0, parts + part_base.off[sl], 0.0};
-                gz_walk_list<VPL, double, true>(lists.cold(), c.begin, c.begin + c.count, op, first_d, carry_d,
+                gz_walk_list<VPL, double, true, true>(lists.cold(), c.begin, c.begin + c.count, op, first_d, carry_d,
                                                 carry_v, edges);
                 sum = op.sum;
             } else {
                 gz_pr_pull_op_hot<false> op{gprops, lds, 0, 0, parts + part_base.off[sl], 0.0};
-                gz_walk_list<VPL, double, true>(lists.cold(), c.begin, c.begin + c.count, op, first_d, carry_d,
+                gz_walk_list<VPL, double, true, true>(lists.cold(), c.begin, c.begin + c.count, op, first_d, carry_d,
                                                 carry_v, edges);
                 sum = op.sum;
             }
         } else {
-            gz_pr_pull_op_hot<false> op{gprops, lds, (sl == x) ? lo : 0, (sl == x) ? hot_count : 0,
+            gz_pr_pull_op_hot<false> op{gprops, lds, (sl == x) ? (uint32_t)lo : 0u, (sl == x) ? (uint32_t)hot_count : 0u,
                                         parts + part_base.off[sl], 0.0};
-            gz_walk_list<VPL, double, true>(lists.window(sl), c.begin, c.begin + c.count, op, first_d, carry_d,
+            gz_walk_list<VPL, double, true, true>(lists.window(sl), c.begin, c.begin + c.count, op, first_d, carry_d,
                                             carry_v, edges);
             sum = op.sum;
         }
@@ -559,7 +582,7 @@ typedef __attribute__((address_space(1))) const uint64_t gz_global_u64;
 struct gz_cc_pull_op_hot {
     gz_global_u64* __restrict__ in;      /* labels, read */
     gz_lds_u64* hot;
-    uint64_t wlo, wn;
+    uint32_t wlo, wn;
     uint64_t* __restrict__ labels;       /* labels, atomic min */
     uint64_t* __restrict__ next_frontier;
     const double* __restrict__ outdeg;
@@ -568,29 +591,12 @@ struct gz_cc_pull_op_hot {
     uint64_t vote;
     __device__ __forceinline__ uint64_t identity() const { return ~0ull; }
     __device__ __forceinline__ uint64_t combine(uint64_t a, uint64_t b) const { return a < b ? a : b; }
-    __device__ __forceinline__ uint64_t fetch(uint64_t i) const
-    {
-        const uint64_t r = i - wlo;
-        uint64_t x;
-        if (r < wn)
-            x = hot[r];
-        else
-            x = in[i];
-        return x;
-    }
     __device__ __forceinline__ uint64_t gather(const gz_packed_vec& v, bool in_range, uint32_t& edges) const
     {
-        const bool v0 = in_range && v.s0 != GZ_PACKED_NONE, v1 = in_range && v.s1 != GZ_PACKED_NONE,
-                   v2 = in_range && v.s2 != GZ_PACKED_NONE, v3 = in_range && v.s3 != GZ_PACKED_NONE;
-        const uint64_t x0 = fetch(v0 ? v.s0 : wlo), x1 = fetch(v1 ? v.s1 : wlo), x2 = fetch(v2 ? v.s2 : wlo),
-                       x3 = fetch(v3 ? v.s3 : wlo);
-        uint64_t m = ~0ull;
-        m = (v0 && x0 < m) ? x0 : m;
-        m = (v1 && x1 < m) ? x1 : m;
-        m = (v2 && x2 < m) ? x2 : m;
-        m = (v3 && x3 < m) ? x3 : m;
-        edges += (uint32_t)v0 + (uint32_t)v1 + (uint32_t)v2 + (uint32_t)v3;
-        return m;
+        uint64_t x[4];
+        gz_fetch4_hot<false>(in, hot, wlo, wn, v, in_range, ~(uint64_t)0, x, edges);
+        const uint64_t a = x[0] < x[1] ? x[0] : x[1], b = x[2] < x[3] ? x[2] : x[3];
+        return a < b ? a : b;
     }
     __device__ __forceinline__ void store(uint64_t d, uint64_t m)
     {
@@ -626,7 +632,9 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
     while (true) {
         uint32_t idx = 0;
         if (lane == 0) idx = atomicAdd(&counters[x * GZ_COUNTER_STRIDE], 1u);
-        idx = (uint32_t)__shfl((int)idx, 0, GZ_WAVE);
+        /* lane 0's ticket in every lane, and known uniform, so that the chunk's
+         * table entry and bounds are scalar */
+        idx = (uint32_t)__builtin_amdgcn_readfirstlane((int)idx);
         if ((uint64_t)idx >= n) break;
         const uint64_t slot =
             ((uint64_t)GZ_XCDS * (idx / GZ_WAVES_PER_BLOCK) + x) * GZ_WAVES_PER_BLOCK + idx % GZ_WAVES_PER_BLOCK;
@@ -634,8 +642,8 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
         const uint32_t sl = c.record >> GZ_SLICE_RECORD_BITS;
         gz_cc_pull_op_hot op{(gz_global_u64*)labels,
                              (gz_lds_u64*)hot,
-                             (sl == x) ? lo : 0,
-                             (sl == x) ? hot_count : 0,
+                             (sl == x) ? (uint32_t)lo : 0u,
+                             (sl == x) ? (uint32_t)hot_count : 0u,
                              labels,
                              next_frontier,
                              outdeg,
@@ -644,10 +652,10 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
                              0};
         uint64_t first_d, carry_d, carry_m;
         if (sl == lists.cold_slice)
-            gz_walk_list<1, uint64_t, true>(lists.cold(), c.begin, c.begin + c.count, op, first_d, carry_d, carry_m,
+            gz_walk_list<1, uint64_t, true, true>(lists.cold(), c.begin, c.begin + c.count, op, first_d, carry_d, carry_m,
                                             edges);
         else
-            gz_walk_list<1, uint64_t, true>(lists.window(sl), c.begin, c.begin + c.count, op, first_d, carry_d,
+            gz_walk_list<1, uint64_t, true, true>(lists.window(sl), c.begin, c.begin + c.count, op, first_d, carry_d,
                                             carry_m, edges);
         if (lane == 0) op.store(carry_d, carry_m);
         vote += op.vote;

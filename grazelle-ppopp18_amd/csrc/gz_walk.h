@@ -293,8 +293,11 @@ __device__ __forceinline__ void gz_load_slice(gz_slice<L, VPL>& s, const L& list
 }
 
 /* One step of the walk: reduce the slice `cur` (vectors base + lane*VPL ...)
- * and prefetch the following step's slice into `nxt`. */
-template <int VPL, typename T, bool kDpp, typename L, typename Op>
+ * and prefetch the following step's slice into `nxt`.  kPrefetchFirst issues
+ * the prefetch before the gathers, for operators whose gathers end in a wait
+ * for every load in flight (a select between memory and LDS values): the
+ * prefetch then overlaps the gathers instead of starting after them. */
+template <int VPL, typename T, bool kDpp, bool kPrefetchFirst = false, typename L, typename Op>
 __device__ __forceinline__ void gz_walk_step(const L& list, uint64_t b, uint64_t e, uint64_t base, Op& op,
                                              const gz_slice<L, VPL>& cur, gz_slice<L, VPL>& nxt,
                                              typename L::cursor& cursor, uint64_t& carry_d, T& carry_v,
@@ -306,9 +309,10 @@ __device__ __forceinline__ void gz_walk_step(const L& list, uint64_t b, uint64_t
     /* issue every gather of the step first, then the next slice's loads:
      * waiting for the gathers then leaves the prefetch in flight */
     T gv[VPL];
+    if constexpr (kPrefetchFirst) gz_load_slice<L, VPL>(nxt, list, my_first + step, e);
 #pragma unroll
     for (int k = 0; k < VPL; ++k) gv[k] = op.gather(cur.v[k], my_first + (uint64_t)k < e, edges);
-    gz_load_slice<L, VPL>(nxt, list, my_first + step, e);
+    if constexpr (!kPrefetchFirst) gz_load_slice<L, VPL>(nxt, list, my_first + step, e);
     uint64_t dd[VPL];
     list.template dsts<VPL>(cur.v, my_first, b, e, cursor, dd);
     /* 1. lane-local runs */
@@ -385,7 +389,7 @@ __device__ __forceinline__ void gz_walk_step(const L& list, uint64_t b, uint64_t
     carry_v = gz_readlane_t<T>(S, last);
 }
 
-template <int VPL, typename T, bool kDpp, typename L, typename Op>
+template <int VPL, typename T, bool kDpp, bool kPrefetchFirst = false, typename L, typename Op>
 __device__ __forceinline__ void gz_walk_list(const L& list, uint64_t b, uint64_t e, Op& op, uint64_t& first_d,
                                              uint64_t& carry_d, T& carry_v, uint32_t& edges)
 {
@@ -401,10 +405,10 @@ __device__ __forceinline__ void gz_walk_list(const L& list, uint64_t b, uint64_t
     gz_load_slice<L, VPL>(sa, list, b + (uint64_t)lane * VPL, e);
     uint64_t base = b;
     while (true) {
-        gz_walk_step<VPL, T, kDpp>(list, b, e, base, op, sa, sb, cursor, carry_d, carry_v, edges);
+        gz_walk_step<VPL, T, kDpp, kPrefetchFirst>(list, b, e, base, op, sa, sb, cursor, carry_d, carry_v, edges);
         base += step;
         if (base >= e) break;
-        gz_walk_step<VPL, T, kDpp>(list, b, e, base, op, sb, sa, cursor, carry_d, carry_v, edges);
+        gz_walk_step<VPL, T, kDpp, kPrefetchFirst>(list, b, e, base, op, sb, sa, cursor, carry_d, carry_v, edges);
         base += step;
         if (base >= e) break;
     }
