@@ -55,7 +55,8 @@ FULL = "full-rmat26-fold41.7m"
 DAMPING = 0.85
 SLICES_LOG2 = 20      # source windows of 2^20 vertices, one per XCD and tier
 SLICE_TIERS = 2       # two tiers of eight windows: the top 16.8 M sources
-HOT_CACHE = 20480     # hottest sources of each tier-0 window kept in LDS (160 KiB)
+SLICE_BLOCKS = 5      # LDS blocks of 20480 sources carved from each tier-0 window
+HOT_CACHE = 20480     # hottest sources of each tier-0 window (and each block in turn) in LDS (160 KiB)
 SLICED_CHUNK = 2048   # vectors per chunk of the sliced pull
 
 
@@ -80,7 +81,7 @@ def headline(gz, args) -> dict:
     t0 = time.time()
     sliced = not args.plain_pull
     gz.load_workload_device(FULL, relabel=not args.no_relabel, slices_log2=SLICES_LOG2 if sliced else 0,
-                            slice_tiers=SLICE_TIERS)
+                            slice_tiers=SLICE_TIERS, slice_blocks=SLICE_BLOCKS)
     gz.lib.gz_synchronize()
     V, E, nvp = gz.num_vertices(), gz.num_edges(), gz.pull_vector_count()
     log(f"[headline] {FULL}: V={V} E={E} Nvp={nvp} ingress {time.time() - t0:.2f}s")
@@ -349,8 +350,8 @@ def main() -> int:
         "config": {"workload": FULL + " PageRank pull (FP64, damping 0.85)", "num_vertices": h["V"],
                    "num_edges": h["E"], "pull_vectors": h["nvp"],
                    "ingress": "device, renumbered by out-degree" if not args.no_relabel else "device",
-                   "pull": (f"source-sliced (S=2^{SLICES_LOG2}, {SLICE_TIERS} tiers, LDS cache {HOT_CACHE}, "
-                            f"chunk {SLICED_CHUNK})"
+                   "pull": (f"source-sliced (S=2^{SLICES_LOG2}, {SLICE_TIERS} tiers, {SLICE_BLOCKS} LDS blocks, "
+                            f"LDS cache {HOT_CACHE}, chunk {SLICED_CHUNK})"
                             if h["sliced"] else "scheduler-aware, ordinary pull list"),
                    "device": dev},
         "roofline": h["roofline"],

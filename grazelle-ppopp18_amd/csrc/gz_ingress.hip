@@ -334,6 +334,7 @@ void gz_ingress_from_keys(uint64_t nv, uint64_t* a, uint64_t* b_buf, uint64_t n,
     const uint32_t slice_log2 = (flags >> 8) & 0x3fu;
     uint32_t tiers = (flags >> 14) & 0x3u;
     if (tiers == 0) tiers = 1;
+    const uint32_t blocks = (flags >> 16) & 0x7u;
     const uint64_t slice_vertices = slice_log2 ? gz_slices_effective(1ull << slice_log2, tiers, nv) : 0;
     uint64_t* old_of_new = nullptr;
     if (flags & GZ_INGRESS_RELABEL_BY_OUTDEGREE)
@@ -378,7 +379,7 @@ void gz_ingress_from_keys(uint64_t nv, uint64_t* a, uint64_t* b_buf, uint64_t n,
     GZ_CHECK(hipStreamSynchronize(gz_stream));
     gz_device_free(eoff);
     gz_graph_adopt_device(&L);
-    if (slice_log2) gz_graph_build_source_slices(1ull << slice_log2, tiers);
+    if (slice_log2) gz_graph_build_source_slices(1ull << slice_log2, tiers, blocks, GZ_SLICE_BLOCK_VERTICES);
 }
 
 void gz_check_vertex_count(uint64_t nv)
@@ -506,14 +507,13 @@ extern "C" int gz_graph_build_rmat_union_device(const gz_rmat_params_t* params, 
 /* ---- source-sliced pull layout ---------------------------------------------
  * From the loaded pull vectors: every valid lane becomes the key
  * (slice << 2b | dst << b | src) (invalid lanes all ones), a stable radix pass
- * over the 5 slice bits groups the keys by slice while keeping the (dst, src)
+ * over the 7 slice bits groups the keys by slice while keeping the (dst, src)
  * order, and each slice's keys are packed by the layout builder above. */
 namespace {
 
 __global__ void gz_k_slice_keys(const gz_vec_t* __restrict__ vecs, uint64_t nvec, uint32_t b, uint64_t slice_vertices,
-                                uint32_t tiers, uint64_t* __restrict__ keys)
+                                uint32_t tiers, uint32_t blocks, uint64_t block_vertices, uint64_t* __restrict__ keys)
 {
-    const uint64_t hot = (uint64_t)GZ_XCDS * tiers * slice_vertices;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < 4 * nvec; i += stride) {
         const gz_vec_t* v = vecs + i / 4;
@@ -529,7 +529,7 @@ __global__ void gz_k_slice_keys(const gz_vec_t* __restrict__ vecs, uint64_t nvec
         w.l3 = v->lane[3];
         const uint64_t d = gz_shared_vertex(w);
         const uint64_t s = gz_lane_id(l);
-        const uint64_t x = (s < hot) ? s / slice_vertices : (uint64_t)GZ_XCDS * tiers;
+        const uint64_t x = gz_slice_of_source(s, slice_vertices, tiers, blocks, block_vertices);
         keys[i] = (x << (2 * b)) | (d << b) | s;
     }
 }
@@ -560,13 +560,14 @@ __global__ void gz_k_strip_slice(uint64_t* __restrict__ keys, uint64_t n, uint32
 
 /* mask bit for slice `bit`, and has[v] = 1 when v has vectors in the slice
  * (has[nv] = 0, so its exclusive scan ends in the slice's vertex count) */
-__global__ void gz_k_slice_mask(const uint64_t* __restrict__ voff, uint64_t nv, uint32_t bit, uint32_t* __restrict__ mask,
+template <typename M>
+__global__ void gz_k_slice_mask(const uint64_t* __restrict__ voff, uint64_t nv, uint32_t bit, M* __restrict__ mask,
                                 uint64_t* __restrict__ has)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v <= nv; v += stride) {
         const bool in = v < nv && voff[v + 1] > voff[v];
-        if (in) mask[v] |= 1u << bit;
+        if (in) mask[v] |= (M)1 << bit;
         has[v] = in ? 1 : 0;
     }
 }
@@ -580,14 +581,15 @@ __global__ void gz_k_slice_gid(const uint64_t* __restrict__ has, const uint64_t*
         if (has[v]) gid[rank[v]] = (uint32_t)v;
 }
 
-/* local rank in slice `x` of the first vertex of each 64-vertex block, at
+/* index in the concatenated slice accumulators (slice x's run starts at
+ * `part_off`) of the first vertex of each 64-vertex block, at
  * out[block * nslices + x] */
 __global__ void gz_k_rank_base(const uint64_t* __restrict__ rank, uint64_t blocks, uint32_t x, uint32_t nslices,
-                               uint32_t* __restrict__ out)
+                               uint64_t part_off, uint32_t* __restrict__ out)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < blocks; k += stride)
-        out[k * nslices + x] = (uint32_t)rank[k * GZ_WAVE];
+        out[k * nslices + x] = (uint32_t)(part_off + rank[k * GZ_WAVE]);
 }
 
 /* the shared vertex of every vector becomes the destination's local rank */
@@ -677,7 +679,8 @@ __global__ void gz_k_pack_u24(const gz_vec_t* __restrict__ vecs, uint64_t n, uin
 
 }  // namespace
 
-extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t tiers)
+extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t tiers, uint32_t blocks,
+                                            uint64_t block_vertices)
 {
     gz_runtime_init(0);
     if (gz_graph.pull_vectors == nullptr) {
@@ -686,17 +689,25 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
     }
     const uint64_t nv = gz_graph.num_vertices, nvec = gz_graph.pull_vector_count;
     const uint32_t b = gz_id_bits(nv);
-    if (2 * b + 5 > 64) {
-        fprintf(stderr, "grazelle: the source-sliced layout needs V <= 2^29 (V = %llu)\n", (unsigned long long)nv);
+    if (2 * b + GZ_SLICE_KEY_BITS > 64) {
+        fprintf(stderr, "grazelle: the source-sliced layout needs V <= 2^28 (V = %llu)\n", (unsigned long long)nv);
         exit(65);
     }
-    if (tiers < 1 || tiers > GZ_MAX_TIERS) {
-        fprintf(stderr, "grazelle: source-sliced tiers must be 1..%d (got %u)\n", GZ_MAX_TIERS, tiers);
+    if (tiers < 1 || tiers > GZ_MAX_TIERS || blocks + tiers > GZ_MAX_LEVELS) {
+        fprintf(stderr, "grazelle: source-sliced tiers must be 1..%d and tiers + blocks at most %d (got %u, %u)\n",
+                GZ_MAX_TIERS, GZ_MAX_LEVELS, tiers, blocks);
         exit(65);
     }
     gz_source_slices_release();
     const uint64_t S = gz_slices_effective(slice_vertices, tiers, nv);
-    const uint32_t ns = GZ_XCDS * tiers + 1;
+    /* as many of the requested blocks as fit in a window after its block 0 */
+    const uint64_t H = blocks ? block_vertices : 0;
+    if (blocks && (H == 0 || S / H < 1))
+        blocks = 0;
+    else if (blocks && S / H - 1 < blocks)
+        blocks = (uint32_t)(S / H - 1);
+    const uint32_t ns = GZ_XCDS * (tiers + blocks) + 1;
+    const bool wide_mask = ns > 32;
     const uint64_t n = 4 * nvec;
     uint64_t* a = (uint64_t*)gz_device_malloc((n ? n : 1) * sizeof(uint64_t));
     uint64_t* bb = (uint64_t*)gz_device_malloc((n ? n : 1) * sizeof(uint64_t));
@@ -705,13 +716,15 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
     uint64_t* sorted = a;
     if (n) {
         hipLaunchKernelGGL(gz_k_slice_keys, dim3(gz_grid_for(n)), dim3(GZ_BLOCK), 0, gz_stream, gz_graph.pull_vectors,
-                           nvec, b, S, tiers, a);
+                           nvec, b, S, tiers, blocks, H, a);
         GZ_LAUNCH_CHECK();
         rocprim::double_buffer<uint64_t> db(a, bb);
         size_t tmp_bytes = 0;
-        GZ_CHECK(rocprim::radix_sort_keys(nullptr, tmp_bytes, db, (size_t)n, 2 * b, 2 * b + 5, gz_stream));
+        GZ_CHECK(rocprim::radix_sort_keys(nullptr, tmp_bytes, db, (size_t)n, 2 * b, 2 * b + GZ_SLICE_KEY_BITS,
+                                          gz_stream));
         void* tmp = gz_device_malloc(tmp_bytes ? tmp_bytes : 1);
-        GZ_CHECK(rocprim::radix_sort_keys(tmp, tmp_bytes, db, (size_t)n, 2 * b, 2 * b + 5, gz_stream));
+        GZ_CHECK(rocprim::radix_sort_keys(tmp, tmp_bytes, db, (size_t)n, 2 * b, 2 * b + GZ_SLICE_KEY_BITS,
+                                          gz_stream));
         GZ_CHECK(hipStreamSynchronize(gz_stream));
         gz_device_free(tmp);
         sorted = db.current();
@@ -724,7 +737,7 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
     } else {
         for (uint32_t x = 0; x <= ns; ++x) lo[x] = 0;
     }
-    gz_slices.mask = (uint32_t*)gz_device_malloc((nv + 8) * sizeof(uint32_t));
+    gz_slices.mask = gz_device_malloc((nv + 8) * (wide_mask ? sizeof(uint64_t) : sizeof(uint32_t)));
     gz_slices.rank_blocks = (nv + GZ_WAVE - 1) / GZ_WAVE;
     gz_slices.rank_base = (uint32_t*)gz_device_malloc(ns * gz_slices.rank_blocks * sizeof(uint32_t));
     uint64_t* eoff = (uint64_t*)gz_device_malloc((nv + 1) * sizeof(uint64_t));
@@ -737,16 +750,21 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
     gz_vec_t* part_vecs[GZ_MAX_SLICES];
     uint32_t* part_gid[GZ_MAX_SLICES];
     uint64_t counts[GZ_MAX_SLICES], ranked[GZ_MAX_SLICES];
+    uint64_t part_off = 0;
     for (uint32_t x = 0; x < ns; ++x) {
         uint64_t* voff = nullptr;
         gz_build_layout(sorted + lo[x], lo[x + 1] - lo[x], b, nv, &part_vecs[x], &voff, &counts[x], eoff);
-        hipLaunchKernelGGL(gz_k_slice_mask, dim3(gz_grid_for(nv + 1)), dim3(GZ_BLOCK), 0, gz_stream, voff, nv,
-                           x, gz_slices.mask, has);
+        if (wide_mask)
+            hipLaunchKernelGGL(gz_k_slice_mask<uint64_t>, dim3(gz_grid_for(nv + 1)), dim3(GZ_BLOCK), 0, gz_stream,
+                               voff, nv, x, (uint64_t*)gz_slices.mask, has);
+        else
+            hipLaunchKernelGGL(gz_k_slice_mask<uint32_t>, dim3(gz_grid_for(nv + 1)), dim3(GZ_BLOCK), 0, gz_stream,
+                               voff, nv, x, (uint32_t*)gz_slices.mask, has);
         GZ_LAUNCH_CHECK();
         GZ_CHECK(rocprim::exclusive_scan(scan_tmp, scan_bytes, has, rank, (uint64_t)0, (size_t)(nv + 1),
                                          rocprim::plus<uint64_t>(), gz_stream));
         hipLaunchKernelGGL(gz_k_rank_base, dim3(gz_grid_for(gz_slices.rank_blocks)), dim3(GZ_BLOCK), 0, gz_stream,
-                           rank, gz_slices.rank_blocks, x, ns, gz_slices.rank_base);
+                           rank, gz_slices.rank_blocks, x, ns, part_off, gz_slices.rank_base);
         GZ_LAUNCH_CHECK();
         if (counts[x]) {
             hipLaunchKernelGGL(gz_k_local_dst, dim3(gz_grid_for(counts[x])), dim3(GZ_BLOCK), 0, gz_stream,
@@ -754,6 +772,11 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
             GZ_LAUNCH_CHECK();
         }
         gz_copy_to_host(&ranked[x], rank + nv, sizeof(uint64_t));
+        part_off += ranked[x];
+        if (part_off + 8 > 0xffffffffull) {
+            fprintf(stderr, "grazelle: the sliced layout needs fewer than 2^32 slice accumulators\n");
+            exit(65);
+        }
         part_gid[x] = (uint32_t*)gz_device_malloc((ranked[x] ? ranked[x] : 1) * sizeof(uint32_t));
         hipLaunchKernelGGL(gz_k_slice_gid, dim3(gz_grid_for(nv)), dim3(GZ_BLOCK), 0, gz_stream, has, rank, nv,
                            part_gid[x]);
@@ -784,7 +807,8 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
             GZ_LAUNCH_CHECK();
             if (x + 1 < ns) {
                 hipLaunchKernelGGL(gz_k_pack_u24, dim3(gz_grid_for(counts[x])), dim3(GZ_BLOCK), 0, gz_stream,
-                                   part_vecs[x], counts[x], (uint64_t)x * S, gz_slices.nbr24 + 3 * gz_slices.begin[x]);
+                                   part_vecs[x], counts[x], gz_slice_window(x, S, blocks),
+                                   gz_slices.nbr24 + 3 * gz_slices.begin[x]);
                 GZ_LAUNCH_CHECK();
             }
         }
@@ -804,6 +828,8 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
     gz_slices.base_count = nvec;
     gz_slices.slice_vertices = S;
     gz_slices.tiers = tiers;
+    gz_slices.blocks = blocks;
+    gz_slices.block_vertices = H;
     gz_slices.nslices = ns;
     return 0;
 }

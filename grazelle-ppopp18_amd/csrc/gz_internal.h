@@ -72,17 +72,53 @@ extern hipStream_t gz_stream;
 extern uint32_t gz_cu_count; /* compute units of the device */
 
 /* ---- source-sliced pull layout, owned by gz_slices.hip -------------------
- * The pull vectors split by source.  Tier t < T holds eight hot slices: slice
- * 8t + x holds the edges whose source lies in [(8t + x) S, (8t + x + 1) S);
- * the last slice (8T) every other edge.  Each slice is an ordinary pull list
- * (dst-grouped), and the slices are stored one after another.  Workgroups are
- * dispatched to the XCDs round robin, so a chunk table that gives XCD x only
- * the chunks of slices x, 8 + x, ... (tier by tier) plus a share of the last
- * slice keeps each XCD's gathers inside one S-vertex window at a time, which
- * its own L2 can hold. */
+ * The pull vectors split by source, in levels of eight slices (slice
+ * 8 level + x belongs to XCD x).  With T tiers of windows of S vertices and G
+ * LDS blocks of H vertices:
+ *   level 0:          tier-0 window x, sources in [x S, (x + 1) S) except
+ *                     those of its blocks;
+ *   level b, 1..G:    block b of window x, sources in [x S + b H, x S + (b + 1) H);
+ *   level G + t:      tier-t window x (t = 1 .. T-1), sources in
+ *                     [(8t + x) S, (8t + x + 1) S);
+ *   slice 8 (G + T):  every other edge (the cold slice).
+ * Each slice is an ordinary pull list (dst-grouped), and the slices are stored
+ * one after another.  Workgroups are dispatched to the XCDs round robin, so a
+ * chunk table that gives XCD x only the chunks of its slices, level by level,
+ * plus a share of the cold slice keeps each XCD's gathers inside one window
+ * at a time, which its own L2 can hold; the persistent pull loads each block
+ * into LDS in turn, so the gathers of a block slice never leave the CU. */
 #define GZ_XCDS 8
 #define GZ_MAX_TIERS 3
-#define GZ_MAX_SLICES (GZ_XCDS * GZ_MAX_TIERS + 1)
+#define GZ_MAX_LEVELS 7 /* tiers + blocks: the slice mask of a vertex fits 64 bits */
+#define GZ_MAX_SLICES (GZ_XCDS * GZ_MAX_LEVELS + 1)
+#define GZ_SLICE_KEY_BITS 7 /* slice field of the layout builder's sort keys */
+
+/* First vertex of the window that slice `sl` (not the cold slice) packs its
+ * 24-bit offsets against: tier-0 window x for levels 0..G, else its tier's. */
+static inline __host__ __device__ uint64_t gz_slice_window(uint32_t sl, uint64_t S, uint32_t blocks)
+{
+    const uint32_t level = sl / GZ_XCDS, x = sl % GZ_XCDS;
+    const uint32_t tier = level <= blocks ? 0u : level - blocks;
+    return ((uint64_t)tier * GZ_XCDS + x) * S;
+}
+
+/* The slice that holds edges from source s (ids after the interleaved
+ * renumbering; any ids for a layout built without it). */
+static inline __host__ __device__ uint32_t gz_slice_of_source(uint64_t s, uint64_t S, uint32_t tiers, uint32_t blocks,
+                                                             uint64_t H)
+{
+    if (S == 0 || s >= (uint64_t)GZ_XCDS * tiers * S) return GZ_XCDS * (blocks + tiers);
+    const uint64_t w = s / S, local = s - w * S;
+    const uint32_t tier = (uint32_t)(w / GZ_XCDS), x = (uint32_t)(w % GZ_XCDS);
+    uint32_t level;
+    if (tier > 0)
+        level = blocks + tier;
+    else if (blocks > 0 && local >= H && local < (uint64_t)(blocks + 1) * H)
+        level = (uint32_t)(local / H);
+    else
+        level = 0;
+    return GZ_XCDS * level + x;
+}
 
 /* Offsets of the slices' accumulator runs in one array. */
 typedef struct gz_slice_offsets {
@@ -94,21 +130,26 @@ typedef struct gz_source_slices_t {
     uint64_t base_count;
     uint64_t slice_vertices;          /* S */
     uint32_t tiers;                   /* T */
-    uint32_t nslices;                 /* 8T + 1 */
+    uint32_t blocks;                  /* G */
+    uint64_t block_vertices;          /* H */
+    uint32_t nslices;                 /* 8 (G + T) + 1 */
     /* packed vectors (gz_walk.h): the hot slices' vectors [0, begin[nslices-1])
      * as 24-bit window offsets, 12 bytes each, the last slice's as 32-bit ids */
     uint32_t* nbr24;                  /* 3 words per hot vector */
     gz_u32x4* nbr;                    /* last slice: vector begin[nslices-1] + j at nbr[j] */
     uint32_t* shared;                 /* every vector's shared vertex (local rank) */
     uint64_t begin[GZ_MAX_SLICES + 1]; /* first vector of each slice */
-    uint32_t* mask;                   /* [V] bit x: the vertex has vectors in slice x */
+    void* mask;                       /* [V] bit x: the vertex has vectors in slice x; uint32 words
+                                       * for at most 32 slices, else uint64 */
     /* Inside slice x, destinations are numbered densely (local rank: the
      * count of lower vertices that have vectors in slice x), and the shared
      * vertex of slice x's vectors is that local rank. */
     double* parts;                    /* slice x's accumulators at parts + part_base.off[x], by local rank */
     gz_slice_offsets part_base;
     uint32_t* gid;                    /* global vertex of each (slice, local rank), same offsets as parts */
-    uint32_t* rank_base;              /* [rank_blocks][nslices] local rank of each 64-vertex block's first vertex */
+    uint32_t* rank_base;              /* [rank_blocks][nslices] index in parts of the accumulator of each
+                                       * 64-vertex block's first vertex in slice x (part_base.off[x] + its
+                                       * local rank) */
     uint64_t rank_blocks;             /* ceil(V / 64) */
     gz_slice_chunk_t* table;          /* chunk slots, built for table_chunk_len */
     uint64_t slots;

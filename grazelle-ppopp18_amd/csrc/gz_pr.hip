@@ -20,6 +20,8 @@
 #include "gz_pr_ops.h"
 #include "gz_walk.h"
 
+#include <type_traits>
+
 /* Set by the edge phases: the scheduler-aware pull overwrites every
  * accumulator it owns, the traditional pull and the push accumulate with
  * atomics, so the following vertex phase must zero them
@@ -217,73 +219,74 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr(double* __restrict__ 
 /* The vertex phase after a source-sliced pull: acc[v] is the sum of the
  * slices holding v (mask bit x), added in slice order, 0 without in-edges.
  * Slice x keeps its sums densely by local rank; one wavefront takes 64
- * consecutive vertices, and v's rank in slice x is the rank of the block's
- * first vertex (rank_base[block][x]) plus the lanes below v that are also in
- * slice x, so each slice's sums are read as one contiguous run. */
+ * consecutive vertices, and v's accumulator in slice x is the block's first
+ * one there (rank_base[block][x], an index into the concatenated runs) plus
+ * the lanes below v that are also in slice x, so each slice's sums are read
+ * as one contiguous run.  Every slice is added, 0 where v is not in it: x +
+ * 0.0 is x, so the sum is the in-order sum of v's own slices.  A level of
+ * eight slices that no vertex of the block is in costs one ballot.  Masks are
+ * 32-bit words for up to 32 slices, else 64-bit. */
 template <int NS>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr_sliced(
     double* __restrict__ props, const double* __restrict__ outdeg, uint64_t start, uint64_t end,
     uint64_t num_vertices, const uint64_t* __restrict__ reduce_buffer, double damping,
-    const uint32_t* __restrict__ mask, const double* __restrict__ parts, gz_slice_offsets part_base,
-    const uint32_t* __restrict__ rank_base)
+    const void* __restrict__ mask_words, const double* __restrict__ parts, const uint32_t* __restrict__ rank_base)
 {
+    typedef typename std::conditional<(NS <= 32), uint32_t, uint64_t>::type mask_t;
+    const mask_t* __restrict__ mask = (const mask_t*)mask_words;
     const double nv = (double)num_vertices;
     const double S = __longlong_as_double((long long)reduce_buffer[0]);
     const double c = (1.0 - S) / nv;
     const double k = (1.0 - damping) / nv;
     const int lane = gz_lane();
-    const uint64_t below = (1ull << lane) - 1ull;
     const uint64_t last = (end + GZ_WAVE - 1) / GZ_WAVE;
     const uint64_t waves = (uint64_t)gridDim.x * GZ_WAVES_PER_BLOCK;
-    /* the wavefront's index, made visibly uniform so that the block's rank
-     * bases are scalar loads and the slice-sum addresses scalar bases */
     const uint32_t wib = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x / GZ_WAVE));
     uint64_t blk = start / GZ_WAVE + (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + wib;
-    /* the next block's mask and rank bases are loaded one iteration ahead, so
-     * that the slice sums (whose addresses depend on them) are the only wait
-     * of an iteration */
-    uint32_t m_next = 0;
-    uint32_t rb_next[NS];
+    /* the next block's mask and accumulator bases (lane x < NS holds slice
+     * x's) are loaded one iteration ahead, so that the slice sums (whose
+     * addresses depend on them) are the only wait of an iteration */
+    mask_t m_next = 0;
+    uint32_t rb_next = 0;
     if (blk < last) {
         const uint64_t v0 = blk * GZ_WAVE + (uint64_t)lane;
-        m_next = (v0 < num_vertices) ? mask[v0] : 0u;
-#pragma unroll
-        for (int x = 0; x < NS; ++x) rb_next[x] = rank_base[blk * (uint64_t)NS + x];
+        m_next = (v0 < num_vertices) ? mask[v0] : (mask_t)0;
+        rb_next = (lane < NS) ? rank_base[blk * (uint64_t)NS + lane] : 0u;
     }
     for (; blk < last; blk += waves) {
         const uint64_t v = blk * GZ_WAVE + (uint64_t)lane;
-        const uint32_t m = m_next;
-        uint32_t rb[NS];
-#pragma unroll
-        for (int x = 0; x < NS; ++x) rb[x] = rb_next[x];
+        const mask_t m = m_next;
+        const uint32_t rb = rb_next;
         const uint64_t nb = blk + waves;
         if (nb < last) {
             const uint64_t vn = nb * GZ_WAVE + (uint64_t)lane;
-            m_next = (vn < num_vertices) ? mask[vn] : 0u;
-#pragma unroll
-            for (int x = 0; x < NS; ++x) rb_next[x] = rank_base[nb * (uint64_t)NS + x];
+            m_next = (vn < num_vertices) ? mask[vn] : (mask_t)0;
+            rb_next = (lane < NS) ? rank_base[nb * (uint64_t)NS + lane] : 0u;
         }
         const bool mine = v >= start && v < end;
         const double od = mine ? outdeg[v] : 1.0;
-        double p[NS];
-#pragma unroll
-        for (int x = 0; x < NS; ++x) {
-            const bool has = (m >> x) & 1u;
-            const uint64_t bal = __ballot(has);
-            p[x] = 0.0;
-            if (bal != 0) { /* wave-uniform: no block vertex in slice x, no load */
-                const double* run = parts + part_base.off[x] + rb[x];
-                if (has) p[x] = run[(uint32_t)__popcll(bal & below)];
-            }
-        }
         double a = 0.0;
-        bool first = true;
 #pragma unroll
-        for (int x = 0; x < NS; ++x)
-            if ((m >> x) & 1u) {
-                a = first ? p[x] : a + p[x];
-                first = false;
+        for (int g = 0; g < NS; g += GZ_XCDS) {
+            if (__ballot(((m >> g) & (mask_t)0xff) != 0) == 0) continue; /* wave-uniform */
+            double p[GZ_XCDS];
+#pragma unroll
+            for (int j = 0; j < GZ_XCDS; ++j) {
+                p[j] = 0.0;
+                if (g + j < NS) {
+                    const bool has = (m >> (g + j)) & 1u;
+                    const uint64_t bal = __ballot(has);
+                    if (bal != 0) { /* wave-uniform: no block vertex in slice g + j, no load */
+                        const double* run = parts + (uint32_t)__builtin_amdgcn_readlane((int)rb, g + j);
+                        if (has) p[j] = run[__builtin_amdgcn_mbcnt_hi((uint32_t)(bal >> 32),
+                                                                       __builtin_amdgcn_mbcnt_lo((uint32_t)bal, 0u))];
+                    }
+                }
             }
+#pragma unroll
+            for (int j = 0; j < GZ_XCDS; ++j)
+                if (g + j < NS) a = a + p[j];
+        }
         if (mine) props[v] = gz_pr_vertex_value(a, od, c, k, damping, nv);
     }
 }
@@ -303,14 +306,25 @@ extern "C" void perform_vertex_phase_pr(uint64_t vertex_start, uint64_t vertex_c
 #define GZ_VERTEX_SLICED_ARGS                                                                                  \
     dim3((unsigned)sblocks), dim3(GZ_BLOCK), 0, gz_stream, (double*)gz_graph.vertex_props,                        \
         gz_graph.vertex_outdegrees, vertex_start, end, gz_graph.num_vertices, reduce_buffer,                     \
-        gz_phase_config.damping, (const uint32_t*)gz_slices.mask, (const double*)gz_slices.parts,                \
-        gz_slices.part_base, (const uint32_t*)gz_slices.rank_base
-        if (gz_slices.nslices == 3 * GZ_XCDS + 1)
-            hipLaunchKernelGGL(gz_k_vertex_pr_sliced<3 * GZ_XCDS + 1>, GZ_VERTEX_SLICED_ARGS);
-        else if (gz_slices.nslices == 2 * GZ_XCDS + 1)
-            hipLaunchKernelGGL(gz_k_vertex_pr_sliced<2 * GZ_XCDS + 1>, GZ_VERTEX_SLICED_ARGS);
-        else
-            hipLaunchKernelGGL(gz_k_vertex_pr_sliced<GZ_XCDS + 1>, GZ_VERTEX_SLICED_ARGS);
+        gz_phase_config.damping, (const void*)gz_slices.mask, (const double*)gz_slices.parts,                    \
+        (const uint32_t*)gz_slices.rank_base
+#define GZ_VERTEX_SLICED_CASE(L)                                                                                \
+    case GZ_XCDS * (L) + 1:                                                                                      \
+        hipLaunchKernelGGL(gz_k_vertex_pr_sliced<GZ_XCDS * (L) + 1>, GZ_VERTEX_SLICED_ARGS);                       \
+        break;
+        switch (gz_slices.nslices) {
+            GZ_VERTEX_SLICED_CASE(1)
+            GZ_VERTEX_SLICED_CASE(2)
+            GZ_VERTEX_SLICED_CASE(3)
+            GZ_VERTEX_SLICED_CASE(4)
+            GZ_VERTEX_SLICED_CASE(5)
+            GZ_VERTEX_SLICED_CASE(6)
+            GZ_VERTEX_SLICED_CASE(7)
+        default:
+            fprintf(stderr, "grazelle: no vertex phase for %u slices\n", gz_slices.nslices);
+            exit(72);
+        }
+#undef GZ_VERTEX_SLICED_CASE
 #undef GZ_VERTEX_SLICED_ARGS
     } else {
         hipLaunchKernelGGL(gz_k_vertex_pr, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,

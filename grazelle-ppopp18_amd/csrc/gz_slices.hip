@@ -5,11 +5,12 @@
  * property per edge from anywhere in the vertex array.  On a power-law graph
  * renumbered by degree, the few million hottest sources carry most edges, but
  * their properties exceed one XCD's L2, and every XCD caches the same hot
- * lines.  Here the pull list is split by source into tiers of eight hot
- * slices of S vertices and a cold rest; the chunk table hands XCD x
- * (workgroups w with w % 8 == x, the dispatcher's round robin) only the
- * chunks of its slices x, 8 + x, ..., tier after tier, plus an even share of
- * the cold slice, so each XCD gathers from one S-vertex window at a time.
+ * lines.  Here the pull list is split by source into levels of eight slices
+ * (tiers of S-vertex windows, and LDS blocks carved from the tier-0 windows)
+ * and a cold rest; the chunk table hands XCD x (workgroups w with w % 8 == x,
+ * the dispatcher's round robin) only the chunks of its slices x, 8 + x, ...,
+ * level after level, plus an even share of the cold slice, so each XCD
+ * gathers from one window at a time, and from LDS alone in a block slice.
  *
  * Each slice is an ordinary pull list whose destinations are numbered
  * densely inside the slice, stored packed (24-bit window offsets in the
@@ -40,11 +41,11 @@ static uint32_t gz_slices_hot_cache = 0;
 
 /* ---- chunk table (host) ---------------------------------------------------- */
 
-extern "C" uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint32_t tiers, uint64_t chunk_len,
+extern "C" uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint32_t levels, uint64_t chunk_len,
                                           uint32_t cold_spread, gz_slice_chunk_t* out, uint64_t capacity)
 {
-    if (chunk_len == 0 || tiers < 1 || tiers > GZ_MAX_TIERS) return 0;
-    const int ns = GZ_XCDS * (int)tiers + 1, cold_slice = ns - 1;
+    if (chunk_len == 0 || levels < 1 || levels > GZ_MAX_LEVELS) return 0;
+    const int ns = GZ_XCDS * (int)levels + 1, cold_slice = ns - 1;
     uint64_t base[GZ_MAX_SLICES + 1], chunks[GZ_MAX_SLICES], rec[GZ_MAX_SLICES + 1];
     base[0] = 0;
     rec[0] = 0;
@@ -57,11 +58,11 @@ extern "C" uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint32_t tier
         fprintf(stderr, "grazelle: %llu chunks exceed the source-sliced record field\n", (unsigned long long)rec[ns]);
         exit(72);
     }
-    /* an XCD's own chunks: its slice in every tier */
+    /* an XCD's own chunks: its slice in every level */
     uint64_t own[GZ_XCDS];
     for (int x = 0; x < GZ_XCDS; ++x) {
         own[x] = 0;
-        for (uint32_t t = 0; t < tiers; ++t) own[x] += chunks[GZ_XCDS * t + x];
+        for (uint32_t t = 0; t < levels; ++t) own[x] += chunks[GZ_XCDS * t + x];
     }
     /* chunks per XCD: an even share of all chunks, at least its own */
     uint64_t target = (rec[ns] + GZ_XCDS - 1) / GZ_XCDS;
@@ -101,7 +102,7 @@ extern "C" uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint32_t tier
     memset(out, 0, slots * sizeof(gz_slice_chunk_t));
     uint64_t cold_next = 0;
     for (int x = 0; x < GZ_XCDS; ++x) {
-        uint32_t tier = 0;
+        uint32_t level = 0;
         uint64_t hot_next = 0;
         const uint64_t n = len[x], g = give[x];
         for (uint64_t i = 0; i < n; ++i) {
@@ -114,11 +115,11 @@ extern "C" uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint32_t tier
                 sl = cold_slice;
                 k = cold_next++;
             } else {
-                while (hot_next >= chunks[GZ_XCDS * tier + x]) {
+                while (hot_next >= chunks[GZ_XCDS * level + x]) {
                     hot_next = 0;
-                    ++tier;
+                    ++level;
                 }
-                sl = GZ_XCDS * (int)tier + x;
+                sl = GZ_XCDS * (int)level + x;
                 k = hot_next++;
             }
             gz_slice_chunk_t c;
@@ -159,17 +160,20 @@ extern "C" void gz_source_slices_release(void)
     memset(&gz_slices, 0, sizeof(gz_slices));
 }
 
-extern "C" int gz_graph_source_slices_info(uint64_t* vectors, uint64_t* slice_vertices, uint32_t* tiers)
+extern "C" int gz_graph_source_slices_info(uint64_t* vectors, uint64_t* slice_vertices, uint32_t* tiers,
+                                           uint32_t* blocks, uint64_t* block_vertices)
 {
     if (gz_slices.nbr == nullptr) return 0;
     if (vectors)
         for (uint32_t x = 0; x < gz_slices.nslices; ++x) vectors[x] = gz_slices.begin[x + 1] - gz_slices.begin[x];
     if (slice_vertices) *slice_vertices = gz_slices.slice_vertices;
     if (tiers) *tiers = gz_slices.tiers;
+    if (blocks) *blocks = gz_slices.blocks;
+    if (block_vertices) *block_vertices = gz_slices.block_vertices;
     return 1;
 }
 
-extern "C" int gz_graph_source_slices_download(gz_vec_t* vectors, uint32_t* mask)
+extern "C" int gz_graph_source_slices_download(gz_vec_t* vectors, uint64_t* mask)
 {
     if (gz_slices.nbr == nullptr) return 0;
     GZ_CHECK(hipStreamSynchronize(gz_stream));
@@ -193,8 +197,9 @@ extern "C" int gz_graph_source_slices_download(gz_vec_t* vectors, uint32_t* mask
                 const uint32_t* q = w24 + 3 * j;
                 const uint32_t f[4] = {q[0] & 0x7fffffu, (q[0] >> 24) | ((q[1] & 0xffffu) << 8),
                                        (q[1] >> 16) | ((q[2] & 0xffu) << 16), q[2] >> 8};
+                const uint64_t window = gz_slice_window(x, gz_slices.slice_vertices, gz_slices.blocks);
                 while (count < 4 && (count == 0 || f[count] != 0xffffffu)) {
-                    ids[count] = (uint64_t)x * gz_slices.slice_vertices + f[count];
+                    ids[count] = window + f[count];
                     ++count;
                 }
             } else {
@@ -210,7 +215,17 @@ extern "C" int gz_graph_source_slices_download(gz_vec_t* vectors, uint32_t* mask
         free(nbr);
         free(shared);
     }
-    if (mask) gz_copy_to_host(mask, gz_slices.mask, gz_graph.num_vertices * sizeof(uint32_t));
+    if (mask) {
+        const uint64_t nv = gz_graph.num_vertices;
+        if (gz_slices.nslices > 32) {
+            gz_copy_to_host(mask, gz_slices.mask, nv * sizeof(uint64_t));
+        } else {
+            uint32_t* m = (uint32_t*)malloc((nv ? nv : 1) * sizeof(uint32_t));
+            gz_copy_to_host(m, gz_slices.mask, nv * sizeof(uint32_t));
+            for (uint64_t v = 0; v < nv; ++v) mask[v] = m[v];
+            free(m);
+        }
+    }
     return 1;
 }
 
@@ -224,10 +239,10 @@ static void gz_slices_prepare_table(uint64_t chunk_len)
     const uint32_t ns = gz_slices.nslices;
     uint64_t vectors[GZ_MAX_SLICES];
     for (uint32_t x = 0; x < ns; ++x) vectors[x] = gz_slices.begin[x + 1] - gz_slices.begin[x];
-    const uint64_t slots =
-        gz_slices_chunk_table(vectors, gz_slices.tiers, chunk_len, gz_slices_cold_spread, nullptr, 0);
+    const uint32_t levels = gz_slices.tiers + gz_slices.blocks;
+    const uint64_t slots = gz_slices_chunk_table(vectors, levels, chunk_len, gz_slices_cold_spread, nullptr, 0);
     gz_slice_chunk_t* h = (gz_slice_chunk_t*)malloc((slots ? slots : 1) * sizeof(gz_slice_chunk_t));
-    gz_slices_chunk_table(vectors, gz_slices.tiers, chunk_len, gz_slices_cold_spread, h, slots);
+    gz_slices_chunk_table(vectors, levels, chunk_len, gz_slices_cold_spread, h, slots);
     uint64_t records = 0;
     for (uint32_t x = 0; x < ns; ++x) records += (vectors[x] + chunk_len - 1) / chunk_len;
     GZ_CHECK(hipStreamSynchronize(gz_stream));
@@ -254,8 +269,8 @@ struct gz_slice_bounds {
     uint64_t rec[GZ_MAX_SLICES + 1];
 };
 
-/* The packed slices as kernel arguments: the window slices in 24-bit form,
- * the last slice in 32-bit form. */
+/* The packed slices as kernel arguments: the window and block slices in
+ * 24-bit form, the last slice in 32-bit form. */
 struct gz_packed_lists {
     const uint32_t* nbr24;
     const gz_u32x4* nbr;
@@ -263,10 +278,11 @@ struct gz_packed_lists {
     uint64_t cold_first;
     uint64_t slice_vertices;
     uint32_t cold_slice;
+    uint32_t blocks;
     __device__ __forceinline__ gz_packed_list cold() const { return gz_packed_list{{shared}, nbr, cold_first}; }
     __device__ __forceinline__ gz_packed24_list window(uint32_t sl) const
     {
-        return gz_packed24_list{{shared}, nbr24, (uint32_t)((uint64_t)sl * slice_vertices)};
+        return gz_packed24_list{{shared}, nbr24, (uint32_t)gz_slice_window(sl, slice_vertices, blocks)};
     }
 };
 
@@ -279,6 +295,7 @@ static gz_packed_lists gz_slices_lists()
     p.cold_first = gz_slices.begin[gz_slices.nslices - 1];
     p.slice_vertices = gz_slices.slice_vertices;
     p.cold_slice = gz_slices.nslices - 1;
+    p.blocks = gz_slices.blocks;
     return p;
 }
 
@@ -382,6 +399,25 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_merge_sliced(const mergeaccum_t
  * wavefronts share one cache; a cache of at most 64 KiB leaves room for two
  * workgroups per CU, and the kernel is then held to the registers that allow
  * both. */
+/* Move a workgroup's LDS copy on to block `target` of its XCD's tier-0
+ * window, block by block: a barrier (every wavefront is done with the block
+ * held so far), the copy of the next block's H values, a barrier (the copy is
+ * complete).  Every wavefront of the workgroup steps through the same blocks
+ * (when it takes a chunk of a later level, or at its exit), so all of them
+ * reach each barrier; `cb` is wave-uniform. */
+template <typename T>
+__device__ __forceinline__ void gz_lds_advance(T* hot, const T* __restrict__ src, uint64_t lo, uint64_t H, uint32_t& cb,
+                                               uint32_t target)
+{
+    while (cb < target) {
+        __syncthreads();
+        ++cb;
+        const uint64_t base = lo + (uint64_t)cb * H;
+        for (uint64_t i = threadIdx.x; i < H; i += blockDim.x) hot[i] = src[base + i];
+        __syncthreads();
+    }
+}
+
 template <int HOT>
 struct gz_hot_wg {
     static constexpr int value = 1024;
@@ -470,13 +506,16 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
     const double* __restrict__ props, double* __restrict__ parts, gz_slice_offsets part_base,
     mergeaccum_t* __restrict__ merge, double* __restrict__ partials, uint64_t* __restrict__ stats, uint32_t collect,
     uint32_t* __restrict__ counters, uint64_t slice_vertices, uint64_t hot_count, uint32_t cold_slice,
-    uint32_t cold_stream)
+    uint32_t cold_stream, uint64_t block_vertices)
 {
     __shared__ double hot[HOT];
     const uint32_t x = blockIdx.x % GZ_XCDS;
     const uint64_t lo = (uint64_t)x * slice_vertices;
     for (uint64_t i = threadIdx.x; i < hot_count; i += blockDim.x) hot[i] = props[lo + i];
     __syncthreads();
+    /* LDS blocks, when they fit the cache; cb = the block the cache holds */
+    const uint32_t G = (block_vertices != 0 && block_vertices <= (uint64_t)HOT) ? lists.blocks : 0u;
+    uint32_t cb = 0;
     const int lane = gz_lane();
     const uint64_t n = lens.len[x];
     while (true) {
@@ -510,10 +549,21 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
                 sum = op.sum;
             }
         } else {
-            gz_pr_pull_op_hot<false> op{gprops, lds, (sl == x) ? (uint32_t)lo : 0u, (sl == x) ? (uint32_t)hot_count : 0u,
-                                        parts + part_base.off[sl], 0.0};
+            const uint32_t level = sl / GZ_XCDS;
+            if (level > cb && cb < G) gz_lds_advance(hot, props, lo, block_vertices, cb, level < G ? level : G);
+            /* the cached range: block 0 for the window, the block itself for a
+             * block slice (every source then hits LDS) */
+            uint32_t wlo = 0, wn = 0;
+            if (level == 0 && cb == 0) {
+                wlo = (uint32_t)lo;
+                wn = (uint32_t)hot_count;
+            } else if (level >= 1 && level == cb) {
+                wlo = (uint32_t)(lo + (uint64_t)level * block_vertices);
+                wn = (uint32_t)block_vertices;
+            }
+            gz_pr_pull_op_hot<false> op{gprops, lds, wlo, wn, parts + part_base.off[sl], 0.0};
             gz_walk_list<VPL, double, true, true>(lists.window(sl), c.begin, c.begin + c.count, op, first_d, carry_d,
-                                            carry_v, edges);
+                                                  carry_v, edges);
             sum = op.sum;
         }
         const double S = gz_wave_sum(sum);
@@ -534,6 +584,7 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
             }
         }
     }
+    gz_lds_advance(hot, props, lo, block_vertices, cb, G);
 }
 
 /* One workgroup per XCD and resident slot: the grid is exactly what fits. */
@@ -551,7 +602,7 @@ static void gz_launch_hot_vpl(const gz_xcd_lens& lens, uint64_t hot_count)
                        gz_slices_lists(), gz_slices.table, lens, (const double*)gz_graph.vertex_props, gz_slices.parts,
                        gz_slices.part_base, gz_slices.merge, (double*)gz_scratch.partials, gz_scratch.stats,
                        gz_phase_config.collect_stats, gz_slices.counters, gz_slices.slice_vertices, hot_count,
-                       gz_slices.nslices - 1, gz_slices_cold_stream);
+                       gz_slices.nslices - 1, gz_slices_cold_stream, gz_slices.block_vertices);
     GZ_LAUNCH_CHECK();
 }
 
@@ -617,13 +668,15 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
     uint64_t* __restrict__ next_frontier, const double* __restrict__ outdeg, uint32_t stat_mode,
     const uint32_t* __restrict__ gid, gz_slice_offsets part_base, uint64_t* __restrict__ partials,
     uint64_t* __restrict__ stats, uint32_t collect, uint32_t* __restrict__ counters, uint64_t slice_vertices,
-    uint64_t hot_count)
+    uint64_t hot_count, uint64_t block_vertices)
 {
     __shared__ uint64_t hot[HOT];
     const uint32_t x = blockIdx.x % GZ_XCDS;
     const uint64_t lo = (uint64_t)x * slice_vertices;
     for (uint64_t i = threadIdx.x; i < hot_count; i += blockDim.x) hot[i] = labels[lo + i];
     __syncthreads();
+    const uint32_t G = (block_vertices != 0 && block_vertices <= (uint64_t)HOT) ? lists.blocks : 0u;
+    uint32_t cb = 0;
     const int lane = gz_lane();
     const uint64_t wave = (uint64_t)blockIdx.x * (blockDim.x / GZ_WAVE) + threadIdx.x / GZ_WAVE;
     const uint64_t n = lens.len[x];
@@ -640,10 +693,22 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
             ((uint64_t)GZ_XCDS * (idx / GZ_WAVES_PER_BLOCK) + x) * GZ_WAVES_PER_BLOCK + idx % GZ_WAVES_PER_BLOCK;
         const gz_slice_chunk_t c = table[slot];
         const uint32_t sl = c.record >> GZ_SLICE_RECORD_BITS;
+        const uint32_t level = sl / GZ_XCDS;
+        uint32_t wlo = 0, wn = 0;
+        if (sl != lists.cold_slice) {
+            if (level > cb && cb < G) gz_lds_advance(hot, labels, lo, block_vertices, cb, level < G ? level : G);
+            if (level == 0 && cb == 0) {
+                wlo = (uint32_t)lo;
+                wn = (uint32_t)hot_count;
+            } else if (level >= 1 && level == cb) {
+                wlo = (uint32_t)(lo + (uint64_t)level * block_vertices);
+                wn = (uint32_t)block_vertices;
+            }
+        }
         gz_cc_pull_op_hot op{(gz_global_u64*)labels,
                              (gz_lds_u64*)hot,
-                             (sl == x) ? (uint32_t)lo : 0u,
-                             (sl == x) ? (uint32_t)hot_count : 0u,
+                             wlo,
+                             wn,
                              labels,
                              next_frontier,
                              outdeg,
@@ -661,6 +726,7 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
         vote += op.vote;
         vectors += c.count;
     }
+    gz_lds_advance(hot, labels, lo, block_vertices, cb, G);
     const uint64_t total = gz_wave_sum_u64(vote);
     if (lane == 0) partials[wave] = total;
     if (collect) {
@@ -692,7 +758,7 @@ static void gz_launch_cc_hot(const gz_xcd_lens& lens, uint64_t hot_count, uint64
                        (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
                        gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, gz_slices.gid, gz_slices.part_base,
                        (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats,
-                       gz_slices.counters, gz_slices.slice_vertices, hot_count);
+                       gz_slices.counters, gz_slices.slice_vertices, hot_count, gz_slices.block_vertices);
     GZ_LAUNCH_CHECK();
 }
 

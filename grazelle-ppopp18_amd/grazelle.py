@@ -95,8 +95,8 @@ _sig("gz_graph_load_pairs", C.c_int, u64, u64, vp, u64, vp)
 _sig("gz_graph_build_pairs_device", C.c_int, u64, u64, vp, C.c_int, u32)
 _sig("gz_graph_build_rmat_device", C.c_int, P(RmatParams), u64, C.c_int, u64, vp, u32)
 _sig("gz_graph_device_vertex", u64, u64)
-_sig("gz_graph_build_source_slices", C.c_int, u64, u32)
-_sig("gz_graph_source_slices_info", C.c_int, vp, P(u64), P(u32))
+_sig("gz_graph_build_source_slices", C.c_int, u64, u32, u32, u64)
+_sig("gz_graph_source_slices_info", C.c_int, vp, P(u64), P(u32), P(u32), P(u64))
 _sig("gz_slices_chunk_table", u64, vp, u32, u64, u32, vp, u64)
 _sig("gz_slices_set_cold_spread", None, u32)
 _sig("gz_slices_set_cold_stream", None, u32)
@@ -289,44 +289,49 @@ def load(g: EdgeLists, pull: bool = True, push: bool = True) -> None:
 
 RELABEL_BY_OUTDEGREE = 1
 SLICE_CHUNK = np.dtype([("begin", np.uint64), ("count", np.uint32), ("record", np.uint32)])
-SLICE_RECORD_BITS = 27
+SLICE_RECORD_BITS = 25
+SLICE_BLOCK_VERTICES = 20480
 
 
-def _ingress_flags(relabel: bool, slices_log2: int, slice_tiers: int = 1) -> int:
-    return (RELABEL_BY_OUTDEGREE if relabel else 0) | ((slices_log2 & 0x3F) << 8) | ((slice_tiers & 0x3) << 14)
+def _ingress_flags(relabel: bool, slices_log2: int, slice_tiers: int = 1, slice_blocks: int = 0) -> int:
+    return ((RELABEL_BY_OUTDEGREE if relabel else 0) | ((slices_log2 & 0x3F) << 8) | ((slice_tiers & 0x3) << 14)
+            | ((slice_blocks & 0x7) << 16))
 
 
 def load_pairs_device(num_vertices: int, pairs: np.ndarray, symmetrize: bool = False, relabel: bool = False,
-                      slices_log2: int = 0, slice_tiers: int = 1) -> None:
+                      slices_log2: int = 0, slice_tiers: int = 1, slice_blocks: int = 0) -> None:
     """Device ingress from raw (src, dst) pairs: sort, dedupe, pack in HBM.
     slices_log2 = k > 0 also builds the source-sliced pull layout with
-    S = 2^k and slice_tiers tiers (and, with relabel, deals each tier's 8*S
+    S = 2^k, slice_tiers tiers and slice_blocks LDS blocks of
+    SLICE_BLOCK_VERTICES sources (and, with relabel, deals each tier's 8*S
     ranks of the degree order into its eight windows)."""
     init()
     p = np.ascontiguousarray(pairs, dtype=np.uint64).reshape(-1, 2)
     lib.gz_graph_build_pairs_device(num_vertices, p.shape[0], _ptr(p) if p.shape[0] else None, int(symmetrize),
-                                    _ingress_flags(relabel, slices_log2, slice_tiers))
+                                    _ingress_flags(relabel, slices_log2, slice_tiers, slice_blocks))
 
 
 def load_rmat_device(scale: int, num_edges: int, seed: int, symmetrize: bool = False, num_vertices: int = 0,
                      extra: np.ndarray | None = None, rmat_vertices: int = 0, a=0.57, b=0.19, c=0.19,
-                     relabel: bool = False, slices_log2: int = 0, slice_tiers: int = 1) -> None:
+                     relabel: bool = False, slices_log2: int = 0, slice_tiers: int = 1,
+                     slice_blocks: int = 0) -> None:
     """Device ingress of a seeded R-MAT graph (the stream of rmat_edges) plus
     optional extra host pairs; V = max(num_vertices, R-MAT vertex count).
     relabel renumbers vertices by out-degree on the device (results keep
-    original ids); slices_log2 and slice_tiers as for load_pairs_device."""
+    original ids); slices_log2, slice_tiers and slice_blocks as for
+    load_pairs_device."""
     init()
     p = RmatParams(scale, rmat_vertices, num_edges, a, b, c, seed, 0)
     ex = np.ascontiguousarray(extra if extra is not None else np.zeros((0, 2), np.uint64), dtype=np.uint64)
     lib.gz_graph_build_rmat_device(C.byref(p), num_vertices, int(symmetrize), ex.shape[0],
                                    _ptr(ex) if ex.shape[0] else None,
-                                   _ingress_flags(relabel, slices_log2, slice_tiers))
+                                   _ingress_flags(relabel, slices_log2, slice_tiers, slice_blocks))
 
 
 def load_workload_device(name: str, seed: int = 1, relabel: bool = False, slices_log2: int = 0,
-                         slice_tiers: int = 1) -> None:
+                         slice_tiers: int = 1, slice_blocks: int = 0) -> None:
     """Build a named workload with device ingress (same graph as workload_graph)."""
-    kw = dict(relabel=relabel, slices_log2=slices_log2, slice_tiers=slice_tiers)
+    kw = dict(relabel=relabel, slices_log2=slices_log2, slice_tiers=slice_tiers, slice_blocks=slice_blocks)
     if name in ("pr-rmat16", "pr-rmat20", "bfs-rmat24"):
         scale = {"pr-rmat16": 16, "pr-rmat20": 20, "bfs-rmat24": 24}[name]
         load_rmat_device(scale, (1 << scale) * 16, seed, **kw)
@@ -343,20 +348,32 @@ def load_workload_device(name: str, seed: int = 1, relabel: bool = False, slices
     raise KeyError(name)
 
 
-def build_source_slices(slice_vertices: int, tiers: int = 1) -> None:
+def build_source_slices(slice_vertices: int, tiers: int = 1, blocks: int = 0,
+                        block_vertices: int = SLICE_BLOCK_VERTICES) -> None:
     """Source-sliced pull layout of the loaded graph (pull_variant
-    "source_sliced" runs PageRank over it)."""
-    lib.gz_graph_build_source_slices(slice_vertices, tiers)
+    "source_sliced" runs PageRank and CC over it): T = tiers windows of S
+    vertices per XCD, and `blocks` LDS blocks of block_vertices sources carved
+    from each tier-0 window (as many as fit)."""
+    lib.gz_graph_build_source_slices(slice_vertices, tiers, blocks, block_vertices)
+
+
+def source_slices_layout():
+    """(S, T, G, H) of the current sliced layout, or None."""
+    s, t, g, h = u64(0), u32(0), u32(0), u64(0)
+    if not lib.gz_graph_source_slices_info(None, C.byref(s), C.byref(t), C.byref(g), C.byref(h)):
+        return None
+    return int(s.value), int(t.value), int(g.value), int(h.value)
 
 
 def source_slices_info():
-    """(vectors per slice [8T + 1], S), or None without a sliced layout."""
-    s, t = u64(0), u32(0)
-    if not lib.gz_graph_source_slices_info(None, C.byref(s), C.byref(t)):
+    """(vectors per slice [8 (G + T) + 1], S), or None without a sliced layout."""
+    lay = source_slices_layout()
+    if lay is None:
         return None
-    v = np.zeros(8 * t.value + 1, np.uint64)
-    lib.gz_graph_source_slices_info(_ptr(v), None, None)
-    return v, int(s.value)
+    S, T, G, _ = lay
+    v = np.zeros(8 * (T + G) + 1, np.uint64)
+    lib.gz_graph_source_slices_info(_ptr(v), None, None, None, None)
+    return v, S
 
 
 def source_slices_download():
@@ -366,21 +383,21 @@ def source_slices_download():
         return None
     n = int(info[0].sum())
     vecs = np.zeros((max(n, 1), 4), np.uint64)
-    mask = np.zeros(max(num_vertices(), 1), np.uint32)
+    mask = np.zeros(max(num_vertices(), 1), np.uint64)
     lib.gz_graph_source_slices_download(_ptr(vecs), _ptr(mask))
     return vecs[:n], mask[:num_vertices()]
 
 
 def slices_chunk_table(vectors, chunk_len: int, cold_spread: bool = True) -> np.ndarray:
     """The chunk slots the source-sliced pull launches for slices of the
-    given vector counts (8T + 1 of them; host computation)."""
+    given vector counts (8 L + 1 of them for L levels; host computation)."""
     v = np.ascontiguousarray(vectors, dtype=np.uint64)
-    tiers = (len(v) - 1) // 8
-    if len(v) != 8 * tiers + 1:
-        raise ValueError("vector counts must have 8T + 1 entries")
-    n = lib.gz_slices_chunk_table(_ptr(v), tiers, chunk_len, int(cold_spread), None, 0)
+    levels = (len(v) - 1) // 8
+    if len(v) != 8 * levels + 1:
+        raise ValueError("vector counts must have 8L + 1 entries")
+    n = lib.gz_slices_chunk_table(_ptr(v), levels, chunk_len, int(cold_spread), None, 0)
     out = np.zeros(max(int(n), 1), SLICE_CHUNK)
-    lib.gz_slices_chunk_table(_ptr(v), tiers, chunk_len, int(cold_spread), _ptr(out), n)
+    lib.gz_slices_chunk_table(_ptr(v), levels, chunk_len, int(cold_spread), _ptr(out), n)
     return out[:n]
 
 

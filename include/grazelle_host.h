@@ -100,6 +100,10 @@ void graph_data_allocate_merge_buffers(uint64_t num_threads, uint32_t num_numa_n
  * so that the slices of a tier carry about the same share of the edges. */
 #define GZ_INGRESS_SOURCE_SLICES(k) (((uint32_t)(k) & 0x3fu) << 8)
 #define GZ_INGRESS_SLICE_TIERS(t) (((uint32_t)(t) & 0x3u) << 14)
+/* ... and g LDS blocks of GZ_SLICE_BLOCK_VERTICES sources per tier-0 window
+ * (0 when absent; see gz_graph_build_source_slices). */
+#define GZ_INGRESS_SLICE_BLOCKS(g) (((uint32_t)(g) & 0x7u) << 16)
+#define GZ_SLICE_BLOCK_VERTICES 20480u
 int gz_graph_build_pairs_device(uint64_t num_vertices, uint64_t num_edges, const uint64_t* pairs, int symmetrize,
                                 uint32_t flags);
 /* As above with the raw edges drawn on the device by the generator of
@@ -116,38 +120,48 @@ int gz_graph_build_rmat_union_device(const gz_rmat_params_t* params, uint32_t co
                                      uint64_t* vertex_bases);
 
 /* -------- source-sliced pull layout (no reference counterpart) ------------
- * Splits the loaded graph's pull vectors by source into 8T + 1 slices, T =
- * `tiers` (1..3): slice x < 8T holds the edges with source in
- * [x*S, (x+1)*S), slice 8T all others.  Each slice is an ordinary
- * dst-grouped pull list whose destinations are renumbered densely inside the
- * slice (in vertex order).  PageRank runs over it with pull_variant
- * GZ_PULL_SOURCE_SLICED, which gives XCD x the chunks of slices x, 8 + x, ...
- * (plus a share of slice 8T).  S is capped at V / (8T).  Needs V <= 2^29.
- * Replaces any earlier sliced layout. */
-int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t tiers);
-/* The current layout: vectors per slice (8T + 1 entries), S and T, each
- * pointer may be NULL; returns 0 when there is none. */
-int gz_graph_source_slices_info(uint64_t* vectors, uint64_t* slice_vertices, uint32_t* tiers);
+ * Splits the loaded graph's pull vectors by source into 8 (G + T) + 1 slices,
+ * in levels of eight (slice 8 level + x belongs to XCD x), T = `tiers` (1..3)
+ * windows of S vertices per XCD, G = `blocks` LDS blocks of H =
+ * `block_vertices` sources carved from each tier-0 window:
+ *   level 0          window x of tier 0, [x*S, (x+1)*S) without its blocks
+ *   level b (1..G)   block b of window x, [x*S + b*H, x*S + (b+1)*H)
+ *   level G + t      window x of tier t >= 1, [(8t+x)*S, (8t+x+1)*S)
+ *   slice 8 (G + T)  all other sources.
+ * Each slice is an ordinary dst-grouped pull list whose destinations are
+ * renumbered densely inside the slice (in vertex order).  PageRank and CC run
+ * over it with pull_variant GZ_PULL_SOURCE_SLICED, which gives XCD x the
+ * chunks of its slices level by level (plus a share of the last slice); with
+ * an LDS cache of at least H sources (gz_slices_set_hot_cache) each workgroup
+ * holds block b in LDS while it takes block b's chunks.  S is capped at
+ * V / (8T) and at 2^23; G + T <= 7; G > 0 needs (G + 1) * H <= S.  Needs
+ * V <= 2^28.  Replaces any earlier sliced layout. */
+int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t tiers, uint32_t blocks, uint64_t block_vertices);
+/* The current layout: vectors per slice (8 (G + T) + 1 entries), S, T, G and
+ * H; each pointer may be NULL; returns 0 when there is none. */
+int gz_graph_source_slices_info(uint64_t* vectors, uint64_t* slice_vertices, uint32_t* tiers, uint32_t* blocks,
+                                uint64_t* block_vertices);
 /* Copy the concatenated slice vectors and the per-vertex slice mask (bit x:
  * the vertex has vectors in slice x) to the host; either may be NULL.
  * Returns 0 when there is no sliced layout. */
-int gz_graph_source_slices_download(gz_vec_t* vectors, uint32_t* mask);
-/* How the chunks of slices with vectors[8T + 1] vectors are laid out at chunk
- * length chunk_len: XCD x (workgroup w with w % 8 == x) takes the chunks of
- * slices x, 8 + x, ... in that order, and slice 8T's chunks fill the XCDs up
- * to an even share, spread between the others (cold_spread) or after them.
- * Four slots per workgroup.  Writes up to `capacity` slots to `out` (NULL:
- * count only) and returns the slot count (0 for tiers outside 1..3).
+int gz_graph_source_slices_download(gz_vec_t* vectors, uint64_t* mask);
+/* How the chunks of slices with vectors[8 levels + 1] vectors are laid out at
+ * chunk length chunk_len: XCD x (workgroup w with w % 8 == x) takes the chunks
+ * of slices x, 8 + x, ... in that order, and the last slice's chunks fill the
+ * XCDs up to an even share, spread between the others (cold_spread) or after
+ * them.  Four slots per workgroup.  Writes up to `capacity` slots to `out`
+ * (NULL: count only) and returns the slot count (0 for levels outside 1..7).
  * Host-only computation. */
-uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint32_t tiers, uint64_t chunk_len, uint32_t cold_spread,
+uint64_t gz_slices_chunk_table(const uint64_t* vectors, uint32_t levels, uint64_t chunk_len, uint32_t cold_spread,
                                gz_slice_chunk_t* out, uint64_t capacity);
 /* Layout of the last slice's chunks for the next source-sliced pull (default 1). */
 void gz_slices_set_cold_spread(uint32_t cold_spread);
 /* Nonzero: the last slice's gathers are nontemporal loads (default 0). */
 void gz_slices_set_cold_stream(uint32_t cold_stream);
-/* Keep the first `vertices` properties of each tier-0 slice in LDS (0 = off,
- * else 4096, 8192, 16384 or 20480); the pull then runs as a persistent kernel.
- * Returns -1 for another value. */
+/* Keep the first `vertices` properties of each tier-0 window in LDS (0 = off,
+ * else 4096, 8192, 16384 or 20480), and each LDS block in turn when the
+ * blocks fit; the pull then runs as a persistent kernel.  Returns -1 for
+ * another value. */
 int gz_slices_set_hot_cache(uint32_t vertices);
 
 /* -------- batches of independent graphs (no reference counterpart) -------

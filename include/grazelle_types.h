@@ -51,14 +51,14 @@ typedef struct mergeaccum_t {
 
 /* One chunk slot of the source-sliced pull (no reference counterpart): the
  * `count` vectors from `begin` of the concatenated slices; `record` is the
- * chunk's merge record with its slice in the top 5 bits.  count 0 marks an
+ * chunk's merge record with its slice in the top 7 bits.  count 0 marks an
  * empty slot. */
 typedef struct gz_slice_chunk_t {
     uint64_t begin;
     uint32_t count;
     uint32_t record;
 } gz_slice_chunk_t;
-#define GZ_SLICE_RECORD_BITS 27
+#define GZ_SLICE_RECORD_BITS 25
 
 /* Recover the shared vertex from its four 15/15/15/3-bit pieces. */
 static inline uint64_t gz_vec_shared_vertex(const gz_vec_t* v)

@@ -1,9 +1,12 @@
 """Source-sliced PageRank pull versus the host packer and the CPU oracle.
 
-Layout (T tiers): slice x < 8T must be the host packer's pull list of the
-edges whose source is in [x*S, (x+1)*S), slice 8T of all other edges, with each
-destination replaced by its rank among the slice's destinations, concatenated
-in slice order; the mask must name the slices holding each destination.
+Layout (T tiers, G LDS blocks of H sources): slice 8*level + x must be the
+host packer's pull list of the edges whose source falls in that slice (level
+0: tier-0 window x without its blocks, levels 1..G: block b of window x,
+[x*S + b*H, x*S + (b+1)*H), level G + t: tier-t window x), the last slice of
+all other edges, with each destination replaced by its rank among the slice's
+destinations, concatenated in slice order; the mask must name the slices
+holding each destination.
 PageRank over it must match the oracle within 1e-12 maximum relative error
 per vertex (each destination's sum is regrouped by slice), be bit-identical
 run to run, and the interleaved renumbering must deal each tier's 8*S ranks
@@ -26,33 +29,56 @@ def rmat14(gz):
     return gz.rmat_graph(14, 16, seed=11)
 
 
-def _expected_layout(gz, pull, V, S, T=1):
-    S = min(S, V // (8 * T))
+def _layout_params(V, S, T, G, H):
+    S = min(S, V // (8 * T), 1 << 23)
+    if G and (H == 0 or S // H < 1):
+        G = 0
+    G = min(G, S // H - 1) if G else 0
+    return S, G
+
+
+def _expected_layout(gz, pull, V, S, T=1, G=0, H=0):
+    S, G = _layout_params(V, S, T, G, H)
     src = pull[:, 0].astype(np.int64)
-    sl = np.where(src < 8 * T * S, src // max(S, 1), 8 * T) if S else np.full(len(src), 8 * T)
-    vecs, mask = [], np.zeros(V, np.uint32)
-    for x in range(8 * T + 1):
+    cold = 8 * (G + T)
+    if S:
+        w = src // S
+        local = src - w * S
+        tier, x = w // 8, w % 8
+        level = np.where(tier > 0, G + tier, 0)
+        if G:
+            level = np.where((tier == 0) & (local >= H) & (local < (G + 1) * H), local // H, level)
+        sl = np.where(src < 8 * T * S, 8 * level + x, cold)
+    else:
+        sl = np.full(len(src), cold)
+    vecs, mask = [], np.zeros(V, np.uint64)
+    for x in range(cold + 1):
         part = pull[sl == x]
         if len(part):
             dsts, local = np.unique(part[:, 1], return_inverse=True)
             ranked = np.stack([part[:, 0], local.astype(np.uint64)], axis=1)
             vecs.append(gz.pack_vectors(ranked, True))
-            mask[dsts.astype(np.int64)] |= np.uint32(1 << x)
+            mask[dsts.astype(np.int64)] |= np.uint64(1 << x)
         else:
             vecs.append(np.zeros((0, 4), np.uint64))
     return vecs, mask
 
 
-@pytest.mark.parametrize("S,T", [(1, 1), (64, 1), (700, 1), (1 << 20, 1), (64, 2), (300, 3), (1 << 20, 3)])
-def test_slices_layout_matches_host_packer(gz, device, rmat14, S, T):
+@pytest.mark.parametrize("S,T,G,H", [(1, 1, 0, 0), (64, 1, 0, 0), (700, 1, 0, 0), (1 << 20, 1, 0, 0),
+                                     (64, 2, 0, 0), (300, 3, 0, 0), (1 << 20, 3, 0, 0),
+                                     (1024, 1, 3, 64), (2048, 1, 6, 100), (512, 2, 2, 100), (300, 3, 4, 10),
+                                     (128, 1, 5, 64), (100, 1, 2, 200)])
+def test_slices_layout_matches_host_packer(gz, device, rmat14, S, T, G, H):
     g = rmat14
     gz.load(g)
-    gz.build_source_slices(S, T)
-    counts, s_eff = gz.source_slices_info()
-    assert len(counts) == 8 * T + 1
-    assert s_eff == min(S, g.num_vertices // (8 * T))
+    gz.build_source_slices(S, T, G, H)
+    s_eff, g_eff = _layout_params(g.num_vertices, S, T, G, H)
+    assert gz.source_slices_layout() == (s_eff, T, g_eff, H if G else 0)
+    counts, got_s = gz.source_slices_info()
+    assert len(counts) == 8 * (T + g_eff) + 1
+    assert got_s == s_eff
     vecs, mask = gz.source_slices_download()
-    want, want_mask = _expected_layout(gz, g.pull, g.num_vertices, S, T)
+    want, want_mask = _expected_layout(gz, g.pull, g.num_vertices, S, T, G, H)
     assert [int(c) for c in counts] == [len(w) for w in want]
     assert np.array_equal(vecs, np.concatenate(want).reshape(-1, 4))
     assert np.array_equal(mask, want_mask)
@@ -243,6 +269,83 @@ def test_sliced_matches_plain_at_scale_20(gz, device):
         sliced = gz.pr_ranks()
         gz.run(gz.options("cc"))
         cc_plain = gz.cc_labels()
+        gz.run(gz.options("cc", pull_variant="source_sliced"))
+        cc_sliced = gz.cc_labels()
+    finally:
+        gz.slices_set_hot_cache(0)
+        gz.set_chunk_vectors(512)
+    assert _rel(sliced, plain) <= PR_TOL
+    assert abs(sliced.sum() - 1.0) < 1e-9
+    assert np.array_equal(cc_sliced, cc_plain)
+
+
+# ---- LDS blocks ----------------------------------------------------------------
+
+
+@pytest.mark.parametrize("hot", [0, 4096])
+@pytest.mark.parametrize("vpl", [1, 2])
+@pytest.mark.parametrize("chunk", [64, 512])
+@pytest.mark.parametrize("S,T,G,H", [(2048, 1, 3, 64), (1024, 2, 4, 100), (512, 3, 4, 64), (256, 1, 6, 32)])
+def test_sliced_pr_blocks_parity(gz, orc, device, rmat14, S, T, G, H, chunk, vpl, hot):
+    # blocks of H <= the cache are held in LDS one after another; without a
+    # cache (hot 0) their gathers read memory
+    g = rmat14
+    gz.set_chunk_vectors(chunk)
+    gz.set_vectors_per_lane(vpl)
+    gz.load(g)
+    gz.build_source_slices(S, T, G, H)
+    gz.slices_set_hot_cache(hot)
+    try:
+        gz.run(gz.options("pr", iterations=20, pull_variant="source_sliced"))
+        a = gz.pr_props()
+        ranks = gz.pr_ranks()
+        gz.run(gz.options("pr", iterations=20, pull_variant="source_sliced"))
+        b = gz.pr_props()
+    finally:
+        gz.slices_set_hot_cache(0)
+        gz.set_chunk_vectors(512)
+        gz.set_vectors_per_lane(1)
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    _, ref, _ = orc.pagerank(orc.Graph(g.num_vertices, g.pull, g.push), orc.options(iterations=20))
+    assert _rel(ranks, ref) <= PR_TOL
+    assert abs(ranks.sum() - 1.0) < 1e-9
+
+
+@pytest.mark.parametrize("hot", [0, 4096])
+@pytest.mark.parametrize("S,T,G,H", [(256, 1, 3, 32), (128, 2, 5, 16)])
+@pytest.mark.parametrize("engine", ["pull", "hybrid"])
+def test_sliced_cc_blocks_bit_exact(gz, orc, device, cc_trees, S, T, G, H, hot, engine):
+    g = cc_trees
+    gz.load(g)
+    gz.build_source_slices(S, T, G, H)
+    assert gz.source_slices_layout()[2] == G
+    gz.slices_set_hot_cache(hot)
+    try:
+        gz.run(gz.options("cc", engine=engine, pull_variant="source_sliced"))
+        got = gz.cc_labels()
+    finally:
+        gz.slices_set_hot_cache(0)
+    labels, _, _ = orc.cc(orc.Graph(g.num_vertices, g.pull, g.push), orc.options(engine=engine))
+    assert np.array_equal(got, labels)
+
+
+@pytest.mark.parametrize("hot", [16384, 20480])
+def test_sliced_blocks_at_scale_20(gz, device, hot):
+    # blocks of 20480 sources in two 2^16-vertex tiers: held in LDS with the
+    # 20480 cache, read from memory with the smaller one; PageRank equals the
+    # plain pull to rounding and CC labels equal the plain labels
+    scale = 20
+    gz.load_rmat_device(scale, 16 << scale, 78, relabel=True, slices_log2=16, slice_tiers=2, slice_blocks=2)
+    assert gz.source_slices_layout() == (1 << 16, 2, 2, gz.SLICE_BLOCK_VERTICES)
+    gz.run(gz.options("pr", iterations=10))
+    plain = gz.pr_ranks()
+    gz.run(gz.options("cc"))
+    cc_plain = gz.cc_labels()
+    gz.set_chunk_vectors(2048)
+    gz.slices_set_hot_cache(hot)
+    try:
+        gz.run(gz.options("pr", iterations=10, pull_variant="source_sliced"))
+        sliced = gz.pr_ranks()
         gz.run(gz.options("cc", pull_variant="source_sliced"))
         cc_sliced = gz.cc_labels()
     finally:

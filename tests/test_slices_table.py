@@ -2,8 +2,8 @@
 
 Properties the kernel relies on: every chunk of every slice is in exactly one
 slot, with its merge record numbered in slice order; a workgroup w only holds
-chunks of slices x = w % 8, 8 + x, ... (tier by tier, in order) or of the cold
-slice 8T; the XCDs get an even number of chunks whenever the cold slice can
+chunks of slices x = w % 8, 8 + x, ... (level by level, in order) or of the
+cold slice 8L; the XCDs get an even number of chunks whenever the cold slice can
 fill them; four slots per workgroup and empty slots marked by count 0.
 """
 import numpy as np
@@ -72,7 +72,7 @@ def _check(vectors, L, spread):
     return t
 
 
-@pytest.mark.parametrize("tiers", [1, 2, 3])
+@pytest.mark.parametrize("tiers", [1, 2, 3, 5, 7])
 @pytest.mark.parametrize("spread", [True, False])
 @pytest.mark.parametrize("L", [64, 512])
 def test_table_random(spread, L, tiers):
@@ -87,7 +87,8 @@ def test_table_random(spread, L, tiers):
 def test_table_rejects_bad_sizes():
     with pytest.raises(ValueError):
         gz.slices_chunk_table([1] * 10, 64)
-    assert len(gz.slices_chunk_table([1] * 33, 64)) == 0  # four tiers are not supported
+    assert len(gz.slices_chunk_table([1] * 65, 64)) == 0  # eight levels are not supported
+    assert len(gz.slices_chunk_table([1] * 57, 64)) > 0
 
 
 @pytest.mark.parametrize("spread", [True, False])
