@@ -24,7 +24,9 @@ HBM bytes per launch from the committed rocprofv3 counter summary
 
 Sub-objects, each one configuration of the workload set:
   cc_rmat22   CC, symmetrised R-MAT s22 + 64 planted 1000-vertex trees,
-              hybrid with the fixed E/5 threshold, time to solution
+              hybrid with the fixed E/5 threshold, time to solution with the
+              source-sliced pull on the renumbered graph (labels checked
+              against the plain pull's, which is reported beside it)
   bfs_rmat24  BFS from 64 seeded roots on R-MAT s24: fixed threshold versus
               the learned selector (selector_bfs.bin), depths checked equal
   batch_1024  1024 R-MAT graphs (scale 10..17 with P(s) ~ 1/s, edge factor
@@ -172,29 +174,64 @@ def full_cc_bfs(gz) -> dict:
     return out
 
 
-def cc_rmat22(gz) -> dict:
-    gz.load_workload_device("cc-rmat22-sym-trees")
-    V, E = gz.num_vertices(), gz.num_edges()
-    gz.run(gz.options("cc"))
+CC_SLICES_LOG2 = 19   # the CC configuration's sliced pull: one tier of 2^19-vertex windows
+CC_SLICE_TIERS = 1
+
+
+def _cc_timed(gz, variant: str):
+    """Median time to solution of 5 runs, then one more run with per-kernel
+    events (which add their own overhead) for the pull kernel's average."""
+    gz.run(gz.options("cc", pull_variant=variant))
     times, res = [], None
-    gz.kernel_timing(True)
     for _ in range(5):
-        res = gz.run(gz.options("cc"))
+        res = gz.run(gz.options("cc", pull_variant=variant))
         times.append(res.elapsed_ms)
+    gz.kernel_timing(True)
+    gz.run(gz.options("cc", pull_variant=variant))
     kt = gz.kernel_times()
     gz.kernel_timing(False)
-    ms = statistics.median(times)
-    pms, pn = kt["pull_cc"]
-    out = {"workload": "cc-rmat22-sym-trees", "V": V, "E": E, "time_to_solution_ms": round(ms, 3),
-           "iterations": res.iterations, "pull_iterations": res.pull_iterations,
-           "push_iterations": res.push_iterations, "gteps": round(E * res.iterations / (ms * 1e-3) / 1e9, 2)}
+    return statistics.median(times), res, kt["pull_cc"]
+
+
+def cc_rmat22(gz) -> dict:
+    # plain pull on the loaded ids, then the source-sliced pull on the graph
+    # renumbered by out-degree; labels (original ids) must be identical
+    gz.load_workload_device("cc-rmat22-sym-trees")
+    V, E = gz.num_vertices(), gz.num_edges()
+    ms_plain, res_plain, (pms, pn) = _cc_timed(gz, "sched_aware")
+    labels = gz.cc_labels()
+    B = pull_bytes(gz)
+    plain = {"time_to_solution_ms": round(ms_plain, 3), "iterations": res_plain.iterations,
+             "pull_iterations": res_plain.pull_iterations, "push_iterations": res_plain.push_iterations}
     if pn:
         avg = pms / pn
-        B = pull_bytes(gz)
+        plain["pull_cc_kernel"] = {"avg_launch_us": round(avg * 1e3, 2), "bytes_per_launch": B,
+                                   "achieved_GBs": round(B / (avg * 1e-3) / 1e9, 1),
+                                   "frac_hbm_peak": round(B / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                   "traffic": read_traffic("gz_k_edge_pull_cc", "cc-rmat22-sym-trees")}
+    gz.load_workload_device("cc-rmat22-sym-trees", relabel=True, slices_log2=CC_SLICES_LOG2,
+                            slice_tiers=CC_SLICE_TIERS)
+    gz.set_chunk_vectors(SLICED_CHUNK)
+    gz.slices_set_hot_cache(HOT_CACHE)
+    try:
+        ms, res, (sms, sn) = _cc_timed(gz, "source_sliced")
+        same = bool(np.array_equal(gz.cc_labels(), labels))
+    finally:
+        gz.set_chunk_vectors(512)
+        gz.slices_set_hot_cache(0)
+    if not same:
+        raise AssertionError("cc_rmat22: source-sliced labels differ from the plain pull's")
+    out = {"workload": "cc-rmat22-sym-trees", "V": V, "E": E,
+           "pull": f"source-sliced (renumbered, S=2^{CC_SLICES_LOG2}, LDS cache {HOT_CACHE}, chunk {SLICED_CHUNK})",
+           "time_to_solution_ms": round(ms, 3), "iterations": res.iterations,
+           "pull_iterations": res.pull_iterations, "push_iterations": res.push_iterations,
+           "gteps": round(E * res.iterations / (ms * 1e-3) / 1e9, 2), "labels_equal_plain": same,
+           "plain_pull": plain}
+    if sn:
+        avg = sms / sn
         out["pull_cc_kernel"] = {"avg_launch_us": round(avg * 1e3, 2), "bytes_per_launch": B,
                                  "achieved_GBs": round(B / (avg * 1e-3) / 1e9, 1),
-                                 "frac_hbm_peak": round(B / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                                 "traffic": read_traffic("gz_k_edge_pull_cc", "cc-rmat22-sym-trees")}
+                                 "frac_hbm_peak": round(B / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     return out
 
 

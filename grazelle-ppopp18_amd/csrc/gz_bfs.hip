@@ -126,6 +126,38 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_bfs_retire_wants(uint64_t* __re
         wants_info[w] &= ~next_frontier[w];
 }
 
+/* Pass A0 of the bottom-up step over the compacted destinations: each
+ * destination's first vector.  When it qualifies it is the destination's
+ * first qualifying vector, and pass A then loads none of the destination's
+ * other vectors (in a level with a large frontier, most destinations stop
+ * here, as the single thread of the reference stops at its first qualifying
+ * vector).  Runs on pass A's grid; the wave's counts go to its stats slot and
+ * pass A adds its own. */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_first(const gz_vec_t* __restrict__ vecs,
+                                                                     gz_slot_source src,
+                                                                     const uint64_t* __restrict__ has_info,
+                                                                     uint64_t* __restrict__ first_vector,
+                                                                     uint64_t* __restrict__ stats, uint32_t collect)
+{
+    const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
+    const uint64_t n = src.totals[0];
+    uint32_t vectors = 0, edges = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t d = src.list[i];
+        const uint64_t vi = src.offsets[d];
+        const gz_lanes v = gz_load_vector(vecs, vi);
+        vectors += 1;
+        if (gz_vec_has_frontier_source(v, has_info, edges)) first_vector[d] = vi;
+    }
+    if (collect) {
+        const uint64_t tv = gz_wave_sum_u64(vectors), te = gz_wave_sum_u64(edges);
+        if (gz_lane() == 0) {
+            stats[2 * gw] = tv;
+            stats[2 * gw + 1] = te;
+        }
+    }
+}
+
 /* Pass A of the bottom-up step. */
 template <bool kActive>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_find(const gz_vec_t* __restrict__ vecs,
@@ -152,10 +184,22 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_find(const gz_vec
             const uint64_t vi = gz_slot_step<kActive>(src, a, base, lane, valid, &entry);
             uint64_t d = ~0ull;
             bool cand = false;
-            if (valid) {
+            if (valid && kActive) {
+                /* the destination comes from the compacted list, so a vector
+                 * whose destination already has a smaller qualifying vector
+                 * (pass A0's first vector, or another chunk's) is not loaded;
+                 * pass A0 has tested every first vector */
+                d = src.list[entry];
+                const uint64_t known = __hip_atomic_load(&first_vector[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (known > vi && base + (uint64_t)lane != src.prefix[entry]) {
+                    const gz_lanes v = gz_load_vector(vecs, vi);
+                    vectors += 1;
+                    cand = gz_vec_has_frontier_source(v, has_info, edges);
+                }
+            } else if (valid) {
                 const gz_lanes v = gz_load_vector(vecs, vi);
                 d = gz_shared_vertex(v);
-                if (kActive || gz_bit_test(wants_info, d)) {
+                if (gz_bit_test(wants_info, d)) {
                     vectors += 1;
                     /* a smaller qualifying vector of d is already known (from
                      * another chunk): this one cannot win, skip its gathers */
@@ -183,8 +227,8 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_find(const gz_vec
     if (collect) {
         const uint64_t tv = gz_wave_sum_u64(vectors), te = gz_wave_sum_u64(edges);
         if (lane == 0) {
-            stats[2 * gw] = tv;
-            stats[2 * gw + 1] = te;
+            stats[2 * gw] = (kActive ? stats[2 * gw] : 0) + tv;
+            stats[2 * gw + 1] = (kActive ? stats[2 * gw + 1] : 0) + te;
         }
     }
 }
@@ -333,6 +377,12 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     const uint64_t level = gz_phase_config.bfs_level;
     const gz_feature_sink sink{gz_scratch.features_indeg ? gz_scratch.features : nullptr, gz_scratch.features_indeg};
     gz_ktimer_begin(GZ_K_PULL_BFS_FIND);
+    if (active) {
+        hipLaunchKernelGGL(gz_k_edge_pull_bfs_first, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
+                           src, gz_graph.frontier_has_info, gz_scratch.bfs_first_vector, gz_scratch.stats,
+                           gz_phase_config.collect_stats);
+        GZ_LAUNCH_CHECK();
+    }
     if (active)
         hipLaunchKernelGGL(gz_k_edge_pull_bfs_find<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
                            edge_list, src, chunk_len, gz_graph.frontier_has_info, gz_graph.frontier_wants_info,
