@@ -29,6 +29,10 @@
  *   -R                          device ingress of base-pull only, vertices renumbered
  *                               by out-degree (self-loops and duplicate edges dropped;
  *                               results and -r keep original ids)
+ *   -L k[,t[,g]]                -R, plus the source-sliced pull layout (windows of 2^k
+ *                               sources, t tiers, g LDS blocks) and the source-sliced pull
+ *                               for PR and CC, with the 20480-source LDS cache (no
+ *                               reference counterpart; chunks of 2048 vectors unless -s)
  */
 #include <ctype.h>
 #include <stdio.h>
@@ -52,14 +56,15 @@ typedef struct gz_cli_t {
     uint64_t sched_granularity;
     int device;
     int renumber;
+    uint32_t slice_log2, slice_tiers, slice_blocks;
     gz_run_options_t run;
 } gz_cli_t;
 
 static int gz_is_switch(char c) { return c == '-'; }
 
-static int gz_recognized(char c) { return strchr("hinNosVuaedrtTEXpSwDR", c) != NULL; }
+static int gz_recognized(char c) { return strchr("hinNosVuaedrtTEXpSwDRL", c) != NULL; }
 
-static int gz_requires_value(char c) { return strchr("inNousaedrtwD", c) != NULL; }
+static int gz_requires_value(char c) { return strchr("inNousaedrtwDL", c) != NULL; }
 
 static void gz_err(const char* argv0, const char* fmt, const char* a, const char* b, int code)
 {
@@ -93,6 +98,9 @@ static void gz_usage(const char* argv0)
     printf("  -D device\n        HIP device index. Defaults to 0.\n");
     printf("  -R\n        Build the graph on the device from input-graph-pull alone, vertices renumbered\n"
            "        by out-degree (drops self-loops and duplicate edges; output keeps original ids).\n");
+    printf("  -L k[,tiers[,blocks]]\n        As -R, plus the source-sliced pull for PageRank and CC: windows of 2^k sources\n"
+           "        (1..23), 1..3 tiers, 0..6 LDS blocks of %u sources (tiers + blocks <= 7).\n",
+           GZ_SLICE_BLOCK_VERTICES);
     printf("  -V\n        Prints version information and exits.\n");
     exit(0);
 }
@@ -214,6 +222,28 @@ static void gz_parse_option(gz_cli_t* c, const char* argv0, const char* opt, con
     case 'R':
         c->renumber = 1;
         break;
+    case 'L': {
+        /* k[,tiers[,blocks]] */
+        unsigned long long f[3] = {0, 1, 0};
+        int n = 0;
+        const char* p = val;
+        while (n < 3) {
+            if (!isdigit((unsigned char)*p)) gz_err(argv0, "%s: Invalid value `%s' for option `%s'.\n", val, opt, 2);
+            char* end = NULL;
+            f[n++] = strtoull(p, &end, 10);
+            p = end;
+            if (*p == '\0') break;
+            if (*p != ',' || n == 3) gz_err(argv0, "%s: Invalid value `%s' for option `%s'.\n", val, opt, 2);
+            p += 1;
+        }
+        if (f[0] < 1 || f[0] > 23 || f[1] < 1 || f[1] > 3 || f[2] > 6 || f[1] + f[2] > 7)
+            gz_err(argv0, "%s: Invalid value `%s' for option `%s'.\n", val, opt, 2);
+        c->slice_log2 = (uint32_t)f[0];
+        c->slice_tiers = (uint32_t)f[1];
+        c->slice_blocks = (uint32_t)f[2];
+        c->renumber = 1;
+        break;
+    }
     default:
         gz_err(argv0, "%s: Unrecognized option `%s'.%s\n", opt, "", 1);
     }
@@ -239,6 +269,8 @@ int gz_main(int argc, char** argv)
     if (c.run.edge_only && c.run.vertex_only) gz_err(argv0, "%s: Specified option combination is incompatible.%s%s\n", "", "", 6);
     if (c.run.engine == GZ_ENGINE_LEARNED && c.weights == NULL)
         gz_err(argv0, "%s: Specified option combination is incompatible.%s%s\n", "", "", 6);
+    if (c.slice_log2 && c.run.pull_variant == GZ_PULL_TRADITIONAL)
+        gz_err(argv0, "%s: Specified option combination is incompatible.%s%s\n", "", "", 6);
     c.run.iterations = c.iterations;
 
     /* unreadable inputs exit 255 before the device is touched (graphdata.c:963-967) */
@@ -256,18 +288,34 @@ int gz_main(int argc, char** argv)
         }
     }
     gz_runtime_init(c.device);
-    if (c.sched_granularity) gz_phase_config.chunk_vectors = (uint32_t)((c.sched_granularity + 63) / 64 * 64);
+    if (c.sched_granularity)
+        gz_phase_config.chunk_vectors = (uint32_t)((c.sched_granularity + 63) / 64 * 64);
+    else if (c.slice_log2)
+        gz_phase_config.chunk_vectors = 2048;
     if (c.renumber) {
         uint64_t nv = 0, ne = 0, *pairs = NULL;
         if (gz_edge_list_read(c.gather, &nv, &ne, &pairs) != 0) {
             fprintf(stderr, "Error: unable to read file \"%s\"\n", c.gather);
             exit(255);
         }
-        gz_graph_build_pairs_device(nv, ne, pairs, 0, GZ_INGRESS_RELABEL_BY_OUTDEGREE);
+        uint32_t flags = GZ_INGRESS_RELABEL_BY_OUTDEGREE;
+        if (c.slice_log2)
+            flags |= GZ_INGRESS_SOURCE_SLICES(c.slice_log2) | GZ_INGRESS_SLICE_TIERS(c.slice_tiers) |
+                     GZ_INGRESS_SLICE_BLOCKS(c.slice_blocks);
+        gz_graph_build_pairs_device(nv, ne, pairs, 0, flags);
         free(pairs);
         printf("Device ingress: %llu vertices, %llu edges, %llu pull / %llu push vectors, renumbered by out-degree\n",
                (unsigned long long)gz_graph_num_vertices(), (unsigned long long)gz_graph_num_edges(),
                (unsigned long long)gz_graph_pull_vector_count(), (unsigned long long)gz_graph_push_vector_count());
+        if (c.slice_log2) {
+            uint64_t S = 0, H = 0;
+            uint32_t T = 0, G = 0;
+            gz_graph_source_slices_info(NULL, &S, &T, &G, &H);
+            c.run.pull_variant = GZ_PULL_SOURCE_SLICED;
+            gz_slices_set_hot_cache(GZ_SLICE_BLOCK_VERTICES);
+            printf("Source-sliced pull: windows of %llu sources, %u tier(s), %u LDS block(s), LDS cache %u\n",
+                   (unsigned long long)S, T, G, GZ_SLICE_BLOCK_VERTICES);
+        }
     } else {
         graph_data_read_from_file(c.gather, need_push ? c.scatter : NULL, 1, NULL);
     }

@@ -209,3 +209,27 @@ def test_cli_output_matches_oracle(gz, orc, device, tmp_path, rmat14):
     _, ranks, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=20))
     got = [float(l.split()[1]) for l in open(out)]
     assert np.allclose(got, ranks, rtol=1e-5, atol=0)
+
+
+@pytest.mark.parametrize("spec", ["5", "4,2", "3,3"])
+def test_cli_source_sliced_matches_oracle(gz, orc, device, tmp_path, spec):
+    # -L: device ingress with renumbering and the source-sliced pull; output
+    # in original ids (labels exact, ranks to the output's 6 digits)
+    import subprocess, os
+    g = gz.rmat_graph(11, 8, seed=19)
+    base = str(tmp_path / "g")
+    gz.write_graph_files(base, g)
+    exe = os.path.join(os.path.dirname(gz.LIB_PATH), "grazelle")
+    out = str(tmp_path / "o.txt")
+    res = subprocess.run([exe, "-i", base, "-L", spec, "-a", "cc", "-o", out], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    assert "Source-sliced pull" in res.stdout
+    lab, _, _ = orc.cc(_orc_graph(orc, g), orc.options())
+    assert open(out).read() == "".join(f"{i} {float(x):.0f}\n" for i, x in enumerate(lab))
+    res = subprocess.run([exe, "-i", base, "-L", spec, "-N", "20", "-o", out], capture_output=True, text=True,
+                         timeout=120)
+    assert res.returncode == 0, res.stdout + res.stderr
+    _, ranks, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=20))
+    got = [float(l.split()[1]) for l in open(out)]
+    assert np.allclose(got, ranks, rtol=1e-5, atol=0)

@@ -56,6 +56,9 @@ def exe(gz):
     (["-s", "0", "-i", "g"], 2), (["-u", "3", "-i", "g"], 2), (["-a", "sssp", "-i", "g"], 2),
     (["-d", "1.5", "-i", "g"], 2), (["-i"], 3), (["-N"], 3), (["-p", "1", "-i", "g"], 4), ([], 5),
     (["-N", "3"], 5), (["-E", "-X", "-i", "g"], 6), (["-e", "learned", "-i", "g"], 6),
+    (["-L", "x", "-i", "g"], 2), (["-L", "0", "-i", "g"], 2), (["-L", "24", "-i", "g"], 2),
+    (["-L", "20,4", "-i", "g"], 2), (["-L", "20,2,6", "-i", "g"], 2), (["-L", "20,", "-i", "g"], 2),
+    (["-L", "20,1,1,1", "-i", "g"], 2), (["-L", "-i", "g"], 3), (["-L", "20", "-T", "-i", "g"], 6),
 ])
 def test_cli_error_codes(gz, args, code):
     r = subprocess.run([exe(gz)] + args, capture_output=True, text=True, timeout=60)
