@@ -645,6 +645,36 @@ __global__ void gz_k_pack_u32(const gz_vec_t* __restrict__ vecs, uint64_t n, gz_
     }
 }
 
+/* Vector-Sparse vectors of one block slice -> 8-byte packed vectors: four
+ * 16-bit offsets from the block's first vertex (GZ_PACKED16_NONE for an empty
+ * lane), bit 15 of the first set on the first vector of each destination. */
+__global__ void gz_k_pack_u16(const gz_vec_t* __restrict__ vecs, uint64_t n, uint64_t window,
+                              uint64_t* __restrict__ words)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        gz_lanes w;
+        w.l0 = vecs[j].lane[0];
+        w.l1 = vecs[j].lane[1];
+        w.l2 = vecs[j].lane[2];
+        w.l3 = vecs[j].lane[3];
+        bool first = true;
+        if (j > 0) {
+            gz_lanes p;
+            p.l0 = vecs[j - 1].lane[0];
+            p.l1 = vecs[j - 1].lane[1];
+            p.l2 = vecs[j - 1].lane[2];
+            p.l3 = vecs[j - 1].lane[3];
+            first = gz_shared_vertex(p) != gz_shared_vertex(w);
+        }
+        const uint64_t f0 = (gz_lane_id(w.l0) - window) | (first ? GZ_PACKED16_FIRST : 0u);
+        const uint64_t f1 = gz_lane_valid(w.l1) ? gz_lane_id(w.l1) - window : GZ_PACKED16_NONE;
+        const uint64_t f2 = gz_lane_valid(w.l2) ? gz_lane_id(w.l2) - window : GZ_PACKED16_NONE;
+        const uint64_t f3 = gz_lane_valid(w.l3) ? gz_lane_id(w.l3) - window : GZ_PACKED16_NONE;
+        words[j] = f0 | (f1 << 16) | (f2 << 32) | (f3 << 48);
+    }
+}
+
 /* Vector-Sparse vectors of one window slice -> 12-byte packed vectors: four
  * 24-bit offsets from the window's first vertex (empty lanes all ones), bit
  * 23 of the first set on the first vector of each destination. */
@@ -702,6 +732,11 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
     const uint64_t S = gz_slices_effective(slice_vertices, tiers, nv);
     /* as many of the requested blocks as fit in a window after its block 0 */
     const uint64_t H = blocks ? block_vertices : 0;
+    if (H > GZ_PACKED16_NONE) {
+        fprintf(stderr, "grazelle: LDS blocks hold at most %u sources (got %llu)\n", GZ_PACKED16_NONE,
+                (unsigned long long)H);
+        exit(65);
+    }
     if (blocks && (H == 0 || S / H < 1))
         blocks = 0;
     else if (blocks && S / H - 1 < blocks)
@@ -794,8 +829,12 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
     gz_slices.begin[0] = 0;
     for (uint32_t x = 0; x < ns; ++x) gz_slices.begin[x + 1] = gz_slices.begin[x] + counts[x];
     const uint64_t total = gz_slices.begin[ns];
-    const uint64_t hot_total = gz_slices.begin[ns - 1];
-    gz_slices.nbr24 = (uint32_t*)gz_device_malloc((hot_total ? 3 * hot_total : 1) * sizeof(uint32_t));
+    const uint64_t block_first = gz_slices.begin[GZ_XCDS];
+    const uint64_t block_span = gz_slices.begin[GZ_XCDS * (blocks + 1)] - block_first;
+    const uint64_t window_total = gz_slices.begin[ns - 1] - block_span;
+    gz_slices.nbr24 = (uint32_t*)gz_device_malloc((window_total ? 3 * window_total : 1) * sizeof(uint32_t));
+    gz_slices.nbr16 = (uint64_t*)gz_device_malloc((block_span ? block_span : 1) * sizeof(uint64_t));
+    gz_slices.block_span = block_span;
     gz_slices.nbr = (gz_u32x4*)gz_device_malloc((counts[ns - 1] ? counts[ns - 1] : 1) * sizeof(gz_u32x4));
     gz_slices.shared = (uint32_t*)gz_device_malloc((total ? total : 1) * sizeof(uint32_t));
     for (uint32_t x = 0; x < ns; ++x) {
@@ -805,10 +844,16 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
                                counts[x], x + 1 == ns ? gz_slices.nbr : (gz_u32x4*)nullptr,
                                gz_slices.shared + gz_slices.begin[x]);
             GZ_LAUNCH_CHECK();
-            if (x + 1 < ns) {
+            const uint32_t level = x / GZ_XCDS;
+            if (x + 1 < ns && level >= 1 && level <= blocks) {
+                hipLaunchKernelGGL(gz_k_pack_u16, dim3(gz_grid_for(counts[x])), dim3(GZ_BLOCK), 0, gz_stream,
+                                   part_vecs[x], counts[x], gz_slice_window(x, S, blocks) + (uint64_t)level * H,
+                                   gz_slices.nbr16 + (gz_slices.begin[x] - block_first));
+                GZ_LAUNCH_CHECK();
+            } else if (x + 1 < ns) {
+                const uint64_t at = gz_slices.begin[x] - (level > blocks ? block_span : 0);
                 hipLaunchKernelGGL(gz_k_pack_u24, dim3(gz_grid_for(counts[x])), dim3(GZ_BLOCK), 0, gz_stream,
-                                   part_vecs[x], counts[x], gz_slice_window(x, S, blocks),
-                                   gz_slices.nbr24 + 3 * gz_slices.begin[x]);
+                                   part_vecs[x], counts[x], gz_slice_window(x, S, blocks), gz_slices.nbr24 + 3 * at);
                 GZ_LAUNCH_CHECK();
             }
         }

@@ -20,6 +20,9 @@ typedef unsigned int gz_u32x4 __attribute__((ext_vector_type(4)));
  * prefix) and d, its destination or a flag while it is being loaded
  * (gz_packed_list, gz_walk.h). */
 #define GZ_PACKED_NONE 0xffffffffu
+/* 16-bit block offsets: empty lane, and the first-of-destination flag */
+#define GZ_PACKED16_NONE 0x7fffu
+#define GZ_PACKED16_FIRST 0x8000u
 struct gz_packed_vec {
     uint32_t s0, s1, s2, s3, d;
 };
@@ -133,9 +136,13 @@ typedef struct gz_source_slices_t {
     uint32_t blocks;                  /* G */
     uint64_t block_vertices;          /* H */
     uint32_t nslices;                 /* 8 (G + T) + 1 */
-    /* packed vectors (gz_walk.h): the hot slices' vectors [0, begin[nslices-1])
-     * as 24-bit window offsets, 12 bytes each, the last slice's as 32-bit ids */
-    uint32_t* nbr24;                  /* 3 words per hot vector */
+    /* packed vectors (gz_walk.h): the window slices' (levels 0 and G+1 ..) as
+     * 24-bit window offsets, 12 bytes each, the block slices' (levels 1..G) as
+     * 16-bit block offsets, 8 bytes each, the last slice's as 32-bit ids */
+    uint32_t* nbr24;                  /* 3 words per window vector: vector i of level 0 at i, of a later
+                                       * tier at i - block_span */
+    uint64_t* nbr16;                  /* block vector i at i - begin[8] */
+    uint64_t block_span;              /* vectors of the block slices, begin[8 (G+1)] - begin[8] */
     gz_u32x4* nbr;                    /* last slice: vector begin[nslices-1] + j at nbr[j] */
     uint32_t* shared;                 /* every vector's shared vertex (local rank) */
     uint64_t begin[GZ_MAX_SLICES + 1]; /* first vector of each slice */

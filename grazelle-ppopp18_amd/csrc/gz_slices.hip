@@ -148,6 +148,7 @@ extern "C" int gz_slices_set_hot_cache(uint32_t vertices)
 extern "C" void gz_source_slices_release(void)
 {
     gz_device_free(gz_slices.nbr24);
+    gz_device_free(gz_slices.nbr16);
     gz_device_free(gz_slices.nbr);
     gz_device_free(gz_slices.shared);
     gz_device_free(gz_slices.mask);
@@ -182,19 +183,34 @@ extern "C" int gz_graph_source_slices_download(gz_vec_t* vectors, uint64_t* mask
          * and the shared vertex, which is all a vector holds */
         const uint64_t n = gz_slices.begin[gz_slices.nslices];
         const uint64_t hot = gz_slices.begin[gz_slices.nslices - 1];
-        uint32_t* w24 = (uint32_t*)malloc((hot ? 3 * hot : 1) * sizeof(uint32_t));
+        const uint64_t span = gz_slices.block_span, bfirst = gz_slices.begin[GZ_XCDS];
+        const uint64_t windows = hot - span;
+        uint32_t* w24 = (uint32_t*)malloc((windows ? 3 * windows : 1) * sizeof(uint32_t));
+        uint64_t* w16 = (uint64_t*)malloc((span ? span : 1) * sizeof(uint64_t));
         uint32_t* nbr = (uint32_t*)malloc((n - hot ? n - hot : 1) * 4 * sizeof(uint32_t));
         uint32_t* shared = (uint32_t*)malloc((n ? n : 1) * sizeof(uint32_t));
-        gz_copy_to_host(w24, gz_slices.nbr24, 3 * hot * sizeof(uint32_t));
+        gz_copy_to_host(w24, gz_slices.nbr24, 3 * windows * sizeof(uint32_t));
+        gz_copy_to_host(w16, gz_slices.nbr16, span * sizeof(uint64_t));
         gz_copy_to_host(nbr, gz_slices.nbr, (n - hot) * 4 * sizeof(uint32_t));
         gz_copy_to_host(shared, gz_slices.shared, n * sizeof(uint32_t));
         uint32_t x = 0;
         for (uint64_t j = 0; j < n; ++j) {
             while (j >= gz_slices.begin[x + 1]) ++x;
+            const uint32_t level = x / GZ_XCDS;
             uint64_t ids[4];
             unsigned count = 0;
-            if (j < hot) {
-                const uint32_t* q = w24 + 3 * j;
+            if (j < hot && level >= 1 && level <= gz_slices.blocks) {
+                const uint64_t q = w16[j - bfirst];
+                const uint32_t f[4] = {(uint32_t)q & 0x7fffu, (uint32_t)(q >> 16) & 0xffffu,
+                                       (uint32_t)(q >> 32) & 0xffffu, (uint32_t)(q >> 48)};
+                const uint64_t window = gz_slice_window(x, gz_slices.slice_vertices, gz_slices.blocks) +
+                                        (uint64_t)level * gz_slices.block_vertices;
+                while (count < 4 && (count == 0 || f[count] != GZ_PACKED16_NONE)) {
+                    ids[count] = window + f[count];
+                    ++count;
+                }
+            } else if (j < hot) {
+                const uint32_t* q = w24 + 3 * (j - (level > gz_slices.blocks ? span : 0));
                 const uint32_t f[4] = {q[0] & 0x7fffffu, (q[0] >> 24) | ((q[1] & 0xffffu) << 8),
                                        (q[1] >> 16) | ((q[2] & 0xffu) << 16), q[2] >> 8};
                 const uint64_t window = gz_slice_window(x, gz_slices.slice_vertices, gz_slices.blocks);
@@ -212,6 +228,7 @@ extern "C" int gz_graph_source_slices_download(gz_vec_t* vectors, uint64_t* mask
             gz_vec_encode(&vectors[j], shared[j], ids, count);
         }
         free(w24);
+        free(w16);
         free(nbr);
         free(shared);
     }
@@ -269,20 +286,37 @@ struct gz_slice_bounds {
     uint64_t rec[GZ_MAX_SLICES + 1];
 };
 
-/* The packed slices as kernel arguments: the window and block slices in
- * 24-bit form, the last slice in 32-bit form. */
+/* The packed slices as kernel arguments: the window slices in 24-bit form,
+ * the block slices in 16-bit form, the last slice in 32-bit form. */
 struct gz_packed_lists {
     const uint32_t* nbr24;
+    const uint64_t* nbr16;
     const gz_u32x4* nbr;
     const uint32_t* shared;
     uint64_t cold_first;
+    uint64_t block_first; /* first vector of the block slices */
+    uint64_t block_span;  /* their vector count */
     uint64_t slice_vertices;
+    uint64_t block_vertices;
     uint32_t cold_slice;
     uint32_t blocks;
+    __device__ __forceinline__ bool is_block(uint32_t sl) const
+    {
+        const uint32_t level = sl / GZ_XCDS;
+        return sl != cold_slice && level >= 1 && level <= blocks;
+    }
     __device__ __forceinline__ gz_packed_list cold() const { return gz_packed_list{{shared}, nbr, cold_first}; }
     __device__ __forceinline__ gz_packed24_list window(uint32_t sl) const
     {
-        return gz_packed24_list{{shared}, nbr24, (uint32_t)gz_slice_window(sl, slice_vertices, blocks)};
+        return gz_packed24_list{{shared}, nbr24, (uint32_t)gz_slice_window(sl, slice_vertices, blocks),
+                                (sl / GZ_XCDS > blocks) ? block_span : 0ull};
+    }
+    __device__ __forceinline__ gz_packed16_list block(uint32_t sl) const
+    {
+        return gz_packed16_list{
+            {shared}, nbr16,
+            (uint32_t)(gz_slice_window(sl, slice_vertices, blocks) + (uint64_t)(sl / GZ_XCDS) * block_vertices),
+            block_first};
     }
 };
 
@@ -296,6 +330,10 @@ static gz_packed_lists gz_slices_lists()
     p.slice_vertices = gz_slices.slice_vertices;
     p.cold_slice = gz_slices.nslices - 1;
     p.blocks = gz_slices.blocks;
+    p.nbr16 = gz_slices.nbr16;
+    p.block_first = gz_slices.begin[GZ_XCDS];
+    p.block_span = gz_slices.block_span;
+    p.block_vertices = gz_slices.block_vertices;
     return p;
 }
 
@@ -354,6 +392,8 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr_sliced(gz_packed_l
     if (x == cold_slice)
         S = kColdStream ? gz_sliced_walk<VPL, true>(lists.cold(), b, e, props, acc, merge + rec, edges)
                         : gz_sliced_walk<VPL, false>(lists.cold(), b, e, props, acc, merge + rec, edges);
+    else if (lists.is_block(x))
+        S = gz_sliced_walk<VPL, false>(lists.block(x), b, e, props, acc, merge + rec, edges);
     else
         S = gz_sliced_walk<VPL, false>(lists.window(x), b, e, props, acc, merge + rec, edges);
     if (lane == 0) partials[slot] = S;
@@ -444,11 +484,23 @@ typedef __attribute__((address_space(1))) const double gz_global_f64;
  * branches share the result register, so each LDS read must wait for the
  * memory read issued before it and a wavefront holds one gather in flight
  * instead of four.) */
-template <bool kStream, typename T, typename GP, typename LP>
+template <bool kStream, bool kAllCached, typename T, typename GP, typename LP>
 __device__ __forceinline__ void gz_fetch4_hot(GP mem, LP hot, uint32_t wlo, uint32_t wn, const gz_packed_vec& v,
                                               bool in_range, T empty, T (&out)[4], uint32_t& edges)
 {
     const uint32_t s[4] = {v.s0, v.s1, v.s2, v.s3};
+    if constexpr (kAllCached) {
+        /* a block slice with its block in LDS: every source is in
+         * [wlo, wlo + wn), and nothing is read from memory */
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool ok = in_range && s[k] != GZ_PACKED_NONE;
+            const T h = hot[ok ? s[k] - wlo : 0u];
+            out[k] = ok ? h : empty;
+            edges += (uint32_t)ok;
+        }
+        return;
+    }
     bool ok[4], cached[4];
     uint32_t r[4];
     T m[4];
@@ -479,7 +531,7 @@ __device__ __forceinline__ void gz_fetch4_hot(GP mem, LP hot, uint32_t wlo, uint
     }
 }
 
-template <bool kStream>
+template <bool kStream, bool kAllCached = false>
 struct gz_pr_pull_op_hot {
     gz_global_f64* __restrict__ props;
     gz_lds_f64* hot;
@@ -492,7 +544,7 @@ struct gz_pr_pull_op_hot {
     __device__ __forceinline__ double gather(const gz_packed_vec& v, bool in_range, uint32_t& edges)
     {
         double g[4];
-        gz_fetch4_hot<kStream>(props, hot, wlo, wn, v, in_range, 0.0, g, edges);
+        gz_fetch4_hot<kStream, kAllCached>(props, hot, wlo, wn, v, in_range, 0.0, g, edges);
         const double t = (g[2] + g[3]) + (g[0] + g[1]);
         sum += t;
         return t;
@@ -553,18 +605,26 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
             if (level > cb && cb < G) gz_lds_advance(hot, props, lo, block_vertices, cb, level < G ? level : G);
             /* the cached range: block 0 for the window, the block itself for a
              * block slice (every source then hits LDS) */
-            uint32_t wlo = 0, wn = 0;
-            if (level == 0 && cb == 0) {
-                wlo = (uint32_t)lo;
-                wn = (uint32_t)hot_count;
-            } else if (level >= 1 && level == cb) {
-                wlo = (uint32_t)(lo + (uint64_t)level * block_vertices);
-                wn = (uint32_t)block_vertices;
+            double* acc = parts + part_base.off[sl];
+            if (lists.is_block(sl) && level == cb) {
+                gz_pr_pull_op_hot<false, true> op{gprops, lds, (uint32_t)(lo + (uint64_t)level * block_vertices),
+                                                  (uint32_t)block_vertices, acc, 0.0};
+                gz_walk_list<VPL, double, true, true>(lists.block(sl), c.begin, c.begin + c.count, op, first_d,
+                                                      carry_d, carry_v, edges);
+                sum = op.sum;
+            } else if (lists.is_block(sl)) { /* blocks larger than the cache */
+                gz_pr_pull_op_hot<false> op{gprops, lds, 0u, 0u, acc, 0.0};
+                gz_walk_list<VPL, double, true, true>(lists.block(sl), c.begin, c.begin + c.count, op, first_d,
+                                                      carry_d, carry_v, edges);
+                sum = op.sum;
+            } else {
+                const bool cached = level == 0 && cb == 0;
+                gz_pr_pull_op_hot<false> op{gprops, lds, cached ? (uint32_t)lo : 0u,
+                                            cached ? (uint32_t)hot_count : 0u, acc, 0.0};
+                gz_walk_list<VPL, double, true, true>(lists.window(sl), c.begin, c.begin + c.count, op, first_d,
+                                                      carry_d, carry_v, edges);
+                sum = op.sum;
             }
-            gz_pr_pull_op_hot<false> op{gprops, lds, wlo, wn, parts + part_base.off[sl], 0.0};
-            gz_walk_list<VPL, double, true, true>(lists.window(sl), c.begin, c.begin + c.count, op, first_d, carry_d,
-                                                  carry_v, edges);
-            sum = op.sum;
         }
         const double S = gz_wave_sum(sum);
         if (lane == 0) {
@@ -630,6 +690,7 @@ static void gz_launch_hot(const gz_xcd_lens& lens, uint64_t hot_count)
 typedef __attribute__((address_space(3))) const uint64_t gz_lds_u64;
 typedef __attribute__((address_space(1))) const uint64_t gz_global_u64;
 
+template <bool kAllCached = false>
 struct gz_cc_pull_op_hot {
     gz_global_u64* __restrict__ in;      /* labels, read */
     gz_lds_u64* hot;
@@ -645,7 +706,7 @@ struct gz_cc_pull_op_hot {
     __device__ __forceinline__ uint64_t gather(const gz_packed_vec& v, bool in_range, uint32_t& edges) const
     {
         uint64_t x[4];
-        gz_fetch4_hot<false>(in, hot, wlo, wn, v, in_range, ~(uint64_t)0, x, edges);
+        gz_fetch4_hot<false, kAllCached>(in, hot, wlo, wn, v, in_range, ~(uint64_t)0, x, edges);
         const uint64_t a = x[0] < x[1] ? x[0] : x[1], b = x[2] < x[3] ? x[2] : x[3];
         return a < b ? a : b;
     }
@@ -661,6 +722,18 @@ struct gz_cc_pull_op_hot {
         }
     }
 };
+
+/* One chunk of the CC pull with operator `op` (by value; its runs apply
+ * their minima); the chunk's last run is applied here.  Returns the vote. */
+template <bool kAll, typename L>
+__device__ __forceinline__ uint64_t gz_cc_walk(const L& list, const gz_slice_chunk_t& c, gz_cc_pull_op_hot<kAll> op,
+                                               uint32_t& edges)
+{
+    uint64_t first_d, carry_d, carry_m;
+    gz_walk_list<1, uint64_t, true, true>(list, c.begin, c.begin + c.count, op, first_d, carry_d, carry_m, edges);
+    if (gz_lane() == 0) op.store(carry_d, carry_m);
+    return op.vote;
+}
 
 template <int HOT>
 __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_simd) void gz_k_edge_pull_cc_sliced_hot(
@@ -705,25 +778,21 @@ __global__ __launch_bounds__(gz_hot_wg<HOT>::value, gz_hot_wg<HOT>::waves_per_si
                 wn = (uint32_t)block_vertices;
             }
         }
-        gz_cc_pull_op_hot op{(gz_global_u64*)labels,
-                             (gz_lds_u64*)hot,
-                             wlo,
-                             wn,
-                             labels,
-                             next_frontier,
-                             outdeg,
-                             gid + part_base.off[sl],
-                             stat_mode,
-                             0};
-        uint64_t first_d, carry_d, carry_m;
-        if (sl == lists.cold_slice)
-            gz_walk_list<1, uint64_t, true, true>(lists.cold(), c.begin, c.begin + c.count, op, first_d, carry_d, carry_m,
-                                            edges);
-        else
-            gz_walk_list<1, uint64_t, true, true>(lists.window(sl), c.begin, c.begin + c.count, op, first_d, carry_d,
-                                            carry_m, edges);
-        if (lane == 0) op.store(carry_d, carry_m);
-        vote += op.vote;
+        const gz_cc_pull_op_hot<false> op{(gz_global_u64*)labels, (gz_lds_u64*)hot, wlo, wn, labels, next_frontier,
+                                          outdeg, gid + part_base.off[sl], stat_mode, 0};
+        if (sl == lists.cold_slice) {
+            vote += gz_cc_walk(lists.cold(), c, op, edges);
+        } else if (lists.is_block(sl)) {
+            if (level == cb) {
+                const gz_cc_pull_op_hot<true> all{op.in, op.hot, wlo, wn, labels, next_frontier,
+                                                  outdeg, op.gid, stat_mode, 0};
+                vote += gz_cc_walk(lists.block(sl), c, all, edges);
+            } else {
+                vote += gz_cc_walk(lists.block(sl), c, op, edges);
+            }
+        } else {
+            vote += gz_cc_walk(lists.window(sl), c, op, edges);
+        }
         vectors += c.count;
     }
     gz_lds_advance(hot, labels, lo, block_vertices, cb, G);

@@ -249,17 +249,19 @@ struct gz_packed_list : gz_flagged_dsts {
     }
 };
 
-/* 12-byte vectors of one window: words w[3i .. 3i+2] hold the offsets
- * f0 | f1 << 24 | f2 << 48 | f3 << 72 (little-endian, 96 bits). */
+/* 12-byte vectors of one window: words w[3j .. 3j+2], j = i - first, hold
+ * the offsets f0 | f1 << 24 | f2 << 48 | f3 << 72 (little-endian, 96 bits). */
 struct gz_packed24_list : gz_flagged_dsts {
     const uint32_t* __restrict__ w;
     uint32_t window; /* the window's first vertex */
+    uint64_t first;  /* vector index stored at w[0] */
     typedef gz_packed_vec vec;
     __device__ __forceinline__ vec load(uint64_t i) const
     {
-        const uint32_t w0 = __builtin_nontemporal_load(w + 3 * i);
-        const uint32_t w1 = __builtin_nontemporal_load(w + 3 * i + 1);
-        const uint32_t w2 = __builtin_nontemporal_load(w + 3 * i + 2);
+        const uint64_t j = i - first;
+        const uint32_t w0 = __builtin_nontemporal_load(w + 3 * j);
+        const uint32_t w1 = __builtin_nontemporal_load(w + 3 * j + 1);
+        const uint32_t w2 = __builtin_nontemporal_load(w + 3 * j + 2);
         const uint32_t f0 = w0 & 0xffffffu;
         const uint32_t f1 = (w0 >> 24) | ((w1 & 0xffffu) << 8);
         const uint32_t f2 = (w1 >> 16) | ((w2 & 0xffu) << 16);
@@ -270,6 +272,29 @@ struct gz_packed24_list : gz_flagged_dsts {
         v.s2 = (f2 == GZ_PACKED24_NONE) ? GZ_PACKED_NONE : window + f2;
         v.s3 = (f3 == GZ_PACKED24_NONE) ? GZ_PACKED_NONE : window + f3;
         v.d = f0 >> 23;
+        return v;
+    }
+};
+
+/* 8-byte vectors of one LDS block: w[i - first] holds four 16-bit offsets
+ * from the block's first vertex (GZ_PACKED16_NONE for an empty lane), bit 15
+ * of the first set on the first vector of each destination. */
+struct gz_packed16_list : gz_flagged_dsts {
+    const uint64_t* __restrict__ w;
+    uint32_t window; /* the block's first vertex */
+    uint64_t first;
+    typedef gz_packed_vec vec;
+    __device__ __forceinline__ vec load(uint64_t i) const
+    {
+        const uint64_t x = __builtin_nontemporal_load(w + (i - first));
+        const uint32_t f0 = (uint32_t)x & 0xffffu, f1 = (uint32_t)(x >> 16) & 0xffffu,
+                       f2 = (uint32_t)(x >> 32) & 0xffffu, f3 = (uint32_t)(x >> 48);
+        vec v;
+        v.s0 = window + (f0 & ~GZ_PACKED16_FIRST);
+        v.s1 = (f1 == GZ_PACKED16_NONE) ? GZ_PACKED_NONE : window + f1;
+        v.s2 = (f2 == GZ_PACKED16_NONE) ? GZ_PACKED_NONE : window + f2;
+        v.s3 = (f3 == GZ_PACKED16_NONE) ? GZ_PACKED_NONE : window + f3;
+        v.d = f0 >> 15;
         return v;
     }
 };
