@@ -283,7 +283,7 @@ def test_sliced_matches_plain_at_scale_20(gz, device):
 
 
 @pytest.mark.parametrize("hot", [0, 4096])
-@pytest.mark.parametrize("vpl", [1, 2])
+@pytest.mark.parametrize("vpl", [1, 2, 4])
 @pytest.mark.parametrize("chunk", [64, 512])
 @pytest.mark.parametrize("S,T,G,H", [(2048, 1, 3, 64), (1024, 2, 4, 100), (512, 3, 4, 64), (256, 1, 6, 32)])
 def test_sliced_pr_blocks_parity(gz, orc, device, rmat14, S, T, G, H, chunk, vpl, hot):
