@@ -435,10 +435,14 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_merge_sliced(const mergeaccum_t
  * permitting) each copy the first `hot` properties of their XCD's tier-0
  * window into LDS; their wavefronts then take that XCD's chunks from a
  * per-XCD counter.  Gathers that fall in the cached range read LDS instead of
- * issuing an L2 request.  Workgroups have 1024 threads, so that 16
- * wavefronts share one cache; a cache of at most 64 KiB leaves room for two
- * workgroups per CU, and the kernel is then held to the registers that allow
- * both. */
+ * issuing an L2 request.  When the layout has LDS blocks no larger than the
+ * cache, the workgroup moves its LDS on to block b once its wavefronts reach
+ * block b's chunks (the queue holds them level by level), and a block
+ * slice's gathers then all read LDS.  Workgroups have 1024 threads, so that
+ * 16 wavefronts share one cache; a cache of at most 64 KiB leaves room for
+ * two workgroups per CU, and the kernel is then held to the registers that
+ * allow both. */
+
 /* Move a workgroup's LDS copy on to block `target` of its XCD's tier-0
  * window, block by block: a barrier (every wavefront is done with the block
  * held so far), the copy of the next block's H values, a barrier (the copy is
