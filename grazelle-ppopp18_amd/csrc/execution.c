@@ -294,8 +294,24 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
         gz_sel_dev_begin(V, E, root, indeg);
         gz_sel_dev_step(gz_graph.frontier_has_info, indeg, gz_reduce_buffer, learned, bfs, 1, 0, 0.0f,
                         bfs ? GZ_SEL_FROM_ROOT : GZ_SEL_FROM_BITMAP, 0);
-        (void)phase_op_combine_global_var_from_buf(gz_reduce_buffer);
-        gz_selector_read(&sel);
+        if (learned && bfs && root < V) {
+            /* the first frontier is the root alone, so the host knows the
+             * features and takes the first decision without waiting for the
+             * device step (which computes the same and advances its counts) */
+            const double* od = gz_graph_outdegrees_host();
+            sel.features[0] = logf(1.f + 1.f);
+            sel.features[1] = logf(1.f + (float)(uint64_t)od[root]);
+            sel.features[2] = logf(1.f + (float)(V - 1));
+            sel.features[3] = logf(1.f + (float)(E - gz_graph_indegree_host(root)));
+            sel.features[4] = 1.f;
+            sel.features[5] = 0.f;
+            sel.features[6] = 0.f;
+            sel.features[7] = V ? (float)E / (float)V : 0.f;
+            sel.choice = gz_selector_host_decide(sel.features, sel.predicted);
+        } else {
+            (void)phase_op_combine_global_var_from_buf(gz_reduce_buffer);
+            gz_selector_read(&sel);
+        }
         /* BFS activates each vertex once: its edge phases fold the statistics
          * of the next frontier into their vote pass */
         if (bfs) gz_features_fold(indeg);

@@ -255,6 +255,7 @@ typedef struct gz_host_graph_t {
     uint64_t* original_ids;      /* host mirror of the above */
     uint64_t* device_ids;        /* device vertex by original id */
     uint32_t* d_indeg;           /* in-degrees, built on first use */
+    uint32_t* h_indeg;           /* host copy of d_indeg, made on first use */
 } gz_host_graph_t;
 
 static gz_host_graph_t G;
@@ -291,6 +292,7 @@ void gz_graph_release(void)
     gz_device_free(G.d_wants);
     gz_device_free(G.d_original_ids);
     gz_device_free(G.d_indeg);
+    free(G.h_indeg);
     free(G.original_ids);
     free(G.device_ids);
     free(G.outdeg);
@@ -377,6 +379,16 @@ const uint32_t* gz_graph_indegrees_device(void)
             gz_dev_indegrees(G.d_push, G.push_vectors, 0, G.d_indeg, G.num_vertices);
     }
     return G.d_indeg;
+}
+
+uint32_t gz_graph_indegree_host(uint64_t v)
+{
+    if (G.h_indeg == NULL && G.num_vertices) {
+        const uint32_t* d = gz_graph_indegrees_device();
+        G.h_indeg = (uint32_t*)malloc(G.num_vertices * sizeof(uint32_t));
+        gz_copy_to_host(G.h_indeg, d, G.num_vertices * sizeof(uint32_t));
+    }
+    return (v < G.num_vertices) ? G.h_indeg[v] : 0u;
 }
 
 uint64_t gz_graph_device_vertex(uint64_t original)

@@ -39,6 +39,13 @@ void gz_graph_reset_roles(void);
 void gz_dev_indegrees(const gz_vec_t* vecs, uint64_t count, int pull_layout, uint32_t* indeg, uint64_t n);
 /* In-degrees of the loaded graph (u32[V+8]), built on first use. */
 const uint32_t* gz_graph_indegrees_device(void);
+/* In-degree of device vertex v from a host copy of the array above (made on
+ * first use). */
+uint32_t gz_graph_indegree_host(uint64_t v);
+/* The loaded selector's forward pass on the host, in the device kernel's
+ * order of operations: features x[8] -> predictions y[2]; returns 1 (pull)
+ * or 2 (push), 0 without weights. */
+uint32_t gz_selector_host_decide(const float* x, float* y);
 
 /* The 8 reduce-buffer words as read by the last
  * phase_op_combine_global_var_from_buf (pinned host memory). */

@@ -2,13 +2,16 @@
  * selector.c -- host side of the learned engine selector: weight storage and
  * files.  Features, the forward pass and the decision run on the device
  * (selector_kernels.hip); the drivers read the decision together with the
- * converge vote.
+ * converge vote.  The first BFS decision, whose features the host knows,
+ * can also be taken here (gz_selector_host_decide).
  */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include "grazelle_selector.h"
+#include "gz_host_internal.h"
 
 int gz_selector_device_upload(const float* params);
 int gz_selector_device_upload_norm(const float* mean, const float* stdev);
@@ -53,4 +56,37 @@ int gz_selector_save_file(const char* path)
              fwrite(gz_sel_std, sizeof(float), GZ_SEL_IN, f) == GZ_SEL_IN;
     fclose(f);
     return ok ? 0 : -2;
+}
+
+uint32_t gz_selector_host_decide(const float* x, float* y)
+{
+    if (!gz_sel_loaded) return 0;
+    /* parameter layout W1 b1 W2 b2 W3 b3, as on the device */
+    const float* W1 = gz_sel_params;
+    const float* b1 = W1 + GZ_SEL_H * GZ_SEL_IN;
+    const float* W2 = b1 + GZ_SEL_H;
+    const float* b2 = W2 + GZ_SEL_H * GZ_SEL_H;
+    const float* W3 = b2 + GZ_SEL_H;
+    const float* b3 = W3 + GZ_SEL_OUT * GZ_SEL_H;
+    float xn[GZ_SEL_IN], h1[GZ_SEL_H], h2[GZ_SEL_H];
+    for (int i = 0; i < GZ_SEL_IN; ++i) {
+        const float sd = gz_sel_std[i];
+        xn[i] = (x[i] - gz_sel_mean[i]) / (sd > 0.f ? sd : 1.f);
+    }
+    for (int j = 0; j < GZ_SEL_H; ++j) {
+        float a = b1[j];
+        for (int i = 0; i < GZ_SEL_IN; ++i) a += W1[j * GZ_SEL_IN + i] * xn[i];
+        h1[j] = a > 0.f ? a : 0.f;
+    }
+    for (int j = 0; j < GZ_SEL_H; ++j) {
+        float a = b2[j];
+        for (int i = 0; i < GZ_SEL_H; ++i) a += W2[j * GZ_SEL_H + i] * h1[i];
+        h2[j] = a > 0.f ? a : 0.f;
+    }
+    for (int j = 0; j < GZ_SEL_OUT; ++j) {
+        float a = b3[j];
+        for (int i = 0; i < GZ_SEL_H; ++i) a += W3[j * GZ_SEL_H + i] * h2[i];
+        y[j] = a;
+    }
+    return (y[0] < y[1]) ? 1u : 2u;
 }
