@@ -222,6 +222,68 @@ static void gz_swap_frontiers(void)
     gz_graph.vertex_accumulators = t;
 }
 
+static int gz_pr_replay_on = 1;
+
+void gz_pr_graph_replay(int on) { gz_pr_replay_on = on ? 1 : 0; }
+
+typedef struct gz_pr_iteration_t {
+    int use_push;
+    uint32_t pull_variant;
+} gz_pr_iteration_t;
+
+/* One PageRank iteration's launches, as the loop below makes them. */
+static void gz_pr_enqueue_iteration(void* p)
+{
+    const gz_pr_iteration_t* c = (const gz_pr_iteration_t*)p;
+    phase_op_reset_global_accum();
+    if (!c->use_push) {
+        perform_edge_pull_phase_pr(gz_graph.pull_vectors, gz_graph.pull_vector_count);
+        if (c->pull_variant == GZ_PULL_SCHED_AWARE)
+            edge_pull_op_merge_with_merge_buffer(gz_merge_buffer, gz_pull_units_for(gz_graph.pull_vector_count),
+                                                 (double*)gz_graph.vertex_accumulators);
+    } else {
+        perform_edge_push_phase_pr(gz_graph.push_vectors, gz_graph.push_vector_count);
+    }
+    phase_op_write_global_accum_to_buf(gz_reduce_buffer);
+    perform_vertex_phase_pr(0, gz_graph.num_vertices, gz_reduce_buffer);
+}
+
+/* Everything the launches of one iteration depend on besides the device
+ * allocations: the engine, the phase configuration, the graph record and
+ * the sliced pull's settings. */
+static size_t gz_pr_iteration_key(const gz_pr_iteration_t* c, uint64_t* k)
+{
+    size_t n = 0;
+    uint64_t damping;
+    memcpy(&damping, &gz_phase_config.damping, sizeof(damping));
+    uint32_t sl[3];
+    gz_slices_settings(sl);
+    k[n++] = (uint64_t)c->use_push;
+    k[n++] = c->pull_variant;
+    k[n++] = damping;
+    k[n++] = gz_phase_config.stat_mode;
+    k[n++] = gz_phase_config.pull_variant;
+    k[n++] = gz_phase_config.chunk_vectors;
+    k[n++] = gz_phase_config.collect_stats;
+    k[n++] = gz_phase_config.push_frontier_skip;
+    k[n++] = gz_phase_config.vectors_per_lane;
+    k[n++] = gz_graph.num_vertices;
+    k[n++] = (uint64_t)(uintptr_t)gz_graph.pull_vectors;
+    k[n++] = gz_graph.pull_vector_count;
+    k[n++] = (uint64_t)(uintptr_t)gz_graph.push_vectors;
+    k[n++] = gz_graph.push_vector_count;
+    k[n++] = (uint64_t)(uintptr_t)gz_graph.vertex_outdegrees;
+    k[n++] = (uint64_t)(uintptr_t)gz_graph.vertex_props;
+    k[n++] = (uint64_t)(uintptr_t)gz_graph.vertex_accumulators;
+    k[n++] = (uint64_t)(uintptr_t)gz_reduce_buffer;
+    k[n++] = (uint64_t)(uintptr_t)gz_merge_buffer;
+    k[n++] = gz_merge_buffer_count;
+    k[n++] = sl[0];
+    k[n++] = sl[1];
+    k[n++] = sl[2];
+    return n;
+}
+
 static void gz_run_pr(const gz_run_options_t* o, gz_run_result_t* r)
 {
     const uint64_t V = gz_graph.num_vertices;
@@ -229,8 +291,26 @@ static void gz_run_pr(const gz_run_options_t* o, gz_run_result_t* r)
     if (use_push && gz_graph.push_vectors == NULL) use_push = 0;
     if (!use_push && gz_graph.pull_vectors == NULL) use_push = 1;
     gz_timer_t* t = o->profile ? gz_timer_create() : NULL;
+    /* with nothing read back per iteration, iterations 2.. replay a graph of
+     * one iteration's launches (the first runs eagerly and makes every
+     * allocation the phases need) */
+    const int replay = gz_pr_replay_on && t == NULL && !o->collect_stats && !o->edge_only && !o->vertex_only &&
+                       !gz_kernel_timing_enabled() && o->iterations > 1 && o->iterations - 1 <= 0xffffffffull;
     uint64_t ctr;
     for (ctr = 0; ctr < o->iterations; ++ctr) {
+        if (replay && ctr == 1) {
+            gz_pr_iteration_t c = {use_push, o->pull_variant};
+            uint64_t key[32];
+            const size_t n = gz_pr_iteration_key(&c, key);
+            const uint64_t rest = o->iterations - 1;
+            gz_graph_replay_cached(key, n, gz_pr_enqueue_iteration, &c, (uint32_t)rest);
+            if (use_push)
+                r->push_iterations += rest;
+            else
+                r->pull_iterations += rest;
+            ctr = o->iterations;
+            break;
+        }
         gz_iteration_record_t rec;
         memset(&rec, 0, sizeof(rec));
         rec.iteration = ctr + 1;

@@ -5,6 +5,7 @@
 #ifndef GZ_HOST_INTERNAL_H
 #define GZ_HOST_INTERNAL_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "grazelle_types.h"
@@ -46,6 +47,13 @@ uint32_t gz_graph_indegree_host(uint64_t v);
  * order of operations: features x[8] -> predictions y[2]; returns 1 (pull)
  * or 2 (push), 0 without weights. */
 uint32_t gz_selector_host_decide(const float* x, float* y);
+/* Launch one graph of `times` rounds of the launches enqueue(ctx) makes on
+ * the phase stream, captured again when key[0..n), `times` or the device
+ * allocations changed since the last capture (enqueue must not allocate or
+ * wait). */
+void gz_graph_replay_cached(const uint64_t* key, size_t n, void (*enqueue)(void*), void* ctx, uint32_t times);
+/* The source-sliced pull's settings: hot cache, cold spread, cold stream. */
+void gz_slices_settings(uint32_t* out3);
 
 /* The 8 reduce-buffer words as read by the last
  * phase_op_combine_global_var_from_buf (pinned host memory). */

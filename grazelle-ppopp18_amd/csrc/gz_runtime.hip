@@ -79,9 +79,14 @@ extern "C" void* gz_runtime_stream(void)
     return (void*)gz_stream;
 }
 
+/* Bumped by every device allocation and free: a captured launch sequence is
+ * stale once any buffer may have moved. */
+static uint64_t gz_alloc_generation = 0;
+
 extern "C" void* gz_device_malloc(size_t bytes)
 {
     gz_require_runtime();
+    ++gz_alloc_generation;
     void* p = nullptr;
     if (bytes == 0) bytes = 256;
     hipError_t err = hipMalloc(&p, bytes);
@@ -95,7 +100,61 @@ extern "C" void* gz_device_malloc(size_t bytes)
 
 extern "C" void gz_device_free(void* ptr)
 {
-    if (ptr != nullptr) GZ_CHECK(hipFree(ptr));
+    if (ptr != nullptr) {
+        ++gz_alloc_generation;
+        GZ_CHECK(hipFree(ptr));
+    }
+}
+
+/* ---- replay of a captured launch sequence ----------------------------------
+ * One cached executable graph: `times` rounds of the launches enqueue() makes
+ * on the phase stream, captured once into one graph (one graph launch costs
+ * more here than the few kernel launches of a round, so the rounds are not
+ * replayed one launch each) and launched again while the caller's key, the
+ * round count and the allocation generation stay the same.  enqueue() must
+ * neither allocate nor synchronise (the capture would fail, and the process
+ * exits with the HIP error). */
+#define GZ_REPLAY_KEY_WORDS 64
+static struct {
+    hipGraphExec_t exec;
+    uint64_t generation;
+    uint32_t times;
+    size_t n;
+    uint64_t key[GZ_REPLAY_KEY_WORDS];
+} gz_replay;
+
+static void gz_replay_release(void)
+{
+    if (gz_replay.exec != nullptr) GZ_CHECK(hipGraphExecDestroy(gz_replay.exec));
+    memset(&gz_replay, 0, sizeof(gz_replay));
+}
+
+extern "C" void gz_graph_replay_cached(const uint64_t* key, size_t n, void (*enqueue)(void*), void* ctx,
+                                       uint32_t times)
+{
+    gz_require_runtime();
+    if (times == 0) return;
+    if (n > GZ_REPLAY_KEY_WORDS) {
+        fprintf(stderr, "grazelle: replay key of %zu words (at most %d)\n", n, GZ_REPLAY_KEY_WORDS);
+        exit(72);
+    }
+    const bool hit = gz_replay.exec != nullptr && gz_replay.generation == gz_alloc_generation &&
+                     gz_replay.times == times && gz_replay.n == n &&
+                     memcmp(gz_replay.key, key, n * sizeof(uint64_t)) == 0;
+    if (!hit) {
+        gz_replay_release();
+        hipGraph_t g = nullptr;
+        GZ_CHECK(hipStreamBeginCapture(gz_stream, hipStreamCaptureModeThreadLocal));
+        for (uint32_t i = 0; i < times; ++i) enqueue(ctx);
+        GZ_CHECK(hipStreamEndCapture(gz_stream, &g));
+        GZ_CHECK(hipGraphInstantiate(&gz_replay.exec, g, nullptr, nullptr, 0));
+        GZ_CHECK(hipGraphDestroy(g));
+        gz_replay.generation = gz_alloc_generation;
+        gz_replay.times = times;
+        gz_replay.n = n;
+        memcpy(gz_replay.key, key, n * sizeof(uint64_t));
+    }
+    GZ_CHECK(hipGraphLaunch(gz_replay.exec, gz_stream));
 }
 
 extern "C" void gz_copy_to_device(void* dst, const void* src, size_t bytes)
@@ -523,6 +582,7 @@ extern "C" void gz_runtime_shutdown(void)
 {
     if (gz_stream == nullptr) return;
     GZ_CHECK(hipStreamSynchronize(gz_stream));
+    gz_replay_release();
     gz_device_free(gz_scratch.partials);
     gz_device_free(gz_scratch.stats);
     gz_device_free(gz_scratch.features);

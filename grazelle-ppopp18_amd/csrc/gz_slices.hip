@@ -14,8 +14,9 @@
  *
  * Each slice is an ordinary pull list whose destinations are numbered
  * densely inside the slice, stored packed (24-bit window offsets in the
- * window slices, 32-bit ids in the last; destinations implied by flags: 12
- * or 16 bytes per vector instead of 32), walked exactly as in gz_pr.hip
+ * window slices, 16-bit block offsets in the block slices, 32-bit ids in the
+ * last; destinations implied by flags: 12, 8 or 16 bytes per vector instead
+ * of 32), walked exactly as in gz_pr.hip
  * into its own accumulator run (complete runs stored directly, the chunk's last run into
  * its merge record).  The merge then runs per slice
  * (edge_pull_op_merge_with_merge_buffer's rule, phases.c:41-70), and the
@@ -143,6 +144,13 @@ extern "C" int gz_slices_set_hot_cache(uint32_t vertices)
     if (vertices != 0 && vertices != 4096 && vertices != 8192 && vertices != 16384 && vertices != 20480) return -1;
     gz_slices_hot_cache = vertices;
     return 0;
+}
+
+extern "C" void gz_slices_settings(uint32_t* out3)
+{
+    out3[0] = gz_slices_hot_cache;
+    out3[1] = gz_slices_cold_spread;
+    out3[2] = gz_slices_cold_stream;
 }
 
 extern "C" void gz_source_slices_release(void)

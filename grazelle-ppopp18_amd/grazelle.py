@@ -96,6 +96,7 @@ _sig("gz_graph_build_pairs_device", C.c_int, u64, u64, vp, C.c_int, u32)
 _sig("gz_graph_build_rmat_device", C.c_int, P(RmatParams), u64, C.c_int, u64, vp, u32)
 _sig("gz_graph_device_vertex", u64, u64)
 _sig("gz_graph_build_source_slices", C.c_int, u64, u32, u32, u64)
+_sig("gz_pr_graph_replay", None, C.c_int)
 _sig("gz_graph_source_slices_info", C.c_int, vp, P(u64), P(u32), P(u32), P(u64))
 _sig("gz_slices_chunk_table", u64, vp, u32, u64, u32, vp, u64)
 _sig("gz_slices_set_cold_spread", None, u32)
@@ -417,6 +418,12 @@ def slices_set_hot_cache(vertices: int) -> None:
 
 def set_chunk_vectors(n: int) -> None:
     phase_config.chunk_vectors = n
+
+
+def pr_graph_replay(on: bool) -> None:
+    """PageRank iterations 2.. from a replayed HIP graph (default on)."""
+    init()
+    lib.gz_pr_graph_replay(int(on))
 
 
 def set_vectors_per_lane(n: int) -> None:

@@ -164,6 +164,11 @@ void gz_slices_set_cold_stream(uint32_t cold_stream);
  * another value. */
 int gz_slices_set_hot_cache(uint32_t vertices);
 
+/* PageRank runs replay iterations 2.. from a HIP graph of one iteration's
+ * launches when no per-iteration record, statistic or kernel timing is asked
+ * for (on by default; no reference counterpart).  Results are the same. */
+void gz_pr_graph_replay(int on);
+
 /* -------- batches of independent graphs (no reference counterpart) -------
  * The loaded graph is a disjoint union; gz_batch_attach names the graphs.
  * CC and BFS need nothing more (the ordinary drivers work on the union).

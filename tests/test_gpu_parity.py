@@ -233,3 +233,37 @@ def test_cli_source_sliced_matches_oracle(gz, orc, device, tmp_path, spec):
     _, ranks, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=20))
     got = [float(l.split()[1]) for l in open(out)]
     assert np.allclose(got, ranks, rtol=1e-5, atol=0)
+
+
+@pytest.mark.parametrize("variant", ["sched_aware", "traditional", "source_sliced", "push"])
+def test_pr_graph_replay_matches_eager(gz, device, variant):
+    # iterations 2.. replayed from a captured graph: bit-identical to eager
+    # launches (within 1e-12 for the atomic variants, whose own eager runs
+    # differ by rounding), over several replays (cache hits, another
+    # iteration count) with eager work and synchronisation in between
+    g = gz.rmat_graph(13, 16, seed=41)
+    gz.load(g)
+    kw = {"engine": "push"} if variant == "push" else {"pull_variant": variant}
+    if variant == "source_sliced":
+        gz.build_source_slices(128, 2, 3, 32)
+        gz.slices_set_hot_cache(4096)
+    try:
+        eager = {}
+        gz.pr_graph_replay(False)
+        for iters in (20, 7):
+            gz.run(gz.options("pr", iterations=iters, **kw))
+            eager[iters] = gz.pr_props().view(np.uint64).copy()
+        gz.pr_graph_replay(True)
+        for iters in (20, 20, 7, 20):
+            gz.run(gz.options("pr", iterations=iters, **kw))
+            got = gz.pr_props()
+            if variant in ("push", "traditional"):
+                want = eager[iters].view(np.float64)
+                assert np.max(np.abs(got - want) / np.abs(want)) <= 1e-12, iters
+            else:
+                assert np.array_equal(got.view(np.uint64), eager[iters]), iters
+            gz.run(gz.options("cc"))
+            gz.lib.gz_synchronize()
+    finally:
+        gz.pr_graph_replay(True)
+        gz.slices_set_hot_cache(0)
