@@ -291,9 +291,9 @@ static void gz_run_pr(const gz_run_options_t* o, gz_run_result_t* r)
     if (use_push && gz_graph.push_vectors == NULL) use_push = 0;
     if (!use_push && gz_graph.pull_vectors == NULL) use_push = 1;
     gz_timer_t* t = o->profile ? gz_timer_create() : NULL;
-    /* with nothing read back per iteration, iterations 2.. replay a graph of
-     * one iteration's launches (the first runs eagerly and makes every
-     * allocation the phases need) */
+    /* with nothing read back per iteration, iterations 2.. run as one graph
+     * of their launches (the first runs eagerly and makes every allocation
+     * the phases need); a configuration is captured when it runs again */
     const int replay = gz_pr_replay_on && t == NULL && !o->collect_stats && !o->edge_only && !o->vertex_only &&
                        !gz_kernel_timing_enabled() && o->iterations > 1 && o->iterations - 1 <= 0xffffffffull;
     uint64_t ctr;
@@ -303,13 +303,14 @@ static void gz_run_pr(const gz_run_options_t* o, gz_run_result_t* r)
             uint64_t key[32];
             const size_t n = gz_pr_iteration_key(&c, key);
             const uint64_t rest = o->iterations - 1;
-            gz_graph_replay_cached(key, n, gz_pr_enqueue_iteration, &c, (uint32_t)rest);
-            if (use_push)
-                r->push_iterations += rest;
-            else
-                r->pull_iterations += rest;
-            ctr = o->iterations;
-            break;
+            if (gz_graph_replay_cached(key, n, gz_pr_enqueue_iteration, &c, (uint32_t)rest)) {
+                if (use_push)
+                    r->push_iterations += rest;
+                else
+                    r->pull_iterations += rest;
+                ctr = o->iterations;
+                break;
+            }
         }
         gz_iteration_record_t rec;
         memset(&rec, 0, sizeof(rec));
@@ -436,6 +437,7 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
             if (t) gz_timer_start(t);
             if (learned) gz_sel_dev_stamp(bfs && gz_phase_config.push_frontier_skip);
             phase_op_reset_global_accum();
+            if (!bfs && !use_pull && ctr > 1) gz_cc_push_vote_hint(vote);
             gz_edge_phase(bfs, use_pull);
         }
         if (use_pull)

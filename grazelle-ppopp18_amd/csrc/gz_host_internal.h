@@ -48,10 +48,15 @@ uint32_t gz_graph_indegree_host(uint64_t v);
  * or 2 (push), 0 without weights. */
 uint32_t gz_selector_host_decide(const float* x, float* y);
 /* Launch one graph of `times` rounds of the launches enqueue(ctx) makes on
- * the phase stream, captured again when key[0..n), `times` or the device
- * allocations changed since the last capture (enqueue must not allocate or
+ * the phase stream and return 1.  The graph is kept while key[0..n), `times`
+ * and the device allocations stay the same.  It is captured the second time
+ * in a row the same request comes; the first time nothing is launched and 0
+ * tells the caller to run the rounds itself (enqueue must not allocate or
  * wait). */
-void gz_graph_replay_cached(const uint64_t* key, size_t n, void (*enqueue)(void*), void* ctx, uint32_t times);
+int gz_graph_replay_cached(const uint64_t* key, size_t n, void (*enqueue)(void*), void* ctx, uint32_t times);
+/* Frontier statistic of the sources the next CC push will read (the vote that
+ * built its frontier); a small one lets the push skip the compaction. */
+void gz_cc_push_vote_hint(uint64_t vote);
 /* The source-sliced pull's settings: hot cache, cold spread, cold stream. */
 void gz_slices_settings(uint32_t* out3);
 

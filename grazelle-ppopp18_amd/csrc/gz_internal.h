@@ -35,6 +35,19 @@ struct gz_packed_vec {
  * 256 CUs several times over, each looping over its share of the work. */
 #define GZ_PERSISTENT_BLOCKS 2048
 
+/* Largest frontier statistic (out-degrees plus count) for which a CC push
+ * scans the HasInfo bitmap directly instead of compacting the frontier. */
+#define GZ_CC_SPARSE_PUSH_VOTE (1u << 16)
+/* Up to this statistic each wavefront of that push reads 64 words and walks
+ * every source itself (the statistic bounds its vectors to 16 batches); above
+ * it each reads 16 words and sends a round of sources with more vectors than
+ * one batch to the overflow queue. */
+#define GZ_CC_SPARSE_PUSH_WIDE_VOTE 4096
+/* Overflow queue: pieces of at most this many vectors (one batch of four
+ * steps of 64), and room for this many pieces. */
+#define GZ_PUSH_PIECE_VECTORS 256
+#define GZ_PUSH_QUEUE_PIECES (1u << 17)
+
 #define GZ_CHECK(call)                                                                   \
     do {                                                                                 \
         hipError_t gz_err_ = (call);                                                     \
@@ -68,6 +81,11 @@ typedef struct gz_scratch_t {
     const uint32_t* features_indeg; /* in-degrees when BFS edge phases fold the features, else nullptr */
     uint64_t* stamp_target;      /* when set, the next compaction writes the wall clock there (once) */
     uint64_t scratch_vertices;   /* V the vertex-sized buffers were sized for */
+    /* overflow queue of the sparse CC push: pieces of at most
+     * GZ_PUSH_PIECE_VECTORS vectors as [source, first vector, count] */
+    uint64_t* push_pieces;       /* [3 * GZ_PUSH_QUEUE_PIECES] */
+    uint64_t* push_queue;        /* [2] piece counters used alternately; the unused one is 0 */
+    uint32_t push_queue_parity;  /* counter the next sparse push appends to */
 } gz_scratch_t;
 
 extern gz_scratch_t gz_scratch;

@@ -108,20 +108,36 @@ extern "C" void gz_device_free(void* ptr)
 
 /* ---- replay of a captured launch sequence ----------------------------------
  * One cached executable graph: `times` rounds of the launches enqueue() makes
- * on the phase stream, captured once into one graph (one graph launch costs
- * more here than the few kernel launches of a round, so the rounds are not
+ * on the phase stream, captured into one graph (one graph launch costs more
+ * here than the few kernel launches of a round, so the rounds are not
  * replayed one launch each) and launched again while the caller's key, the
- * round count and the allocation generation stay the same.  enqueue() must
- * neither allocate nor synchronise (the capture would fail, and the process
- * exits with the HIP error). */
+ * round count and the allocation generation stay the same.  A request is
+ * captured the second time in a row it comes: a run that never repeats (one
+ * run per graph of a batch) pays for no capture it never replays, and a long
+ * run gains nothing from one (its kernels hide the launches; the first
+ * capture in a process costs milliseconds).  enqueue() must neither allocate
+ * nor synchronise (the capture would fail, and the process exits with the
+ * HIP error). */
 #define GZ_REPLAY_KEY_WORDS 64
-static struct {
-    hipGraphExec_t exec;
+struct gz_replay_key_t {
     uint64_t generation;
     uint32_t times;
     size_t n;
     uint64_t key[GZ_REPLAY_KEY_WORDS];
+};
+static struct {
+    hipGraphExec_t exec;
+    gz_replay_key_t captured; /* what exec holds */
+    gz_replay_key_t seen;     /* the last configuration asked for and not captured */
+    bool seen_valid;
 } gz_replay;
+static uint64_t gz_replay_captures, gz_replay_launches;
+
+extern "C" void gz_graph_replay_counts(uint64_t* captures, uint64_t* launches)
+{
+    if (captures) *captures = gz_replay_captures;
+    if (launches) *launches = gz_replay_launches;
+}
 
 static void gz_replay_release(void)
 {
@@ -129,32 +145,46 @@ static void gz_replay_release(void)
     memset(&gz_replay, 0, sizeof(gz_replay));
 }
 
-extern "C" void gz_graph_replay_cached(const uint64_t* key, size_t n, void (*enqueue)(void*), void* ctx,
-                                       uint32_t times)
+static bool gz_replay_key_equal(const gz_replay_key_t& a, const uint64_t* key, size_t n, uint32_t times)
+{
+    return a.generation == gz_alloc_generation && a.times == times && a.n == n &&
+           memcmp(a.key, key, n * sizeof(uint64_t)) == 0;
+}
+
+extern "C" int gz_graph_replay_cached(const uint64_t* key, size_t n, void (*enqueue)(void*), void* ctx,
+                                      uint32_t times)
 {
     gz_require_runtime();
-    if (times == 0) return;
+    if (times == 0) return 1;
     if (n > GZ_REPLAY_KEY_WORDS) {
         fprintf(stderr, "grazelle: replay key of %zu words (at most %d)\n", n, GZ_REPLAY_KEY_WORDS);
         exit(72);
     }
-    const bool hit = gz_replay.exec != nullptr && gz_replay.generation == gz_alloc_generation &&
-                     gz_replay.times == times && gz_replay.n == n &&
-                     memcmp(gz_replay.key, key, n * sizeof(uint64_t)) == 0;
+    const bool hit = gz_replay.exec != nullptr && gz_replay_key_equal(gz_replay.captured, key, n, times);
     if (!hit) {
-        gz_replay_release();
+        const bool repeat = gz_replay.seen_valid && gz_replay_key_equal(gz_replay.seen, key, n, times);
+        gz_replay.seen.generation = gz_alloc_generation;
+        gz_replay.seen.times = times;
+        gz_replay.seen.n = n;
+        memcpy(gz_replay.seen.key, key, n * sizeof(uint64_t));
+        gz_replay.seen_valid = true;
+        /* a first request: the caller runs the rounds itself */
+        if (!repeat) return 0;
+        if (gz_replay.exec != nullptr) GZ_CHECK(hipGraphExecDestroy(gz_replay.exec));
+        gz_replay.exec = nullptr;
         hipGraph_t g = nullptr;
         GZ_CHECK(hipStreamBeginCapture(gz_stream, hipStreamCaptureModeThreadLocal));
         for (uint32_t i = 0; i < times; ++i) enqueue(ctx);
         GZ_CHECK(hipStreamEndCapture(gz_stream, &g));
         GZ_CHECK(hipGraphInstantiate(&gz_replay.exec, g, nullptr, nullptr, 0));
         GZ_CHECK(hipGraphDestroy(g));
-        gz_replay.generation = gz_alloc_generation;
-        gz_replay.times = times;
-        gz_replay.n = n;
-        memcpy(gz_replay.key, key, n * sizeof(uint64_t));
+        gz_replay.captured = gz_replay.seen;
+        gz_replay.seen_valid = false;
+        gz_replay_captures += 1;
     }
     GZ_CHECK(hipGraphLaunch(gz_replay.exec, gz_stream));
+    gz_replay_launches += 1;
+    return 1;
 }
 
 extern "C" void gz_copy_to_device(void* dst, const void* src, size_t bytes)
@@ -388,6 +418,12 @@ void gz_scratch_reserve(uint64_t waves, uint64_t num_vertices)
         gz_scratch.partials_capacity = waves;
     }
     if (gz_scratch.reduce_tmp == nullptr) gz_scratch.reduce_tmp = (uint64_t*)gz_device_malloc(2048 * sizeof(uint64_t));
+    if (gz_scratch.push_queue == nullptr) {
+        gz_scratch.push_pieces = (uint64_t*)gz_device_malloc((size_t)GZ_PUSH_QUEUE_PIECES * 3 * sizeof(uint64_t));
+        gz_scratch.push_queue = (uint64_t*)gz_device_malloc(2 * sizeof(uint64_t));
+        GZ_CHECK(hipMemsetAsync(gz_scratch.push_queue, 0, 2 * sizeof(uint64_t), gz_stream));
+        gz_scratch.push_queue_parity = 0;
+    }
     if (num_vertices > gz_scratch.scratch_vertices) {
         gz_device_free(gz_scratch.active_vertices);
         gz_device_free(gz_scratch.active_prefix);
@@ -592,6 +628,8 @@ extern "C" void gz_runtime_shutdown(void)
     gz_device_free(gz_scratch.active_block_sums);
     gz_device_free(gz_scratch.active_totals);
     gz_device_free(gz_scratch.bfs_first_vector);
+    gz_device_free(gz_scratch.push_pieces);
+    gz_device_free(gz_scratch.push_queue);
     gz_device_free(gz_merge_buffer);
     gz_merge_buffer = nullptr;
     gz_merge_buffer_count = 0;

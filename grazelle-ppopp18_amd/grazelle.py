@@ -97,6 +97,7 @@ _sig("gz_graph_build_rmat_device", C.c_int, P(RmatParams), u64, C.c_int, u64, vp
 _sig("gz_graph_device_vertex", u64, u64)
 _sig("gz_graph_build_source_slices", C.c_int, u64, u32, u32, u64)
 _sig("gz_pr_graph_replay", None, C.c_int)
+_sig("gz_graph_replay_counts", None, C.POINTER(u64), C.POINTER(u64))
 _sig("gz_graph_source_slices_info", C.c_int, vp, P(u64), P(u32), P(u32), P(u64))
 _sig("gz_slices_chunk_table", u64, vp, u32, u64, u32, vp, u64)
 _sig("gz_slices_set_cold_spread", None, u32)
@@ -424,6 +425,14 @@ def pr_graph_replay(on: bool) -> None:
     """PageRank iterations 2.. from a replayed HIP graph (default on)."""
     init()
     lib.gz_pr_graph_replay(int(on))
+
+
+def graph_replay_counts() -> tuple[int, int]:
+    """(graph captures, graph launches) so far."""
+    init()
+    c, n = u64(0), u64(0)
+    lib.gz_graph_replay_counts(C.byref(c), C.byref(n))
+    return int(c.value), int(n.value)
 
 
 def set_vectors_per_lane(n: int) -> None:

@@ -164,10 +164,14 @@ void gz_slices_set_cold_stream(uint32_t cold_stream);
  * another value. */
 int gz_slices_set_hot_cache(uint32_t vertices);
 
-/* PageRank runs replay iterations 2.. from a HIP graph of one iteration's
- * launches when no per-iteration record, statistic or kernel timing is asked
- * for (on by default; no reference counterpart).  Results are the same. */
+/* PageRank runs launch iterations 2.. as one HIP graph of their launches
+ * when no per-iteration record, statistic or kernel timing is asked for (on
+ * by default; no reference counterpart).  A configuration (graph, options,
+ * iteration count, device allocations) is captured the second time in a
+ * row it runs and replayed while it repeats; results are the same. */
 void gz_pr_graph_replay(int on);
+/* Graph captures and graph launches made so far (no reference counterpart). */
+void gz_graph_replay_counts(uint64_t* captures, uint64_t* launches);
 
 /* -------- batches of independent graphs (no reference counterpart) -------
  * The loaded graph is a disjoint union; gz_batch_attach names the graphs.

@@ -146,6 +146,62 @@ def test_cc_labels(gz, orc, device, engine, skip):
         assert np.all(labs[ids] == base + t * 100)
 
 
+@pytest.mark.parametrize("shape", [(10, 8, 100), (14, 16, 300)])
+@pytest.mark.parametrize("engine", ["hybrid", "push"])
+@pytest.mark.parametrize("stat", ["both", "outdeg", "count"])
+def test_cc_small_frontier_push(gz, orc, device, shape, engine, stat):
+    # tree frontiers are small after the first sweeps, so with out-degrees in
+    # the vote the push reads the bitmap directly; with counts only it
+    # compacts.  V = 1,824 ends in a partial bitmap word and fits one
+    # wavefront's words; V = 21,184 spans several wavefronts.
+    scale, trees, size = shape
+    g = graphs.planted_trees(gz, base_scale=scale, trees=trees, tree_size=size)
+    gz.load(g)
+    gz.run(gz.options("cc", engine=engine, stat=stat, collect_stats=True))
+    recs = gz.records()
+    lab, _, _ = orc.cc(_orc_graph(orc, g), orc.options(engine=engine, stat=stat))
+    assert np.array_equal(gz.cc_labels(), lab)
+    small = [r for r in recs[1:] if r.engine == 2 and r.frontier_stat <= 65536]
+    assert small, "no push iteration over a small frontier"
+    if engine == "push" and scale == 14 and stat != "count":
+        # both widths of the bitmap scan: 64 words per wavefront up to a
+        # statistic of 4096, 16 words above it
+        assert any(r.frontier_stat <= 4096 for r in small)
+        assert any(r.frontier_stat > 4096 for r in small)
+    for r in small:
+        # on a symmetric graph every activated vertex has an out-edge; a push
+        # examines each out-edge of its frontier once, and the vote that built
+        # the frontier counted every frontier vertex's out-degree
+        assert 0 < r.vectors <= r.edges <= 4 * r.vectors
+        if stat != "count":
+            assert r.edges <= r.frontier_stat
+
+
+@pytest.mark.parametrize("L", [1200, 20000])
+def test_cc_push_high_degree_source(gz, orc, device, L):
+    # a 201-vertex path whose far end holds the smallest id, into the centre
+    # of an L-leaf star: the smallest label walks the path one hop per push,
+    # and each time it reaches the centre, the centre is a small frontier
+    # with many vectors.  With 1,200 leaves one wavefront walks its 301
+    # vectors in two batches; with 20,000 its 5,001 vectors are cut into
+    # pieces for the overflow queue.
+    P = 200
+    pairs = [(i, i + 1) for i in range(P)] + [(P, P + 1 + j) for j in range(L)]
+    g = gz.make_graph(np.asarray(pairs, dtype=np.uint64), P + 1 + L, symmetrize=True)
+    gz.load(g)
+    gz.run(gz.options("cc", engine="push", collect_stats=True))
+    assert np.all(gz.cc_labels() == 0)
+    lab, _, _ = orc.cc(_orc_graph(orc, g), orc.options(engine="push"))
+    assert np.array_equal(gz.cc_labels(), lab)
+    recs = gz.records()
+    lo, hi = (256, 4096) if L < 4096 else (4096, 65536)
+    hub = [r for r in recs[1:] if lo < r.frontier_stat <= hi]
+    assert hub, "no push over a small frontier holding the centre"
+    for r in recs[1:]:
+        assert r.vectors <= r.edges <= 4 * r.vectors
+        assert r.edges <= r.frontier_stat
+
+
 def test_cc_directed_rmat(gz, orc, device, rmat14):
     gz.load(rmat14)
     gz.run(gz.options("cc"))
@@ -254,7 +310,12 @@ def test_pr_graph_replay_matches_eager(gz, device, variant):
             gz.run(gz.options("pr", iterations=iters, **kw))
             eager[iters] = gz.pr_props().view(np.uint64).copy()
         gz.pr_graph_replay(True)
-        for iters in (20, 20, 7, 20):
+        gz.run(gz.options("cc"))  # the allocations the CC runs below make
+        c0, l0 = gz.graph_replay_counts()
+        # a configuration runs eagerly, then is captured when it repeats:
+        # 20 eager, 20 captured, 7 eager, 7 captured, 20 eager, 7 replays,
+        # 20 captured, 20 replays
+        for iters in (20, 20, 7, 7, 20, 7, 20, 20):
             gz.run(gz.options("pr", iterations=iters, **kw))
             got = gz.pr_props()
             if variant in ("push", "traditional"):
@@ -264,6 +325,8 @@ def test_pr_graph_replay_matches_eager(gz, device, variant):
                 assert np.array_equal(got.view(np.uint64), eager[iters]), iters
             gz.run(gz.options("cc"))
             gz.lib.gz_synchronize()
+        c1, l1 = gz.graph_replay_counts()
+        assert (c1 - c0, l1 - l0) == (3, 5)
     finally:
         gz.pr_graph_replay(True)
         gz.slices_set_hot_cache(0)
