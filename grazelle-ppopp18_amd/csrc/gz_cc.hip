@@ -180,25 +180,6 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_cc(const gz_vec_t* __
     }
 }
 
-/* Position of the k-th (0-based) set bit of x; k < popcount(x). */
-__device__ __forceinline__ uint32_t gz_nth_set_bit(uint64_t x, uint32_t k)
-{
-    uint32_t pos = 0;
-#pragma unroll
-    for (int w = 32; w >= 1; w >>= 1) {
-        const uint64_t low = x & ((1ull << w) - 1);
-        const uint32_t c = (uint32_t)__popcll(low);
-        if (k >= c) {
-            k -= c;
-            x >>= w;
-            pos += (uint32_t)w;
-        } else {
-            x = low;
-        }
-    }
-    return pos;
-}
-
 /* Index of the first lane whose (non-decreasing) inclusive count exceeds x;
  * 63 when none does. */
 __device__ __forceinline__ int gz_wave_owner(uint64_t incl, uint64_t x)

@@ -277,6 +277,25 @@ __device__ __forceinline__ double gz_wave_sum(double x)
     return x;
 }
 
+/* Position of the k-th (0-based) set bit of x; k < popcount(x). */
+__device__ __forceinline__ uint32_t gz_nth_set_bit(uint64_t x, uint32_t k)
+{
+    uint32_t pos = 0;
+#pragma unroll
+    for (int w = 32; w >= 1; w >>= 1) {
+        const uint64_t low = x & ((1ull << w) - 1);
+        const uint32_t c = (uint32_t)__popcll(low);
+        if (k >= c) {
+            k -= c;
+            x >>= w;
+            pos += (uint32_t)w;
+        } else {
+            x = low;
+        }
+    }
+    return pos;
+}
+
 __device__ __forceinline__ uint64_t gz_wave_sum_u64(uint64_t x)
 {
 #pragma unroll
