@@ -198,7 +198,18 @@ static void gz_snapshot_restore(const gz_snapshot_t* s)
  * last run is also bracketed by the selector's device clock, so the next
  * iteration's "previous edge-phase time" feature is measured exactly as the
  * learned engine measures it. */
-static double gz_probe_engine(int bfs, int pull, const gz_snapshot_t* s, gz_timer_t* t, int stamp)
+/* A push over the frontier that a vote of `vote` built may skip the
+ * compaction when the vote is small: tell the push phase. */
+static void gz_push_hint(int bfs, int pull, uint64_t vote)
+{
+    if (pull) return;
+    if (bfs)
+        gz_bfs_push_vote_hint(vote);
+    else
+        gz_cc_push_vote_hint(vote);
+}
+
+static double gz_probe_engine(int bfs, int pull, const gz_snapshot_t* s, gz_timer_t* t, int stamp, uint64_t vote)
 {
     double best = 0.0;
     for (int rep = 0; rep < 3; ++rep) {
@@ -206,6 +217,7 @@ static double gz_probe_engine(int bfs, int pull, const gz_snapshot_t* s, gz_time
         if (stamp && rep == 2) gz_sel_dev_stamp(bfs && gz_phase_config.push_frontier_skip);
         gz_timer_start(t);
         phase_op_reset_global_accum();
+        gz_push_hint(bfs, pull, vote);
         gz_edge_phase(bfs, pull);
         gz_timer_stop(t);
         const double ms = gz_timer_elapsed_ms(t);
@@ -428,8 +440,8 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
         if (probe) {
             /* the chosen engine runs last, so its result is what continues */
             gz_snapshot_take(&snap);
-            const double other = gz_probe_engine(bfs, !use_pull, &snap, t, 0);
-            const double chosen = gz_probe_engine(bfs, use_pull, &snap, t, 1);
+            const double other = gz_probe_engine(bfs, !use_pull, &snap, t, 0, vote);
+            const double chosen = gz_probe_engine(bfs, use_pull, &snap, t, 1, vote);
             rec.pull_ms = use_pull ? chosen : other;
             rec.push_ms = use_pull ? other : chosen;
             rec.edge_ms = chosen;
@@ -437,7 +449,7 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
             if (t) gz_timer_start(t);
             if (learned) gz_sel_dev_stamp(bfs && gz_phase_config.push_frontier_skip);
             phase_op_reset_global_accum();
-            if (!bfs && !use_pull && ctr > 1) gz_cc_push_vote_hint(vote);
+            gz_push_hint(bfs, use_pull, vote);
             gz_edge_phase(bfs, use_pull);
         }
         if (use_pull)

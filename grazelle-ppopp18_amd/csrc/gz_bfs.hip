@@ -17,6 +17,9 @@
  * (level << 48 | source).  Exactly one lane sees the "not visited" old value;
  * it sets HasInfo* and counts the frontier statistic once (the reference's
  * weak check and clear are not atomic, phase_edge_push_bfs.asm:108-156).
+ * When the driver reports a small frontier statistic, the push reads the
+ * HasInfo words directly instead of compacting them (gz_sparse_push.h), with
+ * the same per-edge work.
  * WantsInfo stays fixed during the kernel and loses the visited vertices in a
  * second pass, so every visiting source competes and the surviving parent is
  * the smallest one: the parent the reference picks when one thread walks the
@@ -33,6 +36,7 @@
  * the same words on every run.
  */
 #include "gz_slots.h"
+#include "gz_sparse_push.h"
 
 __device__ __forceinline__ bool gz_vec_has_frontier_source(const gz_lanes& v, const uint64_t* __restrict__ has_info,
                                                            uint32_t& edges)
@@ -113,6 +117,43 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_bfs(const gz_vec_t* _
         }
     }
 }
+
+/* The BFS push's per-edge work for the sparse push: the key (level << 48 |
+ * source), the destination's WantsInfo word and its state read ahead, and the
+ * atomic min, as in gz_k_edge_push_bfs. */
+struct gz_bfs_push_op {
+    const uint64_t* wants_info;
+    uint64_t* words;
+    uint64_t* next_frontier;
+    const double* outdeg;
+    uint32_t stat_mode;
+    uint64_t level;
+    gz_feature_sink sink;
+    __device__ __forceinline__ uint64_t message(uint64_t s) const { return (level << 48) | s; }
+    __device__ __forceinline__ void gather(uint64_t d, uint64_t* g) const
+    {
+        g[0] = wants_info[d >> 6];
+        g[1] = __hip_atomic_load(&words[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ __forceinline__ void apply(uint64_t d, uint64_t key, const uint64_t* g, uint64_t& vote,
+                                          gz_feature_acc& feat) const
+    {
+        if (!((g[0] >> (d & 63)) & 1ull)) return;
+        /* words only decrease: a value already <= key means a source no
+         * larger than this one reached d at this level */
+        if (g[1] <= key) return;
+        const unsigned long long old = atomicMin((unsigned long long*)&words[d], (unsigned long long)key);
+        if (old == GZ_BFS_UNVISITED) {
+            gz_bit_set(next_frontier, d);
+            vote += gz_frontier_stat(outdeg, d, stat_mode);
+            gz_feature_add(feat, sink, outdeg, d);
+        }
+    }
+    __device__ __forceinline__ void finish(uint64_t wave, const gz_feature_acc& feat) const
+    {
+        gz_feature_flush(feat, sink, wave);
+    }
+};
 
 /* WantsInfo &= ~HasInfo*: the push kernel reads WantsInfo as the snapshot of
  * "not visited before this level" and leaves it unchanged, so that every
@@ -310,10 +351,47 @@ static gz_slot_source gz_bfs_source(const uint64_t* offsets, uint64_t count)
     return src;
 }
 
+/* Frontier statistic of the next push, when the driver knows it (the vote of
+ * the level that built the frontier); ~0 when unknown.  Consumed by the next
+ * push. */
+static uint64_t gz_bfs_next_push_vote = ~0ull;
+
+extern "C" void gz_bfs_push_vote_hint(uint64_t vote) { gz_bfs_next_push_vote = vote; }
+
+static void gz_bfs_retire(void)
+{
+    const uint64_t words = (gz_graph.num_vertices + 63) / 64;
+    uint64_t rblocks = (words + GZ_BLOCK - 1) / GZ_BLOCK;
+    if (rblocks > 4096) rblocks = 4096;
+    hipLaunchKernelGGL(gz_k_bfs_retire_wants, dim3((unsigned)rblocks), dim3(GZ_BLOCK), 0, gz_stream,
+                       gz_graph.frontier_wants_info, (const uint64_t*)gz_graph.vertex_accumulators, words);
+    GZ_LAUNCH_CHECK();
+}
+
 extern "C" void perform_edge_push_phase_bfs(const gz_vec_t* edge_list, uint64_t edge_list_count)
 {
     const uint64_t chunk_len = 256;
+    const uint64_t hint = gz_bfs_next_push_vote;
+    gz_bfs_next_push_vote = ~0ull;
     gz_scratch_reserve(0, gz_graph.num_vertices);
+    if (gz_push_sparse_eligible(hint)) {
+        /* reserve first: the feature sink points into the per-wave scratch */
+        gz_scratch_reserve(gz_push_sparse_waves(hint), gz_graph.num_vertices);
+        const gz_bfs_push_op op{gz_graph.frontier_wants_info,
+                                (uint64_t*)gz_graph.vertex_props,
+                                (uint64_t*)gz_graph.vertex_accumulators,
+                                gz_graph.vertex_outdegrees,
+                                gz_phase_config.stat_mode,
+                                (uint64_t)gz_phase_config.bfs_level,
+                                {gz_scratch.features_indeg ? gz_scratch.features : nullptr, gz_scratch.features_indeg}};
+        uint64_t* stamp = gz_scratch.stamp_target;
+        gz_scratch.stamp_target = nullptr;
+        gz_ktimer_begin(GZ_K_PUSH_BFS);
+        gz_launch_push_sparse(op, edge_list, hint, stamp);
+        gz_ktimer_end(GZ_K_PUSH_BFS);
+        gz_bfs_retire();
+        return;
+    }
     const bool active = gz_phase_config.push_frontier_skip && gz_graph.push_vertex_offsets != nullptr;
     gz_slot_source src = gz_bfs_source(gz_graph.push_vertex_offsets, edge_list_count);
     uint64_t blocks;
@@ -346,12 +424,7 @@ extern "C" void perform_edge_push_phase_bfs(const gz_vec_t* edge_list, uint64_t 
                            (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats, sink);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_PUSH_BFS);
-    const uint64_t words = (gz_graph.num_vertices + 63) / 64;
-    uint64_t rblocks = (words + GZ_BLOCK - 1) / GZ_BLOCK;
-    if (rblocks > 4096) rblocks = 4096;
-    hipLaunchKernelGGL(gz_k_bfs_retire_wants, dim3((unsigned)rblocks), dim3(GZ_BLOCK), 0, gz_stream,
-                       gz_graph.frontier_wants_info, (const uint64_t*)gz_graph.vertex_accumulators, words);
-    GZ_LAUNCH_CHECK();
+    gz_bfs_retire();
 }
 
 extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t edge_list_count)

@@ -20,10 +20,11 @@
  * vectors with all lanes busy; each valid lane applies an atomic min of the
  * source's label (read once per vector, as the asm reads props[src] once).
  * When the driver reports a small frontier statistic, one kernel reads the
- * HasInfo words directly instead (gz_k_edge_push_cc_sparse), with the same
- * per-edge semantics.
+ * HasInfo words directly instead (gz_sparse_push.h), with the same per-edge
+ * semantics.
  */
 #include "gz_slots.h"
+#include "gz_sparse_push.h"
 #include "gz_walk.h"
 
 __device__ __forceinline__ void gz_cc_apply(uint64_t* __restrict__ labels, uint64_t* __restrict__ next_frontier,
@@ -180,240 +181,28 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_cc(const gz_vec_t* __
     }
 }
 
-/* Index of the first lane whose (non-decreasing) inclusive count exceeds x;
- * 63 when none does. */
-__device__ __forceinline__ int gz_wave_owner(uint64_t incl, uint64_t x)
-{
-    int o = 0;
-#pragma unroll
-    for (int step = GZ_WAVE / 2; step >= 1; step >>= 1) {
-        const uint64_t y = (uint64_t)__shfl((unsigned long long)incl, o + step - 1, GZ_WAVE);
-        if (y <= x) o += step;
-    }
-    return o;
-}
-
-__device__ __forceinline__ uint64_t gz_wave_inclusive_sum(uint64_t x, int lane)
-{
-#pragma unroll
-    for (int o = 1; o < GZ_WAVE; o <<= 1) {
-        const uint64_t t = (uint64_t)__shfl_up((unsigned long long)x, o, GZ_WAVE);
-        if (lane >= o) x += t;
-    }
-    return x;
-}
-
-/* Push up to kSteps steps of 64 vectors: step u < nsteps holds vector vi[u]
- * of source src[u] on lanes with ok[u].  All vector loads are issued first,
- * then all label reads, so a lane keeps 4 kSteps label reads in flight.
- * Lanes without ok[u] read vector 0 and label 0 and are ignored. */
-template <int kSteps>
-__device__ __forceinline__ void gz_cc_push_steps(const gz_vec_t* __restrict__ vecs, uint64_t* __restrict__ labels,
-                                                 uint64_t* __restrict__ next_frontier,
-                                                 const double* __restrict__ outdeg, uint32_t stat_mode, int nsteps,
-                                                 const bool* ok, const uint64_t* vi, const uint64_t* src,
-                                                 uint64_t& vote, uint32_t& vectors, uint32_t& edges)
-{
-    uint64_t l[kSteps][4];
-#pragma unroll
-    for (int u = 0; u < kSteps; ++u) {
-        l[u][0] = l[u][1] = l[u][2] = l[u][3] = 0;
-        if (u >= nsteps) continue;
-        const gz_lanes vec = gz_load_vector(vecs, ok[u] ? vi[u] : 0);
-        l[u][0] = vec.l0;
-        l[u][1] = vec.l1;
-        l[u][2] = vec.l2;
-        l[u][3] = vec.l3;
-    }
-    uint64_t msg[kSteps], x[kSteps][4];
-#pragma unroll
-    for (int u = 0; u < kSteps; ++u) {
-        msg[u] = ~0ull;
-        x[u][0] = x[u][1] = x[u][2] = x[u][3] = 0;
-        if (u >= nsteps) continue;
-        msg[u] = labels[ok[u] ? src[u] : 0];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) x[u][k] = labels[(ok[u] && gz_lane_valid(l[u][k])) ? gz_lane_id(l[u][k]) : 0];
-    }
-#pragma unroll
-    for (int u = 0; u < kSteps; ++u) {
-        if (u >= nsteps || !ok[u]) continue;
-        vectors += 1;
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (!gz_lane_valid(l[u][k])) continue;
-            edges += 1;
-            const uint64_t d = gz_lane_id(l[u][k]);
-            if (msg[u] < x[u][k]) {
-                const unsigned long long old = atomicMin((unsigned long long*)&labels[d], (unsigned long long)msg[u]);
-                if (old > msg[u]) {
-                    gz_bit_set(next_frontier, d);
-                    vote += gz_frontier_stat(outdeg, d, stat_mode);
-                }
+/* The CC push's per-edge work for the sparse push (gz_sparse_push.h): the
+ * source's label, the destination's label read ahead, and the atomic min. */
+struct gz_cc_push_op {
+    uint64_t* labels;
+    uint64_t* next_frontier;
+    const double* outdeg;
+    uint32_t stat_mode;
+    __device__ __forceinline__ uint64_t message(uint64_t s) const { return labels[s]; }
+    __device__ __forceinline__ void gather(uint64_t d, uint64_t* g) const { g[0] = labels[d]; }
+    __device__ __forceinline__ void apply(uint64_t d, uint64_t m, const uint64_t* g, uint64_t& vote,
+                                          gz_feature_acc&) const
+    {
+        if (m < g[0]) {
+            const unsigned long long old = atomicMin((unsigned long long*)&labels[d], (unsigned long long)m);
+            if (old > m) {
+                gz_bit_set(next_frontier, d);
+                vote += gz_frontier_stat(outdeg, d, stat_mode);
             }
         }
     }
-}
-
-/* Push for a small frontier, with no compaction pass.  Each wavefront reads
- * kWords HasInfo words (64 vertices each) and numbers their set bits with a
- * scan of the word popcounts.  Rounds of 64 sources follow: each lane selects
- * one source (owner word by binary search over the scan, then the n-th set
- * bit), and a scan of the sources' vector counts spreads their vectors over
- * the 64 lanes, each lane finding its source the same way, in batches of
- * four steps of 64 vectors.  With kQueue a round of more than one batch is
- * cut instead into pieces of at most GZ_PUSH_PIECE_VECTORS vectors,
- * appended to the overflow queue (counter *queue) for
- * gz_k_edge_push_cc_pieces, so that no wavefront walks a high-degree source
- * alone; without kQueue every round is pushed here. */
-template <int kWords, bool kQueue>
-__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_cc_sparse(
-    const gz_vec_t* __restrict__ vecs, const uint64_t* __restrict__ offsets, const uint64_t* __restrict__ has_info,
-    uint64_t num_vertices, uint64_t* __restrict__ labels, uint64_t* __restrict__ next_frontier,
-    const double* __restrict__ outdeg, uint32_t stat_mode, uint64_t* __restrict__ pieces, uint64_t* __restrict__ queue,
-    uint64_t* __restrict__ partials, uint64_t* __restrict__ stats, uint32_t collect)
-{
-    constexpr int kSteps = GZ_PUSH_PIECE_VECTORS / GZ_WAVE;
-    const int lane = gz_lane();
-    const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
-    const uint64_t nwords = (num_vertices + 63) >> 6;
-    const uint64_t w = gw * kWords + (uint64_t)lane;
-    const bool word_lane = lane < kWords && w < nwords;
-    uint64_t bits = word_lane ? has_info[w] : 0;
-    if (word_lane && w == nwords - 1 && (num_vertices & 63)) bits &= (1ull << (num_vertices & 63)) - 1;
-    const uint64_t pc = (uint64_t)__popcll(bits);
-    const uint64_t pincl = gz_wave_inclusive_sum(pc, lane);
-    const uint64_t sources = gz_readlane_u64(pincl, GZ_WAVE - 1);
-    uint64_t vote = 0;
-    uint32_t vectors = 0, edges = 0;
-    for (uint64_t r = 0; r < sources; r += GZ_WAVE) {
-        const uint64_t idx = r + (uint64_t)lane;
-        const int ow = gz_wave_owner(pincl, idx);
-        const uint64_t wbits = (uint64_t)__shfl((unsigned long long)bits, ow, GZ_WAVE);
-        const uint64_t wex = (uint64_t)__shfl((unsigned long long)(pincl - pc), ow, GZ_WAVE);
-        uint64_t v = 0, first = 0, n = 0;
-        if (idx < sources) {
-            v = ((gw * kWords + (uint64_t)ow) << 6) + gz_nth_set_bit(wbits, (uint32_t)(idx - wex));
-            first = offsets[v];
-            n = offsets[v + 1] - first;
-        }
-        const uint64_t incl = gz_wave_inclusive_sum(n, lane);
-        const uint64_t total = gz_readlane_u64(incl, GZ_WAVE - 1);
-        if (kQueue && total > (uint64_t)GZ_PUSH_PIECE_VECTORS) {
-            /* one queue reservation for the round's pieces */
-            const uint64_t np = (n + GZ_PUSH_PIECE_VECTORS - 1) / GZ_PUSH_PIECE_VECTORS;
-            const uint64_t npincl = gz_wave_inclusive_sum(np, lane);
-            const uint64_t npall = gz_readlane_u64(npincl, GZ_WAVE - 1);
-            uint64_t at = 0;
-            if (lane == 0) at = (uint64_t)atomicAdd((unsigned long long*)queue, (unsigned long long)npall);
-            at = gz_readlane_u64(at, 0) + (npincl - np);
-            for (uint64_t k = 0; k < np; ++k) {
-                const uint64_t pf = first + k * GZ_PUSH_PIECE_VECTORS;
-                const uint64_t rest = n - k * GZ_PUSH_PIECE_VECTORS;
-                const uint64_t pl = rest < GZ_PUSH_PIECE_VECTORS ? rest : GZ_PUSH_PIECE_VECTORS;
-                const uint64_t slot = at + k;
-                if (slot < GZ_PUSH_QUEUE_PIECES) {
-                    pieces[3 * slot] = v;
-                    pieces[3 * slot + 1] = pf;
-                    pieces[3 * slot + 2] = pl;
-                } else {
-                    /* queue full (more pieces than a frontier within the
-                     * sparse bound can make): this lane pushes the piece */
-                    const uint64_t m = labels[v];
-                    for (uint64_t j = 0; j < pl; ++j) {
-                        const gz_lanes vec = gz_load_vector(vecs, pf + j);
-                        const uint64_t lj[4] = {vec.l0, vec.l1, vec.l2, vec.l3};
-                        vectors += 1;
-#pragma unroll
-                        for (int q = 0; q < 4; ++q) {
-                            if (!gz_lane_valid(lj[q])) continue;
-                            edges += 1;
-                            gz_cc_apply(labels, next_frontier, outdeg, stat_mode, gz_lane_id(lj[q]), m, vote);
-                        }
-                    }
-                }
-            }
-            continue;
-        }
-        const uint64_t excl = incl - n;
-        for (uint64_t base = 0; base < total; base += (uint64_t)kSteps * GZ_WAVE) {
-            /* wave-uniform count of the steps this batch holds */
-            const uint64_t left = (total - base + GZ_WAVE - 1) / GZ_WAVE;
-            const int nsteps = left < (uint64_t)kSteps ? (int)left : kSteps;
-            bool ok[kSteps];
-            uint64_t vi[kSteps], src[kSteps];
-#pragma unroll
-            for (int u = 0; u < kSteps; ++u) {
-                ok[u] = false;
-                vi[u] = src[u] = 0;
-                if (u >= nsteps) continue;
-                const uint64_t slot = base + (uint64_t)(u * GZ_WAVE + lane);
-                const int o = gz_wave_owner(incl, slot);
-                src[u] = (uint64_t)__shfl((unsigned long long)v, o, GZ_WAVE);
-                const uint64_t f = (uint64_t)__shfl((unsigned long long)first, o, GZ_WAVE);
-                const uint64_t e0 = (uint64_t)__shfl((unsigned long long)excl, o, GZ_WAVE);
-                ok[u] = slot < total;
-                vi[u] = f + (slot - e0);
-            }
-            gz_cc_push_steps<kSteps>(vecs, labels, next_frontier, outdeg, stat_mode, nsteps, ok, vi, src, vote,
-                                     vectors, edges);
-        }
-    }
-    const uint64_t tot = gz_wave_sum_u64(vote);
-    if (lane == 0) partials[gw] = tot;
-    if (collect) {
-        const uint64_t tv = gz_wave_sum_u64(vectors), te = gz_wave_sum_u64(edges);
-        if (lane == 0) {
-            stats[2 * gw] = tv;
-            stats[2 * gw + 1] = te;
-        }
-    }
-}
-
-/* The overflow queue of gz_k_edge_push_cc_sparse: wavefront gw pushes pieces
- * gw, gw + W, ... of the queue[parity] appended there.  It also clears the
- * other counter, which the next sparse push appends to (the kernel that read
- * that counter last has finished: launches on one stream run in order). */
-__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_cc_pieces(
-    const gz_vec_t* __restrict__ vecs, const uint64_t* __restrict__ pieces, uint64_t* __restrict__ queue,
-    uint32_t parity, uint64_t* __restrict__ labels, uint64_t* __restrict__ next_frontier,
-    const double* __restrict__ outdeg, uint32_t stat_mode, uint64_t* __restrict__ partials,
-    uint64_t* __restrict__ stats, uint32_t collect)
-{
-    constexpr int kSteps = GZ_PUSH_PIECE_VECTORS / GZ_WAVE;
-    const int lane = gz_lane();
-    const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
-    const uint64_t W = (uint64_t)gridDim.x * GZ_WAVES_PER_BLOCK;
-    if (gw == 0 && lane == 0) queue[parity ^ 1u] = 0;
-    uint64_t count = queue[parity];
-    if (count > GZ_PUSH_QUEUE_PIECES) count = GZ_PUSH_QUEUE_PIECES;
-    uint64_t vote = 0;
-    uint32_t vectors = 0, edges = 0;
-    for (uint64_t p = gw; p < count; p += W) {
-        const uint64_t s = pieces[3 * p], f = pieces[3 * p + 1], len = pieces[3 * p + 2];
-        const int nsteps = (int)((len + GZ_WAVE - 1) / GZ_WAVE);
-        bool ok[kSteps];
-        uint64_t vi[kSteps], src[kSteps];
-#pragma unroll
-        for (int u = 0; u < kSteps; ++u) {
-            const uint64_t slot = (uint64_t)(u * GZ_WAVE + lane);
-            ok[u] = slot < len;
-            vi[u] = f + slot;
-            src[u] = s;
-        }
-        gz_cc_push_steps<kSteps>(vecs, labels, next_frontier, outdeg, stat_mode, nsteps, ok, vi, src, vote, vectors,
-                                 edges);
-    }
-    const uint64_t tot = gz_wave_sum_u64(vote);
-    if (lane == 0) partials[gw] = tot;
-    if (collect) {
-        const uint64_t tv = gz_wave_sum_u64(vectors), te = gz_wave_sum_u64(edges);
-        if (lane == 0) {
-            stats[2 * gw] = tv;
-            stats[2 * gw + 1] = te;
-        }
-    }
-}
+    __device__ __forceinline__ void finish(uint64_t, const gz_feature_acc&) const {}
+};
 
 /* Frontier statistic of the next push, when the driver knows it (the vote of
  * the iteration that built the frontier, which counts every activated source
@@ -427,50 +216,11 @@ static void gz_cc_launch_push(const gz_vec_t* edge_list, uint64_t edge_list_coun
 {
     const uint64_t hint = gz_cc_next_push_vote;
     gz_cc_next_push_vote = ~0ull;
-    /* With out-degrees in the vote, the vote bounds the frontier's vectors
-     * (every frontier source with vectors added at least its out-degree).  Up
-     * to GZ_CC_SPARSE_PUSH_WIDE_VOTE one wavefront may walk them all; up to
-     * GZ_CC_SPARSE_PUSH_VOTE their pieces (at most one per source plus one per
-     * GZ_PUSH_PIECE_VECTORS vectors) fit the overflow queue. */
-    if (gz_phase_config.push_frontier_skip && gz_graph.push_vertex_offsets != nullptr &&
-        gz_phase_config.stat_mode != GZ_STAT_COUNT_ONLY && hint <= GZ_CC_SPARSE_PUSH_VOTE &&
-        gz_graph.num_vertices > 0) {
-        const uint64_t V = gz_graph.num_vertices;
-        /* a tiny frontier pays mostly for the grid; a larger one for the
-         * wavefront that holds the most sources, so it gets fewer words each
-         * and the overflow queue */
-        const bool wide = hint <= GZ_CC_SPARSE_PUSH_WIDE_VOTE;
-        const uint64_t words = wide ? 64 : 16;
-        const uint64_t waves = (((V + 63) >> 6) + words - 1) / words;
-        const uint64_t blocks = (waves + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
-        const uint64_t qblocks = wide ? 0 : GZ_PERSISTENT_BLOCKS / 4;
-        const uint64_t sw = blocks * GZ_WAVES_PER_BLOCK, all = sw + qblocks * GZ_WAVES_PER_BLOCK;
-        gz_scratch_reserve(all, V);
-        gz_scratch.partials_is_double = 0;
-        gz_scratch.partials_count = all;
-        gz_scratch.stats_count = gz_phase_config.collect_stats ? all : 0;
-        uint64_t* partials = (uint64_t*)gz_scratch.partials;
-        const uint32_t parity = gz_scratch.push_queue_parity;
+    if (gz_push_sparse_eligible(hint)) {
+        const gz_cc_push_op op{(uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
+                               gz_graph.vertex_outdegrees, gz_phase_config.stat_mode};
         gz_ktimer_begin(GZ_K_PUSH_CC);
-#define GZ_PUSH_SPARSE_ARGS                                                                                     \
-    dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list, gz_graph.push_vertex_offsets,               \
-        gz_graph.frontier_has_info, V, (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators, \
-        gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, gz_scratch.push_pieces,                             \
-        gz_scratch.push_queue + parity, partials, gz_scratch.stats, gz_phase_config.collect_stats
-        if (wide) {
-            hipLaunchKernelGGL((gz_k_edge_push_cc_sparse<64, false>), GZ_PUSH_SPARSE_ARGS);
-        } else {
-            hipLaunchKernelGGL((gz_k_edge_push_cc_sparse<16, true>), GZ_PUSH_SPARSE_ARGS);
-            GZ_LAUNCH_CHECK();
-            hipLaunchKernelGGL(gz_k_edge_push_cc_pieces, dim3((unsigned)qblocks), dim3(GZ_BLOCK), 0, gz_stream,
-                               edge_list, gz_scratch.push_pieces, gz_scratch.push_queue, parity,
-                               (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
-                               gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, partials + sw,
-                               gz_scratch.stats + 2 * sw, gz_phase_config.collect_stats);
-            gz_scratch.push_queue_parity = parity ^ 1u;
-        }
-#undef GZ_PUSH_SPARSE_ARGS
-        GZ_LAUNCH_CHECK();
+        gz_launch_push_sparse(op, edge_list, hint, nullptr);
         gz_ktimer_end(GZ_K_PUSH_CC);
         return;
     }

@@ -57,6 +57,9 @@ int gz_graph_replay_cached(const uint64_t* key, size_t n, void (*enqueue)(void*)
 /* Frontier statistic of the sources the next CC push will read (the vote that
  * built its frontier); a small one lets the push skip the compaction. */
 void gz_cc_push_vote_hint(uint64_t vote);
+/* The same for the next BFS push (the vote of the level that built its
+ * frontier). */
+void gz_bfs_push_vote_hint(uint64_t vote);
 /* The source-sliced pull's settings: hot cache, cold spread, cold stream. */
 void gz_slices_settings(uint32_t* out3);
 

@@ -35,14 +35,14 @@ struct gz_packed_vec {
  * 256 CUs several times over, each looping over its share of the work. */
 #define GZ_PERSISTENT_BLOCKS 2048
 
-/* Largest frontier statistic (out-degrees plus count) for which a CC push
+/* Largest frontier statistic (out-degrees plus count) for which a CC or BFS push
  * scans the HasInfo bitmap directly instead of compacting the frontier. */
-#define GZ_CC_SPARSE_PUSH_VOTE (1u << 16)
+#define GZ_SPARSE_PUSH_VOTE (1u << 16)
 /* Up to this statistic each wavefront of that push reads 64 words and walks
  * every source itself (the statistic bounds its vectors to 16 batches); above
  * it each reads 16 words and sends a round of sources with more vectors than
  * one batch to the overflow queue. */
-#define GZ_CC_SPARSE_PUSH_WIDE_VOTE 4096
+#define GZ_SPARSE_PUSH_WIDE_VOTE 4096
 /* Overflow queue: pieces of at most this many vectors (one batch of four
  * steps of 64), and room for this many pieces. */
 #define GZ_PUSH_PIECE_VECTORS 256
