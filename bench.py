@@ -335,6 +335,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--config", choices=("cc_rmat22", "bfs_rmat24", "batch_1024", "pr_rmat16"), default=None,
+                    help="only run this configuration and print its result (profiling)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run the device copy kernel on MIB mebibytes (rocprof counter calibration)")
     args = ap.parse_args()
@@ -363,11 +365,17 @@ def main() -> int:
         print("bench.py needs a HIP device", file=sys.stderr)
         return 1
     dev = gz.init(0)
+    configs = (("cc_rmat22", cc_rmat22), ("bfs_rmat24", bfs_rmat24), ("batch_1024", batch_1024),
+               ("pr_rmat16", pr_rmat16))
+    if args.config:
+        out = dict(configs)[args.config](gz)
+        print(json.dumps({args.config: out}), flush=True)
+        gz.release()
+        return 0
     h = headline(gz, args)
     extra = {}
     if not args.only_headline:
-        for name, fn in (("cc_rmat22", cc_rmat22), ("bfs_rmat24", bfs_rmat24), ("batch_1024", batch_1024),
-                         ("pr_rmat16", pr_rmat16)):
+        for name, fn in configs:
             t0 = time.time()
             try:
                 extra[name] = fn(gz)
