@@ -202,6 +202,29 @@ def test_cc_push_high_degree_source(gz, orc, device, L):
         assert r.edges <= r.frontier_stat
 
 
+@pytest.mark.parametrize("engine", ["hybrid", "push"])
+@pytest.mark.parametrize("L", [1200, 20000])
+def test_bfs_push_high_degree_root(gz, orc, device, engine, L):
+    # BFS from the centre of an L-leaf star with a 200-vertex path attached:
+    # the root's level is one source with L + 1 out-edges, walked by one
+    # wavefront (1,200 leaves) or cut into pieces for the overflow queue
+    # (20,000 leaves); the path's levels are one small source each
+    P = 200
+    centre = P
+    pairs = [(i, i + 1) for i in range(P)] + [(centre, P + 1 + j) for j in range(L)]
+    g = gz.make_graph(np.asarray(pairs, dtype=np.uint64), P + 1 + L, symmetrize=True)
+    G = _orc_graph(orc, g)
+    gz.load(g)
+    gz.run(gz.options("bfs", engine=engine, root=centre, profile=True))
+    par, dep, log, _ = orc.bfs(G, orc.options(engine=engine, root=centre))
+    p, d = gz.bfs_result()
+    assert np.array_equal(d, dep)
+    assert np.array_equal(p, par)
+    assert [(x.engine, x.frontier_stat) for x in gz.records()] == log
+    if engine == "push":
+        assert gz.records()[0].engine == 2
+
+
 def test_cc_directed_rmat(gz, orc, device, rmat14):
     gz.load(rmat14)
     gz.run(gz.options("cc"))
