@@ -281,6 +281,7 @@ const double* gz_graph_outdegrees_host(void) { return G.outdeg; }
 void gz_graph_release(void)
 {
     gz_source_slices_release();
+    gz_frontier_counts_release();
     gz_device_free(G.d_pull);
     gz_device_free(G.d_push);
     gz_device_free(G.d_pull_offsets);

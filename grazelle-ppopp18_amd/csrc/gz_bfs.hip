@@ -396,7 +396,8 @@ extern "C" void perform_edge_push_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     gz_slot_source src = gz_bfs_source(gz_graph.push_vertex_offsets, edge_list_count);
     uint64_t blocks;
     if (active) {
-        gz_frontier_compact(gz_graph.frontier_has_info, gz_graph.push_vertex_offsets, gz_graph.num_vertices);
+        gz_frontier_compact(gz_graph.frontier_has_info, gz_graph.push_vertex_offsets, gz_graph.num_vertices,
+                            edge_list_count);
         blocks = GZ_PERSISTENT_BLOCKS;
     } else {
         blocks = ((edge_list_count + chunk_len - 1) / chunk_len + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
@@ -435,7 +436,8 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     gz_slot_source src = gz_bfs_source(gz_graph.pull_vertex_offsets, edge_list_count);
     uint64_t blocks;
     if (active) {
-        gz_frontier_compact(gz_graph.frontier_wants_info, gz_graph.pull_vertex_offsets, gz_graph.num_vertices);
+        gz_frontier_compact(gz_graph.frontier_wants_info, gz_graph.pull_vertex_offsets, gz_graph.num_vertices,
+                            edge_list_count);
         blocks = GZ_PERSISTENT_BLOCKS;
     } else {
         blocks = ((edge_list_count + chunk_len - 1) / chunk_len + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;

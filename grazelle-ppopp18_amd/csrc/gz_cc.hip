@@ -234,7 +234,8 @@ static void gz_cc_launch_push(const gz_vec_t* edge_list, uint64_t edge_list_coun
     const bool active = gz_phase_config.push_frontier_skip && gz_graph.push_vertex_offsets != nullptr;
     uint64_t blocks;
     if (active) {
-        gz_frontier_compact(gz_graph.frontier_has_info, gz_graph.push_vertex_offsets, gz_graph.num_vertices);
+        gz_frontier_compact(gz_graph.frontier_has_info, gz_graph.push_vertex_offsets, gz_graph.num_vertices,
+                            edge_list_count);
         blocks = GZ_PERSISTENT_BLOCKS;
     } else {
         blocks = ((edge_list_count + chunk_len - 1) / chunk_len + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;

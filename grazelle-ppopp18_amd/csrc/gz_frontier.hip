@@ -14,8 +14,10 @@
 /* Vertices v0 .. v0+GZ_COMPACT_VPT-1 (v0 a multiple of GZ_COMPACT_VPT):
  * vector counts c[k] of the vertices whose bit is set, 0 otherwise; a vertex
  * is active when c[k] > 0.  Neighbouring lanes read neighbouring bits of one
- * bitmap word and neighbouring ranges of the offsets. */
-__device__ __forceinline__ void gz_run_counts(const uint64_t* __restrict__ bitmap, const uint64_t* __restrict__ offsets,
+ * bitmap word.  kNarrow reads the 32-bit vector counts (gz_frontier_counts:
+ * 64 bytes per thread in four 16-byte loads), else the 64-bit offsets. */
+template <bool kNarrow>
+__device__ __forceinline__ void gz_run_counts(const uint64_t* __restrict__ bitmap, const void* __restrict__ src,
                                               uint64_t num_vertices, uint64_t v0, uint64_t c[GZ_COMPACT_VPT])
 {
 #pragma unroll
@@ -25,11 +27,35 @@ __device__ __forceinline__ void gz_run_counts(const uint64_t* __restrict__ bitma
     const uint64_t left = num_vertices - v0;
     if (left < GZ_COMPACT_VPT) bits &= (1ull << left) - 1ull;
     if (bits == 0) return;
+    if (kNarrow) {
+        const uint4* q = (const uint4*)((const uint32_t*)src + v0);
+#pragma unroll
+        for (int k = 0; k < GZ_COMPACT_VPT / 4; ++k) {
+            const uint4 x = q[k];
+            c[4 * k] = ((bits >> (4 * k)) & 1ull) ? x.x : 0;
+            c[4 * k + 1] = ((bits >> (4 * k + 1)) & 1ull) ? x.y : 0;
+            c[4 * k + 2] = ((bits >> (4 * k + 2)) & 1ull) ? x.z : 0;
+            c[4 * k + 3] = ((bits >> (4 * k + 3)) & 1ull) ? x.w : 0;
+        }
+        return;
+    }
+    const uint64_t* __restrict__ offsets = (const uint64_t*)src;
     uint64_t o[GZ_COMPACT_VPT + 1];
 #pragma unroll
     for (int k = 0; k <= GZ_COMPACT_VPT; ++k) o[k] = offsets[(v0 + k < num_vertices) ? v0 + k : num_vertices];
 #pragma unroll
     for (int k = 0; k < GZ_COMPACT_VPT; ++k) c[k] = ((bits >> k) & 1ull) ? o[k + 1] - o[k] : 0;
+}
+
+/* counts[v] = offsets[v + 1] - offsets[v] for v < num_vertices, 0 up to
+ * `padded` (a multiple of GZ_COMPACT_VPT). */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_vector_counts(const uint64_t* __restrict__ offsets,
+                                                               uint64_t num_vertices, uint64_t padded,
+                                                               uint32_t* __restrict__ counts)
+{
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < padded;
+         v += (uint64_t)gridDim.x * blockDim.x)
+        counts[v] = (v < num_vertices) ? (uint32_t)(offsets[v + 1] - offsets[v]) : 0u;
 }
 
 template <typename T>
@@ -56,8 +82,9 @@ __device__ __forceinline__ T gz_block_exclusive_scan(T x, T* lds, T& block_total
     return wave_off + incl - x;
 }
 
+template <bool kNarrow>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_count(const uint64_t* __restrict__ bitmap,
-                                                                 const uint64_t* __restrict__ offsets,
+                                                                 const void* __restrict__ src,
                                                                  uint64_t num_vertices,
                                                                  uint64_t* __restrict__ block_sums,
                                                                  uint64_t* __restrict__ stamp)
@@ -66,7 +93,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_count(const uint64_t* 
     if (stamp != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *stamp = (uint64_t)wall_clock64();
     const uint64_t v0 = (uint64_t)blockIdx.x * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
     uint64_t c[GZ_COMPACT_VPT];
-    gz_run_counts(bitmap, offsets, num_vertices, v0, c);
+    gz_run_counts<kNarrow>(bitmap, src, num_vertices, v0, c);
     unsigned long long nv = 0, nvec = 0;
 #pragma unroll
     for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
@@ -169,8 +196,9 @@ __global__ __launch_bounds__(1024) void gz_k_frontier_scan(uint64_t* __restrict_
     }
 }
 
+template <bool kNarrow>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_write(const uint64_t* __restrict__ bitmap,
-                                                                 const uint64_t* __restrict__ offsets,
+                                                                 const void* __restrict__ src,
                                                                  uint64_t num_vertices,
                                                                  const uint64_t* __restrict__ block_sums,
                                                                  uint64_t* __restrict__ list,
@@ -179,7 +207,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_write(const uint64_t* 
     __shared__ unsigned long long lds[GZ_WAVES_PER_BLOCK];
     const uint64_t v0 = (uint64_t)blockIdx.x * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
     uint64_t c[GZ_COMPACT_VPT];
-    gz_run_counts(bitmap, offsets, num_vertices, v0, c);
+    gz_run_counts<kNarrow>(bitmap, src, num_vertices, v0, c);
     unsigned long long nv = 0, nvec = 0;
 #pragma unroll
     for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
@@ -200,7 +228,49 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_write(const uint64_t* 
     }
 }
 
-void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices)
+/* 32-bit vector counts of the two lists' vertices, built on first use for
+ * an offsets array, vertex count and list length, and kept until the graph
+ * is released (gz_graph_release calls gz_frontier_counts_release). */
+static struct {
+    const uint64_t* offsets;
+    uint64_t num_vertices, vector_count;
+    uint32_t* counts;
+} gz_counts_cache[2];
+static int gz_counts_next;
+
+extern "C" void gz_frontier_counts_release(void)
+{
+    for (auto& e : gz_counts_cache) {
+        gz_device_free(e.counts);
+        e = {nullptr, 0, 0, nullptr};
+    }
+    gz_counts_next = 0;
+}
+
+static const uint32_t* gz_frontier_counts(const uint64_t* offsets, uint64_t num_vertices, uint64_t vector_count)
+{
+    for (auto& e : gz_counts_cache)
+        if (e.counts != nullptr && e.offsets == offsets && e.num_vertices == num_vertices &&
+            e.vector_count == vector_count)
+            return e.counts;
+    auto& e = gz_counts_cache[gz_counts_next];
+    gz_counts_next ^= 1;
+    gz_device_free(e.counts);
+    const uint64_t padded = (num_vertices + GZ_COMPACT_VPT - 1) / GZ_COMPACT_VPT * GZ_COMPACT_VPT;
+    e.counts = (uint32_t*)gz_device_malloc(padded * sizeof(uint32_t));
+    uint64_t blocks = (padded + GZ_BLOCK - 1) / GZ_BLOCK;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(gz_k_vector_counts, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, offsets,
+                       num_vertices, padded, e.counts);
+    GZ_LAUNCH_CHECK();
+    e.offsets = offsets;
+    e.num_vertices = num_vertices;
+    e.vector_count = vector_count;
+    return e.counts;
+}
+
+void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
+                         uint64_t vector_count)
 {
     const uint64_t blocks = gz_compact_blocks(num_vertices);
     if (blocks == 0) {
@@ -208,17 +278,30 @@ void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64
         gz_device_zero(gz_scratch.active_prefix, sizeof(uint64_t));
         return;
     }
+    /* below 2^32 vectors in the list every vertex's count fits 32 bits */
+    const bool narrow = vector_count < (1ull << 32);
+    const void* src =
+        narrow ? (const void*)gz_frontier_counts(offsets, num_vertices, vector_count) : (const void*)offsets;
     gz_ktimer_begin(GZ_K_COMPACT);
-    hipLaunchKernelGGL(gz_k_frontier_count, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, offsets,
-                       num_vertices, gz_scratch.active_block_sums, gz_scratch.stamp_target);
+    if (narrow)
+        hipLaunchKernelGGL(gz_k_frontier_count<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, src,
+                           num_vertices, gz_scratch.active_block_sums, gz_scratch.stamp_target);
+    else
+        hipLaunchKernelGGL(gz_k_frontier_count<false>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap,
+                           src, num_vertices, gz_scratch.active_block_sums, gz_scratch.stamp_target);
     gz_scratch.stamp_target = nullptr;
     GZ_LAUNCH_CHECK();
     hipLaunchKernelGGL(gz_k_frontier_scan, dim3(1), dim3(1024), 0, gz_stream, gz_scratch.active_block_sums, blocks,
                        gz_scratch.active_totals, gz_scratch.active_prefix);
     GZ_LAUNCH_CHECK();
-    hipLaunchKernelGGL(gz_k_frontier_write, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, offsets,
-                       num_vertices, gz_scratch.active_block_sums, gz_scratch.active_vertices,
-                       gz_scratch.active_prefix);
+    if (narrow)
+        hipLaunchKernelGGL(gz_k_frontier_write<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, src,
+                           num_vertices, gz_scratch.active_block_sums, gz_scratch.active_vertices,
+                           gz_scratch.active_prefix);
+    else
+        hipLaunchKernelGGL(gz_k_frontier_write<false>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap,
+                           src, num_vertices, gz_scratch.active_block_sums, gz_scratch.active_vertices,
+                           gz_scratch.active_prefix);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_COMPACT);
 }

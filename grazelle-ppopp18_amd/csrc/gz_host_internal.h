@@ -60,6 +60,8 @@ void gz_cc_push_vote_hint(uint64_t vote);
 /* The same for the next BFS push (the vote of the level that built its
  * frontier). */
 void gz_bfs_push_vote_hint(uint64_t vote);
+/* Drop the frontier compaction's per-vertex vector counts (graph release). */
+void gz_frontier_counts_release(void);
 /* The source-sliced pull's settings: hot cache, cold spread, cold stream. */
 void gz_slices_settings(uint32_t* out3);
 

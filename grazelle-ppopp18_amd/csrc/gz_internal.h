@@ -207,7 +207,8 @@ uint32_t gz_vectors_per_lane(void);
 /* Reduce gz_scratch.partials[0..count) into dst[0] (bit pattern). */
 void gz_reduce_partials(uint64_t* dst);
 /* Build the compacted active-vertex list from `bitmap` against `offsets`. */
-void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices);
+void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
+                         uint64_t vector_count);
 
 /* ---- device helpers ------------------------------------------------------- */
 
