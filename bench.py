@@ -341,8 +341,8 @@ def main() -> int:
                     help="only run the device copy kernel on MIB mebibytes (rocprof counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}). "
-              "The per-GPU shard design is described in DESIGN.md.", file=sys.stderr)
+        print(f"bench.py measures one GPU: there is no multi-GPU path (asked for --gpus {args.gpus}); "
+              "the value is one GPU's throughput.", file=sys.stderr)
         return 2
     import grazelle as gz
 
