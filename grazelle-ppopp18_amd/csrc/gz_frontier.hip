@@ -228,6 +228,10 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_write(const uint64_t* 
     }
 }
 
+static int gz_compact_narrow = 1;
+
+extern "C" void gz_frontier_compact_narrow(int on) { gz_compact_narrow = on ? 1 : 0; }
+
 /* 32-bit vector counts of the two lists' vertices, built on first use for
  * an offsets array, vertex count and list length, and kept until the graph
  * is released (gz_graph_release calls gz_frontier_counts_release). */
@@ -279,7 +283,7 @@ void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64
         return;
     }
     /* below 2^32 vectors in the list every vertex's count fits 32 bits */
-    const bool narrow = vector_count < (1ull << 32);
+    const bool narrow = gz_compact_narrow && vector_count < (1ull << 32);
     const void* src =
         narrow ? (const void*)gz_frontier_counts(offsets, num_vertices, vector_count) : (const void*)offsets;
     gz_ktimer_begin(GZ_K_COMPACT);

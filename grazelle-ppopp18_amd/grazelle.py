@@ -98,6 +98,7 @@ _sig("gz_graph_device_vertex", u64, u64)
 _sig("gz_graph_build_source_slices", C.c_int, u64, u32, u32, u64)
 _sig("gz_pr_graph_replay", None, C.c_int)
 _sig("gz_graph_replay_counts", None, C.POINTER(u64), C.POINTER(u64))
+_sig("gz_frontier_compact_narrow", None, C.c_int)
 _sig("gz_graph_source_slices_info", C.c_int, vp, P(u64), P(u32), P(u32), P(u64))
 _sig("gz_slices_chunk_table", u64, vp, u32, u64, u32, vp, u64)
 _sig("gz_slices_set_cold_spread", None, u32)
@@ -425,6 +426,12 @@ def pr_graph_replay(on: bool) -> None:
     """PageRank iterations 2.. from a replayed HIP graph (default on)."""
     init()
     lib.gz_pr_graph_replay(int(on))
+
+
+def compact_narrow(on: bool) -> None:
+    """Frontier compaction over 32-bit vector counts (default) or 64-bit offsets."""
+    init()
+    lib.gz_frontier_compact_narrow(int(on))
 
 
 def graph_replay_counts() -> tuple[int, int]:

@@ -173,6 +173,12 @@ void gz_pr_graph_replay(int on);
 /* Graph captures and graph launches made so far (no reference counterpart). */
 void gz_graph_replay_counts(uint64_t* captures, uint64_t* launches);
 
+/* Frontier compaction reads 32-bit per-vertex vector counts when the list has
+ * fewer than 2^32 vectors (default on); 0 makes it read the 64-bit offsets, as
+ * it does for larger lists, so that both paths can be compared (no reference
+ * counterpart). */
+void gz_frontier_compact_narrow(int on);
+
 /* -------- batches of independent graphs (no reference counterpart) -------
  * The loaded graph is a disjoint union; gz_batch_attach names the graphs.
  * CC and BFS need nothing more (the ordinary drivers work on the union).

@@ -225,6 +225,33 @@ def test_bfs_push_high_degree_root(gz, orc, device, engine, L):
         assert gz.records()[0].engine == 2
 
 
+@pytest.mark.parametrize("narrow", [True, False])
+def test_compaction_counts_and_offsets(gz, orc, device, rmat14, narrow):
+    # the frontier compaction over 32-bit vector counts and over the 64-bit
+    # offsets (the path of lists of 2^32 vectors or more): the same results,
+    # with the push and the bottom-up levels compacting (count-only votes
+    # keep small CC pushes on the compaction too)
+    g = rmat14
+    G = _orc_graph(orc, g)
+    gz.load(g)
+    gz.compact_narrow(narrow)
+    try:
+        engines = set()
+        for root in (0, 777):
+            gz.run(gz.options("bfs", root=root, profile=True))
+            par, dep, log, _ = orc.bfs(G, orc.options(root=root))
+            p, d = gz.bfs_result()
+            assert np.array_equal(d, dep) and np.array_equal(p, par)
+            assert [(x.engine, x.frontier_stat) for x in gz.records()] == log
+            engines |= {x.engine for x in gz.records()}
+        assert engines == {1, 2}
+        gz.run(gz.options("cc", stat="count"))
+        lab, _, _ = orc.cc(G, orc.options(stat="count"))
+        assert np.array_equal(gz.cc_labels(), lab)
+    finally:
+        gz.compact_narrow(True)
+
+
 def test_cc_directed_rmat(gz, orc, device, rmat14):
     gz.load(rmat14)
     gz.run(gz.options("cc"))
