@@ -392,10 +392,21 @@ uint32_t gz_graph_indegree_host(uint64_t v)
     return (v < G.num_vertices) ? G.h_indeg[v] : 0u;
 }
 
+void gz_graph_indegrees(uint32_t* out)
+{
+    if (G.num_vertices == 0) return;
+    gz_copy_to_host(out, gz_graph_indegrees_device(), G.num_vertices * sizeof(uint32_t));
+}
+
 uint64_t gz_graph_device_vertex(uint64_t original)
 {
     if (G.device_ids == NULL || original >= G.num_vertices) return original;
     return G.device_ids[original];
+}
+
+void gz_graph_original_ids(uint64_t* out)
+{
+    for (uint64_t v = 0; v < G.num_vertices; ++v) out[v] = G.original_ids != NULL ? G.original_ids[v] : v;
 }
 
 /* out[original id] = in[device vertex] for V words (identity copy when the

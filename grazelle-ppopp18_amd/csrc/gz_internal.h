@@ -89,6 +89,9 @@ typedef struct gz_scratch_t {
 } gz_scratch_t;
 
 extern gz_scratch_t gz_scratch;
+/* Pieces the overflow queue accepts, 1..GZ_PUSH_QUEUE_PIECES (set by
+ * gz_push_queue_capacity); a piece beyond it is pushed by the lane that made it. */
+extern uint32_t gz_push_queue_cap;
 extern hipStream_t gz_stream;
 extern uint32_t gz_cu_count; /* compute units of the device */
 

@@ -29,6 +29,14 @@ mergeaccum_t* gz_merge_buffer = nullptr;
 uint64_t gz_merge_buffer_count = 0;
 
 gz_scratch_t gz_scratch;
+uint32_t gz_push_queue_cap = GZ_PUSH_QUEUE_PIECES;
+
+extern "C" int gz_push_queue_capacity(uint32_t pieces)
+{
+    if (pieces == 0 || pieces > GZ_PUSH_QUEUE_PIECES) return -1;
+    gz_push_queue_cap = pieces;
+    return 0;
+}
 uint32_t gz_cu_count = 1;
 hipStream_t gz_stream = nullptr;
 static int gz_device_id = -1;

@@ -122,8 +122,8 @@ template <class Op, int kWords, bool kQueue>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_push_sparse(
     Op op, const gz_vec_t* __restrict__ vecs, const uint64_t* __restrict__ offsets,
     const uint64_t* __restrict__ has_info, uint64_t num_vertices, uint64_t* __restrict__ pieces,
-    uint64_t* __restrict__ queue, uint64_t* __restrict__ partials, uint64_t* __restrict__ stats, uint32_t collect,
-    uint64_t* __restrict__ stamp)
+    uint64_t* __restrict__ queue, uint64_t queue_cap, uint64_t* __restrict__ partials, uint64_t* __restrict__ stats,
+    uint32_t collect, uint64_t* __restrict__ stamp)
 {
     constexpr int kSteps = GZ_PUSH_PIECE_VECTORS / GZ_WAVE;
     const int lane = gz_lane();
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_push_sparse(
                 const uint64_t rest = n - k * GZ_PUSH_PIECE_VECTORS;
                 const uint64_t pl = rest < GZ_PUSH_PIECE_VECTORS ? rest : GZ_PUSH_PIECE_VECTORS;
                 const uint64_t slot = at + k;
-                if (slot < GZ_PUSH_QUEUE_PIECES) {
+                if (slot < queue_cap) {
                     pieces[3 * slot] = v;
                     pieces[3 * slot + 1] = pf;
                     pieces[3 * slot + 2] = pl;
@@ -220,11 +220,14 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_push_sparse(
 /* The overflow queue of gz_k_push_sparse: wavefront gw pushes pieces gw,
  * gw + W, ... of the queue[parity] appended there.  It also clears the other
  * counter, which the next sparse push appends to (the kernel that read that
- * counter last has finished: launches on one stream run in order). */
+ * counter last has finished: launches on one stream run in order).  Its
+ * per-wave outputs (vote partial, statistics, the Op's features) go to slot
+ * wave_base + gw, after the wavefronts of gz_k_push_sparse. */
 template <class Op>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_push_pieces(Op op, const gz_vec_t* __restrict__ vecs,
                                                              const uint64_t* __restrict__ pieces,
-                                                             uint64_t* __restrict__ queue, uint32_t parity,
+                                                             uint64_t* __restrict__ queue, uint64_t queue_cap,
+                                                             uint32_t parity, uint64_t wave_base,
                                                              uint64_t* __restrict__ partials,
                                                              uint64_t* __restrict__ stats, uint32_t collect)
 {
@@ -234,7 +237,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_push_pieces(Op op, const gz_vec
     const uint64_t W = (uint64_t)gridDim.x * GZ_WAVES_PER_BLOCK;
     if (gw == 0 && lane == 0) queue[parity ^ 1u] = 0;
     uint64_t count = queue[parity];
-    if (count > GZ_PUSH_QUEUE_PIECES) count = GZ_PUSH_QUEUE_PIECES;
+    if (count > queue_cap) count = queue_cap;
     uint64_t vote = 0;
     gz_feature_acc feat{0, 0, 0};
     uint32_t vectors = 0, edges = 0;
@@ -252,7 +255,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_push_pieces(Op op, const gz_vec
         }
         gz_push_steps<kSteps>(vecs, op, nsteps, ok, vi, src, vote, feat, vectors, edges);
     }
-    gz_push_finish(op, gw, vote, feat, vectors, edges, partials, stats, collect);
+    gz_push_finish(op, wave_base + gw, vote, feat, vectors, edges, partials, stats, collect);
 }
 
 /* Whether a push over the frontier that a vote of `hint` built may skip the
@@ -306,8 +309,8 @@ static void gz_launch_push_sparse(const Op& op, const gz_vec_t* edge_list, uint6
     const uint32_t parity = gz_scratch.push_queue_parity;
 #define GZ_PUSH_SPARSE_ARGS                                                                                   \
     dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, op, edge_list, gz_graph.push_vertex_offsets,           \
-        gz_graph.frontier_has_info, V, gz_scratch.push_pieces, gz_scratch.push_queue + parity, partials,          \
-        gz_scratch.stats, gz_phase_config.collect_stats, stamp
+        gz_graph.frontier_has_info, V, gz_scratch.push_pieces, gz_scratch.push_queue + parity,                    \
+        (uint64_t)gz_push_queue_cap, partials, gz_scratch.stats, gz_phase_config.collect_stats, stamp
     if (wide) {
         hipLaunchKernelGGL((gz_k_push_sparse<Op, 64, false>), GZ_PUSH_SPARSE_ARGS);
         GZ_LAUNCH_CHECK();
@@ -315,8 +318,9 @@ static void gz_launch_push_sparse(const Op& op, const gz_vec_t* edge_list, uint6
         hipLaunchKernelGGL((gz_k_push_sparse<Op, 16, true>), GZ_PUSH_SPARSE_ARGS);
         GZ_LAUNCH_CHECK();
         hipLaunchKernelGGL((gz_k_push_pieces<Op>), dim3((unsigned)qblocks), dim3(GZ_BLOCK), 0, gz_stream, op,
-                           edge_list, gz_scratch.push_pieces, gz_scratch.push_queue, parity, partials + sw,
-                           gz_scratch.stats + 2 * sw, gz_phase_config.collect_stats);
+                           edge_list, gz_scratch.push_pieces, gz_scratch.push_queue, (uint64_t)gz_push_queue_cap,
+                           parity, sw, partials,
+                           gz_scratch.stats, gz_phase_config.collect_stats);
         GZ_LAUNCH_CHECK();
         gz_scratch.push_queue_parity = parity ^ 1u;
     }

@@ -95,6 +95,9 @@ _sig("gz_graph_load_pairs", C.c_int, u64, u64, vp, u64, vp)
 _sig("gz_graph_build_pairs_device", C.c_int, u64, u64, vp, C.c_int, u32)
 _sig("gz_graph_build_rmat_device", C.c_int, P(RmatParams), u64, C.c_int, u64, vp, u32)
 _sig("gz_graph_device_vertex", u64, u64)
+_sig("gz_graph_original_ids", None, vp)
+_sig("gz_graph_indegrees", None, vp)
+_sig("gz_push_queue_capacity", C.c_int, u32)
 _sig("gz_graph_build_source_slices", C.c_int, u64, u32, u32, u64)
 _sig("gz_pr_graph_replay", None, C.c_int)
 _sig("gz_graph_replay_counts", None, C.POINTER(u64), C.POINTER(u64))
@@ -531,6 +534,31 @@ def outdegrees() -> np.ndarray:
     """FP64 outdegrees of the loaded graph (host mirror, V entries)."""
     n = num_vertices()
     return np.ctypeslib.as_array(lib.gz_graph_outdegrees_host(), shape=(n,)).copy()
+
+
+def original_ids() -> np.ndarray:
+    """Original id of every device vertex (identity unless renumbered)."""
+    out = np.empty(num_vertices(), dtype=np.uint64)
+    lib.gz_graph_original_ids(_ptr(out))
+    return out
+
+
+def device_vertex(original: int) -> int:
+    return int(lib.gz_graph_device_vertex(original))
+
+
+def indegrees() -> np.ndarray:
+    """In-degree of every device vertex (device numbering)."""
+    out = np.empty(num_vertices(), dtype=np.uint32)
+    lib.gz_graph_indegrees(_ptr(out))
+    return out
+
+
+def push_queue_capacity(pieces: int) -> None:
+    """Overflow-queue capacity of the small-frontier push (default 131072)."""
+    init()
+    if lib.gz_push_queue_capacity(pieces) != 0:
+        raise ValueError(f"queue capacity {pieces} outside 1..131072")
 
 
 def num_edges() -> int:

@@ -54,12 +54,7 @@ def pick_roots(count: int, seed: int) -> list[int]:
     rng = np.random.default_rng(seed)
     picks = rng.choice(cand_dev.shape[0], size=min(count, cand_dev.shape[0]), replace=False)
     # outdegrees() is indexed by device vertex; map back to original ids
-    n = gz.num_vertices()
-    if gz.lib.gz_graph_device_vertex(0) == 0 and gz.lib.gz_graph_device_vertex(n - 1) == n - 1:
-        return [int(cand_dev[p]) for p in picks]
-    orig = np.empty(n, dtype=np.int64)
-    for v in range(n):
-        orig[gz.lib.gz_graph_device_vertex(v)] = v
+    orig = gz.original_ids()
     return [int(orig[cand_dev[p]]) for p in picks]
 
 

@@ -113,6 +113,13 @@ int gz_graph_build_rmat_device(const gz_rmat_params_t* p, uint64_t num_vertices,
                                uint64_t extra_edges, const uint64_t* extra_pairs, uint32_t flags);
 /* Device id of an original vertex id (identity unless renumbered). */
 uint64_t gz_graph_device_vertex(uint64_t original);
+/* out[v] = original id of device vertex v, V entries (identity unless
+ * renumbered).  The device lists (gz_graph_download_vectors) are in device
+ * numbering; this maps them back. */
+void gz_graph_original_ids(uint64_t* out);
+/* In-degree of every device vertex, V entries (device numbering), counted
+ * on the device from the loaded lists. */
+void gz_graph_indegrees(uint32_t* out);
 /* Disjoint union of `count` R-MAT graphs (params[g] as for
  * gz_graph_build_rmat_device); graph g gets the ids [bases[g], bases[g+1]),
  * bases has count+1 entries and is filled here. */
@@ -178,6 +185,14 @@ void gz_graph_replay_counts(uint64_t* captures, uint64_t* launches);
  * it does for larger lists, so that both paths can be compared (no reference
  * counterpart). */
 void gz_frontier_compact_narrow(int on);
+
+/* Pieces of at most 256 vectors that the overflow queue of the small-frontier
+ * push accepts (default and maximum 131072, which a frontier within that
+ * push's bound cannot fill).  A piece beyond the cap is pushed by the lane
+ * that cut it, with the same results; a smaller cap lets tests reach that
+ * path.  Returns -1 for 0 or a value above the maximum (no reference
+ * counterpart). */
+int gz_push_queue_capacity(uint32_t pieces);
 
 /* -------- batches of independent graphs (no reference counterpart) -------
  * The loaded graph is a disjoint union; gz_batch_attach names the graphs.

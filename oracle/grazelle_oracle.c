@@ -141,11 +141,52 @@ orc_graph_t* orc_graph_create(uint64_t num_vertices, const uint64_t* pull_pairs,
     return g;
 }
 
+orc_graph_t* orc_graph_from_vectors(uint64_t num_vertices, const orc_vec_t* pull, uint64_t pull_count,
+                                    const orc_vec_t* push, uint64_t push_count)
+{
+    orc_graph_t* g = (orc_graph_t*)calloc(1, sizeof(*g));
+    g->num_vertices = num_vertices;
+    g->borrowed = 1;
+    g->pull = (orc_vec_t*)pull;
+    g->pull_count = pull ? pull_count : 0;
+    g->push = (orc_vec_t*)push;
+    g->push_count = push ? push_count : 0;
+    g->outdeg = (double*)calloc(num_vertices + 8, sizeof(double));
+    uint64_t s, e, pull_edges = 0, push_edges = 0;
+    if (pull) {
+        for (uint64_t i = 0; i < pull_count; ++i)
+            for (int k = 0; k < 4; ++k) {
+                if (!(pull[i].lane[k] & VALID_BIT)) continue;
+                pull_edges += 1;
+                if (!push) g->outdeg[pull[i].lane[k] & NBR_MASK] += 1.0;
+            }
+        g->pull_index = (uint64_t*)malloc((num_vertices + 64) * sizeof(uint64_t));
+        orc_vertex_index(g->pull, g->pull_count, num_vertices, g->pull_index, &s, &e);
+    }
+    if (push) {
+        /* the same per-source edge counts, read in source order */
+        for (uint64_t i = 0; i < push_count; ++i) {
+            const uint64_t src = orc_shared_vertex(&push[i]);
+            for (int k = 0; k < 4; ++k)
+                if (push[i].lane[k] & VALID_BIT) {
+                    g->outdeg[src] += 1.0;
+                    push_edges += 1;
+                }
+        }
+        g->push_index = (uint64_t*)malloc((num_vertices + 64) * sizeof(uint64_t));
+        orc_vertex_index(g->push, g->push_count, num_vertices, g->push_index, &s, &e);
+    }
+    g->num_edges = pull ? pull_edges : push_edges;
+    return g;
+}
+
 void orc_graph_destroy(orc_graph_t* g)
 {
     if (!g) return;
-    free(g->pull);
-    free(g->push);
+    if (!g->borrowed) {
+        free(g->pull);
+        free(g->push);
+    }
     free(g->pull_index);
     free(g->push_index);
     free(g->outdeg);
@@ -771,13 +812,13 @@ int orc_pagerank(const orc_graph_t* g, const orc_options_t* o, double* props, do
     return 0;
 }
 
-int orc_cc(const orc_graph_t* g, const orc_options_t* o, uint64_t* labels, orc_iteration_t* log, uint64_t log_cap,
-           orc_result_t* r)
+int orc_cc_from(const orc_graph_t* g, const orc_options_t* o, const uint64_t* init, uint64_t* labels,
+                orc_iteration_t* log, uint64_t log_cap, orc_result_t* r)
 {
     orc_run_t R;
     orc_run_init(&R, g, o, r);
     const uint64_t V = g->num_vertices, E = g->num_edges;
-    for (uint64_t v = 0; v < V; ++v) R.props[v] = (double)v;          /* execution_cc.c:87-90 */
+    for (uint64_t v = 0; v < V; ++v) R.props[v] = (double)(init ? init[v] : v); /* execution_cc.c:87-90 */
     for (uint64_t w = 0; w < R.frontier_words; ++w) R.has[w] = R.wants[w] = ~0ull;
     R.vote = 0;
     if (o->stat_mode != ORC_STAT_COUNT) R.vote += E;                  /* execution_cc.c:106-118 */
@@ -788,6 +829,12 @@ int orc_cc(const orc_graph_t* g, const orc_options_t* o, uint64_t* labels, orc_i
     for (uint64_t v = 0; v < V; ++v) labels[v] = (uint64_t)R.props[v];
     orc_run_free(&R);
     return 0;
+}
+
+int orc_cc(const orc_graph_t* g, const orc_options_t* o, uint64_t* labels, orc_iteration_t* log, uint64_t log_cap,
+           orc_result_t* r)
+{
+    return orc_cc_from(g, o, NULL, labels, log, log_cap, r);
 }
 
 int orc_bfs(const orc_graph_t* g, const orc_options_t* o, int64_t* parents, uint64_t* depths, orc_iteration_t* log,
@@ -824,6 +871,69 @@ int orc_bfs(const orc_graph_t* g, const orc_options_t* o, int64_t* parents, uint
     }
     orc_run_free(&R);
     return 0;
+}
+
+/* ---- BFS certificate --------------------------------------------------------- */
+
+typedef struct orc_cert_t {
+    const orc_graph_t* g;
+    const int64_t* parents;
+    const uint64_t* depths;
+    uint8_t* bad;
+    uint8_t* found;
+    uint64_t first, last;
+} orc_cert_t;
+
+static void* orc_cert_edges(void* arg)
+{
+    orc_cert_t* c = (orc_cert_t*)arg;
+    for (uint64_t i = c->first; i < c->last; ++i) {
+        const orc_vec_t* v = &c->g->pull[i];
+        const uint64_t d = orc_shared_vertex(v);
+        for (int k = 0; k < 4; ++k) {
+            if (!(v->lane[k] & VALID_BIT)) continue;
+            const uint64_t u = v->lane[k] & NBR_MASK;
+            /* a destination's vectors can straddle two threads' ranges */
+            if (c->parents[d] >= 0 && (uint64_t)c->parents[d] == u) __atomic_store_n(&c->found[d], 1, __ATOMIC_RELAXED);
+            if (c->depths[u] != UINT64_MAX && (c->depths[d] == UINT64_MAX || c->depths[d] > c->depths[u] + 1))
+                __atomic_store_n(&c->bad[d], 1, __ATOMIC_RELAXED);
+        }
+    }
+    return NULL;
+}
+
+uint64_t orc_bfs_certificate_violations(const orc_graph_t* g, uint64_t root, const int64_t* parents,
+                                        const uint64_t* depths, uint32_t threads)
+{
+    const uint64_t V = g->num_vertices;
+    const uint32_t T = threads ? threads : 1;
+    uint8_t* bad = (uint8_t*)calloc(V ? V : 1, 1);
+    uint8_t* found = (uint8_t*)calloc(V ? V : 1, 1);
+    orc_cert_t* c = (orc_cert_t*)calloc(T, sizeof(*c));
+    pthread_t* th = (pthread_t*)calloc(T, sizeof(*th));
+    for (uint32_t t = 0; t < T; ++t) {
+        c[t] = (orc_cert_t){g, parents, depths, bad, found, g->pull_count * t / T, g->pull_count * (t + 1) / T};
+        if (t > 0) pthread_create(&th[t], NULL, orc_cert_edges, &c[t]);
+    }
+    orc_cert_edges(&c[0]);
+    for (uint32_t t = 1; t < T; ++t) pthread_join(th[t], NULL);
+    uint64_t n = 0;
+    for (uint64_t v = 0; v < V; ++v) {
+        int ok = !bad[v];
+        if (v == root)
+            ok = ok && depths[v] == 0 && parents[v] == -1;
+        else if (depths[v] == UINT64_MAX)
+            ok = ok && parents[v] == -1;
+        else
+            ok = ok && found[v] && parents[v] >= 0 && (uint64_t)parents[v] < V &&
+                 depths[parents[v]] != UINT64_MAX && depths[parents[v]] + 1 == depths[v];
+        n += ok ? 0 : 1;
+    }
+    free(c);
+    free(th);
+    free(bad);
+    free(found);
+    return n;
 }
 
 /* ---- selector MLP (restated independently of the HIP kernels) ----------------- */

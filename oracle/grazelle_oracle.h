@@ -53,11 +53,21 @@ typedef struct orc_graph_t {
     uint64_t* pull_index;     /* [V+64] reference vertex index */
     uint64_t* push_index;
     double* outdeg;           /* [V+8] */
+    int borrowed;             /* pull / push lists belong to the caller */
 } orc_graph_t;
 
 /* Pair arrays as in the -pull / -push files (grouped by dst / src). */
 orc_graph_t* orc_graph_create(uint64_t num_vertices, const uint64_t* pull_pairs, uint64_t pull_edges,
                               const uint64_t* push_pairs, uint64_t push_edges);
+/* Graph over Vector-Sparse lists that are already packed (for instance a
+ * device ingress's lists copied back to the host).  The lists are borrowed
+ * and must outlive the graph.  E = valid lanes of the pull list (of the push
+ * list when there is no pull list); outdegrees count each valid pull lane for
+ * its neighbour, the source, as graphdata.c:792-821 counts the pull file's
+ * source column; with a push list they are read from it instead, as each
+ * source's valid lanes there, which is the same count in source order). */
+orc_graph_t* orc_graph_from_vectors(uint64_t num_vertices, const orc_vec_t* pull, uint64_t pull_count,
+                                    const orc_vec_t* push, uint64_t push_count);
 void orc_graph_destroy(orc_graph_t* g);
 uint64_t orc_graph_pull_count(const orc_graph_t* g);
 uint64_t orc_graph_push_count(const orc_graph_t* g);
@@ -94,9 +104,29 @@ int orc_pagerank(const orc_graph_t* g, const orc_options_t* o, double* props, do
 /* labels[V]; log (optional, capacity log_cap) receives one entry per iteration. */
 int orc_cc(const orc_graph_t* g, const orc_options_t* o, uint64_t* labels, orc_iteration_t* log, uint64_t log_cap,
            orc_result_t* r);
+/* As orc_cc with the labels starting from init[V] instead of the vertex ids
+ * (execution_cc.c:87-90 starts from the ids).  A graph whose vertices were
+ * renumbered starts from the original ids, so that every label is the
+ * smallest original id that reaches the vertex.  init values must be below
+ * 2^53 (labels are exact FP64 integers, as in the reference). */
+int orc_cc_from(const orc_graph_t* g, const orc_options_t* o, const uint64_t* init, uint64_t* labels,
+                orc_iteration_t* log, uint64_t log_cap, orc_result_t* r);
 /* parents[V] (-1 root / unreached) and depths[V] (UINT64_MAX unreached). */
 int orc_bfs(const orc_graph_t* g, const orc_options_t* o, int64_t* parents, uint64_t* depths, orc_iteration_t* log,
             uint64_t log_cap, orc_result_t* r);
+
+/* BFS certificate over the pull list (a checker, no reference counterpart):
+ * counts the vertices that violate what fixes BFS depths and a valid BFS tree.
+ *   root: depth 0, parent -1;
+ *   every other reached v: (parents[v], v) is an edge and
+ *     depths[parents[v]] == depths[v] - 1;
+ *   unreached v (depth UINT64_MAX): parent -1;
+ *   every edge (u, v) with u reached: depths[v] <= depths[u] + 1.
+ * The parent chains bound each depth from above by the distance and the
+ * edge condition bounds it from below, so zero violations means the depths
+ * are the BFS distances and the parents a BFS tree. */
+uint64_t orc_bfs_certificate_violations(const orc_graph_t* g, uint64_t root, const int64_t* parents,
+                                        const uint64_t* depths, uint32_t threads);
 
 /* ---- learned selector MLP (8->32->32->2, FP32) ---------------------------- */
 void orc_mlp_forward(const float* params, const float* x, float* y);

@@ -47,12 +47,15 @@ _sig("orc_unit_range", None, u64, u64, u64, P(u64), P(u64))
 _sig("orc_merge", None, vp, u64, vp)
 _sig("orc_shared_vertex", u64, vp)
 _sig("orc_graph_create", vp, u64, vp, u64, vp, u64)
+_sig("orc_graph_from_vectors", vp, u64, vp, u64, vp, u64)
 _sig("orc_graph_destroy", None, vp)
 _sig("orc_graph_pull_count", u64, vp)
 _sig("orc_graph_push_count", u64, vp)
 _sig("orc_pagerank", C.c_int, vp, P(Options), vp, vp, P(Result))
 _sig("orc_cc", C.c_int, vp, P(Options), vp, vp, u64, P(Result))
+_sig("orc_cc_from", C.c_int, vp, P(Options), vp, vp, vp, u64, P(Result))
 _sig("orc_bfs", C.c_int, vp, P(Options), vp, vp, vp, u64, P(Result))
+_sig("orc_bfs_certificate_violations", u64, vp, u64, vp, vp, u32)
 _sig("orc_mlp_forward", None, vp, vp, vp)
 _sig("orc_mlp_gradient", C.c_float, vp, vp, vp, u64, vp)
 
@@ -108,6 +111,20 @@ class Graph:
             None if self._pull is None else _ptr(self._pull), 0 if self._pull is None else self._pull.shape[0],
             None if self._push is None else _ptr(self._push), 0 if self._push is None else self._push.shape[0])
 
+    @classmethod
+    def from_vectors(cls, num_vertices: int, pull: np.ndarray | None, push: np.ndarray | None) -> "Graph":
+        """Graph over already packed (N, 4) uint64 Vector-Sparse lists, kept
+        by reference (no copy), e.g. a device ingress's lists copied back."""
+        self = cls.__new__(cls)
+        self._pull = None if pull is None else np.ascontiguousarray(pull, dtype=np.uint64)
+        self._push = None if push is None else np.ascontiguousarray(push, dtype=np.uint64)
+        self.num_vertices = num_vertices
+        self.h = lib.orc_graph_from_vectors(
+            num_vertices,
+            None if self._pull is None else _ptr(self._pull), 0 if self._pull is None else self._pull.shape[0],
+            None if self._push is None else _ptr(self._push), 0 if self._push is None else self._push.shape[0])
+        return self
+
     def __del__(self):
         if getattr(self, "h", None):
             lib.orc_graph_destroy(self.h)
@@ -132,11 +149,18 @@ def pagerank(g: Graph, o: Options):
     return props, ranks, r
 
 
-def cc(g: Graph, o: Options, log_cap: int = 4096):
+def cc(g: Graph, o: Options, log_cap: int = 4096, init: np.ndarray | None = None):
+    """Labels from min-label propagation; init (V uint64, < 2^53) replaces the
+    starting labels, which are the vertex ids otherwise."""
     labels = np.empty(g.num_vertices, dtype=np.uint64)
     log = (Iteration * log_cap)()
     r = Result()
-    lib.orc_cc(g.h, C.byref(o), _ptr(labels), C.cast(log, vp), log_cap, C.byref(r))
+    if init is None:
+        lib.orc_cc(g.h, C.byref(o), _ptr(labels), C.cast(log, vp), log_cap, C.byref(r))
+    else:
+        init = np.ascontiguousarray(init, dtype=np.uint64)
+        assert init.shape[0] >= g.num_vertices
+        lib.orc_cc_from(g.h, C.byref(o), _ptr(init), _ptr(labels), C.cast(log, vp), log_cap, C.byref(r))
     return labels, [(x.engine, x.vote) for x in log[: min(r.iterations, log_cap)]], r
 
 
@@ -149,6 +173,15 @@ def bfs(g: Graph, o: Options, log_cap: int = 4096):
     if rc != 0:
         raise ValueError("invalid BFS root")
     return parents, depths, [(x.engine, x.vote) for x in log[: min(r.iterations, log_cap)]], r
+
+
+def bfs_certificate_violations(g: Graph, root: int, parents: np.ndarray, depths: np.ndarray, threads: int = 1) -> int:
+    """Vertices violating the BFS certificate over the pull list (0: depths are
+    the BFS distances from root and the parents form a BFS tree)."""
+    p = np.ascontiguousarray(parents, dtype=np.int64)
+    d = np.ascontiguousarray(depths, dtype=np.uint64)
+    assert p.shape[0] >= g.num_vertices and d.shape[0] >= g.num_vertices
+    return int(lib.orc_bfs_certificate_violations(g.h, root, _ptr(p), _ptr(d), threads))
 
 
 def mlp_forward(params: np.ndarray, x: np.ndarray) -> np.ndarray:

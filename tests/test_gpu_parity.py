@@ -225,6 +225,37 @@ def test_bfs_push_high_degree_root(gz, orc, device, engine, L):
         assert gz.records()[0].engine == 2
 
 
+@pytest.mark.parametrize("cap", [1, 3, 64])
+def test_push_queue_full_fallback(gz, orc, device, cap):
+    # a 20,000-leaf star behind a 200-vertex path: the centre's 5,001
+    # vectors are cut into 20 pieces for the overflow queue; with room for
+    # fewer pieces the lanes that cut the rest push them themselves, and
+    # labels, depths and parents stay exact
+    P, L = 200, 20000
+    pairs = [(i, i + 1) for i in range(P)] + [(P, P + 1 + j) for j in range(L)]
+    g = gz.make_graph(np.asarray(pairs, dtype=np.uint64), P + 1 + L, symmetrize=True)
+    G = _orc_graph(orc, g)
+    gz.load(g)
+    gz.push_queue_capacity(cap)
+    try:
+        gz.run(gz.options("cc", engine="push", collect_stats=True))
+        lab, _, _ = orc.cc(G, orc.options(engine="push"))
+        assert np.array_equal(gz.cc_labels(), lab)
+        assert any(4096 < r.frontier_stat <= 65536 for r in gz.records()[1:])
+        for engine in ("push", "hybrid"):
+            gz.run(gz.options("bfs", engine=engine, root=P, profile=True))
+            par, dep, log, _ = orc.bfs(G, orc.options(engine=engine, root=P))
+            p, d = gz.bfs_result()
+            assert np.array_equal(d, dep) and np.array_equal(p, par), engine
+            assert [(x.engine, x.frontier_stat) for x in gz.records()] == log
+    finally:
+        gz.push_queue_capacity(131072)
+    with pytest.raises(ValueError):
+        gz.push_queue_capacity(0)
+    with pytest.raises(ValueError):
+        gz.push_queue_capacity(131073)
+
+
 @pytest.mark.parametrize("narrow", [True, False])
 def test_compaction_counts_and_offsets(gz, orc, device, rmat14, narrow):
     # the frontier compaction over 32-bit vector counts and over the 64-bit

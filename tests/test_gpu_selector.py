@@ -65,11 +65,33 @@ def rmat13(gz):
     return gz.rmat_graph(13, 16, seed=17)
 
 
-def test_probe_features_and_sequence(gz, orc, device, rmat13):
-    g = rmat13
+def _star_path(gz, leaves=20000, path=200, other=40000):
+    """A path 0..path into the centre of a star with `leaves` leaves, plus a
+    disconnected star with `other` leaves whose edges raise E/5 above the
+    centre's frontier statistic (1 + leaves + 1), so that the hybrid pushes
+    the centre's level."""
+    pairs = [(i, i + 1) for i in range(path)] + [(path, path + 1 + j) for j in range(leaves)]
+    c2 = path + 1 + leaves
+    pairs += [(c2, c2 + 1 + j) for j in range(other)]
+    return gz.make_graph(np.asarray(pairs, dtype=np.uint64), c2 + 1 + other, symmetrize=True)
+
+
+@pytest.mark.parametrize("case", ["rmat13", "star_path", "rmat18"])
+def test_probe_features_and_sequence(gz, orc, device, rmat13, case):
+    # the star-and-path graph from the far end of the path, and R-MAT s18,
+    # reach push levels whose frontier statistic lies between 4,096 and
+    # 65,536: the small-frontier push with the overflow queue, whose queue
+    # kernel writes its features after the bitmap kernel's wavefronts
+    if case == "rmat13":
+        g, root = rmat13, 3
+    elif case == "star_path":
+        g, root = _star_path(gz), 0
+    else:
+        # root 0 has 15,919 out-edges: its level and the fourth are pushes
+        # with statistics of 15,920 and 30,459
+        g, root = gz.rmat_graph(18, 16, seed=29), 0
     gz.load(g)
     G = orc.Graph(g.num_vertices, g.pull, g.push)
-    root = 3
     gz.run(gz.options("bfs", engine="hybrid", root=root, profile=True))
     hybrid = [r.engine for r in gz.records()]
     gz.run(gz.options("bfs", engine="probe", root=root))
@@ -92,6 +114,9 @@ def test_probe_features_and_sequence(gz, orc, device, rmat13):
                np.log1p(E - ind[seen].sum()), i + 1, 0 if i == 0 else hybrid[i - 1]]
         assert np.allclose(f[:6], exp, rtol=1e-5, atol=1e-5), (i, f, exp)
         assert f[7] == pytest.approx(E / V, rel=1e-6)
+    if case != "rmat13":
+        # at least one level went through the overflow queue's statistic range
+        assert any(4096 < r.frontier_stat <= 65536 and r.engine == 2 for r in recs), case
 
 
 @pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="selector weights not trained yet")

@@ -248,3 +248,135 @@ def test_mlp_gradient_finite_differences(orc):
         assert abs(num - g[i]) <= 2e-3 + 2e-2 * abs(g[i])
     y = orc.mlp_forward(p, X[0])
     assert y.shape == (2,)
+
+
+def _np_mlp(p, X):
+    """The selector's forward pass restated with numpy matmuls over the
+    parameter layout the selector header documents: W1[32][8] b1[32]
+    W2[32][32] b2[32] W3[2][32] b3[2], row-major, ReLU after both hidden
+    layers, a linear output."""
+    p = p.astype(np.float64)
+    o = 0
+    W1 = p[o:o + 256].reshape(32, 8); o += 256
+    b1 = p[o:o + 32]; o += 32
+    W2 = p[o:o + 1024].reshape(32, 32); o += 1024
+    b2 = p[o:o + 32]; o += 32
+    W3 = p[o:o + 64].reshape(2, 32); o += 64
+    b3 = p[o:o + 2]; o += 2
+    assert o == p.shape[0]
+    h1 = np.maximum(X.astype(np.float64) @ W1.T + b1, 0.0)
+    h2 = np.maximum(h1 @ W2.T + b2, 0.0)
+    return h2 @ W3.T + b3
+
+
+def test_mlp_forward_matches_numpy(orc):
+    # random weights with non-zero biases; the hidden layers are non-square
+    # (32x8) or asymmetric (32x32), so a transposed matrix, a swapped bias or
+    # a missing ReLU changes the outputs
+    rng = np.random.default_rng(11)
+    p = (rng.standard_normal(1410) * 0.4).astype(np.float32)
+    X = rng.standard_normal((200, 8)).astype(np.float32)
+    want = _np_mlp(p, X)
+    got = np.stack([orc.mlp_forward(p, x) for x in X])
+    assert np.max(np.abs(got - want)) <= 1e-5 * max(1.0, float(np.max(np.abs(want))))
+    # the loss the gradient routine reports is the mean squared error of
+    # the same forward pass over both outputs
+    T = rng.standard_normal((200, 2)).astype(np.float32)
+    loss, _ = orc.mlp_gradient(p, X, T)
+    assert loss == pytest.approx(float(np.mean((want - T) ** 2)), rel=1e-5)
+
+
+def test_mlp_forward_hand_case(orc):
+    # y0 = relu(relu(x0 - x1) * 2 + 1) * 3 - 4, y1 = relu(x2) via the second
+    # unit of each layer; all other weights zero
+    p = np.zeros(1410, np.float32)
+    W1, b1, W2, b2, W3, b3 = 0, 256, 288, 1312, 1344, 1408
+    p[W1 + 0 * 8 + 0], p[W1 + 0 * 8 + 1] = 1.0, -1.0
+    p[W1 + 1 * 8 + 2] = 1.0
+    p[W2 + 0 * 32 + 0], p[b2 + 0] = 2.0, 1.0
+    p[W2 + 1 * 32 + 1] = 1.0
+    p[W3 + 0 * 32 + 0], p[b3 + 0] = 3.0, -4.0
+    p[W3 + 1 * 32 + 1] = 1.0
+    for x0, x1, x2 in ((3.0, 1.0, 2.0), (1.0, 3.0, -2.0), (0.5, 0.25, 7.0)):
+        x = np.array([x0, x1, x2, 9, 9, 9, 9, 9], np.float32)
+        y = orc.mlp_forward(p, x)
+        assert y[0] == pytest.approx(max(max(x0 - x1, 0) * 2 + 1, 0) * 3 - 4, abs=1e-6)
+        assert y[1] == pytest.approx(max(x2, 0), abs=1e-6)
+
+
+# ---- graphs over packed lists, CC from given labels ------------------------------
+
+def test_graph_from_vectors_matches_pairs(gz, orc):
+    g = gz.rmat_graph(11, 12, seed=23)
+    G = orc.Graph(g.num_vertices, g.pull, g.push)
+    H = orc.Graph.from_vectors(g.num_vertices, orc.pack(g.pull, True), orc.pack(g.push, False))
+    _, r1, _ = orc.pagerank(G, orc.options(iterations=10))
+    _, r2, _ = orc.pagerank(H, orc.options(iterations=10))
+    assert np.array_equal(r1.view(np.uint64), r2.view(np.uint64))
+    assert np.array_equal(orc.cc(G, orc.options())[0], orc.cc(H, orc.options())[0])
+    for root in (0, 9):
+        a = orc.bfs(G, orc.options(root=root))
+        b = orc.bfs(H, orc.options(root=root))
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and a[2] == b[2]
+    # without a pull list: E and outdegrees from the push list
+    P = orc.Graph.from_vectors(g.num_vertices, None, orc.pack(g.push, False))
+    assert np.array_equal(orc.cc(P, orc.options(engine="push"))[0], orc.cc(G, orc.options(engine="push"))[0])
+
+
+def test_cc_from_original_ids_on_renumbered_graph(gz, orc):
+    # renumber a directed graph by a random permutation (new = perm[old]);
+    # CC on the renumbered copy, started from the original ids, gives each
+    # vertex the label the original graph gives it: the smallest original id
+    # that reaches it
+    g = gz.rmat_graph(10, 6, seed=31)
+    V = g.num_vertices
+    perm = np.random.default_rng(2).permutation(V).astype(np.uint64)
+    orig = np.empty(V, np.uint64)
+    orig[perm] = np.arange(V, dtype=np.uint64)
+    h = gz.make_graph(perm[g.pull.astype(np.int64)], V)
+    G = orc.Graph(V, g.pull, g.push)
+    H = orc.Graph(V, h.pull, h.push)
+    for engine in ("hybrid", "pull", "push"):
+        want = orc.cc(G, orc.options(engine=engine))[0]
+        got = orc.cc(H, orc.options(engine=engine), init=orig)[0]
+        assert np.array_equal(got[perm.astype(np.int64)], want)
+    # started from the ids, the renumbered copy's labels differ
+    assert not np.array_equal(orc.cc(H, orc.options())[0][perm.astype(np.int64)], orc.cc(G, orc.options())[0])
+
+
+def test_bfs_certificate(gz, orc):
+    # accepts the BFS the oracle computes and each planted mistake fails it:
+    # a depth one too large, one too small, a parent that is no in-neighbour,
+    # a parent one level too high, an unreached vertex marked reached
+    g = gz.rmat_graph(11, 8, seed=37)
+    G = orc.Graph(g.num_vertices, g.pull, g.push)
+    root = 5
+    par, dep, _, _ = orc.bfs(G, orc.options(root=root))
+    assert orc.bfs_certificate_violations(G, root, par, dep) == 0
+    _, d8, _, _ = orc.bfs(G, orc.options(root=root, threads=8, engine="push"))
+    p8 = orc.bfs(G, orc.options(root=root, threads=8, engine="push"))[0]
+    assert orc.bfs_certificate_violations(G, root, p8, d8, threads=4) == 0
+    reached = np.nonzero((dep != np.iinfo(np.uint64).max) & (np.arange(g.num_vertices) != root))[0]
+    deep = reached[np.argmax(dep[reached])]
+    src, dst = g.pull[:, 0].astype(np.int64), g.pull[:, 1].astype(np.int64)
+    for mutate in ("deeper", "shallower", "not_edge", "wrong_level", "unreached"):
+        p, d = par.copy(), dep.copy()
+        if mutate == "deeper":
+            d[deep] += 1
+        elif mutate == "shallower":
+            d[deep] -= 1
+        elif mutate == "not_edge":
+            ins = set(src[dst == deep].tolist())
+            p[deep] = next(u for u in reached if u not in ins and u != deep)
+        elif mutate == "wrong_level":
+            ins = src[dst == deep]
+            cand = [u for u in ins if dep[u] != np.iinfo(np.uint64).max and dep[u] + 1 != dep[deep]]
+            if not cand:
+                continue
+            p[deep] = cand[0]
+        else:
+            un = np.nonzero(dep == np.iinfo(np.uint64).max)[0]
+            if un.size == 0:
+                continue
+            d[un[0]], p[un[0]] = 3, root
+        assert orc.bfs_certificate_violations(G, root, p, d) > 0, mutate
