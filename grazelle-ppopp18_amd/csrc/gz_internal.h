@@ -209,6 +209,16 @@ uint64_t gz_chunk_len(void);
 uint32_t gz_vectors_per_lane(void);
 /* Reduce gz_scratch.partials[0..count) into dst[0] (bit pattern). */
 void gz_reduce_partials(uint64_t* dst);
+/* Deterministic two-stage FP64 sum of in[count] into out[0] (as bits): up to
+ * 1024 blocks write slots[], the last one sums them and clears *done, which
+ * must be 0 when the launch starts. */
+void gz_reduce_f64_into(const double* in, uint64_t count, double* slots, unsigned int* done, uint64_t* out);
+/* PageRank kernels over explicit arrays (gz_pr.hip). */
+void gz_launch_pr_pull_explicit(const gz_vec_t* vecs, uint64_t nvec, uint64_t chunk_len, uint64_t nchunks,
+                                const double* props, double* acc, mergeaccum_t* merge, double* partials);
+void gz_launch_pr_vertex_explicit(double* props, double* acc, const double* outdeg, uint64_t num_vertices,
+                                  const uint64_t* sum_word, double damping);
+void gz_launch_merge_sum(const mergeaccum_t* merge, uint64_t count, double* acc);
 /* Build the compacted active-vertex list from `bitmap` against `offsets`. */
 void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
                          uint64_t vector_count);

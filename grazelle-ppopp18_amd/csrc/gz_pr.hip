@@ -124,6 +124,28 @@ extern "C" void perform_edge_pull_phase_pr(const gz_vec_t* edge_list, uint64_t e
     gz_ktimer_end(GZ_K_PULL_PR);
 }
 
+/* The scheduler-aware pull over explicit arrays (the allocation-free
+ * PageRank iteration): the same kernel, merge records and partials into the
+ * caller's buffers, no statistics. */
+void gz_launch_pr_pull_explicit(const gz_vec_t* vecs, uint64_t nvec, uint64_t chunk_len, uint64_t nchunks,
+                                const double* props, double* acc, mergeaccum_t* merge, double* partials)
+{
+    if (nchunks == 0) return;
+    const uint64_t blocks = (nchunks + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+    const uint32_t vpl = gz_vectors_per_lane();
+#define GZ_PULL_PR_EXPLICIT_ARGS                                                                               \
+    dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, vecs, nvec, chunk_len, nchunks, props, acc, merge,     \
+        partials, (uint64_t*)nullptr, 0u
+    if (vpl == 4)
+        hipLaunchKernelGGL((gz_k_edge_pull_pr<true, 4>), GZ_PULL_PR_EXPLICIT_ARGS);
+    else if (vpl == 2)
+        hipLaunchKernelGGL((gz_k_edge_pull_pr<true, 2>), GZ_PULL_PR_EXPLICIT_ARGS);
+    else
+        hipLaunchKernelGGL((gz_k_edge_pull_pr<true, 1>), GZ_PULL_PR_EXPLICIT_ARGS);
+#undef GZ_PULL_PR_EXPLICIT_ARGS
+    GZ_LAUNCH_CHECK();
+}
+
 /* Traditional push: every valid lane adds the source's outbound message to
  * its destination's accumulator with a hardware FP64 atomic (the asm's
  * lock cmpxchg loop), and S gains the message once per valid lane. */
@@ -289,6 +311,19 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr_sliced(
         }
         if (mine) props[v] = gz_pr_vertex_value(a, od, c, k, damping, nv);
     }
+}
+
+/* The vertex phase over explicit arrays (the allocation-free PageRank
+ * iteration), S read from sum_word on the device. */
+void gz_launch_pr_vertex_explicit(double* props, double* acc, const double* outdeg, uint64_t num_vertices,
+                                  const uint64_t* sum_word, double damping)
+{
+    if (num_vertices == 0) return;
+    uint64_t blocks = (num_vertices + GZ_BLOCK - 1) / GZ_BLOCK;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(gz_k_vertex_pr, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, props, acc, outdeg,
+                       (uint64_t)0, num_vertices, (double)num_vertices, sum_word, damping, 0u);
+    GZ_LAUNCH_CHECK();
 }
 
 extern "C" void perform_vertex_phase_pr(uint64_t vertex_start, uint64_t vertex_count, const uint64_t* reduce_buffer)

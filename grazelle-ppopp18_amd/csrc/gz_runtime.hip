@@ -41,6 +41,7 @@ uint32_t gz_cu_count = 1;
 hipStream_t gz_stream = nullptr;
 static int gz_device_id = -1;
 static char gz_device_name_buf[256];
+static void gz_noalloc_setup(int device);
 
 /* ---- runtime hooks -------------------------------------------------------- */
 
@@ -67,6 +68,7 @@ extern "C" int gz_runtime_init(int device)
     snprintf(gz_device_name_buf, sizeof(gz_device_name_buf), "%s (%s)", prop.name, prop.gcnArchName);
     gz_cu_count = prop.multiProcessorCount > 0 ? (uint32_t)prop.multiProcessorCount : 1u;
     gz_device_id = device;
+    gz_noalloc_setup(device);
     memset(&gz_scratch, 0, sizeof(gz_scratch));
     return 0;
 }
@@ -113,6 +115,8 @@ extern "C" void gz_device_free(void* ptr)
         GZ_CHECK(hipFree(ptr));
     }
 }
+
+extern "C" uint64_t gz_device_allocation_events(void) { return gz_alloc_generation; }
 
 /* ---- replay of a captured launch sequence ----------------------------------
  * One cached executable graph: `times` rounds of the launches enqueue() makes
@@ -512,23 +516,35 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_reduce(const T* __restrict__ in
     }
 }
 
-void gz_reduce_partials(uint64_t* dst)
+static uint64_t gz_reduce_blocks(uint64_t count)
 {
-    const uint64_t count = gz_scratch.partials_count;
     uint64_t blocks = (count + 4095) / 4096;
     if (blocks < 1) blocks = 1;
     if (blocks > 1024) blocks = 1024;
+    return blocks;
+}
+
+void gz_reduce_f64_into(const double* in, uint64_t count, double* slots, unsigned int* done, uint64_t* out)
+{
+    hipLaunchKernelGGL(gz_k_reduce<double>, dim3((unsigned)gz_reduce_blocks(count)), dim3(GZ_BLOCK), 0, gz_stream,
+                       in, count, slots, done, out);
+    GZ_LAUNCH_CHECK();
+}
+
+void gz_reduce_partials(uint64_t* dst)
+{
+    const uint64_t count = gz_scratch.partials_count;
+    const uint64_t blocks = gz_reduce_blocks(count);
     unsigned int* done = reinterpret_cast<unsigned int*>(gz_scratch.reduce_tmp + 2040);
     gz_ktimer_begin(GZ_K_REDUCE);
     if (gz_scratch.partials_is_double) {
-        hipLaunchKernelGGL(gz_k_reduce<double>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
-                           (const double*)gz_scratch.partials, count, (double*)gz_scratch.reduce_tmp, done, dst);
+        gz_reduce_f64_into((const double*)gz_scratch.partials, count, (double*)gz_scratch.reduce_tmp, done, dst);
     } else {
         hipLaunchKernelGGL(gz_k_reduce<unsigned long long>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
                            (const unsigned long long*)gz_scratch.partials, count, (unsigned long long*)gz_scratch.reduce_tmp,
                            done, dst);
+        GZ_LAUNCH_CHECK();
     }
-    GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_REDUCE);
 }
 
@@ -592,16 +608,89 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_merge_sum(const mergeaccum_t* _
     acc[r.final_vertex_id] = proposed;
 }
 
+void gz_launch_merge_sum(const mergeaccum_t* merge, uint64_t count, double* acc)
+{
+    if (count == 0) return;
+    const uint64_t blocks = (count + GZ_BLOCK - 1) / GZ_BLOCK;
+    hipLaunchKernelGGL(gz_k_merge_sum, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, merge, count, acc);
+    GZ_LAUNCH_CHECK();
+}
+
 extern "C" void edge_pull_op_merge_with_merge_buffer(mergeaccum_t* merge_buffer, uint64_t count, double* vertex_accumulators)
 {
     gz_require_runtime();
     if (count == 0) return;
-    const uint64_t blocks = (count + GZ_BLOCK - 1) / GZ_BLOCK;
     gz_ktimer_begin(GZ_K_MERGE);
-    hipLaunchKernelGGL(gz_k_merge_sum, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, merge_buffer, count,
-                       vertex_accumulators);
-    GZ_LAUNCH_CHECK();
+    gz_launch_merge_sum(merge_buffer, count, vertex_accumulators);
     gz_ktimer_end(GZ_K_MERGE);
+}
+
+/* ---- one PageRank iteration that allocates nothing ----------------------------
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Its scratch (one merge record and one
+ * partial sum per pull unit, the reduction's block slots, its done counter
+ * and the sum S) lives in this file-scope array; every call on a device
+ * shares that device's copy.  The host uses the address looked up per device
+ * at init, never the array's name. */
+#define GZ_NOALLOC_MAX_UNITS (1ull << 19)
+#define GZ_NOALLOC_MERGE_WORDS (4 * GZ_NOALLOC_MAX_UNITS)
+#define GZ_NOALLOC_PARTIALS_WORD GZ_NOALLOC_MERGE_WORDS
+#define GZ_NOALLOC_SLOTS_WORD (GZ_NOALLOC_PARTIALS_WORD + GZ_NOALLOC_MAX_UNITS)
+#define GZ_NOALLOC_DONE_WORD (GZ_NOALLOC_SLOTS_WORD + 1024)
+#define GZ_NOALLOC_SUM_WORD (GZ_NOALLOC_DONE_WORD + 1)
+#define GZ_NOALLOC_WORDS (GZ_NOALLOC_SUM_WORD + 63)
+#define GZ_MAX_DEVICES 64
+static __device__ __align__(256) uint64_t scratch[GZ_NOALLOC_WORDS];
+static uint64_t* gz_noalloc_scratch[GZ_MAX_DEVICES];
+
+/* With `device` current: its copy of the scratch array. */
+static void gz_noalloc_setup(int device)
+{
+    if (device < 0 || device >= GZ_MAX_DEVICES || gz_noalloc_scratch[device] != nullptr) return;
+    void* p = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (err != hipSuccess || p == nullptr) {
+        fprintf(stderr, "grazelle: no address for the allocation-free scratch on device %d: %s\n", device,
+                hipGetErrorString(err));
+        exit(70);
+    }
+    gz_noalloc_scratch[device] = (uint64_t*)p;
+}
+
+extern "C" int gz_pagerank_iteration_noalloc(const gz_vec_t* pull_vectors, uint64_t pull_vector_count,
+                                             uint64_t num_vertices, const double* outdegrees, double* props,
+                                             double* accumulators)
+{
+    gz_require_runtime();
+    const uint64_t units = gz_chunks_for(pull_vector_count);
+    if (units > GZ_NOALLOC_MAX_UNITS) {
+        fprintf(stderr,
+                "grazelle: gz_pagerank_iteration_noalloc: %llu vectors make %llu pull units at %llu vectors per "
+                "unit, above its limit of %llu units\n",
+                (unsigned long long)pull_vector_count, (unsigned long long)units,
+                (unsigned long long)gz_chunk_len(), (unsigned long long)GZ_NOALLOC_MAX_UNITS);
+        return -1;
+    }
+    if (num_vertices == 0) return 0;
+    uint64_t* base = (gz_device_id >= 0 && gz_device_id < GZ_MAX_DEVICES) ? gz_noalloc_scratch[gz_device_id] : nullptr;
+    if (base == nullptr) {
+        fprintf(stderr, "grazelle: gz_pagerank_iteration_noalloc: no scratch on device %d\n", gz_device_id);
+        return -1;
+    }
+    mergeaccum_t* merge = (mergeaccum_t*)base;
+    double* partials = (double*)(base + GZ_NOALLOC_PARTIALS_WORD);
+    double* slots = (double*)(base + GZ_NOALLOC_SLOTS_WORD);
+    unsigned int* done = (unsigned int*)(base + GZ_NOALLOC_DONE_WORD);
+    uint64_t* sum = base + GZ_NOALLOC_SUM_WORD;
+    /* the reduction's done counter must start at 0 (a fill kernel: no memset
+     * node if the caller captures this); S is 0 for an empty list */
+    gz_device_zero(done, 2 * sizeof(uint64_t));
+    gz_launch_pr_pull_explicit(pull_vectors, pull_vector_count, gz_chunk_len(), units, props, accumulators, merge,
+                               partials);
+    gz_launch_merge_sum(merge, units, accumulators);
+    if (units > 0) gz_reduce_f64_into(partials, units, slots, done, sum);
+    gz_launch_pr_vertex_explicit(props, accumulators, outdegrees, num_vertices, sum, gz_phase_config.damping);
+    return 0;
 }
 
 /* ---- statistics ----------------------------------------------------------- */

@@ -151,6 +151,33 @@ static inline int64_t gz_bfs_parent(uint64_t word)
     return (p == GZ_BFS_PARENT_NONE) ? -1 : (int64_t)p;
 }
 
+/* -------- one PageRank iteration that allocates nothing ------------------
+ * No counterpart in the reference.  For callers that cannot call hipMalloc:
+ * code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation.  It is one iteration of the PageRank driver loop
+ * (execution_pr.c:99-158) with the kernels of perform_edge_pull_phase_pr
+ * (scheduler-aware), edge_pull_op_merge_with_merge_buffer,
+ * phase_op_write_global_accum_to_buf and perform_vertex_phase_pr, over
+ * caller-owned device arrays, on the library stream (gz_runtime_stream):
+ *   pull_vectors[pull_vector_count]  dst-grouped Vector-Sparse list
+ *   outdegrees[V]                    FP64 out-degrees (0 for sinks)
+ *   props[V]                         rank / outdeg' in, the next iteration's out
+ *   accumulators[V]                  the pull's per-destination sums; 0.0 on the
+ *                                    first call for vertices without in-edges,
+ *                                    which the pull never writes (as after
+ *                                    gz_execution_prepare)
+ * with the damping and chunk length of gz_phase_config.  Results are
+ * bit-identical to those entry points' for the same list and settings.  Its
+ * scratch (merge records and partial sums, 40 bytes per pull unit, for at
+ * most 524,288 units, i.e. 512 x 2^20 vectors at 1024 vectors per unit; about
+ * 20 MiB in all) is one file-scope device array that each GPU the library
+ * runs on holds until the process ends.  Returns 0, or -1 with a message
+ * when the list needs more units (raise gz_phase_config.chunk_vectors) or the
+ * runtime has no scratch address.  Calls that overlap on one GPU race on the
+ * shared scratch; calls on one stream are ordered. */
+int gz_pagerank_iteration_noalloc(const gz_vec_t* pull_vectors, uint64_t pull_vector_count, uint64_t num_vertices,
+                                  const double* outdegrees, double* props, double* accumulators);
+
 /* -------- runtime hooks (replace threads.c / numanodes.c) ---------------- */
 
 int gz_runtime_init(int device);               /* 0 on success; dies if no device */
@@ -162,6 +189,7 @@ void gz_device_free(void* ptr);
 void gz_copy_to_device(void* dst, const void* src, size_t bytes);
 void gz_copy_to_host(void* dst, const void* src, size_t bytes);
 void gz_device_zero(void* ptr, size_t bytes);  /* fill kernel, safe under graph capture */
+uint64_t gz_device_allocation_events(void);    /* device allocations plus frees the library made so far */
 void gz_device_copy(void* dst, const void* src, size_t bytes); /* device-to-device, stream ordered */
 void gz_synchronize(void);
 void* gz_runtime_stream(void);                 /* hipStream_t every phase is launched on */
