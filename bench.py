@@ -26,15 +26,20 @@ Sub-objects, each one configuration of the workload set:
   cc_rmat22   CC, symmetrised R-MAT s22 + 64 planted 1000-vertex trees,
               hybrid with the fixed E/5 threshold, time to solution with the
               source-sliced pull on the renumbered graph (labels checked
-              against the plain pull's, which is reported beside it)
+              against the plain pull's, which is reported beside it); the
+              rate is E / time to solution (the GPU needs more sweeps than
+              the sequential reference, so E x iterations / t would flatter it)
   bfs_rmat24  BFS from 64 seeded roots on R-MAT s24: fixed threshold versus
-              the learned selector (selector_bfs.bin), depths checked equal
+              the learned selector (selector_bfs.bin), depths checked equal;
+              roofline from the algorithmic bytes per level the kernels count
   batch_1024  1024 R-MAT graphs (scale 10..17 with P(s) ~ 1/s, edge factor
               4..32): batched PageRank (10 iterations, captured HIP graph)
               and CC versus the same graphs run one after another
   pr_rmat16   PageRank 20 iterations on R-MAT s16 (the reference's own size)
 cpu_baseline: the CPU oracle (pthreads restatement of the reference engine)
-on a bounded sample of the headline workload.
+on the headline graph itself, its lists copied back from the device, on the
+job's CPU share (OMP_NUM_THREADS, else every core), median of 5 runs; the
+same oracle is timed beside cc_rmat22, bfs_rmat24 and pr_rmat16.
 """
 from __future__ import annotations
 
@@ -108,7 +113,15 @@ def headline(gz, args) -> dict:
     sliced_vectors = int(gz.source_slices_info()[0].sum()) if sliced else nvp
     gz.set_chunk_vectors(512)
     gz.slices_set_hot_cache(0)
+    indeg = gz.indegrees()
+    max_in, max_out = int(indeg.max()), int(gz.outdegrees().max())
+    del indeg
     full_other = None if args.only_headline else full_cc_bfs(gz)
+    cpu = None
+    if not args.no_cpu:
+        t0 = time.time()
+        cpu = cpu_baseline_full(gz, cpu_threads(args), args.cpu_runs)
+        log(f"[cpu_baseline] {time.time() - t0:.1f}s {cpu}")
     ms_step = r.elapsed_ms / args.steps
     pms, pn = kt["pull_pr"]
     avg = pms / pn
@@ -128,7 +141,9 @@ def headline(gz, args) -> dict:
                      "avg_launch_us": round(avg * 1e3, 2), "launches": pn,
                      "kernel_gteps": round(E / (avg * 1e-3) / 1e9, 2)},
         "rank_sum": float(ranks.sum()),
+        "max_in_degree": max_in, "max_out_degree": max_out,
         "full_cc_bfs": full_other,
+        "cpu_baseline": cpu,
     }
 
 
@@ -145,7 +160,7 @@ def full_cc_bfs(gz) -> dict:
         labels = gz.cc_labels()
         out["cc"] = {"time_to_solution_ms": round(r.elapsed_ms, 2), "iterations": r.iterations,
                      "pull_iterations": r.pull_iterations, "push_iterations": r.push_iterations,
-                     "gteps": round(E * r.iterations / (r.elapsed_ms * 1e-3) / 1e9, 2)}
+                     "edges_per_s_to_solution_G": round(E / (r.elapsed_ms * 1e-3) / 1e9, 2)}
         if gz.source_slices_info() is not None:
             # the pull sweeps over the source-sliced layout (labels must not change)
             gz.set_chunk_vectors(SLICED_CHUNK)
@@ -193,13 +208,26 @@ def _cc_timed(gz, variant: str):
     return statistics.median(times), res, kt["pull_cc"]
 
 
-def cc_rmat22(gz) -> dict:
+def cc_rmat22(gz, cpu: int) -> dict:
     # plain pull on the loaded ids, then the source-sliced pull on the graph
     # renumbered by out-degree; labels (original ids) must be identical
     gz.load_workload_device("cc-rmat22-sym-trees")
     V, E = gz.num_vertices(), gz.num_edges()
     ms_plain, res_plain, (pms, pn) = _cc_timed(gz, "sched_aware")
     labels = gz.cc_labels()
+    cpu_fig = None
+    if cpu:
+        orc = _oracle()
+        G = orc.Graph.from_vectors(V, gz.device_vectors(True), gz.device_vectors(False))
+        times = []
+        for _ in range(5):
+            lab, _, r = orc.cc(G, orc.options(threads=cpu))
+            times.append(r.elapsed_ms)
+        if not np.array_equal(lab, labels):
+            raise AssertionError("cc_rmat22: oracle labels differ from the GPU's")
+        del G
+        cpu_fig = {"time_to_solution_ms": round(statistics.median(times), 2), "threads": cpu,
+                   "iterations": r.iterations, "runs": 5}
     B = pull_bytes(gz)
     plain = {"time_to_solution_ms": round(ms_plain, 3), "iterations": res_plain.iterations,
              "pull_iterations": res_plain.pull_iterations, "push_iterations": res_plain.push_iterations}
@@ -225,17 +253,35 @@ def cc_rmat22(gz) -> dict:
            "pull": f"source-sliced (renumbered, S=2^{CC_SLICES_LOG2}, LDS cache {HOT_CACHE}, chunk {SLICED_CHUNK})",
            "time_to_solution_ms": round(ms, 3), "iterations": res.iterations,
            "pull_iterations": res.pull_iterations, "push_iterations": res.push_iterations,
-           "gteps": round(E * res.iterations / (ms * 1e-3) / 1e9, 2), "labels_equal_plain": same,
+           "edges_per_s_to_solution_G": round(E / (ms * 1e-3) / 1e9, 2), "labels_equal_plain": same,
            "plain_pull": plain}
     if sn:
         avg = sms / sn
         out["pull_cc_kernel"] = {"avg_launch_us": round(avg * 1e3, 2), "bytes_per_launch": B,
                                  "achieved_GBs": round(B / (avg * 1e-3) / 1e9, 1),
                                  "frac_hbm_peak": round(B / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if cpu_fig:
+        out["cpu_oracle"] = cpu_fig
     return out
 
 
-def bfs_rmat24(gz) -> dict:
+def bfs_bytes(gz, root: int) -> tuple[int, int]:
+    """(algorithmic bytes, levels) of one fixed-threshold search: per level
+    32 B per vector loaded, 8 B per valid lane examined (both counted by the
+    kernels), two passes over the V/8-byte frontier bitmap, 8 B per vertex
+    written."""
+    V = gz.num_vertices()
+    gz.run(gz.options("bfs", root=root, collect_stats=True))
+    recs = gz.records()
+    depths = gz.bfs_result()[1]
+    counts = np.bincount(depths[depths != np.iinfo(np.uint64).max].astype(np.int64), minlength=len(recs) + 1)
+    total = 0
+    for i, r in enumerate(recs):
+        total += 32 * r.vectors + 8 * r.edges + 2 * ((V + 7) // 8) + 8 * int(counts[i + 1])
+    return total, len(recs)
+
+
+def bfs_rmat24(gz, cpu: int) -> dict:
     import selector_train as st
 
     gz.load_workload_device("bfs-rmat24")
@@ -267,10 +313,30 @@ def bfs_rmat24(gz) -> dict:
                 "learned_speedup": round(totals["hybrid"] / totals["learned"], 4),
                 "fixed_ms_per_search": round(totals["hybrid"] / len(roots), 4),
                 "fixed_gteps_per_search": round(E / (totals["hybrid"] / len(roots) * 1e-3) / 1e9, 2)})
+    # algorithmic bytes of the fixed-threshold searches (a counting pass
+    # outside the timed runs) over their time
+    nbytes = levels = 0
+    for root in roots:
+        b, n = bfs_bytes(gz, root)
+        nbytes += b
+        levels += n
+    achieved = nbytes / (totals["hybrid"] * 1e-3) / 1e9
+    out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_search": nbytes // len(roots),
+                       "levels_per_search": round(levels / len(roots), 2)}
+    if cpu:
+        orc = _oracle()
+        G = orc.Graph.from_vectors(V, gz.device_vectors(True), gz.device_vectors(False))
+        t = []
+        for root in roots[:8]:
+            t.append(orc.bfs(G, orc.options(threads=cpu, root=root))[3].elapsed_ms)
+        del G
+        out["cpu_oracle"] = {"ms_per_search": round(statistics.median(t), 2), "threads": cpu,
+                             "roots": 8, "statistic": "median"}
     return out
 
 
-def batch_1024(gz) -> dict:
+def batch_1024(gz, cpu: int) -> dict:
     import batch_tune
 
     specs = gz.batch_specs()
@@ -293,35 +359,76 @@ def batch_1024(gz) -> dict:
             "cc": {"batched_ms": round(cc_b, 3), "sequential_ms": round(cc_s, 3), "speedup": round(cc_s / cc_b, 2)}}
 
 
-def pr_rmat16(gz) -> dict:
+def pr_rmat16(gz, cpu: int) -> dict:
     gz.load_workload_device("pr-rmat16")
     gz.run(gz.options("pr", iterations=20, damping=DAMPING))
     ms = min(gz.run(gz.options("pr", iterations=20, damping=DAMPING)).elapsed_ms for _ in range(5))
     E = gz.num_edges()
-    return {"workload": "pr-rmat16", "V": gz.num_vertices(), "E": E, "iterations": 20, "ms": round(ms, 4),
-            "gteps_per_iteration": round(E * 20 / (ms * 1e-3) / 1e9, 2)}
+    out = {"workload": "pr-rmat16", "V": gz.num_vertices(), "E": E, "iterations": 20, "ms": round(ms, 4),
+           "gteps_per_iteration": round(E * 20 / (ms * 1e-3) / 1e9, 2)}
+    if cpu:
+        orc = _oracle()
+        g = gz.workload_graph("pr-rmat16")
+        G = orc.Graph(g.num_vertices, g.pull, g.push)
+        for threads in sorted({1, cpu}):
+            t = [orc.pagerank(G, orc.options(threads=threads, iterations=20, damping=DAMPING))[2].elapsed_ms
+                 for _ in range(5)]
+            m = statistics.median(t)
+            out[f"cpu_oracle_{threads}t"] = {"ms": round(m, 3), "gteps_per_iteration": round(E * 20 / (m * 1e-3) / 1e9, 3),
+                                             "runs": 5, "statistic": "median"}
+    return out
 
 
-def cpu_baseline(gz, threads: int, runs: int) -> dict:
-    """The oracle's PageRank on a bounded sample of the headline workload:
-    the same generator at scale 20 (1/64 of the vertices and edges)."""
+def _oracle():
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc
 
-    scale = 20
-    nv = gz.FULL_VERTICES >> 6
-    raw = gz.rmat_edges(scale, gz.FULL_RAW_EDGES >> 6, 1, num_vertices=nv)
-    g = gz.make_graph(raw, nv)
-    G = orc.Graph(g.num_vertices, g.pull, g.push)
+    return orc
+
+
+def cpu_threads(args) -> int:
+    """Worker threads for the CPU oracle: the job's CPU share (the
+    OMP_NUM_THREADS the host sets for a one-GPU job), else every core."""
+    if args.cpu_threads:
+        return args.cpu_threads
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return int(env) if env.isdigit() and int(env) > 0 else (os.cpu_count() or 1)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_full(gz, threads: int, runs: int, iterations: int = 3) -> dict:
+    """The oracle's PageRank on the headline graph itself: its pull and push
+    lists copied back from the device, the pthreads restatement of the
+    reference engine (scheduler-aware pull, 32 units per thread), timed over
+    thread start to join as main.c:114-120 times threads_spawn."""
+    orc = _oracle()
+    V, E = gz.num_vertices(), gz.num_edges()
+    G = orc.Graph.from_vectors(V, gz.device_vectors(True), gz.device_vectors(False))
     times = []
     for _ in range(runs):
-        _, _, r = orc.pagerank(G, orc.options(threads=threads, iterations=5, damping=DAMPING))
-        times.append(r.elapsed_ms / 5)
+        _, _, r = orc.pagerank(G, orc.options(threads=threads, iterations=iterations, damping=DAMPING))
+        times.append(r.elapsed_ms / iterations)
+    del G
     ms = statistics.median(times)
-    return {"value": round(g.num_edges / (ms * 1e-3) / 1e9, 3), "unit": "GTEPS", "cores": threads, "kind": "port",
-            "sample": f"PageRank pull, 5 iterations, the headline generator at scale 20 folded into {nv} vertices "
-                      f"({g.num_edges} edges, 1/64 of the headline graph); median of {runs} runs of the pthreads "
-                      f"oracle, {ms:.1f} ms per iteration"}
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"value": round(E / (ms * 1e-3) / 1e9, 3), "unit": "GTEPS/iter", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "affinity_cpus": affinity, "cpu_model": cpu_model(),
+            "ms_per_iteration": round(ms, 2),
+            "sample": f"the headline graph itself ({V} vertices, {E} edges, the device's renumbered lists "
+                      f"copied back): PageRank pull, damping {DAMPING}, {iterations} iterations per run, median "
+                      f"of {runs} runs of the pthreads oracle on {threads} threads"}
 
 
 def main() -> int:
@@ -333,7 +440,7 @@ def main() -> int:
     ap.add_argument("--no-relabel", action="store_true")
     ap.add_argument("--plain-pull", action="store_true", help="headline over the ordinary pull list")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", choices=("cc_rmat22", "bfs_rmat24", "batch_1024", "pr_rmat16"), default=None,
                     help="only run this configuration and print its result (profiling)")
@@ -367,8 +474,9 @@ def main() -> int:
     dev = gz.init(0)
     configs = (("cc_rmat22", cc_rmat22), ("bfs_rmat24", bfs_rmat24), ("batch_1024", batch_1024),
                ("pr_rmat16", pr_rmat16))
+    cpu = 0 if args.no_cpu else cpu_threads(args)
     if args.config:
-        out = dict(configs)[args.config](gz)
+        out = dict(configs)[args.config](gz, cpu)
         print(json.dumps({args.config: out}), flush=True)
         gz.release()
         return 0
@@ -378,16 +486,10 @@ def main() -> int:
         for name, fn in configs:
             t0 = time.time()
             try:
-                extra[name] = fn(gz)
+                extra[name] = fn(gz, cpu)
             except Exception as exc:  # a failing configuration is reported, not hidden
                 extra[name] = {"error": f"{type(exc).__name__}: {exc}"}
             log(f"[{name}] {time.time() - t0:.1f}s {json.dumps(extra[name])}")
-    cpu = None
-    if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        t0 = time.time()
-        cpu = cpu_baseline(gz, threads, args.cpu_runs)
-        log(f"[cpu_baseline] {time.time() - t0:.1f}s {cpu}")
     line = {
         "metric": METRIC, "value": round(h["value"], 3), "unit": "GTEPS/iter", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(h["ms_step"], 4), "higher_is_better": True,
@@ -398,9 +500,10 @@ def main() -> int:
                    "pull": (f"source-sliced (S=2^{SLICES_LOG2}, {SLICE_TIERS} tiers, {SLICE_BLOCKS} LDS blocks, "
                             f"LDS cache {HOT_CACHE}, chunk {SLICED_CHUNK})"
                             if h["sliced"] else "scheduler-aware, ordinary pull list"),
+                   "max_in_degree": h["max_in_degree"], "max_out_degree": h["max_out_degree"],
                    "device": dev},
         "roofline": h["roofline"],
-        "cpu_baseline": cpu,
+        "cpu_baseline": h["cpu_baseline"],
         "phases_ms_per_iteration": h["phases_ms_per_iteration"],
         "full_scale_cc_bfs": h["full_cc_bfs"],
         **extra,

@@ -79,6 +79,7 @@ void gz_execution_prepare(const gz_run_options_t* o)
     case GZ_ALG_PR:
         gz_dev_init_pr((double*)gz_graph.vertex_props, (double*)gz_graph.vertex_accumulators,
                        gz_graph.vertex_outdegrees, V);
+        gz_pr_clear_fixed();
         gz_dev_fill_u64(gz_graph.frontier_has_info, ~0ull, fwords);
         gz_dev_fill_u64(gz_graph.frontier_wants_info, ~0ull, fwords);
         break;

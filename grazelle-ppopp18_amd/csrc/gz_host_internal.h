@@ -62,6 +62,10 @@ void gz_cc_push_vote_hint(uint64_t vote);
 void gz_bfs_push_vote_hint(uint64_t vote);
 /* Drop the frontier compaction's per-vertex vector counts (graph release). */
 void gz_frontier_counts_release(void);
+/* The PageRank push's fixed-point accumulators: freed with the graph, and
+ * cleared when a PageRank run starts. */
+void gz_pr_release(void);
+void gz_pr_clear_fixed(void);
 /* The source-sliced pull's settings: hot cache, cold spread, cold stream. */
 void gz_slices_settings(uint32_t* out3);
 

@@ -30,6 +30,8 @@ static uint32_t gz_pr_acc_needs_zero = 0;
 /* Set by the source-sliced pull: the vertex phase sums the slices'
  * accumulators instead of reading the accumulator array. */
 static uint32_t gz_pr_acc_in_slices = 0;
+/* Set by the push: the vertex phase reads its fixed-point accumulators. */
+static uint32_t gz_pr_acc_is_fixed = 0;
 
 template <bool kSchedAware, int VPL>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr(const gz_vec_t* __restrict__ vecs, uint64_t nvec,
@@ -102,10 +104,12 @@ extern "C" void perform_edge_pull_phase_pr(const gz_vec_t* edge_list, uint64_t e
     if (gz_phase_config.pull_variant == GZ_PULL_SOURCE_SLICED) {
         gz_pr_acc_needs_zero = 0;
         gz_pr_acc_in_slices = 1;
+        gz_pr_acc_is_fixed = 0;
         gz_pull_pr_sliced(edge_list, edge_list_count);
         return;
     }
     gz_pr_acc_in_slices = 0;
+    gz_pr_acc_is_fixed = 0;
     const bool sa = gz_phase_config.pull_variant == GZ_PULL_SCHED_AWARE;
     gz_pr_acc_needs_zero = sa ? 0u : 1u;
     if (sa && gz_merge_buffer_count < nchunks) {
@@ -146,13 +150,87 @@ void gz_launch_pr_pull_explicit(const gz_vec_t* vecs, uint64_t nvec, uint64_t ch
     GZ_LAUNCH_CHECK();
 }
 
+/* ---- the traditional push with an order-independent sum -----------------
+ * The reference's push (phase_edge_push_pr.asm:360-425) adds each edge's
+ * message to its destination with a CAS loop, so the sum of a destination
+ * with n in-edges is rounded in whatever order the threads reach it.  Here
+ * each message is added as an unsigned 128-bit fixed-point integer with
+ * GZ_PR_FIXED_BITS fraction bits, by two 64-bit integer atomics (the low word,
+ * then the high word plus the carry the low add produced).  Integer addition
+ * is associative, so every order gives the same bits: the exact sum of the
+ * FP64 messages (rank / outdeg' >= 0.15 / V^2, at least 2^-67 for V <= 2^32,
+ * so no message loses a bit; sums stay below 2^8, since the messages of all
+ * edges add up to at most 1).  The vertex phase rounds it to FP64 once and
+ * clears it. */
+#define GZ_PR_FIXED_BITS 120
+
+/* x >= 0 as a 128-bit fixed-point integer (lo, hi), truncating below 2^-120. */
+__device__ __forceinline__ void gz_pr_to_fixed(double x, uint64_t& lo, uint64_t& hi)
+{
+    const uint64_t bits = (uint64_t)__double_as_longlong(x);
+    const int e = (int)((bits >> 52) & 0x7ff);
+    lo = hi = 0;
+    if (e == 0) return; /* zero (and denormals, far below 2^-120) */
+    const uint64_t m = (bits & 0xfffffffffffffull) | (1ull << 52);
+    const int sh = e - 1075 + GZ_PR_FIXED_BITS; /* value = m * 2^(e - 1075) */
+    if (sh <= -64) return;
+    if (sh < 0) {
+        lo = m >> (-sh);
+    } else if (sh < 64) {
+        lo = m << sh;
+        hi = sh ? (m >> (64 - sh)) : 0;
+    } else if (sh < 128) {
+        hi = m << (sh - 64);
+    }
+}
+
+__device__ __forceinline__ double gz_pr_from_fixed(uint64_t lo, uint64_t hi)
+{
+    return ldexp((double)hi, 64 - GZ_PR_FIXED_BITS) + ldexp((double)lo, -GZ_PR_FIXED_BITS);
+}
+
+__device__ __forceinline__ void gz_pr_fixed_add(uint64_t* __restrict__ a, uint64_t lo, uint64_t hi)
+{
+    const unsigned long long old = atomicAdd((unsigned long long*)a, (unsigned long long)lo);
+    const uint64_t carry = ((uint64_t)old + lo < (uint64_t)old) ? 1u : 0u;
+    if (hi + carry) atomicAdd((unsigned long long*)(a + 1), (unsigned long long)(hi + carry));
+}
+
+/* Accumulators of the push: two words per vertex. */
+static uint64_t* gz_pr_fixed_acc = nullptr;
+static uint64_t gz_pr_fixed_vertices = 0;
+
+extern "C" void gz_pr_release(void)
+{
+    gz_device_free(gz_pr_fixed_acc);
+    gz_pr_fixed_acc = nullptr;
+    gz_pr_fixed_vertices = 0;
+    gz_pr_acc_is_fixed = 0;
+}
+
+/* A zeroed fixed-point accumulator pair per vertex of the loaded graph. */
+static uint64_t* gz_pr_fixed_reserve(uint64_t num_vertices)
+{
+    if (gz_pr_fixed_acc == nullptr || gz_pr_fixed_vertices != num_vertices) {
+        gz_device_free(gz_pr_fixed_acc);
+        gz_pr_fixed_acc = (uint64_t*)gz_device_malloc((num_vertices + 8) * 2 * sizeof(uint64_t));
+        gz_pr_fixed_vertices = num_vertices;
+    }
+    return gz_pr_fixed_acc;
+}
+
+extern "C" void gz_pr_clear_fixed(void)
+{
+    if (gz_pr_fixed_acc != nullptr) gz_device_zero(gz_pr_fixed_acc, (gz_pr_fixed_vertices + 8) * 2 * sizeof(uint64_t));
+}
+
 /* Traditional push: every valid lane adds the source's outbound message to
- * its destination's accumulator with a hardware FP64 atomic (the asm's
- * lock cmpxchg loop), and S gains the message once per valid lane. */
+ * its destination's accumulator with integer atomics (above), and S gains
+ * the message once per valid lane. */
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_pr(const gz_vec_t* __restrict__ vecs, uint64_t nvec,
                                                               uint64_t chunk_len, uint64_t nchunks,
                                                               const double* __restrict__ props,
-                                                              double* __restrict__ acc, double* __restrict__ partials,
+                                                              uint64_t* __restrict__ acc, double* __restrict__ partials,
                                                               uint64_t* __restrict__ stats, uint32_t collect)
 {
     const int lane = gz_lane();
@@ -165,11 +243,13 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_pr(const gz_vec_t* __
     for (uint64_t i = b + (uint64_t)lane; i < e; i += GZ_WAVE) {
         const gz_lanes v = gz_load_vector(vecs, i);
         const double msg = props[gz_shared_vertex(v)];
+        uint64_t mlo, mhi;
+        gz_pr_to_fixed(msg, mlo, mhi);
         const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
             if (gz_lane_valid(l[k])) {
-                unsafeAtomicAdd(&acc[gz_lane_id(l[k])], msg);
+                gz_pr_fixed_add(acc + 2 * gz_lane_id(l[k]), mlo, mhi);
                 s_lane += msg;
                 edges += 1;
             }
@@ -190,6 +270,8 @@ extern "C" void perform_edge_push_phase_pr(const gz_vec_t* edge_list, uint64_t e
 {
     gz_pr_acc_needs_zero = 1;
     gz_pr_acc_in_slices = 0;
+    gz_pr_acc_is_fixed = 1;
+    uint64_t* fixed = gz_pr_fixed_reserve(gz_graph.num_vertices);
     const uint64_t nchunks = gz_chunks_for(edge_list_count);
     gz_scratch_reserve(nchunks, gz_graph.num_vertices);
     gz_scratch.partials_is_double = 1;
@@ -200,9 +282,8 @@ extern "C" void perform_edge_push_phase_pr(const gz_vec_t* edge_list, uint64_t e
     const uint64_t blocks = (nchunks + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
     gz_ktimer_begin(GZ_K_PUSH_PR);
     hipLaunchKernelGGL(gz_k_edge_push_pr, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
-                       edge_list_count, chunk_len, nchunks, (const double*)gz_graph.vertex_props,
-                       (double*)gz_graph.vertex_accumulators, (double*)gz_scratch.partials, gz_scratch.stats,
-                       gz_phase_config.collect_stats);
+                       edge_list_count, chunk_len, nchunks, (const double*)gz_graph.vertex_props, fixed,
+                       (double*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_PUSH_PR);
 }
@@ -313,6 +394,26 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr_sliced(
     }
 }
 
+/* The vertex phase after the push: the fixed-point sums rounded once to
+ * FP64, then cleared for the next push. */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr_fixed(double* __restrict__ props,
+                                                                 uint64_t* __restrict__ fixed,
+                                                                 const double* __restrict__ outdeg, uint64_t start,
+                                                                 uint64_t end, double num_vertices,
+                                                                 const uint64_t* __restrict__ reduce_buffer,
+                                                                 double damping)
+{
+    const double S = __longlong_as_double((long long)reduce_buffer[0]);
+    const double c = (1.0 - S) / num_vertices;
+    const double k = (1.0 - damping) / num_vertices;
+    for (uint64_t v = start + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < end;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        const gz_u64x2 w = *reinterpret_cast<const gz_u64x2*>(fixed + 2 * v);
+        *reinterpret_cast<gz_u64x2*>(fixed + 2 * v) = gz_u64x2{0, 0};
+        props[v] = gz_pr_vertex_value(gz_pr_from_fixed(w.x, w.y), outdeg[v], c, k, damping, num_vertices);
+    }
+}
+
 /* The vertex phase over explicit arrays (the allocation-free PageRank
  * iteration), S read from sum_word on the device. */
 void gz_launch_pr_vertex_explicit(double* props, double* acc, const double* outdeg, uint64_t num_vertices,
@@ -365,6 +466,10 @@ extern "C" void perform_vertex_phase_pr(uint64_t vertex_start, uint64_t vertex_c
         }
 #undef GZ_VERTEX_SLICED_CASE
 #undef GZ_VERTEX_SLICED_ARGS
+    } else if (gz_pr_acc_is_fixed && gz_pr_fixed_acc != nullptr && gz_pr_fixed_vertices == gz_graph.num_vertices) {
+        hipLaunchKernelGGL(gz_k_vertex_pr_fixed, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
+                           (double*)gz_graph.vertex_props, gz_pr_fixed_acc, gz_graph.vertex_outdegrees, vertex_start,
+                           end, (double)gz_graph.num_vertices, reduce_buffer, gz_phase_config.damping);
     } else {
         hipLaunchKernelGGL(gz_k_vertex_pr, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
                            (double*)gz_graph.vertex_props, (double*)gz_graph.vertex_accumulators,

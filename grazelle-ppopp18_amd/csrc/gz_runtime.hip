@@ -65,7 +65,11 @@ extern "C" int gz_runtime_init(int device)
     GZ_CHECK(hipStreamCreateWithFlags(&gz_stream, hipStreamNonBlocking));
     hipDeviceProp_t prop;
     GZ_CHECK(hipGetDeviceProperties(&prop, device));
-    snprintf(gz_device_name_buf, sizeof(gz_device_name_buf), "%s (%s)", prop.name, prop.gcnArchName);
+    /* the marketing name can be empty: the architecture, CU count and
+     * memory size then name the device */
+    snprintf(gz_device_name_buf, sizeof(gz_device_name_buf), "%s%s%s, %d CUs, %.0f GiB", prop.name,
+             prop.name[0] ? " " : "", prop.gcnArchName, prop.multiProcessorCount,
+             (double)prop.totalGlobalMem / (1024.0 * 1024.0 * 1024.0));
     gz_cu_count = prop.multiProcessorCount > 0 ? (uint32_t)prop.multiProcessorCount : 1u;
     gz_device_id = device;
     gz_noalloc_setup(device);

@@ -56,20 +56,42 @@ def test_pr_deterministic(gz, device, rmat14):
 
 @pytest.mark.parametrize("variant", ["traditional", "push"])
 def test_pr_variants(gz, orc, device, rmat14, variant):
+    # the reference's atomic comparison variants.  The traditional pull adds
+    # chunk-boundary runs with FP64 atomics (scheduling order moves a sum by
+    # rounding only): within 1e-12 of the oracle's traditional pull.
     g = rmat14
+    G = _orc_graph(orc, g)
     gz.load(g)
-    if variant == "push":
-        gz.run(gz.options("pr", iterations=10, engine="push"))
-        _, ref, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=10, engine="push"))
-    else:
+    if variant == "traditional":
         gz.run(gz.options("pr", iterations=10, pull_variant="traditional"))
-        _, ref, _ = orc.pagerank(_orc_graph(orc, g), orc.options(iterations=10, traditional=True))
-    # These are the reference's atomic comparison variants: FP64 atomic adds
-    # land in scheduling order, so a destination with in-degree n carries up
-    # to ~n*eps of order-dependent rounding per iteration (n is several
-    # thousand for this graph's hubs).  The scheduler-aware pull, the main
-    # path, is held to 1e-12 above.
-    assert _rel(gz.pr_ranks(), ref) <= 1e-11
+        _, ref, _ = orc.pagerank(G, orc.options(iterations=10, traditional=True))
+        err = _rel(gz.pr_ranks(), ref)
+        print(f"PageRank traditional pull: max relative error {err:.3e} against the oracle")
+        assert err <= PR_TOL
+        return
+    # The push adds each edge's message as a 128-bit fixed-point integer, so
+    # its accumulators are exact and the same in every order, and S is a
+    # tree sum.  The reference's push (phase_edge_push_pr.asm:360-425) adds
+    # every message to S in one sequential chain per thread, which moves S by
+    # rounding over E terms and, through (1 - S) / V, every rank: on this
+    # graph the oracle's own single-thread push differs from its pull by
+    # about 1.9e-12, worst at vertices without in-edges.  So the HIP push is
+    # held to 1e-12 against the oracle's pull (the same iteration with less
+    # rounding), and to the oracle's own push-pull gap plus that bar against
+    # the oracle's push.
+    gz.run(gz.options("pr", iterations=10, engine="push"))
+    got = gz.pr_ranks()
+    _, pull, _ = orc.pagerank(G, orc.options(iterations=10))
+    _, push, _ = orc.pagerank(G, orc.options(iterations=10, engine="push"))
+    gap = _rel(push, pull)
+    e_pull, e_push = _rel(got, pull), _rel(got, push)
+    print(f"PageRank push: max relative error {e_pull:.3e} against the oracle's pull, {e_push:.3e} against its "
+          f"push (the oracle's own push-pull gap {gap:.3e})")
+    assert e_pull <= PR_TOL
+    assert e_push <= gap + PR_TOL
+    first = gz.pr_props().view(np.uint64).copy()
+    gz.run(gz.options("pr", iterations=10, engine="push"))
+    assert np.array_equal(gz.pr_props().view(np.uint64), first)
 
 
 def test_pr_damping_085(gz, orc, device):
