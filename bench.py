@@ -31,7 +31,9 @@ Sub-objects, each one configuration of the workload set:
               the sequential reference, so E x iterations / t would flatter it)
   bfs_rmat24  BFS from 64 seeded roots on R-MAT s24: fixed threshold versus
               the learned selector (selector_bfs.bin), depths checked equal;
-              roofline from the algorithmic bytes per level the kernels count
+              the same 64 roots as one batch (gz_bfs_batch_run, one pass per
+              level for all searches); roofline from the algorithmic bytes per
+              level the kernels count
   batch_1024  1024 R-MAT graphs (scale 10..17 with P(s) ~ 1/s, edge factor
               4..32): batched PageRank (10 iterations, captured HIP graph)
               and CC versus the same graphs run one after another
@@ -313,6 +315,18 @@ def bfs_rmat24(gz, cpu: int) -> dict:
                 "learned_speedup": round(totals["hybrid"] / totals["learned"], 4),
                 "fixed_ms_per_search": round(totals["hybrid"] / len(roots), 4),
                 "fixed_gteps_per_search": round(E / (totals["hybrid"] / len(roots) * 1e-3) / 1e9, 2)})
+    # the same 64 roots in one batch: one pass per level for all searches
+    gz.bfs_batch(roots)
+    batch = [gz.bfs_batch(roots) for _ in range(3)]
+    bms = min(r.elapsed_ms for r in batch)
+    for i, root in enumerate(roots[:4]):
+        gz.run(gz.options("bfs", root=root))
+        if not np.array_equal(gz.bfs_batch_result(i, parents=False)[1], gz.bfs_result()[1]):
+            raise AssertionError(f"batched BFS depths differ from the single search at root {root}")
+    out["batched_64_roots"] = {"ms": round(bms, 3), "levels": int(batch[0].iterations),
+                               "pull_levels": int(batch[0].pull_iterations),
+                               "speedup_vs_fixed_sequential": round(totals["hybrid"] / bms, 2),
+                               "gteps_per_search": round(E * len(roots) / (bms * 1e-3) / 1e9, 2)}
     # algorithmic bytes of the fixed-threshold searches (a counting pass
     # outside the timed runs) over their time
     nbytes = levels = 0

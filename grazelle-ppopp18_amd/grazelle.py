@@ -601,6 +601,32 @@ def main(argv: list[str]) -> int:
     return int(lib.gz_main(len(argv), args))
 
 
+# ---- breadth-first searches from many roots at once --------------------------
+
+BFS_BATCH_MAX_ROOTS = 64
+_sig("gz_bfs_batch_run", C.c_int, vp, u32, P(RunResult))
+_sig("gz_bfs_batch_result", None, u32, vp, vp)
+_sig("gz_bfs_batch_release", None)
+
+
+def bfs_batch(roots) -> RunResult:
+    """BFS from up to 64 roots (original ids) in one pass per level."""
+    r = np.ascontiguousarray(roots, dtype=np.uint64)
+    res = RunResult()
+    if lib.gz_bfs_batch_run(_ptr(r), r.shape[0], C.byref(res)) != 0:
+        raise ValueError(f"BFS batch of {r.shape[0]} roots failed (see stderr)")
+    return res
+
+
+def bfs_batch_result(search: int, parents: bool = True):
+    """(parents or None, depths) of search `search` of the last batch."""
+    n = num_vertices()
+    depths = np.empty(n, dtype=np.uint64)
+    par = np.empty(n, dtype=np.int64) if parents else None
+    lib.gz_bfs_batch_result(search, _ptr(par) if parents else None, _ptr(depths))
+    return par, depths
+
+
 # ---- learned selector -------------------------------------------------------
 
 def selector_forward(params: np.ndarray, X: np.ndarray) -> np.ndarray:

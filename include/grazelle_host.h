@@ -300,6 +300,27 @@ void gz_result_bfs(int64_t* parents, uint64_t* depths);
 /* The -o text output (graphdata.c:1119-1138). */
 int graph_data_write_ranks_to_file(const char* filename, uint32_t algorithm);
 
+/* -------- breadth-first searches from many roots at once ------------------
+ * No reference counterpart (the reference searches from one root,
+ * execution_bfs.c:28,135).  Up to 64 searches share one pass over the graph
+ * per level: every vertex keeps a 64-bit word per state (seen, frontier,
+ * next) with bit i for search i, the level pushes or pulls by the reference's
+ * E/5 rule on the union frontier's statistic (1 + outdeg per vertex), and the
+ * depths are the BFS distances of each root, as a single search from it
+ * gives them.  Needs both layouts; depths are kept as bytes, so a batch
+ * fails past 254 levels.  gz_bfs_batch_run returns 0, or -1 with a message
+ * (bad count or root, missing layout, too many levels); r->iterations is the
+ * level count, r->elapsed_ms the wall time of the level loop. */
+#define GZ_BFS_BATCH_MAX_ROOTS 64
+int gz_bfs_batch_run(const uint64_t* roots, uint32_t count, gz_run_result_t* r);
+/* Depths (UINT64_MAX unreached) and, when `parents` is not NULL, parents
+ * (-1 for the root and unreached vertices) of search i of the last batch, in
+ * original ids; the parent is the smallest in-neighbour one level up (a
+ * valid BFS tree, derived from the depths on request). */
+void gz_bfs_batch_result(uint32_t search, int64_t* parents, uint64_t* depths);
+/* Frees the batch's device buffers (also done by gz_graph_release). */
+void gz_bfs_batch_release(void);
+
 /* Command line (main.c + cmdline.c); returns the process exit code. */
 int gz_main(int argc, char** argv);
 
