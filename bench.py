@@ -360,7 +360,19 @@ def batch_1024(gz, cpu: int) -> dict:
     gz.batch_pr(10, damping=DAMPING, use_graph=True)
     pr_b = min(gz.batch_pr(10, damping=DAMPING, use_graph=True) for _ in range(3))
     gz.run(gz.options("cc"))
-    cc_b = min(gz.run(gz.options("cc")).elapsed_ms for _ in range(2))
+    cc_runs = [gz.run(gz.options("cc")) for _ in range(2)]
+    cc_b = min(r.elapsed_ms for r in cc_runs)
+    # the same batch with the engine chosen on the device and the iterations
+    # replayed from a captured graph (eager until its allocations settle,
+    # captured the second time in a row, then replayed)
+    gz.cc_graph_replay(True)
+    gz.run(gz.options("cc"))
+    gz.run(gz.options("cc"))
+    c0, l0 = gz.graph_replay_counts()
+    cap_runs = [gz.run(gz.options("cc")) for _ in range(3)]
+    c1, l1 = gz.graph_replay_counts()
+    gz.cc_graph_replay(False)
+    cc_cap = min(r.elapsed_ms for r in cap_runs)
     pr_s = cc_s = 0.0
     for s, ef, seed in specs:
         gz.load_rmat_device(s, (1 << s) * ef, seed)
@@ -370,7 +382,11 @@ def batch_1024(gz, cpu: int) -> dict:
             "pagerank_10it": {"batched_ms": round(pr_b, 3), "sequential_ms": round(pr_s, 3),
                               "speedup": round(pr_s / pr_b, 2),
                               "batched_gteps_per_iteration": round(E * 10 / (pr_b * 1e-3) / 1e9, 2)},
-            "cc": {"batched_ms": round(cc_b, 3), "sequential_ms": round(cc_s, 3), "speedup": round(cc_s / cc_b, 2)}}
+            "cc": {"batched_ms": round(cc_b, 3), "sequential_ms": round(cc_s, 3), "speedup": round(cc_s / cc_b, 2),
+                   "iterations": int(cc_runs[-1].iterations),
+                   "captured_device_loop": {"batched_ms": round(cc_cap, 3), "speedup": round(cc_s / cc_cap, 2),
+                                            "iterations": int(cap_runs[-1].iterations),
+                                            "graph_captures": int(c1 - c0), "graph_launches": int(l1 - l0)}}}
 
 
 def pr_rmat16(gz, cpu: int) -> dict:

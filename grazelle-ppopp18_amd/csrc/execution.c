@@ -357,6 +357,98 @@ static void gz_run_pr(const gz_run_options_t* o, gz_run_result_t* r)
     if (t) gz_timer_destroy(t);
 }
 
+/* ---- CC with the engine chosen on the device --------------------------------
+ * A CC run that records nothing per iteration and uses the fixed threshold
+ * (or a forced engine) over the plain pull and the compacted push keeps its
+ * vote and engine choice in device control words (gz_cc_loop_enqueue), so
+ * its iterations can run as a captured HIP graph: GZ_CC_LOOP_ROUND
+ * iterations per graph, relaunched until the engine word says stop.  As with
+ * PageRank, a configuration is captured the second time in a row it runs
+ * (gz_graph_replay_cached); a first request runs the host loop below.  Off
+ * by default: every captured iteration also dispatches the kernels of the
+ * engines it does not run, which cost more than the one host round trip per
+ * iteration they save on the configurations measured (CC on R-MAT s22 with
+ * trees, the 1024-graph batch). */
+#define GZ_CC_LOOP_ROUND 4
+#define GZ_CC_LOOP_LOG 4096
+static int gz_cc_replay_on = 0;
+static uint64_t* gz_cc_ctrl = NULL;
+
+void gz_cc_graph_replay(int on) { gz_cc_replay_on = on ? 1 : 0; }
+
+static void gz_cc_loop_round(void* p)
+{
+    gz_cc_loop_enqueue((uint64_t*)p);
+    gz_swap_frontiers();
+}
+
+static int gz_cc_device_loop(const gz_run_options_t* o, uint64_t vote, uint64_t threshold, gz_run_result_t* r)
+{
+    const uint64_t V = gz_graph.num_vertices;
+    if (!gz_cc_replay_on || o->algorithm != GZ_ALG_CC || o->profile || o->collect_stats || gz_kernel_timing_enabled() ||
+        (o->engine != GZ_ENGINE_HYBRID && o->engine != GZ_ENGINE_PULL && o->engine != GZ_ENGINE_PUSH) ||
+        o->pull_variant != GZ_PULL_SCHED_AWARE || !gz_phase_config.push_frontier_skip ||
+        gz_graph.pull_vectors == NULL || gz_graph.push_vectors == NULL || gz_graph.push_vertex_offsets == NULL ||
+        V == 0)
+        return 0;
+    uint64_t pull_waves, push_waves;
+    const uint64_t waves = gz_cc_loop_waves(&pull_waves, &push_waves);
+    /* every buffer the launches use exists before any capture */
+    gz_scratch_reserve_public(waves, V);
+    if (gz_cc_ctrl == NULL) gz_cc_ctrl = (uint64_t*)gz_device_malloc(GZ_LOOP_WORDS(GZ_CC_LOOP_LOG) * sizeof(uint64_t));
+    uint64_t ctrl[16];
+    memset(ctrl, 0, sizeof(ctrl));
+    const int first_pull = o->engine == GZ_ENGINE_PULL ? 1 : o->engine == GZ_ENGINE_PUSH ? 0 : vote > threshold;
+    /* out-degrees in the vote bound a frontier's vectors (gz_push_sparse_eligible) */
+    const uint64_t sparse_ok = o->stat_mode != GZ_STAT_COUNT_ONLY;
+    ctrl[GZ_LOOP_ENGINE] = gz_loop_engine(vote, first_pull, sparse_ok);
+    ctrl[GZ_LOOP_VOTE] = vote;
+    ctrl[GZ_LOOP_THRESHOLD] = threshold;
+    ctrl[GZ_LOOP_MODE] = o->engine;
+    ctrl[GZ_LOOP_MAX_ITERATIONS] = o->max_iterations;
+    ctrl[GZ_LOOP_LOG_CAP] = GZ_CC_LOOP_LOG;
+    ctrl[GZ_LOOP_SPARSE_OK] = sparse_ok;
+    gz_copy_to_device(gz_cc_ctrl, ctrl, sizeof(ctrl));
+    uint64_t key[24];
+    size_t n = 0;
+    key[n++] = 0x43434c4f4f50ull; /* distinct from the PageRank keys */
+    key[n++] = V;
+    key[n++] = (uint64_t)(uintptr_t)gz_graph.pull_vectors;
+    key[n++] = gz_graph.pull_vector_count;
+    key[n++] = (uint64_t)(uintptr_t)gz_graph.push_vectors;
+    key[n++] = gz_graph.push_vector_count;
+    key[n++] = (uint64_t)(uintptr_t)gz_graph.push_vertex_offsets;
+    key[n++] = (uint64_t)(uintptr_t)gz_graph.vertex_props;
+    key[n++] = (uint64_t)(uintptr_t)gz_graph.vertex_accumulators;
+    key[n++] = (uint64_t)(uintptr_t)gz_graph.frontier_has_info;
+    key[n++] = (uint64_t)(uintptr_t)gz_graph.vertex_outdegrees;
+    key[n++] = (uint64_t)(uintptr_t)gz_cc_ctrl;
+    key[n++] = gz_phase_config.chunk_vectors;
+    key[n++] = gz_phase_config.vectors_per_lane;
+    key[n++] = gz_phase_config.stat_mode;
+    uint64_t launches = 0;
+    while (1) {
+        if (!gz_graph_replay_cached_slot(1, key, n, gz_cc_loop_round, gz_cc_ctrl, GZ_CC_LOOP_ROUND)) {
+            if (launches == 0) return 0; /* a first request: the host loop runs */
+            fprintf(stderr, "grazelle: the captured CC loop was dropped mid-run\n");
+            exit(72);
+        }
+        launches += 1;
+        gz_copy_to_host(ctrl, gz_cc_ctrl, sizeof(ctrl));
+        if (ctrl[GZ_LOOP_ENGINE] == 0) break;
+        /* labels only decrease, so a CC run ends within V + 1 iterations */
+        if (ctrl[GZ_LOOP_ITERATIONS] > V + 1) {
+            fprintf(stderr, "grazelle: the CC loop ran %llu iterations on %llu vertices\n",
+                    (unsigned long long)ctrl[GZ_LOOP_ITERATIONS], (unsigned long long)V);
+            exit(72);
+        }
+    }
+    r->iterations = ctrl[GZ_LOOP_ITERATIONS];
+    r->pull_iterations = ctrl[GZ_LOOP_PULLS];
+    r->push_iterations = ctrl[GZ_LOOP_PUSHES];
+    return 1;
+}
+
 /* Shared loop of execution_cc.c:94-268 and execution_bfs.c:120-276. */
 static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
 {
@@ -372,6 +464,7 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
         if (o->stat_mode != GZ_STAT_OUTDEG_ONLY) vote += V;
     }
     const uint64_t threshold = (o->stat_mode == GZ_STAT_COUNT_ONLY) ? V / 2 : E / 5;
+    if (!bfs && gz_cc_device_loop(o, vote, threshold, r)) return;
     const int learned = (o->engine == GZ_ENGINE_LEARNED) && gz_selector_loaded();
     const int probe = (o->engine == GZ_ENGINE_PROBE) && gz_graph.pull_vectors != NULL && gz_graph.push_vectors != NULL;
     const int features = learned || probe;

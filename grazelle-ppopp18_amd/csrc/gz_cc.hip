@@ -23,6 +23,8 @@
  * HasInfo words directly instead (gz_sparse_push.h), with the same per-edge
  * semantics.
  */
+#include "grazelle_host.h"
+#include "gz_host_internal.h"
 #include "gz_slots.h"
 #include "gz_sparse_push.h"
 #include "gz_walk.h"
@@ -138,8 +140,10 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_cc(const gz_vec_t* __
                                                               uint64_t* __restrict__ next_frontier,
                                                               const double* __restrict__ outdeg, uint32_t stat_mode,
                                                               uint64_t* __restrict__ partials,
-                                                              uint64_t* __restrict__ stats, uint32_t collect)
+                                                              uint64_t* __restrict__ stats, uint32_t collect,
+                                                              gz_guard guard)
 {
+    if (gz_guard_off(guard)) return;
     const int lane = gz_lane();
     const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
     const uint64_t W = (uint64_t)gridDim.x * GZ_WAVES_PER_BLOCK;
@@ -252,13 +256,13 @@ static void gz_cc_launch_push(const gz_vec_t* edge_list, uint64_t edge_list_coun
                            chunk_len, gz_graph.frontier_has_info, (uint64_t*)gz_graph.vertex_props,
                            (uint64_t*)gz_graph.vertex_accumulators, gz_graph.vertex_outdegrees,
                            gz_phase_config.stat_mode, (uint64_t*)gz_scratch.partials, gz_scratch.stats,
-                           gz_phase_config.collect_stats);
+                           gz_phase_config.collect_stats, gz_guard{nullptr, 0});
     else
         hipLaunchKernelGGL(gz_k_edge_push_cc<false>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
                            src, chunk_len, gz_graph.frontier_has_info, (uint64_t*)gz_graph.vertex_props,
                            (uint64_t*)gz_graph.vertex_accumulators, gz_graph.vertex_outdegrees,
                            gz_phase_config.stat_mode, (uint64_t*)gz_scratch.partials, gz_scratch.stats,
-                           gz_phase_config.collect_stats);
+                           gz_phase_config.collect_stats, gz_guard{nullptr, 0});
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_PUSH_CC);
 }
@@ -271,8 +275,10 @@ extern "C" void perform_edge_push_phase_cc(const gz_vec_t* edge_list, uint64_t e
 
 /* Clear the HasInfo* words covering [vertex_start, vertex_start+count)
  * (phase_vertex_cc.asm:64-170 zeroes 512 vertices per step). */
-__global__ __launch_bounds__(GZ_BLOCK) void gz_k_clear_words(uint64_t* __restrict__ words, uint64_t first, uint64_t end)
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_clear_words(uint64_t* __restrict__ words, uint64_t first, uint64_t end,
+                                                             gz_guard guard)
 {
+    if (gz_guard_off(guard)) return;
     for (uint64_t w = first + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < end; w += (uint64_t)gridDim.x * blockDim.x)
         words[w] = 0;
 }
@@ -284,7 +290,8 @@ extern "C" void gz_clear_bitmap_range(uint64_t* bitmap, uint64_t vertex_start, u
     const uint64_t end = (vertex_start + vertex_count + 63) >> 6;
     uint64_t blocks = (end - first + GZ_BLOCK - 1) / GZ_BLOCK;
     if (blocks > 4096) blocks = 4096;
-    hipLaunchKernelGGL(gz_k_clear_words, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, first, end);
+    hipLaunchKernelGGL(gz_k_clear_words, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, first, end,
+                       gz_guard{nullptr, 0});
     GZ_LAUNCH_CHECK();
 }
 
@@ -292,4 +299,175 @@ extern "C" void perform_vertex_phase_cc(uint64_t vertex_start, uint64_t vertex_c
 {
     (void)reduce_buffer;
     gz_clear_bitmap_range((uint64_t*)gz_graph.vertex_accumulators, vertex_start, vertex_count);
+}
+
+/* ---- the CC iteration loop with the engine chosen on the device -------------
+ * (execution_cc.c:124-142 decides on the host after reading the vote.)  Each
+ * iteration's launches read the loop's control words instead: the pull,
+ * the frontier compaction and the compacted push each run only when the
+ * engine word names them, a one-block control kernel sums the vote partials
+ * of the engine that ran, logs (engine, vote) and picks the next engine by
+ * the same rule, 0 when the vote is 0 or the iteration cap is reached, and
+ * the clear of the frontier that becomes the next HasInfo* runs while the
+ * loop goes on.  No launch depends on a host value that changes between
+ * iterations, so a run of iterations can be captured into a HIP graph (in
+ * pairs, the frontier bitmaps changing places as execution_cc.c:245-250
+ * swaps them) and replayed until the engine word is 0. */
+
+/* The pull over a persistent grid whose waves take chunks from the loop's
+ * counter, one vote partial per wave. */
+template <int VPL>
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_cc_loop(const gz_vec_t* __restrict__ vecs, uint64_t nvec,
+                                                                   uint64_t chunk_len, uint64_t nchunks,
+                                                                   uint64_t* __restrict__ labels,
+                                                                   uint64_t* __restrict__ next_frontier,
+                                                                   const double* __restrict__ outdeg,
+                                                                   uint32_t stat_mode, uint64_t* __restrict__ partials,
+                                                                   uint64_t* __restrict__ counter, gz_guard guard)
+{
+    if (gz_guard_off(guard)) return;
+    const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
+    gz_cc_pull_op op{labels, next_frontier, outdeg, stat_mode, 0};
+    while (true) {
+        uint64_t chunk = 0;
+        if (gz_lane() == 0) chunk = atomicAdd((unsigned long long*)counter, 1ull);
+        chunk = gz_readlane_u64(chunk, 0);
+        if (chunk >= nchunks) break;
+        const uint64_t b = chunk * chunk_len;
+        const uint64_t e = (b + chunk_len < nvec) ? b + chunk_len : nvec;
+        uint64_t first_d, carry_d, carry_m;
+        uint32_t edges = 0;
+        gz_walk_chunk<VPL, uint64_t>(vecs, b, e, op, first_d, carry_d, carry_m, edges);
+        if (gz_lane() == 0) op.store(carry_d, carry_m);
+    }
+    const uint64_t total = gz_wave_sum_u64(op.vote);
+    if (gz_lane() == 0) partials[gw] = total;
+}
+
+struct gz_loop_waves {
+    uint64_t n[5]; /* vote partials each engine writes, by engine code */
+};
+
+/* One block: the vote of the engine that ran, the log entry, the next
+ * engine; the pull counter and the overflow queue's counters are cleared for
+ * the next iteration. */
+__global__ __launch_bounds__(1024) void gz_k_cc_loop_control(uint64_t* __restrict__ ctrl,
+                                                             const uint64_t* __restrict__ partials, gz_loop_waves waves,
+                                                             uint64_t* __restrict__ queue)
+{
+    __shared__ unsigned long long lds[1024 / GZ_WAVE];
+    const uint64_t engine = ctrl[GZ_LOOP_ENGINE];
+    if (engine == 0 || engine > 4) return;
+    const uint64_t n = waves.n[engine];
+    unsigned long long x = 0;
+    for (uint64_t i = threadIdx.x; i < n; i += blockDim.x) x += partials[i];
+    x = gz_wave_sum_u64(x);
+    if (gz_lane() == 0) lds[threadIdx.x / GZ_WAVE] = x;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    uint64_t vote = 0;
+    for (uint32_t w = 0; w < blockDim.x / GZ_WAVE; ++w) vote += lds[w];
+    const uint64_t it = ctrl[GZ_LOOP_ITERATIONS];
+    if (it < ctrl[GZ_LOOP_LOG_CAP]) {
+        ctrl[GZ_LOOP_LOG + 2 * it] = engine == 1 ? 1 : 2;
+        ctrl[GZ_LOOP_LOG + 2 * it + 1] = ctrl[GZ_LOOP_VOTE];
+    }
+    ctrl[GZ_LOOP_ITERATIONS] = it + 1;
+    ctrl[engine == 1 ? GZ_LOOP_PULLS : GZ_LOOP_PUSHES] += 1;
+    ctrl[GZ_LOOP_VOTE] = vote;
+    ctrl[GZ_LOOP_PULL_NEXT] = 0;
+    queue[0] = 0;
+    queue[1] = 0;
+    const uint64_t cap = ctrl[GZ_LOOP_MAX_ITERATIONS];
+    uint64_t next = 0;
+    if (vote != 0 && (cap == 0 || it + 1 < cap)) {
+        const uint64_t mode = ctrl[GZ_LOOP_MODE];
+        const int pull = mode == GZ_ENGINE_PULL ? 1 : mode == GZ_ENGINE_PUSH ? 0 : (vote > ctrl[GZ_LOOP_THRESHOLD]);
+        next = gz_loop_engine(vote, pull, ctrl[GZ_LOOP_SPARSE_OK]);
+    }
+    ctrl[GZ_LOOP_ENGINE] = next;
+}
+
+static gz_loop_waves gz_cc_loop_wave_counts()
+{
+    gz_loop_waves w;
+    uint64_t pull_waves, push_waves, wide, narrow;
+    gz_cc_loop_waves(&pull_waves, &push_waves);
+    gz_push_sparse_loop_waves(&wide, &narrow);
+    w.n[0] = 0;
+    w.n[1] = pull_waves;
+    w.n[2] = push_waves;
+    w.n[3] = wide;
+    w.n[4] = narrow;
+    return w;
+}
+
+extern "C" uint64_t gz_cc_loop_waves(uint64_t* pull_waves, uint64_t* push_waves)
+{
+    const uint64_t nchunks = gz_chunks_for(gz_graph.pull_vector_count);
+    uint64_t pw = nchunks;
+    if (pw > (uint64_t)GZ_PERSISTENT_BLOCKS * GZ_WAVES_PER_BLOCK) pw = (uint64_t)GZ_PERSISTENT_BLOCKS * GZ_WAVES_PER_BLOCK;
+    pw = (pw + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK * GZ_WAVES_PER_BLOCK;
+    if (pw == 0) pw = GZ_WAVES_PER_BLOCK;
+    *pull_waves = pw;
+    *push_waves = (uint64_t)GZ_PERSISTENT_BLOCKS * GZ_WAVES_PER_BLOCK;
+    uint64_t wide, narrow;
+    gz_push_sparse_loop_waves(&wide, &narrow);
+    uint64_t m = pw > *push_waves ? pw : *push_waves;
+    if (wide > m) m = wide;
+    if (narrow > m) m = narrow;
+    return m;
+}
+
+/* One iteration's launches, over the current HasInfo / HasInfo* roles. */
+extern "C" void gz_cc_loop_enqueue(uint64_t* ctrl)
+{
+    const uint64_t V = gz_graph.num_vertices;
+    const gz_loop_waves waves = gz_cc_loop_wave_counts();
+    uint64_t* labels = (uint64_t*)gz_graph.vertex_props;
+    uint64_t* next = (uint64_t*)gz_graph.vertex_accumulators;
+    uint64_t* partials = (uint64_t*)gz_scratch.partials;
+    const uint64_t* engine = ctrl + GZ_LOOP_ENGINE;
+    const gz_guard pull_only{engine, 1ull << 1}, compacted{engine, 1ull << 2}, wide{engine, 1ull << 3},
+        narrow{engine, 1ull << 4}, going_on{engine, 0x1eull};
+    const uint64_t chunk_len = gz_chunk_len();
+    const uint64_t nchunks = gz_chunks_for(gz_graph.pull_vector_count);
+    const uint32_t vpl = gz_vectors_per_lane();
+#define GZ_PULL_LOOP_ARGS                                                                                      \
+    dim3((unsigned)(waves.n[1] / GZ_WAVES_PER_BLOCK)), dim3(GZ_BLOCK), 0, gz_stream, gz_graph.pull_vectors,     \
+        gz_graph.pull_vector_count, chunk_len, nchunks, labels, next, gz_graph.vertex_outdegrees,               \
+        gz_phase_config.stat_mode, partials, ctrl + GZ_LOOP_PULL_NEXT, pull_only
+    if (vpl == 4)
+        hipLaunchKernelGGL(gz_k_edge_pull_cc_loop<4>, GZ_PULL_LOOP_ARGS);
+    else if (vpl == 2)
+        hipLaunchKernelGGL(gz_k_edge_pull_cc_loop<2>, GZ_PULL_LOOP_ARGS);
+    else
+        hipLaunchKernelGGL(gz_k_edge_pull_cc_loop<1>, GZ_PULL_LOOP_ARGS);
+#undef GZ_PULL_LOOP_ARGS
+    GZ_LAUNCH_CHECK();
+    const gz_cc_push_op op{labels, next, gz_graph.vertex_outdegrees, gz_phase_config.stat_mode};
+    gz_launch_push_sparse_guarded(op, gz_graph.push_vectors, gz_graph.frontier_has_info, wide, narrow);
+    gz_frontier_compact_guarded(gz_graph.frontier_has_info, gz_graph.push_vertex_offsets, V,
+                                gz_graph.push_vector_count, compacted, GZ_PERSISTENT_BLOCKS);
+    gz_slot_source src;
+    src.offsets = gz_graph.push_vertex_offsets;
+    src.list = gz_scratch.active_vertices;
+    src.prefix = gz_scratch.active_prefix;
+    src.totals = gz_scratch.active_totals;
+    src.sweep_count = gz_graph.push_vector_count;
+    hipLaunchKernelGGL(gz_k_edge_push_cc<true>, dim3(GZ_PERSISTENT_BLOCKS), dim3(GZ_BLOCK), 0, gz_stream,
+                       gz_graph.push_vectors, src, (uint64_t)256, gz_graph.frontier_has_info, labels, next,
+                       gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, partials, gz_scratch.stats, 0u, compacted);
+    GZ_LAUNCH_CHECK();
+    hipLaunchKernelGGL(gz_k_cc_loop_control, dim3(1), dim3(1024), 0, gz_stream, ctrl, partials, waves,
+                       gz_scratch.push_queue);
+    GZ_LAUNCH_CHECK();
+    /* the frontier just pushed from becomes the next HasInfo*: cleared
+     * while the loop goes on (perform_vertex_phase_cc after the swap) */
+    const uint64_t words = (V + 63) >> 6;
+    uint64_t blocks = (words + GZ_BLOCK - 1) / GZ_BLOCK;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(gz_k_clear_words, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
+                       gz_graph.frontier_has_info, (uint64_t)0, words, going_on);
+    GZ_LAUNCH_CHECK();
 }

@@ -82,30 +82,36 @@ __device__ __forceinline__ T gz_block_exclusive_scan(T x, T* lds, T& block_total
     return wave_off + incl - x;
 }
 
+/* Pass 1: per block of GZ_COMPACT_VPB vertices, its active vertices and
+ * their vectors.  Workgroups stride over the `nblocks` vertex blocks. */
 template <bool kNarrow>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_count(const uint64_t* __restrict__ bitmap,
                                                                  const void* __restrict__ src,
                                                                  uint64_t num_vertices,
                                                                  uint64_t* __restrict__ block_sums,
-                                                                 uint64_t* __restrict__ stamp)
+                                                                 uint64_t* __restrict__ stamp, uint64_t nblocks,
+                                                                 gz_guard guard)
 {
     __shared__ unsigned long long lds[GZ_WAVES_PER_BLOCK];
+    if (gz_guard_off(guard)) return;
     if (stamp != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *stamp = (uint64_t)wall_clock64();
-    const uint64_t v0 = (uint64_t)blockIdx.x * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
-    uint64_t c[GZ_COMPACT_VPT];
-    gz_run_counts<kNarrow>(bitmap, src, num_vertices, v0, c);
-    unsigned long long nv = 0, nvec = 0;
+    for (uint64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const uint64_t v0 = b * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
+        uint64_t c[GZ_COMPACT_VPT];
+        gz_run_counts<kNarrow>(bitmap, src, num_vertices, v0, c);
+        unsigned long long nv = 0, nvec = 0;
 #pragma unroll
-    for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
-        nv += c[k] ? 1 : 0;
-        nvec += c[k];
-    }
-    unsigned long long tv, tvec;
-    gz_block_exclusive_scan<unsigned long long>(nv, lds, tv);
-    gz_block_exclusive_scan<unsigned long long>(nvec, lds, tvec);
-    if (threadIdx.x == 0) {
-        block_sums[2 * blockIdx.x] = tv;
-        block_sums[2 * blockIdx.x + 1] = tvec;
+        for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
+            nv += c[k] ? 1 : 0;
+            nvec += c[k];
+        }
+        unsigned long long tv, tvec;
+        gz_block_exclusive_scan<unsigned long long>(nv, lds, tv);
+        gz_block_exclusive_scan<unsigned long long>(nvec, lds, tvec);
+        if (threadIdx.x == 0) {
+            block_sums[2 * b] = tv;
+            block_sums[2 * b + 1] = tvec;
+        }
     }
 }
 
@@ -116,11 +122,13 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_count(const uint64_t* 
  * the thread sums, and the pass is written back coalesced. */
 #define GZ_SCAN_PASS 4096
 __global__ __launch_bounds__(1024) void gz_k_frontier_scan(uint64_t* __restrict__ block_sums, uint64_t blocks,
-                                                           uint64_t* __restrict__ totals, uint64_t* __restrict__ prefix)
+                                                           uint64_t* __restrict__ totals, uint64_t* __restrict__ prefix,
+                                                           gz_guard guard)
 {
     __shared__ unsigned long long stage[2 * GZ_SCAN_PASS];
     __shared__ unsigned long long lds_v[2][1024 / GZ_WAVE];
     __shared__ unsigned long long carry[2];
+    if (gz_guard_off(guard)) return;
     const int per = GZ_SCAN_PASS / 1024;
     if (threadIdx.x == 0) {
         carry[0] = 0;
@@ -196,34 +204,39 @@ __global__ __launch_bounds__(1024) void gz_k_frontier_scan(uint64_t* __restrict_
     }
 }
 
+/* Pass 3: each block's active vertices and their vector prefix, in order. */
 template <bool kNarrow>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_write(const uint64_t* __restrict__ bitmap,
                                                                  const void* __restrict__ src,
                                                                  uint64_t num_vertices,
                                                                  const uint64_t* __restrict__ block_sums,
                                                                  uint64_t* __restrict__ list,
-                                                                 uint64_t* __restrict__ prefix)
+                                                                 uint64_t* __restrict__ prefix, uint64_t nblocks,
+                                                                 gz_guard guard)
 {
     __shared__ unsigned long long lds[GZ_WAVES_PER_BLOCK];
-    const uint64_t v0 = (uint64_t)blockIdx.x * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
-    uint64_t c[GZ_COMPACT_VPT];
-    gz_run_counts<kNarrow>(bitmap, src, num_vertices, v0, c);
-    unsigned long long nv = 0, nvec = 0;
+    if (gz_guard_off(guard)) return;
+    for (uint64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const uint64_t v0 = b * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
+        uint64_t c[GZ_COMPACT_VPT];
+        gz_run_counts<kNarrow>(bitmap, src, num_vertices, v0, c);
+        unsigned long long nv = 0, nvec = 0;
 #pragma unroll
-    for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
-        nv += c[k] ? 1 : 0;
-        nvec += c[k];
-    }
-    unsigned long long tv, tvec;
-    uint64_t pos = block_sums[2 * blockIdx.x] + gz_block_exclusive_scan<unsigned long long>(nv, lds, tv);
-    uint64_t vpos = block_sums[2 * blockIdx.x + 1] + gz_block_exclusive_scan<unsigned long long>(nvec, lds, tvec);
+        for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
+            nv += c[k] ? 1 : 0;
+            nvec += c[k];
+        }
+        unsigned long long tv, tvec;
+        uint64_t pos = block_sums[2 * b] + gz_block_exclusive_scan<unsigned long long>(nv, lds, tv);
+        uint64_t vpos = block_sums[2 * b + 1] + gz_block_exclusive_scan<unsigned long long>(nvec, lds, tvec);
 #pragma unroll
-    for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
-        if (c[k]) {
-            list[pos] = v0 + (uint64_t)k;
-            prefix[pos] = vpos;
-            pos += 1;
-            vpos += c[k];
+        for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
+            if (c[k]) {
+                list[pos] = v0 + (uint64_t)k;
+                prefix[pos] = vpos;
+                pos += 1;
+                vpos += c[k];
+            }
         }
     }
 }
@@ -251,6 +264,16 @@ extern "C" void gz_frontier_counts_release(void)
     gz_counts_next = 0;
 }
 
+/* The counts of a list if they are built, else nullptr (allocates nothing). */
+static const uint32_t* gz_frontier_counts_cached(const uint64_t* offsets, uint64_t num_vertices, uint64_t vector_count)
+{
+    for (auto& e : gz_counts_cache)
+        if (e.counts != nullptr && e.offsets == offsets && e.num_vertices == num_vertices &&
+            e.vector_count == vector_count)
+            return e.counts;
+    return nullptr;
+}
+
 static const uint32_t* gz_frontier_counts(const uint64_t* offsets, uint64_t num_vertices, uint64_t vector_count)
 {
     for (auto& e : gz_counts_cache)
@@ -273,8 +296,8 @@ static const uint32_t* gz_frontier_counts(const uint64_t* offsets, uint64_t num_
     return e.counts;
 }
 
-void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
-                         uint64_t vector_count)
+static void gz_frontier_compact_impl(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
+                                     uint64_t vector_count, gz_guard guard, uint64_t max_grid, bool allow_build)
 {
     const uint64_t blocks = gz_compact_blocks(num_vertices);
     if (blocks == 0) {
@@ -282,30 +305,51 @@ void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64
         gz_device_zero(gz_scratch.active_prefix, sizeof(uint64_t));
         return;
     }
+    const uint64_t grid = blocks < max_grid ? blocks : max_grid;
     /* below 2^32 vectors in the list every vertex's count fits 32 bits */
-    const bool narrow = gz_compact_narrow && vector_count < (1ull << 32);
-    const void* src =
-        narrow ? (const void*)gz_frontier_counts(offsets, num_vertices, vector_count) : (const void*)offsets;
+    bool narrow = gz_compact_narrow && vector_count < (1ull << 32);
+    const void* src = (const void*)offsets;
+    if (narrow) {
+        src = allow_build ? (const void*)gz_frontier_counts(offsets, num_vertices, vector_count)
+                          : (const void*)gz_frontier_counts_cached(offsets, num_vertices, vector_count);
+        if (src == nullptr) {
+            narrow = false;
+            src = (const void*)offsets;
+        }
+    }
+    uint64_t* stamp = guard.word == nullptr ? gz_scratch.stamp_target : nullptr;
+    if (guard.word == nullptr) gz_scratch.stamp_target = nullptr;
     gz_ktimer_begin(GZ_K_COMPACT);
     if (narrow)
-        hipLaunchKernelGGL(gz_k_frontier_count<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, src,
-                           num_vertices, gz_scratch.active_block_sums, gz_scratch.stamp_target);
+        hipLaunchKernelGGL(gz_k_frontier_count<true>, dim3((unsigned)grid), dim3(GZ_BLOCK), 0, gz_stream, bitmap, src,
+                           num_vertices, gz_scratch.active_block_sums, stamp, blocks, guard);
     else
-        hipLaunchKernelGGL(gz_k_frontier_count<false>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap,
-                           src, num_vertices, gz_scratch.active_block_sums, gz_scratch.stamp_target);
-    gz_scratch.stamp_target = nullptr;
+        hipLaunchKernelGGL(gz_k_frontier_count<false>, dim3((unsigned)grid), dim3(GZ_BLOCK), 0, gz_stream, bitmap,
+                           src, num_vertices, gz_scratch.active_block_sums, stamp, blocks, guard);
     GZ_LAUNCH_CHECK();
     hipLaunchKernelGGL(gz_k_frontier_scan, dim3(1), dim3(1024), 0, gz_stream, gz_scratch.active_block_sums, blocks,
-                       gz_scratch.active_totals, gz_scratch.active_prefix);
+                       gz_scratch.active_totals, gz_scratch.active_prefix, guard);
     GZ_LAUNCH_CHECK();
     if (narrow)
-        hipLaunchKernelGGL(gz_k_frontier_write<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, src,
+        hipLaunchKernelGGL(gz_k_frontier_write<true>, dim3((unsigned)grid), dim3(GZ_BLOCK), 0, gz_stream, bitmap, src,
                            num_vertices, gz_scratch.active_block_sums, gz_scratch.active_vertices,
-                           gz_scratch.active_prefix);
+                           gz_scratch.active_prefix, blocks, guard);
     else
-        hipLaunchKernelGGL(gz_k_frontier_write<false>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap,
+        hipLaunchKernelGGL(gz_k_frontier_write<false>, dim3((unsigned)grid), dim3(GZ_BLOCK), 0, gz_stream, bitmap,
                            src, num_vertices, gz_scratch.active_block_sums, gz_scratch.active_vertices,
-                           gz_scratch.active_prefix);
+                           gz_scratch.active_prefix, blocks, guard);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_COMPACT);
+}
+
+void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
+                         uint64_t vector_count)
+{
+    gz_frontier_compact_impl(bitmap, offsets, num_vertices, vector_count, gz_guard{nullptr, 0}, ~0ull, true);
+}
+
+void gz_frontier_compact_guarded(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
+                                 uint64_t vector_count, gz_guard guard, uint64_t max_grid)
+{
+    gz_frontier_compact_impl(bitmap, offsets, num_vertices, vector_count, guard, max_grid, false);
 }

@@ -54,9 +54,51 @@ uint32_t gz_selector_host_decide(const float* x, float* y);
  * tells the caller to run the rounds itself (enqueue must not allocate or
  * wait). */
 int gz_graph_replay_cached(const uint64_t* key, size_t n, void (*enqueue)(void*), void* ctx, uint32_t times);
+/* The same in cache slot `slot` (0: PageRank iterations, 1: CC iterations). */
+int gz_graph_replay_cached_slot(uint32_t slot, const uint64_t* key, size_t n, void (*enqueue)(void*), void* ctx,
+                                uint32_t times);
 /* Frontier statistic of the sources the next CC push will read (the vote that
  * built its frontier); a small one lets the push skip the compaction. */
 void gz_cc_push_vote_hint(uint64_t vote);
+/* Control words of a device-driven iteration loop (CC, and the log of
+ * (engine, vote) pairs from GZ_LOOP_LOG on). */
+#define GZ_LOOP_ENGINE 0          /* engine of the iteration about to run: 0 stop, 1 pull, 2 push over the
+                                   * compacted frontier, 3 / 4 the small-frontier push reading 64 / 16
+                                   * bitmap words per wavefront (gz_loop_engine) */
+#define GZ_LOOP_VOTE 1            /* the vote that chose it */
+#define GZ_LOOP_ITERATIONS 2
+#define GZ_LOOP_PULLS 3
+#define GZ_LOOP_PUSHES 4
+#define GZ_LOOP_THRESHOLD 5
+#define GZ_LOOP_MODE 6            /* enum gz_engine_mode: hybrid, pull or push */
+#define GZ_LOOP_MAX_ITERATIONS 7  /* 0: none */
+#define GZ_LOOP_LOG_CAP 8
+#define GZ_LOOP_PULL_NEXT 9       /* next pull chunk to hand out (cleared after each iteration) */
+#define GZ_LOOP_SPARSE_OK 10      /* the vote bounds the frontier's vectors: small-frontier pushes allowed */
+#define GZ_LOOP_LOG 16
+#define GZ_LOOP_WORDS(cap) (GZ_LOOP_LOG + 2 * (cap))
+
+/* The CC iteration loop driven by control words on the device (gz_cc.hip):
+ * the vote partials each engine writes, and one iteration's launches over
+ * the current frontier roles. */
+uint64_t gz_cc_loop_waves(uint64_t* pull_waves, uint64_t* push_waves);
+/* The engine code of an iteration chosen by `vote` (pull: 1, push: 2..4),
+ * used by the host before the loop and by the loop's control kernel. */
+#ifdef __HIPCC__
+#define GZ_HOST_DEVICE __host__ __device__
+#else
+#define GZ_HOST_DEVICE
+#endif
+static inline GZ_HOST_DEVICE uint64_t gz_loop_engine(uint64_t vote, int pull, uint64_t sparse_ok)
+{
+    if (pull) return 1;
+    if (!sparse_ok || vote > (1u << 16)) return 2;
+    return vote <= 4096 ? 3 : 4;
+}
+void gz_cc_loop_enqueue(uint64_t* ctrl);
+/* gz_scratch_reserve for host C code: per-wave buffers for `waves`
+ * wavefronts and vertex-sized buffers for V. */
+void gz_scratch_reserve_public(uint64_t waves, uint64_t num_vertices);
 /* The same for the next BFS push (the vote of the level that built its
  * frontier). */
 void gz_bfs_push_vote_hint(uint64_t vote);

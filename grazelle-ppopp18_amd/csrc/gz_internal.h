@@ -359,6 +359,25 @@ __device__ __forceinline__ void gz_feature_flush(const gz_feature_acc& f, const 
     }
 }
 
+/* A kernel launched inside a captured iteration loop reads the loop's
+ * engine word first and runs only when the engine is one of `mask` (bit e for
+ * engine e); word == nullptr: always runs. */
+struct gz_guard {
+    const uint64_t* word;
+    uint64_t mask;
+};
+
+__device__ __forceinline__ bool gz_guard_off(const gz_guard& g)
+{
+    return g.word != nullptr && ((g.mask >> (*g.word & 63)) & 1ull) == 0;
+}
+
+/* gz_frontier_compact whose three passes run under `guard`, on grids of at
+ * most `max_grid` workgroups (the counts of the narrow path must already be
+ * built: nothing is allocated, so the launches can be captured). */
+void gz_frontier_compact_guarded(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
+                                 uint64_t vector_count, gz_guard guard, uint64_t max_grid);
+
 /* Bitmaps are arrays of uint64 words, bit v%64 of word v/64 (LSB first).  The
  * device updates them through 32-bit atomics on the same bytes. */
 __device__ __forceinline__ bool gz_bit_test(const uint64_t* __restrict__ bm, uint64_t v)

@@ -141,12 +141,15 @@ struct gz_replay_key_t {
     size_t n;
     uint64_t key[GZ_REPLAY_KEY_WORDS];
 };
-static struct {
+/* One cached graph per slot: GZ_REPLAY_SLOTS kinds of loops (PageRank
+ * iterations, CC iterations) keep their captures apart. */
+#define GZ_REPLAY_SLOTS 2
+static struct gz_replay_slot_t {
     hipGraphExec_t exec;
     gz_replay_key_t captured; /* what exec holds */
     gz_replay_key_t seen;     /* the last configuration asked for and not captured */
     bool seen_valid;
-} gz_replay;
+} gz_replay[GZ_REPLAY_SLOTS];
 static uint64_t gz_replay_captures, gz_replay_launches;
 
 extern "C" void gz_graph_replay_counts(uint64_t* captures, uint64_t* launches)
@@ -157,8 +160,9 @@ extern "C" void gz_graph_replay_counts(uint64_t* captures, uint64_t* launches)
 
 static void gz_replay_release(void)
 {
-    if (gz_replay.exec != nullptr) GZ_CHECK(hipGraphExecDestroy(gz_replay.exec));
-    memset(&gz_replay, 0, sizeof(gz_replay));
+    for (auto& r : gz_replay)
+        if (r.exec != nullptr) GZ_CHECK(hipGraphExecDestroy(r.exec));
+    memset(gz_replay, 0, sizeof(gz_replay));
 }
 
 static bool gz_replay_key_equal(const gz_replay_key_t& a, const uint64_t* key, size_t n, uint32_t times)
@@ -167,40 +171,51 @@ static bool gz_replay_key_equal(const gz_replay_key_t& a, const uint64_t* key, s
            memcmp(a.key, key, n * sizeof(uint64_t)) == 0;
 }
 
-extern "C" int gz_graph_replay_cached(const uint64_t* key, size_t n, void (*enqueue)(void*), void* ctx,
-                                      uint32_t times)
+extern "C" int gz_graph_replay_cached_slot(uint32_t slot, const uint64_t* key, size_t n, void (*enqueue)(void*),
+                                           void* ctx, uint32_t times)
 {
     gz_require_runtime();
+    if (slot >= GZ_REPLAY_SLOTS) {
+        fprintf(stderr, "grazelle: replay slot %u (there are %d)\n", slot, GZ_REPLAY_SLOTS);
+        exit(72);
+    }
+    gz_replay_slot_t& R = gz_replay[slot];
     if (times == 0) return 1;
     if (n > GZ_REPLAY_KEY_WORDS) {
         fprintf(stderr, "grazelle: replay key of %zu words (at most %d)\n", n, GZ_REPLAY_KEY_WORDS);
         exit(72);
     }
-    const bool hit = gz_replay.exec != nullptr && gz_replay_key_equal(gz_replay.captured, key, n, times);
+    const bool hit = R.exec != nullptr && gz_replay_key_equal(R.captured, key, n, times);
     if (!hit) {
-        const bool repeat = gz_replay.seen_valid && gz_replay_key_equal(gz_replay.seen, key, n, times);
-        gz_replay.seen.generation = gz_alloc_generation;
-        gz_replay.seen.times = times;
-        gz_replay.seen.n = n;
-        memcpy(gz_replay.seen.key, key, n * sizeof(uint64_t));
-        gz_replay.seen_valid = true;
+        const bool repeat = R.seen_valid && gz_replay_key_equal(R.seen, key, n, times);
+        R.seen.generation = gz_alloc_generation;
+        R.seen.times = times;
+        R.seen.n = n;
+        memcpy(R.seen.key, key, n * sizeof(uint64_t));
+        R.seen_valid = true;
         /* a first request: the caller runs the rounds itself */
         if (!repeat) return 0;
-        if (gz_replay.exec != nullptr) GZ_CHECK(hipGraphExecDestroy(gz_replay.exec));
-        gz_replay.exec = nullptr;
+        if (R.exec != nullptr) GZ_CHECK(hipGraphExecDestroy(R.exec));
+        R.exec = nullptr;
         hipGraph_t g = nullptr;
         GZ_CHECK(hipStreamBeginCapture(gz_stream, hipStreamCaptureModeThreadLocal));
         for (uint32_t i = 0; i < times; ++i) enqueue(ctx);
         GZ_CHECK(hipStreamEndCapture(gz_stream, &g));
-        GZ_CHECK(hipGraphInstantiate(&gz_replay.exec, g, nullptr, nullptr, 0));
+        GZ_CHECK(hipGraphInstantiate(&R.exec, g, nullptr, nullptr, 0));
         GZ_CHECK(hipGraphDestroy(g));
-        gz_replay.captured = gz_replay.seen;
-        gz_replay.seen_valid = false;
+        R.captured = R.seen;
+        R.seen_valid = false;
         gz_replay_captures += 1;
     }
-    GZ_CHECK(hipGraphLaunch(gz_replay.exec, gz_stream));
+    GZ_CHECK(hipGraphLaunch(R.exec, gz_stream));
     gz_replay_launches += 1;
     return 1;
+}
+
+extern "C" int gz_graph_replay_cached(const uint64_t* key, size_t n, void (*enqueue)(void*), void* ctx,
+                                      uint32_t times)
+{
+    return gz_graph_replay_cached_slot(0, key, n, enqueue, ctx, times);
 }
 
 extern "C" void gz_copy_to_device(void* dst, const void* src, size_t bytes)
@@ -456,6 +471,8 @@ void gz_scratch_reserve(uint64_t waves, uint64_t num_vertices)
         gz_scratch.active_capacity = num_vertices;
     }
 }
+
+extern "C" void gz_scratch_reserve_public(uint64_t waves, uint64_t num_vertices) { gz_scratch_reserve(waves, num_vertices); }
 
 extern "C" void gz_merge_buffer_allocate(uint64_t units)
 {

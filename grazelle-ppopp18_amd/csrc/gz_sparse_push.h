@@ -123,8 +123,9 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_push_sparse(
     Op op, const gz_vec_t* __restrict__ vecs, const uint64_t* __restrict__ offsets,
     const uint64_t* __restrict__ has_info, uint64_t num_vertices, uint64_t* __restrict__ pieces,
     uint64_t* __restrict__ queue, uint64_t queue_cap, uint64_t* __restrict__ partials, uint64_t* __restrict__ stats,
-    uint32_t collect, uint64_t* __restrict__ stamp)
+    uint32_t collect, uint64_t* __restrict__ stamp, gz_guard guard)
 {
+    if (gz_guard_off(guard)) return;
     constexpr int kSteps = GZ_PUSH_PIECE_VECTORS / GZ_WAVE;
     const int lane = gz_lane();
     const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
@@ -229,8 +230,10 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_push_pieces(Op op, const gz_vec
                                                              uint64_t* __restrict__ queue, uint64_t queue_cap,
                                                              uint32_t parity, uint64_t wave_base,
                                                              uint64_t* __restrict__ partials,
-                                                             uint64_t* __restrict__ stats, uint32_t collect)
+                                                             uint64_t* __restrict__ stats, uint32_t collect,
+                                                             gz_guard guard)
 {
+    if (gz_guard_off(guard)) return;
     constexpr int kSteps = GZ_PUSH_PIECE_VECTORS / GZ_WAVE;
     const int lane = gz_lane();
     const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
@@ -310,7 +313,8 @@ static void gz_launch_push_sparse(const Op& op, const gz_vec_t* edge_list, uint6
 #define GZ_PUSH_SPARSE_ARGS                                                                                   \
     dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, op, edge_list, gz_graph.push_vertex_offsets,           \
         gz_graph.frontier_has_info, V, gz_scratch.push_pieces, gz_scratch.push_queue + parity,                    \
-        (uint64_t)gz_push_queue_cap, partials, gz_scratch.stats, gz_phase_config.collect_stats, stamp
+        (uint64_t)gz_push_queue_cap, partials, gz_scratch.stats, gz_phase_config.collect_stats, stamp,             \
+        gz_guard{nullptr, 0}
     if (wide) {
         hipLaunchKernelGGL((gz_k_push_sparse<Op, 64, false>), GZ_PUSH_SPARSE_ARGS);
         GZ_LAUNCH_CHECK();
@@ -320,11 +324,52 @@ static void gz_launch_push_sparse(const Op& op, const gz_vec_t* edge_list, uint6
         hipLaunchKernelGGL((gz_k_push_pieces<Op>), dim3((unsigned)qblocks), dim3(GZ_BLOCK), 0, gz_stream, op,
                            edge_list, gz_scratch.push_pieces, gz_scratch.push_queue, (uint64_t)gz_push_queue_cap,
                            parity, sw, partials,
-                           gz_scratch.stats, gz_phase_config.collect_stats);
+                           gz_scratch.stats, gz_phase_config.collect_stats, gz_guard{nullptr, 0});
         GZ_LAUNCH_CHECK();
         gz_scratch.push_queue_parity = parity ^ 1u;
     }
 #undef GZ_PUSH_SPARSE_ARGS
+}
+
+/* Wavefronts of the small-frontier push at each width for the loaded graph:
+ * 64 words per wavefront, and 16 words per wavefront plus the queue's. */
+static inline void gz_push_sparse_loop_waves(uint64_t* wide_waves, uint64_t* narrow_waves)
+{
+    const uint64_t words = (gz_graph.num_vertices + 63) >> 6;
+    const uint64_t wblocks = ((words + 63) / 64 + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+    const uint64_t nblocks = ((words + 15) / 16 + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+    *wide_waves = (wblocks ? wblocks : 1) * GZ_WAVES_PER_BLOCK;
+    *narrow_waves = ((nblocks ? nblocks : 1) + GZ_PERSISTENT_BLOCKS / 4) * GZ_WAVES_PER_BLOCK;
+}
+
+/* Both widths of the small-frontier push, each under its own guard (for a
+ * loop that picks the width on the device from the vote).  The queue
+ * counters must be zero when a narrow push starts: the loop's control kernel
+ * clears them after every iteration. */
+template <class Op>
+static void gz_launch_push_sparse_guarded(const Op& op, const gz_vec_t* edge_list, const uint64_t* has_info,
+                                          gz_guard wide, gz_guard narrow)
+{
+    const uint64_t V = gz_graph.num_vertices;
+    uint64_t ww, nw;
+    gz_push_sparse_loop_waves(&ww, &nw);
+    uint64_t* partials = (uint64_t*)gz_scratch.partials;
+    hipLaunchKernelGGL((gz_k_push_sparse<Op, 64, false>), dim3((unsigned)(ww / GZ_WAVES_PER_BLOCK)), dim3(GZ_BLOCK), 0,
+                       gz_stream, op, edge_list, gz_graph.push_vertex_offsets, has_info, V, gz_scratch.push_pieces,
+                       gz_scratch.push_queue, (uint64_t)gz_push_queue_cap, partials, gz_scratch.stats, 0u,
+                       (uint64_t*)nullptr, wide);
+    GZ_LAUNCH_CHECK();
+    const uint64_t qblocks = GZ_PERSISTENT_BLOCKS / 4;
+    const uint64_t sblocks = nw / GZ_WAVES_PER_BLOCK - qblocks;
+    hipLaunchKernelGGL((gz_k_push_sparse<Op, 16, true>), dim3((unsigned)sblocks), dim3(GZ_BLOCK), 0, gz_stream, op,
+                       edge_list, gz_graph.push_vertex_offsets, has_info, V, gz_scratch.push_pieces,
+                       gz_scratch.push_queue, (uint64_t)gz_push_queue_cap, partials, gz_scratch.stats, 0u,
+                       (uint64_t*)nullptr, narrow);
+    GZ_LAUNCH_CHECK();
+    hipLaunchKernelGGL((gz_k_push_pieces<Op>), dim3((unsigned)qblocks), dim3(GZ_BLOCK), 0, gz_stream, op, edge_list,
+                       gz_scratch.push_pieces, gz_scratch.push_queue, (uint64_t)gz_push_queue_cap, 0u,
+                       sblocks * GZ_WAVES_PER_BLOCK, partials, gz_scratch.stats, 0u, narrow);
+    GZ_LAUNCH_CHECK();
 }
 
 #endif /* GZ_SPARSE_PUSH_H */

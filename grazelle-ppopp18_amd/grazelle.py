@@ -100,6 +100,7 @@ _sig("gz_graph_indegrees", None, vp)
 _sig("gz_push_queue_capacity", C.c_int, u32)
 _sig("gz_graph_build_source_slices", C.c_int, u64, u32, u32, u64)
 _sig("gz_pr_graph_replay", None, C.c_int)
+_sig("gz_cc_graph_replay", None, C.c_int)
 _sig("gz_graph_replay_counts", None, C.POINTER(u64), C.POINTER(u64))
 _sig("gz_frontier_compact_narrow", None, C.c_int)
 _sig("gz_graph_source_slices_info", C.c_int, vp, P(u64), P(u32), P(u32), P(u64))
@@ -429,6 +430,12 @@ def pr_graph_replay(on: bool) -> None:
     """PageRank iterations 2.. from a replayed HIP graph (default on)."""
     init()
     lib.gz_pr_graph_replay(int(on))
+
+
+def cc_graph_replay(on: bool) -> None:
+    """CC iterations with the engine chosen on the device, from a captured graph (default off)."""
+    init()
+    lib.gz_cc_graph_replay(int(on))
 
 
 def compact_narrow(on: bool) -> None:

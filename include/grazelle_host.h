@@ -177,6 +177,15 @@ int gz_slices_set_hot_cache(uint32_t vertices);
  * iteration count, device allocations) is captured the second time in a
  * row it runs and replayed while it repeats; results are the same. */
 void gz_pr_graph_replay(int on);
+/* CC runs that record nothing per iteration, with the fixed threshold or a
+ * forced engine over the scheduler-aware pull, keep the vote and the engine
+ * choice on the device and run their iterations as a captured HIP graph,
+ * four iterations per launch (off by default, since each captured iteration
+ * also dispatches the kernels of the engines it does not run; no reference
+ * counterpart, execution_cc.c:124-142 decides on the host).  A configuration
+ * is captured the second time in a row it runs; labels are the same either
+ * way. */
+void gz_cc_graph_replay(int on);
 /* Graph captures and graph launches made so far (no reference counterpart). */
 void gz_graph_replay_counts(uint64_t* captures, uint64_t* launches);
 
