@@ -21,6 +21,7 @@
  *   -r root                     BFS root (reference constant 0)
  *   -t both|outdeg|count        frontier statistic (THRESHOLD_WITHOUT_COUNT / _OUTDEGREES)
  *   -T                          traditional pull (EDGE_PULL_WITHOUT_SCHED_AWARE)
+ *   -M                          CC pull through the merge buffer (EDGE_PULL_FORCE_MERGE)
  *   -E / -X                     edge phase only / vertex phase only
  *   -p                          per-iteration profile CSV on stderr (ITERATION_PROFILE)
  *   -S                          per-iteration vector statistics on stderr (ITERATION_STATS)
@@ -62,7 +63,7 @@ typedef struct gz_cli_t {
 
 static int gz_is_switch(char c) { return c == '-'; }
 
-static int gz_recognized(char c) { return strchr("hinNosVuaedrtTEXpSwDRL", c) != NULL; }
+static int gz_recognized(char c) { return strchr("hinNosVuaedrtTMEXpSwDRL", c) != NULL; }
 
 static int gz_requires_value(char c) { return strchr("inNousaedrtwDL", c) != NULL; }
 
@@ -91,6 +92,7 @@ static void gz_usage(const char* argv0)
     printf("  -r root\n        BFS search root. Defaults to 0.\n");
     printf("  -t both|outdeg|count\n        Frontier statistic for the engine threshold. Defaults to both.\n");
     printf("  -T\n        Traditional (atomic) pull instead of scheduler-aware merge buffers.\n");
+    printf("  -M\n        CC: boundary runs of the pull through the merge buffer (plain stores elsewhere).\n");
     printf("  -E, -X\n        Edge phase only / vertex phase only (PageRank).\n");
     printf("  -p\n        Per-iteration profile CSV on stderr.\n");
     printf("  -S\n        Per-iteration vector statistics CSV on stderr.\n");
@@ -201,6 +203,9 @@ static void gz_parse_option(gz_cli_t* c, const char* argv0, const char* opt, con
     case 'T':
         c->run.pull_variant = GZ_PULL_TRADITIONAL;
         break;
+    case 'M':
+        c->run.pull_variant = GZ_PULL_MERGE_BUFFER;
+        break;
     case 'E':
         c->run.edge_only = 1;
         break;
@@ -269,7 +274,7 @@ int gz_main(int argc, char** argv)
     if (c.run.edge_only && c.run.vertex_only) gz_err(argv0, "%s: Specified option combination is incompatible.%s%s\n", "", "", 6);
     if (c.run.engine == GZ_ENGINE_LEARNED && c.weights == NULL)
         gz_err(argv0, "%s: Specified option combination is incompatible.%s%s\n", "", "", 6);
-    if (c.slice_log2 && c.run.pull_variant == GZ_PULL_TRADITIONAL)
+    if (c.slice_log2 && (c.run.pull_variant == GZ_PULL_TRADITIONAL || c.run.pull_variant == GZ_PULL_MERGE_BUFFER))
         gz_err(argv0, "%s: Specified option combination is incompatible.%s%s\n", "", "", 6);
     c.run.iterations = c.iterations;
 

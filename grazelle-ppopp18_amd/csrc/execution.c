@@ -251,7 +251,7 @@ static void gz_pr_enqueue_iteration(void* p)
     phase_op_reset_global_accum();
     if (!c->use_push) {
         perform_edge_pull_phase_pr(gz_graph.pull_vectors, gz_graph.pull_vector_count);
-        if (c->pull_variant == GZ_PULL_SCHED_AWARE)
+        if (c->pull_variant == GZ_PULL_SCHED_AWARE || c->pull_variant == GZ_PULL_MERGE_BUFFER)
             edge_pull_op_merge_with_merge_buffer(gz_merge_buffer, gz_pull_units_for(gz_graph.pull_vector_count),
                                                  (double*)gz_graph.vertex_accumulators);
     } else {
@@ -335,7 +335,7 @@ static void gz_run_pr(const gz_run_options_t* o, gz_run_result_t* r)
                 r->pull_iterations += 1;
                 rec.engine = 1;
                 perform_edge_pull_phase_pr(gz_graph.pull_vectors, gz_graph.pull_vector_count);
-                if (o->pull_variant == GZ_PULL_SCHED_AWARE)
+                if (o->pull_variant == GZ_PULL_SCHED_AWARE || o->pull_variant == GZ_PULL_MERGE_BUFFER)
                     edge_pull_op_merge_with_merge_buffer(gz_merge_buffer, gz_pull_units_for(gz_graph.pull_vector_count),
                                                          (double*)gz_graph.vertex_accumulators);
             } else {

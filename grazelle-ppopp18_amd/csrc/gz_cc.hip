@@ -15,6 +15,10 @@
  * labels in place, so updates made earlier in the sweep propagate within it,
  * as in the reference.
  *
+ * With the merge-buffer variant (GZ_PULL_MERGE_BUFFER) complete runs use
+ * plain stores and each chunk's last run goes through its merge record,
+ * merged with min after the sweep, as EXPERIMENT_EDGE_PULL_FORCE_MERGE does.
+ *
  * Push: only sources in HasInfo send (the strong frontier check).  The
  * frontier is compacted first and the kernel walks the active sources'
  * vectors with all lanes busy; each valid lane applies an atomic min of the
@@ -102,6 +106,147 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_cc(const gz_vec_t* __
     }
 }
 
+/* ---- the merge-buffer pull (EXPERIMENT_EDGE_PULL_FORCE_MERGE) ----------------
+ * phase_edge_pull_cc.asm:136-181 and :328-354, execution_cc.c:165-181: a
+ * run that is complete inside its unit lowers the label with a plain store
+ * (no other unit holds that destination); the unit's last run goes to its
+ * merge-buffer record as min(run, current label), setting HasInfo* and
+ * counting the frontier statistic when it is lower; afterwards the records
+ * are merged with min (edge_pull_op_merge_with_merge_buffer with
+ * SCALAR_REDUCE_OP = min), including the label the next unit stored
+ * directly when it starts on the same destination. */
+struct gz_cc_merge_op {
+    uint64_t* __restrict__ labels;
+    uint64_t* __restrict__ next_frontier;
+    const double* __restrict__ outdeg;
+    uint32_t stat_mode;
+    uint64_t vote;
+    __device__ __forceinline__ uint64_t identity() const { return ~0ull; }
+    __device__ __forceinline__ uint64_t combine(uint64_t a, uint64_t b) const { return a < b ? a : b; }
+    __device__ __forceinline__ uint64_t gather(const gz_lanes& v, bool in_range, uint32_t& edges) const
+    {
+        const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
+        uint64_t x[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) x[k] = labels[(in_range && gz_lane_valid(l[k])) ? gz_lane_id(l[k]) : 0];
+        uint64_t m = ~0ull;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool ok = in_range && gz_lane_valid(l[k]);
+            m = (ok && x[k] < m) ? x[k] : m;
+            edges += ok ? 1u : 0u;
+        }
+        return m;
+    }
+    __device__ __forceinline__ void store(uint64_t d, uint64_t m)
+    {
+        if (m < labels[d]) {
+            labels[d] = m;
+            gz_bit_set(next_frontier, d);
+            vote += gz_frontier_stat(outdeg, d, stat_mode);
+        }
+    }
+};
+
+template <int VPL>
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_cc_merge(const gz_vec_t* __restrict__ vecs, uint64_t nvec,
+                                                                    uint64_t chunk_len, uint64_t nchunks,
+                                                                    uint64_t* __restrict__ labels,
+                                                                    uint64_t* __restrict__ next_frontier,
+                                                                    const double* __restrict__ outdeg,
+                                                                    uint32_t stat_mode, mergeaccum_t* __restrict__ merge,
+                                                                    uint64_t* __restrict__ partials,
+                                                                    uint64_t* __restrict__ stats, uint32_t collect)
+{
+    const int lane = gz_lane();
+    const uint64_t chunk = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
+    if (chunk >= nchunks) return;
+    const uint64_t b = chunk * chunk_len;
+    const uint64_t e = (b + chunk_len < nvec) ? b + chunk_len : nvec;
+    gz_cc_merge_op op{labels, next_frontier, outdeg, stat_mode, 0};
+    uint64_t first_d, carry_d, carry_m;
+    uint32_t edges = 0;
+    gz_walk_chunk<VPL, uint64_t>(vecs, b, e, op, first_d, carry_d, carry_m, edges);
+    if (lane == 0) {
+        const uint64_t cur = labels[carry_d];
+        const uint64_t m = carry_m < cur ? carry_m : cur;
+        if (m < cur) {
+            gz_bit_set(next_frontier, carry_d);
+            op.vote += gz_frontier_stat(outdeg, carry_d, stat_mode);
+        }
+        mergeaccum_t r;
+        r.initial_vertex_id = first_d;
+        r.final_vertex_id = carry_d;
+        r.final_partial_value = __longlong_as_double((long long)m);
+        r.__padding__ = 0;
+        merge[chunk] = r;
+    }
+    const uint64_t total = gz_wave_sum_u64(op.vote);
+    if (lane == 0) partials[chunk] = total;
+    if (collect) {
+        const uint64_t tot_edges = gz_wave_sum_u64(edges);
+        if (lane == 0) {
+            stats[2 * chunk] = e - b;
+            stats[2 * chunk + 1] = tot_edges;
+        }
+    }
+}
+
+/* The records' merge with min (phases.c:41-70), one thread per run head. */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_merge_min(const mergeaccum_t* __restrict__ mb, uint64_t count,
+                                                           uint64_t* __restrict__ labels)
+{
+    const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= count) return;
+    const mergeaccum_t r = mb[u];
+    if (u > 0 && mb[u - 1].final_vertex_id == r.final_vertex_id) return;
+    uint64_t proposed = (uint64_t)__double_as_longlong(r.final_partial_value);
+    uint64_t j = u + 1;
+    while (j < count && mb[j].final_vertex_id == r.final_vertex_id) {
+        const uint64_t x = (uint64_t)__double_as_longlong(mb[j].final_partial_value);
+        proposed = x < proposed ? x : proposed;
+        ++j;
+    }
+    if (j < count && mb[j].initial_vertex_id == r.final_vertex_id) {
+        const uint64_t x = labels[mb[j].initial_vertex_id];
+        proposed = x < proposed ? x : proposed;
+    }
+    labels[r.final_vertex_id] = proposed;
+}
+
+static void gz_pull_cc_merge(const gz_vec_t* edge_list, uint64_t edge_list_count)
+{
+    const uint64_t nchunks = gz_chunks_for(edge_list_count);
+    if (gz_merge_buffer_count < nchunks) {
+        fprintf(stderr, "grazelle: merge buffer holds %llu units, the pull phase needs %llu\n",
+                (unsigned long long)gz_merge_buffer_count, (unsigned long long)nchunks);
+        exit(72);
+    }
+    const uint64_t chunk_len = gz_chunk_len();
+    const uint64_t blocks = (nchunks + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
+    const uint32_t vpl = gz_vectors_per_lane();
+    uint64_t* labels = (uint64_t*)gz_graph.vertex_props;
+#define GZ_PULL_CC_MERGE_ARGS                                                                                 \
+    dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list, edge_list_count, chunk_len, nchunks,      \
+        labels, (uint64_t*)gz_graph.vertex_accumulators, gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, \
+        gz_merge_buffer, (uint64_t*)gz_scratch.partials, gz_scratch.stats, gz_phase_config.collect_stats
+    gz_ktimer_begin(GZ_K_PULL_CC);
+    if (vpl == 4)
+        hipLaunchKernelGGL(gz_k_edge_pull_cc_merge<4>, GZ_PULL_CC_MERGE_ARGS);
+    else if (vpl == 2)
+        hipLaunchKernelGGL(gz_k_edge_pull_cc_merge<2>, GZ_PULL_CC_MERGE_ARGS);
+    else
+        hipLaunchKernelGGL(gz_k_edge_pull_cc_merge<1>, GZ_PULL_CC_MERGE_ARGS);
+#undef GZ_PULL_CC_MERGE_ARGS
+    GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_PULL_CC);
+    gz_ktimer_begin(GZ_K_MERGE);
+    hipLaunchKernelGGL(gz_k_merge_min, dim3((unsigned)((nchunks + GZ_BLOCK - 1) / GZ_BLOCK)), dim3(GZ_BLOCK), 0,
+                       gz_stream, gz_merge_buffer, nchunks, labels);
+    GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_MERGE);
+}
+
 extern "C" void perform_edge_pull_phase_cc(const gz_vec_t* edge_list, uint64_t edge_list_count)
 {
     if (gz_phase_config.pull_variant == GZ_PULL_SOURCE_SLICED && edge_list_count > 0) {
@@ -114,6 +259,10 @@ extern "C" void perform_edge_pull_phase_cc(const gz_vec_t* edge_list, uint64_t e
     gz_scratch.partials_count = nchunks;
     gz_scratch.stats_count = gz_phase_config.collect_stats ? nchunks : 0;
     if (nchunks == 0) return;
+    if (gz_phase_config.pull_variant == GZ_PULL_MERGE_BUFFER) {
+        gz_pull_cc_merge(edge_list, edge_list_count);
+        return;
+    }
     const uint64_t chunk_len = gz_chunk_len();
     const uint64_t blocks = (nchunks + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;
     const uint32_t vpl = gz_vectors_per_lane();

@@ -110,7 +110,10 @@ extern "C" void perform_edge_pull_phase_pr(const gz_vec_t* edge_list, uint64_t e
     }
     gz_pr_acc_in_slices = 0;
     gz_pr_acc_is_fixed = 0;
-    const bool sa = gz_phase_config.pull_variant == GZ_PULL_SCHED_AWARE;
+    /* PageRank always merges a unit's boundary runs; the merge-buffer variant
+     * is the scheduler-aware pull here. */
+    const bool sa = gz_phase_config.pull_variant == GZ_PULL_SCHED_AWARE ||
+                    gz_phase_config.pull_variant == GZ_PULL_MERGE_BUFFER;
     gz_pr_acc_needs_zero = sa ? 0u : 1u;
     if (sa && gz_merge_buffer_count < nchunks) {
         fprintf(stderr, "grazelle: merge buffer holds %llu units, the pull phase needs %llu\n",

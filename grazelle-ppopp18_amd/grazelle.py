@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "_build", "libgrazelle.so")
 ALG = {"pr": 0, "cc": 1, "bfs": 2}
 ENGINE = {"hybrid": 0, "pull": 1, "push": 2, "learned": 3, "probe": 4}
 STAT = {"both": 0, "outdeg": 1, "count": 2}
-PULL_VARIANT = {"sched_aware": 0, "traditional": 1, "source_sliced": 2}
+PULL_VARIANT = {"sched_aware": 0, "traditional": 1, "source_sliced": 2, "merge_buffer": 3}
 
 if not os.path.exists(LIB_PATH):
     raise ImportError(f"libgrazelle.so is not built ({LIB_PATH}); run __graft_entry__.build()")

@@ -81,10 +81,14 @@ enum gz_stat_mode {                        /* phase_helper_add_frontier_stat, ph
 enum gz_pull_variant {
     GZ_PULL_SCHED_AWARE = 0,               /* merge buffers, plain stores (default) */
     GZ_PULL_TRADITIONAL = 1,               /* EXPERIMENT_EDGE_PULL_WITHOUT_SCHED_AWARE: atomics, no merge */
-    GZ_PULL_SOURCE_SLICED = 2              /* over the source-sliced layout built by
+    GZ_PULL_SOURCE_SLICED = 2,             /* over the source-sliced layout built by
                                             * gz_graph_build_source_slices for this edge list, PR
                                             * (merges inside the phase) and CC (no reference
                                             * counterpart) */
+    GZ_PULL_MERGE_BUFFER = 3               /* CC: EXPERIMENT_EDGE_PULL_FORCE_MERGE -- complete runs lower
+                                            * labels with plain stores, unit-boundary runs go through
+                                            * the merge buffer with min (phase_edge_pull_cc.asm:136-181);
+                                            * the merge runs inside the phase.  PR: as GZ_PULL_SCHED_AWARE */
 };
 
 typedef struct gz_phase_config_t {

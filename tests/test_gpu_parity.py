@@ -131,6 +131,34 @@ def test_cc_pull_vectors_per_lane(gz, orc, device, rmat14, vpl):
     gz.set_vectors_per_lane(1)
 
 
+@pytest.mark.parametrize("case", ["rmat14", "planted_trees", "hub", "star_in", "path"])
+@pytest.mark.parametrize("engine", ["hybrid", "pull"])
+def test_cc_merge_buffer_pull(gz, orc, device, rmat14, case, engine):
+    # EDGE_PULL_FORCE_MERGE: plain stores inside units, the unit-boundary
+    # runs through the merge buffer with min.  Small units put the hub's and
+    # the star centre's runs across many unit boundaries, so the merge sees
+    # runs spanning several records followed by a directly stored head.
+    if case == "rmat14":
+        g = rmat14
+    elif case == "planted_trees":
+        g = graphs.planted_trees(gz)
+    elif case == "hub":
+        g = graphs.hub_graph(gz)
+    else:
+        g = dict(graphs.small_cases(gz))[case]
+    lab, _, _ = orc.cc(_orc_graph(orc, g), orc.options(engine=engine))
+    try:
+        for chunk, vpl in ((64, 1), (512, 2), (4096, 4)):
+            gz.set_chunk_vectors(chunk)
+            gz.set_vectors_per_lane(vpl)
+            gz.load(g)
+            gz.run(gz.options("cc", engine=engine, pull_variant="merge_buffer"))
+            assert np.array_equal(gz.cc_labels(), lab), (case, chunk, vpl)
+    finally:
+        gz.set_chunk_vectors(512)
+        gz.set_vectors_per_lane(1)
+
+
 @pytest.mark.parametrize("name_idx", range(6))
 def test_small_cases_all_algorithms(gz, orc, device, name_idx):
     name, g = graphs.small_cases(gz)[name_idx]

@@ -59,6 +59,7 @@ def exe(gz):
     (["-L", "x", "-i", "g"], 2), (["-L", "0", "-i", "g"], 2), (["-L", "24", "-i", "g"], 2),
     (["-L", "20,4", "-i", "g"], 2), (["-L", "20,2,6", "-i", "g"], 2), (["-L", "20,", "-i", "g"], 2),
     (["-L", "20,1,1,1", "-i", "g"], 2), (["-L", "-i", "g"], 3), (["-L", "20", "-T", "-i", "g"], 6),
+    (["-L", "20", "-M", "-i", "g"], 6),
 ])
 def test_cli_error_codes(gz, args, code):
     r = subprocess.run([exe(gz)] + args, capture_output=True, text=True, timeout=60)
