@@ -362,6 +362,7 @@ def batch_1024(gz, cpu: int) -> dict:
     gz.run(gz.options("cc"))
     cc_runs = [gz.run(gz.options("cc")) for _ in range(2)]
     cc_b = min(r.elapsed_ms for r in cc_runs)
+    host_labels = gz.cc_labels()
     # the same batch with the engine chosen on the device and the iterations
     # replayed from a captured graph (eager until its allocations settle,
     # captured the second time in a row, then replayed)
@@ -371,6 +372,8 @@ def batch_1024(gz, cpu: int) -> dict:
     c0, l0 = gz.graph_replay_counts()
     cap_runs = [gz.run(gz.options("cc")) for _ in range(3)]
     c1, l1 = gz.graph_replay_counts()
+    cap_equal = bool(np.array_equal(gz.cc_labels(), host_labels))
+    del host_labels
     gz.cc_graph_replay(False)
     cc_cap = min(r.elapsed_ms for r in cap_runs)
     pr_s = cc_s = 0.0
@@ -386,7 +389,8 @@ def batch_1024(gz, cpu: int) -> dict:
                    "iterations": int(cc_runs[-1].iterations),
                    "captured_device_loop": {"batched_ms": round(cc_cap, 3), "speedup": round(cc_s / cc_cap, 2),
                                             "iterations": int(cap_runs[-1].iterations),
-                                            "graph_captures": int(c1 - c0), "graph_launches": int(l1 - l0)}}}
+                                            "graph_captures": int(c1 - c0), "graph_launches": int(l1 - l0),
+                                            "labels_equal_host_loop": cap_equal}}}
 
 
 def pr_rmat16(gz, cpu: int) -> dict:

@@ -426,6 +426,9 @@ static void gz_to_original_order(uint64_t* out, const uint64_t* in)
 static void gz_graph_finish(void)
 {
     const uint64_t num_vertices = G.num_vertices;
+    /* the compaction's per-vertex vector counts are keyed by the offsets'
+     * address; a new graph may reuse an address, so they are rebuilt */
+    gz_frontier_counts_release();
     const size_t vbytes = (num_vertices + 8) * sizeof(uint64_t);
     const size_t fwords = (num_vertices + 63) / 64 + 8;
     G.d_props = gz_device_malloc(vbytes);

@@ -247,7 +247,8 @@ extern "C" void gz_frontier_compact_narrow(int on) { gz_compact_narrow = on ? 1 
 
 /* 32-bit vector counts of the two lists' vertices, built on first use for
  * an offsets array, vertex count and list length, and kept until the graph
- * is released (gz_graph_release calls gz_frontier_counts_release). */
+ * is released or another is published (gz_graph_release and every load
+ * path's finish call gz_frontier_counts_release). */
 static struct {
     const uint64_t* offsets;
     uint64_t num_vertices, vector_count;

@@ -163,8 +163,9 @@ void gz_launch_pr_pull_explicit(const gz_vec_t* vecs, uint64_t nvec, uint64_t ch
  * is associative, so every order gives the same bits: the exact sum of the
  * FP64 messages (rank / outdeg' >= 0.15 / V^2, at least 2^-67 for V <= 2^32,
  * so no message loses a bit; sums stay below 2^8, since the messages of all
- * edges add up to at most 1).  The vertex phase rounds it to FP64 once and
- * clears it. */
+ * edges add up to at most 1).  The vertex phase converts it to FP64 (two
+ * conversions and one add: within two units in the last place of the exact
+ * sum) and clears it. */
 #define GZ_PR_FIXED_BITS 120
 
 /* x >= 0 as a 128-bit fixed-point integer (lo, hi), truncating below 2^-120. */
