@@ -394,12 +394,27 @@ def batch_1024(gz, cpu: int) -> dict:
 
 
 def pr_rmat16(gz, cpu: int) -> dict:
+    # 257,559 pull vectors: chunks of 256 give about 1000 wavefronts (512
+    # leaves most CUs with two); the merge, the sum of S and the vertex phase
+    # run as one launch after each pull (the fused tail)
+    chunk = 256
     gz.load_workload_device("pr-rmat16")
-    gz.run(gz.options("pr", iterations=20, damping=DAMPING))
-    ms = min(gz.run(gz.options("pr", iterations=20, damping=DAMPING)).elapsed_ms for _ in range(5))
+    gz.set_chunk_vectors(chunk)
+    try:
+        opts = gz.options("pr", iterations=20, damping=DAMPING)
+        gz.run(opts)
+        ms = min(gz.run(opts).elapsed_ms for _ in range(5))
+        fused = bool(gz.lib.gz_pr_fused_tail_applies())
+        gz.pr_fused_tail(False)
+        gz.run(opts)
+        ms_four = min(gz.run(opts).elapsed_ms for _ in range(5))
+        gz.pr_fused_tail(True)
+    finally:
+        gz.set_chunk_vectors(512)
     E = gz.num_edges()
-    out = {"workload": "pr-rmat16", "V": gz.num_vertices(), "E": E, "iterations": 20, "ms": round(ms, 4),
-           "gteps_per_iteration": round(E * 20 / (ms * 1e-3) / 1e9, 2)}
+    out = {"workload": "pr-rmat16", "V": gz.num_vertices(), "E": E, "iterations": 20, "chunk_vectors": chunk,
+           "fused_tail": fused, "ms": round(ms, 4), "gteps_per_iteration": round(E * 20 / (ms * 1e-3) / 1e9, 2),
+           "four_launches_per_iteration_ms": round(ms_four, 4)}
     if cpu:
         orc = _oracle()
         g = gz.workload_graph("pr-rmat16")

@@ -242,6 +242,7 @@ void gz_pr_graph_replay(int on) { gz_pr_replay_on = on ? 1 : 0; }
 typedef struct gz_pr_iteration_t {
     int use_push;
     uint32_t pull_variant;
+    int fused_tail; /* merge, sum and vertex phase in one launch (gz_pr_vertex_merged) */
 } gz_pr_iteration_t;
 
 /* One PageRank iteration's launches, as the loop below makes them. */
@@ -249,6 +250,11 @@ static void gz_pr_enqueue_iteration(void* p)
 {
     const gz_pr_iteration_t* c = (const gz_pr_iteration_t*)p;
     phase_op_reset_global_accum();
+    if (c->fused_tail) {
+        perform_edge_pull_phase_pr(gz_graph.pull_vectors, gz_graph.pull_vector_count);
+        gz_pr_vertex_merged(gz_reduce_buffer);
+        return;
+    }
     if (!c->use_push) {
         perform_edge_pull_phase_pr(gz_graph.pull_vectors, gz_graph.pull_vector_count);
         if (c->pull_variant == GZ_PULL_SCHED_AWARE || c->pull_variant == GZ_PULL_MERGE_BUFFER)
@@ -273,6 +279,7 @@ static size_t gz_pr_iteration_key(const gz_pr_iteration_t* c, uint64_t* k)
     gz_slices_settings(sl);
     k[n++] = (uint64_t)c->use_push;
     k[n++] = c->pull_variant;
+    k[n++] = (uint64_t)c->fused_tail;
     k[n++] = damping;
     k[n++] = gz_phase_config.stat_mode;
     k[n++] = gz_phase_config.pull_variant;
@@ -309,10 +316,28 @@ static void gz_run_pr(const gz_run_options_t* o, gz_run_result_t* r)
      * the phases need); a configuration is captured when it runs again */
     const int replay = gz_pr_replay_on && t == NULL && !o->collect_stats && !o->edge_only && !o->vertex_only &&
                        !gz_kernel_timing_enabled() && o->iterations > 1 && o->iterations - 1 <= 0xffffffffull;
+    const int plain = t == NULL && !use_push && !o->collect_stats && !o->edge_only && !o->vertex_only &&
+                      !gz_kernel_timing_enabled() &&
+                      (o->pull_variant == GZ_PULL_SCHED_AWARE || o->pull_variant == GZ_PULL_MERGE_BUFFER);
+    /* with few enough chunks, the merge, the sum of S and the vertex phase
+     * run as one launch (gz_pr_vertex_merged) */
+    const int fused_tail = plain && gz_pr_fused_tail_applies();
+    /* every iteration in one launch (gz_pr_run_persistent), when switched on */
+    if (plain) {
+        const int one = gz_pr_run_persistent(o->iterations, gz_reduce_buffer);
+        if (one > 0) {
+            r->pull_iterations = o->iterations;
+            r->iterations = o->iterations;
+            return;
+        }
+        if (one < 0) /* a barrier gave up mid-run: start again from the initial ranks */
+            gz_dev_init_pr((double*)gz_graph.vertex_props, (double*)gz_graph.vertex_accumulators,
+                           gz_graph.vertex_outdegrees, V);
+    }
     uint64_t ctr;
     for (ctr = 0; ctr < o->iterations; ++ctr) {
         if (replay && ctr == 1) {
-            gz_pr_iteration_t c = {use_push, o->pull_variant};
+            gz_pr_iteration_t c = {use_push, o->pull_variant, fused_tail};
             uint64_t key[32];
             const size_t n = gz_pr_iteration_key(&c, key);
             const uint64_t rest = o->iterations - 1;
@@ -324,6 +349,14 @@ static void gz_run_pr(const gz_run_options_t* o, gz_run_result_t* r)
                 ctr = o->iterations;
                 break;
             }
+        }
+        if (fused_tail) {
+            phase_op_reset_global_accum();
+            perform_edge_pull_phase_pr(gz_graph.pull_vectors, gz_graph.pull_vector_count);
+            if (ctr == 0) gz_pr_run_heads_build(); /* the first pull's records fix the runs */
+            gz_pr_vertex_merged(gz_reduce_buffer);
+            r->pull_iterations += 1;
+            continue;
         }
         gz_iteration_record_t rec;
         memset(&rec, 0, sizeof(rec));

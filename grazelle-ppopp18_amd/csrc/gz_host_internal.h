@@ -107,6 +107,21 @@ void gz_frontier_counts_release(void);
 /* The PageRank push's fixed-point accumulators: freed with the graph, and
  * cleared when a PageRank run starts. */
 void gz_pr_release(void);
+/* Every PageRank iteration of the loaded graph in one cooperative launch
+ * (the scheduler-aware pull, merge, sum and vertex phase); returns 1 when it
+ * ran, 0 when it is switched off, does not apply or cannot launch
+ * (gz_pr_persistent), -1 when a grid barrier gave up (the vertex arrays
+ * then hold a partial run). */
+int gz_pr_run_persistent(uint64_t iterations, uint64_t* reduce_buffer);
+/* The merge, the sum of S and the PageRank vertex phase in one launch after
+ * a scheduler-aware pull of the loaded graph's pull list: applies when it
+ * has at most 4096 chunks (gz_pr_fused_tail_applies); the per-vertex run
+ * heads are built once per run from the first pull's records
+ * (gz_pr_run_heads_build); the result is bit-identical to the merge, reduce
+ * and vertex launches. */
+int gz_pr_fused_tail_applies(void);
+void gz_pr_run_heads_build(void);
+void gz_pr_vertex_merged(uint64_t* reduce_buffer);
 void gz_pr_clear_fixed(void);
 /* The source-sliced pull's settings: hot cache, cold spread, cold stream. */
 void gz_slices_settings(uint32_t* out3);

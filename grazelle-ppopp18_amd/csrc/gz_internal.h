@@ -213,6 +213,9 @@ void gz_reduce_partials(uint64_t* dst);
  * 1024 blocks write slots[], the last one sums them and clears *done, which
  * must be 0 when the launch starts. */
 void gz_reduce_f64_into(const double* in, uint64_t count, double* slots, unsigned int* done, uint64_t* out);
+/* Workgroups of the first stage of that reduction for `count` inputs; each
+ * sums ceil(count / blocks) consecutive inputs (at most 1024 workgroups). */
+uint64_t gz_reduce_blocks(uint64_t count);
 /* PageRank kernels over explicit arrays (gz_pr.hip). */
 void gz_launch_pr_pull_explicit(const gz_vec_t* vecs, uint64_t nvec, uint64_t chunk_len, uint64_t nchunks,
                                 const double* props, double* acc, mergeaccum_t* merge, double* partials);
@@ -289,6 +292,48 @@ __device__ __forceinline__ double gz_wave_sum(double x)
 #pragma unroll
     for (int o = GZ_WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, GZ_WAVE);
     return x;
+}
+
+/* Sum over a workgroup: a butterfly inside each wavefront, then thread 0
+ * adds the wavefront sums in wavefront order (the result is returned to
+ * thread 0 only).  Every thread of the workgroup must call it. */
+template <typename T>
+__device__ __forceinline__ T gz_block_sum(T x, T* lds)
+{
+    for (int o = GZ_WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, GZ_WAVE);
+    const int w = threadIdx.x / GZ_WAVE;
+    if ((threadIdx.x & (GZ_WAVE - 1)) == 0) lds[w] = x;
+    __syncthreads();
+    T r = 0;
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < (int)(blockDim.x / GZ_WAVE); ++i) r += lds[i];
+    }
+    __syncthreads();
+    return r;
+}
+
+/* One record of the parallel merge of phases.c:41-70.  Records sharing a
+ * final vertex form a run; the first record of each run (its head) folds the
+ * run's partials in record order and, when the record after the run starts
+ * on the same vertex (that unit stored its first vertex directly), folds the
+ * accumulator too.  A record whose initial id has bit 63 set is unused. */
+__device__ __forceinline__ void gz_merge_sum_record(const mergeaccum_t* __restrict__ mb, uint64_t count, uint64_t u,
+                                                    double* __restrict__ acc)
+{
+    const mergeaccum_t r = mb[u];
+    if (r.initial_vertex_id & GZ_LANE_VALID_BIT) return;
+    if (u > 0) {
+        const mergeaccum_t p = mb[u - 1];
+        if (!(p.initial_vertex_id & GZ_LANE_VALID_BIT) && p.final_vertex_id == r.final_vertex_id) return;
+    }
+    double proposed = r.final_partial_value;
+    uint64_t j = u + 1;
+    while (j < count && mb[j].final_vertex_id == r.final_vertex_id) {
+        proposed = proposed + mb[j].final_partial_value;
+        ++j;
+    }
+    if (j < count && mb[j].initial_vertex_id == r.final_vertex_id) proposed = proposed + acc[mb[j].initial_vertex_id];
+    acc[r.final_vertex_id] = proposed;
 }
 
 /* Position of the k-th (0-based) set bit of x; k < popcount(x). */

@@ -32,19 +32,20 @@ static uint32_t gz_pr_acc_needs_zero = 0;
 static uint32_t gz_pr_acc_in_slices = 0;
 /* Set by the push: the vertex phase reads its fixed-point accumulators. */
 static uint32_t gz_pr_acc_is_fixed = 0;
+/* The one-launch PageRank's per-vertex run heads (gz_k_pr_persistent). */
+static uint32_t* gz_pr_run_head = nullptr;
+static uint64_t gz_pr_run_head_cap = 0;
 
+/* One wavefront's pull over chunk `chunk`: complete runs stored, the last
+ * run to the chunk's merge record (or added atomically), the chunk's share
+ * of S to partials[chunk].  Returns the valid lanes the lane examined. */
 template <bool kSchedAware, int VPL>
-__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr(const gz_vec_t* __restrict__ vecs, uint64_t nvec,
-                                                              uint64_t chunk_len, uint64_t nchunks,
-                                                              const double* __restrict__ props,
-                                                              double* __restrict__ acc,
-                                                              mergeaccum_t* __restrict__ merge,
-                                                              double* __restrict__ partials,
-                                                              uint64_t* __restrict__ stats, uint32_t collect)
+__device__ __forceinline__ uint32_t gz_pr_pull_chunk(const gz_vec_t* __restrict__ vecs, uint64_t nvec,
+                                                     uint64_t chunk_len, uint64_t chunk,
+                                                     const double* __restrict__ props, double* __restrict__ acc,
+                                                     mergeaccum_t* __restrict__ merge, double* __restrict__ partials)
 {
     const int lane = gz_lane();
-    const uint64_t chunk = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
-    if (chunk >= nchunks) return;
     const uint64_t b = chunk * chunk_len;
     const uint64_t e = (b + chunk_len < nvec) ? b + chunk_len : nvec;
     gz_pr_pull_op op{props, acc, kSchedAware, 0.0};
@@ -66,9 +67,27 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr(const gz_vec_t* __
         }
         partials[chunk] = S;
     }
+    return edges;
+}
+
+template <bool kSchedAware, int VPL>
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_pr(const gz_vec_t* __restrict__ vecs, uint64_t nvec,
+                                                              uint64_t chunk_len, uint64_t nchunks,
+                                                              const double* __restrict__ props,
+                                                              double* __restrict__ acc,
+                                                              mergeaccum_t* __restrict__ merge,
+                                                              double* __restrict__ partials,
+                                                              uint64_t* __restrict__ stats, uint32_t collect)
+{
+    const uint64_t chunk = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
+    if (chunk >= nchunks) return;
+    const uint32_t edges =
+        gz_pr_pull_chunk<kSchedAware, VPL>(vecs, nvec, chunk_len, chunk, props, acc, merge, partials);
     if (collect) {
+        const uint64_t b = chunk * chunk_len;
+        const uint64_t e = (b + chunk_len < nvec) ? b + chunk_len : nvec;
         const uint64_t tot_edges = gz_wave_sum_u64(edges);
-        if (lane == 0) {
+        if (gz_lane() == 0) {
             stats[2 * chunk] = e - b;
             stats[2 * chunk + 1] = tot_edges;
         }
@@ -210,6 +229,9 @@ extern "C" void gz_pr_release(void)
     gz_pr_fixed_acc = nullptr;
     gz_pr_fixed_vertices = 0;
     gz_pr_acc_is_fixed = 0;
+    gz_device_free(gz_pr_run_head);
+    gz_pr_run_head = nullptr;
+    gz_pr_run_head_cap = 0;
 }
 
 /* A zeroed fixed-point accumulator pair per vertex of the loaded graph. */
@@ -321,6 +343,418 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr(double* __restrict__ 
         if (zero_acc) acc[v] = 0.0;
         props[v] = gz_pr_vertex_value(a, outdeg[v], c, k, damping, num_vertices);
     }
+}
+
+/* ---- every iteration of a small graph in one launch -------------------------
+ * On a graph whose iteration takes a few tens of microseconds (R-MAT s16:
+ * about 21 us over four launches), the kernels' own start and drain cost
+ * more than their work.  gz_k_pr_persistent runs all iterations in one
+ * cooperative launch (every workgroup resident), with two grid barriers per
+ * iteration:
+ *   pull        the chunks of gz_k_edge_pull_pr<true>, strided over the grid
+ *   S, vertex   every workgroup sums the chunks' partials itself in the two
+ *               stages' order of gz_k_reduce (up to 8192 chunks; beyond, the
+ *               first stage's slices are spread over the workgroups and a
+ *               third barrier precedes the second stage), then the vertices
+ *               of gz_k_vertex_pr, where a vertex that ends a run of merge
+ *               records first adds them as gz_merge_sum_record does
+ * The same arithmetic in the same order on the same data, so its results are
+ * bit-identical to the launches it replaces.
+ *
+ * The barrier: every wavefront waits until its stores have reached the L2;
+ * the workgroup's thread 0 then releases at agent scope (the XCDs' L2 caches
+ * are not coherent with each other, so the L2 is written back), arrives on
+ * one of eight group counters (the last of a group on the global one), polls
+ * the generation word with relaxed coherent loads until it moves, and
+ * acquires at agent scope (L1 and L2 invalidated) before the workgroup goes
+ * on.  A wait gives up after `spin_limit`
+ * polls: it raises the abort word, which every waiting workgroup sees, so
+ * every workgroup leaves the kernel; the host reports it and the driver
+ * restarts the run with the per-phase launches. */
+#define GZ_BAR_GROUPS 8
+struct gz_grid_bar_t {
+    uint32_t group[GZ_BAR_GROUPS][16]; /* arrivals per group, one 64-byte line each */
+    uint32_t count;                    /* groups arrived */
+    uint32_t gen;
+    uint32_t abort;
+    uint32_t pad[13];
+};
+
+/* Workgroup w arrives on group w % 8; the last of a group arrives on the
+ * global count, whose last arrival moves the generation word. */
+__device__ __forceinline__ bool gz_grid_sync(gz_grid_bar_t* bar, uint32_t spin_limit)
+{
+    __shared__ uint32_t s_ok;
+    /* every wavefront's stores have reached the XCD's L2 */
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ok = 1;
+        const uint32_t grp = blockIdx.x % GZ_BAR_GROUPS;
+        const uint32_t members = gridDim.x / GZ_BAR_GROUPS + (grp < gridDim.x % GZ_BAR_GROUPS ? 1u : 0u);
+        const uint32_t groups = gridDim.x < GZ_BAR_GROUPS ? gridDim.x : GZ_BAR_GROUPS;
+        const uint32_t g = __hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        /* a release arrival: the XCD's L2 is written back first, so the other
+         * XCDs read what this workgroup wrote */
+        const uint32_t a =
+            __hip_atomic_fetch_add(&bar->group[grp][0], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+        bool last = false;
+        if (a == members) {
+            __hip_atomic_store(&bar->group[grp][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t c =
+                __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+            if (c == groups) {
+                last = true;
+                __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&bar->gen, g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        }
+        if (!last) {
+            /* relaxed polls read the coherent word without touching the
+             * caches; the acquire below follows the last one */
+            uint32_t polls = 0;
+            while (__hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == g) {
+                if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u ||
+                    ++polls > spin_limit) {
+                    __hip_atomic_store(&bar->abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    ok = 0;
+                    break;
+                }
+            }
+        }
+        if (__hip_atomic_load(&bar->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) ok = 0;
+        /* acquire: the CU's L1 and the XCD's L2 drop what other XCDs may
+         * have rewritten; the wait orders it before the workgroup's loads */
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __builtin_amdgcn_s_waitcnt(0);
+        s_ok = ok;
+    }
+    __syncthreads();
+    return s_ok != 0;
+}
+
+/* S from the chunks' partials in the order of gz_k_reduce: slice b of the
+ * first stage is ceil(n / blocks) consecutive partials summed by 256 threads
+ * with stride 256 and gz_block_sum, the second stage the same over the slice
+ * sums.  Every thread of the workgroup calls it; all get the result. */
+__device__ __forceinline__ double gz_pr_sum_partials(const double* partials, uint64_t n, uint32_t blocks,
+                                                     double* lds, double* slot_lds, double* total_lds)
+{
+    const uint64_t per = (n + blocks - 1) / blocks;
+    for (uint32_t b = 0; b < blocks; ++b) {
+        const uint64_t lo = (uint64_t)b * per;
+        const uint64_t hi = (lo + per < n) ? lo + per : n;
+        double x = 0.0;
+        for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) x += partials[i];
+        const double r = gz_block_sum<double>(x, lds);
+        if (threadIdx.x == 0) slot_lds[b] = r;
+    }
+    __syncthreads();
+    double y = 0.0;
+    for (uint32_t i = threadIdx.x; i < blocks; i += blockDim.x) y += slot_lds[i];
+    const double total = gz_block_sum<double>(y, lds);
+    if (threadIdx.x == 0) *total_lds = total;
+    __syncthreads();
+    return *total_lds;
+}
+
+/* The chunks whose partials every workgroup sums itself (S in LDS, no grid
+ * barrier for the first stage's slice sums). */
+#define GZ_PR_ONE_LAUNCH_LOCAL_SUM_CHUNKS 8192u
+
+/* acc[v] after the merge of the pull's records (gz_merge_sum_record), for a
+ * vertex phase that merges as it goes: a vertex that ends a run of records
+ * (run_head[v] = the run's first record + 1) adds the run's partials in
+ * record order, then the accumulator the next unit stored directly when it
+ * starts on v, and keeps the result in acc[v]; any other vertex reads
+ * acc[v].  The same additions in the same order as the merge kernel. */
+__device__ __forceinline__ double gz_pr_merged_acc(uint64_t v, const uint32_t* __restrict__ run_head,
+                                                   const mergeaccum_t* __restrict__ merge, uint64_t count,
+                                                   double* __restrict__ acc)
+{
+    const uint32_t h = run_head[v];
+    if (h == 0u) return acc[v];
+    uint64_t j = h - 1u;
+    double a = merge[j].final_partial_value;
+    ++j;
+    while (j < count && merge[j].final_vertex_id == v) {
+        a = a + merge[j].final_partial_value;
+        ++j;
+    }
+    if (j < count && merge[j].initial_vertex_id == v) a = a + acc[v];
+    acc[v] = a;
+    return a;
+}
+
+/* run_head[final vertex of a run of records] = first record of the run + 1
+ * (the array is cleared first).  The chunking and the graph fix the records'
+ * vertex ids, so one pull's records serve every later iteration. */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_pr_run_heads(const mergeaccum_t* __restrict__ merge, uint64_t count,
+                                                              uint32_t* __restrict__ run_head)
+{
+    const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= count) return;
+    const uint64_t f = merge[u].final_vertex_id;
+    if (u == 0 || merge[u - 1].final_vertex_id != f) run_head[f] = (uint32_t)(u + 1);
+}
+
+/* The merge, the sum of S and the vertex phase in one launch, after a
+ * scheduler-aware pull of at most GZ_PR_FUSED_TAIL_CHUNKS chunks: every
+ * workgroup sums the chunks' partials itself in gz_k_reduce's order, then
+ * its vertices merge as they go (gz_pr_merged_acc).  Bit-identical to
+ * gz_k_merge_sum + gz_k_reduce + gz_k_vertex_pr. */
+#define GZ_PR_FUSED_TAIL_CHUNKS 4096u
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr_merged(double* __restrict__ props, double* __restrict__ acc,
+                                                                  const double* __restrict__ outdeg,
+                                                                  uint64_t num_vertices,
+                                                                  const mergeaccum_t* __restrict__ merge,
+                                                                  uint64_t nchunks, const double* __restrict__ partials,
+                                                                  uint32_t reduce_blocks,
+                                                                  const uint32_t* __restrict__ run_head,
+                                                                  uint64_t* __restrict__ reduce_buffer, double damping)
+{
+    __shared__ double lds[GZ_WAVES_PER_BLOCK];
+    __shared__ double slot_lds[2];
+    __shared__ double s_total;
+    const double S = gz_pr_sum_partials(partials, nchunks, reduce_blocks, lds, slot_lds, &s_total);
+    if (blockIdx.x == 0 && threadIdx.x == 0) reduce_buffer[0] = (uint64_t)__double_as_longlong(S);
+    const double nv = (double)num_vertices;
+    const double c = (1.0 - S) / nv;
+    const double k = (1.0 - damping) / nv;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < num_vertices;
+         v += (uint64_t)gridDim.x * blockDim.x)
+        props[v] = gz_pr_vertex_value(gz_pr_merged_acc(v, run_head, merge, nchunks, acc), outdeg[v], c, k, damping,
+                                      nv);
+}
+
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_pr_clear_u32(uint32_t* __restrict__ p, uint64_t n)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+static int gz_pr_fused_tail_on = 1;
+
+extern "C" void gz_pr_fused_tail(int on) { gz_pr_fused_tail_on = on ? 1 : 0; }
+
+static uint32_t* gz_pr_run_heads_reserve(uint64_t V)
+{
+    if (gz_pr_run_head_cap < V) {
+        gz_device_free(gz_pr_run_head);
+        gz_pr_run_head = (uint32_t*)gz_device_malloc(V * sizeof(uint32_t));
+        gz_pr_run_head_cap = V;
+    }
+    return gz_pr_run_head;
+}
+
+extern "C" int gz_pr_fused_tail_applies(void)
+{
+    const uint64_t nvec = gz_graph.pull_vector_count;
+    if (!gz_pr_fused_tail_on || gz_graph.pull_vectors == nullptr || nvec == 0 || gz_graph.num_vertices == 0 ||
+        gz_graph.num_vertices >= 0xffffffffull)
+        return 0;
+    const uint64_t nchunks = gz_chunks_for(nvec);
+    return nchunks <= GZ_PR_FUSED_TAIL_CHUNKS && gz_merge_buffer_count >= nchunks;
+}
+
+extern "C" void gz_pr_run_heads_build(void)
+{
+    const uint64_t V = gz_graph.num_vertices, nchunks = gz_chunks_for(gz_graph.pull_vector_count);
+    uint32_t* rh = gz_pr_run_heads_reserve(V);
+    uint64_t cb = (V + GZ_BLOCK - 1) / GZ_BLOCK;
+    if (cb > 4096) cb = 4096;
+    hipLaunchKernelGGL(gz_k_pr_clear_u32, dim3((unsigned)cb), dim3(GZ_BLOCK), 0, gz_stream, rh, V);
+    GZ_LAUNCH_CHECK();
+    hipLaunchKernelGGL(gz_k_pr_run_heads, dim3((unsigned)((nchunks + GZ_BLOCK - 1) / GZ_BLOCK)), dim3(GZ_BLOCK), 0,
+                       gz_stream, gz_merge_buffer, nchunks, rh);
+    GZ_LAUNCH_CHECK();
+}
+
+extern "C" void gz_pr_vertex_merged(uint64_t* reduce_buffer)
+{
+    const uint64_t V = gz_graph.num_vertices, nchunks = gz_chunks_for(gz_graph.pull_vector_count);
+    uint64_t blocks = (V + GZ_BLOCK - 1) / GZ_BLOCK;
+    if (blocks > 1024) blocks = 1024;
+    gz_ktimer_begin(GZ_K_VERTEX_PR);
+    hipLaunchKernelGGL(gz_k_vertex_pr_merged, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
+                       (double*)gz_graph.vertex_props, (double*)gz_graph.vertex_accumulators,
+                       gz_graph.vertex_outdegrees, V, gz_merge_buffer, nchunks, (const double*)gz_scratch.partials,
+                       (uint32_t)gz_reduce_blocks(nchunks), gz_pr_run_head, reduce_buffer, gz_phase_config.damping);
+    GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_VERTEX_PR);
+}
+
+template <int VPL>
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_pr_persistent(
+    const gz_vec_t* __restrict__ vecs, uint64_t nvec, uint64_t chunk_len, uint64_t nchunks, double* props,
+    double* acc, const double* __restrict__ outdeg, uint64_t num_vertices, mergeaccum_t* merge, double* partials,
+    double* slots, uint32_t reduce_blocks, uint32_t* run_head, uint64_t* __restrict__ reduce_buffer,
+    double damping, uint64_t iterations, gz_grid_bar_t* bar, uint32_t spin_limit)
+{
+    __shared__ double lds[GZ_WAVES_PER_BLOCK];
+    __shared__ double slot_lds[2];
+    __shared__ double s_total;
+    const uint64_t waves = (uint64_t)gridDim.x * GZ_WAVES_PER_BLOCK;
+    const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
+    const uint64_t threads = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t gt = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const double nv = (double)num_vertices;
+    const double k = (1.0 - damping) / nv;
+    const bool local_sum = nchunks <= GZ_PR_ONE_LAUNCH_LOCAL_SUM_CHUNKS;
+    const uint64_t per = (nchunks + reduce_blocks - 1) / reduce_blocks;
+    for (uint64_t it = 0; it < iterations; ++it) {
+        for (uint64_t c = gw; c < nchunks; c += waves)
+            gz_pr_pull_chunk<true, VPL>(vecs, nvec, chunk_len, c, props, acc, merge, partials);
+        if (!gz_grid_sync(bar, spin_limit)) return;
+        if (it == 0) {
+            /* run_head[final vertex of a run of records] = first record of
+             * the run + 1 (zero elsewhere, cleared before the launch); the
+             * chunking and the graph fix the records' ids for every
+             * iteration */
+            for (uint64_t u = gt; u < nchunks; u += threads) {
+                const uint64_t f = merge[u].final_vertex_id;
+                if (u == 0 || merge[u - 1].final_vertex_id != f) run_head[f] = (uint32_t)(u + 1);
+            }
+            if (!gz_grid_sync(bar, spin_limit)) return;
+        }
+        double S;
+        if (local_sum) {
+            /* at most two slices: every workgroup sums all partials itself */
+            S = gz_pr_sum_partials(partials, nchunks, reduce_blocks, lds, slot_lds, &s_total);
+        } else {
+            for (uint32_t b = blockIdx.x; b < reduce_blocks; b += gridDim.x) {
+                const uint64_t lo = (uint64_t)b * per;
+                const uint64_t hi = (lo + per < nchunks) ? lo + per : nchunks;
+                double x = 0.0;
+                for (uint64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) x += partials[i];
+                const double r = gz_block_sum<double>(x, lds);
+                if (threadIdx.x == 0) slots[b] = r;
+            }
+            if (!gz_grid_sync(bar, spin_limit)) return;
+            double y = 0.0;
+            for (uint64_t i = threadIdx.x; i < reduce_blocks; i += blockDim.x) y += slots[i];
+            const double total = gz_block_sum<double>(y, lds);
+            if (threadIdx.x == 0) s_total = total;
+            __syncthreads();
+            S = s_total;
+        }
+        if (gt == 0) reduce_buffer[0] = (uint64_t)__double_as_longlong(S);
+        const double c = (1.0 - S) / nv;
+        /* the vertex phase with the merge folded in: a vertex that ends a
+         * chunk's last run (run_head[v] = first record of its run + 1) adds
+         * the run's records as gz_merge_sum_record does */
+        for (uint64_t v = gt; v < num_vertices; v += threads)
+            props[v] = gz_pr_vertex_value(gz_pr_merged_acc(v, run_head, merge, nchunks, acc), outdeg[v], c, k,
+                                          damping, nv);
+        if (!gz_grid_sync(bar, spin_limit)) return;
+    }
+}
+
+/* 0: off (the default: see gz_pr_persistent); 1: graphs whose per-iteration
+ * bytes fit 128 MiB; 2: any graph. */
+static int gz_pr_persistent_mode = 0;
+static uint32_t gz_pr_persistent_per_cu = 2;
+static uint32_t gz_pr_persistent_spin = 1u << 22;
+static uint64_t gz_pr_persistent_count = 0;
+static uint64_t gz_pr_persistent_aborts = 0;
+static int gz_pr_persistent_unavailable = 0;
+static gz_grid_bar_t* gz_pr_bar = nullptr;
+static uint32_t* gz_pr_bar_host = nullptr;
+
+extern "C" int gz_pr_persistent(int mode, uint32_t workgroups_per_cu)
+{
+    if (mode < 0 || mode > 2 || workgroups_per_cu < 1 || workgroups_per_cu > 8) return -1;
+    gz_pr_persistent_mode = mode;
+    gz_pr_persistent_per_cu = workgroups_per_cu;
+    return 0;
+}
+
+extern "C" uint64_t gz_pr_persistent_runs(void) { return gz_pr_persistent_count; }
+
+extern "C" uint64_t gz_pr_persistent_gave_up(void) { return gz_pr_persistent_aborts; }
+
+extern "C" void gz_pr_persistent_spin_limit(uint32_t polls) { gz_pr_persistent_spin = polls; }
+
+extern "C" int gz_pr_run_persistent(uint64_t iterations, uint64_t* reduce_buffer)
+{
+    const uint64_t nvec = gz_graph.pull_vector_count, V = gz_graph.num_vertices;
+    if (gz_pr_persistent_mode == 0 || gz_pr_persistent_unavailable || iterations == 0 ||
+        gz_graph.pull_vectors == nullptr || nvec == 0 || V == 0)
+        return 0;
+    if (gz_pr_persistent_mode == 1 && 32 * nvec + 24 * V > (128ull << 20)) return 0;
+    int dev = 0, coop = 0, cus = 0;
+    GZ_CHECK(hipGetDevice(&dev));
+    GZ_CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    GZ_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    if (!coop || cus <= 0) return 0;
+    const uint32_t vpl = gz_vectors_per_lane();
+    const void* fn = vpl == 4 ? (const void*)gz_k_pr_persistent<4>
+                   : vpl == 2 ? (const void*)gz_k_pr_persistent<2>
+                              : (const void*)gz_k_pr_persistent<1>;
+    int resident = 0;
+    GZ_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, fn, GZ_BLOCK, 0));
+    if (resident <= 0) return 0;
+    const uint32_t per_cu = (uint32_t)resident < gz_pr_persistent_per_cu ? (uint32_t)resident : gz_pr_persistent_per_cu;
+    const uint32_t grid = per_cu * (uint32_t)cus;
+
+    const uint64_t nchunks = gz_chunks_for(nvec);
+    if (nchunks >= 0xffffffffull) return 0; /* run heads are 32-bit */
+    if (gz_merge_buffer_count < nchunks) {
+        fprintf(stderr, "grazelle: merge buffer holds %llu units, the pull phase needs %llu\n",
+                (unsigned long long)gz_merge_buffer_count, (unsigned long long)nchunks);
+        exit(72);
+    }
+    gz_scratch_reserve(nchunks, V);
+    gz_scratch.partials_is_double = 1;
+    gz_scratch.partials_count = nchunks;
+    gz_scratch.stats_count = 0;
+    gz_pr_acc_needs_zero = 0;
+    gz_pr_acc_in_slices = 0;
+    gz_pr_acc_is_fixed = 0;
+    if (gz_pr_bar == nullptr) {
+        gz_pr_bar = (gz_grid_bar_t*)gz_device_malloc(sizeof(gz_grid_bar_t));
+        GZ_CHECK(hipHostMalloc((void**)&gz_pr_bar_host, sizeof(uint32_t)));
+    }
+    GZ_CHECK(hipMemsetAsync(gz_pr_bar, 0, sizeof(gz_grid_bar_t), gz_stream));
+    GZ_CHECK(hipMemsetAsync(gz_pr_run_heads_reserve(V), 0, V * sizeof(uint32_t), gz_stream));
+
+    const gz_vec_t* vecs = gz_graph.pull_vectors;
+    uint64_t chunk_len = gz_chunk_len();
+    double* props = (double*)gz_graph.vertex_props;
+    double* acc = (double*)gz_graph.vertex_accumulators;
+    const double* outdeg = gz_graph.vertex_outdegrees;
+    mergeaccum_t* merge = gz_merge_buffer;
+    double* partials = (double*)gz_scratch.partials;
+    double* slots = (double*)gz_scratch.reduce_tmp;
+    uint32_t reduce_blocks = (uint32_t)gz_reduce_blocks(nchunks);
+    uint32_t* run_head = gz_pr_run_head;
+    double damping = gz_phase_config.damping;
+    gz_grid_bar_t* bar = gz_pr_bar;
+    uint32_t spin = gz_pr_persistent_spin;
+    uint64_t n_vec = nvec, n_chunks = nchunks, n_v = V, iters = iterations;
+    void* args[] = {&vecs, &n_vec, &chunk_len, &n_chunks, &props, &acc, &outdeg, &n_v, &merge, &partials,
+                    &slots, &reduce_blocks, &run_head, &reduce_buffer, &damping, &iters, &bar, &spin};
+    /* A cooperative launch can be refused while another process holds the
+     * device's cooperative resources; the per-phase launches then run, for
+     * the rest of this process. */
+    const hipError_t err = hipLaunchCooperativeKernel(fn, dim3(grid), dim3(GZ_BLOCK), args, 0, gz_stream);
+    if (err != hipSuccess) {
+        (void)hipGetLastError();
+        gz_pr_persistent_unavailable = 1;
+        fprintf(stderr, "grazelle: one-launch PageRank unavailable (%s); per-phase launches from now on\n",
+                hipGetErrorString(err));
+        return 0;
+    }
+    GZ_CHECK(hipMemcpyAsync(gz_pr_bar_host, &gz_pr_bar->abort, sizeof(uint32_t), hipMemcpyDeviceToHost, gz_stream));
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    if (*gz_pr_bar_host != 0) {
+        gz_pr_persistent_aborts += 1;
+        fprintf(stderr, "grazelle: a grid barrier of the one-launch PageRank gave up waiting; "
+                        "the run restarts with per-phase launches\n");
+        return -1;
+    }
+    gz_pr_persistent_count += 1;
+    return 1;
 }
 
 /* The vertex phase after a source-sliced pull: acc[v] is the sum of the

@@ -489,21 +489,6 @@ extern "C" void gz_merge_buffer_allocate(uint64_t units)
 
 /* ---- reductions ----------------------------------------------------------- */
 
-template <typename T>
-__device__ __forceinline__ T gz_block_sum(T x, T* lds)
-{
-    for (int o = GZ_WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, GZ_WAVE);
-    const int w = threadIdx.x / GZ_WAVE;
-    if ((threadIdx.x & (GZ_WAVE - 1)) == 0) lds[w] = x;
-    __syncthreads();
-    T r = 0;
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < (int)(blockDim.x / GZ_WAVE); ++i) r += lds[i];
-    }
-    __syncthreads();
-    return r;
-}
-
 /* Two-stage deterministic reduction: block b sums a fixed slice, the last
  * block to finish sums the block results in index order. */
 template <typename T>
@@ -537,7 +522,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_reduce(const T* __restrict__ in
     }
 }
 
-static uint64_t gz_reduce_blocks(uint64_t count)
+uint64_t gz_reduce_blocks(uint64_t count)
 {
     uint64_t blocks = (count + 4095) / 4096;
     if (blocks < 1) blocks = 1;
@@ -603,30 +588,14 @@ extern "C" const uint64_t* gz_reduce_words(void)
     return gz_reduce_mirror;
 }
 
-/* Parallel form of the serial merge in phases.c:41-70.  Records sharing a
- * final vertex form a run; the first record of each run (its head) folds the
- * run's partials in record order and, when the record after the run starts
- * on the same vertex (that unit stored its first vertex directly), folds the
- * accumulator too.  A record whose initial id has bit 63 set is unused. */
+/* Parallel form of the serial merge in phases.c:41-70, one thread per record
+ * (gz_merge_sum_record). */
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_merge_sum(const mergeaccum_t* __restrict__ mb, uint64_t count,
                                                            double* __restrict__ acc)
 {
     const uint64_t u = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= count) return;
-    const mergeaccum_t r = mb[u];
-    if (r.initial_vertex_id & GZ_LANE_VALID_BIT) return;
-    if (u > 0) {
-        const mergeaccum_t p = mb[u - 1];
-        if (!(p.initial_vertex_id & GZ_LANE_VALID_BIT) && p.final_vertex_id == r.final_vertex_id) return;
-    }
-    double proposed = r.final_partial_value;
-    uint64_t j = u + 1;
-    while (j < count && mb[j].final_vertex_id == r.final_vertex_id) {
-        proposed = proposed + mb[j].final_partial_value;
-        ++j;
-    }
-    if (j < count && mb[j].initial_vertex_id == r.final_vertex_id) proposed = proposed + acc[mb[j].initial_vertex_id];
-    acc[r.final_vertex_id] = proposed;
+    gz_merge_sum_record(mb, count, u, acc);
 }
 
 void gz_launch_merge_sum(const mergeaccum_t* merge, uint64_t count, double* acc)

@@ -100,6 +100,11 @@ _sig("gz_graph_indegrees", None, vp)
 _sig("gz_push_queue_capacity", C.c_int, u32)
 _sig("gz_graph_build_source_slices", C.c_int, u64, u32, u32, u64)
 _sig("gz_pr_graph_replay", None, C.c_int)
+_sig("gz_pr_persistent", C.c_int, C.c_int, u32)
+_sig("gz_pr_fused_tail", None, C.c_int)
+_sig("gz_pr_persistent_runs", u64)
+_sig("gz_pr_persistent_spin_limit", None, u32)
+_sig("gz_pr_persistent_gave_up", u64)
 _sig("gz_cc_graph_replay", None, C.c_int)
 _sig("gz_graph_replay_counts", None, C.POINTER(u64), C.POINTER(u64))
 _sig("gz_frontier_compact_narrow", None, C.c_int)
@@ -430,6 +435,37 @@ def pr_graph_replay(on: bool) -> None:
     """PageRank iterations 2.. from a replayed HIP graph (default on)."""
     init()
     lib.gz_pr_graph_replay(int(on))
+
+
+def pr_fused_tail(on: bool) -> None:
+    """PageRank's merge, sum and vertex phase as one launch on small lists (default on)."""
+    init()
+    lib.gz_pr_fused_tail(int(on))
+
+
+def pr_persistent(mode: int, workgroups_per_cu: int = 2) -> None:
+    """PageRank in one cooperative launch: 0 never (default), 1 small graphs, 2 any graph."""
+    init()
+    if lib.gz_pr_persistent(int(mode), int(workgroups_per_cu)) != 0:
+        raise ValueError(f"bad one-launch PageRank setting {mode}, {workgroups_per_cu}")
+
+
+def pr_persistent_runs() -> int:
+    """PageRank runs that took the one-launch path so far."""
+    init()
+    return int(lib.gz_pr_persistent_runs())
+
+
+def pr_persistent_spin_limit(polls: int) -> None:
+    """Polls a grid barrier of the one-launch PageRank makes before giving up."""
+    init()
+    lib.gz_pr_persistent_spin_limit(int(polls))
+
+
+def pr_persistent_gave_up() -> int:
+    """One-launch PageRank runs whose grid barrier gave up so far."""
+    init()
+    return int(lib.gz_pr_persistent_gave_up())
 
 
 def cc_graph_replay(on: bool) -> None:

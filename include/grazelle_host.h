@@ -177,6 +177,33 @@ int gz_slices_set_hot_cache(uint32_t vertices);
  * iteration count, device allocations) is captured the second time in a
  * row it runs and replayed while it repeats; results are the same. */
 void gz_pr_graph_replay(int on);
+/* PageRank runs that record nothing per iteration and use the
+ * scheduler-aware pull, on a pull list of at most 4096 chunks, run the merge,
+ * the sum of S and the vertex phase as one launch after each pull (two
+ * launches per iteration instead of four; results bit-identical).  On by
+ * default; 0 switches it off. */
+void gz_pr_fused_tail(int on);
+/* PageRank runs that record nothing per iteration and use the
+ * scheduler-aware pull can run all their iterations in one cooperative
+ * launch, with grid barriers between the pull and the sum-and-vertex step
+ * (results bit-identical to the per-phase launches).  Off by default: a
+ * barrier across the eight XCDs costs more than the launch boundary it
+ * replaces.  mode 0: never (the default); 1: when 32 B per pull vector + 24 B
+ * per vertex fit 128 MiB; 2: any graph.  workgroups_per_cu (1..8, capped by
+ * occupancy) sizes the grid.  Returns -1 for other values. */
+int gz_pr_persistent(int mode, uint32_t workgroups_per_cu);
+/* Runs that took the one-launch path since the library was loaded.  When a
+ * cooperative launch is refused (another process may hold the device's
+ * cooperative resources), a note goes to stderr and the per-phase launches
+ * run from then on. */
+uint64_t gz_pr_persistent_runs(void);
+/* Polls a grid barrier makes before it gives up (default 2^22, a few
+ * seconds).  The kernel then leaves at once, a note goes to stderr, and the
+ * run restarts from the initial ranks with the per-phase launches.  For
+ * tests of that path. */
+void gz_pr_persistent_spin_limit(uint32_t polls);
+/* One-launch runs whose barrier gave up since the library was loaded. */
+uint64_t gz_pr_persistent_gave_up(void);
 /* CC runs that record nothing per iteration, with the fixed threshold or a
  * forced engine over the scheduler-aware pull, keep the vote and the engine
  * choice on the device and run their iterations as a captured HIP graph,

@@ -84,11 +84,21 @@ def test_c1_pr_rmat16(gz, orc, device, ingress):
     else:
         gz.load_workload_device("pr-rmat16")
         assert np.array_equal(gz.device_vectors(True), orc.pack(g.pull, True))
-    gz.run(gz.options("pr", iterations=20, damping=0.85))
-    ranks = gz.pr_ranks()
     _, ref, _ = orc.pagerank(orc.Graph(g.num_vertices, g.pull, g.push), orc.options(iterations=20, damping=0.85))
-    assert _rel(ranks, ref) <= PR_TOL
-    assert abs(ranks.sum() - 1.0) < SUM_TOL
+    # the default chunks and the benchmark's (256 vectors); both take the
+    # fused merge-sum-vertex launch
+    try:
+        for chunk in (512, 256):
+            gz.set_chunk_vectors(chunk)
+            if ingress == "host":
+                gz.load(g)
+            gz.run(gz.options("pr", iterations=20, damping=0.85))
+            assert gz.lib.gz_pr_fused_tail_applies()
+            ranks = gz.pr_ranks()
+            assert _rel(ranks, ref) <= PR_TOL, chunk
+            assert abs(ranks.sum() - 1.0) < SUM_TOL
+    finally:
+        gz.set_chunk_vectors(512)
 
 
 # ---- C2 ---------------------------------------------------------------------------
