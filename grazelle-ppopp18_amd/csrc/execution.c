@@ -227,6 +227,37 @@ static double gz_probe_engine(int bfs, int pull, const gz_snapshot_t* s, gz_time
     return best;
 }
 
+static int gz_probe_wide_on = 0;
+
+void gz_probe_wide(int on) { gz_probe_wide_on = on ? 1 : 0; }
+
+/* As gz_probe_engine for an edge-phase variant: the frontier skip (the
+ * compaction) on or off and the push's vote hint (~0: no direct bitmap
+ * read) given; the configuration is restored afterwards. */
+static double gz_probe_variant(int bfs, int pull, const gz_snapshot_t* s, gz_timer_t* t, int skip, uint64_t hint)
+{
+    const uint32_t saved = gz_phase_config.push_frontier_skip;
+    gz_phase_config.push_frontier_skip = (uint32_t)skip;
+    double best = 0.0;
+    for (int rep = 0; rep < 3; ++rep) {
+        gz_snapshot_restore(s);
+        gz_timer_start(t);
+        phase_op_reset_global_accum();
+        if (!pull) {
+            if (bfs)
+                gz_bfs_push_vote_hint(hint);
+            else
+                gz_cc_push_vote_hint(hint);
+        }
+        gz_edge_phase(bfs, pull);
+        gz_timer_stop(t);
+        const double ms = gz_timer_elapsed_ms(t);
+        if (rep == 0 || ms < best) best = ms;
+    }
+    gz_phase_config.push_frontier_skip = saved;
+    return best;
+}
+
 static void gz_swap_frontiers(void)
 {
     /* execution_cc.c:245-250 / execution_bfs.c:253-258 */
@@ -567,6 +598,11 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
         if (probe) {
             /* the chosen engine runs last, so its result is what continues */
             gz_snapshot_take(&snap);
+            if (gz_probe_wide_on && bfs) {
+                rec.variant_ms[0] = gz_probe_variant(bfs, 1, &snap, t, 0, ~0ull);
+                rec.variant_ms[1] = gz_probe_variant(bfs, 0, &snap, t, 1, ~0ull);
+                rec.variant_ms[2] = gz_probe_variant(bfs, 0, &snap, t, 0, ~0ull);
+            }
             const double other = gz_probe_engine(bfs, !use_pull, &snap, t, 0, vote);
             const double chosen = gz_probe_engine(bfs, use_pull, &snap, t, 1, vote);
             rec.pull_ms = use_pull ? chosen : other;

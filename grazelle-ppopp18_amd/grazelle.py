@@ -47,7 +47,8 @@ class RunResult(C.Structure):
 class IterationRecord(C.Structure):
     _fields_ = [("iteration", u64), ("engine", u32), ("pad", u32), ("frontier_stat", u64),
                 ("vectors", u64), ("edges", u64), ("edge_ms", C.c_double), ("features", C.c_float * 8),
-                ("predicted", C.c_float * 2), ("pull_ms", C.c_double), ("push_ms", C.c_double)]
+                ("predicted", C.c_float * 2), ("pull_ms", C.c_double), ("push_ms", C.c_double),
+                ("variant_ms", C.c_double * 3)]
 
 
 class RmatParams(C.Structure):
@@ -102,6 +103,7 @@ _sig("gz_graph_build_source_slices", C.c_int, u64, u32, u32, u64)
 _sig("gz_pr_graph_replay", None, C.c_int)
 _sig("gz_pr_persistent", C.c_int, C.c_int, u32)
 _sig("gz_pr_fused_tail", None, C.c_int)
+_sig("gz_probe_wide", None, C.c_int)
 _sig("gz_pr_persistent_runs", u64)
 _sig("gz_pr_persistent_spin_limit", None, u32)
 _sig("gz_pr_persistent_gave_up", u64)
@@ -435,6 +437,12 @@ def pr_graph_replay(on: bool) -> None:
     """PageRank iterations 2.. from a replayed HIP graph (default on)."""
     init()
     lib.gz_pr_graph_replay(int(on))
+
+
+def probe_wide(on: bool) -> None:
+    """The BFS probe engine also times the uncompacted pull, the compacted push and the uncompacted push."""
+    init()
+    lib.gz_probe_wide(int(on))
 
 
 def pr_fused_tail(on: bool) -> None:

@@ -19,6 +19,7 @@ committed is decided on held-out graphs of the training distribution.
     python selector_train.py collect --graphs 200 --out data.npz
     python selector_train.py fit --data data.npz --out selector_bfs.bin
     python selector_train.py evaluate --data held.npz --weights selector_bfs.bin
+    python selector_train.py collect --graphs 20 --first-seed 90000 --wide --out wide.npz
 """
 from __future__ import annotations
 
@@ -59,30 +60,54 @@ def pick_roots(count: int, seed: int) -> list[int]:
 
 
 def probe_bfs(root: int) -> dict:
-    """One probed BFS: per-level features, both engines' times, fixed choice."""
+    """One probed BFS: per-level features, both engines' times, fixed choice,
+    and with gz.probe_wide(True) the times of three more edge-phase variants
+    (uncompacted pull, compacted push, uncompacted push)."""
     gz.run(gz.options("bfs", engine="probe", root=root))
     recs = gz.records()
     return {
         "features": np.array([list(r.features) for r in recs], dtype=np.float32).reshape(-1, 8),
         "pull_ms": np.array([r.pull_ms for r in recs]),
         "push_ms": np.array([r.push_ms for r in recs]),
+        "variant_ms": np.array([list(r.variant_ms) for r in recs], dtype=np.float64).reshape(-1, 3),
         "fixed": np.array([r.engine for r in recs], dtype=np.int32),
     }
 
 
-def collect(specs, roots_per_graph: int, log=print) -> dict:
+def collect(specs, roots_per_graph: int, log=print, wide: bool = False) -> dict:
     runs = []
     t0 = time.time()
-    for k, (scale, ef, seed) in enumerate(specs):
-        gz.load_rmat_device(scale, (1 << scale) * ef, seed)
-        for root in pick_roots(roots_per_graph, seed):
+    gz.probe_wide(wide)
+    try:
+        for k, (scale, ef, seed) in enumerate(specs):
+            gz.load_rmat_device(scale, (1 << scale) * ef, seed)
+            for root in pick_roots(roots_per_graph, seed):
+                run = probe_bfs(root)
+                run["graph"] = (scale, ef, seed)
+                run["root"] = root
+                run["edges"] = gz.num_edges()
+                runs.append(run)
+            log(f"[collect] graph {k + 1}/{len(specs)} s{scale} ef{ef} seed {seed}: "
+                f"{sum(r['features'].shape[0] for r in runs)} samples, {time.time() - t0:.1f} s")
+    finally:
+        gz.probe_wide(False)
+    return pack_runs(runs)
+
+
+def collect_workload(name: str, roots: int, seed: int, wide: bool = False) -> dict:
+    """Probed searches from `roots` seeded roots on a named workload graph."""
+    gz.load_workload_device(name)
+    gz.probe_wide(wide)
+    try:
+        runs = []
+        for root in pick_roots(roots, seed):
             run = probe_bfs(root)
-            run["graph"] = (scale, ef, seed)
+            run["graph"] = (0, 0, seed)
             run["root"] = root
             run["edges"] = gz.num_edges()
             runs.append(run)
-        log(f"[collect] graph {k + 1}/{len(specs)} s{scale} ef{ef} seed {seed}: "
-            f"{sum(r['features'].shape[0] for r in runs)} samples, {time.time() - t0:.1f} s")
+    finally:
+        gz.probe_wide(False)
     return pack_runs(runs)
 
 
@@ -92,6 +117,7 @@ def pack_runs(runs: list[dict]) -> dict:
         "features": np.concatenate([r["features"] for r in runs]) if runs else np.zeros((0, 8), np.float32),
         "pull_ms": np.concatenate([r["pull_ms"] for r in runs]) if runs else np.zeros(0),
         "push_ms": np.concatenate([r["push_ms"] for r in runs]) if runs else np.zeros(0),
+        "variant_ms": np.concatenate([r["variant_ms"] for r in runs]) if runs else np.zeros((0, 3)),
         "fixed": np.concatenate([r["fixed"] for r in runs]) if runs else np.zeros(0, np.int32),
         "run_lengths": lens,
         "graphs": np.array([r["graph"] for r in runs], dtype=np.int64).reshape(-1, 3),
@@ -143,6 +169,12 @@ def policy_times(data: dict, params=None, mean=None, std=None) -> dict:
     tp, ts = data["pull_ms"], data["push_ms"]
     fixed = np.where(data["fixed"] == 1, tp, ts)
     out = {"fixed": float(fixed.sum()), "best": float(np.minimum(tp, ts).sum())}
+    var = data.get("variant_ms")
+    if var is not None and var.shape[0] == tp.shape[0] and np.all(var > 0):
+        # the per-level best over five choices: the two engines as they run
+        # and the uncompacted pull, compacted push and uncompacted push
+        out["best_of_five"] = float(np.minimum(np.minimum(tp, ts), var.min(axis=1)).sum())
+        out["levels_won_by_variant"] = [int(np.sum(var[:, j] < np.minimum(tp, ts))) for j in range(3)]
     if params is not None:
         start = 0
         total = 0.0
@@ -187,6 +219,9 @@ def main(argv=None) -> int:
     c.add_argument("--first-seed", type=int, default=1000)
     c.add_argument("--roots", type=int, default=2)
     c.add_argument("--out", required=True)
+    c.add_argument("--wide", action="store_true", help="also time three more edge-phase variants per level")
+    c.add_argument("--workload", default=None, help="probe a named workload graph (e.g. bfs-rmat24) instead")
+    c.add_argument("--root-seed", type=int, default=24, help="root seed with --workload")
     f = sub.add_parser("fit")
     f.add_argument("--data", required=True)
     f.add_argument("--out", default=DEFAULT_WEIGHTS)
@@ -199,9 +234,13 @@ def main(argv=None) -> int:
     a = ap.parse_args(argv)
     gz.init(0)
     if a.cmd == "collect":
-        data = collect(training_graphs(a.graphs, a.first_seed), a.roots)
+        if a.workload:
+            data = collect_workload(a.workload, a.roots, a.root_seed, a.wide)
+        else:
+            data = collect(training_graphs(a.graphs, a.first_seed), a.roots, wide=a.wide)
         np.savez(a.out, **data)
-        print(json.dumps({"samples": int(data["features"].shape[0]), "runs": int(data["run_lengths"].shape[0])}))
+        print(json.dumps({"samples": int(data["features"].shape[0]), "runs": int(data["run_lengths"].shape[0]),
+                          **policy_times(data)}))
     elif a.cmd == "fit":
         data = dict(np.load(a.data))
         params, mean, std, hist = fit(data, a.steps, a.lr, per_edge=a.per_edge)

@@ -310,6 +310,9 @@ typedef struct gz_iteration_record_t {
     float features[8];          /* learned / probe engines: selector features (before normalisation) */
     float predicted[2];         /* learned engine: predicted log ms of pull, push */
     double pull_ms, push_ms;    /* probe engine: edge-phase time of each engine */
+    double variant_ms[3];       /* BFS probe with gz_probe_wide(1): the pull over every vector (no
+                                 * compaction of WantsInfo), the push after a compaction (never the
+                                 * direct bitmap read), the push over every vector; 0 otherwise */
 } gz_iteration_record_t;
 
 typedef struct gz_run_result_t {
@@ -318,6 +321,12 @@ typedef struct gz_run_result_t {
     uint64_t push_iterations;   /* total_iterations_used_scatter */
     double elapsed_ms;          /* wall time of the iteration loop, device-synchronised */
 } gz_run_result_t;
+
+/* The BFS probe engine also times three more edge-phase variants per level
+ * (gz_iteration_record_t.variant_ms), each from the same state, for bounds
+ * on what a selector over a wider set of choices could gain.  Off by
+ * default. */
+void gz_probe_wide(int on);
 
 /* Initialise props, accumulators and frontiers for the algorithm
  * (graph_helper_initialize_vertex_info / _create_and_initialize_frontiers). */

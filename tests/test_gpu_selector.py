@@ -119,6 +119,31 @@ def test_probe_features_and_sequence(gz, orc, device, rmat13, case):
         assert any(4096 < r.frontier_stat <= 65536 and r.engine == 2 for r in recs), case
 
 
+@pytest.mark.parametrize("case", ["rmat13", "star_path"])
+def test_wide_probe_times_every_variant(gz, orc, device, rmat13, case):
+    # the wide probe times the uncompacted pull, the compacted push and the
+    # uncompacted push from the same state as the two engines; the search
+    # still follows the fixed threshold and its depths stay exact
+    g, root = (rmat13, 3) if case == "rmat13" else (_star_path(gz), 0)
+    gz.load(g)
+    G = orc.Graph(g.num_vertices, g.pull, g.push)
+    gz.run(gz.options("bfs", engine="hybrid", root=root, profile=True))
+    hybrid = [r.engine for r in gz.records()]
+    gz.probe_wide(True)
+    try:
+        gz.run(gz.options("bfs", engine="probe", root=root))
+    finally:
+        gz.probe_wide(False)
+    recs = gz.records()
+    _, dep, _, _ = orc.bfs(G, orc.options(root=root))
+    assert np.array_equal(gz.bfs_result()[1], dep)
+    assert [r.engine for r in recs] == hybrid
+    assert all(r.pull_ms > 0 and r.push_ms > 0 and min(r.variant_ms) > 0 for r in recs)
+    # without the switch no variant is timed
+    gz.run(gz.options("bfs", engine="probe", root=root))
+    assert all(max(r.variant_ms) == 0 for r in gz.records())
+
+
 @pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="selector weights not trained yet")
 def test_learned_engine_keeps_results_exact(gz, orc, device, rmat13):
     g = rmat13
