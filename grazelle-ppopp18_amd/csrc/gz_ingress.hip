@@ -584,6 +584,14 @@ __global__ void gz_k_slice_gid(const uint64_t* __restrict__ has, const uint64_t*
 /* index in the concatenated slice accumulators (slice x's run starts at
  * `part_off`) of the first vertex of each 64-vertex block, at
  * out[block * nslices + x] */
+/* 32-bit copy of the FP64 out-degrees (integral, below 2^32: V <= 2^28 here). */
+__global__ void gz_k_outdeg32(const double* __restrict__ outdeg, uint64_t n, uint32_t* __restrict__ out)
+{
+    /* grid-stride: gz_grid_for caps the grid at 65,536 workgroups */
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        out[i] = (uint32_t)outdeg[i];
+}
+
 __global__ void gz_k_rank_base(const uint64_t* __restrict__ rank, uint64_t blocks, uint32_t x, uint32_t nslices,
                                uint64_t part_off, uint32_t* __restrict__ out)
 {
@@ -869,6 +877,10 @@ extern "C" int gz_graph_build_source_slices(uint64_t slice_vertices, uint32_t ti
         GZ_CHECK(hipStreamSynchronize(gz_stream));
         gz_device_free(part_gid[x]);
     }
+    gz_slices.outdeg32 = (uint32_t*)gz_device_malloc((nv + 8) * sizeof(uint32_t));
+    hipLaunchKernelGGL(gz_k_outdeg32, dim3(gz_grid_for(nv)), dim3(GZ_BLOCK), 0, gz_stream, gz_graph.vertex_outdegrees,
+                       nv, gz_slices.outdeg32);
+    GZ_LAUNCH_CHECK();
     gz_slices.base_list = gz_graph.pull_vectors;
     gz_slices.base_count = nvec;
     gz_slices.slice_vertices = S;

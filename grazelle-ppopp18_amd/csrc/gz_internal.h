@@ -179,6 +179,8 @@ typedef struct gz_source_slices_t {
                                        * 64-vertex block's first vertex in slice x (part_base.off[x] + its
                                        * local rank) */
     uint64_t rank_blocks;             /* ceil(V / 64) */
+    uint32_t* outdeg32;               /* [V] out-degree as a 32-bit count (0 for sinks), read by the
+                                       * vertex phase in place of the FP64 array */
     gz_slice_chunk_t* table;          /* chunk slots, built for table_chunk_len */
     uint64_t slots;
     uint64_t table_chunk_len;

@@ -769,7 +769,7 @@ extern "C" int gz_pr_run_persistent(uint64_t iterations, uint64_t* reduce_buffer
  * 32-bit words for up to 32 slices, else 64-bit. */
 template <int NS>
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr_sliced(
-    double* __restrict__ props, const double* __restrict__ outdeg, uint64_t start, uint64_t end,
+    double* __restrict__ props, const uint32_t* __restrict__ outdeg32, uint64_t start, uint64_t end,
     uint64_t num_vertices, const uint64_t* __restrict__ reduce_buffer, double damping,
     const void* __restrict__ mask_words, const double* __restrict__ parts, const uint32_t* __restrict__ rank_base)
 {
@@ -805,7 +805,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_vertex_pr_sliced(
             rb_next = (lane < NS) ? rank_base[nb * (uint64_t)NS + lane] : 0u;
         }
         const bool mine = v >= start && v < end;
-        const double od = mine ? outdeg[v] : 1.0;
+        const double od = mine ? (double)outdeg32[v] : 1.0;
         double a = 0.0;
 #pragma unroll
         for (int g = 0; g < NS; g += GZ_XCDS) {
@@ -883,7 +883,7 @@ extern "C" void perform_vertex_phase_pr(uint64_t vertex_start, uint64_t vertex_c
         if (sblocks > 65536) sblocks = 65536;
 #define GZ_VERTEX_SLICED_ARGS                                                                                  \
     dim3((unsigned)sblocks), dim3(GZ_BLOCK), 0, gz_stream, (double*)gz_graph.vertex_props,                        \
-        gz_graph.vertex_outdegrees, vertex_start, end, gz_graph.num_vertices, reduce_buffer,                     \
+        (const uint32_t*)gz_slices.outdeg32, vertex_start, end, gz_graph.num_vertices, reduce_buffer,            \
         gz_phase_config.damping, (const void*)gz_slices.mask, (const double*)gz_slices.parts,                    \
         (const uint32_t*)gz_slices.rank_base
 #define GZ_VERTEX_SLICED_CASE(L)                                                                                \

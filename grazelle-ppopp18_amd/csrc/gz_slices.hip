@@ -162,6 +162,7 @@ extern "C" void gz_source_slices_release(void)
     gz_device_free(gz_slices.mask);
     gz_device_free(gz_slices.parts);
     gz_device_free(gz_slices.rank_base);
+    gz_device_free(gz_slices.outdeg32);
     gz_device_free(gz_slices.gid);
     gz_device_free(gz_slices.table);
     gz_device_free(gz_slices.merge);
