@@ -533,7 +533,10 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
     const int probe = (o->engine == GZ_ENGINE_PROBE) && gz_graph.pull_vectors != NULL && gz_graph.push_vectors != NULL;
     const int features = learned || probe;
     gz_timer_t* t = (o->profile || probe) ? gz_timer_create() : NULL;
-    const uint32_t* indeg = features ? gz_graph_indegrees_device() : NULL;
+    /* the in-degrees feed the "unvisited in-edges" feature; a learned model
+     * that does not use it (std 0 in its weights) skips their reads */
+    const uint32_t* indeg =
+        (probe || (learned && gz_selector_feature_used(3))) ? gz_graph_indegrees_device() : NULL;
     gz_snapshot_t snap;
     memset(&snap, 0, sizeof(snap));
     if (probe) gz_snapshot_alloc(&snap);
@@ -553,7 +556,7 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
             sel.features[0] = logf(1.f + 1.f);
             sel.features[1] = logf(1.f + (float)(uint64_t)od[root]);
             sel.features[2] = logf(1.f + (float)(V - 1));
-            sel.features[3] = logf(1.f + (float)(E - gz_graph_indegree_host(root)));
+            sel.features[3] = gz_selector_feature_used(3) ? logf(1.f + (float)(E - gz_graph_indegree_host(root))) : 0.f;
             sel.features[4] = 1.f;
             sel.features[5] = 0.f;
             sel.features[6] = 0.f;
@@ -565,7 +568,7 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
         }
         /* BFS activates each vertex once: its edge phases fold the statistics
          * of the next frontier into their vote pass */
-        if (bfs) gz_features_fold(indeg);
+        if (bfs) gz_features_fold(1, indeg);
     }
     uint64_t ctr = 0;
     while (1) {
@@ -644,7 +647,7 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
             perform_vertex_phase_cc(0, V, NULL);
     }
     r->iterations = ctr;
-    gz_features_fold(NULL);
+    gz_features_fold(0, NULL);
     if (t) gz_timer_destroy(t);
     if (probe) gz_snapshot_free(&snap);
 }

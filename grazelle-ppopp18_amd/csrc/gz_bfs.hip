@@ -383,7 +383,7 @@ extern "C" void perform_edge_push_phase_bfs(const gz_vec_t* edge_list, uint64_t 
                                 gz_graph.vertex_outdegrees,
                                 gz_phase_config.stat_mode,
                                 (uint64_t)gz_phase_config.bfs_level,
-                                {gz_scratch.features_indeg ? gz_scratch.features : nullptr, gz_scratch.features_indeg}};
+                                {gz_scratch.features_on ? gz_scratch.features : nullptr, gz_scratch.features_indeg}};
         uint64_t* stamp = gz_scratch.stamp_target;
         gz_scratch.stamp_target = nullptr;
         gz_ktimer_begin(GZ_K_PUSH_BFS);
@@ -409,7 +409,7 @@ extern "C" void perform_edge_push_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     gz_scratch.partials_count = waves;
     gz_scratch.stats_count = gz_phase_config.collect_stats ? waves : 0;
     const uint64_t level = gz_phase_config.bfs_level;
-    const gz_feature_sink sink{gz_scratch.features_indeg ? gz_scratch.features : nullptr, gz_scratch.features_indeg};
+    const gz_feature_sink sink{gz_scratch.features_on ? gz_scratch.features : nullptr, gz_scratch.features_indeg};
     gz_ktimer_begin(GZ_K_PUSH_BFS);
     if (active)
         hipLaunchKernelGGL(gz_k_edge_push_bfs<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
@@ -450,7 +450,7 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     gz_scratch.partials_count = visit_blocks * GZ_WAVES_PER_BLOCK;
     gz_scratch.stats_count = gz_phase_config.collect_stats ? blocks * GZ_WAVES_PER_BLOCK : 0;
     const uint64_t level = gz_phase_config.bfs_level;
-    const gz_feature_sink sink{gz_scratch.features_indeg ? gz_scratch.features : nullptr, gz_scratch.features_indeg};
+    const gz_feature_sink sink{gz_scratch.features_on ? gz_scratch.features : nullptr, gz_scratch.features_indeg};
     gz_ktimer_begin(GZ_K_PULL_BFS_FIND);
     if (active) {
         hipLaunchKernelGGL(gz_k_edge_pull_bfs_first, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,

@@ -139,7 +139,9 @@ void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg, uint64_t* 
                      int bfs, uint64_t iteration, uint32_t prev_engine, float prev_ms, int source, int with_vote);
 /* BFS edge phases fold the selector's frontier statistics into their vote
  * pass while indeg != NULL (NULL turns it off). */
-void gz_features_fold(const uint32_t* indeg);
+/* BFS edge phases fold the selector's features (vertices, out-edges and,
+ * with `indeg`, in-edges of the vertices they activate) when `on`. */
+void gz_features_fold(int on, const uint32_t* indeg);
 /* Free the source-sliced layout (graph release). */
 void gz_source_slices_release(void);
 void gz_clear_bitmap_range(uint64_t* bitmap, uint64_t vertex_start, uint64_t vertex_count);

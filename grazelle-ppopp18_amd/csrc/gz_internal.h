@@ -78,7 +78,8 @@ typedef struct gz_scratch_t {
     uint64_t active_capacity;
     uint64_t* bfs_first_vector;  /* [V] first qualifying pull vector of each dst, ~0 when none */
     uint64_t* features;          /* [3 * partials_capacity] selector features per wave (BFS edge phases) */
-    const uint32_t* features_indeg; /* in-degrees when BFS edge phases fold the features, else nullptr */
+    uint32_t features_on;         /* BFS edge phases fold the selector's features */
+    const uint32_t* features_indeg; /* in-degrees for the in-edge feature, or nullptr (not folded) */
     uint64_t* stamp_target;      /* when set, the next compaction writes the wall clock there (once) */
     uint64_t scratch_vertices;   /* V the vertex-sized buffers were sized for */
     /* overflow queue of the sparse CC push: pieces of at most
@@ -383,7 +384,7 @@ __device__ __forceinline__ void gz_feature_add(gz_feature_acc& f, const gz_featu
     if (s.out) {
         f.count += 1;
         f.out_edges += (uint64_t)outdeg[v];
-        f.in_edges += (uint64_t)s.indeg[v];
+        f.in_edges += s.indeg ? (uint64_t)s.indeg[v] : 0ull;
     }
 }
 

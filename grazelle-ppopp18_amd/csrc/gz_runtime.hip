@@ -365,7 +365,11 @@ static void gz_kt_flush(void)
 
 extern "C" int gz_kernel_timing_enabled(void) { return gz_kt_on; }
 
-extern "C" void gz_features_fold(const uint32_t* indeg) { gz_scratch.features_indeg = indeg; }
+extern "C" void gz_features_fold(int on, const uint32_t* indeg)
+{
+    gz_scratch.features_on = on ? 1u : 0u;
+    gz_scratch.features_indeg = on ? indeg : nullptr;
+}
 
 extern "C" void gz_kernel_timing_enable(int on)
 {

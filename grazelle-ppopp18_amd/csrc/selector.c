@@ -58,6 +58,11 @@ int gz_selector_save_file(const char* path)
     return ok ? 0 : -2;
 }
 
+int gz_selector_feature_used(uint32_t i)
+{
+    return gz_sel_loaded && i < GZ_SEL_IN && gz_sel_std[i] > 0.f;
+}
+
 uint32_t gz_selector_host_decide(const float* x, float* y)
 {
     if (!gz_sel_loaded) return 0;
@@ -70,8 +75,9 @@ uint32_t gz_selector_host_decide(const float* x, float* y)
     const float* b3 = W3 + GZ_SEL_OUT * GZ_SEL_H;
     float xn[GZ_SEL_IN], h1[GZ_SEL_H], h2[GZ_SEL_H];
     for (int i = 0; i < GZ_SEL_IN; ++i) {
+        /* a feature with std 0 is unused by the model: its input is 0 */
         const float sd = gz_sel_std[i];
-        xn[i] = (x[i] - gz_sel_mean[i]) / (sd > 0.f ? sd : 1.f);
+        xn[i] = sd > 0.f ? (x[i] - gz_sel_mean[i]) / sd : 0.f;
     }
     for (int j = 0; j < GZ_SEL_H; ++j) {
         float a = b1[j];

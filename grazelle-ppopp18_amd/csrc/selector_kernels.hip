@@ -187,7 +187,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_stats(const uint64_t* __res
         if ((word >> lane) & 1ull) {
             const uint64_t v = (w0 + (uint64_t)k) * 64 + (uint64_t)lane;
             fe += (uint64_t)outdeg[v];
-            fi += (uint64_t)indeg[v];
+            fi += indeg ? (uint64_t)indeg[v] : 0ull;
         }
     }
     fv = gz_wave_sum_u64(fv);
@@ -266,8 +266,9 @@ __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restri
         x[6] = logf(1.f + ms);
         x[7] = num_vertices ? (float)num_edges / (float)num_vertices : 0.f;
         for (int i = 0; i < GZ_SEL_IN; ++i) {
+            /* a feature with std 0 is unused by the model: its input is 0 */
             const float sd = norm[GZ_SEL_IN + i];
-            xn[i] = (x[i] - norm[i]) / (sd > 0.f ? sd : 1.f);
+            xn[i] = sd > 0.f ? (x[i] - norm[i]) / sd : 0.f;
         }
     }
     __syncthreads();
@@ -336,7 +337,7 @@ __global__ void gz_k_sel_init(gz_sel_dev_state* st, uint64_t num_vertices, uint6
         if (root < num_vertices) {
             partials[0] = 1;
             partials[1] = (uint64_t)outdeg[root];
-            partials[2] = (uint64_t)indeg[root];
+            partials[2] = indeg ? (uint64_t)indeg[root] : 0ull;
         }
     }
 }

@@ -142,6 +142,7 @@ _sig("gz_selector_clear", None)
 _sig("gz_selector_load_file", C.c_int, C.c_char_p)
 _sig("gz_selector_save_file", C.c_int, C.c_char_p)
 _sig("gz_selector_loaded", C.c_int)
+_sig("gz_selector_feature_used", C.c_int, u32)
 _sig("gz_selector_forward_batch", C.c_int, vp, vp, vp, u64)
 _sig("gz_selector_gradient", C.c_float, vp, vp, vp, u64, vp)
 _sig("gz_selector_train", C.c_int, vp, vp, vp, u64, u32, C.c_float, vp)
@@ -727,6 +728,11 @@ def selector_save(path: str) -> None:
     rc = lib.gz_selector_save_file(path.encode())
     if rc != 0:
         raise OSError(f"cannot write selector weights to {path} ({rc})")
+
+
+def selector_feature_used(i: int) -> bool:
+    """Whether the loaded selector model uses input feature i (std > 0 in its weights)."""
+    return bool(lib.gz_selector_feature_used(int(i)))
 
 
 def selector_loaded() -> bool:

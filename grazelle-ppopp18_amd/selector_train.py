@@ -148,12 +148,28 @@ def init_params(seed: int) -> np.ndarray:
     return np.concatenate(parts).astype(np.float32)
 
 
-def fit(data: dict, steps: int = 6000, lr: float = 1e-3, seed: int = 7, per_edge: bool = False):
+def normalize(X: np.ndarray, mean: np.ndarray, std: np.ndarray) -> np.ndarray:
+    """(X - mean) / std per feature, and 0 for a feature whose std is 0 (a
+    feature the model does not use), as the engine normalises."""
+    used = std > 0
+    out = np.zeros_like(X, dtype=np.float32)
+    out[..., used] = ((X[..., used] - mean[used]) / std[used]).astype(np.float32)
+    return out
+
+
+def fit(data: dict, steps: int = 6000, lr: float = 1e-3, seed: int = 7, per_edge: bool = False,
+        drop: tuple = ()):
+    """Fit the MLP; features in `drop` get mean 0 and std 0 in the weights,
+    so the model sees 0 for them and the engine need not compute them (for
+    feature 3 the BFS edge phases then skip their in-degree reads)."""
     X = data["features"].astype(np.float32)
     mean = X.mean(axis=0)
     std = X.std(axis=0)
     std[std < 1e-6] = 1.0
-    Xn = ((X - mean) / std).astype(np.float32)
+    for j in drop:
+        mean[j] = 0.0
+        std[j] = 0.0
+    Xn = normalize(X, mean, std)
     params, hist = gz.selector_train(init_params(seed), Xn, targets(data, per_edge), steps, lr)
     return params, mean.astype(np.float32), std.astype(np.float32), hist
 
@@ -184,7 +200,7 @@ def policy_times(data: dict, params=None, mean=None, std=None) -> dict:
                 x = X[i].copy()
                 x[5] = prev_engine
                 x[6] = np.log1p(prev_ms)
-                y = gz.selector_forward(params, ((x - mean) / std)[None, :].astype(np.float32))[0]
+                y = gz.selector_forward(params, normalize(x[None, :], mean, std))[0]
                 pull = y[0] < y[1]
                 t = tp[i] if pull else ts[i]
                 total += t
@@ -228,6 +244,8 @@ def main(argv=None) -> int:
     f.add_argument("--steps", type=int, default=6000)
     f.add_argument("--lr", type=float, default=1e-3)
     f.add_argument("--per-edge", action="store_true", help="regress log ms - log E")
+    f.add_argument("--drop-features", default="", help="comma-separated features the model does not use "
+                   "(3: unvisited in-edges, whose in-degree reads the engine then skips)")
     e = sub.add_parser("evaluate")
     e.add_argument("--data", required=True)
     e.add_argument("--weights", default=DEFAULT_WEIGHTS)
@@ -243,7 +261,8 @@ def main(argv=None) -> int:
                           **policy_times(data)}))
     elif a.cmd == "fit":
         data = dict(np.load(a.data))
-        params, mean, std, hist = fit(data, a.steps, a.lr, per_edge=a.per_edge)
+        drop = tuple(int(x) for x in a.drop_features.split(",") if x.strip())
+        params, mean, std, hist = fit(data, a.steps, a.lr, per_edge=a.per_edge, drop=drop)
         save_weights(a.out, params, mean, std)
         print(json.dumps({"samples": int(data["features"].shape[0]), "loss_first": float(hist[0]),
                           "loss_last": float(hist[-1]), **policy_times(data, params, mean, std)}))

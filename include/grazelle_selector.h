@@ -34,6 +34,11 @@ extern "C" {
 #define GZ_SEL_PARAMS (GZ_SEL_H * GZ_SEL_IN + GZ_SEL_H + GZ_SEL_H * GZ_SEL_H + GZ_SEL_H + GZ_SEL_OUT * GZ_SEL_H + GZ_SEL_OUT)
 
 int gz_selector_loaded(void);
+/* Whether the loaded model uses input feature i (0..7).  A feature whose
+ * normalisation std is 0 in the weights is unused: the model sees 0 there,
+ * and for feature 3 (unvisited in-edges) the BFS edge phases then skip the
+ * per-vertex in-degree reads that feed it. */
+int gz_selector_feature_used(uint32_t i);
 
 /* Weights: GZ_SEL_PARAMS floats plus feature normalisation (mean[8], std[8]). */
 int gz_selector_set_weights(const float* params, const float* feat_mean, const float* feat_std);

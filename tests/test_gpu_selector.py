@@ -167,6 +167,40 @@ def test_learned_engine_keeps_results_exact(gz, orc, device, rmat13):
 
 
 @pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="selector weights not trained yet")
+def test_unused_feature_skips_its_inputs(gz, orc, device, rmat13, tmp_path):
+    # the committed model does not use feature 3 (std 0): the engine feeds it
+    # 0 and skips the in-degree reads; a copy that marks it used (std 1)
+    # reads them.  Depths stay exact either way, and with std 0 the model's
+    # output cannot depend on the feature's value
+    import selector_train as st
+    params, mean, std = st.load_weights(WEIGHTS)
+    assert std[3] == 0 and np.all(np.delete(std, 3) > 0)
+    used = tmp_path / "used.bin"
+    std_used = std.copy()
+    std_used[3] = 1.0
+    st.save_weights(str(used), params, mean, std_used)
+    x = np.random.default_rng(4).normal(size=(16, 8)).astype(np.float32)
+    x2 = x.copy()
+    x2[:, 3] += 5.0
+    assert np.array_equal(st.normalize(x, mean, std), st.normalize(x2, mean, std))
+    g = rmat13
+    gz.load(g)
+    G = orc.Graph(g.num_vertices, g.pull, g.push)
+    try:
+        for path, flag in ((WEIGHTS, False), (str(used), True)):
+            gz.selector_load(path)
+            assert gz.selector_feature_used(3) == flag
+            assert all(gz.selector_feature_used(i) for i in (0, 1, 2, 4, 5, 6, 7))
+            for root in (0, 3):
+                gz.run(gz.options("bfs", engine="learned", root=root))
+                _, dep, _, _ = orc.bfs(G, orc.options(root=root))
+                assert np.array_equal(gz.bfs_result()[1], dep)
+    finally:
+        gz.selector_clear()
+    assert not gz.selector_feature_used(3)
+
+
+@pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="selector weights not trained yet")
 def test_learned_gain_on_held_out_graphs(gz, device):
     """Probe both engines on graphs the network never saw and compare the
     edge-phase total of the fixed threshold with the network's choices (both
