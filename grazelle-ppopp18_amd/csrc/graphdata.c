@@ -284,6 +284,7 @@ void gz_graph_release(void)
     gz_frontier_counts_release();
     gz_pr_release();
     gz_bfs_batch_release();
+    gz_bfs_pending_release();
     gz_device_free(G.d_pull);
     gz_device_free(G.d_push);
     gz_device_free(G.d_pull_offsets);

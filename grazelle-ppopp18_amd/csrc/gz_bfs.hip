@@ -35,6 +35,10 @@
  * resolves the parent and updates the frontiers.  Both engines therefore give
  * the same words on every run.
  */
+#include <string.h>
+
+#include "gz_frontier.h"
+#include "gz_host_internal.h"
 #include "gz_slots.h"
 #include "gz_sparse_push.h"
 
@@ -172,23 +176,51 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_bfs_retire_wants(uint64_t* __re
  * first qualifying vector, and pass A then loads none of the destination's
  * other vectors (in a level with a large frontier, most destinations stop
  * here, as the single thread of the reference stops at its first qualifying
- * vector).  Runs on pass A's grid; the wave's counts go to its stats slot and
- * pass A adds its own. */
+ * vector).  A destination whose first vector does not qualify and that has
+ * more vectors gets its bit in `pending` (cleared beforehand), and pass A
+ * walks the vectors of those destinations only.  Runs on pass A's grid; the
+ * wave's counts go to its stats slot and pass A adds its own. */
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_first(const gz_vec_t* __restrict__ vecs,
                                                                      gz_slot_source src,
                                                                      const uint64_t* __restrict__ has_info,
                                                                      uint64_t* __restrict__ first_vector,
+                                                                     uint64_t* __restrict__ pending,
                                                                      uint64_t* __restrict__ stats, uint32_t collect)
 {
+    const int lane = gz_lane();
     const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
     const uint64_t n = src.totals[0];
     uint32_t vectors = 0, edges = 0;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t d = src.list[i];
-        const uint64_t vi = src.offsets[d];
-        const gz_lanes v = gz_load_vector(vecs, vi);
-        vectors += 1;
-        if (gz_vec_has_frontier_source(v, has_info, edges)) first_vector[d] = vi;
+    /* wave-uniform loop: the list is in vertex order, so pending vertices of
+     * neighbouring lanes share bitmap words; one lane per run of equal words
+     * sets the run's bits with one atomic */
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(uint64_t)(GZ_WAVE - 1)); i0 < n;
+         i0 += stride) {
+        const uint64_t i = i0 + (uint64_t)lane;
+        bool pend = false;
+        uint64_t d = 0;
+        if (i < n) {
+            d = src.list[i];
+            const uint64_t vi = src.offsets[d];
+            const gz_lanes v = gz_load_vector(vecs, vi);
+            vectors += 1;
+            if (gz_vec_has_frontier_source(v, has_info, edges))
+                first_vector[d] = vi;
+            else
+                pend = src.offsets[d + 1] - vi > 1;
+        }
+        const uint64_t word = pend ? (d >> 5) : ~0ull;
+        uint32_t bits = pend ? (1u << (d & 31)) : 0u;
+#pragma unroll
+        for (int o = 1; o < GZ_WAVE; o <<= 1) {
+            const uint32_t xb = __shfl_down(bits, o, GZ_WAVE);
+            const uint64_t xw = (uint64_t)__shfl_down((unsigned long long)word, o, GZ_WAVE);
+            if (lane + o < GZ_WAVE && xw == word) bits |= xb;
+        }
+        const uint64_t prev_word = (uint64_t)__shfl_up((unsigned long long)word, 1, GZ_WAVE);
+        if (pend && (lane == 0 || prev_word != word))
+            atomicOr(reinterpret_cast<unsigned int*>(pending) + word, bits);
     }
     if (collect) {
         const uint64_t tv = gz_wave_sum_u64(vectors), te = gz_wave_sum_u64(edges);
@@ -351,6 +383,40 @@ static gz_slot_source gz_bfs_source(const uint64_t* offsets, uint64_t count)
     return src;
 }
 
+/* The bottom-up step's pending destinations (first vector not qualifying,
+ * more vectors to test): a bitmap and its compaction, sized for the graph. */
+static struct {
+    uint64_t* bits;
+    gz_compact_out out;
+    uint64_t vertices;
+} gz_bfs_pend;
+
+static void gz_bfs_pending_reserve(uint64_t V)
+{
+    if (gz_bfs_pend.vertices >= V && gz_bfs_pend.bits != nullptr) return;
+    gz_device_free(gz_bfs_pend.bits);
+    gz_device_free(gz_bfs_pend.out.list);
+    gz_device_free(gz_bfs_pend.out.prefix);
+    gz_device_free(gz_bfs_pend.out.block_sums);
+    gz_device_free(gz_bfs_pend.out.totals);
+    gz_bfs_pend.bits = (uint64_t*)gz_device_malloc(((V + 63) / 64 + 8) * sizeof(uint64_t));
+    gz_bfs_pend.out.list = (uint64_t*)gz_device_malloc((V + 8) * sizeof(uint64_t));
+    gz_bfs_pend.out.prefix = (uint64_t*)gz_device_malloc((V + 8) * sizeof(uint64_t));
+    gz_bfs_pend.out.block_sums = (uint64_t*)gz_device_malloc((2 * (gz_compact_blocks(V) + 1) + 8) * sizeof(uint64_t));
+    gz_bfs_pend.out.totals = (uint64_t*)gz_device_malloc(8 * sizeof(uint64_t));
+    gz_bfs_pend.vertices = V;
+}
+
+extern "C" void gz_bfs_pending_release(void)
+{
+    gz_device_free(gz_bfs_pend.bits);
+    gz_device_free(gz_bfs_pend.out.list);
+    gz_device_free(gz_bfs_pend.out.prefix);
+    gz_device_free(gz_bfs_pend.out.block_sums);
+    gz_device_free(gz_bfs_pend.out.totals);
+    memset(&gz_bfs_pend, 0, sizeof(gz_bfs_pend));
+}
+
 /* Frontier statistic of the next push, when the driver knows it (the vote of
  * the level that built the frontier); ~0 when unknown.  Consumed by the next
  * push. */
@@ -452,15 +518,28 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     const uint64_t level = gz_phase_config.bfs_level;
     const gz_feature_sink sink{gz_scratch.features_on ? gz_scratch.features : nullptr, gz_scratch.features_indeg};
     gz_ktimer_begin(GZ_K_PULL_BFS_FIND);
+    gz_slot_source pend_src = src;
     if (active) {
+        const uint64_t V = gz_graph.num_vertices;
+        gz_bfs_pending_reserve(V);
+        gz_dev_fill_u64(gz_bfs_pend.bits, 0, (V + 63) / 64);
         hipLaunchKernelGGL(gz_k_edge_pull_bfs_first, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
-                           src, gz_graph.frontier_has_info, gz_scratch.bfs_first_vector, gz_scratch.stats,
-                           gz_phase_config.collect_stats);
+                           src, gz_graph.frontier_has_info, gz_scratch.bfs_first_vector, gz_bfs_pend.bits,
+                           gz_scratch.stats, gz_phase_config.collect_stats);
         GZ_LAUNCH_CHECK();
+        /* pass A walks the pending destinations' vectors only (it skips each
+         * one's first vector, which pass A0 tested); the compaction has its
+         * own kernel timer */
+        gz_ktimer_end(GZ_K_PULL_BFS_FIND);
+        gz_frontier_compact_to(gz_bfs_pend.bits, gz_graph.pull_vertex_offsets, V, edge_list_count, gz_bfs_pend.out);
+        gz_ktimer_begin(GZ_K_PULL_BFS_FIND);
+        pend_src.list = gz_bfs_pend.out.list;
+        pend_src.prefix = gz_bfs_pend.out.prefix;
+        pend_src.totals = gz_bfs_pend.out.totals;
     }
     if (active)
         hipLaunchKernelGGL(gz_k_edge_pull_bfs_find<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
-                           edge_list, src, chunk_len, gz_graph.frontier_has_info, gz_graph.frontier_wants_info,
+                           edge_list, pend_src, chunk_len, gz_graph.frontier_has_info, gz_graph.frontier_wants_info,
                            gz_scratch.bfs_first_vector, gz_scratch.stats, gz_phase_config.collect_stats);
     else
         hipLaunchKernelGGL(gz_k_edge_pull_bfs_find<false>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,

@@ -298,12 +298,13 @@ static const uint32_t* gz_frontier_counts(const uint64_t* offsets, uint64_t num_
 }
 
 static void gz_frontier_compact_impl(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
-                                     uint64_t vector_count, gz_guard guard, uint64_t max_grid, bool allow_build)
+                                     uint64_t vector_count, gz_guard guard, uint64_t max_grid, bool allow_build,
+                                     const gz_compact_out& out)
 {
     const uint64_t blocks = gz_compact_blocks(num_vertices);
     if (blocks == 0) {
-        gz_device_zero(gz_scratch.active_totals, 2 * sizeof(uint64_t));
-        gz_device_zero(gz_scratch.active_prefix, sizeof(uint64_t));
+        gz_device_zero(out.totals, 2 * sizeof(uint64_t));
+        gz_device_zero(out.prefix, sizeof(uint64_t));
         return;
     }
     const uint64_t grid = blocks < max_grid ? blocks : max_grid;
@@ -323,34 +324,46 @@ static void gz_frontier_compact_impl(const uint64_t* bitmap, const uint64_t* off
     gz_ktimer_begin(GZ_K_COMPACT);
     if (narrow)
         hipLaunchKernelGGL(gz_k_frontier_count<true>, dim3((unsigned)grid), dim3(GZ_BLOCK), 0, gz_stream, bitmap, src,
-                           num_vertices, gz_scratch.active_block_sums, stamp, blocks, guard);
+                           num_vertices, out.block_sums, stamp, blocks, guard);
     else
         hipLaunchKernelGGL(gz_k_frontier_count<false>, dim3((unsigned)grid), dim3(GZ_BLOCK), 0, gz_stream, bitmap,
-                           src, num_vertices, gz_scratch.active_block_sums, stamp, blocks, guard);
+                           src, num_vertices, out.block_sums, stamp, blocks, guard);
     GZ_LAUNCH_CHECK();
-    hipLaunchKernelGGL(gz_k_frontier_scan, dim3(1), dim3(1024), 0, gz_stream, gz_scratch.active_block_sums, blocks,
-                       gz_scratch.active_totals, gz_scratch.active_prefix, guard);
+    hipLaunchKernelGGL(gz_k_frontier_scan, dim3(1), dim3(1024), 0, gz_stream, out.block_sums, blocks, out.totals,
+                       out.prefix, guard);
     GZ_LAUNCH_CHECK();
     if (narrow)
         hipLaunchKernelGGL(gz_k_frontier_write<true>, dim3((unsigned)grid), dim3(GZ_BLOCK), 0, gz_stream, bitmap, src,
-                           num_vertices, gz_scratch.active_block_sums, gz_scratch.active_vertices,
-                           gz_scratch.active_prefix, blocks, guard);
+                           num_vertices, out.block_sums, out.list, out.prefix, blocks, guard);
     else
         hipLaunchKernelGGL(gz_k_frontier_write<false>, dim3((unsigned)grid), dim3(GZ_BLOCK), 0, gz_stream, bitmap,
-                           src, num_vertices, gz_scratch.active_block_sums, gz_scratch.active_vertices,
-                           gz_scratch.active_prefix, blocks, guard);
+                           src, num_vertices, out.block_sums, out.list, out.prefix, blocks, guard);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_COMPACT);
+}
+
+static gz_compact_out gz_compact_scratch_out(void)
+{
+    return gz_compact_out{gz_scratch.active_vertices, gz_scratch.active_prefix, gz_scratch.active_block_sums,
+                          gz_scratch.active_totals};
 }
 
 void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
                          uint64_t vector_count)
 {
-    gz_frontier_compact_impl(bitmap, offsets, num_vertices, vector_count, gz_guard{nullptr, 0}, ~0ull, true);
+    gz_frontier_compact_impl(bitmap, offsets, num_vertices, vector_count, gz_guard{nullptr, 0}, ~0ull, true,
+                             gz_compact_scratch_out());
+}
+
+void gz_frontier_compact_to(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
+                            uint64_t vector_count, const gz_compact_out& out)
+{
+    gz_frontier_compact_impl(bitmap, offsets, num_vertices, vector_count, gz_guard{nullptr, 0}, ~0ull, true, out);
 }
 
 void gz_frontier_compact_guarded(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
                                  uint64_t vector_count, gz_guard guard, uint64_t max_grid)
 {
-    gz_frontier_compact_impl(bitmap, offsets, num_vertices, vector_count, guard, max_grid, false);
+    gz_frontier_compact_impl(bitmap, offsets, num_vertices, vector_count, guard, max_grid, false,
+                             gz_compact_scratch_out());
 }

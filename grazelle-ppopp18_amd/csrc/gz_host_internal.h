@@ -104,6 +104,8 @@ void gz_scratch_reserve_public(uint64_t waves, uint64_t num_vertices);
 void gz_bfs_push_vote_hint(uint64_t vote);
 /* Drop the frontier compaction's per-vertex vector counts (graph release). */
 void gz_frontier_counts_release(void);
+/* Frees the BFS bottom-up step's pending-destination buffers. */
+void gz_bfs_pending_release(void);
 /* The PageRank push's fixed-point accumulators: freed with the graph, and
  * cleared when a PageRank run starts. */
 void gz_pr_release(void);

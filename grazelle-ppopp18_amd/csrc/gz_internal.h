@@ -228,6 +228,19 @@ void gz_launch_merge_sum(const mergeaccum_t* merge, uint64_t count, double* acc)
 /* Build the compacted active-vertex list from `bitmap` against `offsets`. */
 void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
                          uint64_t vector_count);
+/* Where a compaction writes: the vertices with vectors whose bit is set, in
+ * vertex order ([V]), the exclusive prefix of their vector counts ([V+1]),
+ * per-block sums (2 gz_compact_blocks(V) + 8 words) and the totals ([0]
+ * vertices, [1] vectors). */
+struct gz_compact_out {
+    uint64_t* list;
+    uint64_t* prefix;
+    uint64_t* block_sums;
+    uint64_t* totals;
+};
+/* As gz_frontier_compact, into other buffers than the scratch's. */
+void gz_frontier_compact_to(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
+                            uint64_t vector_count, const gz_compact_out& out);
 
 /* ---- device helpers ------------------------------------------------------- */
 
