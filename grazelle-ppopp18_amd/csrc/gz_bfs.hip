@@ -502,8 +502,10 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     gz_slot_source src = gz_bfs_source(gz_graph.pull_vertex_offsets, edge_list_count);
     uint64_t blocks;
     if (active) {
-        gz_frontier_compact(gz_graph.frontier_wants_info, gz_graph.pull_vertex_offsets, gz_graph.num_vertices,
-                            edge_list_count);
+        /* pass A0 and the visit walk the unvisited list; pass A has a list of
+         * its own (below), so no vector prefix is needed here */
+        gz_frontier_list(gz_graph.frontier_wants_info, gz_graph.pull_vertex_offsets, gz_graph.num_vertices,
+                         edge_list_count);
         blocks = GZ_PERSISTENT_BLOCKS;
     } else {
         blocks = ((edge_list_count + chunk_len - 1) / chunk_len + GZ_WAVES_PER_BLOCK - 1) / GZ_WAVES_PER_BLOCK;

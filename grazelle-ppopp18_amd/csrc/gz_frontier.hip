@@ -241,6 +241,76 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_write(const uint64_t* 
     }
 }
 
+/* The vertex list alone (no vector prefix): vertices whose bit is set in
+ * `bitmap` and in `hasvec` (the vertices that own vectors), in order. */
+__device__ __forceinline__ uint32_t gz_list_bits(const uint64_t* __restrict__ bitmap,
+                                                 const uint64_t* __restrict__ hasvec, uint64_t num_vertices,
+                                                 uint64_t v0)
+{
+    if (v0 >= num_vertices) return 0u;
+    const uint64_t mask = (1ull << GZ_COMPACT_VPT) - 1ull;
+    uint64_t bits = ((bitmap[v0 >> 6] & hasvec[v0 >> 6]) >> (v0 & 63)) & mask;
+    const uint64_t left = num_vertices - v0;
+    if (left < GZ_COMPACT_VPT) bits &= (1ull << left) - 1ull;
+    return (uint32_t)bits;
+}
+
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_list_count(const uint64_t* __restrict__ bitmap,
+                                                                     const uint64_t* __restrict__ hasvec,
+                                                                     uint64_t num_vertices,
+                                                                     uint64_t* __restrict__ block_sums,
+                                                                     uint64_t* __restrict__ stamp, uint64_t nblocks)
+{
+    __shared__ unsigned long long lds[GZ_WAVES_PER_BLOCK];
+    if (stamp != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *stamp = (uint64_t)wall_clock64();
+    for (uint64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const uint64_t v0 = b * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
+        const unsigned long long nv = (unsigned long long)__popc(gz_list_bits(bitmap, hasvec, num_vertices, v0));
+        unsigned long long tv;
+        gz_block_exclusive_scan<unsigned long long>(nv, lds, tv);
+        if (threadIdx.x == 0) {
+            block_sums[2 * b] = tv;
+            block_sums[2 * b + 1] = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_list_write(const uint64_t* __restrict__ bitmap,
+                                                                     const uint64_t* __restrict__ hasvec,
+                                                                     uint64_t num_vertices,
+                                                                     const uint64_t* __restrict__ block_sums,
+                                                                     uint64_t* __restrict__ list, uint64_t nblocks)
+{
+    __shared__ unsigned long long lds[GZ_WAVES_PER_BLOCK];
+    for (uint64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
+        const uint64_t v0 = b * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
+        uint32_t bits = gz_list_bits(bitmap, hasvec, num_vertices, v0);
+        unsigned long long tv;
+        uint64_t pos = block_sums[2 * b] +
+                       gz_block_exclusive_scan<unsigned long long>((unsigned long long)__popc(bits), lds, tv);
+        while (bits) {
+            const int k = __ffs(bits) - 1;
+            bits &= bits - 1u;
+            list[pos++] = v0 + (uint64_t)k;
+        }
+    }
+}
+
+/* hasvec bit v: vertex v owns at least one vector of the list. */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_has_vectors(const uint64_t* __restrict__ offsets,
+                                                             uint64_t num_vertices, uint64_t words,
+                                                             uint64_t* __restrict__ hasvec)
+{
+    const int lane = gz_lane();
+    for (uint64_t w = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) / GZ_WAVE; w < words;
+         w += (uint64_t)gridDim.x * blockDim.x / GZ_WAVE) {
+        const uint64_t v = w * 64 + (uint64_t)lane;
+        const bool has = v < num_vertices && offsets[v + 1] > offsets[v];
+        const uint64_t word = __ballot(has);
+        if (lane == 0) hasvec[w] = word;
+    }
+}
+
 static int gz_compact_narrow = 1;
 
 extern "C" void gz_frontier_compact_narrow(int on) { gz_compact_narrow = on ? 1 : 0; }
@@ -256,6 +326,15 @@ static struct {
 } gz_counts_cache[2];
 static int gz_counts_next;
 
+/* Bitmaps of the vertices that own vectors, per offsets array, kept as the
+ * counts are. */
+static struct {
+    const uint64_t* offsets;
+    uint64_t num_vertices, vector_count;
+    uint64_t* bits;
+} gz_hasvec_cache[2];
+static int gz_hasvec_next;
+
 extern "C" void gz_frontier_counts_release(void)
 {
     for (auto& e : gz_counts_cache) {
@@ -263,6 +342,33 @@ extern "C" void gz_frontier_counts_release(void)
         e = {nullptr, 0, 0, nullptr};
     }
     gz_counts_next = 0;
+    for (auto& e : gz_hasvec_cache) {
+        gz_device_free(e.bits);
+        e = {nullptr, 0, 0, nullptr};
+    }
+    gz_hasvec_next = 0;
+}
+
+static const uint64_t* gz_frontier_hasvec(const uint64_t* offsets, uint64_t num_vertices, uint64_t vector_count)
+{
+    for (auto& e : gz_hasvec_cache)
+        if (e.bits != nullptr && e.offsets == offsets && e.num_vertices == num_vertices &&
+            e.vector_count == vector_count)
+            return e.bits;
+    auto& e = gz_hasvec_cache[gz_hasvec_next];
+    gz_hasvec_next ^= 1;
+    gz_device_free(e.bits);
+    const uint64_t words = (num_vertices + 63) / 64;
+    e.bits = (uint64_t*)gz_device_malloc((words + 8) * sizeof(uint64_t));
+    uint64_t blocks = (words * GZ_WAVE + GZ_BLOCK - 1) / GZ_BLOCK;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(gz_k_has_vectors, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, offsets, num_vertices,
+                       words, e.bits);
+    GZ_LAUNCH_CHECK();
+    e.offsets = offsets;
+    e.num_vertices = num_vertices;
+    e.vector_count = vector_count;
+    return e.bits;
 }
 
 /* The counts of a list if they are built, else nullptr (allocates nothing). */
@@ -353,6 +459,29 @@ void gz_frontier_compact(const uint64_t* bitmap, const uint64_t* offsets, uint64
 {
     gz_frontier_compact_impl(bitmap, offsets, num_vertices, vector_count, gz_guard{nullptr, 0}, ~0ull, true,
                              gz_compact_scratch_out());
+}
+
+void gz_frontier_list(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices, uint64_t vector_count)
+{
+    const uint64_t blocks = gz_compact_blocks(num_vertices);
+    if (blocks == 0) {
+        gz_device_zero(gz_scratch.active_totals, 2 * sizeof(uint64_t));
+        return;
+    }
+    const uint64_t* hasvec = gz_frontier_hasvec(offsets, num_vertices, vector_count);
+    uint64_t* stamp = gz_scratch.stamp_target;
+    gz_scratch.stamp_target = nullptr;
+    gz_ktimer_begin(GZ_K_COMPACT);
+    hipLaunchKernelGGL(gz_k_frontier_list_count, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, hasvec,
+                       num_vertices, gz_scratch.active_block_sums, stamp, blocks);
+    GZ_LAUNCH_CHECK();
+    hipLaunchKernelGGL(gz_k_frontier_scan, dim3(1), dim3(1024), 0, gz_stream, gz_scratch.active_block_sums, blocks,
+                       gz_scratch.active_totals, gz_scratch.active_prefix, gz_guard{nullptr, 0});
+    GZ_LAUNCH_CHECK();
+    hipLaunchKernelGGL(gz_k_frontier_list_write, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, hasvec,
+                       num_vertices, gz_scratch.active_block_sums, gz_scratch.active_vertices, blocks);
+    GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_COMPACT);
 }
 
 void gz_frontier_compact_to(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,

@@ -238,6 +238,10 @@ struct gz_compact_out {
     uint64_t* block_sums;
     uint64_t* totals;
 };
+/* The vertex list alone (scratch active_vertices, totals[0]; no vector
+ * prefix, totals[1] = 0): vertices with their bit set that own vectors. */
+void gz_frontier_list(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
+                      uint64_t vector_count);
 /* As gz_frontier_compact, into other buffers than the scratch's. */
 void gz_frontier_compact_to(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
                             uint64_t vector_count, const gz_compact_out& out);
