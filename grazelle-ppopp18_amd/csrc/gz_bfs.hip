@@ -42,6 +42,25 @@
 #include "gz_slots.h"
 #include "gz_sparse_push.h"
 
+/* The largest valid source of v in HasInfo (the asm's vmaxpd over the
+ * qualifying lanes, phase_edge_pull_bfs.asm:175-196), or ~0 when none is;
+ * counts v's valid lanes into `edges`. */
+__device__ __forceinline__ uint64_t gz_vec_frontier_parent(const gz_lanes& v, const uint64_t* __restrict__ has_info,
+                                                           uint32_t& edges)
+{
+    uint64_t parent = ~0ull;
+    const uint64_t l[4] = {v.l0, v.l1, v.l2, v.l3};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (gz_lane_valid(l[k])) {
+            edges += 1;
+            const uint64_t s = gz_lane_id(l[k]);
+            if (gz_bit_test(has_info, s) && (parent == ~0ull || s > parent)) parent = s;
+        }
+    }
+    return parent;
+}
+
 __device__ __forceinline__ bool gz_vec_has_frontier_source(const gz_lanes& v, const uint64_t* __restrict__ has_info,
                                                            uint32_t& edges)
 {
@@ -173,55 +192,85 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_bfs_retire_wants(uint64_t* __re
 
 /* Pass A0 of the bottom-up step over the compacted destinations: each
  * destination's first vector.  When it qualifies it is the destination's
- * first qualifying vector, and pass A then loads none of the destination's
- * other vectors (in a level with a large frontier, most destinations stop
- * here, as the single thread of the reference stops at its first qualifying
- * vector).  A destination whose first vector does not qualify and that has
- * more vectors gets its bit in `pending` (cleared beforehand), and pass A
- * walks the vectors of those destinations only.  Runs on pass A's grid; the
- * wave's counts go to its stats slot and pass A adds its own. */
+ * first qualifying vector, so A0 visits the destination at once, as pass B
+ * would: the parent is the vector's largest frontier source, the word, the
+ * frontier bits, the vote and the selector's features are written here (in a
+ * level with a large frontier, most destinations stop here, as the single
+ * thread of the reference stops at its first qualifying vector).  A
+ * destination whose first vector does not qualify and that has more vectors
+ * gets its bit in `pending` (cleared beforehand); pass A walks those
+ * destinations' vectors and pass B visits the ones it resolves.  Runs on pass
+ * A's grid; the wave's counts go to its stats slot and pass A adds its own;
+ * its vote and features go to wave slot gw, pass B's to the slots after the
+ * grid's. */
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_first(const gz_vec_t* __restrict__ vecs,
                                                                      gz_slot_source src,
                                                                      const uint64_t* __restrict__ has_info,
-                                                                     uint64_t* __restrict__ first_vector,
+                                                                     uint64_t* __restrict__ wants_info,
                                                                      uint64_t* __restrict__ pending,
+                                                                     uint64_t* __restrict__ words,
+                                                                     uint64_t* __restrict__ next_frontier,
+                                                                     const double* __restrict__ outdeg,
+                                                                     uint32_t stat_mode, uint64_t level,
+                                                                     uint64_t* __restrict__ partials,
+                                                                     gz_feature_sink sink,
                                                                      uint64_t* __restrict__ stats, uint32_t collect)
 {
     const int lane = gz_lane();
     const uint64_t gw = (uint64_t)blockIdx.x * GZ_WAVES_PER_BLOCK + (threadIdx.x / GZ_WAVE);
     const uint64_t n = src.totals[0];
     uint32_t vectors = 0, edges = 0;
-    /* wave-uniform loop: the list is in vertex order, so pending vertices of
+    uint64_t vote = 0;
+    gz_feature_acc feat{0, 0, 0};
+    /* wave-uniform loop: the list is in vertex order, so the vertices of
      * neighbouring lanes share bitmap words; one lane per run of equal words
-     * sets the run's bits with one atomic */
+     * applies the run's bits with one atomic per bitmap */
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(uint64_t)(GZ_WAVE - 1)); i0 < n;
          i0 += stride) {
         const uint64_t i = i0 + (uint64_t)lane;
-        bool pend = false;
+        bool pend = false, found = false;
         uint64_t d = 0;
         if (i < n) {
             d = src.list[i];
             const uint64_t vi = src.offsets[d];
             const gz_lanes v = gz_load_vector(vecs, vi);
             vectors += 1;
-            if (gz_vec_has_frontier_source(v, has_info, edges))
-                first_vector[d] = vi;
-            else
+            const uint64_t parent = gz_vec_frontier_parent(v, has_info, edges);
+            if (parent != ~0ull) {
+                found = true;
+                words[d] = ((uint64_t)level << 48) | parent;
+                vote += gz_frontier_stat(outdeg, d, stat_mode);
+                gz_feature_add(feat, sink, outdeg, d);
+            } else {
                 pend = src.offsets[d + 1] - vi > 1;
+            }
         }
-        const uint64_t word = pend ? (d >> 5) : ~0ull;
-        uint32_t bits = pend ? (1u << (d & 31)) : 0u;
+        const uint64_t word = (i < n) ? (d >> 5) : ~0ull;
+        uint32_t pbits = pend ? (1u << (d & 31)) : 0u;
+        uint32_t fbits = found ? (1u << (d & 31)) : 0u;
 #pragma unroll
         for (int o = 1; o < GZ_WAVE; o <<= 1) {
-            const uint32_t xb = __shfl_down(bits, o, GZ_WAVE);
+            const uint32_t xp = __shfl_down(pbits, o, GZ_WAVE);
+            const uint32_t xf = __shfl_down(fbits, o, GZ_WAVE);
             const uint64_t xw = (uint64_t)__shfl_down((unsigned long long)word, o, GZ_WAVE);
-            if (lane + o < GZ_WAVE && xw == word) bits |= xb;
+            if (lane + o < GZ_WAVE && xw == word) {
+                pbits |= xp;
+                fbits |= xf;
+            }
         }
         const uint64_t prev_word = (uint64_t)__shfl_up((unsigned long long)word, 1, GZ_WAVE);
-        if (pend && (lane == 0 || prev_word != word))
-            atomicOr(reinterpret_cast<unsigned int*>(pending) + word, bits);
+        if (i < n && (lane == 0 || prev_word != word)) {
+            if (pbits) atomicOr(reinterpret_cast<unsigned int*>(pending) + word, pbits);
+            if (fbits) {
+                atomicAnd(reinterpret_cast<unsigned int*>(wants_info) + word, ~fbits);
+                atomicOr(reinterpret_cast<unsigned int*>(next_frontier) + word, fbits);
+            }
+        }
     }
+    const uint64_t total = gz_wave_sum_u64(vote);
+    if (lane == 0) partials[gw] = total;
+    gz_feature_flush(feat, sink, gw);
     if (collect) {
         const uint64_t tv = gz_wave_sum_u64(vectors), te = gz_wave_sum_u64(edges);
         if (gz_lane() == 0) {
@@ -513,9 +562,12 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     }
     uint64_t visit_blocks = GZ_PERSISTENT_BLOCKS;
     const uint64_t waves = ((blocks > visit_blocks) ? blocks : visit_blocks) * GZ_WAVES_PER_BLOCK;
-    gz_scratch_reserve(waves, gz_graph.num_vertices);
+    /* with the frontier skip, pass A0 (on pass A's grid of `blocks`) and pass
+     * B each have a slot per wave: A0's first, then B's */
+    const uint64_t vote_slots = active ? (blocks + visit_blocks) * GZ_WAVES_PER_BLOCK : visit_blocks * GZ_WAVES_PER_BLOCK;
+    gz_scratch_reserve(waves > vote_slots ? waves : vote_slots, gz_graph.num_vertices);
     gz_scratch.partials_is_double = 0;
-    gz_scratch.partials_count = visit_blocks * GZ_WAVES_PER_BLOCK;
+    gz_scratch.partials_count = vote_slots;
     gz_scratch.stats_count = gz_phase_config.collect_stats ? blocks * GZ_WAVES_PER_BLOCK : 0;
     const uint64_t level = gz_phase_config.bfs_level;
     const gz_feature_sink sink{gz_scratch.features_on ? gz_scratch.features : nullptr, gz_scratch.features_indeg};
@@ -526,8 +578,10 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
         gz_bfs_pending_reserve(V);
         gz_dev_fill_u64(gz_bfs_pend.bits, 0, (V + 63) / 64);
         hipLaunchKernelGGL(gz_k_edge_pull_bfs_first, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
-                           src, gz_graph.frontier_has_info, gz_scratch.bfs_first_vector, gz_bfs_pend.bits,
-                           gz_scratch.stats, gz_phase_config.collect_stats);
+                           src, gz_graph.frontier_has_info, gz_graph.frontier_wants_info, gz_bfs_pend.bits,
+                           (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
+                           gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, level,
+                           (uint64_t*)gz_scratch.partials, sink, gz_scratch.stats, gz_phase_config.collect_stats);
         GZ_LAUNCH_CHECK();
         /* pass A walks the pending destinations' vectors only (it skips each
          * one's first vector, which pass A0 tested); the compaction has its
@@ -550,13 +604,17 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_PULL_BFS_FIND);
     gz_ktimer_begin(GZ_K_PULL_BFS_VISIT);
-    if (active)
+    if (active) {
+        /* pass B visits the pending destinations pass A resolved */
+        const uint64_t a0_waves = blocks * GZ_WAVES_PER_BLOCK;
+        const gz_feature_sink bsink{sink.out ? sink.out + 3 * a0_waves : nullptr, sink.indeg};
         hipLaunchKernelGGL(gz_k_edge_pull_bfs_visit<true>, dim3((unsigned)visit_blocks), dim3(GZ_BLOCK), 0, gz_stream,
-                           edge_list, gz_scratch.active_vertices, gz_scratch.active_totals, gz_graph.num_vertices,
+                           edge_list, gz_bfs_pend.out.list, gz_bfs_pend.out.totals, gz_graph.num_vertices,
                            gz_graph.frontier_has_info, gz_graph.frontier_wants_info, gz_scratch.bfs_first_vector,
                            (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators,
                            gz_graph.vertex_outdegrees, gz_phase_config.stat_mode, level,
-                           (uint64_t*)gz_scratch.partials, sink);
+                           (uint64_t*)gz_scratch.partials + a0_waves, bsink);
+    }
     else
         hipLaunchKernelGGL(gz_k_edge_pull_bfs_visit<false>, dim3((unsigned)visit_blocks), dim3(GZ_BLOCK), 0, gz_stream,
                            edge_list, gz_scratch.active_vertices, gz_scratch.active_totals, gz_graph.num_vertices,
