@@ -229,6 +229,13 @@ static double gz_probe_engine(int bfs, int pull, const gz_snapshot_t* s, gz_time
 
 static int gz_probe_wide_on = 0;
 
+/* The learned engine decides only while the vote is within a factor of
+ * gz_sel_trust of the fixed threshold; further out, the fixed rule's choice
+ * stands (0: the model decides every level). */
+static double gz_sel_trust = 4.0;
+
+void gz_selector_set_trust(double k) { gz_sel_trust = (k >= 1.0) ? k : 0.0; }
+
 void gz_probe_wide(int on) { gz_probe_wide_on = on ? 1 : 0; }
 
 /* As gz_probe_engine for an edge-phase variant: the frontier skip (the
@@ -582,8 +589,15 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
             use_pull = 1;
         else if (o->engine == GZ_ENGINE_PUSH)
             use_pull = 0;
-        else if (learned)
+        else if (learned) {
             use_pull = (sel.choice == 1);
+            if (gz_sel_trust > 0.0) {
+                if ((double)vote > gz_sel_trust * (double)threshold)
+                    use_pull = 1;
+                else if ((double)vote * gz_sel_trust < (double)threshold)
+                    use_pull = 0;
+            }
+        }
         else
             use_pull = vote > threshold;
         if (use_pull && gz_graph.pull_vectors == NULL) use_pull = 0;

@@ -143,6 +143,7 @@ _sig("gz_selector_load_file", C.c_int, C.c_char_p)
 _sig("gz_selector_save_file", C.c_int, C.c_char_p)
 _sig("gz_selector_loaded", C.c_int)
 _sig("gz_selector_feature_used", C.c_int, u32)
+_sig("gz_selector_set_trust", None, C.c_double)
 _sig("gz_selector_forward_batch", C.c_int, vp, vp, vp, u64)
 _sig("gz_selector_gradient", C.c_float, vp, vp, vp, u64, vp)
 _sig("gz_selector_train", C.c_int, vp, vp, vp, u64, u32, C.c_float, vp)
@@ -728,6 +729,12 @@ def selector_save(path: str) -> None:
     rc = lib.gz_selector_save_file(path.encode())
     if rc != 0:
         raise OSError(f"cannot write selector weights to {path} ({rc})")
+
+
+def selector_set_trust(k: float) -> None:
+    """The learned engine follows the model only within a factor k of the fixed threshold (default 4; < 1: always)."""
+    init()
+    lib.gz_selector_set_trust(float(k))
 
 
 def selector_feature_used(i: int) -> bool:

@@ -39,6 +39,12 @@ int gz_selector_loaded(void);
  * and for feature 3 (unvisited in-edges) the BFS edge phases then skip the
  * per-vertex in-degree reads that feed it. */
 int gz_selector_feature_used(uint32_t i);
+/* The learned engine follows the model only while the frontier statistic
+ * lies within a factor k of the fixed threshold (E/5, or V/2 with counts
+ * only); beyond it the fixed rule's choice stands, so that a model is not
+ * trusted far outside the frontiers of its training graphs.  Default 4;
+ * k < 1 lets the model decide every level. */
+void gz_selector_set_trust(double k);
 
 /* Weights: GZ_SEL_PARAMS floats plus feature normalisation (mean[8], std[8]). */
 int gz_selector_set_weights(const float* params, const float* feat_mean, const float* feat_std);

@@ -200,6 +200,41 @@ def test_unused_feature_skips_its_inputs(gz, orc, device, rmat13, tmp_path):
     assert not gz.selector_feature_used(3)
 
 
+@pytest.mark.parametrize("prefer", ["push", "pull"])
+def test_trust_band_bounds_the_model(gz, orc, device, rmat13, tmp_path, prefer):
+    # a model that always prefers one engine (zero weights, output biases
+    # +-10): inside the band the engine follows it, outside (vote beyond a
+    # factor k of E/5) the fixed rule's choice stands; depths stay exact
+    import selector_train as st
+    params = np.zeros(gz.SEL_PARAMS, dtype=np.float32)
+    params[-2:] = [10.0, -10.0] if prefer == "push" else [-10.0, 10.0]
+    path = tmp_path / f"{prefer}.bin"
+    st.save_weights(str(path), params, np.zeros(8, np.float32), np.ones(8, np.float32))
+    g = rmat13
+    gz.load(g)
+    G = orc.Graph(g.num_vertices, g.pull, g.push)
+    thr = g.num_edges // 5
+    model = 2 if prefer == "push" else 1
+    gz.selector_load(str(path))
+    try:
+        for k in (4.0, 1.5, 0.0):
+            gz.selector_set_trust(k)
+            for root in (0, 3):
+                gz.run(gz.options("bfs", engine="learned", root=root, profile=True))
+                _, dep, _, _ = orc.bfs(G, orc.options(root=root))
+                assert np.array_equal(gz.bfs_result()[1], dep)
+                for r in gz.records():
+                    want = model
+                    if k >= 1.0 and r.frontier_stat > k * thr:
+                        want = 1
+                    elif k >= 1.0 and r.frontier_stat * k < thr:
+                        want = 2
+                    assert r.engine == want, (k, root, r.iteration, r.frontier_stat, thr)
+    finally:
+        gz.selector_set_trust(4.0)
+        gz.selector_clear()
+
+
 @pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="selector weights not trained yet")
 def test_learned_gain_on_held_out_graphs(gz, device):
     """Probe both engines on graphs the network never saw and compare the
