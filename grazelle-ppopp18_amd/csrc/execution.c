@@ -520,6 +520,10 @@ static int gz_cc_device_loop(const gz_run_options_t* o, uint64_t vote, uint64_t 
     return 1;
 }
 
+static uint64_t gz_cc_tail_rounds = 0;
+
+uint64_t gz_cc_push_tail_rounds(void) { return gz_cc_tail_rounds; }
+
 /* Shared loop of execution_cc.c:94-268 and execution_bfs.c:120-276. */
 static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
 {
@@ -612,6 +616,12 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
         memcpy(rec.predicted, sel.predicted, sizeof(rec.predicted));
         gz_phase_config.bfs_level = (uint32_t)ctr;
         float prev_ms = -1.0f;
+        /* a small CC frontier: its push iterations run on the device back to
+         * back (gz_cc_launch_push_tail); the vote comes back with their count */
+        const int tail = !bfs && !use_pull && !features && !o->profile && !o->collect_stats &&
+                         (o->engine == GZ_ENGINE_PUSH || o->engine == GZ_ENGINE_HYBRID ||
+                          o->engine == GZ_ENGINE_LEARNED) &&
+                         gz_cc_push_tail_applies(vote);
         if (probe) {
             /* the chosen engine runs last, so its result is what continues */
             gz_snapshot_take(&snap);
@@ -629,8 +639,14 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
             if (t) gz_timer_start(t);
             if (learned) gz_sel_dev_stamp(bfs && gz_phase_config.push_frontier_skip);
             phase_op_reset_global_accum();
-            gz_push_hint(bfs, use_pull, vote);
-            gz_edge_phase(bfs, use_pull);
+            if (tail) {
+                const uint64_t cap = o->engine == GZ_ENGINE_PUSH ? ~0ull : threshold;
+                const uint64_t rounds = o->max_iterations ? o->max_iterations - ctr + 1 : ~0ull;
+                gz_cc_launch_push_tail(cap, rounds, gz_reduce_buffer);
+            } else {
+                gz_push_hint(bfs, use_pull, vote);
+                gz_edge_phase(bfs, use_pull);
+            }
         }
         if (use_pull)
             r->pull_iterations += 1;
@@ -639,7 +655,7 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
         /* with BFS statistics folded into the edge phase, the selector's
          * decision kernel also sums the vote */
         const int vote_in_step = features && bfs;
-        if (!vote_in_step) phase_op_write_global_accum_to_buf(gz_reduce_buffer);
+        if (!vote_in_step && !tail) phase_op_write_global_accum_to_buf(gz_reduce_buffer);
         if (t && !probe) gz_timer_stop(t);
         if (features) {
             /* the vertices this phase activated are the next frontier */
@@ -648,6 +664,12 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
                             vote_in_step);
         }
         const uint64_t next_vote = phase_op_combine_global_var_from_buf(gz_reduce_buffer);
+        if (tail) {
+            const uint64_t rounds = gz_reduce_words()[7];
+            ctr += rounds - 1;
+            r->push_iterations += rounds - 1;
+            gz_cc_tail_rounds += rounds;
+        }
         if (features) gz_selector_read(&sel);
         if (t && !probe) rec.edge_ms = gz_timer_elapsed_ms(t);
         if (o->collect_stats) gz_phase_stats_read(&rec.vectors, &rec.edges);

@@ -422,6 +422,241 @@ extern "C" void perform_edge_push_phase_cc(const gz_vec_t* edge_list, uint64_t e
     gz_cc_launch_push(edge_list, edge_list_count);
 }
 
+/* ---- push tail: small-frontier iterations in one workgroup -------------------
+ * The last CC iterations of a graph with long paths (planted trees, R-MAT's
+ * sparse fringe) push tiny frontiers, and each pays its launches and the
+ * host's read of the vote more than its edges.  While the frontier statistic
+ * stays at most `vote_cap` (<= GZ_CC_TAIL_CAP), gz_k_cc_push_tail runs these
+ * push iterations ("rounds") back to back in one workgroup: round 1 pushes
+ * the compacted HasInfo list, and each later round pushes the vertices whose
+ * labels the round before lowered, listed in LDS.  Every edge gets the update
+ * of the push (gz_cc_apply: atomic min, HasInfo* bit, frontier statistic per
+ * lowering), so labels reach the same fixpoint as one launch per iteration.
+ *
+ * A round's list cannot outgrow the LDS: its vertices are targets of edges of
+ * the round before, at most that frontier's out-degrees, which its statistic
+ * counts and vote_cap bounds (the list check is kept as a guard).
+ *
+ * Between rounds the HasInfo* bitmap is also the set of vertices listed for
+ * the next round: a round first clears its own vertices' bits, so a vertex
+ * lowered again after its label was read is listed once more.  A source's
+ * label is read with a device-scope atomic load after that clear: another
+ * wavefront may have lowered it in L2 since this compute unit's L1 cached
+ * the line, and the value sent must be the latest.  (The destination's label
+ * is read plainly: a stale, higher value only costs one more atomic min.)
+ *
+ * The workgroup stops after a round that activates nothing, whose statistic
+ * exceeds vote_cap (with vote_cap at most the hybrid threshold, the next
+ * iteration would pull), or after max_rounds, and leaves what one push
+ * iteration leaves: HasInfo* holds the last round's activations, out[0] its
+ * statistic (the vote) and out[7] the rounds run. */
+#define GZ_CC_TAIL_THREADS 1024
+#define GZ_CC_TAIL_CAP 16384
+#define GZ_CC_TAIL_SLOTS 1
+/* Default entry bound.  Measured on CC over R-MAT s22 with planted trees
+ * (MI355X): bounds of 1024 and 4096 give the same time to solution and 16384
+ * a slower one, since a round of a few thousand vectors is faster on the
+ * whole grid than in one workgroup, whose rounds are latency-bound. */
+#define GZ_CC_TAIL_VOTE 4096
+
+/* Words the tail's no-op atomics go to, GZ_CC_TAIL_SLOTS * 4 per thread.
+ * Nothing changes them: an atomic min of ~0 and an atomic or of 0 leave a
+ * word as it is, and their returned values are never used. */
+__device__ unsigned long long gz_cc_tail_sink[GZ_CC_TAIL_THREADS * GZ_CC_TAIL_SLOTS * 4];
+
+__global__ __launch_bounds__(GZ_CC_TAIL_THREADS) void gz_k_cc_push_tail(
+    uint64_t* __restrict__ labels, uint64_t* __restrict__ next_frontier, const gz_vec_t* __restrict__ vecs,
+    const uint64_t* __restrict__ offsets, const double* __restrict__ outdeg, uint32_t stat_mode,
+    const uint64_t* __restrict__ first_list, const uint64_t* __restrict__ first_count, uint64_t vote_cap,
+    uint64_t max_rounds, uint64_t* __restrict__ out)
+{
+    constexpr uint32_t T = GZ_CC_TAIL_THREADS;
+    constexpr int kWaves = GZ_CC_TAIL_THREADS / GZ_WAVE;
+    __shared__ uint32_t lists[2][GZ_CC_TAIL_CAP];
+    __shared__ uint32_t incl[T];   /* inclusive vector counts of the chunk's sources */
+    __shared__ uint32_t first[T];  /* their first push vector */
+    __shared__ uint64_t msg[T];    /* their labels */
+    __shared__ uint32_t srcv[T];   /* the sources */
+    __shared__ uint64_t wave_tot[kWaves];
+    __shared__ uint32_t next_count;
+    __shared__ unsigned long long round_vote;
+    const uint32_t tid = threadIdx.x;
+    const int lane = gz_lane(), wave = (int)(tid / GZ_WAVE);
+    uint64_t n = *first_count;
+    uint32_t cur = 0;
+    uint64_t rounds = 0, vote = 0;
+    for (;;) {
+        if (rounds > 0)
+            for (uint64_t i = tid; i < n; i += T) gz_bit_clear(next_frontier, lists[cur][i]);
+        if (tid == 0) {
+            next_count = 0;
+            round_vote = 0;
+        }
+        /* the clears are done in L2 before any wavefront sets a bit this round */
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        uint64_t my_vote = 0;
+        for (uint64_t base = 0; base < n; base += T) {
+            const uint64_t i = base + tid;
+            uint64_t c = 0, f = 0, m = 0, v = 0;
+            if (i < n) {
+                v = rounds == 0 ? first_list[i] : (uint64_t)lists[cur][i];
+                f = offsets[v];
+                c = offsets[v + 1] - f;
+                m = __hip_atomic_load(&labels[v], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            const uint64_t wincl = gz_wave_inclusive_sum(c, lane);
+            if (lane == GZ_WAVE - 1) wave_tot[wave] = wincl;
+            __syncthreads();
+            uint64_t before = 0, total = 0;
+            for (int w = 0; w < kWaves; ++w) {
+                const uint64_t x = wave_tot[w];
+                if (w < wave) before += x;
+                total += x;
+            }
+            incl[tid] = (uint32_t)(before + wincl);
+            first[tid] = (uint32_t)f;
+            msg[tid] = m;
+            srcv[tid] = (uint32_t)v;
+            __syncthreads();
+            const uint32_t nsrc = (uint32_t)(n - base < T ? n - base : T);
+            /* kSlots vectors per thread at a time.  The edge steps are
+             * branch-free, so each step's loads and atomics are in flight
+             * together: an edge whose update does not apply (or a lane
+             * without an edge) sends its atomic min (of ~0) and atomic or
+             * (of 0) to its own word of gz_cc_tail_sink, which no other lane
+             * touches, instead of to a label that many edges share. */
+            constexpr int kSlots = GZ_CC_TAIL_SLOTS;
+            unsigned long long* sink = gz_cc_tail_sink + (uint64_t)tid * (kSlots * 4);
+            for (uint64_t s0 = tid; s0 < total; s0 += (uint64_t)kSlots * T) {
+                uint64_t l[kSlots][4], mm[kSlots];
+                uint32_t sv[kSlots];
+#pragma unroll
+                for (int b = 0; b < kSlots; ++b) {
+                    const uint64_t slot = s0 + (uint64_t)b * T;
+                    l[b][0] = l[b][1] = l[b][2] = l[b][3] = 0;
+                    mm[b] = ~0ull;
+                    sv[b] = srcv[0];
+                    if (slot >= total) continue;
+                    /* the first source whose inclusive count exceeds the slot */
+                    uint32_t lo = 0, hi = nsrc - 1;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if ((uint64_t)incl[mid] > slot)
+                            hi = mid;
+                        else
+                            lo = mid + 1;
+                    }
+                    const uint64_t excl = lo ? incl[lo - 1] : 0;
+                    const gz_lanes vec = gz_load_vector(vecs, (uint64_t)first[lo] + (slot - excl));
+                    l[b][0] = vec.l0;
+                    l[b][1] = vec.l1;
+                    l[b][2] = vec.l2;
+                    l[b][3] = vec.l3;
+                    mm[b] = msg[lo];
+                    sv[b] = srcv[lo];
+                }
+                uint32_t d[kSlots][4]; /* vertex ids fit 32 bits here */
+                uint64_t g[kSlots][4];
+#pragma unroll
+                for (int b = 0; b < kSlots; ++b)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        d[b][k] = (uint32_t)(gz_lane_valid(l[b][k]) ? gz_lane_id(l[b][k]) : sv[b]);
+                        g[b][k] = labels[d[b][k]];
+                    }
+                /* gz_frontier_stat for the modes with out-degrees (the
+                 * only ones the tail runs with), its loads issued before
+                 * the atomics (memory results return in order) */
+                const uint64_t with_count = stat_mode != GZ_STAT_OUTDEG_ONLY ? 1 : 0;
+                double od[kSlots][4];
+#pragma unroll
+                for (int b = 0; b < kSlots; ++b)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) od[b][k] = outdeg[d[b][k]];
+                uint64_t old[kSlots][4];
+                bool pass[kSlots][4];
+#pragma unroll
+                for (int b = 0; b < kSlots; ++b)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        pass[b][k] = gz_lane_valid(l[b][k]) && mm[b] < g[b][k];
+                        unsigned long long* at = pass[b][k] ? (unsigned long long*)&labels[d[b][k]] : sink + 4 * b + k;
+                        old[b][k] = atomicMin(at, pass[b][k] ? (unsigned long long)mm[b] : ~0ull);
+                    }
+                unsigned int prev[kSlots][4];
+                bool low[kSlots][4];
+#pragma unroll
+                for (int b = 0; b < kSlots; ++b)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        low[b][k] = pass[b][k] && old[b][k] > mm[b];
+                        const uint64_t st = with_count + (uint64_t)llrint(od[b][k]);
+                        my_vote += low[b][k] ? st : 0;
+                        unsigned int* w = low[b][k] ? reinterpret_cast<unsigned int*>(next_frontier) + (d[b][k] >> 5)
+                                                    : reinterpret_cast<unsigned int*>(sink + 4 * b + k);
+                        prev[b][k] = atomicOr(w, low[b][k] ? 1u << (d[b][k] & 31) : 0u);
+                    }
+#pragma unroll
+                for (int b = 0; b < kSlots; ++b)
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        if (!low[b][k] || ((prev[b][k] >> (d[b][k] & 31)) & 1u)) continue;
+                        const uint32_t at = atomicAdd(&next_count, 1u);
+                        if (at < GZ_CC_TAIL_CAP) lists[cur ^ 1u][at] = (uint32_t)d[b][k];
+                    }
+            }
+            __syncthreads(); /* the next chunk rewrites incl, first, msg, wave_tot */
+        }
+        const uint64_t wv = gz_wave_sum_u64(my_vote);
+        if (lane == 0 && wv) atomicAdd(&round_vote, (unsigned long long)wv);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        rounds += 1;
+        vote = round_vote;
+        const uint64_t listed = next_count;
+        if (vote == 0 || vote > vote_cap || listed > GZ_CC_TAIL_CAP || rounds >= max_rounds) break;
+        n = listed;
+        cur ^= 1u;
+        __syncthreads(); /* every wavefront has read round_vote and next_count */
+    }
+    if (tid == 0) {
+        out[0] = vote;
+        out[7] = rounds;
+    }
+}
+
+static uint64_t gz_cc_tail_vote = GZ_CC_TAIL_VOTE;
+
+extern "C" void gz_cc_push_tail(uint64_t max_vote)
+{
+    gz_cc_tail_vote = max_vote < GZ_CC_TAIL_CAP ? max_vote : GZ_CC_TAIL_CAP;
+}
+
+extern "C" int gz_cc_push_tail_applies(uint64_t vote)
+{
+    return gz_cc_tail_vote > 0 && vote <= gz_cc_tail_vote && gz_phase_config.push_frontier_skip &&
+           gz_graph.push_vertex_offsets != nullptr && gz_graph.push_vectors != nullptr &&
+           gz_phase_config.stat_mode != GZ_STAT_COUNT_ONLY && gz_graph.num_vertices > 0 &&
+           gz_graph.num_vertices < (1ull << 32) && gz_graph.push_vector_count < (1ull << 32);
+}
+
+extern "C" void gz_cc_launch_push_tail(uint64_t vote_cap, uint64_t max_rounds, uint64_t* reduce_buffer)
+{
+    const uint64_t V = gz_graph.num_vertices;
+    if (vote_cap > gz_cc_tail_vote) vote_cap = gz_cc_tail_vote;
+    if (max_rounds == 0) max_rounds = 1;
+    gz_scratch_reserve(0, V);
+    gz_frontier_list(gz_graph.frontier_has_info, gz_graph.push_vertex_offsets, V, gz_graph.push_vector_count);
+    gz_ktimer_begin(GZ_K_PUSH_CC);
+    hipLaunchKernelGGL(gz_k_cc_push_tail, dim3(1), dim3(GZ_CC_TAIL_THREADS), 0, gz_stream,
+                       (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators, gz_graph.push_vectors,
+                       gz_graph.push_vertex_offsets, gz_graph.vertex_outdegrees, gz_phase_config.stat_mode,
+                       gz_scratch.active_vertices, gz_scratch.active_totals, vote_cap, max_rounds, reduce_buffer);
+    GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_PUSH_CC);
+}
+
 /* Clear the HasInfo* words covering [vertex_start, vertex_start+count)
  * (phase_vertex_cc.asm:64-170 zeroes 512 vertices per step). */
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_clear_words(uint64_t* __restrict__ words, uint64_t first, uint64_t end,

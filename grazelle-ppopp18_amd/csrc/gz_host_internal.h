@@ -60,6 +60,13 @@ int gz_graph_replay_cached_slot(uint32_t slot, const uint64_t* key, size_t n, vo
 /* Frontier statistic of the sources the next CC push will read (the vote that
  * built its frontier); a small one lets the push skip the compaction. */
 void gz_cc_push_vote_hint(uint64_t vote);
+/* The one-workgroup push tail (gz_cc_push_tail): whether a frontier of
+ * statistic `vote` qualifies, and its launch, which compacts HasInfo, runs up
+ * to max_rounds push iterations while their statistic stays at most
+ * vote_cap, and writes the last vote to reduce_buffer[0] and the rounds run
+ * to reduce_buffer[7]. */
+int gz_cc_push_tail_applies(uint64_t vote);
+void gz_cc_launch_push_tail(uint64_t vote_cap, uint64_t max_rounds, uint64_t* reduce_buffer);
 /* Control words of a device-driven iteration loop (CC, and the log of
  * (engine, vote) pairs from GZ_LOOP_LOG on). */
 #define GZ_LOOP_ENGINE 0          /* engine of the iteration about to run: 0 stop, 1 pull, 2 push over the

@@ -108,6 +108,8 @@ _sig("gz_pr_persistent_runs", u64)
 _sig("gz_pr_persistent_spin_limit", None, u32)
 _sig("gz_pr_persistent_gave_up", u64)
 _sig("gz_cc_graph_replay", None, C.c_int)
+_sig("gz_cc_push_tail", None, u64)
+_sig("gz_cc_push_tail_rounds", u64)
 _sig("gz_graph_replay_counts", None, C.POINTER(u64), C.POINTER(u64))
 _sig("gz_frontier_compact_narrow", None, C.c_int)
 _sig("gz_graph_source_slices_info", C.c_int, vp, P(u64), P(u32), P(u32), P(u64))
@@ -482,6 +484,19 @@ def cc_graph_replay(on: bool) -> None:
     """CC iterations with the engine chosen on the device, from a captured graph (default off)."""
     init()
     lib.gz_cc_graph_replay(int(on))
+
+
+def cc_push_tail(max_vote: int = 4096) -> None:
+    """CC push iterations over frontiers of statistic <= max_vote run back to
+    back in one workgroup (default 4096, at most 16384; 0 turns it off)."""
+    init()
+    lib.gz_cc_push_tail(int(max_vote))
+
+
+def cc_push_tail_rounds() -> int:
+    """Push iterations the one-workgroup tail ran since the library was loaded."""
+    init()
+    return int(lib.gz_cc_push_tail_rounds())
 
 
 def compact_narrow(on: bool) -> None:

@@ -213,6 +213,16 @@ uint64_t gz_pr_persistent_gave_up(void);
  * is captured the second time in a row it runs; labels are the same either
  * way. */
 void gz_cc_graph_replay(int on);
+/* CC runs that record nothing per iteration, with the fixed threshold or a
+ * forced push, run their push iterations over a frontier whose statistic is
+ * at most `max_vote` (clamped to 16384; 0 switches it off) back to back in
+ * one workgroup, the round trip to the host for each vote saved (default
+ * 4096; no reference counterpart, execution_cc.c:124-142 returns to the
+ * host after each iteration).  Every edge gets the same update, so labels
+ * are the same; iterations keep their count, one per round. */
+void gz_cc_push_tail(uint64_t max_vote);
+/* Push iterations run that way since the library was loaded. */
+uint64_t gz_cc_push_tail_rounds(void);
 /* Graph captures and graph launches made so far (no reference counterpart). */
 void gz_graph_replay_counts(uint64_t* captures, uint64_t* launches);
 
