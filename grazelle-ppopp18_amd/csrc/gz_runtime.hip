@@ -526,6 +526,30 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_reduce(const T* __restrict__ in
     }
 }
 
+/* Integer partials (votes) up to GZ_REDUCE_ONE_MAX: one workgroup sums them.
+ * An integer sum does not depend on its order, so this gives gz_k_reduce's
+ * result without its second stage, release fence and ticket, which cost
+ * more than the loads at a few thousand partials. */
+#define GZ_REDUCE_ONE_MAX 65536
+#define GZ_REDUCE_ONE_THREADS 1024
+__global__ __launch_bounds__(GZ_REDUCE_ONE_THREADS) void gz_k_reduce_one_u64(const unsigned long long* __restrict__ in,
+                                                                             uint64_t count, uint64_t* __restrict__ out)
+{
+    constexpr uint64_t T = GZ_REDUCE_ONE_THREADS;
+    __shared__ unsigned long long lds[GZ_REDUCE_ONE_THREADS / GZ_WAVE];
+    unsigned long long x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+    uint64_t i = threadIdx.x;
+    for (; i + 3 * T < count; i += 4 * T) {
+        x0 += in[i];
+        x1 += in[i + T];
+        x2 += in[i + 2 * T];
+        x3 += in[i + 3 * T];
+    }
+    for (; i < count; i += T) x0 += in[i];
+    const unsigned long long r = gz_block_sum<unsigned long long>((x0 + x1) + (x2 + x3), lds);
+    if (threadIdx.x == 0) out[0] = r;
+}
+
 uint64_t gz_reduce_blocks(uint64_t count)
 {
     uint64_t blocks = (count + 4095) / 4096;
@@ -549,6 +573,10 @@ void gz_reduce_partials(uint64_t* dst)
     gz_ktimer_begin(GZ_K_REDUCE);
     if (gz_scratch.partials_is_double) {
         gz_reduce_f64_into((const double*)gz_scratch.partials, count, (double*)gz_scratch.reduce_tmp, done, dst);
+    } else if (count <= GZ_REDUCE_ONE_MAX) {
+        hipLaunchKernelGGL(gz_k_reduce_one_u64, dim3(1), dim3(GZ_REDUCE_ONE_THREADS), 0, gz_stream,
+                           (const unsigned long long*)gz_scratch.partials, count, dst);
+        GZ_LAUNCH_CHECK();
     } else {
         hipLaunchKernelGGL(gz_k_reduce<unsigned long long>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
                            (const unsigned long long*)gz_scratch.partials, count, (unsigned long long*)gz_scratch.reduce_tmp,

@@ -219,13 +219,40 @@ __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restri
 {
     __shared__ uint64_t red[4][1024 / GZ_WAVE];
     __shared__ float x[GZ_SEL_IN], xn[GZ_SEL_IN], h1[GZ_SEL_H], h2[GZ_SEL_H], y[GZ_SEL_OUT];
+    __shared__ float sp[GZ_SEL_PARAMS], snorm[2 * GZ_SEL_IN];
     const int j = threadIdx.x;
     const uint64_t t_end = (uint64_t)wall_clock64();
+    /* the weights and normalisation go to LDS first: their loads overlap the
+     * partial sums instead of following them */
+    if (have_model)
+        for (int i = j; i < GZ_SEL_PARAMS; i += (int)blockDim.x) sp[i] = p[i];
+    for (int i = j; i < 2 * GZ_SEL_IN; i += (int)blockDim.x) snorm[i] = norm[i];
     uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    for (uint64_t b = j; b < blocks; b += blockDim.x) {
-        s0 += partials[3 * (uint64_t)b];
-        s1 += partials[3 * (uint64_t)b + 1];
-        s2 += partials[3 * (uint64_t)b + 2];
+    /* four slots per step, so that sixteen loads per thread are in flight */
+    const uint64_t T = blockDim.x;
+    uint64_t b = j;
+    for (; b + 3 * T < blocks; b += 4 * T) {
+        uint64_t q[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t c = b + (uint64_t)u * T;
+            q[u][0] = partials[3 * c];
+            q[u][1] = partials[3 * c + 1];
+            q[u][2] = partials[3 * c + 2];
+            q[u][3] = votes ? votes[c] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            s0 += q[u][0];
+            s1 += q[u][1];
+            s2 += q[u][2];
+            s3 += q[u][3];
+        }
+    }
+    for (; b < blocks; b += T) {
+        s0 += partials[3 * b];
+        s1 += partials[3 * b + 1];
+        s2 += partials[3 * b + 2];
         if (votes) s3 += votes[b];
     }
     s0 = gz_wave_sum_u64(s0);
@@ -267,27 +294,27 @@ __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restri
         x[7] = num_vertices ? (float)num_edges / (float)num_vertices : 0.f;
         for (int i = 0; i < GZ_SEL_IN; ++i) {
             /* a feature with std 0 is unused by the model: its input is 0 */
-            const float sd = norm[GZ_SEL_IN + i];
-            xn[i] = sd > 0.f ? (x[i] - norm[i]) / sd : 0.f;
+            const float sd = snorm[GZ_SEL_IN + i];
+            xn[i] = sd > 0.f ? (x[i] - snorm[i]) / sd : 0.f;
         }
     }
     __syncthreads();
     if (have_model) {
         if (j < GZ_SEL_H) {
-            float a = p[O_B1 + j];
-            for (int i = 0; i < GZ_SEL_IN; ++i) a += p[O_W1 + j * GZ_SEL_IN + i] * xn[i];
+            float a = sp[O_B1 + j];
+            for (int i = 0; i < GZ_SEL_IN; ++i) a += sp[O_W1 + j * GZ_SEL_IN + i] * xn[i];
             h1[j] = a > 0.f ? a : 0.f;
         }
         __syncthreads();
         if (j < GZ_SEL_H) {
-            float a = p[O_B2 + j];
-            for (int i = 0; i < GZ_SEL_H; ++i) a += p[O_W2 + j * GZ_SEL_H + i] * h1[i];
+            float a = sp[O_B2 + j];
+            for (int i = 0; i < GZ_SEL_H; ++i) a += sp[O_W2 + j * GZ_SEL_H + i] * h1[i];
             h2[j] = a > 0.f ? a : 0.f;
         }
         __syncthreads();
         if (j < GZ_SEL_OUT) {
-            float a = p[O_B3 + j];
-            for (int i = 0; i < GZ_SEL_H; ++i) a += p[O_W3 + j * GZ_SEL_H + i] * h2[i];
+            float a = sp[O_B3 + j];
+            for (int i = 0; i < GZ_SEL_H; ++i) a += sp[O_W3 + j * GZ_SEL_H + i] * h2[i];
             y[j] = a;
         }
         __syncthreads();
