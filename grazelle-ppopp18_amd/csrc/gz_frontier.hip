@@ -82,6 +82,67 @@ __device__ __forceinline__ T gz_block_exclusive_scan(T x, T* lds, T& block_total
     return wave_off + incl - x;
 }
 
+/* ---- the scan-free form of the passes -------------------------------------
+ * The count pass also adds each block's totals to the counters of its
+ * segment: GZ_SEGMENTS ranges of seg_blocks consecutive blocks, whose
+ * counters sit on separate 256-byte lines so that their atomics proceed in
+ * parallel (4096 workgroup atomics on one word take about 50 us on MI355X;
+ * spread over 64 lines, no longer than the launch).  The write pass then finds
+ * its block's position from the counters of the segments before it and the
+ * block sums of its own segment before it, which replaces the scan kernel.
+ * The first block of the write pass also writes the totals and the prefix
+ * sentinel, and zeroes the other bank of counters, which the next compaction
+ * counts into (the compaction before this one read that bank, and its write
+ * pass has finished: launches on one stream run in order). */
+#define GZ_SEGMENTS 64
+#define GZ_SEG_STRIDE 32  /* words between counters */
+#define GZ_SEG_VECTORS 16 /* a segment's vector counter follows its entry counter, 128 bytes on */
+
+__device__ __forceinline__ void gz_seg_add(unsigned long long* seg, uint64_t s, unsigned long long entries,
+                                           unsigned long long vectors)
+{
+    atomicAdd(&seg[s * GZ_SEG_STRIDE], entries);
+    if (vectors != 0) atomicAdd(&seg[s * GZ_SEG_STRIDE + GZ_SEG_VECTORS], vectors);
+}
+
+/* Entries and vectors of the blocks before block b.  Wavefront-collective. */
+__device__ __forceinline__ void gz_seg_base(const unsigned long long* __restrict__ seg,
+                                            const uint64_t* __restrict__ block_sums, uint64_t b, uint64_t seg_blocks,
+                                            uint64_t& entries, uint64_t& vectors)
+{
+    const int lane = gz_lane();
+    const uint64_t s = b / seg_blocks;
+    uint64_t e = 0, v = 0;
+    if ((uint64_t)lane < s) {
+        e = seg[(uint64_t)lane * GZ_SEG_STRIDE];
+        v = seg[(uint64_t)lane * GZ_SEG_STRIDE + GZ_SEG_VECTORS];
+    }
+    for (uint64_t q = s * seg_blocks + (uint64_t)lane; q < b; q += GZ_WAVE) {
+        e += block_sums[2 * q];
+        v += block_sums[2 * q + 1];
+    }
+    entries = gz_wave_sum_u64(e);
+    vectors = gz_wave_sum_u64(v);
+}
+
+/* First block of a write pass: totals, the sentinel prefix[entries] =
+ * vectors, and the other bank cleared.  Run by wavefront 0 of block 0. */
+__device__ __forceinline__ void gz_seg_finish(const unsigned long long* __restrict__ seg,
+                                              unsigned long long* __restrict__ other, uint64_t* __restrict__ totals,
+                                              uint64_t* __restrict__ prefix)
+{
+    const int lane = gz_lane();
+    const uint64_t e = gz_wave_sum_u64(seg[(uint64_t)lane * GZ_SEG_STRIDE]);
+    const uint64_t v = gz_wave_sum_u64(seg[(uint64_t)lane * GZ_SEG_STRIDE + GZ_SEG_VECTORS]);
+    other[(uint64_t)lane * GZ_SEG_STRIDE] = 0;
+    other[(uint64_t)lane * GZ_SEG_STRIDE + GZ_SEG_VECTORS] = 0;
+    if (lane == 0) {
+        totals[0] = e;
+        totals[1] = v;
+        if (prefix != nullptr) prefix[e] = v;
+    }
+}
+
 /* Pass 1: per block of GZ_COMPACT_VPB vertices, its active vertices and
  * their vectors.  Workgroups stride over the `nblocks` vertex blocks. */
 template <bool kNarrow>
@@ -90,7 +151,8 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_count(const uint64_t* 
                                                                  uint64_t num_vertices,
                                                                  uint64_t* __restrict__ block_sums,
                                                                  uint64_t* __restrict__ stamp, uint64_t nblocks,
-                                                                 gz_guard guard)
+                                                                 gz_guard guard, unsigned long long* seg,
+                                                                 uint64_t seg_blocks)
 {
     __shared__ unsigned long long lds[GZ_WAVES_PER_BLOCK];
     if (gz_guard_off(guard)) return;
@@ -111,6 +173,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_count(const uint64_t* 
         if (threadIdx.x == 0) {
             block_sums[2 * b] = tv;
             block_sums[2 * b + 1] = tvec;
+            if (seg != nullptr && tv != 0) gz_seg_add(seg, b / seg_blocks, tv, tvec);
         }
     }
 }
@@ -212,10 +275,13 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_write(const uint64_t* 
                                                                  const uint64_t* __restrict__ block_sums,
                                                                  uint64_t* __restrict__ list,
                                                                  uint64_t* __restrict__ prefix, uint64_t nblocks,
-                                                                 gz_guard guard)
+                                                                 gz_guard guard, const unsigned long long* seg,
+                                                                 unsigned long long* seg_other, uint64_t seg_blocks,
+                                                                 uint64_t* __restrict__ totals)
 {
     __shared__ unsigned long long lds[GZ_WAVES_PER_BLOCK];
     if (gz_guard_off(guard)) return;
+    if (seg != nullptr && blockIdx.x == 0 && threadIdx.x < GZ_WAVE) gz_seg_finish(seg, seg_other, totals, prefix);
     for (uint64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
         const uint64_t v0 = b * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
         uint64_t c[GZ_COMPACT_VPT];
@@ -226,9 +292,16 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_write(const uint64_t* 
             nv += c[k] ? 1 : 0;
             nvec += c[k];
         }
+        uint64_t pos0, vpos0;
+        if (seg != nullptr)
+            gz_seg_base(seg, block_sums, b, seg_blocks, pos0, vpos0);
+        else {
+            pos0 = block_sums[2 * b];
+            vpos0 = block_sums[2 * b + 1];
+        }
         unsigned long long tv, tvec;
-        uint64_t pos = block_sums[2 * b] + gz_block_exclusive_scan<unsigned long long>(nv, lds, tv);
-        uint64_t vpos = block_sums[2 * b + 1] + gz_block_exclusive_scan<unsigned long long>(nvec, lds, tvec);
+        uint64_t pos = pos0 + gz_block_exclusive_scan<unsigned long long>(nv, lds, tv);
+        uint64_t vpos = vpos0 + gz_block_exclusive_scan<unsigned long long>(nvec, lds, tvec);
 #pragma unroll
         for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
             if (c[k]) {
@@ -259,7 +332,8 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_list_count(const uint6
                                                                      const uint64_t* __restrict__ hasvec,
                                                                      uint64_t num_vertices,
                                                                      uint64_t* __restrict__ block_sums,
-                                                                     uint64_t* __restrict__ stamp, uint64_t nblocks)
+                                                                     uint64_t* __restrict__ stamp, uint64_t nblocks,
+                                                                     unsigned long long* seg, uint64_t seg_blocks)
 {
     __shared__ unsigned long long lds[GZ_WAVES_PER_BLOCK];
     if (stamp != nullptr && blockIdx.x == 0 && threadIdx.x == 0) *stamp = (uint64_t)wall_clock64();
@@ -271,6 +345,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_list_count(const uint6
         if (threadIdx.x == 0) {
             block_sums[2 * b] = tv;
             block_sums[2 * b + 1] = 0;
+            if (seg != nullptr && tv != 0) gz_seg_add(seg, b / seg_blocks, tv, 0);
         }
     }
 }
@@ -279,15 +354,24 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_list_write(const uint6
                                                                      const uint64_t* __restrict__ hasvec,
                                                                      uint64_t num_vertices,
                                                                      const uint64_t* __restrict__ block_sums,
-                                                                     uint64_t* __restrict__ list, uint64_t nblocks)
+                                                                     uint64_t* __restrict__ list, uint64_t nblocks,
+                                                                     const unsigned long long* seg,
+                                                                     unsigned long long* seg_other, uint64_t seg_blocks,
+                                                                     uint64_t* __restrict__ totals,
+                                                                     uint64_t* __restrict__ prefix)
 {
     __shared__ unsigned long long lds[GZ_WAVES_PER_BLOCK];
+    if (seg != nullptr && blockIdx.x == 0 && threadIdx.x < GZ_WAVE) gz_seg_finish(seg, seg_other, totals, prefix);
     for (uint64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
         const uint64_t v0 = b * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
         uint32_t bits = gz_list_bits(bitmap, hasvec, num_vertices, v0);
+        uint64_t pos0, unused;
+        if (seg != nullptr)
+            gz_seg_base(seg, block_sums, b, seg_blocks, pos0, unused);
+        else
+            pos0 = block_sums[2 * b];
         unsigned long long tv;
-        uint64_t pos = block_sums[2 * b] +
-                       gz_block_exclusive_scan<unsigned long long>((unsigned long long)__popc(bits), lds, tv);
+        uint64_t pos = pos0 + gz_block_exclusive_scan<unsigned long long>((unsigned long long)__popc(bits), lds, tv);
         while (bits) {
             const int k = __ffs(bits) - 1;
             bits &= bits - 1u;
@@ -403,6 +487,32 @@ static const uint32_t* gz_frontier_counts(const uint64_t* offsets, uint64_t num_
     return e.counts;
 }
 
+/* Two banks of segment counters for the scan-free passes. */
+static unsigned long long* gz_seg_banks = nullptr;
+static int gz_seg_parity = 0;
+
+/* The bank the next scan-free compaction counts into and the one its write
+ * pass clears, then the parity advances.  False while the stream is being
+ * captured: a graph would replay one parity every time, so captured
+ * compactions keep the scan kernel. */
+static bool gz_seg_take(unsigned long long** bank, unsigned long long** other)
+{
+    hipStreamCaptureStatus status = hipStreamCaptureStatusNone;
+    GZ_CHECK(hipStreamIsCapturing(gz_stream, &status));
+    if (status != hipStreamCaptureStatusNone) return false;
+    const size_t words = (size_t)GZ_SEGMENTS * GZ_SEG_STRIDE;
+    if (gz_seg_banks == nullptr) {
+        gz_seg_banks = (unsigned long long*)gz_device_malloc(2 * words * sizeof(uint64_t));
+        gz_device_zero(gz_seg_banks, 2 * words * sizeof(uint64_t));
+    }
+    *bank = gz_seg_banks + (size_t)gz_seg_parity * words;
+    *other = gz_seg_banks + (size_t)(gz_seg_parity ^ 1) * words;
+    gz_seg_parity ^= 1;
+    return true;
+}
+
+static uint64_t gz_seg_blocks(uint64_t nblocks) { return (nblocks + GZ_SEGMENTS - 1) / GZ_SEGMENTS; }
+
 static void gz_frontier_compact_impl(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
                                      uint64_t vector_count, gz_guard guard, uint64_t max_grid, bool allow_build,
                                      const gz_compact_out& out)
@@ -427,23 +537,31 @@ static void gz_frontier_compact_impl(const uint64_t* bitmap, const uint64_t* off
     }
     uint64_t* stamp = guard.word == nullptr ? gz_scratch.stamp_target : nullptr;
     if (guard.word == nullptr) gz_scratch.stamp_target = nullptr;
+    /* guarded passes are captured into the device loop's graph */
+    unsigned long long *seg = nullptr, *other = nullptr;
+    if (guard.word == nullptr && !gz_seg_take(&seg, &other)) seg = other = nullptr;
+    const uint64_t seg_blocks = gz_seg_blocks(blocks);
     gz_ktimer_begin(GZ_K_COMPACT);
     if (narrow)
         hipLaunchKernelGGL(gz_k_frontier_count<true>, dim3((unsigned)grid), dim3(GZ_BLOCK), 0, gz_stream, bitmap, src,
-                           num_vertices, out.block_sums, stamp, blocks, guard);
+                           num_vertices, out.block_sums, stamp, blocks, guard, seg, seg_blocks);
     else
         hipLaunchKernelGGL(gz_k_frontier_count<false>, dim3((unsigned)grid), dim3(GZ_BLOCK), 0, gz_stream, bitmap,
-                           src, num_vertices, out.block_sums, stamp, blocks, guard);
+                           src, num_vertices, out.block_sums, stamp, blocks, guard, seg, seg_blocks);
     GZ_LAUNCH_CHECK();
-    hipLaunchKernelGGL(gz_k_frontier_scan, dim3(1), dim3(1024), 0, gz_stream, out.block_sums, blocks, out.totals,
-                       out.prefix, guard);
-    GZ_LAUNCH_CHECK();
+    if (seg == nullptr) {
+        hipLaunchKernelGGL(gz_k_frontier_scan, dim3(1), dim3(1024), 0, gz_stream, out.block_sums, blocks, out.totals,
+                           out.prefix, guard);
+        GZ_LAUNCH_CHECK();
+    }
     if (narrow)
         hipLaunchKernelGGL(gz_k_frontier_write<true>, dim3((unsigned)grid), dim3(GZ_BLOCK), 0, gz_stream, bitmap, src,
-                           num_vertices, out.block_sums, out.list, out.prefix, blocks, guard);
+                           num_vertices, out.block_sums, out.list, out.prefix, blocks, guard, seg, other, seg_blocks,
+                           out.totals);
     else
         hipLaunchKernelGGL(gz_k_frontier_write<false>, dim3((unsigned)grid), dim3(GZ_BLOCK), 0, gz_stream, bitmap,
-                           src, num_vertices, out.block_sums, out.list, out.prefix, blocks, guard);
+                           src, num_vertices, out.block_sums, out.list, out.prefix, blocks, guard, seg, other,
+                           seg_blocks, out.totals);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_COMPACT);
 }
@@ -471,15 +589,21 @@ void gz_frontier_list(const uint64_t* bitmap, const uint64_t* offsets, uint64_t 
     const uint64_t* hasvec = gz_frontier_hasvec(offsets, num_vertices, vector_count);
     uint64_t* stamp = gz_scratch.stamp_target;
     gz_scratch.stamp_target = nullptr;
+    unsigned long long *seg = nullptr, *other = nullptr;
+    if (!gz_seg_take(&seg, &other)) seg = other = nullptr;
+    const uint64_t seg_blocks = gz_seg_blocks(blocks);
     gz_ktimer_begin(GZ_K_COMPACT);
     hipLaunchKernelGGL(gz_k_frontier_list_count, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, hasvec,
-                       num_vertices, gz_scratch.active_block_sums, stamp, blocks);
+                       num_vertices, gz_scratch.active_block_sums, stamp, blocks, seg, seg_blocks);
     GZ_LAUNCH_CHECK();
-    hipLaunchKernelGGL(gz_k_frontier_scan, dim3(1), dim3(1024), 0, gz_stream, gz_scratch.active_block_sums, blocks,
-                       gz_scratch.active_totals, gz_scratch.active_prefix, gz_guard{nullptr, 0});
-    GZ_LAUNCH_CHECK();
+    if (seg == nullptr) {
+        hipLaunchKernelGGL(gz_k_frontier_scan, dim3(1), dim3(1024), 0, gz_stream, gz_scratch.active_block_sums, blocks,
+                           gz_scratch.active_totals, gz_scratch.active_prefix, gz_guard{nullptr, 0});
+        GZ_LAUNCH_CHECK();
+    }
     hipLaunchKernelGGL(gz_k_frontier_list_write, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, bitmap, hasvec,
-                       num_vertices, gz_scratch.active_block_sums, gz_scratch.active_vertices, blocks);
+                       num_vertices, gz_scratch.active_block_sums, gz_scratch.active_vertices, blocks, seg, other,
+                       seg_blocks, gz_scratch.active_totals, gz_scratch.active_prefix);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_COMPACT);
 }

@@ -333,6 +333,35 @@ def test_compaction_counts_and_offsets(gz, orc, device, rmat14, narrow):
         gz.compact_narrow(True)
 
 
+def test_compaction_many_segments(gz, orc, device):
+    # 2^25 + 12,345 vertices (8,195 blocks of 4,096, so 129 blocks per
+    # segment of the scan-free count and write passes) holding a star of
+    # 150,000 leaves spread over every segment, and a 10-vertex path behind
+    # one leaf: the root's level and the leaves' level compact (statistics
+    # above the small-frontier bound), the hybrid pulls over the unvisited
+    # list, and every depth, parent and statistic must equal the oracle's
+    V = (1 << 25) + 12345
+    rng = np.random.default_rng(17)
+    ids = rng.choice(V, size=150_000 + 11, replace=False).astype(np.uint64)
+    root, leaves, path = ids[0], ids[1:150_001], ids[150_001:]
+    pairs = [np.stack([np.full(leaves.size, root, dtype=np.uint64), leaves], axis=1),
+             np.asarray([(leaves[7], path[0])] + [(path[i], path[i + 1]) for i in range(path.size - 1)],
+                        dtype=np.uint64)]
+    g = gz.make_graph(np.concatenate(pairs), V, symmetrize=True)
+    G = _orc_graph(orc, g)
+    gz.load(g)
+    for engine in ("push", "hybrid"):
+        gz.run(gz.options("bfs", engine=engine, root=int(root), profile=True))
+        par, dep, log, _ = orc.bfs(G, orc.options(engine=engine, root=int(root)))
+        p, d = gz.bfs_result()
+        assert np.array_equal(d, dep), engine
+        assert np.array_equal(p, par), engine
+        assert [(x.engine, x.frontier_stat) for x in gz.records()] == log, engine
+    gz.run(gz.options("cc", engine="push"))
+    lab, _, _ = orc.cc(G, orc.options(engine="push"))
+    assert np.array_equal(gz.cc_labels(), lab)
+
+
 def test_cc_directed_rmat(gz, orc, device, rmat14):
     gz.load(rmat14)
     gz.run(gz.options("cc"))
