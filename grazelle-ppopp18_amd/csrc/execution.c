@@ -229,12 +229,16 @@ static double gz_probe_engine(int bfs, int pull, const gz_snapshot_t* s, gz_time
 
 static int gz_probe_wide_on = 0;
 
-/* The learned engine decides only while the vote is within a factor of
- * gz_sel_trust of the fixed threshold; further out, the fixed rule's choice
- * stands (0: the model decides every level). */
-static double gz_sel_trust = 4.0;
+/* The learned engine decides only while threshold / gz_sel_below <= vote <=
+ * gz_sel_above * threshold; outside, the fixed rule's choice stands (pull
+ * above, push below).  A factor of 0 lifts that side of the band. */
+static double gz_sel_above = 1.0, gz_sel_below = 32.0;
 
-void gz_selector_set_trust(double k) { gz_sel_trust = (k >= 1.0) ? k : 0.0; }
+void gz_selector_set_trust(double above, double below)
+{
+    gz_sel_above = (above >= 1.0) ? above : 0.0;
+    gz_sel_below = (below >= 1.0) ? below : 0.0;
+}
 
 void gz_probe_wide(int on) { gz_probe_wide_on = on ? 1 : 0; }
 
@@ -595,12 +599,10 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
             use_pull = 0;
         else if (learned) {
             use_pull = (sel.choice == 1);
-            if (gz_sel_trust > 0.0) {
-                if ((double)vote > gz_sel_trust * (double)threshold)
-                    use_pull = 1;
-                else if ((double)vote * gz_sel_trust < (double)threshold)
-                    use_pull = 0;
-            }
+            if (gz_sel_above > 0.0 && (double)vote > gz_sel_above * (double)threshold)
+                use_pull = 1;
+            else if (gz_sel_below > 0.0 && (double)vote * gz_sel_below < (double)threshold)
+                use_pull = 0;
         }
         else
             use_pull = vote > threshold;

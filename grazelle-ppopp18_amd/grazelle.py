@@ -145,7 +145,7 @@ _sig("gz_selector_load_file", C.c_int, C.c_char_p)
 _sig("gz_selector_save_file", C.c_int, C.c_char_p)
 _sig("gz_selector_loaded", C.c_int)
 _sig("gz_selector_feature_used", C.c_int, u32)
-_sig("gz_selector_set_trust", None, C.c_double)
+_sig("gz_selector_set_trust", None, C.c_double, C.c_double)
 _sig("gz_selector_forward_batch", C.c_int, vp, vp, vp, u64)
 _sig("gz_selector_gradient", C.c_float, vp, vp, vp, u64, vp)
 _sig("gz_selector_train", C.c_int, vp, vp, vp, u64, u32, C.c_float, vp)
@@ -746,10 +746,15 @@ def selector_save(path: str) -> None:
         raise OSError(f"cannot write selector weights to {path} ({rc})")
 
 
-def selector_set_trust(k: float) -> None:
-    """The learned engine follows the model only within a factor k of the fixed threshold (default 4; < 1: always)."""
+TRUST_ABOVE, TRUST_BELOW = 1.0, 32.0  # the engine's default band
+
+
+def selector_set_trust(above: float, below: float | None = None) -> None:
+    """The learned engine follows the model only while threshold/below <=
+    vote <= above*threshold (defaults 1 and 32; a factor < 1 lifts that side;
+    one factor sets both)."""
     init()
-    lib.gz_selector_set_trust(float(k))
+    lib.gz_selector_set_trust(float(above), float(above if below is None else below))
 
 
 def selector_feature_used(i: int) -> bool:

@@ -174,10 +174,10 @@ def fit(data: dict, steps: int = 6000, lr: float = 1e-3, seed: int = 7, per_edge
     return params, mean.astype(np.float32), std.astype(np.float32), hist
 
 
-TRUST = 4.0  # the engine's default band (gz_selector_set_trust)
+TRUST = (gz.TRUST_ABOVE, gz.TRUST_BELOW)  # the engine's default band (gz_selector_set_trust)
 
 
-def policy_times(data: dict, params=None, mean=None, std=None, trust: float = TRUST) -> dict:
+def policy_times(data: dict, params=None, mean=None, std=None, trust: tuple = TRUST) -> dict:
     """Total edge-phase ms of the fixed threshold, the learned selector and
     the per-level best, summed over the probed runs.  The learned policy sees
     its own previous choice and time (features 5 and 6): where it chose what
@@ -185,7 +185,8 @@ def policy_times(data: dict, params=None, mean=None, std=None, trust: float = TR
     that run) is exact; otherwise the other engine's time plus the same
     clock overhead stands in for it.  As in the engine, the model decides
     only while the level's vote (vertices plus out-edges of the frontier,
-    features 0 and 1) is within a factor `trust` of E/5."""
+    features 0 and 1) lies in [E/5 / below, above * E/5], with (above, below)
+    = `trust`; a factor below 1 lifts that side."""
     X = data["features"].astype(np.float32)
     tp, ts = data["pull_ms"], data["push_ms"]
     fixed = np.where(data["fixed"] == 1, tp, ts)
@@ -207,13 +208,13 @@ def policy_times(data: dict, params=None, mean=None, std=None, trust: float = TR
                 x[6] = np.log1p(prev_ms)
                 y = gz.selector_forward(params, normalize(x[None, :], mean, std))[0]
                 pull = bool(y[0] < y[1])
-                if trust >= 1.0:
-                    vote = float(np.expm1(X[i][0]) + np.expm1(X[i][1]))
-                    threshold = float(data["edges"][i] // 5)
-                    if vote > trust * threshold:
-                        pull = True
-                    elif vote * trust < threshold:
-                        pull = False
+                above, below = trust
+                vote = float(np.expm1(X[i][0]) + np.expm1(X[i][1]))
+                threshold = float(data["edges"][i] // 5)
+                if above >= 1.0 and vote > above * threshold:
+                    pull = True
+                elif below >= 1.0 and vote * below < threshold:
+                    pull = False
                 t = tp[i] if pull else ts[i]
                 total += t
                 chose_fixed = (1 if pull else 2) == int(data["fixed"][i])

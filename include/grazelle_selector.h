@@ -40,11 +40,14 @@ int gz_selector_loaded(void);
  * per-vertex in-degree reads that feed it. */
 int gz_selector_feature_used(uint32_t i);
 /* The learned engine follows the model only while the frontier statistic
- * lies within a factor k of the fixed threshold (E/5, or V/2 with counts
- * only); beyond it the fixed rule's choice stands, so that a model is not
- * trusted far outside the frontiers of its training graphs.  Default 4;
- * k < 1 lets the model decide every level. */
-void gz_selector_set_trust(double k);
+ * lies in [threshold / below, above * threshold], the threshold being the
+ * fixed rule's (E/5, or V/2 with counts only); outside, the fixed rule's
+ * choice stands (pull above, push below), so that a model is not trusted
+ * where its training graphs gave it no reason to differ from the rule.
+ * Defaults 1 and 32: on the training graphs the push never beat the pull
+ * above the threshold, and the pull often beat the push down to about a
+ * 32nd of it.  A factor below 1 lifts that side of the band. */
+void gz_selector_set_trust(double above, double below);
 
 /* Weights: GZ_SEL_PARAMS floats plus feature normalisation (mean[8], std[8]). */
 int gz_selector_set_weights(const float* params, const float* feat_mean, const float* feat_std);

@@ -203,8 +203,9 @@ def test_unused_feature_skips_its_inputs(gz, orc, device, rmat13, tmp_path):
 @pytest.mark.parametrize("prefer", ["push", "pull"])
 def test_trust_band_bounds_the_model(gz, orc, device, rmat13, tmp_path, prefer):
     # a model that always prefers one engine (zero weights, output biases
-    # +-10): inside the band the engine follows it, outside (vote beyond a
-    # factor k of E/5) the fixed rule's choice stands; depths stay exact
+    # +-10): inside the band [E/5 / below, above * E/5] the engine follows
+    # it, outside the fixed rule's choice stands (pull above, push below);
+    # depths stay exact
     import selector_train as st
     params = np.zeros(gz.SEL_PARAMS, dtype=np.float32)
     params[-2:] = [10.0, -10.0] if prefer == "push" else [-10.0, 10.0]
@@ -217,21 +218,22 @@ def test_trust_band_bounds_the_model(gz, orc, device, rmat13, tmp_path, prefer):
     model = 2 if prefer == "push" else 1
     gz.selector_load(str(path))
     try:
-        for k in (4.0, 1.5, 0.0):
-            gz.selector_set_trust(k)
+        for above, below in ((gz.TRUST_ABOVE, gz.TRUST_BELOW), (4.0, 4.0), (1.5, 1.5), (1.0, 0.0), (0.0, 8.0),
+                             (0.0, 0.0)):
+            gz.selector_set_trust(above, below)
             for root in (0, 3):
                 gz.run(gz.options("bfs", engine="learned", root=root, profile=True))
                 _, dep, _, _ = orc.bfs(G, orc.options(root=root))
                 assert np.array_equal(gz.bfs_result()[1], dep)
                 for r in gz.records():
                     want = model
-                    if k >= 1.0 and r.frontier_stat > k * thr:
+                    if above >= 1.0 and r.frontier_stat > above * thr:
                         want = 1
-                    elif k >= 1.0 and r.frontier_stat * k < thr:
+                    elif below >= 1.0 and r.frontier_stat * below < thr:
                         want = 2
-                    assert r.engine == want, (k, root, r.iteration, r.frontier_stat, thr)
+                    assert r.engine == want, (above, below, root, r.iteration, r.frontier_stat, thr)
     finally:
-        gz.selector_set_trust(4.0)
+        gz.selector_set_trust(gz.TRUST_ABOVE, gz.TRUST_BELOW)
         gz.selector_clear()
 
 
