@@ -560,13 +560,16 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
     if (features) {
         /* features of the first iteration: the initial frontier */
         const uint64_t root = bfs ? gz_graph_device_vertex(o->root) : V;
-        gz_sel_dev_begin(V, E, root, indeg);
-        gz_sel_dev_step(gz_graph.frontier_has_info, indeg, gz_reduce_buffer, learned, bfs, 1, 0, 0.0f,
-                        bfs ? GZ_SEL_FROM_ROOT : GZ_SEL_FROM_BITMAP, 0);
-        if (learned && bfs && root < V) {
+        const int host_first = learned && bfs && root < V;
+        /* with the first decision on the host, the device counts start past
+         * the root and no first step runs */
+        gz_sel_dev_begin(V, E, root, indeg, host_first);
+        if (!host_first)
+            gz_sel_dev_step(gz_graph.frontier_has_info, indeg, gz_reduce_buffer, learned, bfs, 1, 0, 0.0f,
+                            bfs ? GZ_SEL_FROM_ROOT : GZ_SEL_FROM_BITMAP, 0);
+        if (host_first) {
             /* the first frontier is the root alone, so the host knows the
-             * features and takes the first decision without waiting for the
-             * device step (which computes the same and advances its counts) */
+             * features and takes the first decision itself */
             const double* od = gz_graph_outdegrees_host();
             sel.features[0] = logf(1.f + 1.f);
             sel.features[1] = logf(1.f + (float)(uint64_t)od[root]);

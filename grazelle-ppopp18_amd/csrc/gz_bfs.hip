@@ -123,8 +123,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_push_bfs(const gz_vec_t* _
                 const unsigned long long old = atomicMin((unsigned long long*)&words[d], key);
                 if (old == GZ_BFS_UNVISITED) {
                     gz_bit_set(next_frontier, d);
-                    vote += gz_frontier_stat(outdeg, d, stat_mode);
-                    gz_feature_add(feat, sink, outdeg, d);
+                    vote += gz_activate_stat(feat, sink, outdeg, d, stat_mode);
                 }
             }
         }
@@ -168,8 +167,7 @@ struct gz_bfs_push_op {
         const unsigned long long old = atomicMin((unsigned long long*)&words[d], (unsigned long long)key);
         if (old == GZ_BFS_UNVISITED) {
             gz_bit_set(next_frontier, d);
-            vote += gz_frontier_stat(outdeg, d, stat_mode);
-            gz_feature_add(feat, sink, outdeg, d);
+            vote += gz_activate_stat(feat, sink, outdeg, d, stat_mode);
         }
     }
     __device__ __forceinline__ void finish(uint64_t wave, const gz_feature_acc& feat) const
@@ -240,8 +238,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_first(const gz_ve
             if (parent != ~0ull) {
                 found = true;
                 words[d] = ((uint64_t)level << 48) | parent;
-                vote += gz_frontier_stat(outdeg, d, stat_mode);
-                gz_feature_add(feat, sink, outdeg, d);
+                vote += gz_activate_stat(feat, sink, outdeg, d, stat_mode);
             } else {
                 pend = src.offsets[d + 1] - vi > 1;
             }
@@ -399,8 +396,7 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_visit(const gz_ve
                 }
             }
             words[d] = ((uint64_t)level << 48) | parent;
-            vote += gz_frontier_stat(outdeg, d, stat_mode);
-            gz_feature_add(feat, sink, outdeg, d);
+            vote += gz_activate_stat(feat, sink, outdeg, d, stat_mode);
         }
         const uint64_t word = found ? (d >> 5) : ~0ull;
         uint32_t bits = found ? (1u << (d & 31)) : 0u;

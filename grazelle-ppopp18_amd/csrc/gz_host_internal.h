@@ -142,7 +142,8 @@ const uint64_t* gz_reduce_words(void);
 /* Learned selector on the device (selector_kernels.hip). */
 enum { GZ_SEL_FROM_BITMAP = 0, GZ_SEL_FROM_EDGE_PHASE = 1, GZ_SEL_FROM_ROOT = 2 };
 /* root < V stages a one-vertex initial frontier (BFS); pass V for none. */
-void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges, uint64_t root, const uint32_t* indeg);
+void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges, uint64_t root, const uint32_t* indeg,
+                      int root_removed);
 void gz_sel_dev_stamp(int by_compaction);
 void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg, uint64_t* reduce_buffer, int have_model,
                      int bfs, uint64_t iteration, uint32_t prev_engine, float prev_ms, int source, int with_vote);

@@ -395,14 +395,22 @@ struct gz_feature_acc {
     uint64_t count, out_edges, in_edges;
 };
 
-__device__ __forceinline__ void gz_feature_add(gz_feature_acc& f, const gz_feature_sink& s,
-                                               const double* __restrict__ outdeg, uint64_t v)
+/* One activated vertex: its frontier statistic (gz_frontier_stat) and, when
+ * the sink is on, (1, out-degree, in-degree) added to the lane's feature
+ * counters, with the out-degree read once. */
+__device__ __forceinline__ uint64_t gz_activate_stat(gz_feature_acc& f, const gz_feature_sink& s,
+                                                     const double* __restrict__ outdeg, uint64_t v, uint32_t mode)
 {
+    const double od = (s.out != nullptr || mode != GZ_STAT_COUNT_ONLY) ? outdeg[v] : 0.0;
+    uint64_t st = 0;
+    if (mode != GZ_STAT_OUTDEG_ONLY) st += 1;
+    if (mode != GZ_STAT_COUNT_ONLY) st += (uint64_t)llrint(od);
     if (s.out) {
         f.count += 1;
-        f.out_edges += (uint64_t)outdeg[v];
+        f.out_edges += (uint64_t)od;
         f.in_edges += s.indeg ? (uint64_t)s.indeg[v] : 0ull;
     }
+    return st;
 }
 
 /* Lane `src`'s value of x in every lane. */

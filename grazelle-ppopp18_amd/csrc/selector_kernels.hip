@@ -209,13 +209,19 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_stats(const uint64_t* __res
 /* out[1] = decision (1 pull, 2 push, 0 no model), out[2..5] = the raw
  * features as 8 floats, out[6] = the two outputs as floats.  The unvisited
  * counts advance by the frontier (BFS marks each vertex once). */
+/* out_from_vote: 1 when the vote counts every activated vertex plus its
+ * out-degree (stat mode both), so out-edges = vote - vertices; 2 when it
+ * counts out-degrees only (out-edges = vote); 0: read the out-edge partials.
+ * with_in: read the in-edge partials (the folds write 0 there without
+ * in-degrees). */
 __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restrict__ partials, uint64_t blocks,
                                                             const uint64_t* __restrict__ votes,
                                                             gz_sel_dev_state* st, const float* __restrict__ p,
                                                             const float* __restrict__ norm, int have_model, int bfs,
                                                             uint64_t num_vertices, uint64_t num_edges,
                                                             float iteration, float prev_engine, float prev_ms,
-                                                            float clock_khz, uint64_t* __restrict__ out)
+                                                            float clock_khz, uint64_t* __restrict__ out,
+                                                            int out_from_vote, int with_in)
 {
     __shared__ uint64_t red[4][1024 / GZ_WAVE];
     __shared__ float x[GZ_SEL_IN], xn[GZ_SEL_IN], h1[GZ_SEL_H], h2[GZ_SEL_H], y[GZ_SEL_OUT];
@@ -237,8 +243,8 @@ __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restri
         for (int u = 0; u < 4; ++u) {
             const uint64_t c = b + (uint64_t)u * T;
             q[u][0] = partials[3 * c];
-            q[u][1] = partials[3 * c + 1];
-            q[u][2] = partials[3 * c + 2];
+            q[u][1] = out_from_vote ? 0 : partials[3 * c + 1];
+            q[u][2] = with_in ? partials[3 * c + 2] : 0;
             q[u][3] = votes ? votes[c] : 0;
         }
 #pragma unroll
@@ -251,8 +257,8 @@ __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restri
     }
     for (; b < blocks; b += T) {
         s0 += partials[3 * b];
-        s1 += partials[3 * b + 1];
-        s2 += partials[3 * b + 2];
+        if (!out_from_vote) s1 += partials[3 * b + 1];
+        if (with_in) s2 += partials[3 * b + 2];
         if (votes) s3 += votes[b];
     }
     s0 = gz_wave_sum_u64(s0);
@@ -275,6 +281,8 @@ __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restri
             vote += red[3][i];
         }
         if (votes) out[0] = vote;
+        if (out_from_vote == 1) ce = vote - cv;
+        if (out_from_vote == 2) ce = vote;
         uint64_t uv = num_vertices, ue = num_edges;
         if (bfs) {
             st->unvisited -= cv;
@@ -355,13 +363,16 @@ static uint64_t gz_sel_stats_blocks(uint64_t words)
 
 __global__ void gz_k_sel_init(gz_sel_dev_state* st, uint64_t num_vertices, uint64_t num_edges, uint64_t root,
                               const double* __restrict__ outdeg, const uint32_t* __restrict__ indeg,
-                              uint64_t* __restrict__ partials)
+                              uint64_t* __restrict__ partials, int root_removed)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         st->unvisited = num_vertices;
         st->unvisited_in_edges = num_edges;
         st->t_begin = 0;
-        if (root < num_vertices) {
+        if (root_removed) {
+            st->unvisited = num_vertices - 1;
+            st->unvisited_in_edges = num_edges - (indeg ? (uint64_t)indeg[root] : 0ull);
+        } else if (root < num_vertices) {
             partials[0] = 1;
             partials[1] = (uint64_t)outdeg[root];
             partials[2] = indeg ? (uint64_t)indeg[root] : 0ull;
@@ -371,8 +382,11 @@ __global__ void gz_k_sel_init(gz_sel_dev_state* st, uint64_t num_vertices, uint6
 
 /* Start a run: unvisited = V / E (the first step removes the initial
  * frontier).  With root < V the initial frontier is that one vertex and its
- * statistics are staged for gz_sel_dev_step(..., GZ_SEL_FROM_ROOT). */
-extern "C" void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges, uint64_t root, const uint32_t* indeg)
+ * statistics are staged for gz_sel_dev_step(..., GZ_SEL_FROM_ROOT); with
+ * `root_removed` (root < V) the counts start past the root instead, for a
+ * driver that takes the first decision on the host and runs no first step. */
+extern "C" void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges, uint64_t root, const uint32_t* indeg,
+                                 int root_removed)
 {
     if (gz_sel_dev_st == nullptr) {
         gz_sel_dev_st = (gz_sel_dev_state*)gz_device_malloc(sizeof(gz_sel_dev_state));
@@ -394,7 +408,7 @@ extern "C" void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges, uint
         gz_sel_partials_blocks = blocks;
     }
     hipLaunchKernelGGL(gz_k_sel_init, dim3(1), dim3(64), 0, gz_stream, gz_sel_dev_st, num_vertices, num_edges, root,
-                       gz_graph.vertex_outdegrees, indeg, gz_sel_dev_partials);
+                       gz_graph.vertex_outdegrees, indeg, gz_sel_dev_partials, root_removed && root < num_vertices);
     GZ_LAUNCH_CHECK();
 }
 
@@ -442,10 +456,19 @@ extern "C" void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg,
     const uint64_t* votes = (with_vote && source == GZ_SEL_FROM_EDGE_PHASE && !gz_scratch.partials_is_double)
                                 ? (const uint64_t*)gz_scratch.partials
                                 : nullptr;
+    /* the vote already sums the activated vertices' out-degrees in these
+     * statistic modes, and the in-edge partials are 0 without in-degrees */
+    int out_from_vote = 0;
+    if (votes != nullptr && gz_phase_config.stat_mode == GZ_STAT_COUNT_AND_OUTDEG) out_from_vote = 1;
+    if (votes != nullptr && gz_phase_config.stat_mode == GZ_STAT_OUTDEG_ONLY) out_from_vote = 2;
+    const int with_in = indeg != nullptr || source != GZ_SEL_FROM_EDGE_PHASE;
+    /* timed as the vote reduction it replaces */
+    gz_ktimer_begin(GZ_K_REDUCE);
     hipLaunchKernelGGL(gz_k_sel_decide, dim3(1), dim3(1024), 0, gz_stream, feat, blocks, votes, gz_sel_dev_st,
                        gz_sel_dev_params, gz_sel_dev_norm, have_model, bfs, V, gz_graph.num_edges, (float)iteration,
-                       (float)prev_engine, prev_ms, gz_sel_clock_khz, reduce_buffer);
+                       (float)prev_engine, prev_ms, gz_sel_clock_khz, reduce_buffer, out_from_vote, with_in);
     GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_REDUCE);
 }
 
 extern "C" int gz_selector_device_upload(const float* params)
