@@ -228,6 +228,13 @@ __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restri
     __shared__ float sp[GZ_SEL_PARAMS], snorm[2 * GZ_SEL_IN];
     const int j = threadIdx.x;
     const uint64_t t_end = (uint64_t)wall_clock64();
+    /* the run state is read up front too (thread 0 updates it below) */
+    uint64_t st_unvisited = 0, st_unvisited_in = 0, st_t_begin = 0;
+    if (j == 0) {
+        st_unvisited = st->unvisited;
+        st_unvisited_in = st->unvisited_in_edges;
+        st_t_begin = st->t_begin;
+    }
     /* the weights and normalisation go to LDS first: their loads overlap the
      * partial sums instead of following them */
     if (have_model)
@@ -285,13 +292,13 @@ __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restri
         if (out_from_vote == 2) ce = vote;
         uint64_t uv = num_vertices, ue = num_edges;
         if (bfs) {
-            st->unvisited -= cv;
-            st->unvisited_in_edges -= ci;
-            uv = st->unvisited;
-            ue = st->unvisited_in_edges;
+            uv = st_unvisited - cv;
+            ue = st_unvisited_in - ci;
+            st->unvisited = uv;
+            st->unvisited_in_edges = ue;
         }
         float ms = prev_ms;
-        if (ms < 0.f) ms = (t_end > st->t_begin) ? (float)(t_end - st->t_begin) / clock_khz : 0.f;
+        if (ms < 0.f) ms = (t_end > st_t_begin) ? (float)(t_end - st_t_begin) / clock_khz : 0.f;
         x[0] = logf(1.f + (float)cv);
         x[1] = logf(1.f + (float)ce);
         x[2] = logf(1.f + (float)uv);
