@@ -73,6 +73,7 @@ void gz_execution_prepare(const gz_run_options_t* o)
     gz_graph_reset_roles();
     if (gz_reduce_buffer == NULL) gz_reduce_buffer = gz_reduce_buffer_create();
     gz_device_zero(gz_reduce_buffer, 8 * sizeof(uint64_t));
+    gz_reduce_mirror_stale();
     if (gz_graph.pull_vectors != NULL) gz_merge_buffer_allocate(gz_pull_units_for(gz_graph.pull_vector_count));
 
     switch (o->algorithm) {

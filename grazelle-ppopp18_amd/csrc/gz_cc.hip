@@ -468,7 +468,7 @@ __global__ __launch_bounds__(GZ_CC_TAIL_THREADS) void gz_k_cc_push_tail(
     uint64_t* __restrict__ labels, uint64_t* __restrict__ next_frontier, const gz_vec_t* __restrict__ vecs,
     const uint64_t* __restrict__ offsets, const double* __restrict__ outdeg, uint32_t stat_mode,
     const uint64_t* __restrict__ first_list, const uint64_t* __restrict__ first_count, uint64_t vote_cap,
-    uint64_t max_rounds, uint64_t* __restrict__ out)
+    uint64_t max_rounds, uint64_t* __restrict__ out, uint64_t* __restrict__ mirror)
 {
     constexpr uint32_t T = GZ_CC_TAIL_THREADS;
     constexpr int kWaves = GZ_CC_TAIL_THREADS / GZ_WAVE;
@@ -623,6 +623,7 @@ __global__ __launch_bounds__(GZ_CC_TAIL_THREADS) void gz_k_cc_push_tail(
     if (tid == 0) {
         out[0] = vote;
         out[7] = rounds;
+        gz_mirror_words(mirror, out);
     }
 }
 
@@ -648,11 +649,13 @@ extern "C" void gz_cc_launch_push_tail(uint64_t vote_cap, uint64_t max_rounds, u
     if (max_rounds == 0) max_rounds = 1;
     gz_scratch_reserve(0, V);
     gz_frontier_list(gz_graph.frontier_has_info, gz_graph.push_vertex_offsets, V, gz_graph.push_vector_count);
+    uint64_t* mirror = gz_reduce_mirror_target(reduce_buffer);
     gz_ktimer_begin(GZ_K_PUSH_CC);
     hipLaunchKernelGGL(gz_k_cc_push_tail, dim3(1), dim3(GZ_CC_TAIL_THREADS), 0, gz_stream,
                        (uint64_t*)gz_graph.vertex_props, (uint64_t*)gz_graph.vertex_accumulators, gz_graph.push_vectors,
                        gz_graph.push_vertex_offsets, gz_graph.vertex_outdegrees, gz_phase_config.stat_mode,
-                       gz_scratch.active_vertices, gz_scratch.active_totals, vote_cap, max_rounds, reduce_buffer);
+                       gz_scratch.active_vertices, gz_scratch.active_totals, vote_cap, max_rounds, reduce_buffer,
+                       mirror);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_PUSH_CC);
 }

@@ -138,6 +138,9 @@ void gz_slices_settings(uint32_t* out3);
 /* The 8 reduce-buffer words as read by the last
  * phase_op_combine_global_var_from_buf (pinned host memory). */
 const uint64_t* gz_reduce_words(void);
+/* The pinned copy no longer matches a reduce buffer another writer changed
+ * (gz_runtime.hip). */
+void gz_reduce_mirror_stale(void);
 
 /* Learned selector on the device (selector_kernels.hip). */
 enum { GZ_SEL_FROM_BITMAP = 0, GZ_SEL_FROM_EDGE_PHASE = 1, GZ_SEL_FROM_ROOT = 2 };

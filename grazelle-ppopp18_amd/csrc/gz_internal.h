@@ -246,6 +246,14 @@ void gz_frontier_list(const uint64_t* bitmap, const uint64_t* offsets, uint64_t 
 void gz_frontier_compact_to(const uint64_t* bitmap, const uint64_t* offsets, uint64_t num_vertices,
                             uint64_t vector_count, const gz_compact_out& out);
 
+/* The pinned host copy of the reduce buffer (gz_runtime.hip).  A kernel
+ * handed gz_reduce_mirror_target(rb) writes the eight words of rb there too
+ * (gz_mirror_words, after its own writes to rb), so that
+ * phase_op_combine_global_var_from_buf(rb) needs no copy; a launcher that
+ * writes a reduce buffer otherwise calls gz_reduce_mirror_stale. */
+extern "C" uint64_t* gz_reduce_mirror_target(const uint64_t* reduce_buffer);
+extern "C" void gz_reduce_mirror_stale(void);
+
 /* ---- device helpers ------------------------------------------------------- */
 
 typedef unsigned long long gz_u64x2 __attribute__((ext_vector_type(2)));
@@ -394,6 +402,15 @@ struct gz_feature_sink {
 struct gz_feature_acc {
     uint64_t count, out_edges, in_edges;
 };
+
+/* Copy the eight reduce-buffer words to the pinned host copy (one thread,
+ * after it wrote its own words of `out`); nothing when mirror is null. */
+__device__ __forceinline__ void gz_mirror_words(uint64_t* __restrict__ mirror, const uint64_t* out)
+{
+    if (mirror == nullptr) return;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) mirror[i] = out[i];
+}
 
 /* One activated vertex: its frontier statistic (gz_frontier_stat) and, when
  * the sink is on, (1, out-degree, in-degree) added to the lane's feature

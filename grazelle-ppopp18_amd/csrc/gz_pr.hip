@@ -572,6 +572,7 @@ extern "C" void gz_pr_run_heads_build(void)
 
 extern "C" void gz_pr_vertex_merged(uint64_t* reduce_buffer)
 {
+    gz_reduce_mirror_stale();
     const uint64_t V = gz_graph.num_vertices, nchunks = gz_chunks_for(gz_graph.pull_vector_count);
     uint64_t blocks = (V + GZ_BLOCK - 1) / GZ_BLOCK;
     if (blocks > 1024) blocks = 1024;
@@ -677,6 +678,7 @@ extern "C" void gz_pr_persistent_spin_limit(uint32_t polls) { gz_pr_persistent_s
 
 extern "C" int gz_pr_run_persistent(uint64_t iterations, uint64_t* reduce_buffer)
 {
+    gz_reduce_mirror_stale();
     const uint64_t nvec = gz_graph.pull_vector_count, V = gz_graph.num_vertices;
     if (gz_pr_persistent_mode == 0 || gz_pr_persistent_unavailable || iterations == 0 ||
         gz_graph.pull_vectors == nullptr || nvec == 0 || V == 0)

@@ -533,7 +533,8 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_reduce(const T* __restrict__ in
 #define GZ_REDUCE_ONE_MAX 65536
 #define GZ_REDUCE_ONE_THREADS 1024
 __global__ __launch_bounds__(GZ_REDUCE_ONE_THREADS) void gz_k_reduce_one_u64(const unsigned long long* __restrict__ in,
-                                                                             uint64_t count, uint64_t* __restrict__ out)
+                                                                             uint64_t count, uint64_t* __restrict__ out,
+                                                                             uint64_t* __restrict__ mirror)
 {
     constexpr uint64_t T = GZ_REDUCE_ONE_THREADS;
     __shared__ unsigned long long lds[GZ_REDUCE_ONE_THREADS / GZ_WAVE];
@@ -547,7 +548,10 @@ __global__ __launch_bounds__(GZ_REDUCE_ONE_THREADS) void gz_k_reduce_one_u64(con
     }
     for (; i < count; i += T) x0 += in[i];
     const unsigned long long r = gz_block_sum<unsigned long long>((x0 + x1) + (x2 + x3), lds);
-    if (threadIdx.x == 0) out[0] = r;
+    if (threadIdx.x == 0) {
+        out[0] = r;
+        gz_mirror_words(mirror, out);
+    }
 }
 
 uint64_t gz_reduce_blocks(uint64_t count)
@@ -560,6 +564,7 @@ uint64_t gz_reduce_blocks(uint64_t count)
 
 void gz_reduce_f64_into(const double* in, uint64_t count, double* slots, unsigned int* done, uint64_t* out)
 {
+    gz_reduce_mirror_stale();
     hipLaunchKernelGGL(gz_k_reduce<double>, dim3((unsigned)gz_reduce_blocks(count)), dim3(GZ_BLOCK), 0, gz_stream,
                        in, count, slots, done, out);
     GZ_LAUNCH_CHECK();
@@ -574,10 +579,12 @@ void gz_reduce_partials(uint64_t* dst)
     if (gz_scratch.partials_is_double) {
         gz_reduce_f64_into((const double*)gz_scratch.partials, count, (double*)gz_scratch.reduce_tmp, done, dst);
     } else if (count <= GZ_REDUCE_ONE_MAX) {
+        uint64_t* mirror = gz_reduce_mirror_target(dst);
         hipLaunchKernelGGL(gz_k_reduce_one_u64, dim3(1), dim3(GZ_REDUCE_ONE_THREADS), 0, gz_stream,
-                           (const unsigned long long*)gz_scratch.partials, count, dst);
+                           (const unsigned long long*)gz_scratch.partials, count, dst, mirror);
         GZ_LAUNCH_CHECK();
     } else {
+        gz_reduce_mirror_stale();
         hipLaunchKernelGGL(gz_k_reduce<unsigned long long>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream,
                            (const unsigned long long*)gz_scratch.partials, count, (unsigned long long*)gz_scratch.reduce_tmp,
                            done, dst);
@@ -603,14 +610,38 @@ extern "C" void phase_op_write_global_accum_to_buf(uint64_t* reduce_buffer)
 
 /* Pinned host copy of the whole 8-word reduce buffer: the vote is word 0,
  * and the learned selector leaves its decision and features in words 1-6,
- * so one small copy serves both. */
-static uint64_t* gz_reduce_mirror = nullptr;
+ * so one small copy serves both.  The kernels that end a CC or BFS iteration
+ * (the one-workgroup vote sum, the selector's decision, the CC push tail)
+ * write the eight words into it themselves: `gz_mirror_holds` names the
+ * reduce buffer it then holds once the stream drains, and the combine needs
+ * no copy.  Every other writer of a reduce buffer marks it stale. */
+static uint64_t* gz_reduce_mirror = nullptr;     /* host address */
+static uint64_t* gz_reduce_mirror_dev = nullptr; /* its device address */
+static const uint64_t* gz_mirror_holds = nullptr;
+
+static void gz_reduce_mirror_alloc(void)
+{
+    if (gz_reduce_mirror != nullptr) return;
+    GZ_CHECK(hipHostMalloc((void**)&gz_reduce_mirror, 8 * sizeof(uint64_t), hipHostMallocMapped | hipHostMallocCoherent));
+    GZ_CHECK(hipHostGetDevicePointer((void**)&gz_reduce_mirror_dev, gz_reduce_mirror, 0));
+}
+
+extern "C" uint64_t* gz_reduce_mirror_target(const uint64_t* reduce_buffer)
+{
+    gz_reduce_mirror_alloc();
+    gz_mirror_holds = reduce_buffer;
+    return gz_reduce_mirror_dev;
+}
+
+extern "C" void gz_reduce_mirror_stale(void) { gz_mirror_holds = nullptr; }
 
 extern "C" uint64_t phase_op_combine_global_var_from_buf(uint64_t* reduce_buffer)
 {
     gz_require_runtime();
-    if (gz_reduce_mirror == nullptr) GZ_CHECK(hipHostMalloc((void**)&gz_reduce_mirror, 8 * sizeof(uint64_t)));
-    GZ_CHECK(hipMemcpyAsync(gz_reduce_mirror, reduce_buffer, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, gz_stream));
+    gz_reduce_mirror_alloc();
+    if (gz_mirror_holds != reduce_buffer)
+        GZ_CHECK(hipMemcpyAsync(gz_reduce_mirror, reduce_buffer, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, gz_stream));
+    gz_mirror_holds = nullptr;
     GZ_CHECK(hipStreamSynchronize(gz_stream));
     return gz_reduce_mirror[0];
 }

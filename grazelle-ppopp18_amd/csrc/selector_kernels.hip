@@ -221,7 +221,8 @@ __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restri
                                                             uint64_t num_vertices, uint64_t num_edges,
                                                             float iteration, float prev_engine, float prev_ms,
                                                             float clock_khz, uint64_t* __restrict__ out,
-                                                            int out_from_vote, int with_in)
+                                                            int out_from_vote, int with_in,
+                                                            uint64_t* __restrict__ mirror)
 {
     __shared__ uint64_t red[4][1024 / GZ_WAVE];
     __shared__ float x[GZ_SEL_IN], xn[GZ_SEL_IN], h1[GZ_SEL_H], h2[GZ_SEL_H], y[GZ_SEL_OUT];
@@ -341,6 +342,7 @@ __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restri
         yo[0] = have_model ? y[0] : 0.f;
         yo[1] = have_model ? y[1] : 0.f;
         out[1] = have_model ? ((y[0] < y[1]) ? 1u : 2u) : 0u;
+        gz_mirror_words(mirror, out);
     }
 }
 
@@ -469,11 +471,12 @@ extern "C" void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg,
     if (votes != nullptr && gz_phase_config.stat_mode == GZ_STAT_COUNT_AND_OUTDEG) out_from_vote = 1;
     if (votes != nullptr && gz_phase_config.stat_mode == GZ_STAT_OUTDEG_ONLY) out_from_vote = 2;
     const int with_in = indeg != nullptr || source != GZ_SEL_FROM_EDGE_PHASE;
+    uint64_t* mirror = gz_reduce_mirror_target(reduce_buffer);
     /* timed as the vote reduction it replaces */
     gz_ktimer_begin(GZ_K_REDUCE);
     hipLaunchKernelGGL(gz_k_sel_decide, dim3(1), dim3(1024), 0, gz_stream, feat, blocks, votes, gz_sel_dev_st,
                        gz_sel_dev_params, gz_sel_dev_norm, have_model, bfs, V, gz_graph.num_edges, (float)iteration,
-                       (float)prev_engine, prev_ms, gz_sel_clock_khz, reduce_buffer, out_from_vote, with_in);
+                       (float)prev_engine, prev_ms, gz_sel_clock_khz, reduce_buffer, out_from_vote, with_in, mirror);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_REDUCE);
 }
