@@ -149,6 +149,31 @@ static void gz_selector_read(gz_selector_reading_t* s)
     memcpy(s->predicted, &w[6], sizeof(s->predicted));
 }
 
+/* The learned BFS engine's decision on the host (gz_sel_dev_sums left the
+ * frontier's sums in the reduce words): the features in the decision
+ * kernel's formulas, with the unvisited counts kept here, and the forward
+ * pass of gz_selector_host_decide. */
+static void gz_selector_host_step(gz_selector_reading_t* s, uint64_t* unvisited, uint64_t* unvisited_in,
+                                  uint64_t V, uint64_t E, float iteration, float prev_engine)
+{
+    const uint64_t* w = gz_reduce_words();
+    const uint64_t cv = w[2], ce = w[3], ci = w[4];
+    const uint32_t msb = (uint32_t)w[5];
+    float ms;
+    memcpy(&ms, &msb, sizeof(ms));
+    *unvisited -= cv;
+    *unvisited_in -= ci;
+    s->features[0] = logf(1.f + (float)cv);
+    s->features[1] = logf(1.f + (float)ce);
+    s->features[2] = logf(1.f + (float)*unvisited);
+    s->features[3] = logf(1.f + (float)*unvisited_in);
+    s->features[4] = iteration;
+    s->features[5] = prev_engine;
+    s->features[6] = logf(1.f + ms);
+    s->features[7] = V ? (float)E / (float)V : 0.f;
+    s->choice = gz_selector_host_decide(s->features, s->predicted);
+}
+
 /* Everything a CC / BFS edge phase writes: vertex words and the three
  * bitmaps (HasInfo may be updated by CC push, WantsInfo by BFS, HasInfo* by
  * both). */
@@ -558,6 +583,10 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
     if (probe) gz_snapshot_alloc(&snap);
     gz_selector_reading_t sel;
     memset(&sel, 0, sizeof(sel));
+    /* learned BFS from a root: the host takes every decision, from sums the
+     * device leaves next to the vote, and keeps the unvisited counts */
+    int host_decides = 0;
+    uint64_t sel_unvisited = V, sel_unvisited_in = E;
     if (features) {
         /* features of the first iteration: the initial frontier */
         const uint64_t root = bfs ? gz_graph_device_vertex(o->root) : V;
@@ -581,6 +610,9 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
             sel.features[6] = 0.f;
             sel.features[7] = V ? (float)E / (float)V : 0.f;
             sel.choice = gz_selector_host_decide(sel.features, sel.predicted);
+            host_decides = 1;
+            sel_unvisited = V - 1;
+            sel_unvisited_in = E - (gz_selector_feature_used(3) ? gz_graph_indegree_host(root) : 0);
         } else {
             (void)phase_op_combine_global_var_from_buf(gz_reduce_buffer);
             gz_selector_read(&sel);
@@ -663,7 +695,9 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
         const int vote_in_step = features && bfs;
         if (!vote_in_step && !tail) phase_op_write_global_accum_to_buf(gz_reduce_buffer);
         if (t && !probe) gz_timer_stop(t);
-        if (features) {
+        if (host_decides) {
+            gz_sel_dev_sums(gz_reduce_buffer, indeg);
+        } else if (features) {
             /* the vertices this phase activated are the next frontier */
             gz_sel_dev_step((const uint64_t*)gz_graph.vertex_accumulators, indeg, gz_reduce_buffer, learned, bfs,
                             ctr + 1, rec.engine, prev_ms, bfs ? GZ_SEL_FROM_EDGE_PHASE : GZ_SEL_FROM_BITMAP,
@@ -676,7 +710,10 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
             r->push_iterations += rounds - 1;
             gz_cc_tail_rounds += rounds;
         }
-        if (features) gz_selector_read(&sel);
+        if (host_decides)
+            gz_selector_host_step(&sel, &sel_unvisited, &sel_unvisited_in, V, E, (float)(ctr + 1), (float)rec.engine);
+        else if (features)
+            gz_selector_read(&sel);
         if (t && !probe) rec.edge_ms = gz_timer_elapsed_ms(t);
         if (o->collect_stats) gz_phase_stats_read(&rec.vectors, &rec.edges);
         if (o->profile || o->collect_stats || features) gz_record(&rec);

@@ -148,6 +148,11 @@ enum { GZ_SEL_FROM_BITMAP = 0, GZ_SEL_FROM_EDGE_PHASE = 1, GZ_SEL_FROM_ROOT = 2 
 void gz_sel_dev_begin(uint64_t num_vertices, uint64_t num_edges, uint64_t root, const uint32_t* indeg,
                       int root_removed);
 void gz_sel_dev_stamp(int by_compaction);
+/* BFS with the learned engine deciding on the host: the last edge phase's
+ * vote and frontier sums into the reduce words (0: vote; 2, 3, 4: activated
+ * vertices, their out-edges, their in-edges; 5: the phase's device-clock ms
+ * as a float). */
+void gz_sel_dev_sums(uint64_t* reduce_buffer, const uint32_t* indeg);
 void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg, uint64_t* reduce_buffer, int have_model,
                      int bfs, uint64_t iteration, uint32_t prev_engine, float prev_ms, int source, int with_vote);
 /* BFS edge phases fold the selector's frontier statistics into their vote

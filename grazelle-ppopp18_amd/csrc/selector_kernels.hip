@@ -346,6 +346,81 @@ __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restri
     }
 }
 
+/* The learned BFS engine's per-level sums when the host takes the decision:
+ * the edge phase's per-wave vote and feature partials summed by one
+ * workgroup into reduce words 0 (the vote), 2 (the activated vertices), 3
+ * (their out-edges), 4 (their in-edges) and 5 (the edge phase's time on the
+ * device clock, in ms, as a float), and copied to the pinned host copy.
+ * out_from_vote and with_in as for gz_k_sel_decide. */
+__global__ __launch_bounds__(1024) void gz_k_sel_sums(const uint64_t* __restrict__ partials, uint64_t blocks,
+                                                          const uint64_t* __restrict__ votes,
+                                                          const gz_sel_dev_state* __restrict__ st, float clock_khz,
+                                                          int out_from_vote, int with_in, uint64_t* __restrict__ out,
+                                                          uint64_t* __restrict__ mirror)
+{
+    __shared__ uint64_t red[4][1024 / GZ_WAVE];
+    const int j = threadIdx.x;
+    const uint64_t t_end = (uint64_t)wall_clock64();
+    const uint64_t t_begin = j == 0 ? st->t_begin : 0;
+    uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    const uint64_t T = blockDim.x;
+    uint64_t b = j;
+    for (; b + 3 * T < blocks; b += 4 * T) {
+        uint64_t q[4][4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint64_t c = b + (uint64_t)u * T;
+            q[u][0] = partials[3 * c];
+            q[u][1] = out_from_vote ? 0 : partials[3 * c + 1];
+            q[u][2] = with_in ? partials[3 * c + 2] : 0;
+            q[u][3] = votes[c];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            s0 += q[u][0];
+            s1 += q[u][1];
+            s2 += q[u][2];
+            s3 += q[u][3];
+        }
+    }
+    for (; b < blocks; b += T) {
+        s0 += partials[3 * b];
+        if (!out_from_vote) s1 += partials[3 * b + 1];
+        if (with_in) s2 += partials[3 * b + 2];
+        s3 += votes[b];
+    }
+    s0 = gz_wave_sum_u64(s0);
+    s1 = gz_wave_sum_u64(s1);
+    s2 = gz_wave_sum_u64(s2);
+    s3 = gz_wave_sum_u64(s3);
+    if (gz_lane() == 0) {
+        red[0][j / GZ_WAVE] = s0;
+        red[1][j / GZ_WAVE] = s1;
+        red[2][j / GZ_WAVE] = s2;
+        red[3][j / GZ_WAVE] = s3;
+    }
+    __syncthreads();
+    if (j == 0) {
+        uint64_t cv = 0, ce = 0, ci = 0, vote = 0;
+        for (int i = 0; i < (int)blockDim.x / GZ_WAVE; ++i) {
+            cv += red[0][i];
+            ce += red[1][i];
+            ci += red[2][i];
+            vote += red[3][i];
+        }
+        if (out_from_vote == 1) ce = vote - cv;
+        if (out_from_vote == 2) ce = vote;
+        const float ms = (t_end > t_begin) ? (float)(t_end - t_begin) / clock_khz : 0.f;
+        const uint32_t msb = __float_as_uint(ms);
+        out[0] = vote;
+        out[2] = cv;
+        out[3] = ce;
+        out[4] = ci;
+        out[5] = msb;
+        gz_mirror_words(mirror, out);
+    }
+}
+
 /* ---- host entry points ---------------------------------------------------- */
 
 static float* gz_sel_dev_params = nullptr;
@@ -477,6 +552,20 @@ extern "C" void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg,
     hipLaunchKernelGGL(gz_k_sel_decide, dim3(1), dim3(1024), 0, gz_stream, feat, blocks, votes, gz_sel_dev_st,
                        gz_sel_dev_params, gz_sel_dev_norm, have_model, bfs, V, gz_graph.num_edges, (float)iteration,
                        (float)prev_engine, prev_ms, gz_sel_clock_khz, reduce_buffer, out_from_vote, with_in, mirror);
+    GZ_LAUNCH_CHECK();
+    gz_ktimer_end(GZ_K_REDUCE);
+}
+
+extern "C" void gz_sel_dev_sums(uint64_t* reduce_buffer, const uint32_t* indeg)
+{
+    int out_from_vote = 0;
+    if (gz_phase_config.stat_mode == GZ_STAT_COUNT_AND_OUTDEG) out_from_vote = 1;
+    if (gz_phase_config.stat_mode == GZ_STAT_OUTDEG_ONLY) out_from_vote = 2;
+    uint64_t* mirror = gz_reduce_mirror_target(reduce_buffer);
+    gz_ktimer_begin(GZ_K_REDUCE);
+    hipLaunchKernelGGL(gz_k_sel_sums, dim3(1), dim3(1024), 0, gz_stream, gz_scratch.features, gz_scratch.partials_count,
+                       (const uint64_t*)gz_scratch.partials, gz_sel_dev_st, gz_sel_clock_khz, out_from_vote,
+                       indeg != nullptr ? 1 : 0, reduce_buffer, mirror);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_REDUCE);
 }
