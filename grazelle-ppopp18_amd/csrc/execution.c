@@ -614,7 +614,7 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
             sel_unvisited = V - 1;
             sel_unvisited_in = E - (gz_selector_feature_used(3) ? gz_graph_indegree_host(root) : 0);
         } else {
-            (void)phase_op_combine_global_var_from_buf(gz_reduce_buffer);
+            (void)gz_combine_vote(gz_reduce_buffer);
             gz_selector_read(&sel);
         }
         /* BFS activates each vertex once: its edge phases fold the statistics
@@ -703,7 +703,7 @@ static void gz_run_hybrid(const gz_run_options_t* o, gz_run_result_t* r)
                             ctr + 1, rec.engine, prev_ms, bfs ? GZ_SEL_FROM_EDGE_PHASE : GZ_SEL_FROM_BITMAP,
                             vote_in_step);
         }
-        const uint64_t next_vote = phase_op_combine_global_var_from_buf(gz_reduce_buffer);
+        const uint64_t next_vote = gz_combine_vote(gz_reduce_buffer);
         if (tail) {
             const uint64_t rounds = gz_reduce_words()[7];
             ctr += rounds - 1;

@@ -141,6 +141,10 @@ const uint64_t* gz_reduce_words(void);
 /* The pinned copy no longer matches a reduce buffer another writer changed
  * (gz_runtime.hip). */
 void gz_reduce_mirror_stale(void);
+/* phase_op_combine_global_var_from_buf for the library's own drivers: reads
+ * the pinned copy without a copy when the vote kernels wrote it for this
+ * buffer. */
+uint64_t gz_combine_vote(uint64_t* reduce_buffer);
 
 /* Learned selector on the device (selector_kernels.hip). */
 enum { GZ_SEL_FROM_BITMAP = 0, GZ_SEL_FROM_EDGE_PHASE = 1, GZ_SEL_FROM_ROOT = 2 };

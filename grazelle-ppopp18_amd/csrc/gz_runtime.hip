@@ -611,10 +611,11 @@ extern "C" void phase_op_write_global_accum_to_buf(uint64_t* reduce_buffer)
 /* Pinned host copy of the whole 8-word reduce buffer: the vote is word 0,
  * and the learned selector leaves its decision and features in words 1-6,
  * so one small copy serves both.  The kernels that end a CC or BFS iteration
- * (the one-workgroup vote sum, the selector's decision, the CC push tail)
- * write the eight words into it themselves: `gz_mirror_holds` names the
- * reduce buffer it then holds once the stream drains, and the combine needs
- * no copy.  Every other writer of a reduce buffer marks it stale. */
+ * (the one-workgroup vote sum, the selector's decision and sums, the CC push
+ * tail) write the eight words into it themselves: `gz_mirror_holds` names
+ * the reduce buffer it then holds once the stream drains, and the drivers'
+ * read (gz_combine_vote) needs no copy.  Every other writer of a reduce
+ * buffer marks it stale. */
 static uint64_t* gz_reduce_mirror = nullptr;     /* host address */
 static uint64_t* gz_reduce_mirror_dev = nullptr; /* its device address */
 static const uint64_t* gz_mirror_holds = nullptr;
@@ -635,7 +636,21 @@ extern "C" uint64_t* gz_reduce_mirror_target(const uint64_t* reduce_buffer)
 
 extern "C" void gz_reduce_mirror_stale(void) { gz_mirror_holds = nullptr; }
 
+/* The phase API's read copies the buffer: a caller may have changed it
+ * since the kernels that wrote the pinned copy. */
 extern "C" uint64_t phase_op_combine_global_var_from_buf(uint64_t* reduce_buffer)
+{
+    gz_require_runtime();
+    gz_reduce_mirror_alloc();
+    GZ_CHECK(hipMemcpyAsync(gz_reduce_mirror, reduce_buffer, 8 * sizeof(uint64_t), hipMemcpyDeviceToHost, gz_stream));
+    gz_mirror_holds = nullptr;
+    GZ_CHECK(hipStreamSynchronize(gz_stream));
+    return gz_reduce_mirror[0];
+}
+
+/* The library's own drivers, which write the reduce buffer only through
+ * the launches above: no copy when the pinned copy holds this buffer. */
+extern "C" uint64_t gz_combine_vote(uint64_t* reduce_buffer)
 {
     gz_require_runtime();
     gz_reduce_mirror_alloc();
