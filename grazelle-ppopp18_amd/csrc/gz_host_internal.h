@@ -159,8 +159,6 @@ void gz_sel_dev_stamp(int by_compaction);
 void gz_sel_dev_sums(uint64_t* reduce_buffer, const uint32_t* indeg);
 void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg, uint64_t* reduce_buffer, int have_model,
                      int bfs, uint64_t iteration, uint32_t prev_engine, float prev_ms, int source, int with_vote);
-/* BFS edge phases fold the selector's frontier statistics into their vote
- * pass while indeg != NULL (NULL turns it off). */
 /* BFS edge phases fold the selector's features (vertices, out-edges and,
  * with `indeg`, in-edges of the vertices they activate) when `on`. */
 void gz_features_fold(int on, const uint32_t* indeg);
