@@ -283,6 +283,27 @@ def bfs_bytes(gz, root: int) -> tuple[int, int]:
     return total, len(recs)
 
 
+BFS_VOTE_BANDS = ((4096, "<=4K"), (65536, "<=64K"), (1 << 20, "<=1M"), (8 << 20, "<=8M"), (None, ">8M"))
+
+
+def bfs_level_split(gz, roots, engine: str) -> dict:
+    """Where the searches' edge-phase time goes: per engine and band of the
+    frontier statistic (vote) that chose it, the levels and their time, from
+    the per-iteration profile (HIP events around each edge phase) of an
+    untimed pass over the same roots."""
+    acc = {}
+    for root in roots:
+        gz.run(gz.options("bfs", engine=engine, root=root, profile=True))
+        for r in gz.records():
+            band = next(name for lim, name in BFS_VOTE_BANDS if lim is None or r.frontier_stat <= lim)
+            key = ("pull" if r.engine == 1 else "push") + " vote " + band
+            e = acc.setdefault(key, [0, 0.0])
+            e[0] += 1
+            e[1] += r.edge_ms
+    return {k: {"levels": n, "ms_per_search": round(ms / len(roots), 4), "us_per_level": round(1e3 * ms / n, 1)}
+            for k, (n, ms) in sorted(acc.items())}
+
+
 def bfs_rmat24(gz, cpu: int) -> dict:
     import selector_train as st
 
@@ -310,7 +331,9 @@ def bfs_rmat24(gz, cpu: int) -> dict:
         gz.selector_load(st.DEFAULT_WEIGHTS)
         if not np.array_equal(d_l, d_f):
             raise AssertionError(f"BFS depths differ between engines at root {root}")
+    split_learned = bfs_level_split(gz, roots, "learned")
     gz.selector_clear()
+    out["edge_time_by_level"] = {"fixed_threshold": bfs_level_split(gz, roots, "hybrid"), "learned": split_learned}
     out.update({"fixed_threshold_ms": round(totals["hybrid"], 3), "learned_ms": round(totals["learned"], 3),
                 "learned_speedup": round(totals["hybrid"] / totals["learned"], 4),
                 "fixed_ms_per_search": round(totals["hybrid"] / len(roots), 4),

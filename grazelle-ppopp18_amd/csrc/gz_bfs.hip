@@ -494,7 +494,8 @@ extern "C" void perform_edge_push_phase_bfs(const gz_vec_t* edge_list, uint64_t 
                                 gz_graph.vertex_outdegrees,
                                 gz_phase_config.stat_mode,
                                 (uint64_t)gz_phase_config.bfs_level,
-                                {gz_scratch.features_on ? gz_scratch.features : nullptr, gz_scratch.features_indeg}};
+                                {gz_scratch.features_on ? gz_scratch.features : nullptr, gz_scratch.features_indeg,
+                                 gz_scratch.partials_capacity}};
         uint64_t* stamp = gz_scratch.stamp_target;
         gz_scratch.stamp_target = nullptr;
         gz_ktimer_begin(GZ_K_PUSH_BFS);
@@ -520,7 +521,8 @@ extern "C" void perform_edge_push_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     gz_scratch.partials_count = waves;
     gz_scratch.stats_count = gz_phase_config.collect_stats ? waves : 0;
     const uint64_t level = gz_phase_config.bfs_level;
-    const gz_feature_sink sink{gz_scratch.features_on ? gz_scratch.features : nullptr, gz_scratch.features_indeg};
+    const gz_feature_sink sink{gz_scratch.features_on ? gz_scratch.features : nullptr, gz_scratch.features_indeg,
+                                 gz_scratch.partials_capacity};
     gz_ktimer_begin(GZ_K_PUSH_BFS);
     if (active)
         hipLaunchKernelGGL(gz_k_edge_push_bfs<true>, dim3((unsigned)blocks), dim3(GZ_BLOCK), 0, gz_stream, edge_list,
@@ -566,7 +568,8 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     gz_scratch.partials_count = vote_slots;
     gz_scratch.stats_count = gz_phase_config.collect_stats ? blocks * GZ_WAVES_PER_BLOCK : 0;
     const uint64_t level = gz_phase_config.bfs_level;
-    const gz_feature_sink sink{gz_scratch.features_on ? gz_scratch.features : nullptr, gz_scratch.features_indeg};
+    const gz_feature_sink sink{gz_scratch.features_on ? gz_scratch.features : nullptr, gz_scratch.features_indeg,
+                                 gz_scratch.partials_capacity};
     gz_ktimer_begin(GZ_K_PULL_BFS_FIND);
     gz_slot_source pend_src = src;
     if (active) {
@@ -603,7 +606,7 @@ extern "C" void perform_edge_pull_phase_bfs(const gz_vec_t* edge_list, uint64_t 
     if (active) {
         /* pass B visits the pending destinations pass A resolved */
         const uint64_t a0_waves = blocks * GZ_WAVES_PER_BLOCK;
-        const gz_feature_sink bsink{sink.out ? sink.out + 3 * a0_waves : nullptr, sink.indeg};
+        const gz_feature_sink bsink{sink.out ? sink.out + a0_waves : nullptr, sink.indeg, sink.stride};
         hipLaunchKernelGGL(gz_k_edge_pull_bfs_visit<true>, dim3((unsigned)visit_blocks), dim3(GZ_BLOCK), 0, gz_stream,
                            edge_list, gz_bfs_pend.out.list, gz_bfs_pend.out.totals, gz_graph.num_vertices,
                            gz_graph.frontier_has_info, gz_graph.frontier_wants_info, gz_scratch.bfs_first_vector,

@@ -77,7 +77,7 @@ typedef struct gz_scratch_t {
     uint64_t* active_totals;     /* [0] = active vertices, [1] = active vectors */
     uint64_t active_capacity;
     uint64_t* bfs_first_vector;  /* [V] first qualifying pull vector of each dst, ~0 when none */
-    uint64_t* features;          /* [3 * partials_capacity] selector features per wave (BFS edge phases) */
+    uint64_t* features;          /* [3][partials_capacity] selector features per wave (BFS edge phases) */
     uint32_t features_on;         /* BFS edge phases fold the selector's features */
     const uint32_t* features_indeg; /* in-degrees for the in-edge feature, or nullptr (not folded) */
     uint64_t* stamp_target;      /* when set, the next compaction writes the wall clock there (once) */
@@ -392,11 +392,15 @@ __device__ __forceinline__ uint64_t gz_wave_sum_u64(uint64_t x)
 
 /* Selector features folded into a BFS edge phase: every vertex the phase
  * activates (exactly once in BFS) adds (1, outdeg, indeg) to the lane's
- * counters, and each wavefront writes its sums next to its vote partial at
- * out[3 * wave + 0..2].  out == nullptr turns it off. */
+ * counters, and each wavefront writes its sums next to its vote partial, one
+ * component after another: out[wave], out[stride + wave] and
+ * out[2 * stride + wave] (vertices, out-edges, in-edges), so that a reader
+ * of one component reads it as densely as the votes.  out == nullptr turns
+ * it off. */
 struct gz_feature_sink {
     uint64_t* out;
     const uint32_t* indeg;
+    uint64_t stride;
 };
 
 struct gz_feature_acc {
@@ -442,9 +446,9 @@ __device__ __forceinline__ void gz_feature_flush(const gz_feature_acc& f, const 
         const uint64_t c = gz_wave_sum_u64(f.count), o = gz_wave_sum_u64(f.out_edges),
                        i = gz_wave_sum_u64(f.in_edges);
         if ((threadIdx.x & (GZ_WAVE - 1)) == 0) {
-            s.out[3 * wave] = c;
-            s.out[3 * wave + 1] = o;
-            s.out[3 * wave + 2] = i;
+            s.out[wave] = c;
+            s.out[s.stride + wave] = o;
+            s.out[2 * s.stride + wave] = i;
         }
     }
 }

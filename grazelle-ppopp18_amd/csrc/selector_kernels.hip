@@ -165,8 +165,8 @@ __global__ void gz_k_sel_stamp(gz_sel_dev_state* st)
 /* One wavefront per 64 bitmap words: lane l holds word l of the wave's range
  * for the population count, then the wave walks the words one by one with
  * lane l testing vertex 64w + l, so the degree reads of a dense frontier are
- * coalesced.  partials[3b .. 3b+2] = (vertices, out-edges, in-edges) of
- * block b. */
+ * coalesced.  partials[b], partials[B + b], partials[2B + b] = (vertices,
+ * out-edges, in-edges) of block b of the grid's B. */
 __global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_stats(const uint64_t* __restrict__ frontier,
                                                            const double* __restrict__ outdeg,
                                                            const uint32_t* __restrict__ indeg, uint64_t num_vertices,
@@ -202,20 +202,22 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_sel_stats(const uint64_t* __res
     if (threadIdx.x < 3) {
         uint64_t s = 0;
         for (int i = 0; i < GZ_WAVES_PER_BLOCK; ++i) s += red[threadIdx.x][i];
-        partials[3 * (uint64_t)blockIdx.x + threadIdx.x] = s;
+        partials[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = s;
     }
 }
 
 /* out[1] = decision (1 pull, 2 push, 0 no model), out[2..5] = the raw
  * features as 8 floats, out[6] = the two outputs as floats.  The unvisited
  * counts advance by the frontier (BFS marks each vertex once). */
-/* out_from_vote: 1 when the vote counts every activated vertex plus its
+/* partials: `blocks` slots of (vertices, out-edges, in-edges), component k
+ * of slot c at partials[k * stride + c].
+ * out_from_vote: 1 when the vote counts every activated vertex plus its
  * out-degree (stat mode both), so out-edges = vote - vertices; 2 when it
  * counts out-degrees only (out-edges = vote); 0: read the out-edge partials.
  * with_in: read the in-edge partials (the folds write 0 there without
  * in-degrees). */
 __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restrict__ partials, uint64_t blocks,
-                                                            const uint64_t* __restrict__ votes,
+                                                            uint64_t stride, const uint64_t* __restrict__ votes,
                                                             gz_sel_dev_state* st, const float* __restrict__ p,
                                                             const float* __restrict__ norm, int have_model, int bfs,
                                                             uint64_t num_vertices, uint64_t num_edges,
@@ -250,9 +252,9 @@ __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restri
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const uint64_t c = b + (uint64_t)u * T;
-            q[u][0] = partials[3 * c];
-            q[u][1] = out_from_vote ? 0 : partials[3 * c + 1];
-            q[u][2] = with_in ? partials[3 * c + 2] : 0;
+            q[u][0] = partials[c];
+            q[u][1] = out_from_vote ? 0 : partials[stride + c];
+            q[u][2] = with_in ? partials[2 * stride + c] : 0;
             q[u][3] = votes ? votes[c] : 0;
         }
 #pragma unroll
@@ -264,9 +266,9 @@ __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restri
         }
     }
     for (; b < blocks; b += T) {
-        s0 += partials[3 * b];
-        if (!out_from_vote) s1 += partials[3 * b + 1];
-        if (with_in) s2 += partials[3 * b + 2];
+        s0 += partials[b];
+        if (!out_from_vote) s1 += partials[stride + b];
+        if (with_in) s2 += partials[2 * stride + b];
         if (votes) s3 += votes[b];
     }
     s0 = gz_wave_sum_u64(s0);
@@ -353,7 +355,7 @@ __global__ __launch_bounds__(1024) void gz_k_sel_decide(const uint64_t* __restri
  * device clock, in ms, as a float), and copied to the pinned host copy.
  * out_from_vote and with_in as for gz_k_sel_decide. */
 __global__ __launch_bounds__(1024) void gz_k_sel_sums(const uint64_t* __restrict__ partials, uint64_t blocks,
-                                                          const uint64_t* __restrict__ votes,
+                                                          uint64_t stride, const uint64_t* __restrict__ votes,
                                                           const gz_sel_dev_state* __restrict__ st, float clock_khz,
                                                           int out_from_vote, int with_in, uint64_t* __restrict__ out,
                                                           uint64_t* __restrict__ mirror)
@@ -370,9 +372,9 @@ __global__ __launch_bounds__(1024) void gz_k_sel_sums(const uint64_t* __restrict
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const uint64_t c = b + (uint64_t)u * T;
-            q[u][0] = partials[3 * c];
-            q[u][1] = out_from_vote ? 0 : partials[3 * c + 1];
-            q[u][2] = with_in ? partials[3 * c + 2] : 0;
+            q[u][0] = partials[c];
+            q[u][1] = out_from_vote ? 0 : partials[stride + c];
+            q[u][2] = with_in ? partials[2 * stride + c] : 0;
             q[u][3] = votes[c];
         }
 #pragma unroll
@@ -384,9 +386,9 @@ __global__ __launch_bounds__(1024) void gz_k_sel_sums(const uint64_t* __restrict
         }
     }
     for (; b < blocks; b += T) {
-        s0 += partials[3 * b];
-        if (!out_from_vote) s1 += partials[3 * b + 1];
-        if (with_in) s2 += partials[3 * b + 2];
+        s0 += partials[b];
+        if (!out_from_vote) s1 += partials[stride + b];
+        if (with_in) s2 += partials[2 * stride + b];
         s3 += votes[b];
     }
     s0 = gz_wave_sum_u64(s0);
@@ -525,10 +527,11 @@ extern "C" void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg,
     if (have_model && gz_sel_dev_params == nullptr) have_model = 0;
     const uint64_t V = gz_graph.num_vertices;
     const uint64_t* feat = gz_scratch.features;
-    uint64_t blocks = gz_scratch.partials_count;
+    uint64_t blocks = gz_scratch.partials_count, stride = gz_scratch.partials_capacity;
     if (source == GZ_SEL_FROM_ROOT) {
         feat = gz_sel_dev_partials;
         blocks = 1;
+        stride = 1;
     } else if (source == GZ_SEL_FROM_BITMAP) {
         const uint64_t words = (V + 63) / 64;
         blocks = gz_sel_stats_blocks(words);
@@ -536,6 +539,7 @@ extern "C" void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg,
                            gz_graph.vertex_outdegrees, indeg, V, words, gz_sel_dev_partials);
         GZ_LAUNCH_CHECK();
         feat = gz_sel_dev_partials;
+        stride = blocks;
     }
     const uint64_t* votes = (with_vote && source == GZ_SEL_FROM_EDGE_PHASE && !gz_scratch.partials_is_double)
                                 ? (const uint64_t*)gz_scratch.partials
@@ -549,7 +553,7 @@ extern "C" void gz_sel_dev_step(const uint64_t* frontier, const uint32_t* indeg,
     uint64_t* mirror = gz_reduce_mirror_target(reduce_buffer);
     /* timed as the vote reduction it replaces */
     gz_ktimer_begin(GZ_K_REDUCE);
-    hipLaunchKernelGGL(gz_k_sel_decide, dim3(1), dim3(1024), 0, gz_stream, feat, blocks, votes, gz_sel_dev_st,
+    hipLaunchKernelGGL(gz_k_sel_decide, dim3(1), dim3(1024), 0, gz_stream, feat, blocks, stride, votes, gz_sel_dev_st,
                        gz_sel_dev_params, gz_sel_dev_norm, have_model, bfs, V, gz_graph.num_edges, (float)iteration,
                        (float)prev_engine, prev_ms, gz_sel_clock_khz, reduce_buffer, out_from_vote, with_in, mirror);
     GZ_LAUNCH_CHECK();
@@ -564,7 +568,7 @@ extern "C" void gz_sel_dev_sums(uint64_t* reduce_buffer, const uint32_t* indeg)
     uint64_t* mirror = gz_reduce_mirror_target(reduce_buffer);
     gz_ktimer_begin(GZ_K_REDUCE);
     hipLaunchKernelGGL(gz_k_sel_sums, dim3(1), dim3(1024), 0, gz_stream, gz_scratch.features, gz_scratch.partials_count,
-                       (const uint64_t*)gz_scratch.partials, gz_sel_dev_st, gz_sel_clock_khz, out_from_vote,
+                       gz_scratch.partials_capacity, (const uint64_t*)gz_scratch.partials, gz_sel_dev_st, gz_sel_clock_khz, out_from_vote,
                        indeg != nullptr ? 1 : 0, reduce_buffer, mirror);
     GZ_LAUNCH_CHECK();
     gz_ktimer_end(GZ_K_REDUCE);
