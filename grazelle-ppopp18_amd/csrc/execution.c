@@ -101,13 +101,8 @@ void gz_execution_prepare(const gz_run_options_t* o)
             exit(66);
         }
         const uint64_t r = gz_graph_device_vertex(o->root);
-        gz_dev_fill_u64((uint64_t*)gz_graph.vertex_props, GZ_BFS_UNVISITED, V);
-        gz_dev_store_u64((uint64_t*)gz_graph.vertex_props, r, GZ_BFS_PARENT_NONE);
-        gz_dev_fill_u64((uint64_t*)gz_graph.vertex_accumulators, 0, fwords);
-        gz_dev_fill_u64(gz_graph.frontier_has_info, 0, fwords);
-        gz_dev_store_u64(gz_graph.frontier_has_info, r >> 6, 1ull << (r & 63));
-        gz_dev_fill_u64(gz_graph.frontier_wants_info, ~0ull, fwords);
-        gz_dev_store_u64(gz_graph.frontier_wants_info, r >> 6, ~(1ull << (r & 63)));
+        gz_dev_bfs_reset((uint64_t*)gz_graph.vertex_props, V, (uint64_t*)gz_graph.vertex_accumulators,
+                         gz_graph.frontier_has_info, gz_graph.frontier_wants_info, fwords, r);
         break;
     }
     default:

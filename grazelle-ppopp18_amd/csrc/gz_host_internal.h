@@ -36,6 +36,11 @@ void gz_dev_init_pr(double* props, double* acc, const double* outdeg, uint64_t n
 void gz_dev_fill_u64(uint64_t* p, uint64_t value, uint64_t n);
 void gz_dev_iota_u64(uint64_t* p, uint64_t n);
 void gz_dev_store_u64(uint64_t* p, uint64_t index, uint64_t value);
+/* A BFS start from device vertex `root` in one launch: words[0..n) unvisited
+ * but the root's (no parent), acc[0..fwords) = 0, has = {root}, wants = all
+ * but root. */
+void gz_dev_bfs_reset(uint64_t* words, uint64_t n, uint64_t* acc, uint64_t* has, uint64_t* wants, uint64_t fwords,
+                      uint64_t root);
 void gz_graph_reset_roles(void);
 void gz_dev_indegrees(const gz_vec_t* vecs, uint64_t count, int pull_layout, uint32_t* indeg, uint64_t n);
 /* In-degrees of the loaded graph (u32[V+8]), built on first use. */

@@ -38,12 +38,38 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_iota(uint64_t* __restrict__ p, 
 
 __global__ void gz_k_store(uint64_t* __restrict__ p, uint64_t index, uint64_t value) { p[index] = value; }
 
+/* The start of a BFS from `root` in one launch (execution_bfs.c:106-116 and
+ * :70-102): every word unvisited but the root's (no parent), HasInfo* clear,
+ * HasInfo = {root}, WantsInfo = every vertex but the root. */
+__global__ __launch_bounds__(GZ_BLOCK) void gz_k_bfs_reset(uint64_t* __restrict__ words, uint64_t n,
+                                                           uint64_t* __restrict__ acc, uint64_t* __restrict__ has,
+                                                           uint64_t* __restrict__ wants, uint64_t fwords,
+                                                           uint64_t root)
+{
+    const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = first; i < n; i += stride) words[i] = (i == root) ? GZ_BFS_PARENT_NONE : GZ_BFS_UNVISITED;
+    const uint64_t rw = root >> 6, rbit = 1ull << (root & 63);
+    for (uint64_t w = first; w < fwords; w += stride) {
+        acc[w] = 0;
+        has[w] = (w == rw) ? rbit : 0ull;
+        wants[w] = (w == rw) ? ~rbit : ~0ull;
+    }
+}
+
 static unsigned gz_blocks_for(uint64_t n)
 {
     uint64_t b = (n + GZ_BLOCK - 1) / GZ_BLOCK;
     if (b < 1) b = 1;
     if (b > 8192) b = 8192;
     return (unsigned)b;
+}
+
+void gz_dev_bfs_reset(uint64_t* words, uint64_t n, uint64_t* acc, uint64_t* has, uint64_t* wants, uint64_t fwords,
+                      uint64_t root)
+{
+    hipLaunchKernelGGL(gz_k_bfs_reset, dim3(gz_blocks_for(n > fwords ? n : fwords)), dim3(GZ_BLOCK), 0, gz_stream, words,
+                       n, acc, has, wants, fwords, root);
+    GZ_LAUNCH_CHECK();
 }
 
 extern "C" void gz_dev_init_pr(double* props, double* acc, const double* outdeg, uint64_t n)
