@@ -361,22 +361,32 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_list_write(const uint6
                                                                      uint64_t* __restrict__ prefix)
 {
     __shared__ unsigned long long lds[GZ_WAVES_PER_BLOCK];
+    /* the block's ids as offsets from its first vertex, in list order, so
+     * that the list is written by consecutive lanes (a thread writing its own
+     * ids would store 64 lanes up to 16 entries apart) */
+    __shared__ uint16_t stage[GZ_COMPACT_VPB];
+    static_assert(GZ_COMPACT_VPB <= 65536, "block offsets are 16-bit");
     if (seg != nullptr && blockIdx.x == 0 && threadIdx.x < GZ_WAVE) gz_seg_finish(seg, seg_other, totals, prefix);
     for (uint64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
-        const uint64_t v0 = b * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
-        uint32_t bits = gz_list_bits(bitmap, hasvec, num_vertices, v0);
+        const uint64_t vb = b * GZ_COMPACT_VPB;
+        const uint32_t t0 = GZ_COMPACT_VPT * (uint32_t)threadIdx.x;
+        uint32_t bits = gz_list_bits(bitmap, hasvec, num_vertices, vb + t0);
         uint64_t pos0, unused;
         if (seg != nullptr)
             gz_seg_base(seg, block_sums, b, seg_blocks, pos0, unused);
         else
             pos0 = block_sums[2 * b];
         unsigned long long tv;
-        uint64_t pos = pos0 + gz_block_exclusive_scan<unsigned long long>((unsigned long long)__popc(bits), lds, tv);
+        /* the scan ends with a barrier, after every thread's copy-out of the
+         * previous block, so the stage is free again */
+        uint32_t p = (uint32_t)gz_block_exclusive_scan<unsigned long long>((unsigned long long)__popc(bits), lds, tv);
         while (bits) {
             const int k = __ffs(bits) - 1;
             bits &= bits - 1u;
-            list[pos++] = v0 + (uint64_t)k;
+            stage[p++] = (uint16_t)(t0 + (uint32_t)k);
         }
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < (uint32_t)tv; i += GZ_BLOCK) list[pos0 + i] = vb + stage[i];
     }
 }
 
