@@ -280,10 +280,18 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_write(const uint64_t* 
                                                                  uint64_t* __restrict__ totals)
 {
     __shared__ unsigned long long lds[GZ_WAVES_PER_BLOCK];
+    /* the block's entries in list order (vertex as an offset from the block's
+     * first, vector prefix as an offset from the block's first vector), so
+     * that list and prefix are written by consecutive lanes */
+    __shared__ uint16_t stage_v[GZ_COMPACT_VPB];
+    __shared__ uint32_t stage_p[GZ_COMPACT_VPB];
+    static_assert(GZ_COMPACT_VPB <= 65536, "block offsets are 16-bit");
     if (gz_guard_off(guard)) return;
     if (seg != nullptr && blockIdx.x == 0 && threadIdx.x < GZ_WAVE) gz_seg_finish(seg, seg_other, totals, prefix);
     for (uint64_t b = blockIdx.x; b < nblocks; b += gridDim.x) {
-        const uint64_t v0 = b * GZ_COMPACT_VPB + GZ_COMPACT_VPT * (uint64_t)threadIdx.x;
+        const uint64_t vb = b * GZ_COMPACT_VPB;
+        const uint32_t t0 = GZ_COMPACT_VPT * (uint32_t)threadIdx.x;
+        const uint64_t v0 = vb + t0;
         uint64_t c[GZ_COMPACT_VPT];
         gz_run_counts<kNarrow>(bitmap, src, num_vertices, v0, c);
         unsigned long long nv = 0, nvec = 0;
@@ -300,15 +308,36 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_frontier_write(const uint64_t* 
             vpos0 = block_sums[2 * b + 1];
         }
         unsigned long long tv, tvec;
-        uint64_t pos = pos0 + gz_block_exclusive_scan<unsigned long long>(nv, lds, tv);
-        uint64_t vpos = vpos0 + gz_block_exclusive_scan<unsigned long long>(nvec, lds, tvec);
+        /* each scan ends with a barrier, after every thread's copy-out of the
+         * previous block, so the stage is free again */
+        const uint64_t p0 = gz_block_exclusive_scan<unsigned long long>(nv, lds, tv);
+        const uint64_t q0 = gz_block_exclusive_scan<unsigned long long>(nvec, lds, tvec);
+        if (tvec < (1ull << 32)) { /* block-uniform */
+            uint32_t p = (uint32_t)p0, q = (uint32_t)q0;
 #pragma unroll
-        for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
-            if (c[k]) {
-                list[pos] = v0 + (uint64_t)k;
-                prefix[pos] = vpos;
-                pos += 1;
-                vpos += c[k];
+            for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
+                if (c[k]) {
+                    stage_v[p] = (uint16_t)(t0 + (uint32_t)k);
+                    stage_p[p] = q;
+                    p += 1;
+                    q += (uint32_t)c[k];
+                }
+            }
+            __syncthreads();
+            for (uint32_t i = threadIdx.x; i < (uint32_t)tv; i += GZ_BLOCK) {
+                list[pos0 + i] = vb + stage_v[i];
+                prefix[pos0 + i] = vpos0 + stage_p[i];
+            }
+        } else {
+            uint64_t pos = pos0 + p0, vpos = vpos0 + q0;
+#pragma unroll
+            for (int k = 0; k < GZ_COMPACT_VPT; ++k) {
+                if (c[k]) {
+                    list[pos] = v0 + (uint64_t)k;
+                    prefix[pos] = vpos;
+                    pos += 1;
+                    vpos += c[k];
+                }
             }
         }
     }
