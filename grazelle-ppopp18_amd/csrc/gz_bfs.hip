@@ -224,23 +224,37 @@ __global__ __launch_bounds__(GZ_BLOCK) void gz_k_edge_pull_bfs_first(const gz_ve
      * neighbouring lanes share bitmap words; one lane per run of equal words
      * applies the run's bits with one atomic per bitmap */
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i0 = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(uint64_t)(GZ_WAVE - 1)); i0 < n;
-         i0 += stride) {
+    const uint64_t first = (uint64_t)blockIdx.x * blockDim.x + (threadIdx.x & ~(uint64_t)(GZ_WAVE - 1));
+    /* the next slot's destination and vector range are loaded one iteration
+     * ahead (the destination while this vector loads, its range while this
+     * vector's frontier bits load), so that an iteration waits for its vector
+     * and bits only */
+    uint64_t dn = 0, vin = 0, ven = 0;
+    if (first + (uint64_t)lane < n) {
+        dn = src.list[first + (uint64_t)lane];
+        vin = src.offsets[dn];
+        ven = src.offsets[dn + 1];
+    }
+    for (uint64_t i0 = first; i0 < n; i0 += stride) {
         const uint64_t i = i0 + (uint64_t)lane;
         bool pend = false, found = false;
-        uint64_t d = 0;
+        const uint64_t d = dn, vi = vin, vend = ven;
+        const uint64_t inext = i + stride;
         if (i < n) {
-            d = src.list[i];
-            const uint64_t vi = src.offsets[d];
             const gz_lanes v = gz_load_vector(vecs, vi);
+            if (inext < n) dn = src.list[inext];
             vectors += 1;
             const uint64_t parent = gz_vec_frontier_parent(v, has_info, edges);
+            if (inext < n) {
+                vin = src.offsets[dn];
+                ven = src.offsets[dn + 1];
+            }
             if (parent != ~0ull) {
                 found = true;
                 words[d] = ((uint64_t)level << 48) | parent;
                 vote += gz_activate_stat(feat, sink, outdeg, d, stat_mode);
             } else {
-                pend = src.offsets[d + 1] - vi > 1;
+                pend = vend - vi > 1;
             }
         }
         const uint64_t word = (i < n) ? (d >> 5) : ~0ull;
